@@ -1,0 +1,1491 @@
+// mwref — FP64 CPU restatement of the multiwavelet Green's-function apply.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the oracle: it is compiled into
+// oracle/libmwref.so and may be loaded only by tests/, by the smoke check and by
+// the CPU-baseline leg of bench.py.  The product library (mrchem_amd) never
+// links, loads or calls it.
+//
+// What it restates (reference call sites; the MRCPP sources that implement the
+// apply are not vendored, external/mrcpp/ is an empty submodule, .gitmodules:4-6):
+//   * mrcpp::apply(prec, out, oper, inp) as called from
+//       src/scf_solver/HelmholtzVector.cpp:176,180
+//       src/qmoperators/two_electron/CoulombPotential.cpp:85
+//       src/qmoperators/two_electron/ExchangePotential.cpp:137,141,259,263,318,323
+//     adaptive build: compute work nodes -> split check -> children, until no
+//     new nodes; then additive top-down transform, bottom-up compression and
+//     the square norm.
+//   * mrcpp::PoissonOperator(MRA, prec)     (src/SCFDriver.cpp:279)
+//     mrcpp::HelmholtzOperator(MRA, mu, prec) (src/scf_solver/HelmholtzVector.cpp:171)
+//     kernel 1/r or exp(-mu r)/r as a sum of Gaussians, separated into 1D
+//     non-standard (NS) operator blocks T/C/B/A per scale and translation.
+//   * mrcpp::project of analytic functions (HydrogenFunction,
+//     src/analyticfunctions/HydrogenFunction.cpp:34-165, used by
+//     tests/qmoperators/coulomb_operator.cpp:68-71) and Gaussian sums.
+//   * MultiResolutionAnalysis/BoundingBox (src/mrenv.cpp:46-72,
+//     tests/unit_tests.cpp:28-44): root scale, corner, boxes, Legendre or
+//     Interpolating scaling basis of order k.
+//
+// The algorithm follows the published multiwavelet NS-form apply
+// (Beylkin-Coifman-Rokhlin NS form; Alpert multiwavelets; Harrison, Fann,
+// Yanai, Gan, Beylkin, J. Chem. Phys. 121, 11587 (2004); Frediani et al.,
+// Mol. Phys. 111, 1143 (2013)).  Every convention below (component bit order,
+// coefficient layout, screening bound, split rule, kernel fit) is this
+// project's own and is shared with the product library so that both produce
+// the same work list and the same output grid.
+//
+// Parity status (see the header of oracle/mwref.py for the list of checks):
+//   basis/filters: pinned by orthonormality, perfect reconstruction and the
+//   two-scale identity; kernel fit: pinned by the max relative error against
+//   1/r and exp(-mu r)/r; operator blocks: pinned by direct quadrature against
+//   the two-scale NS construction and the constant-kernel closed form; apply:
+//   pinned by the erf / erfc closed forms for Gaussian charges, the p-type
+//   derivative closed forms and the hydrogenic Slater-Condon integrals.
+
+#include <algorithm>
+#include <array>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+#include <omp.h>
+
+namespace {
+
+// ---------------------------------------------------------------- quadrature
+
+// Gauss-Legendre rule with q points mapped to [0,1].
+void gl01(int q, double *x, double *w) {
+    for (int i = 0; i < q; i++) {
+        long double t = std::cos(M_PI * (i + 0.75L) / (q + 0.5L));
+        for (int it = 0; it < 100; it++) {
+            long double p0 = 1.0L, p1 = t;
+            for (int n = 2; n <= q; n++) {
+                long double p2 = ((2.0L * n - 1.0L) * t * p1 - (n - 1.0L) * p0) / n;
+                p0 = p1;
+                p1 = p2;
+            }
+            long double dp = q * (t * p1 - p0) / (t * t - 1.0L);
+            long double dt = p1 / dp;
+            t -= dt;
+            if (std::fabs((double)dt) < 1e-19) break;
+        }
+        long double p0 = 1.0L, p1 = t;
+        for (int n = 2; n <= q; n++) {
+            long double p2 = ((2.0L * n - 1.0L) * t * p1 - (n - 1.0L) * p0) / n;
+            p0 = p1;
+            p1 = p2;
+        }
+        long double dp = q * (t * p1 - p0) / (t * t - 1.0L);
+        long double ww = 2.0L / ((1.0L - t * t) * dp * dp);
+        // ascending order on [0,1]
+        x[q - 1 - i] = (double)(0.5L * (1.0L + t));
+        w[q - 1 - i] = (double)(0.5L * ww);
+    }
+}
+
+// Legendre scaling functions phi_i(x) = sqrt(2i+1) P_i(2x-1) on [0,1].
+void legendre_phi(int K, double x, double *out) {
+    double t = 2.0 * x - 1.0;
+    double p0 = 1.0, p1 = t;
+    out[0] = 1.0;
+    if (K > 1) out[1] = std::sqrt(3.0) * t;
+    for (int n = 2; n < K; n++) {
+        double p2 = ((2.0 * n - 1.0) * t * p1 - (n - 1.0) * p0) / n;
+        p0 = p1;
+        p1 = p2;
+        out[n] = std::sqrt(2.0 * n + 1.0) * p2;
+    }
+}
+
+// ---------------------------------------------------------------- basis
+
+struct Basis {
+    int k = 0, K = 0, K2 = 0, K3 = 0, type = 0;  // type 0 Legendre, 1 Interpolating
+    std::vector<double> U;       // K x K, phi^B_i = sum_j U[i][j] phi^L_j
+    std::vector<double> qx, qw;  // K-point Gauss-Legendre on [0,1]
+    std::vector<double> phiq;    // [q][i] = phi^B_i(qx[q])
+    std::vector<double> W;       // 2K x 2K two-scale matrix [[H0 H1][G0 G1]]
+    void phi(double x, double *out) const {
+        double tmp[32];
+        legendre_phi(K, x, tmp);
+        for (int i = 0; i < K; i++) {
+            double s = 0.0;
+            for (int j = 0; j < K; j++) s += U[i * K + j] * tmp[j];
+            out[i] = s;
+        }
+    }
+};
+
+void build_basis(Basis &B, int k, int type) {
+    B.k = k;
+    B.K = k + 1;
+    B.K2 = B.K * B.K;
+    B.K3 = B.K2 * B.K;
+    B.type = type;
+    int K = B.K;
+    B.qx.resize(K);
+    B.qw.resize(K);
+    gl01(K, B.qx.data(), B.qw.data());
+    B.U.assign(K * K, 0.0);
+    if (type == 0) {
+        for (int i = 0; i < K; i++) B.U[i * K + i] = 1.0;
+    } else {
+        // interpolating scaling functions: phi^I_j = sqrt(w_j) sum_i phi^L_i(x_j) phi^L_i
+        double tmp[32];
+        for (int j = 0; j < K; j++) {
+            legendre_phi(K, B.qx[j], tmp);
+            for (int i = 0; i < K; i++) B.U[j * K + i] = std::sqrt(B.qw[j]) * tmp[i];
+        }
+    }
+    B.phiq.resize(K * K);
+    for (int q = 0; q < K; q++) B.phi(B.qx[q], &B.phiq[q * K]);
+
+    // H_a[i][j] = <phi^0_i, phi^1_{a,j}> = 2^{-1/2} int_0^1 phi_i((u+a)/2) phi_j(u) du
+    int N = 2 * K;
+    std::vector<long double> Wl(N * N, 0.0L);
+    double pi_[32], pj[32];
+    for (int a = 0; a < 2; a++) {
+        for (int q = 0; q < K; q++) {
+            double u = B.qx[q];
+            B.phi(0.5 * (u + a), pi_);
+            B.phi(u, pj);
+            for (int i = 0; i < K; i++)
+                for (int j = 0; j < K; j++)
+                    Wl[i * N + a * K + j] += (long double)B.qw[q] * pi_[i] * pj[j] / std::sqrt(2.0L);
+        }
+    }
+    // wavelet rows: Gram-Schmidt of [H0_j, -H1_j] against the scaling rows (two passes)
+    for (int j = 0; j < K; j++) {
+        std::vector<long double> v(N);
+        for (int c = 0; c < K; c++) {
+            v[c] = Wl[j * N + c];
+            v[K + c] = -Wl[j * N + K + c];
+        }
+        for (int pass = 0; pass < 2; pass++) {
+            for (int r = 0; r < K + j; r++) {
+                long double d = 0.0L;
+                for (int c = 0; c < N; c++) d += v[c] * Wl[r * N + c];
+                for (int c = 0; c < N; c++) v[c] -= d * Wl[r * N + c];
+            }
+        }
+        long double nrm = 0.0L;
+        for (int c = 0; c < N; c++) nrm += v[c] * v[c];
+        nrm = std::sqrt(nrm);
+        for (int c = 0; c < N; c++) Wl[(K + j) * N + c] = v[c] / nrm;
+    }
+    B.W.resize(N * N);
+    for (int i = 0; i < N * N; i++) B.W[i] = (double)Wl[i];
+}
+
+// ---------------------------------------------------------------- MRA
+
+struct MRA {
+    int n0 = 0;
+    int corner[3] = {0, 0, 0};
+    int boxes[3] = {1, 1, 1};
+    int maxDepth = 16;
+    double sfac = 1.0;
+    Basis B;
+    double lo(int d) const { return sfac * std::ldexp((double)corner[d], -n0); }
+    double width(int d) const { return sfac * std::ldexp((double)boxes[d], -n0); }
+};
+
+inline uint64_t make_key(int depth, const int64_t *lr) {
+    return ((uint64_t)depth << 57) | ((uint64_t)lr[0] << 38) | ((uint64_t)lr[1] << 19) | (uint64_t)lr[2];
+}
+
+// ---------------------------------------------------------------- tree
+
+struct Tree {
+    const MRA *mra = nullptr;
+    // per node
+    std::vector<int> depth;
+    std::vector<int64_t> lr;   // 3 per node, translation relative to the root corner at that depth
+    std::vector<int> parent;
+    std::vector<int> child;    // index of the first of 8 consecutive children, -1 for leaves
+    std::vector<double> coef;  // 8*K3 per node
+    std::vector<double> cnorm; // 8 per node (component norms)
+    std::unordered_map<uint64_t, int> index;
+    std::vector<int> roots;
+    double sqnorm = 0.0;
+    int nNodes() const { return (int)depth.size(); }
+    double *c(int i) { return &coef[(size_t)i * 8 * mra->B.K3]; }
+    const double *c(int i) const { return &coef[(size_t)i * 8 * mra->B.K3]; }
+    void clear() {
+        depth.clear();
+        lr.clear();
+        parent.clear();
+        child.clear();
+        coef.clear();
+        cnorm.clear();
+        index.clear();
+        roots.clear();
+        sqnorm = 0.0;
+    }
+    int add_node(int d, int64_t x, int64_t y, int64_t z, int par) {
+        int id = nNodes();
+        depth.push_back(d);
+        lr.push_back(x);
+        lr.push_back(y);
+        lr.push_back(z);
+        parent.push_back(par);
+        child.push_back(-1);
+        coef.resize((size_t)(id + 1) * 8 * mra->B.K3, 0.0);
+        cnorm.resize((size_t)(id + 1) * 8, 0.0);
+        int64_t l3[3] = {x, y, z};
+        index[make_key(d, l3)] = id;
+        return id;
+    }
+    void make_roots() {
+        clear();
+        for (int z = 0; z < mra->boxes[2]; z++)
+            for (int y = 0; y < mra->boxes[1]; y++)
+                for (int x = 0; x < mra->boxes[0]; x++) roots.push_back(add_node(0, x, y, z, -1));
+    }
+    // split node i: append its 8 children (bit 0 = x, bit 1 = y, bit 2 = z)
+    void split(int i) {
+        int d = depth[i];
+        int64_t x = lr[3 * i], y = lr[3 * i + 1], z = lr[3 * i + 2];
+        int first = nNodes();
+        for (int a = 0; a < 8; a++) add_node(d + 1, 2 * x + (a & 1), 2 * y + ((a >> 1) & 1), 2 * z + ((a >> 2) & 1), i);
+        child[i] = first;
+    }
+    int find(int d, const int64_t *l3) const {
+        auto it = index.find(make_key(d, l3));
+        return it == index.end() ? -1 : it->second;
+    }
+    void calc_norms(int i) {
+        int K3 = mra->B.K3;
+        const double *p = c(i);
+        for (int cc = 0; cc < 8; cc++) {
+            double s = 0.0;
+            for (int j = 0; j < K3; j++) s += p[cc * K3 + j] * p[cc * K3 + j];
+            cnorm[8 * i + cc] = std::sqrt(s);
+        }
+    }
+    void calc_sqnorm() {
+        double s = 0.0;
+        for (int r : roots) s += cnorm[8 * r] * cnorm[8 * r];
+        for (int i = 0; i < nNodes(); i++)
+            for (int cc = 1; cc < 8; cc++) s += cnorm[8 * i + cc] * cnorm[8 * i + cc];
+        sqnorm = s;
+    }
+};
+
+// ---------------------------------------------------------------- tensor transforms
+
+// 8 components (bit per dim) of K^3 <-> one (2K)^3 tensor with per-dim index bit*K+i.
+void comps_to_tensor(int K, const double *comps, double *t) {
+    int N = 2 * K, K3 = K * K * K;
+    for (int cc = 0; cc < 8; cc++) {
+        int bx = cc & 1, by = (cc >> 1) & 1, bz = (cc >> 2) & 1;
+        for (int z = 0; z < K; z++)
+            for (int y = 0; y < K; y++)
+                for (int x = 0; x < K; x++)
+                    t[((bz * K + z) * N + (by * K + y)) * N + bx * K + x] = comps[cc * K3 + (z * K + y) * K + x];
+    }
+}
+void tensor_to_comps(int K, const double *t, double *comps) {
+    int N = 2 * K, K3 = K * K * K;
+    for (int cc = 0; cc < 8; cc++) {
+        int bx = cc & 1, by = (cc >> 1) & 1, bz = (cc >> 2) & 1;
+        for (int z = 0; z < K; z++)
+            for (int y = 0; y < K; y++)
+                for (int x = 0; x < K; x++)
+                    comps[cc * K3 + (z * K + y) * K + x] = t[((bz * K + z) * N + (by * K + y)) * N + bx * K + x];
+    }
+}
+// out = (M (x) M (x) M) in on an N^3 tensor, M row-major N x N (out index = row).
+void mat3(int N, const double *M, const double *in, double *out, double *tmp) {
+    int N2 = N * N;
+    // x
+    for (int zy = 0; zy < N2; zy++)
+        for (int a = 0; a < N; a++) {
+            double s = 0.0;
+            for (int i = 0; i < N; i++) s += M[a * N + i] * in[zy * N + i];
+            out[zy * N + a] = s;
+        }
+    // y
+    for (int z = 0; z < N; z++)
+        for (int b = 0; b < N; b++)
+            for (int a = 0; a < N; a++) {
+                double s = 0.0;
+                for (int j = 0; j < N; j++) s += M[b * N + j] * out[(z * N + j) * N + a];
+                tmp[(z * N + b) * N + a] = s;
+            }
+    // z
+    for (int cz = 0; cz < N; cz++)
+        for (int ba = 0; ba < N2; ba++) {
+            double s = 0.0;
+            for (int l = 0; l < N; l++) s += M[cz * N + l] * tmp[l * N2 + ba];
+            out[cz * N2 + ba] = s;
+        }
+}
+
+// children scaling blocks (8 x K3, child bits per dim) -> parent (s, d) components
+void compress(const Basis &B, const double *children, double *parent) {
+    int N = 2 * B.K;
+    std::vector<double> t(N * N * N), o(N * N * N), tmp(N * N * N);
+    comps_to_tensor(B.K, children, t.data());
+    mat3(N, B.W.data(), t.data(), o.data(), tmp.data());
+    tensor_to_comps(B.K, o.data(), parent);
+}
+// parent (s, d) components -> children scaling blocks
+void reconstruct(const Basis &B, const double *parent, double *children) {
+    int N = 2 * B.K;
+    std::vector<double> WT(N * N);
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++) WT[i * N + j] = B.W[j * N + i];
+    std::vector<double> t(N * N * N), o(N * N * N), tmp(N * N * N);
+    comps_to_tensor(B.K, parent, t.data());
+    mat3(N, WT.data(), t.data(), o.data(), tmp.data());
+    tensor_to_comps(B.K, o.data(), children);
+}
+
+// out[c][b][a] += w * sum X[a][i] Y[b][j] Z[c][l] f[l][j][i]
+inline void contract3(int K, const double *X, const double *Y, const double *Z, double w, const double *f,
+                      double *out, double *t1, double *t2) {
+    int K2 = K * K;
+    for (int lj = 0; lj < K2; lj++) {
+        const double *fr = f + lj * K;
+        for (int a = 0; a < K; a++) {
+            double s = 0.0;
+            for (int i = 0; i < K; i++) s += X[a * K + i] * fr[i];
+            t1[lj * K + a] = s;
+        }
+    }
+    for (int l = 0; l < K; l++)
+        for (int b = 0; b < K; b++) {
+            double *dst = t2 + (l * K + b) * K;
+            for (int a = 0; a < K; a++) dst[a] = 0.0;
+            for (int j = 0; j < K; j++) {
+                double y = Y[b * K + j];
+                const double *src = t1 + (l * K + j) * K;
+                for (int a = 0; a < K; a++) dst[a] += y * src[a];
+            }
+        }
+    for (int c = 0; c < K; c++) {
+        double *dst = out + c * K2;
+        for (int l = 0; l < K; l++) {
+            double z = w * Z[c * K + l];
+            const double *src = t2 + l * K2;
+            for (int ba = 0; ba < K2; ba++) dst[ba] += z * src[ba];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- analytic functions
+
+struct GaussTerm {
+    double coef, alpha, center[3];
+    double poly[3][5];  // per-dim polynomial in (x_d - center_d), degree <= 4
+};
+
+struct HydTerm {
+    double coef;
+    int q1[3], q2[3];  // (n,l,m) of one or two hydrogen orbitals (q2[0] == 0: single orbital)
+};
+
+struct Func {
+    int kind = 0;  // 0 Gaussian sum, 1 sum of hydrogen orbitals / products of two
+    std::vector<GaussTerm> g;
+    std::vector<HydTerm> h;
+    double hZ = 1.0, horig[3] = {0, 0, 0};
+};
+
+// hydrogen-like orbitals as in src/analyticfunctions/HydrogenFunction.cpp:34-165
+double hydrogen_orb(int N, int L, int M, double Z, double x, double y, double z) {
+    double r = std::sqrt(x * x + y * y + z * z);
+    double c0;
+    if (N == 1 && L == 0) c0 = 2.0;
+    else if (N == 2 && L == 0) c0 = 1.0 / (2.0 * std::sqrt(2.0));
+    else if (N == 3 && L == 0) c0 = 1.0 / (9.0 * std::sqrt(3.0));
+    else if (N == 2 && L == 1) c0 = 1.0 / (2.0 * std::sqrt(6.0));
+    else if (N == 3 && L == 1) c0 = 1.0 / (9.0 * std::sqrt(6.0));
+    else if (N == 3 && L == 2) c0 = 1.0 / (9.0 * std::sqrt(30.0));
+    else return 0.0;
+    c0 *= std::pow(Z, 1.5);
+    double rho = 2.0 * Z / N * r;
+    double poly;
+    if (N == 1 && L == 0) poly = 1.0;
+    else if (N == 2 && L == 0) poly = 2.0 - rho;
+    else if (N == 3 && L == 0) poly = 6.0 - 6.0 * rho + rho * rho;
+    else if (N == 2 && L == 1) poly = rho;
+    else if (N == 3 && L == 1) poly = rho * (4.0 - rho);
+    else poly = rho * rho;
+    double R = c0 * poly * std::exp(-rho / 2.0);
+    double ang, ca;
+    const double sqpi2 = 2.0 * std::sqrt(M_PI);
+    if (L == 0) {
+        ca = 1.0;
+        ang = 1.0;
+    } else if (L == 1) {
+        ca = std::sqrt(3.0);
+        double q = (M == 0) ? x : (M == 1) ? y : z;
+        ang = (r > 0.0) ? q / r : 0.0;
+    } else {
+        double r2 = r * r;
+        if (M <= 2) ca = std::sqrt(15.0);
+        else if (M == 3) ca = std::sqrt(15.0 / 4.0);
+        else ca = std::sqrt(5.0 / 4.0);
+        if (r2 == 0.0) ang = 0.0;
+        else if (M == 0) ang = x * y / r2;
+        else if (M == 1) ang = x * z / r2;
+        else if (M == 2) ang = y * z / r2;
+        else if (M == 3) ang = (x * x - y * y) / r2;
+        else ang = (2.0 * z * z - x * x - y * y) / r2;
+    }
+    return R * ca / sqpi2 * ang;
+}
+
+double hydrogen_eval(const Func &F, double x, double y, double z) {
+    x -= F.horig[0];
+    y -= F.horig[1];
+    z -= F.horig[2];
+    double v = 0.0;
+    for (const HydTerm &t : F.h) {
+        double a = hydrogen_orb(t.q1[0], t.q1[1], t.q1[2], F.hZ, x, y, z);
+        if (t.q2[0] > 0) a *= hydrogen_orb(t.q2[0], t.q2[1], t.q2[2], F.hZ, x, y, z);
+        v += t.coef * a;
+    }
+    return v;
+}
+
+// scaling coefficients of F on the cell (scale n = n0+depth, translation lr) by K^3 quadrature
+void project_cell(const MRA &mra, const Func &F, int depth, const int64_t *lr, double *s) {
+    const Basis &B = mra.B;
+    int K = B.K, K3 = B.K3;
+    double h = mra.sfac * std::ldexp(1.0, -(mra.n0 + depth));
+    double pre = std::sqrt(h);
+    double xq[3][32];
+    for (int d = 0; d < 3; d++) {
+        double x0 = mra.lo(d) + h * (double)lr[d];
+        for (int q = 0; q < K; q++) xq[d][q] = x0 + h * B.qx[q];
+    }
+    std::fill(s, s + K3, 0.0);
+    if (F.kind == 0) {
+        double pd[3][32];
+        for (const GaussTerm &t : F.g) {
+            for (int d = 0; d < 3; d++) {
+                double vq[32];
+                for (int q = 0; q < K; q++) {
+                    double u = xq[d][q] - t.center[d];
+                    double p = 0.0, up = 1.0;
+                    for (int e = 0; e < 5; e++) {
+                        p += t.poly[d][e] * up;
+                        up *= u;
+                    }
+                    vq[q] = p * std::exp(-t.alpha * u * u) * B.qw[q];
+                }
+                for (int i = 0; i < K; i++) {
+                    double acc = 0.0;
+                    for (int q = 0; q < K; q++) acc += B.phiq[q * K + i] * vq[q];
+                    pd[d][i] = acc * pre;
+                }
+            }
+            for (int z = 0; z < K; z++)
+                for (int y = 0; y < K; y++) {
+                    double czy = t.coef * pd[2][z] * pd[1][y];
+                    double *dst = s + (z * K + y) * K;
+                    for (int x = 0; x < K; x++) dst[x] += czy * pd[0][x];
+                }
+        }
+    } else {
+        std::vector<double> fv(K3);
+        for (int z = 0; z < K; z++)
+            for (int y = 0; y < K; y++)
+                for (int x = 0; x < K; x++)
+                    fv[(z * K + y) * K + x] =
+                        hydrogen_eval(F, xq[0][x], xq[1][y], xq[2][z]) * B.qw[x] * B.qw[y] * B.qw[z];
+        // s[c][b][a] = pre^3 sum phi_a(x)phi_b(y)phi_c(z) fv
+        std::vector<double> t1(K3), t2(K3);
+        for (int zy = 0; zy < K * K; zy++)
+            for (int a = 0; a < K; a++) {
+                double acc = 0.0;
+                for (int x = 0; x < K; x++) acc += B.phiq[x * K + a] * fv[zy * K + x];
+                t1[zy * K + a] = acc;
+            }
+        for (int z = 0; z < K; z++)
+            for (int b = 0; b < K; b++)
+                for (int a = 0; a < K; a++) {
+                    double acc = 0.0;
+                    for (int y = 0; y < K; y++) acc += B.phiq[y * K + b] * t1[(z * K + y) * K + a];
+                    t2[(z * K + b) * K + a] = acc;
+                }
+        for (int c = 0; c < K; c++)
+            for (int ba = 0; ba < K * K; ba++) {
+                double acc = 0.0;
+                for (int z = 0; z < K; z++) acc += B.phiq[z * K + c] * t2[z * K * K + ba];
+                s[c * K * K + ba] = acc * pre * pre * pre;
+            }
+    }
+}
+
+// Analytic projection cannot see a Gaussian narrower than the quadrature
+// spacing: a node whose quadrature cell (the children's cell) is wider than
+// K * 2 sigma (sigma = (2 alpha)^{-1/2}) of a term that is not negligible on the
+// node (alpha * dist^2 <= 40) is split regardless of its wavelet norm.
+bool must_refine(const MRA &mra, const Func &F, int depth, const int64_t *lr) {
+    if (F.kind != 0) return false;
+    int K = mra.B.K;
+    double h = mra.sfac * std::ldexp(1.0, -(mra.n0 + depth));
+    double hc = 0.5 * h;
+    for (const GaussTerm &t : F.g) {
+        double two_sigma = 2.0 / std::sqrt(2.0 * t.alpha);
+        if (hc <= K * two_sigma) continue;
+        double d2 = 0.0;
+        for (int d = 0; d < 3; d++) {
+            double lo = mra.lo(d) + h * (double)lr[d], hi = lo + h;
+            double c = t.center[d];
+            double dd = c < lo ? lo - c : (c > hi ? c - hi : 0.0);
+            d2 += dd * dd;
+        }
+        if (t.alpha * d2 <= 40.0) return true;
+    }
+    return false;
+}
+
+// split rule shared by projection and apply: ||w|| > prec * ||tree|| * 2^{-(n+1)/2}
+inline bool split_check(const MRA &mra, int depth, double wnorm, double prec, double tnorm) {
+    int n = mra.n0 + depth;
+    double thr = std::max(2.0 * 2.220446049250313e-16, prec * tnorm * std::pow(2.0, -0.5 * (n + 1)));
+    return wnorm > thr;
+}
+
+void bottom_up(Tree &T) {
+    const Basis &B = T.mra->B;
+    int K3 = B.K3;
+    std::vector<double> ch(8 * K3);
+    for (int i = T.nNodes() - 1; i >= 0; i--) {
+        int c0 = T.child[i];
+        if (c0 < 0) continue;
+        for (int a = 0; a < 8; a++) std::memcpy(&ch[a * K3], T.c(c0 + a), K3 * sizeof(double));
+        compress(B, ch.data(), T.c(i));
+        T.calc_norms(i);
+    }
+    T.calc_sqnorm();
+}
+
+void project(Tree &T, const Func &F, double prec) {
+    const MRA &mra = *T.mra;
+    const Basis &B = mra.B;
+    int K3 = B.K3;
+    T.make_roots();
+    std::vector<int> work = T.roots;
+    double sNorm = 0.0, wNorm = 0.0;
+    int iter = 0;
+    while (!work.empty()) {
+        int nw = (int)work.size();
+#pragma omp parallel
+        {
+            std::vector<double> ch(8 * K3);
+#pragma omp for schedule(dynamic, 4)
+            for (int w = 0; w < nw; w++) {
+                int i = work[w];
+                int d = T.depth[i];
+                const int64_t *l = &T.lr[3 * i];
+                for (int a = 0; a < 8; a++) {
+                    int64_t lc[3] = {2 * l[0] + (a & 1), 2 * l[1] + ((a >> 1) & 1), 2 * l[2] + ((a >> 2) & 1)};
+                    project_cell(mra, F, d + 1, lc, &ch[a * K3]);
+                }
+                compress(B, ch.data(), T.c(i));
+                T.calc_norms(i);
+            }
+        }
+        for (int i : work) {
+            if (iter == 0) sNorm += T.cnorm[8 * i] * T.cnorm[8 * i];
+            for (int cc = 1; cc < 8; cc++) wNorm += T.cnorm[8 * i + cc] * T.cnorm[8 * i + cc];
+        }
+        double tnorm = std::sqrt(sNorm + wNorm);
+        std::vector<int> next;
+        for (int i : work) {
+            int d = T.depth[i];
+            if (d >= mra.maxDepth) continue;
+            double wn = 0.0;
+            for (int cc = 1; cc < 8; cc++) wn += T.cnorm[8 * i + cc] * T.cnorm[8 * i + cc];
+            if (split_check(mra, d, std::sqrt(wn), prec, tnorm) || must_refine(mra, F, d, &T.lr[3 * i])) {
+                T.split(i);
+                for (int a = 0; a < 8; a++) next.push_back(T.child[i] + a);
+            }
+        }
+        work.swap(next);
+        iter++;
+    }
+    bottom_up(T);
+}
+
+// value of the tree's function at a point
+double eval_point(const Tree &T, const double *p) {
+    const MRA &mra = *T.mra;
+    const Basis &B = mra.B;
+    int K = B.K, K3 = B.K3;
+    double u0[3];
+    for (int d = 0; d < 3; d++) {
+        u0[d] = (p[d] - mra.lo(d)) / (mra.sfac * std::ldexp(1.0, -mra.n0));  // in root-cell units
+        if (u0[d] < 0.0 || u0[d] >= mra.boxes[d]) return 0.0;
+    }
+    int64_t l[3] = {(int64_t)u0[0], (int64_t)u0[1], (int64_t)u0[2]};
+    int node = T.find(0, l);
+    if (node < 0) return 0.0;
+    int depth = 0;
+    while (T.child[node] >= 0) {
+        depth++;
+        int a = 0;
+        int64_t lc[3];
+        for (int d = 0; d < 3; d++) {
+            double u = std::ldexp(u0[d], depth);
+            lc[d] = (int64_t)u;
+        }
+        for (int d = 0; d < 3; d++) a |= (int)(lc[d] & 1) << d;
+        node = T.child[node] + a;
+    }
+    // leaf at depth: reconstruct children and evaluate in the child containing p
+    std::vector<double> ch(8 * K3);
+    reconstruct(B, T.c(node), ch.data());
+    int cd = depth + 1;
+    int a = 0;
+    double uc[3];
+    for (int d = 0; d < 3; d++) {
+        double u = std::ldexp(u0[d], cd);
+        int64_t li = (int64_t)u;
+        uc[d] = u - (double)li;
+        a |= (int)(li & 1) << d;
+    }
+    double px[32], py[32], pz[32];
+    B.phi(uc[0], px);
+    B.phi(uc[1], py);
+    B.phi(uc[2], pz);
+    const double *s = &ch[a * K3];
+    double acc = 0.0;
+    for (int z = 0; z < K; z++)
+        for (int y = 0; y < K; y++)
+            for (int x = 0; x < K; x++) acc += s[(z * K + y) * K + x] * px[x] * py[y] * pz[z];
+    double h = mra.sfac * std::ldexp(1.0, -(mra.n0 + cd));
+    return acc / std::pow(h, 1.5);
+}
+
+double tree_dot(const Tree &A, const Tree &Bt) {
+    int K3 = A.mra->B.K3;
+    double s = 0.0;
+    for (size_t r = 0; r < A.roots.size(); r++) {
+        const double *a = A.c(A.roots[r]);
+        const double *b = Bt.c(Bt.roots[r]);
+        for (int j = 0; j < K3; j++) s += a[j] * b[j];
+    }
+    for (int i = 0; i < A.nNodes(); i++) {
+        int j = Bt.find(A.depth[i], &A.lr[3 * i]);
+        if (j < 0) continue;
+        const double *a = A.c(i);
+        const double *b = Bt.c(j);
+        for (int q = K3; q < 8 * K3; q++) s += a[q] * b[q];
+    }
+    return s;
+}
+
+// ---------------------------------------------------------------- kernel fit
+
+struct KernelFit {
+    std::vector<double> a, w;  // exp(-a r^2) terms, physical units
+    double rmin = 0, rmax = 0, eps = 0, h = 0, err = 0;
+};
+
+double fit_error(const KernelFit &F, double mu) {
+    const int J = 400;
+    double err = 0.0;
+    for (int j = 0; j < J; j++) {
+        double r = F.rmin * std::pow(F.rmax / F.rmin, j / (double)(J - 1));
+        double f = 0.0;
+        for (size_t i = 0; i < F.a.size(); i++) f += F.w[i] * std::exp(-F.a[i] * r * r);
+        double ex = std::exp(-mu * r) / r;
+        err = std::max(err, std::fabs(f - ex) * r);
+    }
+    return err;
+}
+
+// 1/r = (2/sqrt(pi)) int exp(-r^2 e^{2s} + s) ds,
+// exp(-mu r)/r = (2/sqrt(pi)) int exp(-r^2 e^{2s} - mu^2 e^{-2s}/4 + s) ds, trapezoid in s.
+// Terms that are constant over the box to eps^0.4 are merged into one term.
+KernelFit fit_kernel(double mu, double eps, double rmin, double rmax) {
+    double L = std::log(10.0 / eps);
+    double thi = std::sqrt(L);
+    double shi = std::log(thi / rmin);
+    double slo = std::log(eps * std::sqrt(M_PI) / (20.0 * rmax));
+    if (mu > 0.0) slo = std::max(slo, -0.5 * std::log(4.0 * L / (mu * mu)));
+    double tau = std::pow(eps, 0.4);
+    KernelFit F;
+    F.rmin = rmin;
+    F.rmax = rmax;
+    F.eps = eps;
+    for (int m = 0; m < 400; m++) {
+        double h0 = std::pow(0.95, m);
+        int N = (int)std::ceil((shi - slo) / h0) + 1;
+        double h = (shi - slo) / (N - 1);
+        std::vector<double> a, w;
+        double Wm = 0.0, Am = 0.0;
+        int nm = 0;
+        for (int i = 0; i < N; i++) {
+            double s = slo + h * i;
+            double ai = std::exp(2.0 * s);
+            double wi = 2.0 / std::sqrt(M_PI) * h * std::exp(s) * std::exp(-mu * mu * std::exp(-2.0 * s) / 4.0);
+            if (ai * rmax * rmax < tau) {
+                Wm += wi;
+                Am += wi * ai;
+                nm++;
+            } else {
+                a.push_back(ai);
+                w.push_back(wi);
+            }
+        }
+        F.a.clear();
+        F.w.clear();
+        if (nm > 0 && Wm > 0.0) {
+            F.a.push_back(Am / Wm);
+            F.w.push_back(Wm);
+        }
+        for (size_t i = 0; i < a.size(); i++) {
+            F.a.push_back(a[i]);
+            F.w.push_back(w[i]);
+        }
+        // drop terms that never reach eps*1e-3 on [rmin, rmax] (relative to 1/r)
+        std::vector<double> a2, w2;
+        for (size_t i = 0; i < F.a.size(); i++) {
+            double rs = std::min(std::max(1.0 / std::sqrt(2.0 * F.a[i]), rmin), rmax);
+            if (F.w[i] * rs * std::exp(-F.a[i] * rs * rs) >= eps * 1e-3) {
+                a2.push_back(F.a[i]);
+                w2.push_back(F.w[i]);
+            }
+        }
+        F.a = a2;
+        F.w = w2;
+        F.h = h;
+        F.err = fit_error(F, mu);
+        if (F.err <= eps) break;
+    }
+    return F;
+}
+
+// ---------------------------------------------------------------- operator
+
+struct OperLevel {
+    // one Gaussian term at one depth
+    bool active = false;
+    int band = -1;               // delta in [-band, band]
+    std::vector<double> blocks;  // [(delta+band)*4 + type][a*K+i], type = 2*gt_bit + ft_bit (T,C,B,A)
+    std::vector<double> norms;   // [(delta+band)*4 + type]
+    std::vector<double> env;     // [type*(band+1) + m] = max_{|delta|>=m} norm
+    double maxn[4] = {0, 0, 0, 0};
+};
+
+struct Oper {
+    const MRA *mra = nullptr;
+    double mu = 0.0, prec = 0.0;
+    KernelFit fit;
+    std::vector<double> beta;  // unit-coordinate exponents a*sfac^2
+    std::vector<double> weff;  // w*sfac^3
+    std::vector<std::vector<OperLevel>> lev;  // [depth][term]
+    std::mutex mtx;
+};
+
+// cross-correlation C_ij(z) = int phi_i(u) phi_j(u - z) du, z in [-1,1]
+void cross_corr(const Basis &B, double z, double *Cz) {
+    int K = B.K;
+    double lo = std::max(0.0, z), hi = std::min(1.0, 1.0 + z);
+    std::fill(Cz, Cz + K * K, 0.0);
+    if (hi <= lo) return;
+    double pu[32], pv[32];
+    for (int q = 0; q < K; q++) {
+        double u = lo + (hi - lo) * B.qx[q];
+        double wq = (hi - lo) * B.qw[q];
+        B.phi(u, pu);
+        B.phi(u - z, pv);
+        for (int i = 0; i < K; i++)
+            for (int j = 0; j < K; j++) Cz[i * K + j] += wq * pu[i] * pv[j];
+    }
+}
+
+// t[i][j] = 2^{-m} int_{-1}^{1} exp(-b (delta+z)^2) C_ij(z) dz with b = beta 4^{-m}
+void gauss_block(const Basis &B, double b, double pre, int64_t delta, double *t) {
+    int K = B.K, K2 = K * K;
+    std::fill(t, t + K2, 0.0);
+    static thread_local std::vector<double> gx, gw;
+    const int QP = 24;
+    if (gx.empty()) {
+        gx.resize(QP);
+        gw.resize(QP);
+        gl01(QP, gx.data(), gw.data());
+    }
+    double zc = -(double)delta;
+    double R = std::sqrt(46.0 / b);
+    double pw = (b > 4.0) ? 0.5 / std::sqrt(b) : 0.25;
+    std::vector<double> Cz(K2);
+    for (int half = 0; half < 2; half++) {
+        double lo = half == 0 ? -1.0 : 0.0, hi = half == 0 ? 0.0 : 1.0;
+        double a0 = std::max(lo, zc - R), a1 = std::min(hi, zc + R);
+        if (a1 <= a0) continue;
+        int np = std::max(1, (int)std::ceil((a1 - a0) / pw));
+        double ph = (a1 - a0) / np;
+        for (int p = 0; p < np; p++) {
+            double p0 = a0 + p * ph;
+            for (int q = 0; q < QP; q++) {
+                double z = p0 + ph * gx[q];
+                double g = std::exp(-b * (delta + z) * (delta + z)) * ph * gw[q];
+                if (g == 0.0) continue;
+                cross_corr(B, z, Cz.data());
+                for (int ij = 0; ij < K2; ij++) t[ij] += g * Cz[ij];
+            }
+        }
+    }
+    for (int ij = 0; ij < K2; ij++) t[ij] *= pre;
+}
+
+double fro(const double *m, int n) {
+    double s = 0.0;
+    for (int i = 0; i < n; i++) s += m[i] * m[i];
+    return std::sqrt(s);
+}
+
+// NS blocks of term `term` at depth d for deltas in [dlo, dhi] (into a dense buffer)
+void ns_blocks(const Oper &O, int term, int d, int64_t dlo, int64_t dhi, std::vector<double> &out) {
+    const MRA &mra = *O.mra;
+    const Basis &B = mra.B;
+    int K = B.K, K2 = K * K, N = 2 * K;
+    int m1 = mra.n0 + d + 1;  // child scale
+    double b1 = O.beta[term] * std::ldexp(1.0, -2 * m1);
+    double pre1 = std::ldexp(1.0, -m1);
+    int64_t boxmax = 0;
+    for (int q = 0; q < 3; q++) boxmax = std::max<int64_t>(boxmax, mra.boxes[q]);
+    int64_t lmax1 = (boxmax << (d + 1)) - 1;
+    int64_t band1 = std::min<int64_t>(lmax1, (int64_t)std::ceil(1.0 + std::sqrt(46.0 / b1)));
+    int64_t t_lo = std::max<int64_t>(-band1, 2 * dlo - 1), t_hi = std::min<int64_t>(band1, 2 * dhi + 1);
+    int64_t nt = t_hi >= t_lo ? t_hi - t_lo + 1 : 0;
+    std::vector<double> t1(std::max<int64_t>(nt, 1) * K2, 0.0);
+    for (int64_t q = 0; q < nt; q++) gauss_block(B, b1, pre1, t_lo + q, &t1[q * K2]);
+    auto T1 = [&](int64_t dd) -> const double * {
+        static thread_local std::vector<double> zero;
+        if (dd < t_lo || dd > t_hi) {
+            zero.assign(K2, 0.0);
+            return zero.data();
+        }
+        return &t1[(dd - t_lo) * K2];
+    };
+    int64_t nd = dhi - dlo + 1;
+    out.assign(nd * 4 * K2, 0.0);
+    std::vector<double> M(N * N), R1(N * N), R(N * N);
+    for (int64_t q = 0; q < nd; q++) {
+        int64_t dl = dlo + q;
+        const double *blk[4] = {T1(2 * dl), T1(2 * dl - 1), T1(2 * dl + 1), T1(2 * dl)};
+        for (int ab = 0; ab < 4; ab++) {
+            int a = ab >> 1, bb = ab & 1;
+            for (int i = 0; i < K; i++)
+                for (int j = 0; j < K; j++) M[(a * K + i) * N + bb * K + j] = blk[ab][i * K + j];
+        }
+        // R = W M W^T
+        for (int i = 0; i < N; i++)
+            for (int j = 0; j < N; j++) {
+                double s = 0.0;
+                for (int l = 0; l < N; l++) s += B.W[i * N + l] * M[l * N + j];
+                R1[i * N + j] = s;
+            }
+        for (int i = 0; i < N; i++)
+            for (int j = 0; j < N; j++) {
+                double s = 0.0;
+                for (int l = 0; l < N; l++) s += R1[i * N + l] * B.W[j * N + l];
+                R[i * N + j] = s;
+            }
+        for (int type = 0; type < 4; type++) {
+            int gb = type >> 1, fb = type & 1;
+            double *dst = &out[(q * 4 + type) * K2];
+            for (int a = 0; a < K; a++)
+                for (int i = 0; i < K; i++) dst[a * K + i] = R[(gb * K + a) * N + fb * K + i];
+        }
+    }
+}
+
+// activity threshold: a term is kept at depth > 0 only if its wavelet blocks reach
+// ACT_REL times its scaling block near the diagonal
+const double ACT_REL = 1e-4;
+
+void build_level(Oper &O, int d) {
+    const MRA &mra = *O.mra;
+    const Basis &B = mra.B;
+    int K2 = B.K2;
+    int nt = (int)O.beta.size();
+    std::vector<OperLevel> L(nt);
+    int64_t boxmax = 0;
+    for (int q = 0; q < 3; q++) boxmax = std::max<int64_t>(boxmax, mra.boxes[q]);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int t = 0; t < nt; t++) {
+        OperLevel &lv = L[t];
+        int m1 = mra.n0 + d + 1;
+        double b1 = O.beta[t] * std::ldexp(1.0, -2 * m1);
+        int64_t lmax1 = (boxmax << (d + 1)) - 1;
+        int64_t band1 = std::min<int64_t>(lmax1, (int64_t)std::ceil(1.0 + std::sqrt(46.0 / b1)));
+        int64_t band = std::min<int64_t>((boxmax << d) - 1, (band1 + 2) / 2);
+        std::vector<double> near;
+        int64_t nb = std::min<int64_t>(band, 1);
+        ns_blocks(O, t, d, -nb, nb, near);
+        double tn = 0.0, wn = 0.0;
+        for (int64_t q = 0; q < 2 * nb + 1; q++) {
+            tn = std::max(tn, fro(&near[(q * 4 + 0) * K2], K2));
+            for (int type = 1; type < 4; type++) wn = std::max(wn, fro(&near[(q * 4 + type) * K2], K2));
+        }
+        lv.active = (d == 0) || (wn > ACT_REL * O.prec * tn);
+        if (!lv.active) continue;
+        lv.band = (int)band;
+        ns_blocks(O, t, d, -band, band, lv.blocks);
+        int64_t nd = 2 * band + 1;
+        lv.norms.resize(nd * 4);
+        for (int64_t q = 0; q < nd; q++)
+            for (int type = 0; type < 4; type++) {
+                double nrm = fro(&lv.blocks[(q * 4 + type) * K2], K2);
+                lv.norms[q * 4 + type] = nrm;
+                lv.maxn[type] = std::max(lv.maxn[type], nrm);
+            }
+        lv.env.assign(4 * (band + 1), 0.0);
+        for (int type = 0; type < 4; type++) {
+            double run = 0.0;
+            for (int64_t m = band; m >= 0; m--) {
+                run = std::max(run, std::max(lv.norms[(band + m) * 4 + type], lv.norms[(band - m) * 4 + type]));
+                lv.env[type * (band + 1) + m] = run;
+            }
+        }
+    }
+    O.lev[d] = std::move(L);
+}
+
+void ensure_levels(Oper &O, int d) {
+    std::lock_guard<std::mutex> g(O.mtx);
+    if ((int)O.lev.size() <= d) O.lev.resize(d + 1);
+    for (int q = 0; q <= d; q++)
+        if (O.lev[q].empty()) build_level(O, q);
+}
+
+void build_oper(Oper &O, const MRA *mra, double mu, double prec) {
+    O.mra = mra;
+    O.mu = mu;
+    O.prec = prec;
+    double eps = prec / 10.0;
+    double rmin = 0.1 * std::sqrt(eps);
+    double w = 0.0;
+    for (int d = 0; d < 3; d++) w = std::max(w, mra->width(d));
+    double rmax = std::sqrt(3.0) * w;
+    O.fit = fit_kernel(mu, eps, rmin, rmax);
+    int nt = (int)O.fit.a.size();
+    O.beta.resize(nt);
+    O.weff.resize(nt);
+    double s = mra->sfac;
+    for (int t = 0; t < nt; t++) {
+        O.beta[t] = O.fit.a[t] * s * s;
+        O.weff[t] = O.fit.w[t] * s * s * s;
+    }
+}
+
+// ---------------------------------------------------------------- apply
+
+struct ApplyStats {
+    int64_t tuples = 0;       // surviving (g, f, ft, gt, term) tuples
+    int64_t gen_nodes = 0;    // generated input nodes
+    int64_t nodes_out = 0;    // nodes in the output tree
+    int64_t iters = 0;
+    double t_total = 0.0;     // seconds
+    int64_t tuples_iter[64] = {0};
+    int64_t nodes_iter[64] = {0};
+};
+
+struct GenEntry {
+    std::vector<double> s;
+    double norm;
+};
+
+struct ApplyCtx {
+    const Oper *O;
+    const Tree *in;
+    const MRA *mra;
+    std::unordered_map<uint64_t, std::unique_ptr<GenEntry>> gen;
+    std::mutex gmtx;
+    std::vector<std::array<double, 8>> fmax;  // per depth, max component norm (ft=0 includes ancestors' full norms)
+};
+
+// scaling block of the input at (d, lr) where the input tree is coarser: refine from the leaf ancestor
+const GenEntry *get_gen(ApplyCtx &C, int d, const int64_t *lr) {
+    int64_t l3[3] = {lr[0], lr[1], lr[2]};
+    uint64_t key = make_key(d, l3);
+    {
+        std::lock_guard<std::mutex> g(C.gmtx);
+        auto it = C.gen.find(key);
+        if (it != C.gen.end()) return it->second.get();
+    }
+    const Tree &T = *C.in;
+    const Basis &B = C.mra->B;
+    int K = B.K, K3 = B.K3;
+    // deepest existing ancestor
+    int anc = -1, ad = -1;
+    for (int dd = d - 1; dd >= 0; dd--) {
+        int64_t la[3] = {lr[0] >> (d - dd), lr[1] >> (d - dd), lr[2] >> (d - dd)};
+        anc = T.find(dd, la);
+        if (anc >= 0) {
+            ad = dd;
+            break;
+        }
+    }
+    auto e = std::make_unique<GenEntry>();
+    e->s.assign(K3, 0.0);
+    if (anc >= 0) {
+        std::vector<double> ch(8 * K3), cur(K3), nxt(K3), t1(K3), t2(K3);
+        // first step uses the leaf's wavelets
+        reconstruct(B, T.c(anc), ch.data());
+        int step = d - ad;
+        {
+            int sh = step - 1;
+            int a = (int)((lr[0] >> sh) & 1) | (int)(((lr[1] >> sh) & 1) << 1) | (int)(((lr[2] >> sh) & 1) << 2);
+            std::memcpy(cur.data(), &ch[a * K3], K3 * sizeof(double));
+        }
+        for (int sh = step - 2; sh >= 0; sh--) {
+            int ax = (int)((lr[0] >> sh) & 1), ay = (int)((lr[1] >> sh) & 1), az = (int)((lr[2] >> sh) & 1);
+            // s_child = (H_ax^T (x) H_ay^T (x) H_az^T) s ; H_a[i][j] = W[i][a*K+j]
+            const double *W = B.W.data();
+            int N = 2 * K;
+            for (int zy = 0; zy < K * K; zy++)
+                for (int j = 0; j < K; j++) {
+                    double acc = 0.0;
+                    for (int i = 0; i < K; i++) acc += W[i * N + ax * K + j] * cur[zy * K + i];
+                    t1[zy * K + j] = acc;
+                }
+            for (int z = 0; z < K; z++)
+                for (int j = 0; j < K; j++)
+                    for (int x = 0; x < K; x++) {
+                        double acc = 0.0;
+                        for (int i = 0; i < K; i++) acc += W[i * N + ay * K + j] * t1[(z * K + i) * K + x];
+                        t2[(z * K + j) * K + x] = acc;
+                    }
+            for (int j = 0; j < K; j++)
+                for (int yx = 0; yx < K * K; yx++) {
+                    double acc = 0.0;
+                    for (int i = 0; i < K; i++) acc += W[i * N + az * K + j] * t2[i * K * K + yx];
+                    nxt[j * K * K + yx] = acc;
+                }
+            cur.swap(nxt);
+        }
+        e->s = cur;
+    }
+    e->norm = fro(e->s.data(), K3);
+    std::lock_guard<std::mutex> g(C.gmtx);
+    auto it = C.gen.find(key);
+    if (it != C.gen.end()) return it->second.get();
+    const GenEntry *p = e.get();
+    C.gen[key] = std::move(e);
+    return p;
+}
+
+struct Box {
+    int m[3];
+};
+
+// tuple screening: a (g, f, ft, gt, term) tuple is computed when
+// |w| * prod_d ||X_d|| * ||f_ft|| > SCREEN_REL * prec * ||g|| / sqrt(n_terms),
+// ||g|| being the running norm estimate of the output (no screening on the roots).
+const double SCREEN_REL = 0.1;
+// multiplies the tuple screening threshold (1 = the apply's rule; 0 = no screening)
+double g_screen_factor = 1.0;
+
+void apply(double prec, Tree &out, Oper &O, const Tree &inp, int maxIter, ApplyStats *st) {
+    auto t0 = std::chrono::steady_clock::now();
+    const MRA &mra = *inp.mra;
+    const Basis &B = mra.B;
+    int K = B.K, K2 = B.K2, K3 = B.K3;
+    ApplyCtx C;
+    C.O = &O;
+    C.in = &inp;
+    C.mra = &mra;
+    // per-depth maxima of input component norms
+    int maxd_in = 0;
+    for (int i = 0; i < inp.nNodes(); i++) maxd_in = std::max(maxd_in, inp.depth[i]);
+    C.fmax.assign(mra.maxDepth + 2, std::array<double, 8>{0, 0, 0, 0, 0, 0, 0, 0});
+    std::vector<double> leafmax(mra.maxDepth + 2, 0.0);
+    for (int i = 0; i < inp.nNodes(); i++) {
+        int d = inp.depth[i];
+        for (int cc = 0; cc < 8; cc++) C.fmax[d][cc] = std::max(C.fmax[d][cc], inp.cnorm[8 * i + cc]);
+        if (inp.child[i] < 0) {
+            double fn = 0.0;
+            for (int cc = 0; cc < 8; cc++) fn += inp.cnorm[8 * i + cc] * inp.cnorm[8 * i + cc];
+            leafmax[d] = std::max(leafmax[d], std::sqrt(fn));
+        }
+    }
+    {
+        double run = 0.0;
+        for (int d = 0; d < mra.maxDepth + 2; d++) {
+            C.fmax[d][0] = std::max(C.fmax[d][0], run);
+            run = std::max(run, leafmax[d]);
+        }
+    }
+    out.make_roots();
+    std::vector<int> work = out.roots;
+    double sNorm = 0.0, wNorm = 0.0, gsq = -1.0;
+    int iter = 0;
+    int nterms = (int)O.beta.size();
+    std::atomic<int64_t> ntup{0};
+    while (!work.empty()) {
+        int d = out.depth[work[0]];
+        ensure_levels(O, d);
+        const std::vector<OperLevel> &LV = O.lev[d];
+        double thr = gsq > 0.0 ? g_screen_factor * SCREEN_REL * prec * std::sqrt(gsq / nterms) : 0.0;
+        // effective boxes per (term, combo)
+        std::vector<Box> box(nterms * 64);
+        std::vector<Box> ubox(nterms);
+        for (int t = 0; t < nterms; t++) {
+            ubox[t] = Box{{-1, -1, -1}};
+            for (int combo = 0; combo < 64; combo++) box[t * 64 + combo] = Box{{-1, -1, -1}};
+            const OperLevel &lv = LV[t];
+            if (!lv.active) continue;
+            for (int gt = 0; gt < 8; gt++)
+                for (int ft = 0; ft < 8; ft++) {
+                    if (d > 0 && gt == 0 && ft == 0) continue;
+                    double F = C.fmax[d][ft];
+                    if (F <= 0.0) continue;
+                    int ty[3];
+                    for (int q = 0; q < 3; q++) ty[q] = 2 * ((gt >> q) & 1) + ((ft >> q) & 1);
+                    double base = std::fabs(O.weff[t]) * F;
+                    double full = base * lv.maxn[ty[0]] * lv.maxn[ty[1]] * lv.maxn[ty[2]];
+                    if (!(full > thr)) continue;
+                    Box b;
+                    for (int q = 0; q < 3; q++) {
+                        double other = base;
+                        for (int q2 = 0; q2 < 3; q2++)
+                            if (q2 != q) other *= lv.maxn[ty[q2]];
+                        double tau = thr / other;
+                        int m = -1;
+                        for (int mm = lv.band; mm >= 0; mm--)
+                            if (lv.env[ty[q] * (lv.band + 1) + mm] > tau) {
+                                m = mm;
+                                break;
+                            }
+                        b.m[q] = m;
+                    }
+                    if (b.m[0] < 0 || b.m[1] < 0 || b.m[2] < 0) continue;
+                    box[t * 64 + gt * 8 + ft] = b;
+                    for (int q = 0; q < 3; q++) ubox[t].m[q] = std::max(ubox[t].m[q], b.m[q]);
+                }
+        }
+        int nw = (int)work.size();
+        int64_t lmax[3];
+        for (int q = 0; q < 3; q++) lmax[q] = ((int64_t)mra.boxes[q] << d);
+#pragma omp parallel
+        {
+            std::vector<double> acc(8 * K3), t1(K3), t2(K3);
+            int64_t mytup = 0;
+#pragma omp for schedule(dynamic, 1)
+            for (int wi = 0; wi < nw; wi++) {
+                int g = work[wi];
+                const int64_t *Lg = &out.lr[3 * g];
+                std::fill(acc.begin(), acc.end(), 0.0);
+                for (int t = 0; t < nterms; t++) {
+                    const OperLevel &lv = LV[t];
+                    const Box &ub = ubox[t];
+                    if (ub.m[0] < 0) continue;
+                    double w = O.weff[t];
+                    int band = lv.band;
+                    for (int dz = -ub.m[2]; dz <= ub.m[2]; dz++)
+                        for (int dy = -ub.m[1]; dy <= ub.m[1]; dy++)
+                            for (int dx = -ub.m[0]; dx <= ub.m[0]; dx++) {
+                                int64_t Lf[3] = {Lg[0] - dx, Lg[1] - dy, Lg[2] - dz};
+                                if (Lf[0] < 0 || Lf[1] < 0 || Lf[2] < 0 || Lf[0] >= lmax[0] || Lf[1] >= lmax[1] ||
+                                    Lf[2] >= lmax[2])
+                                    continue;
+                                int fi = inp.find(d, Lf);
+                                const double *fc = nullptr;
+                                double fn[8];
+                                const GenEntry *ge = nullptr;
+                                bool gen = fi < 0;
+                                if (!gen) {
+                                    fc = inp.c(fi);
+                                    for (int cc = 0; cc < 8; cc++) fn[cc] = inp.cnorm[8 * fi + cc];
+                                } else {
+                                    for (int cc = 0; cc < 8; cc++) fn[cc] = 0.0;
+                                    fn[0] = -1.0;  // unknown until generated
+                                }
+                                int dl[3] = {dx, dy, dz};
+                                for (int ft = 0; ft < 8; ft++) {
+                                    if (gen && ft > 0) break;
+                                    for (int gt = 0; gt < 8; gt++) {
+                                        const Box &b = box[t * 64 + gt * 8 + ft];
+                                        if (b.m[0] < 0) continue;
+                                        if (std::abs(dx) > b.m[0] || std::abs(dy) > b.m[1] || std::abs(dz) > b.m[2])
+                                            continue;
+                                        double on = std::fabs(w);
+                                        int ty[3];
+                                        for (int q = 0; q < 3; q++) {
+                                            ty[q] = 2 * ((gt >> q) & 1) + ((ft >> q) & 1);
+                                            on *= lv.norms[(dl[q] + band) * 4 + ty[q]];
+                                        }
+                                        if (gen && fn[0] < 0.0) {
+                                            ge = get_gen(C, d, Lf);
+                                            fc = ge->s.data();
+                                            fn[0] = ge->norm;
+                                        }
+                                        double bound = on * fn[ft];
+                                        if (!(bound > thr)) continue;
+                                        const double *X = &lv.blocks[((dx + band) * 4 + ty[0]) * K2];
+                                        const double *Y = &lv.blocks[((dy + band) * 4 + ty[1]) * K2];
+                                        const double *Z = &lv.blocks[((dz + band) * 4 + ty[2]) * K2];
+                                        contract3(K, X, Y, Z, w, fc + ft * K3, &acc[gt * K3], t1.data(), t2.data());
+                                        mytup++;
+                                    }
+                                }
+                            }
+                }
+                std::memcpy(out.c(g), acc.data(), 8 * K3 * sizeof(double));
+                out.calc_norms(g);
+            }
+            ntup += mytup;
+        }
+        int64_t tup_before = 0;
+        for (int q = 0; q < iter; q++) tup_before += st ? st->tuples_iter[q] : 0;
+        if (st && iter < 64) {
+            st->tuples_iter[iter] = ntup.load() - tup_before;
+            st->nodes_iter[iter] = nw;
+        }
+        for (int g : work) {
+            if (iter == 0) sNorm += out.cnorm[8 * g] * out.cnorm[8 * g];
+            for (int cc = 1; cc < 8; cc++) wNorm += out.cnorm[8 * g + cc] * out.cnorm[8 * g + cc];
+        }
+        gsq = sNorm + wNorm;
+        double tnorm = std::sqrt(gsq);
+        std::vector<int> next;
+        if (!(maxIter >= 0 && iter >= maxIter)) {
+            for (int g : work) {
+                if (d >= mra.maxDepth) continue;
+                double wn = 0.0;
+                for (int cc = 1; cc < 8; cc++) wn += out.cnorm[8 * g + cc] * out.cnorm[8 * g + cc];
+                if (split_check(mra, d, std::sqrt(wn), prec, tnorm)) {
+                    out.split(g);
+                    for (int a = 0; a < 8; a++) next.push_back(out.child[g] + a);
+                }
+            }
+        }
+        work.swap(next);
+        iter++;
+    }
+    // additive top-down: push coarse scaling contributions to the children
+    std::vector<double> ch(8 * K3);
+    for (int i = 0; i < out.nNodes(); i++) {
+        int c0 = out.child[i];
+        if (c0 < 0) continue;
+        reconstruct(B, out.c(i), ch.data());
+        for (int a = 0; a < 8; a++) {
+            double *s = out.c(c0 + a);
+            for (int j = 0; j < K3; j++) s[j] += ch[a * K3 + j];
+        }
+    }
+    for (int i = 0; i < out.nNodes(); i++) out.calc_norms(i);
+    bottom_up(out);
+    if (st) {
+        st->tuples = ntup.load();
+        st->gen_nodes = (int64_t)C.gen.size();
+        st->nodes_out = out.nNodes();
+        st->iters = iter;
+        st->t_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    (void)K;
+}
+
+}  // namespace
+
+// ================================================================ C ABI (ctypes)
+
+extern "C" {
+
+struct mwref_stats {
+    int64_t tuples, gen_nodes, nodes_out, iters;
+    double t_total;
+    int64_t tuples_iter[64];
+    int64_t nodes_iter[64];
+};
+
+void *mwref_mra_new(int n0, const int *corner, const int *boxes, double sfac, int k, int basis_type, int max_depth) {
+    MRA *m = new MRA();
+    m->n0 = n0;
+    for (int d = 0; d < 3; d++) {
+        m->corner[d] = corner[d];
+        m->boxes[d] = boxes[d];
+    }
+    m->sfac = sfac;
+    m->maxDepth = max_depth;
+    build_basis(m->B, k, basis_type);
+    return m;
+}
+void mwref_mra_free(void *m) { delete (MRA *)m; }
+// two-scale matrix (2K x 2K) and quadrature values, for basis checks
+void mwref_mra_filters(void *m, double *W) {
+    const MRA *M = (const MRA *)m;
+    std::memcpy(W, M->B.W.data(), M->B.W.size() * sizeof(double));
+}
+void mwref_basis_eval(void *m, double x, double *out) { ((const MRA *)m)->B.phi(x, out); }
+
+void *mwref_tree_new(void *m) {
+    Tree *t = new Tree();
+    t->mra = (const MRA *)m;
+    return t;
+}
+void mwref_tree_free(void *t) { delete (Tree *)t; }
+int mwref_tree_nnodes(void *t) { return ((Tree *)t)->nNodes(); }
+double mwref_tree_sqnorm(void *t) { return ((Tree *)t)->sqnorm; }
+int mwref_tree_maxdepth(void *t) {
+    Tree *T = (Tree *)t;
+    int m = 0;
+    for (int d : T->depth) m = std::max(m, d);
+    return m;
+}
+
+static Func make_gauss(int n, const double *coef, const double *alpha, const double *center, const double *poly) {
+    Func F;
+    F.kind = 0;
+    F.g.resize(n);
+    for (int i = 0; i < n; i++) {
+        F.g[i].coef = coef[i];
+        F.g[i].alpha = alpha[i];
+        for (int d = 0; d < 3; d++) F.g[i].center[d] = center[3 * i + d];
+        for (int d = 0; d < 3; d++)
+            for (int e = 0; e < 5; e++) F.g[i].poly[d][e] = poly[15 * i + 5 * d + e];
+    }
+    return F;
+}
+
+// Gaussian sum: coef[n], alpha[n], center[3n], poly[15n] (per dim 5 coefficients in powers of x-center)
+void mwref_project_gauss(void *t, double prec, int n, const double *coef, const double *alpha, const double *center,
+                         const double *poly) {
+    Func F = make_gauss(n, coef, alpha, center, poly);
+    project(*(Tree *)t, F, prec);
+}
+// sum_i coef[i] * H_{q[6i..6i+2]} (* H_{q[6i+3..6i+5]} when q[6i+3] > 0)
+void mwref_project_hydrogen(void *t, double prec, int n, const double *coef, const int *q, double Z,
+                            const double *origin) {
+    Func F;
+    F.kind = 1;
+    F.h.resize(n);
+    for (int i = 0; i < n; i++) {
+        F.h[i].coef = coef[i];
+        for (int e = 0; e < 3; e++) {
+            F.h[i].q1[e] = q[6 * i + e];
+            F.h[i].q2[e] = q[6 * i + 3 + e];
+        }
+    }
+    F.hZ = Z;
+    for (int d = 0; d < 3; d++) F.horig[d] = origin[d];
+    project(*(Tree *)t, F, prec);
+}
+void mwref_eval(void *t, int n, const double *xyz, double *out) {
+    const Tree *T = (const Tree *)t;
+#pragma omp parallel for
+    for (int i = 0; i < n; i++) out[i] = eval_point(*T, &xyz[3 * i]);
+}
+double mwref_dot(void *a, void *b) { return tree_dot(*(Tree *)a, *(Tree *)b); }
+void mwref_tree_rescale(void *t, double c) {
+    Tree *T = (Tree *)t;
+    for (double &v : T->coef) v *= c;
+    for (double &v : T->cnorm) v *= std::fabs(c);
+    T->sqnorm *= c * c;
+}
+
+// export: depth[n], lr[3n], coefs[n*8*K3] in node order (parents before children)
+void mwref_tree_export(void *t, int *depth, int64_t *lr, double *coefs) {
+    Tree *T = (Tree *)t;
+    int n = T->nNodes();
+    std::memcpy(depth, T->depth.data(), n * sizeof(int));
+    std::memcpy(lr, T->lr.data(), 3 * n * sizeof(int64_t));
+    std::memcpy(coefs, T->coef.data(), T->coef.size() * sizeof(double));
+}
+// import a tree given in parent-before-children order with complete sibling groups
+int mwref_tree_import(void *t, int n, const int *depth, const int64_t *lr, const double *coefs) {
+    Tree *T = (Tree *)t;
+    T->clear();
+    int K3 = T->mra->B.K3;
+    for (int i = 0; i < n; i++) {
+        int par = -1;
+        if (depth[i] > 0) {
+            int64_t lp[3] = {lr[3 * i] >> 1, lr[3 * i + 1] >> 1, lr[3 * i + 2] >> 1};
+            par = T->find(depth[i] - 1, lp);
+            if (par < 0) return -1;
+        }
+        int id = T->add_node(depth[i], lr[3 * i], lr[3 * i + 1], lr[3 * i + 2], par);
+        if (par < 0) T->roots.push_back(id);
+        else {
+            int a = (int)(lr[3 * i] & 1) | (int)((lr[3 * i + 1] & 1) << 1) | (int)((lr[3 * i + 2] & 1) << 2);
+            if (a == 0) T->child[par] = id;
+        }
+        std::memcpy(T->c(id), &coefs[(size_t)i * 8 * K3], 8 * K3 * sizeof(double));
+        T->calc_norms(id);
+    }
+    T->calc_sqnorm();
+    return 0;
+}
+
+void *mwref_oper_new(void *m, double mu, double prec) {
+    Oper *O = new Oper();
+    build_oper(*O, (const MRA *)m, mu, prec);
+    return O;
+}
+void mwref_oper_free(void *o) { delete (Oper *)o; }
+int mwref_oper_nterms(void *o) { return (int)((Oper *)o)->fit.a.size(); }
+void mwref_oper_terms(void *o, double *a, double *w, double *info) {
+    Oper *O = (Oper *)o;
+    for (size_t i = 0; i < O->fit.a.size(); i++) {
+        a[i] = O->fit.a[i];
+        w[i] = O->fit.w[i];
+    }
+    info[0] = O->fit.rmin;
+    info[1] = O->fit.rmax;
+    info[2] = O->fit.eps;
+    info[3] = O->fit.h;
+    info[4] = O->fit.err;
+}
+// NS blocks for term at depth for deltas [dlo, dhi]: out[(delta-dlo)*4 + type][K2]
+void mwref_oper_blocks(void *o, int term, int depth, int64_t dlo, int64_t dhi, double *out) {
+    Oper *O = (Oper *)o;
+    std::vector<double> b;
+    ns_blocks(*O, term, depth, dlo, dhi, b);
+    std::memcpy(out, b.data(), b.size() * sizeof(double));
+}
+// direct scaling block T^m_delta of unit-coordinate term at scale n0+depth
+void mwref_oper_tblock(void *o, int term, int depth, int64_t delta, double *out) {
+    Oper *O = (Oper *)o;
+    int m = O->mra->n0 + depth;
+    gauss_block(O->mra->B, O->beta[term] * std::ldexp(1.0, -2 * m), std::ldexp(1.0, -m), delta, out);
+}
+// level info: active flag and band per term at depth
+void mwref_oper_level(void *o, int depth, int *active, int *band, double *maxn) {
+    Oper *O = (Oper *)o;
+    ensure_levels(*O, depth);
+    for (size_t t = 0; t < O->beta.size(); t++) {
+        active[t] = O->lev[depth][t].active;
+        band[t] = O->lev[depth][t].band;
+        for (int q = 0; q < 4; q++) maxn[4 * t + q] = O->lev[depth][t].maxn[q];
+    }
+}
+
+int mwref_apply(double prec, void *out, void *oper, void *inp, int max_iter, mwref_stats *st) {
+    ApplyStats s;
+    apply(prec, *(Tree *)out, *(Oper *)oper, *(Tree *)inp, max_iter, &s);
+    if (st) {
+        st->tuples = s.tuples;
+        st->gen_nodes = s.gen_nodes;
+        st->nodes_out = s.nodes_out;
+        st->iters = s.iters;
+        st->t_total = s.t_total;
+        std::memcpy(st->tuples_iter, s.tuples_iter, sizeof(s.tuples_iter));
+        std::memcpy(st->nodes_iter, s.nodes_iter, sizeof(s.nodes_iter));
+    }
+    return 0;
+}
+
+void mwref_set_threads(int n) { omp_set_num_threads(n); }
+void mwref_set_screen_factor(double f) { g_screen_factor = f; }
+int mwref_get_threads() { return omp_get_max_threads(); }
+
+}  // extern "C"
