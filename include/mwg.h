@@ -1,0 +1,128 @@
+/* mwg.h — C ABI of the MI355X multiwavelet Green's-function apply (libmwgreen.so).
+ *
+ * Drop-in boundary for the hot path of MRChem: mrcpp::apply(prec, out, oper, inp)
+ * with mrcpp::PoissonOperator / mrcpp::HelmholtzOperator, as called from
+ *   src/scf_solver/HelmholtzVector.cpp:171-181           (Helmholtz, per orbital)
+ *   src/qmoperators/two_electron/CoulombPotential.cpp:85  (Poisson on the density)
+ *   src/qmoperators/two_electron/ExchangePotential.cpp:259,318 (pair Poissons)
+ * The reference interface is a set of C++ templates in namespace mrcpp (MRCPP is an
+ * external, non-vendored library, .gitmodules:4-6); this header is the C ABI the
+ * C++ facade in include/mwg_mrcpp.hpp (and any FFI) binds to.
+ *
+ * Conventions
+ *  - Every call returns 0 on success and a non-zero status on failure; the message
+ *    is available from mwg_last_error(ctx).  No exception crosses the ABI.
+ *  - The caller owns every handle and destroys it explicitly.  Operators are
+ *    borrowed by mwg_apply (ExchangePotential.cpp:19 "does not take ownership").
+ *  - One context per host thread.  Calls are ordered on the context's HIP stream;
+ *    functions that return host data synchronise that stream.
+ *  - Trees live in device memory (HBM).  Import/export use host buffers in the
+ *    documented packed layout: nodes in parent-before-children order, complete
+ *    sibling groups of 8, translations relative to the root corner at the node's
+ *    depth, and 8*(k+1)^3 FP64 coefficients per node, component-major
+ *    (component bit 0 = x, bit 1 = y, bit 2 = z; 0 = scaling), within a component
+ *    index (iz*(k+1) + iy)*(k+1) + ix.
+ */
+#ifndef MWG_H
+#define MWG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mwg_ctx_s *mwg_ctx;
+typedef struct mwg_mra_s *mwg_mra;
+typedef struct mwg_tree_s *mwg_tree;
+typedef struct mwg_oper_s *mwg_oper;
+
+/* status codes */
+#define MWG_OK 0
+#define MWG_ERR_ARG 1
+#define MWG_ERR_HIP 2
+#define MWG_ERR_STATE 3
+#define MWG_ERR_RANGE 4
+
+/* apply flags */
+#define MWG_APPLY_FP64 0u     /* plain method: FP64 contractions, norm-test refinement */
+#define MWG_APPLY_MIXED 1u    /* FP32 MFMA contractions with FP64 accumulation */
+#define MWG_APPLY_PREDICT 2u  /* learned refinement predictor (norm test stays the judge) */
+
+/* basis types (src/mrenv.cpp:62-69: "I" Interpolating, "L" Legendre) */
+#define MWG_BASIS_LEGENDRE 0
+#define MWG_BASIS_INTERPOLATING 1
+
+typedef struct mwg_apply_stats {
+    int64_t nodes_out;       /* nodes of the output tree (output nodes of this apply) */
+    int64_t tuples;          /* surviving (g, f, ft, gt, term) tuples contracted */
+    int64_t tuples_fp64;     /* of which contracted in FP64 (mixed mode routes large bounds to FP64) */
+    int64_t gen_nodes;       /* generated (scaling-only) input nodes */
+    int64_t iters;           /* refinement iterations */
+    int64_t predicted_splits;/* nodes split ahead of the norm test by the predictor */
+    double t_apply;          /* seconds: work-list build + contraction + refine + transforms */
+    double t_contract;       /* seconds spent in contraction kernels (HIP events) */
+    double flops;            /* algorithmic flops: tuples * 6 (k+1)^4 */
+} mwg_apply_stats;
+
+/* ---- context (mrenv / device) ------------------------------------------------ */
+int mwg_ctx_create(int device, mwg_ctx *out);
+int mwg_ctx_destroy(mwg_ctx ctx);
+const char *mwg_last_error(mwg_ctx ctx);
+int mwg_sync(mwg_ctx ctx);
+
+/* ---- MRA: BoundingBox + scaling basis + max depth
+ * replaces mrcpp::MultiResolutionAnalysis<3>(BoundingBox<3>(min_scale, corner, boxes), basis, max_depth)
+ * (src/mrenv.cpp:46-72, tests/unit_tests.cpp:28-44); scaling_factor stretches the box. */
+int mwg_mra_create(mwg_ctx ctx, int root_scale, const int *corner, const int *boxes, double scaling_factor,
+                   int order, int basis_type, int max_depth, mwg_mra *out);
+int mwg_mra_destroy(mwg_mra mra);
+
+/* ---- function trees: replaces mrcpp::FunctionTree<3>(MRA) (QMFunction.cpp:68-77) */
+int mwg_tree_create(mwg_mra mra, mwg_tree *out);
+int mwg_tree_destroy(mwg_tree tree);
+int mwg_tree_nnodes(mwg_tree tree, int64_t *n);              /* FunctionTree::getNNodes */
+int mwg_tree_sqnorm(mwg_tree tree, double *out);             /* FunctionTree::getSquareNorm */
+int mwg_tree_maxdepth(mwg_tree tree, int *out);
+int mwg_tree_import(mwg_tree tree, int64_t n, const int *depth, const int64_t *lr, const double *coefs);
+int mwg_tree_export(mwg_tree tree, int *depth, int64_t *lr, double *coefs);
+int mwg_tree_rescale(mwg_tree tree, double c);               /* FunctionTree::rescale (HelmholtzVector.cpp:81) */
+int mwg_tree_dot(mwg_tree a, mwg_tree b, double *out);       /* mrcpp::dot */
+int mwg_tree_eval(mwg_tree tree, int64_t n, const double *xyz, double *out);
+
+/* ---- projection of analytic functions: replaces mrcpp::project (qmfunction_utils.cpp:88-120)
+ * Gaussian sum: term i = coef[i] * prod_d poly[15i+5d+e] (x_d-c_d)^e * exp(-alpha[i] |x-c|^2). */
+int mwg_project_gauss(double prec, mwg_tree out, int n, const double *coef, const double *alpha,
+                      const double *center, const double *poly);
+/* hydrogen orbitals (src/analyticfunctions/HydrogenFunction.cpp:34-165) and products of two:
+ * term i = coef[i] * H_{q[6i..6i+2]} (* H_{q[6i+3..6i+5]} when q[6i+3] > 0), (n,l,m). */
+int mwg_project_hydrogen(double prec, mwg_tree out, int n, const double *coef, const int *q, double Z,
+                         const double *origin);
+
+/* ---- operators: replace new mrcpp::PoissonOperator(MRA, prec) (SCFDriver.cpp:279) and
+ * mrcpp::HelmholtzOperator(MRA, mu, prec) (HelmholtzVector.cpp:171).  Kernels 1/r and
+ * exp(-mu r)/r (the -1/(2 pi) rescale stays with the caller, HelmholtzVector.cpp:81,135). */
+int mwg_oper_poisson(mwg_mra mra, double prec, mwg_oper *out);
+int mwg_oper_helmholtz(mwg_mra mra, double mu, double prec, mwg_oper *out);
+int mwg_oper_destroy(mwg_oper oper);
+int mwg_oper_nterms(mwg_oper oper, int *n);
+int mwg_oper_terms(mwg_oper oper, double *exponents, double *weights);
+/* NS blocks of one term at one depth for translations [dlo, dhi]:
+ * out[((delta-dlo)*4 + type)*(k+1)^2 + a*(k+1) + i], type = 2*gt_bit + ft_bit (T, C, B, A) */
+int mwg_oper_blocks(mwg_oper oper, int term, int depth, int64_t dlo, int64_t dhi, double *out);
+
+/* ---- apply: replaces mrcpp::apply(prec, out, oper, inp[, maxIter])
+ * (HelmholtzVector.cpp:176,180; CoulombPotential.cpp:85; ExchangePotential.cpp:137,259,318).
+ * `out` must be empty (no nodes) — it is built adaptively; `inp` is not modified. */
+int mwg_apply(double prec, mwg_tree out, mwg_oper oper, mwg_tree inp, int max_iter, unsigned flags,
+              mwg_apply_stats *stats);
+/* n independent applies (exchange pair Poissons, per-orbital Helmholtz) in one batch:
+ * one work list and one contraction launch per refinement level for all of them. */
+int mwg_apply_batch(int n, const double *prec, mwg_tree *out, mwg_oper *oper, mwg_tree *inp, int max_iter,
+                    unsigned flags, mwg_apply_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MWG_H */

@@ -1,0 +1,102 @@
+"""ctypes binding of libmwgreen.so (the C ABI declared in include/mwg.h).
+
+The library is built in-tree by ``mrchem_amd.build``.  There is no fallback:
+if the shared object is missing or cannot be loaded, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmwgreen.so")
+
+MWG_APPLY_FP64 = 0
+MWG_APPLY_MIXED = 1
+MWG_APPLY_PREDICT = 2
+
+
+class ApplyStats(C.Structure):
+    _fields_ = [("nodes_out", C.c_int64), ("tuples", C.c_int64), ("tuples_fp64", C.c_int64),
+                ("gen_nodes", C.c_int64), ("iters", C.c_int64), ("predicted_splits", C.c_int64),
+                ("t_apply", C.c_double), ("t_contract", C.c_double), ("flops", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class MwgError(RuntimeError):
+    pass
+
+
+_LIB = None
+
+# name -> (restype, argtypes); every function returns an int status except mwg_last_error
+_SIG = None
+
+
+def _sig():
+    vp, d, i, i64, u = C.c_void_p, C.c_double, C.c_int, C.c_int64, C.c_uint
+    fd = np.ctypeslib.ndpointer(dtype=np.float64, flags="C")
+    fi = np.ctypeslib.ndpointer(dtype=np.int32, flags="C")
+    fl = np.ctypeslib.ndpointer(dtype=np.int64, flags="C")
+    pvp = C.POINTER(C.c_void_p)
+    return {
+        "mwg_ctx_create": [i, pvp],
+        "mwg_ctx_destroy": [vp],
+        "mwg_sync": [vp],
+        "mwg_mra_create": [vp, i, fi, fi, d, i, i, i, pvp],
+        "mwg_mra_destroy": [vp],
+        "mwg_tree_create": [vp, pvp],
+        "mwg_tree_destroy": [vp],
+        "mwg_tree_nnodes": [vp, C.POINTER(C.c_int64)],
+        "mwg_tree_sqnorm": [vp, C.POINTER(C.c_double)],
+        "mwg_tree_maxdepth": [vp, C.POINTER(C.c_int)],
+        "mwg_tree_import": [vp, i64, fi, fl, fd],
+        "mwg_tree_export": [vp, fi, fl, fd],
+        "mwg_tree_rescale": [vp, d],
+        "mwg_tree_dot": [vp, vp, C.POINTER(C.c_double)],
+        "mwg_tree_eval": [vp, i64, fd, fd],
+        "mwg_project_gauss": [d, vp, i, fd, fd, fd, fd],
+        "mwg_project_hydrogen": [d, vp, i, fd, fi, d, fd],
+        "mwg_oper_poisson": [vp, d, pvp],
+        "mwg_oper_helmholtz": [vp, d, d, pvp],
+        "mwg_oper_destroy": [vp],
+        "mwg_oper_nterms": [vp, C.POINTER(C.c_int)],
+        "mwg_oper_terms": [vp, fd, fd],
+        "mwg_oper_blocks": [vp, i, i, i64, i64, fd],
+        "mwg_apply": [d, vp, vp, vp, i, u, C.POINTER(ApplyStats)],
+        "mwg_apply_batch": [i, fd, pvp, pvp, pvp, i, u, C.POINTER(ApplyStats)],
+    }
+
+
+EXPORTED = ["mwg_ctx_create", "mwg_ctx_destroy", "mwg_last_error", "mwg_sync", "mwg_mra_create", "mwg_mra_destroy",
+            "mwg_tree_create", "mwg_tree_destroy", "mwg_tree_nnodes", "mwg_tree_sqnorm", "mwg_tree_maxdepth",
+            "mwg_tree_import", "mwg_tree_export", "mwg_tree_rescale", "mwg_tree_dot", "mwg_tree_eval",
+            "mwg_project_gauss", "mwg_project_hydrogen", "mwg_oper_poisson", "mwg_oper_helmholtz",
+            "mwg_oper_destroy", "mwg_oper_nterms", "mwg_oper_terms", "mwg_oper_blocks", "mwg_apply",
+            "mwg_apply_batch"]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise MwgError(f"{LIB_PATH} is not built; run `python -m mrchem_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, args in _sig().items():
+            fn = getattr(L, name)
+            fn.restype = C.c_int
+            fn.argtypes = args
+        L.mwg_last_error.restype = C.c_char_p
+        L.mwg_last_error.argtypes = [C.c_void_p]
+        _LIB = L
+    return _LIB
+
+
+def check(rc: int, ctx=None):
+    if rc != 0:
+        msg = lib().mwg_last_error(ctx)
+        raise MwgError(f"mwg status {rc}: {msg.decode() if msg else ''}")

@@ -1,0 +1,1219 @@
+// Host orchestration and C ABI of libmwgreen.so.
+//
+// mrcpp::apply(prec, out, oper, inp) is restated as an adaptive build whose
+// every numerical step runs on the GPU; the host keeps only the tree topology
+// (node keys, parent/child links) and 8 norms per node, and decides refinement:
+//   per level: screened boxes (host, from operator norm envelopes)
+//            -> generated input nodes (device mark + compute)
+//            -> work list of (g, f, ft, gt, term) tuples (device count + fill)
+//            -> contraction (device) -> norms (device) -> split test (host)
+//   after the last level: additive top-down push of the coarse scaling parts,
+//   bottom-up compression and norms (device).
+// Operators (Gaussian-sum fit on the host, NS blocks on the device) are built
+// level by level as the output trees reach new depths.
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/mwg.h"
+#include "mwg_core.h"
+#include "mwg_kernels.h"
+
+namespace mwg {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HC(x)                                                                                         \
+    do {                                                                                              \
+        hipError_t e_ = (x);                                                                          \
+        if (e_ != hipSuccess) throw Error(MWG_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() {
+        if (p) (void)hipFree(p);
+    }
+    void alloc(size_t m) {  // discard contents
+        if (m <= n && p) return;
+        if (p) HC(hipFree(p));
+        p = nullptr;
+        n = 0;
+        HC(hipMalloc(&p, std::max<size_t>(m, 1) * sizeof(T)));
+        n = std::max<size_t>(m, 1);
+    }
+    void grow_keep(size_t m, hipStream_t s) {
+        if (m <= n && p) return;
+        size_t nn = std::max<size_t>(m, 2 * n);
+        T *q = nullptr;
+        HC(hipMalloc(&q, std::max<size_t>(nn, 1) * sizeof(T)));
+        if (p) {
+            HC(hipMemcpyAsync(q, p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+            HC(hipStreamSynchronize(s));
+            HC(hipFree(p));
+        }
+        p = q;
+        n = std::max<size_t>(nn, 1);
+    }
+    void upload(const T *h, size_t m, hipStream_t s) {
+        alloc(m);
+        if (m) HC(hipMemcpyAsync(p, h, m * sizeof(T), hipMemcpyHostToDevice, s));
+        HC(hipStreamSynchronize(s));
+    }
+    void download(T *h, size_t m, hipStream_t s) const {
+        if (m) HC(hipMemcpyAsync(h, p, m * sizeof(T), hipMemcpyDeviceToHost, s));
+        HC(hipStreamSynchronize(s));
+    }
+};
+
+// ------------------------------------------------------------------ handles
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    DBuf<double> gx, gw;  // 24-point Gauss-Legendre (operator quadrature)
+    // apply scratch
+    DBuf<WorkItem> work;
+    DBuf<JobTab> jobs;
+    DBuf<TermTab> tabs;
+    DBuf<int> thread_off, gtotal;
+    DBuf<long long> gofs;
+    DBuf<Tuple> tuples;
+    DBuf<double> wnorms;
+    DBuf<uint64_t> genKeys;
+    DBuf<int> genJob, genCount, genOverflow;
+    DBuf<double> genPool, genNorm;
+    DBuf<unsigned long long> genHashKeys;
+    DBuf<int> genHashVals;
+    DBuf<TransformItem> titems;
+};
+
+struct MraH {
+    Ctx *ctx = nullptr;
+    int n0 = 0, corner[3] = {0, 0, 0}, boxes[3] = {1, 1, 1}, maxDepth = 16;
+    double sfac = 1.0;
+    Basis B;
+    DBuf<double> U, qx, qw, phiq, W, WT;
+    double lo(int d) const { return sfac * std::ldexp((double)corner[d], -n0); }
+    double width(int d) const { return sfac * std::ldexp((double)boxes[d], -n0); }
+    DevBasis dev() const { return DevBasis{B.K, U.p, qx.p, qw.p, phiq.p, W.p}; }
+    ProjGeom geom() const { return ProjGeom{{lo(0), lo(1), lo(2)}, sfac, n0}; }
+};
+
+struct TreeH {
+    MraH *mra = nullptr;
+    std::vector<int> depth, parent, child, lr;  // lr: 3 per node
+    std::unordered_map<uint64_t, int> index;
+    std::vector<int> roots;
+    std::vector<double> cnorm;  // 8 per node (host copy)
+    double sqnorm = 0.0;
+    DBuf<double> coef, norm;
+    DBuf<unsigned long long> hkeys;
+    DBuf<int> hvals;
+    uint32_t hmask = 0;
+    bool hvalid = false;
+
+    int K3() const { return mra->B.K3; }
+    int n() const { return (int)depth.size(); }
+    void clear() {
+        depth.clear();
+        parent.clear();
+        child.clear();
+        lr.clear();
+        index.clear();
+        roots.clear();
+        cnorm.clear();
+        sqnorm = 0.0;
+        hvalid = false;
+    }
+    int add(int d, int x, int y, int z, int par) {
+        const int id = n();
+        depth.push_back(d);
+        parent.push_back(par);
+        child.push_back(-1);
+        lr.push_back(x);
+        lr.push_back(y);
+        lr.push_back(z);
+        cnorm.resize(8 * (size_t)(id + 1), 0.0);
+        index[node_key(d, x, y, z)] = id;
+        hvalid = false;
+        return id;
+    }
+    void make_roots() {
+        clear();
+        for (int z = 0; z < mra->boxes[2]; z++)
+            for (int y = 0; y < mra->boxes[1]; y++)
+                for (int x = 0; x < mra->boxes[0]; x++) roots.push_back(add(0, x, y, z, -1));
+    }
+    void split(int i) {
+        const int d = depth[i], x = lr[3 * i], y = lr[3 * i + 1], z = lr[3 * i + 2];
+        const int first = n();
+        for (int a = 0; a < 8; a++) add(d + 1, 2 * x + (a & 1), 2 * y + ((a >> 1) & 1), 2 * z + ((a >> 2) & 1), i);
+        child[i] = first;
+    }
+    int find(int d, int x, int y, int z) const {
+        auto it = index.find(node_key(d, x, y, z));
+        return it == index.end() ? -1 : it->second;
+    }
+    void ensure_dev(size_t nodes) {
+        hipStream_t s = mra->ctx->stream;
+        coef.grow_keep(nodes * 8 * (size_t)K3(), s);
+        norm.grow_keep(nodes * 8, s);
+    }
+    void build_hash() {
+        if (hvalid) return;
+        uint32_t cap = 1024;
+        while (cap < 2u * (uint32_t)n() + 16) cap <<= 1;
+        std::vector<unsigned long long> keys(cap, 0ull);
+        std::vector<int> vals(cap, -1);
+        for (int i = 0; i < n(); i++) {
+            const uint64_t k = node_key(depth[i], lr[3 * i], lr[3 * i + 1], lr[3 * i + 2]);
+            uint32_t h = dkey_hash(k) & (cap - 1);
+            while (keys[h] != 0ull) h = (h + 1) & (cap - 1);
+            keys[h] = k + 1ull;
+            vals[h] = i;
+        }
+        hipStream_t s = mra->ctx->stream;
+        hkeys.upload(keys.data(), cap, s);
+        hvals.upload(vals.data(), cap, s);
+        hmask = cap - 1;
+        hvalid = true;
+    }
+    DHash dhash() const { return DHash{hkeys.p, hvals.p, hmask}; }
+    void refresh_norms() {  // device norms -> host, square norm
+        hipStream_t s = mra->ctx->stream;
+        HC(launch_norms(s, n(), nullptr, coef.p, norm.p, K3()));
+        cnorm.resize(8 * (size_t)n());
+        norm.download(cnorm.data(), 8 * (size_t)n(), s);
+        double sq = 0.0;
+        for (int r : roots) sq += cnorm[8 * r] * cnorm[8 * r];
+        for (int i = 0; i < n(); i++)
+            for (int c = 1; c < 8; c++) sq += cnorm[8 * i + c] * cnorm[8 * i + c];
+        sqnorm = sq;
+    }
+    // per-level lists of internal nodes (parents of complete sibling groups)
+    std::vector<std::vector<int>> internal_by_depth() const {
+        std::vector<std::vector<int>> L;
+        for (int i = 0; i < n(); i++)
+            if (child[i] >= 0) {
+                if ((int)L.size() <= depth[i]) L.resize(depth[i] + 1);
+                L[depth[i]].push_back(i);
+            }
+        return L;
+    }
+};
+
+struct TermLevel {
+    bool active = false;
+    int band = -1;
+    long long blk = 0, nrm = 0;  // offsets (doubles) into the level arrays
+    std::vector<double> norms;   // (2 band + 1) * 4
+    std::vector<double> env;     // 4 * (band + 1)
+    double maxn[4] = {0, 0, 0, 0};
+};
+
+struct LevelH {
+    std::vector<TermLevel> t;
+    DBuf<double> blocks, norms;
+};
+
+struct OperH {
+    MraH *mra = nullptr;
+    double mu = 0.0, prec = 0.0;
+    KernelFit fit;
+    std::vector<double> beta, weff;
+    std::vector<std::unique_ptr<LevelH>> lev;
+    DBuf<double> t1buf, tmpblk, tmpnrm, bvals;
+    DBuf<int> iterm, irow0, itlo, ithi;
+    DBuf<long long> idelta;
+
+    int nterms() const { return (int)beta.size(); }
+    void build_level(int d);
+    void ensure(int d) {
+        if ((int)lev.size() <= d) lev.resize(d + 1);
+        for (int q = 0; q <= d; q++)
+            if (!lev[q]) build_level(q);
+    }
+};
+
+// NS blocks for a set of (term, delta range) requests into dense device arrays
+struct NSRequest {
+    int term;
+    long long dlo, dhi;
+};
+
+static void compute_ns(OperH &O, int d, const std::vector<NSRequest> &req, DBuf<double> &blocks, DBuf<double> &norms,
+                       std::vector<long long> &first_row) {
+    MraH &M = *O.mra;
+    Ctx &C = *M.ctx;
+    const Basis &B = M.B;
+    const int K2 = B.K2;
+    const int m1 = M.n0 + d + 1;
+    const double pre1 = std::ldexp(1.0, -m1);
+    long long boxmax = std::max({M.boxes[0], M.boxes[1], M.boxes[2]});
+    const long long lmax1 = (boxmax << (d + 1)) - 1;
+    // child-scale scaling blocks
+    std::vector<int> tterm, row0(req.size()), tlo(req.size()), thi(req.size());
+    std::vector<long long> tdelta;
+    std::vector<double> b1(O.nterms());
+    for (int t = 0; t < O.nterms(); t++) b1[t] = O.beta[t] * std::ldexp(1.0, -2 * m1);
+    for (size_t r = 0; r < req.size(); r++) {
+        const int t = req[r].term;
+        const long long band1 = std::min<long long>(lmax1, (long long)std::ceil(1.0 + std::sqrt(GAUSS_CUT / b1[t])));
+        const long long lo = std::max<long long>(-band1, 2 * req[r].dlo - 1);
+        const long long hi = std::min<long long>(band1, 2 * req[r].dhi + 1);
+        row0[r] = (int)tdelta.size();
+        tlo[r] = (int)lo;
+        thi[r] = (int)hi;
+        for (long long q = lo; q <= hi; q++) {
+            tterm.push_back(t);
+            tdelta.push_back(q);
+        }
+    }
+    hipStream_t s = C.stream;
+    O.bvals.upload(b1.data(), b1.size(), s);
+    const int nt1 = (int)tdelta.size();
+    O.iterm.upload(tterm.data(), tterm.size(), s);
+    O.idelta.upload(tdelta.data(), tdelta.size(), s);
+    O.t1buf.alloc((size_t)std::max(nt1, 1) * K2);
+    HC(launch_tblock(s, nt1, O.iterm.p, O.idelta.p, O.bvals.p, pre1, M.dev(), C.gx.p, C.gw.p, O.t1buf.p));
+    // NS transform per delta
+    std::vector<long long> ndelta;
+    std::vector<int> nrow0, ntlo, nthi;
+    first_row.assign(req.size(), 0);
+    for (size_t r = 0; r < req.size(); r++) {
+        first_row[r] = (long long)ndelta.size();
+        for (long long q = req[r].dlo; q <= req[r].dhi; q++) {
+            ndelta.push_back(q);
+            nrow0.push_back(row0[r]);
+            ntlo.push_back(tlo[r]);
+            nthi.push_back(thi[r]);
+        }
+    }
+    const int nn = (int)ndelta.size();
+    DBuf<long long> dd;
+    dd.upload(ndelta.data(), ndelta.size(), s);
+    O.irow0.upload(nrow0.data(), nrow0.size(), s);
+    O.itlo.upload(ntlo.data(), ntlo.size(), s);
+    O.ithi.upload(nthi.data(), nthi.size(), s);
+    blocks.alloc((size_t)std::max(nn, 1) * 4 * K2);
+    norms.alloc((size_t)std::max(nn, 1) * 4);
+    HC(launch_nsblocks(s, nn, dd.p, O.irow0.p, O.itlo.p, O.ithi.p, O.t1buf.p, M.W.p, B.K, blocks.p, norms.p));
+    HC(hipStreamSynchronize(s));
+}
+
+void OperH::build_level(int d) {
+    MraH &M = *mra;
+    Ctx &C = *M.ctx;
+    const int nt = nterms();
+    auto L = std::make_unique<LevelH>();
+    L->t.resize(nt);
+    const long long boxmax = std::max({M.boxes[0], M.boxes[1], M.boxes[2]});
+    const int m1 = M.n0 + d + 1;
+    std::vector<long long> band(nt);
+    for (int t = 0; t < nt; t++) {
+        const double b1 = beta[t] * std::ldexp(1.0, -2 * m1);
+        const long long lmax1 = (boxmax << (d + 1)) - 1;
+        const long long band1 = std::min<long long>(lmax1, (long long)std::ceil(1.0 + std::sqrt(GAUSS_CUT / b1)));
+        band[t] = std::min<long long>((boxmax << d) - 1, (band1 + 2) / 2);
+    }
+    // phase 1: near-diagonal blocks decide which terms are active at this depth
+    std::vector<NSRequest> req;
+    for (int t = 0; t < nt; t++) {
+        const long long nb = std::min<long long>(band[t], 1);
+        req.push_back({t, -nb, nb});
+    }
+    std::vector<long long> rows;
+    compute_ns(*this, d, req, tmpblk, tmpnrm, rows);
+    std::vector<double> nn(tmpnrm.n);
+    tmpnrm.download(nn.data(), tmpnrm.n, C.stream);
+    std::vector<NSRequest> full;
+    for (int t = 0; t < nt; t++) {
+        double tn = 0.0, wn = 0.0;
+        for (long long q = 0; q <= req[t].dhi - req[t].dlo; q++) {
+            const double *p = &nn[(rows[t] + q) * 4];
+            tn = std::max(tn, p[0]);
+            wn = std::max({wn, p[1], p[2], p[3]});
+        }
+        const bool active = (d == 0) || (wn > ACT_REL * prec * tn);
+        L->t[t].active = active;
+        if (active) {
+            L->t[t].band = (int)band[t];
+            full.push_back({t, -band[t], band[t]});
+        }
+    }
+    // phase 2: full bands of the active terms
+    if (!full.empty()) {
+        compute_ns(*this, d, full, L->blocks, L->norms, rows);
+        std::vector<double> hn(L->norms.n);
+        L->norms.download(hn.data(), hn.size(), C.stream);
+        for (size_t r = 0; r < full.size(); r++) {
+            TermLevel &tl = L->t[full[r].term];
+            const int b = tl.band;
+            tl.blk = rows[r] * 4 * (long long)M.B.K2;
+            tl.nrm = rows[r] * 4;
+            tl.norms.assign(hn.begin() + rows[r] * 4, hn.begin() + (rows[r] + 2 * b + 1) * 4);
+            for (int q = 0; q < 2 * b + 1; q++)
+                for (int type = 0; type < 4; type++) tl.maxn[type] = std::max(tl.maxn[type], tl.norms[q * 4 + type]);
+            tl.env.assign(4 * (b + 1), 0.0);
+            for (int type = 0; type < 4; type++) {
+                double run = 0.0;
+                for (int m = b; m >= 0; m--) {
+                    run = std::max(run, std::max(tl.norms[(b + m) * 4 + type], tl.norms[(b - m) * 4 + type]));
+                    tl.env[type * (b + 1) + m] = run;
+                }
+            }
+        }
+    }
+    lev[d] = std::move(L);
+}
+
+static void build_oper(OperH &O, MraH *mra, double mu, double prec) {
+    O.mra = mra;
+    O.mu = mu;
+    O.prec = prec;
+    const double eps = prec / 10.0;
+    const double rmin = 0.1 * std::sqrt(eps);
+    double w = 0.0;
+    for (int d = 0; d < 3; d++) w = std::max(w, mra->width(d));
+    O.fit = fit_kernel(mu, eps, rmin, std::sqrt(3.0) * w);
+    const int nt = (int)O.fit.a.size();
+    if (nt > 255) throw Error(MWG_ERR_RANGE, "kernel fit needs more than 255 terms");
+    const double s = mra->sfac;
+    O.beta.resize(nt);
+    O.weff.resize(nt);
+    for (int t = 0; t < nt; t++) {
+        O.beta[t] = O.fit.a[t] * s * s;
+        O.weff[t] = O.fit.w[t] * s * s * s;
+    }
+}
+
+// ------------------------------------------------------------------ projection
+
+template <class LaunchFn>
+static void project_adaptive(TreeH &T, double prec, LaunchFn launch, const std::vector<double> *gauss_alpha,
+                             const std::vector<double> *gauss_center) {
+    MraH &M = *T.mra;
+    Ctx &C = *M.ctx;
+    hipStream_t s = C.stream;
+    T.make_roots();
+    std::vector<int> work = T.roots;
+    double sNorm = 0.0, wNorm = 0.0;
+    int iter = 0;
+    DBuf<int> dwork, ddepth, dlr;
+    while (!work.empty()) {
+        T.ensure_dev(T.n());
+        dwork.upload(work.data(), work.size(), s);
+        ddepth.upload(T.depth.data(), T.depth.size(), s);
+        dlr.upload(T.lr.data(), T.lr.size(), s);
+        launch((int)work.size(), dwork.p, ddepth.p, dlr.p, T.coef.p);
+        HC(launch_norms(s, (int)work.size(), dwork.p, T.coef.p, T.norm.p, T.K3()));
+        std::vector<double> nrm(8 * (size_t)T.n());
+        T.norm.download(nrm.data(), nrm.size(), s);
+        for (int i : work) {
+            std::memcpy(&T.cnorm[8 * (size_t)i], &nrm[8 * (size_t)i], 8 * sizeof(double));
+            if (iter == 0) sNorm += nrm[8 * i] * nrm[8 * i];
+            for (int c = 1; c < 8; c++) wNorm += nrm[8 * i + c] * nrm[8 * i + c];
+        }
+        const double tnorm = std::sqrt(sNorm + wNorm);
+        std::vector<int> next;
+        for (int i : work) {
+            const int d = T.depth[i];
+            if (d >= M.maxDepth) continue;
+            double wn = 0.0;
+            for (int c = 1; c < 8; c++) wn += T.cnorm[8 * i + c] * T.cnorm[8 * i + c];
+            bool must = false;
+            if (gauss_alpha) {
+                // a Gaussian narrower than the quadrature spacing is invisible to the norm test
+                const double h = M.sfac * std::ldexp(1.0, -(M.n0 + d));
+                const double hc = 0.5 * h;
+                for (size_t t = 0; t < gauss_alpha->size() && !must; t++) {
+                    const double two_sigma = 2.0 / std::sqrt(2.0 * (*gauss_alpha)[t]);
+                    if (hc <= M.B.K * two_sigma) continue;
+                    double d2 = 0.0;
+                    for (int q = 0; q < 3; q++) {
+                        const double lo = M.lo(q) + h * (double)T.lr[3 * i + q], hi = lo + h;
+                        const double c = (*gauss_center)[3 * t + q];
+                        const double dd = c < lo ? lo - c : (c > hi ? c - hi : 0.0);
+                        d2 += dd * dd;
+                    }
+                    if ((*gauss_alpha)[t] * d2 <= 40.0) must = true;
+                }
+            }
+            if (split_check(M.n0 + d, std::sqrt(wn), prec, tnorm) || must) {
+                T.split(i);
+                for (int a = 0; a < 8; a++) next.push_back(T.child[i] + a);
+            }
+        }
+        work.swap(next);
+        iter++;
+    }
+    // bottom-up compression of the internal nodes
+    auto L = T.internal_by_depth();
+    std::vector<TransformItem> items;
+    for (int d = (int)L.size() - 1; d >= 0; d--) {
+        items.clear();
+        for (int p : L[d]) items.push_back({T.coef.p, p, T.child[p]});
+        C.titems.upload(items.data(), items.size(), s);
+        HC(launch_transform(s, 0, (int)items.size(), C.titems.p, M.W.p, M.B.K));
+    }
+    T.refresh_norms();
+}
+
+// ------------------------------------------------------------------ apply
+
+struct JobH {
+    double prec;
+    TreeH *out;
+    OperH *op;
+    TreeH *in;
+    std::vector<std::array<double, 8>> fmax;
+    double sNorm = 0.0, wNorm = 0.0, gsq = -1.0;
+    std::vector<int> work;
+    int64_t tuples = 0;
+    size_t genOff = 0;
+    uint32_t genMask = 0;
+};
+
+struct Box3 {
+    short m[3];
+};
+
+static void job_boxes(const JobH &J, int d, std::vector<TermTab> &tabs, int &tab0, int &ntab, double &thr) {
+    const OperH &O = *J.op;
+    const LevelH &L = *O.lev[d];
+    const int nterms = O.nterms();
+    thr = J.gsq > 0.0 ? SCREEN_REL * J.prec * std::sqrt(J.gsq / nterms) : 0.0;
+    tab0 = (int)tabs.size();
+    for (int t = 0; t < nterms; t++) {
+        const TermLevel &lv = L.t[t];
+        if (!lv.active) continue;
+        TermTab tb;
+        std::memset(&tb, 0, sizeof(tb));
+        tb.w = O.weff[t];
+        tb.band = lv.band;
+        tb.blk = lv.blk;
+        tb.nrm = lv.nrm;
+        for (int q = 0; q < 3; q++) {
+            tb.ub[q] = -1;
+            tb.ub0[q] = -1;
+        }
+        for (int combo = 0; combo < 64; combo++)
+            for (int q = 0; q < 3; q++) tb.box[combo][q] = -1;
+        bool any = false;
+        for (int gt = 0; gt < 8; gt++)
+            for (int ft = 0; ft < 8; ft++) {
+                if (d > 0 && gt == 0 && ft == 0) continue;
+                const double F = J.fmax[d][ft];
+                if (F <= 0.0) continue;
+                int ty[3];
+                for (int q = 0; q < 3; q++) ty[q] = 2 * ((gt >> q) & 1) + ((ft >> q) & 1);
+                const double base = std::fabs(O.weff[t]) * F;
+                const double full = base * lv.maxn[ty[0]] * lv.maxn[ty[1]] * lv.maxn[ty[2]];
+                if (!(full > thr)) continue;
+                short m3[3];
+                bool ok = true;
+                for (int q = 0; q < 3; q++) {
+                    double other = base;
+                    for (int q2 = 0; q2 < 3; q2++)
+                        if (q2 != q) other *= lv.maxn[ty[q2]];
+                    const double tau = thr / other;
+                    int m = -1;
+                    for (int mm = lv.band; mm >= 0; mm--)
+                        if (lv.env[ty[q] * (lv.band + 1) + mm] > tau) {
+                            m = mm;
+                            break;
+                        }
+                    if (m < 0) ok = false;
+                    m3[q] = (short)m;
+                }
+                if (!ok) continue;
+                any = true;
+                for (int q = 0; q < 3; q++) {
+                    tb.box[gt * 8 + ft][q] = m3[q];
+                    tb.ub[q] = std::max(tb.ub[q], m3[q]);
+                    if (ft == 0) tb.ub0[q] = std::max(tb.ub0[q], m3[q]);
+                }
+            }
+        if (!any) continue;
+        tabs.push_back(tb);
+    }
+    ntab = (int)tabs.size() - tab0;
+    if (ntab > 255) throw Error(MWG_ERR_RANGE, "more than 255 active terms at one depth");
+}
+
+static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, std::vector<mwg_apply_stats> &stats) {
+    if (jobs.empty()) return;
+    MraH &M = *jobs[0].in->mra;
+    Ctx &C = *M.ctx;
+    hipStream_t s = C.stream;
+    const int K = M.B.K, K3 = M.B.K3;
+    (void)flags;
+    auto t_start = std::chrono::steady_clock::now();
+    float t_contract_ms = 0.0f;
+    // inputs: device hash, norm maxima per depth
+    size_t genTotal = 0;
+    for (JobH &J : jobs) {
+        if (J.in->mra != &M || J.out->mra != &M || J.op->mra != &M)
+            throw Error(MWG_ERR_ARG, "all trees and operators of an apply must share one MRA");
+        if (J.in->n() == 0) throw Error(MWG_ERR_STATE, "input tree is empty");
+        J.in->build_hash();
+        J.fmax.assign(M.maxDepth + 2, std::array<double, 8>{0, 0, 0, 0, 0, 0, 0, 0});
+        std::vector<double> leafmax(M.maxDepth + 2, 0.0);
+        for (int i = 0; i < J.in->n(); i++) {
+            const int d = J.in->depth[i];
+            for (int c = 0; c < 8; c++) J.fmax[d][c] = std::max(J.fmax[d][c], J.in->cnorm[8 * (size_t)i + c]);
+            if (J.in->child[i] < 0) {
+                double fn = 0.0;
+                for (int c = 0; c < 8; c++) fn += J.in->cnorm[8 * (size_t)i + c] * J.in->cnorm[8 * (size_t)i + c];
+                leafmax[d] = std::max(leafmax[d], std::sqrt(fn));
+            }
+        }
+        double run = 0.0;
+        for (int d = 0; d < M.maxDepth + 2; d++) {
+            J.fmax[d][0] = std::max(J.fmax[d][0], run);
+            run = std::max(run, leafmax[d]);
+        }
+        J.out->make_roots();
+        J.out->ensure_dev(std::max(64, J.out->n()));
+        J.work = J.out->roots;
+        uint32_t cap = 1u << 14;
+        while (cap < 4u * (uint32_t)J.in->n()) cap <<= 1;
+        J.genOff = genTotal;
+        J.genMask = cap - 1;
+        genTotal += cap;
+    }
+    // generated-node tables
+    C.genHashKeys.alloc(genTotal);
+    C.genHashVals.alloc(genTotal);
+    HC(hipMemsetAsync(C.genHashKeys.p, 0, genTotal * sizeof(unsigned long long), s));
+    const int genCap = (int)std::min<size_t>(genTotal / 2, (size_t)1 << 24);
+    C.genKeys.alloc(genCap);
+    C.genJob.alloc(genCap);
+    C.genCount.alloc(1);
+    C.genOverflow.alloc(1);
+    C.genNorm.alloc(1024);
+    C.genPool.alloc(1024 * (size_t)K3);
+    HC(hipMemsetAsync(C.genCount.p, 0, sizeof(int), s));
+    int genDone = 0;
+
+    int iter = 0;
+    while (true) {
+        std::vector<int> active;
+        for (int j = 0; j < (int)jobs.size(); j++)
+            if (!jobs[j].work.empty()) active.push_back(j);
+        if (active.empty()) break;
+        const int d = iter;
+        std::vector<TermTab> tabs;
+        std::vector<JobTab> jt(jobs.size());
+        std::vector<WorkItem> work;
+        for (int j : active) {
+            JobH &J = jobs[j];
+            J.op->ensure(d);
+            JobTab &T = jt[j];
+            std::memset(&T, 0, sizeof(T));
+            job_boxes(J, d, tabs, T.tab0, T.ntab, T.thr);
+            J.out->ensure_dev(J.out->n());
+            T.inHash = J.in->dhash();
+            T.genHash = DHash{C.genHashKeys.p + J.genOff, C.genHashVals.p + J.genOff, J.genMask};
+            T.inCoef = J.in->coef.p;
+            T.inNorm = J.in->norm.p;
+            T.operBlocks = J.op->lev[d]->blocks.p;
+            T.operNorms = J.op->lev[d]->norms.p;
+            T.outCoef = J.out->coef.p;
+            T.outNorm = J.out->norm.p;
+            for (int q = 0; q < 3; q++) T.lmax[q] = M.boxes[q] << d;
+            T.depth = d;
+            for (int g : J.work) work.push_back({j, g, {J.out->lr[3 * g], J.out->lr[3 * g + 1], J.out->lr[3 * g + 2]}});
+        }
+        const int nw = (int)work.size();
+        C.work.upload(work.data(), work.size(), s);
+        C.jobs.upload(jt.data(), jt.size(), s);
+        if (tabs.empty()) {
+            TermTab dummy;
+            std::memset(&dummy, 0, sizeof(dummy));
+            tabs.push_back(dummy);
+        }
+        C.tabs.upload(tabs.data(), tabs.size(), s);
+        // generated input nodes
+        HC(hipMemsetAsync(C.genOverflow.p, 0, sizeof(int), s));
+        GenState gs{C.genKeys.p, C.genJob.p, C.genCount.p, genCap, C.genOverflow.p, C.genPool.p, C.genNorm.p};
+        HC(launch_gen_mark(s, nw, C.work.p, C.jobs.p, C.tabs.p, gs));
+        int hc[2];
+        HC(hipMemcpyAsync(&hc[0], C.genCount.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        HC(hipMemcpyAsync(&hc[1], C.genOverflow.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        HC(hipStreamSynchronize(s));
+        if (hc[1]) throw Error(MWG_ERR_RANGE, "generated-node table overflow");
+        const int genNow = hc[0];
+        if (genNow > genDone) {
+            C.genPool.grow_keep((size_t)genNow * K3, s);
+            C.genNorm.grow_keep((size_t)genNow, s);
+            gs.pool = C.genPool.p;
+            gs.norm = C.genNorm.p;
+            HC(launch_gen_compute(s, genDone, genNow - genDone, C.jobs.p, gs, M.W.p, K));
+            genDone = genNow;
+        }
+        // work list
+        C.thread_off.alloc((size_t)nw * 256);
+        C.gtotal.alloc(nw);
+        HC(launch_enum(s, 0, nw, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, nullptr, nullptr));
+        std::vector<int> tot(nw);
+        C.gtotal.download(tot.data(), nw, s);
+        std::vector<long long> ofs(nw);
+        long long total = 0;
+        for (int i = 0; i < nw; i++) {
+            ofs[i] = total;
+            total += tot[i];
+            jobs[work[i].job].tuples += tot[i];
+        }
+        C.gofs.upload(ofs.data(), nw, s);
+        C.tuples.alloc((size_t)std::max<long long>(total, 1));
+        HC(launch_enum(s, 1, nw, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, C.gofs.p,
+                       C.tuples.p));
+        // contraction
+        HC(hipEventRecord(C.ev0, s));
+        HC(launch_contract_fp64(s, K, nw, C.work.p, C.jobs.p, C.tabs.p, C.gofs.p, C.gtotal.p, C.tuples.p,
+                                C.genPool.p));
+        HC(hipEventRecord(C.ev1, s));
+        // norms and refinement
+        C.wnorms.alloc((size_t)nw * 8);
+        HC(launch_work_norms(s, nw, C.work.p, C.jobs.p, K3, C.wnorms.p));
+        std::vector<double> wn((size_t)nw * 8);
+        C.wnorms.download(wn.data(), wn.size(), s);
+        float ms = 0.0f;
+        HC(hipEventElapsedTime(&ms, C.ev0, C.ev1));
+        t_contract_ms += ms;
+        for (int i = 0; i < nw; i++) {
+            JobH &J = jobs[work[i].job];
+            std::memcpy(&J.out->cnorm[8 * (size_t)work[i].node], &wn[8 * (size_t)i], 8 * sizeof(double));
+        }
+        for (int j : active) {
+            JobH &J = jobs[j];
+            for (int g : J.work) {
+                const double *p = &J.out->cnorm[8 * (size_t)g];
+                if (iter == 0) J.sNorm += p[0] * p[0];
+                for (int c = 1; c < 8; c++) J.wNorm += p[c] * p[c];
+            }
+            J.gsq = J.sNorm + J.wNorm;
+        }
+        for (int j : active) {
+            JobH &J = jobs[j];
+            const double tnorm = std::sqrt(J.gsq);
+            std::vector<int> next;
+            for (int g : J.work) {
+                if (d >= M.maxDepth || (max_iter >= 0 && iter >= max_iter)) continue;
+                double w2 = 0.0;
+                const double *p = &J.out->cnorm[8 * (size_t)g];
+                for (int c = 1; c < 8; c++) w2 += p[c] * p[c];
+                if (split_check(M.n0 + d, std::sqrt(w2), J.prec, tnorm)) {
+                    J.out->split(g);
+                    for (int a = 0; a < 8; a++) next.push_back(J.out->child[g] + a);
+                }
+            }
+            J.work.swap(next);
+        }
+        iter++;
+    }
+    // top-down additive transform, then bottom-up compression, for all output trees
+    std::vector<std::vector<std::vector<int>>> L(jobs.size());
+    int maxd = 0;
+    for (size_t j = 0; j < jobs.size(); j++) {
+        L[j] = jobs[j].out->internal_by_depth();
+        maxd = std::max(maxd, (int)L[j].size());
+    }
+    std::vector<TransformItem> items;
+    for (int d = 0; d < maxd; d++) {
+        items.clear();
+        for (size_t j = 0; j < jobs.size(); j++)
+            if (d < (int)L[j].size())
+                for (int p : L[j][d]) items.push_back({jobs[j].out->coef.p, p, jobs[j].out->child[p]});
+        C.titems.upload(items.data(), items.size(), s);
+        HC(launch_transform(s, 1, (int)items.size(), C.titems.p, M.WT.p, K));
+    }
+    for (int d = maxd - 1; d >= 0; d--) {
+        items.clear();
+        for (size_t j = 0; j < jobs.size(); j++)
+            if (d < (int)L[j].size())
+                for (int p : L[j][d]) items.push_back({jobs[j].out->coef.p, p, jobs[j].out->child[p]});
+        C.titems.upload(items.data(), items.size(), s);
+        HC(launch_transform(s, 0, (int)items.size(), C.titems.p, M.W.p, K));
+    }
+    for (JobH &J : jobs) J.out->refresh_norms();
+    const double t_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    stats.assign(jobs.size(), mwg_apply_stats{});
+    const double fl = 6.0 * K * K * K * K;
+    for (size_t j = 0; j < jobs.size(); j++) {
+        mwg_apply_stats &st = stats[j];
+        st.nodes_out = jobs[j].out->n();
+        st.tuples = jobs[j].tuples;
+        st.tuples_fp64 = jobs[j].tuples;
+        st.gen_nodes = genDone;
+        st.iters = iter;
+        st.predicted_splits = 0;
+        st.t_apply = t_total;
+        st.t_contract = t_contract_ms * 1e-3;
+        st.flops = fl * (double)jobs[j].tuples;
+    }
+}
+
+}  // namespace mwg
+
+// ================================================================== C ABI
+
+using namespace mwg;
+
+struct mwg_ctx_s : Ctx {};
+struct mwg_mra_s : MraH {};
+struct mwg_tree_s : TreeH {};
+struct mwg_oper_s : OperH {};
+
+static thread_local std::string g_err;
+
+template <class F>
+static int guarded(Ctx *ctx, F f) {
+    try {
+        f();
+        return MWG_OK;
+    } catch (const Error &e) {
+        (ctx ? ctx->err : g_err) = e.what();
+        return e.code;
+    } catch (const std::exception &e) {
+        (ctx ? ctx->err : g_err) = e.what();
+        return MWG_ERR_STATE;
+    }
+}
+
+extern "C" {
+
+int mwg_ctx_create(int device, mwg_ctx *out) {
+    if (!out) return MWG_ERR_ARG;
+    auto *c = new mwg_ctx_s();
+    int rc = guarded(nullptr, [&] {
+        c->device = device;
+        HC(hipSetDevice(device));
+        HC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HC(hipEventCreate(&c->ev0));
+        HC(hipEventCreate(&c->ev1));
+        double x[24], w[24];
+        gauss_legendre01(24, x, w);
+        c->gx.upload(x, 24, c->stream);
+        c->gw.upload(w, 24, c->stream);
+    });
+    if (rc) {
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return MWG_OK;
+}
+
+int mwg_ctx_destroy(mwg_ctx ctx) {
+    if (!ctx) return MWG_OK;
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    hipStream_t s = ctx->stream;
+    delete ctx;
+    if (s) (void)hipStreamDestroy(s);
+    return MWG_OK;
+}
+
+const char *mwg_last_error(mwg_ctx ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
+
+int mwg_sync(mwg_ctx ctx) {
+    if (!ctx) return MWG_ERR_ARG;
+    return guarded(ctx, [&] { HC(hipStreamSynchronize(ctx->stream)); });
+}
+
+int mwg_mra_create(mwg_ctx ctx, int root_scale, const int *corner, const int *boxes, double scaling_factor, int order,
+                   int basis_type, int max_depth, mwg_mra *out) {
+    if (!ctx || !corner || !boxes || !out) return MWG_ERR_ARG;
+    auto *m = new mwg_mra_s();
+    int rc = guarded(ctx, [&] {
+        if (order < 3 || order > 9) throw Error(MWG_ERR_ARG, "order must be in [3, 9]");
+        if (basis_type != MWG_BASIS_LEGENDRE && basis_type != MWG_BASIS_INTERPOLATING)
+            throw Error(MWG_ERR_ARG, "basis type must be Legendre or Interpolating");
+        if (!(scaling_factor > 0.0)) throw Error(MWG_ERR_ARG, "scaling factor must be positive");
+        for (int d = 0; d < 3; d++)
+            if (boxes[d] < 1) throw Error(MWG_ERR_ARG, "boxes must be >= 1");
+        int bmax = std::max({boxes[0], boxes[1], boxes[2]});
+        if (max_depth < 1 || ((int64_t)bmax << (max_depth + 1)) > (1 << 19))
+            throw Error(MWG_ERR_RANGE, "max depth too large for 19-bit translations");
+        m->ctx = ctx;
+        m->n0 = root_scale;
+        for (int d = 0; d < 3; d++) {
+            m->corner[d] = corner[d];
+            m->boxes[d] = boxes[d];
+        }
+        m->sfac = scaling_factor;
+        m->maxDepth = max_depth;
+        build_basis(m->B, order, basis_type);
+        hipStream_t s = ctx->stream;
+        m->U.upload(m->B.U.data(), m->B.U.size(), s);
+        m->qx.upload(m->B.qx.data(), m->B.qx.size(), s);
+        m->qw.upload(m->B.qw.data(), m->B.qw.size(), s);
+        m->phiq.upload(m->B.phiq.data(), m->B.phiq.size(), s);
+        m->W.upload(m->B.W.data(), m->B.W.size(), s);
+        const int N = 2 * m->B.K;
+        std::vector<double> WT(N * N);
+        for (int i = 0; i < N; i++)
+            for (int j = 0; j < N; j++) WT[i * N + j] = m->B.W[j * N + i];
+        m->WT.upload(WT.data(), WT.size(), s);
+    });
+    if (rc) {
+        delete m;
+        return rc;
+    }
+    *out = m;
+    return MWG_OK;
+}
+
+int mwg_mra_destroy(mwg_mra mra) {
+    delete mra;
+    return MWG_OK;
+}
+
+int mwg_tree_create(mwg_mra mra, mwg_tree *out) {
+    if (!mra || !out) return MWG_ERR_ARG;
+    auto *t = new mwg_tree_s();
+    t->mra = mra;
+    *out = t;
+    return MWG_OK;
+}
+int mwg_tree_destroy(mwg_tree tree) {
+    delete tree;
+    return MWG_OK;
+}
+int mwg_tree_nnodes(mwg_tree tree, int64_t *n) {
+    if (!tree || !n) return MWG_ERR_ARG;
+    *n = tree->n();
+    return MWG_OK;
+}
+int mwg_tree_sqnorm(mwg_tree tree, double *out) {
+    if (!tree || !out) return MWG_ERR_ARG;
+    *out = tree->sqnorm;
+    return MWG_OK;
+}
+int mwg_tree_maxdepth(mwg_tree tree, int *out) {
+    if (!tree || !out) return MWG_ERR_ARG;
+    int m = 0;
+    for (int d : tree->depth) m = std::max(m, d);
+    *out = m;
+    return MWG_OK;
+}
+
+int mwg_tree_import(mwg_tree tree, int64_t n, const int *depth, const int64_t *lr, const double *coefs) {
+    if (!tree || n < 0 || (n > 0 && (!depth || !lr || !coefs))) return MWG_ERR_ARG;
+    Ctx *C = tree->mra->ctx;
+    return guarded(C, [&] {
+        TreeH &T = *tree;
+        T.clear();
+        const int nr = T.mra->boxes[0] * T.mra->boxes[1] * T.mra->boxes[2];
+        for (int64_t i = 0; i < n; i++) {
+            int par = -1;
+            if (depth[i] > 0) {
+                par = T.find(depth[i] - 1, (int)(lr[3 * i] >> 1), (int)(lr[3 * i + 1] >> 1), (int)(lr[3 * i + 2] >> 1));
+                if (par < 0) throw Error(MWG_ERR_ARG, "tree import: node before its parent");
+            }
+            const int id = T.add(depth[i], (int)lr[3 * i], (int)lr[3 * i + 1], (int)lr[3 * i + 2], par);
+            if (par < 0) T.roots.push_back(id);
+            else if (((lr[3 * i] & 1) | ((lr[3 * i + 1] & 1) << 1) | ((lr[3 * i + 2] & 1) << 2)) == 0) T.child[par] = id;
+        }
+        if ((int)T.roots.size() != nr) throw Error(MWG_ERR_ARG, "tree import: root count does not match the MRA");
+        T.ensure_dev(std::max<int64_t>(n, 1));
+        HC(hipMemcpyAsync(T.coef.p, coefs, (size_t)n * 8 * T.K3() * sizeof(double), hipMemcpyHostToDevice,
+                          C->stream));
+        T.refresh_norms();
+    });
+}
+
+int mwg_tree_export(mwg_tree tree, int *depth, int64_t *lr, double *coefs) {
+    if (!tree) return MWG_ERR_ARG;
+    Ctx *C = tree->mra->ctx;
+    return guarded(C, [&] {
+        TreeH &T = *tree;
+        for (int i = 0; i < T.n(); i++) {
+            if (depth) depth[i] = T.depth[i];
+            if (lr)
+                for (int q = 0; q < 3; q++) lr[3 * i + q] = T.lr[3 * i + q];
+        }
+        if (coefs) T.coef.download(coefs, (size_t)T.n() * 8 * T.K3(), C->stream);
+    });
+}
+
+int mwg_tree_rescale(mwg_tree tree, double c) {
+    if (!tree) return MWG_ERR_ARG;
+    Ctx *C = tree->mra->ctx;
+    return guarded(C, [&] {
+        TreeH &T = *tree;
+        HC(launch_scale(C->stream, (size_t)T.n() * 8 * T.K3(), T.coef.p, c));
+        HC(launch_scale(C->stream, (size_t)T.n() * 8, T.norm.p, std::fabs(c)));
+        for (double &v : T.cnorm) v *= std::fabs(c);
+        T.sqnorm *= c * c;
+        HC(hipStreamSynchronize(C->stream));
+    });
+}
+
+int mwg_tree_dot(mwg_tree a, mwg_tree b, double *out) {
+    if (!a || !b || !out) return MWG_ERR_ARG;
+    if (a->mra != b->mra) return MWG_ERR_ARG;
+    Ctx *C = a->mra->ctx;
+    return guarded(C, [&] {
+        std::vector<int> ia, ib, c0;
+        for (size_t r = 0; r < a->roots.size(); r++) {
+            ia.push_back(a->roots[r]);
+            ib.push_back(b->roots[r]);
+            c0.push_back(0);
+        }
+        for (int i = 0; i < a->n(); i++) {
+            if (a->depth[i] == 0) continue;
+            const int j = b->find(a->depth[i], a->lr[3 * i], a->lr[3 * i + 1], a->lr[3 * i + 2]);
+            if (j < 0) continue;
+            ia.push_back(i);
+            ib.push_back(j);
+            c0.push_back(1);
+        }
+        hipStream_t s = C->stream;
+        DBuf<int> dia, dib, dc0;
+        DBuf<double> part;
+        dia.upload(ia.data(), ia.size(), s);
+        dib.upload(ib.data(), ib.size(), s);
+        dc0.upload(c0.data(), c0.size(), s);
+        part.alloc(ia.size());
+        HC(launch_dot(s, (int)ia.size(), dia.p, dib.p, dc0.p, a->coef.p, b->coef.p, a->K3(), part.p));
+        std::vector<double> hp(ia.size());
+        part.download(hp.data(), hp.size(), s);
+        double sum = 0.0;
+        for (double v : hp) sum += v;
+        *out = sum;
+    });
+}
+
+int mwg_tree_eval(mwg_tree tree, int64_t n, const double *xyz, double *out) {
+    if (!tree || n < 0 || (n > 0 && (!xyz || !out))) return MWG_ERR_ARG;
+    Ctx *C = tree->mra->ctx;
+    return guarded(C, [&] {
+        TreeH &T = *tree;
+        const MraH &M = *T.mra;
+        const Basis &B = M.B;
+        const int K = B.K, K3 = B.K3, N = 2 * K;
+        std::vector<double> host((size_t)T.n() * 8 * K3);
+        T.coef.download(host.data(), host.size(), C->stream);
+        std::vector<double> WT(N * N);
+        for (int i = 0; i < N; i++)
+            for (int j = 0; j < N; j++) WT[i * N + j] = B.W[j * N + i];
+        for (int64_t pi = 0; pi < n; pi++) {
+            const double *p = &xyz[3 * pi];
+            double u0[3];
+            bool inside = true;
+            for (int d = 0; d < 3; d++) {
+                u0[d] = (p[d] - M.lo(d)) / (M.sfac * std::ldexp(1.0, -M.n0));
+                if (u0[d] < 0.0 || u0[d] >= M.boxes[d]) inside = false;
+            }
+            if (!inside) {
+                out[pi] = 0.0;
+                continue;
+            }
+            int node = T.find(0, (int)u0[0], (int)u0[1], (int)u0[2]);
+            int depth = 0;
+            while (node >= 0 && T.child[node] >= 0) {
+                depth++;
+                int a = 0;
+                for (int d = 0; d < 3; d++) a |= (int)(((int64_t)std::ldexp(u0[d], depth)) & 1) << d;
+                node = T.child[node] + a;
+            }
+            if (node < 0) {
+                out[pi] = 0.0;
+                continue;
+            }
+            // reconstruct the child of the leaf that holds the point, then evaluate
+            const int cd = depth + 1;
+            int a = 0;
+            double uc[3];
+            for (int d = 0; d < 3; d++) {
+                const double u = std::ldexp(u0[d], cd);
+                const int64_t li = (int64_t)u;
+                uc[d] = u - (double)li;
+                a |= (int)(li & 1) << d;
+            }
+            const double *cf = &host[(size_t)node * 8 * K3];
+            double px[MAXK], py[MAXK], pz[MAXK];
+            B.phi(uc[0], px);
+            B.phi(uc[1], py);
+            B.phi(uc[2], pz);
+            // child a scaling: s[z][y][x] = sum_c sum_{i,j,l} W[c_x K+i][a_x K+x] ... comp_c
+            double acc = 0.0;
+            for (int c = 0; c < 8; c++) {
+                const int bx = c & 1, by = (c >> 1) & 1, bz = (c >> 2) & 1;
+                double vx[MAXK], vy[MAXK], vz[MAXK];
+                for (int i = 0; i < K; i++) {
+                    double sx = 0, sy = 0, sz = 0;
+                    for (int x = 0; x < K; x++) {
+                        sx += B.W[(bx * K + i) * N + (a & 1) * K + x] * px[x];
+                        sy += B.W[(by * K + i) * N + ((a >> 1) & 1) * K + x] * py[x];
+                        sz += B.W[(bz * K + i) * N + ((a >> 2) & 1) * K + x] * pz[x];
+                    }
+                    vx[i] = sx;
+                    vy[i] = sy;
+                    vz[i] = sz;
+                }
+                const double *cc = cf + c * K3;
+                for (int z = 0; z < K; z++)
+                    for (int y = 0; y < K; y++) {
+                        const double yz = vz[z] * vy[y];
+                        for (int x = 0; x < K; x++) acc += cc[(z * K + y) * K + x] * yz * vx[x];
+                    }
+            }
+            const double h = M.sfac * std::ldexp(1.0, -(M.n0 + cd));
+            out[pi] = acc / std::pow(h, 1.5);
+        }
+        (void)WT;
+    });
+}
+
+int mwg_project_gauss(double prec, mwg_tree out, int n, const double *coef, const double *alpha,
+                      const double *center, const double *poly) {
+    if (!out || n < 0 || (n > 0 && (!coef || !alpha || !center || !poly))) return MWG_ERR_ARG;
+    Ctx *C = out->mra->ctx;
+    return guarded(C, [&] {
+        hipStream_t s = C->stream;
+        DBuf<double> dc, da, dce, dp;
+        dc.upload(coef, n, s);
+        da.upload(alpha, n, s);
+        dce.upload(center, 3 * (size_t)n, s);
+        dp.upload(poly, 15 * (size_t)n, s);
+        std::vector<double> va(alpha, alpha + n), vc(center, center + 3 * (size_t)n);
+        MraH &M = *out->mra;
+        project_adaptive(
+            *out, prec,
+            [&](int nw, const int *work, const int *depth, const int *lr, double *dst) {
+                HC(launch_project_gauss(s, nw, work, depth, lr, M.geom(), M.dev(), n, dc.p, da.p, dce.p, dp.p, dst));
+            },
+            &va, &vc);
+    });
+}
+
+int mwg_project_hydrogen(double prec, mwg_tree out, int n, const double *coef, const int *q, double Z,
+                         const double *origin) {
+    if (!out || n < 0 || (n > 0 && (!coef || !q || !origin))) return MWG_ERR_ARG;
+    Ctx *C = out->mra->ctx;
+    return guarded(C, [&] {
+        hipStream_t s = C->stream;
+        DBuf<double> dc, dorig;
+        DBuf<int> dq;
+        dc.upload(coef, n, s);
+        dq.upload(q, 6 * (size_t)n, s);
+        dorig.upload(origin, 3, s);
+        MraH &M = *out->mra;
+        project_adaptive(
+            *out, prec,
+            [&](int nw, const int *work, const int *depth, const int *lr, double *dst) {
+                HC(launch_project_hyd(s, nw, work, depth, lr, M.geom(), M.dev(), n, dc.p, dq.p, Z, dorig.p, dst));
+            },
+            nullptr, nullptr);
+    });
+}
+
+static int make_oper(mwg_mra mra, double mu, double prec, mwg_oper *out) {
+    if (!mra || !out) return MWG_ERR_ARG;
+    auto *o = new mwg_oper_s();
+    int rc = guarded(mra->ctx, [&] {
+        if (!(prec > 0.0)) throw Error(MWG_ERR_ARG, "operator precision must be positive");
+        if (mu < 0.0) throw Error(MWG_ERR_ARG, "Helmholtz mu must be non-negative");
+        build_oper(*o, mra, mu, prec);
+        o->ensure(0);
+    });
+    if (rc) {
+        delete o;
+        return rc;
+    }
+    *out = o;
+    return MWG_OK;
+}
+
+int mwg_oper_poisson(mwg_mra mra, double prec, mwg_oper *out) { return make_oper(mra, 0.0, prec, out); }
+int mwg_oper_helmholtz(mwg_mra mra, double mu, double prec, mwg_oper *out) {
+    if (!(mu > 0.0)) {
+        if (mra) mra->ctx->err = "Helmholtz mu must be positive (mu = sqrt(-2 epsilon), epsilon < 0)";
+        return MWG_ERR_ARG;
+    }
+    return make_oper(mra, mu, prec, out);
+}
+int mwg_oper_destroy(mwg_oper oper) {
+    delete oper;
+    return MWG_OK;
+}
+int mwg_oper_nterms(mwg_oper oper, int *n) {
+    if (!oper || !n) return MWG_ERR_ARG;
+    *n = oper->nterms();
+    return MWG_OK;
+}
+int mwg_oper_terms(mwg_oper oper, double *exponents, double *weights) {
+    if (!oper) return MWG_ERR_ARG;
+    for (int t = 0; t < oper->nterms(); t++) {
+        if (exponents) exponents[t] = oper->fit.a[t];
+        if (weights) weights[t] = oper->fit.w[t];
+    }
+    return MWG_OK;
+}
+int mwg_oper_blocks(mwg_oper oper, int term, int depth, int64_t dlo, int64_t dhi, double *out) {
+    if (!oper || !out || term < 0 || term >= oper->nterms() || depth < 0 || dhi < dlo) return MWG_ERR_ARG;
+    Ctx *C = oper->mra->ctx;
+    return guarded(C, [&] {
+        DBuf<double> b, nrm;
+        std::vector<long long> rows;
+        compute_ns(*oper, depth, {NSRequest{term, dlo, dhi}}, b, nrm, rows);
+        b.download(out, (size_t)(dhi - dlo + 1) * 4 * oper->mra->B.K2, C->stream);
+    });
+}
+
+int mwg_apply_batch(int n, const double *prec, mwg_tree *out, mwg_oper *oper, mwg_tree *inp, int max_iter,
+                    unsigned flags, mwg_apply_stats *stats) {
+    if (n <= 0 || !prec || !out || !oper || !inp) return MWG_ERR_ARG;
+    for (int i = 0; i < n; i++)
+        if (!out[i] || !oper[i] || !inp[i]) return MWG_ERR_ARG;
+    Ctx *C = inp[0]->mra->ctx;
+    return guarded(C, [&] {
+        if (flags & ~(MWG_APPLY_MIXED | MWG_APPLY_PREDICT)) throw Error(MWG_ERR_ARG, "unknown apply flags");
+        if (flags != MWG_APPLY_FP64) throw Error(MWG_ERR_ARG, "only the FP64 apply is available in this build");
+        std::vector<JobH> jobs(n);
+        for (int i = 0; i < n; i++) {
+            if (!(prec[i] > 0.0)) throw Error(MWG_ERR_ARG, "apply precision must be positive");
+            for (int j = 0; j < n; j++)
+                if (out[i] == inp[j] || (j != i && out[i] == out[j]))
+                    throw Error(MWG_ERR_ARG, "output trees must be distinct from inputs and each other");
+            jobs[i].prec = prec[i];
+            jobs[i].out = out[i];
+            jobs[i].op = oper[i];
+            jobs[i].in = inp[i];
+        }
+        std::vector<mwg_apply_stats> st;
+        apply_jobs(jobs, max_iter, flags, st);
+        if (stats)
+            for (int i = 0; i < n; i++) stats[i] = st[i];
+    });
+}
+
+int mwg_apply(double prec, mwg_tree out, mwg_oper oper, mwg_tree inp, int max_iter, unsigned flags,
+              mwg_apply_stats *stats) {
+    return mwg_apply_batch(1, &prec, &out, &oper, &inp, max_iter, flags, stats);
+}
+
+}  // extern "C"

@@ -1,0 +1,194 @@
+// Host-side numerics: quadrature, scaling bases, two-scale filters, kernel fit.
+//
+// Scaling bases follow MRCPP's two choices as selected by src/mrenv.cpp:62-69
+// (Legendre "L", Interpolating "I"); the wavelet rows of the two-scale matrix are
+// any orthonormal complement (norms, and hence every screening and refinement
+// decision, do not depend on that choice).  The kernel fit writes 1/r and
+// exp(-mu r)/r as (2/sqrt(pi)) int exp(-r^2 e^{2s} - mu^2 e^{-2s}/4 + s) ds and
+// applies the trapezoidal rule in s (Harrison et al., J. Chem. Phys. 121, 11587
+// (2004)); terms that are constant over the box to eps^0.4 are merged.
+#include "mwg_core.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace mwg {
+
+void gauss_legendre01(int q, double *x, double *w) {
+    for (int i = 0; i < q; i++) {
+        long double t = std::cos(M_PI * (i + 0.75L) / (q + 0.5L));
+        long double p0 = 1, p1 = t, dp = 1;
+        for (int it = 0; it < 100; it++) {
+            p0 = 1.0L;
+            p1 = t;
+            for (int n = 2; n <= q; n++) {
+                long double p2 = ((2.0L * n - 1.0L) * t * p1 - (n - 1.0L) * p0) / n;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = q * (t * p1 - p0) / (t * t - 1.0L);
+            long double dt = p1 / dp;
+            t -= dt;
+            if (std::fabs((double)dt) < 1e-19) break;
+        }
+        p0 = 1.0L;
+        p1 = t;
+        for (int n = 2; n <= q; n++) {
+            long double p2 = ((2.0L * n - 1.0L) * t * p1 - (n - 1.0L) * p0) / n;
+            p0 = p1;
+            p1 = p2;
+        }
+        dp = q * (t * p1 - p0) / (t * t - 1.0L);
+        x[q - 1 - i] = (double)(0.5L * (1.0L + t));
+        w[q - 1 - i] = (double)(1.0L / ((1.0L - t * t) * dp * dp));
+    }
+}
+
+static void legendre_scaled(int K, double x, double *out) {
+    double t = 2.0 * x - 1.0, p0 = 1.0, p1 = t;
+    out[0] = 1.0;
+    if (K > 1) out[1] = std::sqrt(3.0) * t;
+    for (int n = 2; n < K; n++) {
+        double p2 = ((2.0 * n - 1.0) * t * p1 - (n - 1.0) * p0) / n;
+        p0 = p1;
+        p1 = p2;
+        out[n] = std::sqrt(2.0 * n + 1.0) * p2;
+    }
+}
+
+void Basis::phi(double x, double *out) const {
+    double l[MAXK];
+    legendre_scaled(K, x, l);
+    for (int i = 0; i < K; i++) {
+        double s = 0.0;
+        for (int j = 0; j < K; j++) s += U[i * K + j] * l[j];
+        out[i] = s;
+    }
+}
+
+void build_basis(Basis &B, int k, int type) {
+    B.k = k;
+    B.K = k + 1;
+    B.K2 = B.K * B.K;
+    B.K3 = B.K2 * B.K;
+    B.type = type;
+    const int K = B.K, N = 2 * K;
+    B.qx.assign(K, 0.0);
+    B.qw.assign(K, 0.0);
+    gauss_legendre01(K, B.qx.data(), B.qw.data());
+    B.U.assign(K * K, 0.0);
+    if (type == 0) {
+        for (int i = 0; i < K; i++) B.U[i * K + i] = 1.0;
+    } else {
+        double l[MAXK];
+        for (int j = 0; j < K; j++) {
+            legendre_scaled(K, B.qx[j], l);
+            for (int i = 0; i < K; i++) B.U[j * K + i] = std::sqrt(B.qw[j]) * l[i];
+        }
+    }
+    B.phiq.assign(K * K, 0.0);
+    for (int q = 0; q < K; q++) B.phi(B.qx[q], &B.phiq[q * K]);
+
+    std::vector<long double> Wl(N * N, 0.0L);
+    double pa[MAXK], pb[MAXK];
+    for (int a = 0; a < 2; a++)
+        for (int q = 0; q < K; q++) {
+            B.phi(0.5 * (B.qx[q] + a), pa);
+            B.phi(B.qx[q], pb);
+            for (int i = 0; i < K; i++)
+                for (int j = 0; j < K; j++)
+                    Wl[i * N + a * K + j] += (long double)B.qw[q] * pa[i] * pb[j] / std::sqrt(2.0L);
+        }
+    // wavelet rows: orthonormal complement built from the sign-flipped scaling rows
+    for (int j = 0; j < K; j++) {
+        long double v[2 * MAXK];
+        for (int c = 0; c < K; c++) {
+            v[c] = Wl[j * N + c];
+            v[K + c] = -Wl[j * N + K + c];
+        }
+        for (int pass = 0; pass < 2; pass++)
+            for (int r = 0; r < K + j; r++) {
+                long double d = 0.0L;
+                for (int c = 0; c < N; c++) d += v[c] * Wl[r * N + c];
+                for (int c = 0; c < N; c++) v[c] -= d * Wl[r * N + c];
+            }
+        long double nrm = 0.0L;
+        for (int c = 0; c < N; c++) nrm += v[c] * v[c];
+        nrm = std::sqrt(nrm);
+        for (int c = 0; c < N; c++) Wl[(K + j) * N + c] = v[c] / nrm;
+    }
+    B.W.assign(N * N, 0.0);
+    for (int i = 0; i < N * N; i++) B.W[i] = (double)Wl[i];
+}
+
+static double fit_error(const KernelFit &F, double mu) {
+    const int J = 400;
+    double err = 0.0;
+    for (int j = 0; j < J; j++) {
+        double r = F.rmin * std::pow(F.rmax / F.rmin, j / (double)(J - 1));
+        double f = 0.0;
+        for (size_t i = 0; i < F.a.size(); i++) f += F.w[i] * std::exp(-F.a[i] * r * r);
+        err = std::max(err, std::fabs(f - std::exp(-mu * r) / r) * r);
+    }
+    return err;
+}
+
+KernelFit fit_kernel(double mu, double eps, double rmin, double rmax) {
+    const double L = std::log(10.0 / eps);
+    const double s_hi = std::log(std::sqrt(L) / rmin);
+    double s_lo = std::log(eps * std::sqrt(M_PI) / (20.0 * rmax));
+    if (mu > 0.0) s_lo = std::max(s_lo, -0.5 * std::log(4.0 * L / (mu * mu)));
+    const double tau = std::pow(eps, 0.4);
+    KernelFit F;
+    F.rmin = rmin;
+    F.rmax = rmax;
+    F.eps = eps;
+    for (int m = 0; m < 400; m++) {
+        const double h0 = std::pow(0.95, m);
+        const int N = (int)std::ceil((s_hi - s_lo) / h0) + 1;
+        const double h = (s_hi - s_lo) / (N - 1);
+        std::vector<double> a, w;
+        double wsum = 0.0, awsum = 0.0;
+        int merged = 0;
+        for (int i = 0; i < N; i++) {
+            const double s = s_lo + h * i;
+            const double ai = std::exp(2.0 * s);
+            const double wi = 2.0 / std::sqrt(M_PI) * h * std::exp(s) * std::exp(-mu * mu * std::exp(-2.0 * s) / 4.0);
+            if (ai * rmax * rmax < tau) {
+                wsum += wi;
+                awsum += wi * ai;
+                merged++;
+            } else {
+                a.push_back(ai);
+                w.push_back(wi);
+            }
+        }
+        std::vector<double> a1, w1;
+        if (merged > 0 && wsum > 0.0) {
+            a1.push_back(awsum / wsum);
+            w1.push_back(wsum);
+        }
+        a1.insert(a1.end(), a.begin(), a.end());
+        w1.insert(w1.end(), w.begin(), w.end());
+        F.a.clear();
+        F.w.clear();
+        for (size_t i = 0; i < a1.size(); i++) {
+            const double rs = std::min(std::max(1.0 / std::sqrt(2.0 * a1[i]), rmin), rmax);
+            if (w1[i] * rs * std::exp(-a1[i] * rs * rs) >= eps * 1e-3) {
+                F.a.push_back(a1[i]);
+                F.w.push_back(w1[i]);
+            }
+        }
+        F.h = h;
+        F.err = fit_error(F, mu);
+        if (F.err <= eps) break;
+    }
+    return F;
+}
+
+bool split_check(int scale, double wnorm, double prec, double tnorm) {
+    const double thr = std::max(2.0 * 2.220446049250313e-16, prec * tnorm * std::pow(2.0, -0.5 * (scale + 1)));
+    return wnorm > thr;
+}
+
+}  // namespace mwg

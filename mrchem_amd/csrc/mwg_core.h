@@ -1,0 +1,55 @@
+// Host-side numerics of the MI355X multiwavelet apply: scaling bases, two-scale
+// filters, Gaussian-sum kernel fit and the per-level operator bookkeeping that
+// decides which NS blocks the device builds and which tuples the device screens.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace mwg {
+
+constexpr int MAXK = 10;  // k <= 9
+
+// Gauss-Legendre rule on [0,1], ascending nodes.
+void gauss_legendre01(int q, double *x, double *w);
+
+struct Basis {
+    int k = 0, K = 0, K2 = 0, K3 = 0, type = 0;  // type 0 Legendre, 1 Interpolating
+    std::vector<double> U;       // K x K: phi^B_i = sum_j U[i][j] sqrt(2j+1) P_j(2x-1)
+    std::vector<double> qx, qw;  // K-point rule on [0,1]
+    std::vector<double> phiq;    // [q][i] = phi^B_i(qx[q])
+    std::vector<double> W;       // 2K x 2K two-scale matrix [[H0 H1][G0 G1]]
+    void phi(double x, double *out) const;
+};
+void build_basis(Basis &B, int k, int type);
+
+struct KernelFit {
+    std::vector<double> a, w;  // exp(-a r^2) terms (physical units)
+    double rmin = 0, rmax = 0, eps = 0, h = 0, err = 0;
+};
+// 1/r (mu = 0) or exp(-mu r)/r as a Gaussian sum with max |fit - K| r <= eps on [rmin, rmax]
+KernelFit fit_kernel(double mu, double eps, double rmin, double rmax);
+
+// node key: depth (6 bits) | 3 x 19-bit translations relative to the root corner
+inline uint64_t node_key(int depth, int64_t x, int64_t y, int64_t z) {
+    return ((uint64_t)depth << 57) | ((uint64_t)x << 38) | ((uint64_t)y << 19) | (uint64_t)z;
+}
+inline uint32_t key_hash(uint64_t k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdULL;
+    k ^= k >> 33;
+    k *= 0xc4ceb53fe1a85ec3ULL;
+    k ^= k >> 33;
+    return (uint32_t)k;
+}
+
+// screening constants shared with the device code
+constexpr double SCREEN_REL = 0.1;  // tuple threshold = SCREEN_REL * prec * ||g|| / sqrt(n_terms)
+constexpr double ACT_REL = 1e-4;    // term active at depth > 0 iff max ||C,B,A|| > ACT_REL * prec * max ||T||
+constexpr double GAUSS_CUT = 46.0;  // exp(-46) ~ 1e-20: translation band of a Gaussian
+
+// split rule: ||w|| > prec * ||tree|| * 2^{-(n+1)/2}
+bool split_check(int scale, double wnorm, double prec, double tnorm);
+
+}  // namespace mwg

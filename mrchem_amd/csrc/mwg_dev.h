@@ -1,0 +1,62 @@
+// Device-side data layouts shared by the HIP kernels and the host orchestration.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace mwg {
+
+// open-addressing hash: keys[] holds key+1 (0 = empty), vals[] the node index
+struct DHash {
+    unsigned long long *keys = nullptr;
+    int *vals = nullptr;
+    uint32_t mask = 0;
+};
+
+__host__ __device__ inline uint32_t dkey_hash(uint64_t k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdULL;
+    k ^= k >> 33;
+    k *= 0xc4ceb53fe1a85ec3ULL;
+    k ^= k >> 33;
+    return (uint32_t)k;
+}
+__host__ __device__ inline uint64_t dnode_key(int depth, int64_t x, int64_t y, int64_t z) {
+    return ((uint64_t)depth << 57) | ((uint64_t)x << 38) | ((uint64_t)y << 19) | (uint64_t)z;
+}
+
+__device__ inline int dhash_find(const DHash &h, uint64_t key) {
+    const unsigned long long kk = (unsigned long long)key + 1ull;
+    uint32_t i = dkey_hash(key) & h.mask;
+    for (uint32_t probe = 0; probe <= h.mask; probe++) {
+        unsigned long long v = h.keys[i];
+        if (v == kk) return h.vals[i];
+        if (v == 0ull) return -1;
+        i = (i + 1) & h.mask;
+    }
+    return -1;
+}
+
+// one active Gaussian term at the current depth, as the work-list build sees it
+struct TermTab {
+    double w;          // weight (unit-coordinate, includes scaling factor^3)
+    long long blk;     // offset (doubles) of delta=-band, type 0 block in the level block array
+    long long nrm;     // offset (doubles) of delta=-band, type 0 norm in the level norm array
+    int band;
+    short ub[3];       // union box half-widths over all combos (-1: term unused)
+    short ub0[3];      // union box over combos with ft = 0
+    short box[64][3];  // per combo (gt*8+ft) half-widths, -1 = combo screened out entirely
+};
+
+// one surviving (g, f, ft, gt, term) tuple
+struct Tuple {
+    int g;          // index into the work list
+    int f;          // input node index (>= 0) or -(generated index + 1)
+    uint8_t term;   // index into the TermTab array
+    uint8_t combo;  // gt * 8 + ft
+    int16_t dx, dy, dz;
+};
+static_assert(sizeof(Tuple) == 16, "tuple record is 16 bytes");
+
+}  // namespace mwg

@@ -1,0 +1,93 @@
+// Kernel launchers and the device-side tables they consume.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "mwg_core.h"
+#include "mwg_dev.h"
+
+namespace mwg {
+
+constexpr double GAUSS_CUT_D = GAUSS_CUT;
+
+struct DevBasis {
+    int K;
+    const double *U;     // K x K
+    const double *qx;    // K
+    const double *qw;    // K
+    const double *phiq;  // K x K, [q][i]
+    const double *W;     // 2K x 2K
+};
+
+struct ProjGeom {
+    double lo[3];
+    double sfac;
+    int n0;
+};
+
+struct WorkItem {
+    int job;
+    int node;  // index in the job's output tree
+    int l[3];  // translation relative to the root corner
+};
+
+struct JobTab {
+    DHash inHash;
+    DHash genHash;
+    const double *inCoef;
+    const double *inNorm;
+    const double *operBlocks;  // this depth's block array of the job's operator
+    const double *operNorms;   // this depth's norm array
+    double *outCoef;
+    double *outNorm;
+    double thr;
+    int tab0, ntab;
+    int lmax[3];
+    int depth;
+};
+
+struct GenState {
+    uint64_t *keys;
+    int *job;
+    int *count;
+    int cap;
+    int *overflow;
+    double *pool;  // cap x K^3
+    double *norm;  // cap
+};
+
+hipError_t launch_tblock(hipStream_t s, int n, const int *item_term, const long long *item_delta, const double *b,
+                         double pre, const DevBasis &B, const double *gx, const double *gw, double *out);
+hipError_t launch_nsblocks(hipStream_t s, int n, const long long *item_delta, const int *item_row0,
+                           const int *item_tlo, const int *item_thi, const double *t1, const double *W, int K,
+                           double *blocks, double *norms);
+hipError_t launch_norms(hipStream_t s, int n, const int *nodes, const double *coef, double *norm, int K3);
+hipError_t launch_dot(hipStream_t s, int n, const int *ia, const int *ib, const int *c0, const double *A,
+                      const double *B, int K3, double *part);
+hipError_t launch_scale(hipStream_t s, size_t n, double *x, double c);
+hipError_t launch_project_gauss(hipStream_t s, int nw, const int *work, const int *depth, const int *lr,
+                                const ProjGeom &geo, const DevBasis &B, int nterm, const double *coef,
+                                const double *alpha, const double *center, const double *poly, double *out);
+hipError_t launch_project_hyd(hipStream_t s, int nw, const int *work, const int *depth, const int *lr,
+                              const ProjGeom &geo, const DevBasis &B, int nterm, const double *coef, const int *q,
+                              double Z, const double *origin, double *out);
+hipError_t launch_gen_mark(hipStream_t s, int nw, const WorkItem *work, const JobTab *jobs, const TermTab *tabs,
+                           const GenState &gs);
+hipError_t launch_gen_compute(hipStream_t s, int first, int n, const JobTab *jobs, const GenState &gs,
+                              const double *W, int K);
+hipError_t launch_enum(hipStream_t s, int mode, int nw, const WorkItem *work, const JobTab *jobs,
+                       const TermTab *tabs, const GenState &gs, int *thread_off, int *gtotal, const long long *gofs,
+                       Tuple *out);
+hipError_t launch_work_norms(hipStream_t s, int nw, const WorkItem *work, const JobTab *jobs, int K3,
+                             double *compact);
+struct TransformItem {
+    double *coef;
+    int parent;
+    int first_child;
+};
+hipError_t launch_transform(hipStream_t s, int mode, int n, const TransformItem *items, const double *M, int K);
+hipError_t launch_contract_fp64(hipStream_t s, int K, int nw, const WorkItem *work, const JobTab *jobs,
+                                const TermTab *tabs, const long long *gofs, const int *gtotal, const Tuple *tuples,
+                                const double *genPool);
+
+}  // namespace mwg

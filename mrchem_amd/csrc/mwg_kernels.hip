@@ -1,0 +1,914 @@
+// HIP kernels of the MI355X multiwavelet apply (gfx950, wave64).
+//
+// Coefficient layout of a node: 8 components x K^3 FP64 (K = k+1), component bit
+// 0 = x, 1 = y, 2 = z (0 = scaling), index (iz*K + iy)*K + ix inside a component.
+// Two-scale transforms work on the interleaved (2K)^3 tensor whose per-dimension
+// index is bit*K + i; each 1D pass multiplies every line of 2K values in place.
+#include "mwg_kernels.h"
+
+#include <cmath>
+
+namespace mwg {
+
+// ----------------------------------------------------------------- device basis
+
+__device__ inline void d_phi(const DevBasis &B, double x, double *out) {
+    const int K = B.K;
+    double l[MAXK];
+    double t = 2.0 * x - 1.0, p0 = 1.0, p1 = t;
+    l[0] = 1.0;
+    if (K > 1) l[1] = sqrt(3.0) * t;
+    for (int n = 2; n < K; n++) {
+        double p2 = ((2.0 * n - 1.0) * t * p1 - (n - 1.0) * p0) / n;
+        p0 = p1;
+        p1 = p2;
+        l[n] = sqrt(2.0 * n + 1.0) * p2;
+    }
+    for (int i = 0; i < K; i++) {
+        double s = 0.0;
+        for (int j = 0; j < K; j++) s += B.U[i * K + j] * l[j];
+        out[i] = s;
+    }
+}
+
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ----------------------------------------------------------------- operator blocks
+
+// Scaling-scaling block of one unit Gaussian exp(-b (delta + u - v)^2) at cell scale m:
+// t[i][j] = pre * int_{-1}^{1} exp(-b (delta+z)^2) C_ij(z) dz,
+// C_ij(z) = int phi_i(u) phi_j(u - z) du (K-point rule on the overlap of the two cells),
+// outer integral by 24-point Gauss-Legendre panels of width <= 0.5/sqrt(b) around the peak.
+__global__ void k_tblock(int n, const int *item_term, const long long *item_delta, const double *bvals, double pre,
+                         DevBasis B, const double *gx, const double *gw, double *out) {
+    const int it = blockIdx.x;
+    if (it >= n) return;
+    const int K = B.K, K2 = K * K, tid = threadIdx.x;
+    __shared__ double pu[MAXK * MAXK], pv[MAXK * MAXK], wq[MAXK];
+    const double b = bvals[item_term[it]];
+    const double delta = (double)item_delta[it];
+    const double zc = -delta;
+    const double R = sqrt(GAUSS_CUT_D / b);
+    const double pw = (b > 4.0) ? 0.5 / sqrt(b) : 0.25;
+    double acc = 0.0;
+    for (int half = 0; half < 2; half++) {
+        const double lo = half == 0 ? -1.0 : 0.0, hi = half == 0 ? 0.0 : 1.0;
+        const double a0 = fmax(lo, zc - R), a1 = fmin(hi, zc + R);
+        if (a1 <= a0) continue;
+        int np = (int)ceil((a1 - a0) / pw);
+        if (np < 1) np = 1;
+        const double ph = (a1 - a0) / np;
+        for (int p = 0; p < np; p++) {
+            const double p0 = a0 + p * ph;
+            for (int q = 0; q < 24; q++) {
+                const double z = p0 + ph * gx[q];
+                const double g = exp(-b * (delta + z) * (delta + z)) * ph * gw[q];
+                if (g == 0.0) continue;
+                const double ulo = fmax(0.0, z), uhi = fmin(1.0, 1.0 + z);
+                __syncthreads();
+                if (tid < K) {
+                    double v[MAXK];
+                    d_phi(B, ulo + (uhi - ulo) * B.qx[tid], v);
+                    for (int i = 0; i < K; i++) pu[tid * K + i] = v[i];
+                    wq[tid] = (uhi - ulo) * B.qw[tid];
+                } else if (tid < 2 * K) {
+                    const int q2 = tid - K;
+                    double v[MAXK];
+                    d_phi(B, ulo + (uhi - ulo) * B.qx[q2] - z, v);
+                    for (int i = 0; i < K; i++) pv[q2 * K + i] = v[i];
+                }
+                __syncthreads();
+                if (tid < K2) {
+                    const int i = tid / K, j = tid % K;
+                    double c = 0.0;
+                    for (int qq = 0; qq < K; qq++) c += wq[qq] * pu[qq * K + i] * pv[qq * K + j];
+                    acc += g * c;
+                }
+            }
+        }
+    }
+    if (tid < K2) out[(size_t)it * K2 + tid] = acc * pre;
+}
+
+// NS blocks from the child-scale scaling blocks: R = W M W^T with
+// M = [[t(2d), t(2d-1)], [t(2d+1), t(2d)]]; type = 2*gt_bit + ft_bit.
+__global__ void k_nsblocks(int n, const long long *item_delta, const int *item_row0, const int *item_tlo,
+                           const int *item_thi, const double *t1, const double *W, int K, double *blocks,
+                           double *norms) {
+    const int it = blockIdx.x;
+    if (it >= n) return;
+    const int N = 2 * K, K2 = K * K, tid = threadIdx.x, nt = blockDim.x;
+    __shared__ double Ms[4 * MAXK * MAXK], R1[4 * MAXK * MAXK], Ws[4 * MAXK * MAXK];
+    __shared__ double red[4][256];
+    const long long d = item_delta[it];
+    const int row0 = item_row0[it], tlo = item_tlo[it], thi = item_thi[it];
+    for (int e = tid; e < N * N; e += nt) Ws[e] = W[e];
+    for (int e = tid; e < N * N; e += nt) {
+        const int r = e / N, c = e % N;
+        const int a = r / K, i = r % K, bb = c / K, j = c % K;
+        const long long dd = 2 * d + a - bb;
+        double v = 0.0;
+        if (dd >= tlo && dd <= thi) v = t1[(size_t)(row0 + (dd - tlo)) * K2 + i * K + j];
+        Ms[e] = v;
+    }
+    __syncthreads();
+    for (int e = tid; e < N * N; e += nt) {
+        const int i = e / N, j = e % N;
+        double s = 0.0;
+        for (int l = 0; l < N; l++) s += Ws[i * N + l] * Ms[l * N + j];
+        R1[e] = s;
+    }
+    __syncthreads();
+    double part[4] = {0, 0, 0, 0};
+    for (int e = tid; e < N * N; e += nt) {
+        const int i = e / N, j = e % N;
+        double s = 0.0;
+        for (int l = 0; l < N; l++) s += R1[i * N + l] * Ws[j * N + l];
+        const int gb = i / K, a = i % K, fb = j / K, i2 = j % K;
+        const int type = 2 * gb + fb;
+        blocks[((size_t)it * 4 + type) * K2 + a * K + i2] = s;
+        part[type] += s * s;
+    }
+    for (int q = 0; q < 4; q++) red[q][tid] = part[q];
+    __syncthreads();
+    for (int s = nt / 2; s > 0; s >>= 1) {
+        if (tid < s)
+            for (int q = 0; q < 4; q++) red[q][tid] += red[q][tid + s];
+        __syncthreads();
+    }
+    if (tid < 4) norms[(size_t)it * 4 + tid] = sqrt(red[tid][0]);
+}
+
+// ----------------------------------------------------------------- tensor line passes
+
+// in-place pass along dimension `dim` of an N^3 LDS tensor: line <- M line (M is N x N)
+__device__ inline void line_pass(double *T, const double *M, int N, int dim) {
+    const int N2 = N * N;
+    const int stride = dim == 0 ? 1 : (dim == 1 ? N : N2);
+    for (int line = threadIdx.x; line < N2; line += blockDim.x) {
+        int base;
+        const int u = line % N, v = line / N;
+        if (dim == 0) base = line * N;
+        else if (dim == 1) base = v * N2 + u;
+        else base = line;
+        double in[2 * MAXK];
+        for (int q = 0; q < N; q++) in[q] = T[base + q * stride];
+        for (int p = 0; p < N; p++) {
+            double s = 0.0;
+            for (int q = 0; q < N; q++) s += M[p * N + q] * in[q];
+            T[base + p * stride] = s;
+        }
+    }
+}
+
+__device__ inline int interleaved_index(int K, int c, int e) {
+    // component c, in-component index e = (z*K + y)*K + x -> (2K)^3 index
+    const int N = 2 * K;
+    const int x = e % K, y = (e / K) % K, z = e / (K * K);
+    const int bx = c & 1, by = (c >> 1) & 1, bz = (c >> 2) & 1;
+    return ((bz * K + z) * N + (by * K + y)) * N + bx * K + x;
+}
+
+// component norms of listed nodes (one wave per node)
+__global__ void k_norms(int n, const int *nodes, const double *coef, double *norm, int K3) {
+    const int i = blockIdx.x;
+    if (i >= n) return;
+    const int node = nodes ? nodes[i] : i;
+    const double *p = coef + (size_t)node * 8 * K3;
+    for (int c = 0; c < 8; c++) {
+        double s = 0.0;
+        for (int e = threadIdx.x; e < K3; e += 64) s += p[c * K3 + e] * p[c * K3 + e];
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+        if (threadIdx.x == 0) norm[(size_t)node * 8 + c] = sqrt(s);
+    }
+}
+
+// per-pair partial dot products: sum over components [c0, 8)
+__global__ void k_dot(int n, const int *ia, const int *ib, const int *c0, const double *A, const double *B, int K3,
+                      double *part) {
+    const int i = blockIdx.x;
+    if (i >= n) return;
+    const double *a = A + (size_t)ia[i] * 8 * K3;
+    const double *b = B + (size_t)ib[i] * 8 * K3;
+    double s = 0.0;
+    for (int e = c0[i] * K3 + threadIdx.x; e < 8 * K3; e += 64) s += a[e] * b[e];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (threadIdx.x == 0) part[i] = s;
+}
+
+__global__ void k_scale(size_t n, double *x, double c) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] *= c;
+}
+
+// ----------------------------------------------------------------- projection
+
+// Gaussian sums are separable: the children's scaling tensor is a sum of outer
+// products of 2K-vectors (one per dimension), and so is its compression.
+template <int K>
+__global__ void __launch_bounds__(256) k_project_gauss(int nw, const int *work, const int *depth, const int *lr,
+                                                       ProjGeom geo, DevBasis B, int nterm, const double *coef,
+                                                       const double *alpha, const double *center,
+                                                       const double *poly, double *out) {
+    constexpr int N = 2 * K, N3 = N * N * N, K3 = K * K * K;
+    constexpr int PER = (N3 + 255) / 256;
+    constexpr int CH = 8;
+    __shared__ double P[CH][3][N], WP[CH][3][N], Ws[N * N], cf[CH];
+    const int node = work[blockIdx.x];
+    const int d = depth[node];
+    const double hc = geo.sfac * ldexp(1.0, -(geo.n0 + d + 1));
+    const double pre = sqrt(hc);
+    const int tid = threadIdx.x;
+    for (int e = tid; e < N * N; e += 256) Ws[e] = B.W[e];
+    double acc[PER];
+    for (int r = 0; r < PER; r++) acc[r] = 0.0;
+    for (int t0 = 0; t0 < nterm; t0 += CH) {
+        __syncthreads();
+        for (int e = tid; e < CH * 3 * N; e += 256) {
+            const int tt = e / (3 * N), dim = (e / N) % 3, p = e % N;
+            const int bit = p / K, i = p % K;
+            const int t = t0 + tt;
+            double s = 0.0;
+            if (t < nterm) {
+                const double x0 = geo.lo[dim] + hc * (double)(2 * lr[3 * node + dim] + bit);
+                const double c = center[3 * t + dim];
+                for (int q = 0; q < K; q++) {
+                    const double u = x0 + hc * B.qx[q] - c;
+                    double pv = 0.0, up = 1.0;
+                    for (int ee = 0; ee < 5; ee++) {
+                        pv += poly[15 * t + 5 * dim + ee] * up;
+                        up *= u;
+                    }
+                    s += B.phiq[q * K + i] * (pv * exp(-alpha[t] * u * u) * B.qw[q]);
+                }
+                s *= pre;
+            }
+            P[tt][dim][p] = s;
+            if (e < CH) cf[e] = (t0 + e < nterm) ? coef[t0 + e] : 0.0;
+        }
+        __syncthreads();
+        for (int e = tid; e < CH * 3 * N; e += 256) {
+            const int tt = e / (3 * N), dim = (e / N) % 3, p = e % N;
+            double s = 0.0;
+            for (int q = 0; q < N; q++) s += Ws[p * N + q] * P[tt][dim][q];
+            WP[tt][dim][p] = s;
+        }
+        __syncthreads();
+        for (int r = 0; r < PER; r++) {
+            const int e = tid + 256 * r;
+            if (e < N3) {
+                const int px = e % N, py = (e / N) % N, pz = e / (N * N);
+                double s = acc[r];
+                for (int tt = 0; tt < CH; tt++) s += cf[tt] * WP[tt][2][pz] * WP[tt][1][py] * WP[tt][0][px];
+                acc[r] = s;
+            }
+        }
+    }
+    for (int r = 0; r < PER; r++) {
+        const int e = tid + 256 * r;
+        if (e < N3) {
+            const int px = e % N, py = (e / N) % N, pz = e / (N * N);
+            const int c = (px / K) | ((py / K) << 1) | ((pz / K) << 2);
+            const int idx = ((pz % K) * K + (py % K)) * K + (px % K);
+            out[(size_t)node * 8 * K3 + c * K3 + idx] = acc[r];
+        }
+    }
+    (void)nw;
+}
+
+__device__ double d_hydrogen_orb(int N, int L, int M, double Z, double x, double y, double z) {
+    const double r = sqrt(x * x + y * y + z * z);
+    double c0;
+    if (N == 1 && L == 0) c0 = 2.0;
+    else if (N == 2 && L == 0) c0 = 1.0 / (2.0 * sqrt(2.0));
+    else if (N == 3 && L == 0) c0 = 1.0 / (9.0 * sqrt(3.0));
+    else if (N == 2 && L == 1) c0 = 1.0 / (2.0 * sqrt(6.0));
+    else if (N == 3 && L == 1) c0 = 1.0 / (9.0 * sqrt(6.0));
+    else if (N == 3 && L == 2) c0 = 1.0 / (9.0 * sqrt(30.0));
+    else return 0.0;
+    c0 *= pow(Z, 1.5);
+    const double rho = 2.0 * Z / N * r;
+    double poly;
+    if (N == 1 && L == 0) poly = 1.0;
+    else if (N == 2 && L == 0) poly = 2.0 - rho;
+    else if (N == 3 && L == 0) poly = 6.0 - 6.0 * rho + rho * rho;
+    else if (N == 2 && L == 1) poly = rho;
+    else if (N == 3 && L == 1) poly = rho * (4.0 - rho);
+    else poly = rho * rho;
+    const double Rr = c0 * poly * exp(-rho / 2.0);
+    double ang, ca;
+    if (L == 0) {
+        ca = 1.0;
+        ang = 1.0;
+    } else if (L == 1) {
+        ca = sqrt(3.0);
+        const double q = (M == 0) ? x : (M == 1) ? y : z;
+        ang = (r > 0.0) ? q / r : 0.0;
+    } else {
+        const double r2 = r * r;
+        ca = (M <= 2) ? sqrt(15.0) : (M == 3 ? sqrt(15.0 / 4.0) : sqrt(5.0 / 4.0));
+        if (r2 == 0.0) ang = 0.0;
+        else if (M == 0) ang = x * y / r2;
+        else if (M == 1) ang = x * z / r2;
+        else if (M == 2) ang = y * z / r2;
+        else if (M == 3) ang = (x * x - y * y) / r2;
+        else ang = (2.0 * z * z - x * x - y * y) / r2;
+    }
+    return Rr * ca / (2.0 * sqrt(M_PI)) * ang;
+}
+
+// hydrogen functions are not separable: values at the K^3 quadrature points of
+// each child, then three 1D quadrature passes, then compression
+__global__ void k_project_hyd(int nw, const int *work, const int *depth, const int *lr, ProjGeom geo, DevBasis B,
+                              int nterm, const double *coef, const int *q, double Z, const double *origin,
+                              double *out) {
+    extern __shared__ double smem[];
+    const int K = B.K, N = 2 * K, N3 = N * N * N, K3 = K * K * K;
+    double *T = smem;
+    double *Ws = smem + N3;
+    double *Qm = Ws + N * N;  // N x N block-diagonal quadrature matrix: phi_i(x_q) w_q * pre
+    const int node = work[blockIdx.x];
+    const int d = depth[node];
+    const double hc = geo.sfac * ldexp(1.0, -(geo.n0 + d + 1));
+    const double pre = sqrt(hc);
+    for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
+        Ws[e] = B.W[e];
+        const int p = e / N, qq = e % N;
+        Qm[e] = ((p / K) == (qq / K)) ? B.phiq[(qq % K) * K + (p % K)] * B.qw[qq % K] * pre : 0.0;
+    }
+    for (int e = threadIdx.x; e < N3; e += blockDim.x) {
+        const int px = e % N, py = (e / N) % N, pz = e / (N * N);
+        const double x = geo.lo[0] + hc * ((double)(2 * lr[3 * node] + px / K) + B.qx[px % K]) - origin[0];
+        const double y = geo.lo[1] + hc * ((double)(2 * lr[3 * node + 1] + py / K) + B.qx[py % K]) - origin[1];
+        const double z = geo.lo[2] + hc * ((double)(2 * lr[3 * node + 2] + pz / K) + B.qx[pz % K]) - origin[2];
+        double v = 0.0;
+        for (int t = 0; t < nterm; t++) {
+            double a = d_hydrogen_orb(q[6 * t], q[6 * t + 1], q[6 * t + 2], Z, x, y, z);
+            if (q[6 * t + 3] > 0) a *= d_hydrogen_orb(q[6 * t + 3], q[6 * t + 4], q[6 * t + 5], Z, x, y, z);
+            v += coef[t] * a;
+        }
+        T[e] = v;
+    }
+    __syncthreads();
+    for (int dim = 0; dim < 3; dim++) {
+        line_pass(T, Qm, N, dim);
+        __syncthreads();
+    }
+    for (int dim = 0; dim < 3; dim++) {
+        line_pass(T, Ws, N, dim);
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < 8 * K3; e += blockDim.x) {
+        const int c = e / K3, r = e % K3;
+        out[(size_t)node * 8 * K3 + e] = T[interleaved_index(K, c, r)];
+    }
+    (void)nw;
+}
+
+// ----------------------------------------------------------------- apply: generated nodes
+
+__device__ inline bool in_box(const short *b, int dx, int dy, int dz) {
+    return b[0] >= 0 && abs(dx) <= b[0] && abs(dy) <= b[1] && abs(dz) <= b[2];
+}
+
+// mark the input nodes that the output nodes need at this depth but the input
+// tree does not have (scaling-only "generated" nodes)
+__global__ void k_gen_mark(int nw, const WorkItem *work, const JobTab *jobs, const TermTab *tabs, GenState gs) {
+    const WorkItem wi = work[blockIdx.x];
+    const JobTab &J = jobs[wi.job];
+    for (int t = J.tab0; t < J.tab0 + J.ntab; t++) {
+        const TermTab &tb = tabs[t];
+        if (tb.ub0[0] < 0) continue;
+        const int ux = tb.ub0[0], uy = tb.ub0[1], uz = tb.ub0[2];
+        const int cells = (2 * ux + 1) * (2 * uy + 1) * (2 * uz + 1);
+        for (int c = threadIdx.x; c < cells; c += blockDim.x) {
+            const int dx = c % (2 * ux + 1) - ux;
+            const int dy = (c / (2 * ux + 1)) % (2 * uy + 1) - uy;
+            const int dz = c / ((2 * ux + 1) * (2 * uy + 1)) - uz;
+            const int fx = wi.l[0] - dx, fy = wi.l[1] - dy, fz = wi.l[2] - dz;
+            if (fx < 0 || fy < 0 || fz < 0 || fx >= J.lmax[0] || fy >= J.lmax[1] || fz >= J.lmax[2]) continue;
+            bool need = false;
+            for (int gt = 0; gt < 8 && !need; gt++) need = in_box(tb.box[gt * 8], dx, dy, dz);
+            if (!need) continue;
+            const uint64_t key = dnode_key(J.depth, fx, fy, fz);
+            if (dhash_find(J.inHash, key) >= 0) continue;
+            // insert into this job's generated-node table
+            const unsigned long long kk = (unsigned long long)key + 1ull;
+            uint32_t i = dkey_hash(key) & J.genHash.mask;
+            for (uint32_t probe = 0; probe <= J.genHash.mask; probe++) {
+                unsigned long long old = atomicCAS(&J.genHash.keys[i], 0ull, kk);
+                if (old == 0ull) {
+                    const int idx = atomicAdd(gs.count, 1);
+                    if (idx < gs.cap) {
+                        gs.keys[idx] = key;
+                        gs.job[idx] = wi.job;
+                    } else {
+                        atomicOr(gs.overflow, 1);
+                    }
+                    J.genHash.vals[i] = idx;
+                    break;
+                }
+                if (old == kk) break;
+                i = (i + 1) & J.genHash.mask;
+                if (probe == J.genHash.mask) atomicOr(gs.overflow, 2);
+            }
+        }
+    }
+}
+
+// scaling block of a generated node: reconstruct from the leaf ancestor (using its
+// wavelets for the first step), then refine with the low-pass filters only
+__global__ void k_gen_compute(int first, int n, const JobTab *jobs, GenState gs, const double *W, int K) {
+    extern __shared__ double smem[];
+    const int g = first + blockIdx.x;
+    if (g >= first + n) return;
+    const int N = 2 * K, N2 = N * N, K3 = K * K * K;
+    double *T = smem;
+    const uint64_t key = gs.keys[g];
+    const JobTab &J = jobs[gs.job[g]];
+    const int d = (int)(key >> 57);
+    const int64_t l[3] = {(int64_t)((key >> 38) & 0x7FFFF), (int64_t)((key >> 19) & 0x7FFFF), (int64_t)(key & 0x7FFFF)};
+    __shared__ int s_anc, s_ad;
+    if (threadIdx.x == 0) {
+        s_anc = -1;
+        s_ad = -1;
+        for (int dd = d - 1; dd >= 0; dd--) {
+            const int sh = d - dd;
+            const int a = dhash_find(J.inHash, dnode_key(dd, l[0] >> sh, l[1] >> sh, l[2] >> sh));
+            if (a >= 0) {
+                s_anc = a;
+                s_ad = dd;
+                break;
+            }
+        }
+    }
+    __syncthreads();
+    const int anc = s_anc, ad = s_ad;
+    double *outp = gs.pool + (size_t)g * K3;
+    if (anc < 0) {
+        for (int e = threadIdx.x; e < K3; e += blockDim.x) outp[e] = 0.0;
+        if (threadIdx.x == 0) gs.norm[g] = 0.0;
+        return;
+    }
+    const double *src = J.inCoef + (size_t)anc * 8 * K3;
+    for (int e = threadIdx.x; e < 8 * K3; e += blockDim.x) T[interleaved_index(K, e / K3, e % K3)] = src[e];
+    __syncthreads();
+    const int steps = d - ad;
+    // first step: child a of the (s, d) parent; line (2K) -> K values, s_j = sum_p W[p][a*K+j] in_p
+    {
+        const int sh = steps - 1;
+        const int ax = (int)((l[0] >> sh) & 1), ay = (int)((l[1] >> sh) & 1), az = (int)((l[2] >> sh) & 1);
+        // x: lines (z, y) over all N x N
+        for (int line = threadIdx.x; line < N2; line += blockDim.x) {
+            const int base = line * N;
+            double in[2 * MAXK];
+            for (int p = 0; p < N; p++) in[p] = T[base + p];
+            for (int j = 0; j < K; j++) {
+                double s = 0.0;
+                for (int p = 0; p < N; p++) s += W[p * N + ax * K + j] * in[p];
+                T[base + j] = s;
+            }
+        }
+        __syncthreads();
+        // y: lines (z, x<K)
+        for (int line = threadIdx.x; line < N * K; line += blockDim.x) {
+            const int z = line / K, x = line % K;
+            const int base = z * N2 + x;
+            double in[2 * MAXK];
+            for (int p = 0; p < N; p++) in[p] = T[base + p * N];
+            for (int j = 0; j < K; j++) {
+                double s = 0.0;
+                for (int p = 0; p < N; p++) s += W[p * N + ay * K + j] * in[p];
+                T[base + j * N] = s;
+            }
+        }
+        __syncthreads();
+        // z: lines (y<K, x<K)
+        for (int line = threadIdx.x; line < K * K; line += blockDim.x) {
+            const int y = line / K, x = line % K;
+            const int base = y * N + x;
+            double in[2 * MAXK];
+            for (int p = 0; p < N; p++) in[p] = T[base + p * N2];
+            for (int j = 0; j < K; j++) {
+                double s = 0.0;
+                for (int p = 0; p < N; p++) s += W[p * N + az * K + j] * in[p];
+                T[base + j * N2] = s;
+            }
+        }
+        __syncthreads();
+    }
+    for (int sh = steps - 2; sh >= 0; sh--) {
+        const int a3[3] = {(int)((l[0] >> sh) & 1), (int)((l[1] >> sh) & 1), (int)((l[2] >> sh) & 1)};
+        for (int dim = 0; dim < 3; dim++) {
+            const int stride = dim == 0 ? 1 : (dim == 1 ? N : N2);
+            for (int line = threadIdx.x; line < K * K; line += blockDim.x) {
+                const int u = line % K, v = line / K;
+                int base;
+                if (dim == 0) base = v * N2 + u * N;
+                else if (dim == 1) base = v * N2 + u;
+                else base = v * N + u;
+                double in[MAXK];
+                for (int p = 0; p < K; p++) in[p] = T[base + p * stride];
+                for (int j = 0; j < K; j++) {
+                    double s = 0.0;
+                    for (int p = 0; p < K; p++) s += W[p * N + a3[dim] * K + j] * in[p];
+                    T[base + j * stride] = s;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    __shared__ double red[256];
+    double s2 = 0.0;
+    for (int e = threadIdx.x; e < K3; e += blockDim.x) {
+        const int x = e % K, y = (e / K) % K, z = e / (K * K);
+        const double v = T[(z * N + y) * N + x];
+        outp[e] = v;
+        s2 += v * v;
+    }
+    red[threadIdx.x] = s2;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) gs.norm[g] = sqrt(red[0]);
+}
+
+// ----------------------------------------------------------------- apply: work list
+
+// Enumerates, for one output node, every (term, delta, ft, gt) candidate inside the
+// screened boxes and keeps those whose bound |w| prod_d ||X_d|| ||f_ft|| exceeds the
+// job threshold.  mode 0 counts per thread (and writes the block-exclusive offsets),
+// mode 1 writes the tuples at gofs[g] + thread offset.
+template <int MODE>
+__global__ void __launch_bounds__(256) k_enum(int nw, const WorkItem *work, const JobTab *jobs,
+                                              const TermTab *tabs, GenState gs, int *thread_off, int *gtotal,
+                                              const long long *gofs, Tuple *out) {
+    const int g = blockIdx.x;
+    const WorkItem wi = work[g];
+    const JobTab &J = jobs[wi.job];
+    __shared__ int scan[256];
+    int cnt = 0;
+    long long base = 0;
+    if (MODE == 1) base = gofs[g] + thread_off[(size_t)g * 256 + threadIdx.x];
+    for (int t = J.tab0; t < J.tab0 + J.ntab; t++) {
+        const TermTab &tb = tabs[t];
+        if (tb.ub[0] < 0) continue;
+        const int ux = tb.ub[0], uy = tb.ub[1], uz = tb.ub[2];
+        const int cells = (2 * ux + 1) * (2 * uy + 1) * (2 * uz + 1);
+        const double aw = fabs(tb.w);
+        const double *nrm = J.operNorms + tb.nrm;
+        for (int c = threadIdx.x; c < cells; c += blockDim.x) {
+            const int dx = c % (2 * ux + 1) - ux;
+            const int dy = (c / (2 * ux + 1)) % (2 * uy + 1) - uy;
+            const int dz = c / ((2 * ux + 1) * (2 * uy + 1)) - uz;
+            const int fx = wi.l[0] - dx, fy = wi.l[1] - dy, fz = wi.l[2] - dz;
+            if (fx < 0 || fy < 0 || fz < 0 || fx >= J.lmax[0] || fy >= J.lmax[1] || fz >= J.lmax[2]) continue;
+            const uint64_t key = dnode_key(J.depth, fx, fy, fz);
+            int f = dhash_find(J.inHash, key);
+            double fn[8];
+            int nft = 8;
+            if (f >= 0) {
+                for (int q = 0; q < 8; q++) fn[q] = J.inNorm[(size_t)f * 8 + q];
+            } else {
+                const int gi = dhash_find(J.genHash, key);
+                if (gi < 0) continue;
+                fn[0] = gs.norm[gi];
+                nft = 1;
+                f = -(gi + 1);
+            }
+            for (int ft = 0; ft < nft; ft++) {
+                const double fv = fn[ft];
+                for (int gt = 0; gt < 8; gt++) {
+                    if (!in_box(tb.box[gt * 8 + ft], dx, dy, dz)) continue;
+                    const int tx = 2 * (gt & 1) + (ft & 1);
+                    const int ty = 2 * ((gt >> 1) & 1) + ((ft >> 1) & 1);
+                    const int tz = 2 * ((gt >> 2) & 1) + ((ft >> 2) & 1);
+                    double on = aw;
+                    on *= nrm[(dx + tb.band) * 4 + tx];
+                    on *= nrm[(dy + tb.band) * 4 + ty];
+                    on *= nrm[(dz + tb.band) * 4 + tz];
+                    const double bound = on * fv;
+                    if (!(bound > J.thr)) continue;
+                    if (MODE == 1) {
+                        Tuple tp;
+                        tp.g = g;
+                        tp.f = f;
+                        tp.term = (uint8_t)(t - J.tab0);
+                        tp.combo = (uint8_t)(gt * 8 + ft);
+                        tp.dx = (int16_t)dx;
+                        tp.dy = (int16_t)dy;
+                        tp.dz = (int16_t)dz;
+                        out[base + cnt] = tp;
+                    }
+                    cnt++;
+                }
+            }
+        }
+    }
+    if (MODE == 0) {
+        scan[threadIdx.x] = cnt;
+        __syncthreads();
+        // inclusive Hillis-Steele scan
+        for (int off = 1; off < 256; off <<= 1) {
+            int v = threadIdx.x >= off ? scan[threadIdx.x - off] : 0;
+            __syncthreads();
+            scan[threadIdx.x] += v;
+            __syncthreads();
+        }
+        thread_off[(size_t)g * 256 + threadIdx.x] = scan[threadIdx.x] - cnt;
+        if (threadIdx.x == 255) gtotal[g] = scan[255];
+    }
+    (void)nw;
+}
+
+// ----------------------------------------------------------------- apply: FP64 contraction
+
+// One block (8 waves) per output node; wave w owns output component gt = w and
+// walks the node's tuples in order, contracting those with that gt:
+//   out_gt[c][b][a] += w * sum X[a][i] Y[b][j] Z[c][l] f_ft[l][j][i]
+// Pass 1 keeps a row of f per lane, pass 2 and 3 exchange through the wave's LDS slice.
+template <int K>
+__global__ void __launch_bounds__(512) k_contract_fp64(int nw, const WorkItem *work, const JobTab *jobs,
+                                                       const TermTab *tabs, const long long *gofs,
+                                                       const int *gtotal, const Tuple *tuples,
+                                                       const double *genPool) {
+    constexpr int K2 = K * K, K3 = K2 * K;
+    constexpr int NP = (K2 + 63) / 64;
+    __shared__ double buf[8][K3];
+    const int g = blockIdx.x;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const WorkItem wi = work[g];
+    const JobTab &J = jobs[wi.job];
+    double *B = buf[wave];
+    double acc[NP][K];
+    for (int p = 0; p < NP; p++)
+        for (int c = 0; c < K; c++) acc[p][c] = 0.0;
+    const long long t0 = gofs[g];
+    const int nt = gtotal[g];
+    for (int it = 0; it < nt; it++) {
+        const Tuple tp = tuples[t0 + it];
+        const int gt = tp.combo >> 3;
+        if (gt != wave) continue;
+        const int ft = tp.combo & 7;
+        const TermTab &tb = tabs[J.tab0 + tp.term];
+        const int tx = 2 * (gt & 1) + (ft & 1);
+        const int ty = 2 * ((gt >> 1) & 1) + ((ft >> 1) & 1);
+        const int tz = 2 * ((gt >> 2) & 1) + ((ft >> 2) & 1);
+        const double *X = J.operBlocks + tb.blk + (size_t)((tp.dx + tb.band) * 4 + tx) * K2;
+        const double *Y = J.operBlocks + tb.blk + (size_t)((tp.dy + tb.band) * 4 + ty) * K2;
+        const double *Z = J.operBlocks + tb.blk + (size_t)((tp.dz + tb.band) * 4 + tz) * K2;
+        const double *f = tp.f >= 0 ? J.inCoef + (size_t)tp.f * 8 * K3 + ft * K3 : genPool + (size_t)(-tp.f - 1) * K3;
+        const double w = tb.w;
+        // pass 1: rows r = l*K + j
+        for (int r = lane; r < K2; r += 64) {
+            double fr[K];
+#pragma unroll
+            for (int i = 0; i < K; i++) fr[i] = f[r * K + i];
+#pragma unroll
+            for (int a = 0; a < K; a++) {
+                double s = 0.0;
+#pragma unroll
+                for (int i = 0; i < K; i++) s += X[a * K + i] * fr[i];
+                B[r * K + a] = s;
+            }
+        }
+        wave_sync();
+        // pass 2: columns (l, a)
+        for (int col = lane; col < K2; col += 64) {
+            const int l = col / K, a = col % K;
+            double tc[K];
+#pragma unroll
+            for (int j = 0; j < K; j++) tc[j] = B[(l * K + j) * K + a];
+#pragma unroll
+            for (int b = 0; b < K; b++) {
+                double s = 0.0;
+#pragma unroll
+                for (int j = 0; j < K; j++) s += Y[b * K + j] * tc[j];
+                B[(l * K + b) * K + a] = s;
+            }
+        }
+        wave_sync();
+        // pass 3: (b, a) pairs accumulate all c
+#pragma unroll
+        for (int p = 0; p < NP; p++) {
+            const int ba = lane + 64 * p;
+            if (ba < K2) {
+                double tc[K];
+#pragma unroll
+                for (int l = 0; l < K; l++) tc[l] = B[l * K2 + ba];
+#pragma unroll
+                for (int c = 0; c < K; c++) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int l = 0; l < K; l++) s += Z[c * K + l] * tc[l];
+                    acc[p][c] += w * s;
+                }
+            }
+        }
+        wave_sync();
+    }
+    double *dst = J.outCoef + (size_t)wi.node * 8 * K3 + wave * K3;
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+        const int ba = lane + 64 * p;
+        if (ba < K2) {
+#pragma unroll
+            for (int c = 0; c < K; c++) dst[c * K2 + ba] = acc[p][c];
+        }
+    }
+    (void)nw;
+}
+
+// norms of the work nodes of all jobs, into their trees and into a compact array
+__global__ void k_work_norms(int nw, const WorkItem *work, const JobTab *jobs, int K3, double *compact) {
+    const int i = blockIdx.x;
+    if (i >= nw) return;
+    const WorkItem wi = work[i];
+    const JobTab &J = jobs[wi.job];
+    const double *p = J.outCoef + (size_t)wi.node * 8 * K3;
+    for (int c = 0; c < 8; c++) {
+        double s = 0.0;
+        for (int e = threadIdx.x; e < K3; e += 64) s += p[c * K3 + e] * p[c * K3 + e];
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+        if (threadIdx.x == 0) {
+            const double v = sqrt(s);
+            J.outNorm[(size_t)wi.node * 8 + c] = v;
+            compact[(size_t)i * 8 + c] = v;
+        }
+    }
+}
+
+// batched two-scale transforms over (tree, parent) items:
+// mode 0 compress children -> parent, mode 1 add reconstruct(parent) to the children
+__global__ void k_transform(int mode, int n, const TransformItem *items, const double *M, int K) {
+    extern __shared__ double smem[];
+    const int N = 2 * K, K3 = K * K * K, N3 = N * N * N;
+    double *T = smem;
+    double *Ms = smem + N3;
+    const TransformItem it = items[blockIdx.x];
+    for (int e = threadIdx.x; e < N * N; e += blockDim.x) Ms[e] = M[e];
+    if (mode == 0) {
+        for (int e = threadIdx.x; e < 8 * K3; e += blockDim.x) {
+            const int a = e / K3, r = e % K3;
+            T[interleaved_index(K, a, r)] = it.coef[(size_t)(it.first_child + a) * 8 * K3 + r];
+        }
+    } else {
+        for (int e = threadIdx.x; e < 8 * K3; e += blockDim.x)
+            T[interleaved_index(K, e / K3, e % K3)] = it.coef[(size_t)it.parent * 8 * K3 + e];
+    }
+    __syncthreads();
+    for (int dim = 0; dim < 3; dim++) {
+        line_pass(T, Ms, N, dim);
+        __syncthreads();
+    }
+    if (mode == 0) {
+        for (int e = threadIdx.x; e < 8 * K3; e += blockDim.x)
+            it.coef[(size_t)it.parent * 8 * K3 + e] = T[interleaved_index(K, e / K3, e % K3)];
+    } else {
+        for (int e = threadIdx.x; e < 8 * K3; e += blockDim.x) {
+            const int a = e / K3, r = e % K3;
+            it.coef[(size_t)(it.first_child + a) * 8 * K3 + r] += T[interleaved_index(K, a, r)];
+        }
+    }
+    (void)n;
+}
+
+// ----------------------------------------------------------------- launchers
+
+#define MWG_DISPATCH_K(K, CALL)                                       \
+    switch (K) {                                                      \
+        case 4: { constexpr int KK = 4; CALL; } break;                \
+        case 5: { constexpr int KK = 5; CALL; } break;                \
+        case 6: { constexpr int KK = 6; CALL; } break;                \
+        case 7: { constexpr int KK = 7; CALL; } break;                \
+        case 8: { constexpr int KK = 8; CALL; } break;                \
+        case 9: { constexpr int KK = 9; CALL; } break;                \
+        case 10: { constexpr int KK = 10; CALL; } break;              \
+        default: return hipErrorInvalidValue;                         \
+    }
+
+hipError_t launch_tblock(hipStream_t s, int n, const int *item_term, const long long *item_delta, const double *b,
+                         double pre, const DevBasis &B, const double *gx, const double *gw, double *out) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tblock, dim3(n), dim3(128), 0, s, n, item_term, item_delta, b, pre, B, gx, gw, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_nsblocks(hipStream_t s, int n, const long long *item_delta, const int *item_row0,
+                           const int *item_tlo, const int *item_thi, const double *t1, const double *W, int K,
+                           double *blocks, double *norms) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_nsblocks, dim3(n), dim3(256), 0, s, n, item_delta, item_row0, item_tlo, item_thi, t1, W, K,
+                       blocks, norms);
+    return hipGetLastError();
+}
+
+static size_t tensor_smem(int K) { return (size_t)(8 * K * K * K + 4 * K * K) * sizeof(double); }
+
+hipError_t launch_norms(hipStream_t s, int n, const int *nodes, const double *coef, double *norm, int K3) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_norms, dim3(n), dim3(64), 0, s, n, nodes, coef, norm, K3);
+    return hipGetLastError();
+}
+
+hipError_t launch_dot(hipStream_t s, int n, const int *ia, const int *ib, const int *c0, const double *A,
+                      const double *B, int K3, double *part) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_dot, dim3(n), dim3(64), 0, s, n, ia, ib, c0, A, B, K3, part);
+    return hipGetLastError();
+}
+
+hipError_t launch_scale(hipStream_t s, size_t n, double *x, double c) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scale, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, x, c);
+    return hipGetLastError();
+}
+
+hipError_t launch_project_gauss(hipStream_t s, int nw, const int *work, const int *depth, const int *lr,
+                                const ProjGeom &geo, const DevBasis &B, int nterm, const double *coef,
+                                const double *alpha, const double *center, const double *poly, double *out) {
+    if (nw <= 0) return hipSuccess;
+    MWG_DISPATCH_K(B.K, hipLaunchKernelGGL(k_project_gauss<KK>, dim3(nw), dim3(256), 0, s, nw, work, depth, lr, geo,
+                                           B, nterm, coef, alpha, center, poly, out));
+    return hipGetLastError();
+}
+
+hipError_t launch_project_hyd(hipStream_t s, int nw, const int *work, const int *depth, const int *lr,
+                              const ProjGeom &geo, const DevBasis &B, int nterm, const double *coef, const int *q,
+                              double Z, const double *origin, double *out) {
+    if (nw <= 0) return hipSuccess;
+    const int K = B.K;
+    const size_t sm = (size_t)(8 * K * K * K + 8 * K * K) * sizeof(double);
+    if (sm > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_project_hyd, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)sm);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_project_hyd, dim3(nw), dim3(256), sm, s, nw, work, depth, lr, geo, B, nterm, coef, q, Z,
+                       origin, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_gen_mark(hipStream_t s, int nw, const WorkItem *work, const JobTab *jobs, const TermTab *tabs,
+                           const GenState &gs) {
+    if (nw <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gen_mark, dim3(nw), dim3(256), 0, s, nw, work, jobs, tabs, gs);
+    return hipGetLastError();
+}
+
+hipError_t launch_gen_compute(hipStream_t s, int first, int n, const JobTab *jobs, const GenState &gs,
+                              const double *W, int K) {
+    if (n <= 0) return hipSuccess;
+    const size_t sm = (size_t)(8 * K * K * K) * sizeof(double);
+    hipLaunchKernelGGL(k_gen_compute, dim3(n), dim3(256), sm, s, first, n, jobs, gs, W, K);
+    return hipGetLastError();
+}
+
+hipError_t launch_enum(hipStream_t s, int mode, int nw, const WorkItem *work, const JobTab *jobs,
+                       const TermTab *tabs, const GenState &gs, int *thread_off, int *gtotal, const long long *gofs,
+                       Tuple *out) {
+    if (nw <= 0) return hipSuccess;
+    if (mode == 0)
+        hipLaunchKernelGGL(k_enum<0>, dim3(nw), dim3(256), 0, s, nw, work, jobs, tabs, gs, thread_off, gtotal, gofs,
+                           out);
+    else
+        hipLaunchKernelGGL(k_enum<1>, dim3(nw), dim3(256), 0, s, nw, work, jobs, tabs, gs, thread_off, gtotal, gofs,
+                           out);
+    return hipGetLastError();
+}
+
+hipError_t launch_work_norms(hipStream_t s, int nw, const WorkItem *work, const JobTab *jobs, int K3,
+                             double *compact) {
+    if (nw <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_work_norms, dim3(nw), dim3(64), 0, s, nw, work, jobs, K3, compact);
+    return hipGetLastError();
+}
+
+hipError_t launch_transform(hipStream_t s, int mode, int n, const TransformItem *items, const double *M, int K) {
+    if (n <= 0) return hipSuccess;
+    const size_t sm = tensor_smem(K);
+    if (sm > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_transform, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)sm);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_transform, dim3(n), dim3(256), sm, s, mode, n, items, M, K);
+    return hipGetLastError();
+}
+
+hipError_t launch_contract_fp64(hipStream_t s, int K, int nw, const WorkItem *work, const JobTab *jobs,
+                                const TermTab *tabs, const long long *gofs, const int *gtotal, const Tuple *tuples,
+                                const double *genPool) {
+    if (nw <= 0) return hipSuccess;
+    MWG_DISPATCH_K(K, hipLaunchKernelGGL(k_contract_fp64<KK>, dim3(nw), dim3(512), 0, s, nw, work, jobs, tabs, gofs,
+                                         gtotal, tuples, genPool));
+    return hipGetLastError();
+}
+
+}  // namespace mwg

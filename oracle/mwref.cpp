@@ -1094,6 +1094,8 @@ struct Box {
 const double SCREEN_REL = 0.1;
 // multiplies the tuple screening threshold (1 = the apply's rule; 0 = no screening)
 double g_screen_factor = 1.0;
+// debug histogram: number of surviving tuples per (g, f, term) group
+std::atomic<int64_t> g_hist[65];
 
 void apply(double prec, Tree &out, Oper &O, const Tree &inp, int maxIter, ApplyStats *st) {
     auto t0 = std::chrono::steady_clock::now();
@@ -1211,6 +1213,7 @@ void apply(double prec, Tree &out, Oper &O, const Tree &inp, int maxIter, ApplyS
                                     fn[0] = -1.0;  // unknown until generated
                                 }
                                 int dl[3] = {dx, dy, dz};
+                                int64_t grp_before = mytup;
                                 for (int ft = 0; ft < 8; ft++) {
                                     if (gen && ft > 0) break;
                                     for (int gt = 0; gt < 8; gt++) {
@@ -1238,6 +1241,7 @@ void apply(double prec, Tree &out, Oper &O, const Tree &inp, int maxIter, ApplyS
                                         mytup++;
                                     }
                                 }
+                                g_hist[mytup - grp_before]++;
                             }
                 }
                 std::memcpy(out.c(g), acc.data(), 8 * K3 * sizeof(double));
@@ -1486,6 +1490,12 @@ int mwref_apply(double prec, void *out, void *oper, void *inp, int max_iter, mwr
 
 void mwref_set_threads(int n) { omp_set_num_threads(n); }
 void mwref_set_screen_factor(double f) { g_screen_factor = f; }
+void mwref_hist(int64_t *h) {
+    for (int i = 0; i < 65; i++) {
+        h[i] = g_hist[i].load();
+        g_hist[i] = 0;
+    }
+}
 int mwref_get_threads() { return omp_get_max_threads(); }
 
 }  // extern "C"
