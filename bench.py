@@ -1,0 +1,160 @@
+#!/usr/bin/env python3
+"""Benchmark: output nodes/s of the Poisson/Helmholtz apply at prec 1e-6 on one MI355X.
+
+Workload (one GPU): Helmholtz apply (mu ~ U[0.5,2]) to 5 synthetic orbitals, each a
+sum of 6 s/p Gaussians on 3 seeded nuclei within 2 bohr (exponents log-U[0.3,30],
+coefficients N(0,1), Gram-Schmidt orthonormalised), projected at k=7, prec 1e-6 in
+the box [-20,20]^3.  One step = the 5 applies as one batch, inputs resident in HBM,
+operators built beforehand (their construction time is reported separately).
+
+Prints one JSON line.  ``value`` is output nodes/s of the fastest variant; the
+plain FP64 GPU apply's rate is reported beside it.  ``cpu_baseline`` times the FP64
+CPU oracle (OpenMP) on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"f64": 78.6e12, "f32": 157.3e12}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_inputs(G, configs, k, prec, n_orb, seed):
+    mra = G.mra_from(configs.BOX20, k, "L", max_depth=16)
+    orbs, nuc, mus = configs.orbitals(seed, n_orb=n_orb)
+    t0 = time.perf_counter()
+    trees = [G.project(prec, G.FunctionTree(mra), o) for o in orbs]
+    t_proj = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ops = [G.HelmholtzOperator(mra, mu, prec) for mu in mus]
+    t_oper = time.perf_counter() - t0
+    return mra, orbs, mus, trees, ops, t_proj, t_oper
+
+
+def run_variant(G, mra, trees, ops, prec, flags, steps, warmup):
+    n = len(trees)
+    for _ in range(warmup):
+        outs = [G.FunctionTree(mra) for _ in range(n)]
+        G.apply_batch(prec, outs, ops, trees, flags=flags)
+    mra.ctx.sync()
+    nodes = 0
+    flops = 0.0
+    t_con = 0.0
+    tuples = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        outs = [G.FunctionTree(mra) for _ in range(n)]
+        st = G.apply_batch(prec, outs, ops, trees, flags=flags)
+        nodes += sum(s["nodes_out"] for s in st)
+        flops += sum(s["flops"] for s in st)
+        tuples += sum(s["tuples"] for s in st)
+        t_con += st[0]["t_contract"]
+    mra.ctx.sync()
+    dt = time.perf_counter() - t0
+    return dict(ms_per_step=1e3 * dt / steps, nodes_per_step=nodes / steps, nodes_per_s=nodes / dt,
+                flops_per_step=flops / steps, tuples_per_step=tuples / steps, t_contract_per_step=t_con / steps,
+                outs=outs)
+
+
+def cpu_baseline(orbs, mus, k, prec, budget_s=15.0):
+    from mrchem_amd import configs
+    from oracle import mwref as M
+    threads = M.get_threads()
+    mra = M.MRA(**configs.BOX20, k=k, basis="L", max_depth=16)
+    nodes = 0
+    t_apply = 0.0
+    done = 0
+    for i in range(len(orbs)):
+        f = M.Tree(mra).project_gauss(prec, orbs[i])
+        H = M.helmholtz(mra, mus[i], prec)
+        out = M.Tree(mra)
+        st = M.apply(prec, out, H, f)
+        nodes += st["nodes_out"]
+        t_apply += st["t_total"]
+        done += 1
+        if t_apply > budget_s:
+            break
+    return dict(value=nodes / t_apply, unit="output nodes/s", cores=threads, kind="port",
+                sample=f"FP64 OpenMP oracle (oracle/libmwref.so), Helmholtz apply of {done} of {len(orbs)} "
+                       f"orbitals of the same workload, {t_apply:.2f} s of apply time")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--k", type=int, default=7)
+    ap.add_argument("--prec", type=float, default=1e-6)
+    ap.add_argument("--orbitals", type=int, default=5)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(the batched applies would shard by orbital/pair across GPUs)", file=sys.stderr)
+        sys.exit(2)
+
+    from mrchem_amd import configs
+    from mrchem_amd import mrcpp as G
+
+    log(f"[bench] building inputs: {args.orbitals} orbitals, k={args.k}, prec={args.prec}")
+    mra, orbs, mus, trees, ops, t_proj, t_oper = build_inputs(G, configs, args.k, args.prec, args.orbitals,
+                                                              args.seed)
+    log(f"[bench] input nodes {[t.getNNodes() for t in trees]}, projection {t_proj:.3f} s, operators {t_oper:.3f} s")
+
+    res64 = run_variant(G, mra, trees, ops, args.prec, G.MWG_APPLY_FP64, args.steps, args.warmup)
+    log(f"[bench] fp64: {res64['ms_per_step']:.2f} ms/step, {res64['nodes_per_s']:.1f} nodes/s, "
+        f"contraction {1e3 * res64['t_contract_per_step']:.2f} ms/step")
+    best, dtype = res64, "f64"
+
+    achieved = best["flops_per_step"] / best["t_contract_per_step"] if best["t_contract_per_step"] > 0 else 0.0
+    roofline = dict(bound="mfma", achieved=achieved / 1e12, peak=PEAK[dtype] / 1e12, unit="TFLOP/s",
+                    frac=achieved / PEAK[dtype], traffic=None)
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU oracle baseline ...")
+        cpu = cpu_baseline(orbs, mus, args.k, args.prec)
+        log(f"[bench] cpu: {cpu['value']:.1f} nodes/s on {cpu['cores']} threads")
+
+    line = {
+        "metric": "output nodes/s for Poisson/Helmholtz apply at prec 1e-6 vs plain FP64 GPU",
+        "value": best["nodes_per_s"],
+        "unit": "output nodes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": best["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic (seeded Gaussian orbitals)",
+        "config": {
+            "workload": f"Helmholtz apply, {args.orbitals} orbitals, k={args.k}, prec={args.prec}, box [-20,20]^3",
+            "orbitals": args.orbitals, "k": args.k, "prec": args.prec, "seed": args.seed,
+            "output_nodes_per_step": best["nodes_per_step"], "tuples_per_step": best["tuples_per_step"],
+            "fp64_nodes_per_s": res64["nodes_per_s"], "speedup_vs_fp64": best["nodes_per_s"] / res64["nodes_per_s"],
+            "operator_build_s": t_oper, "projection_s": t_proj,
+            "contraction_ms_per_step": 1e3 * best["t_contract_per_step"],
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -59,10 +59,11 @@ typedef struct mwg_apply_stats {
     int64_t tuples_fp64;     /* of which contracted in FP64 (mixed mode routes large bounds to FP64) */
     int64_t gen_nodes;       /* generated (scaling-only) input nodes */
     int64_t iters;           /* refinement iterations */
+    int64_t slices;          /* FP32 MFMA slices (g, f, ft, term: all 8 gt at once), mixed mode */
     int64_t predicted_splits;/* nodes split ahead of the norm test by the predictor */
     double t_apply;          /* seconds: work-list build + contraction + refine + transforms */
     double t_contract;       /* seconds spent in contraction kernels (HIP events) */
-    double flops;            /* algorithmic flops: tuples * 6 (k+1)^4 */
+    double flops;            /* flops issued: FP64 tuples * 6 (k+1)^4 + slices * 2*2(k+1)*(k+1)*7(k+1)^2 */
 } mwg_apply_stats;
 
 /* ---- context (mrenv / device) ------------------------------------------------ */

@@ -20,7 +20,8 @@ MWG_APPLY_PREDICT = 2
 
 class ApplyStats(C.Structure):
     _fields_ = [("nodes_out", C.c_int64), ("tuples", C.c_int64), ("tuples_fp64", C.c_int64),
-                ("gen_nodes", C.c_int64), ("iters", C.c_int64), ("predicted_splits", C.c_int64),
+                ("gen_nodes", C.c_int64), ("iters", C.c_int64), ("slices", C.c_int64),
+                ("predicted_splits", C.c_int64),
                 ("t_apply", C.c_double), ("t_contract", C.c_double), ("flops", C.c_double)]
 
     def as_dict(self):

@@ -103,6 +103,13 @@ struct Ctx {
     DBuf<unsigned long long> genHashKeys;
     DBuf<int> genHashVals;
     DBuf<TransformItem> titems;
+    DBuf<ContractTask> tasks;
+    DBuf<ReduceItem> reds;
+    DBuf<double> partials, nodeslots;
+    DBuf<Slice> slices;
+    DBuf<SliceTask> stasks;
+    int chunk = 64;        // tuples per FP64 contraction task
+    int slice_chunk = 16;  // slices per MFMA task
 };
 
 struct MraH {
@@ -232,6 +239,8 @@ struct TermLevel {
 struct LevelH {
     std::vector<TermLevel> t;
     DBuf<double> blocks, norms;
+    DBuf<float> blocks32;  // FP32 copy for the mixed-precision contraction (made on first use)
+    bool has32 = false;
 };
 
 struct OperH {
@@ -487,7 +496,7 @@ struct JobH {
     std::vector<std::array<double, 8>> fmax;
     double sNorm = 0.0, wNorm = 0.0, gsq = -1.0;
     std::vector<int> work;
-    int64_t tuples = 0;
+    int64_t tuples = 0, slices = 0;
     size_t genOff = 0;
     uint32_t genMask = 0;
 };
@@ -565,7 +574,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
     Ctx &C = *M.ctx;
     hipStream_t s = C.stream;
     const int K = M.B.K, K3 = M.B.K3;
-    (void)flags;
+    const bool mixed = (flags & MWG_APPLY_MIXED) && K == 8;
     auto t_start = std::chrono::steady_clock::now();
     float t_contract_ms = 0.0f;
     // inputs: device hash, norm maxima per depth
@@ -636,6 +645,15 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             T.inCoef = J.in->coef.p;
             T.inNorm = J.in->norm.p;
             T.operBlocks = J.op->lev[d]->blocks.p;
+            if (mixed) {
+                LevelH &LV = *J.op->lev[d];
+                if (!LV.has32) {
+                    LV.blocks32.alloc(LV.blocks.n);
+                    HC(launch_to_f32(s, LV.blocks.n, LV.blocks.p, LV.blocks32.p));
+                    LV.has32 = true;
+                }
+                T.operBlocks32 = LV.blocks32.p;
+            }
             T.operNorms = J.op->lev[d]->norms.p;
             T.outCoef = J.out->coef.p;
             T.outNorm = J.out->norm.p;
@@ -670,28 +688,107 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             HC(launch_gen_compute(s, genDone, genNow - genDone, C.jobs.p, gs, M.W.p, K));
             genDone = genNow;
         }
-        // work list
-        C.thread_off.alloc((size_t)nw * 256);
-        C.gtotal.alloc(nw);
-        HC(launch_enum(s, 0, nw, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, nullptr, nullptr));
-        std::vector<int> tot(nw);
-        C.gtotal.download(tot.data(), nw, s);
-        std::vector<long long> ofs(nw);
-        long long total = 0;
-        for (int i = 0; i < nw; i++) {
-            ofs[i] = total;
-            total += tot[i];
-            jobs[work[i].job].tuples += tot[i];
+        if (!mixed) {
+            // work list: tuples grouped by segment (g, gt)
+            C.thread_off.alloc((size_t)nw * 8 * 256);
+            C.gtotal.alloc((size_t)nw * 8);
+            HC(launch_enum(s, 0, nw, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, nullptr, nullptr));
+            std::vector<int> tot((size_t)nw * 8);
+            C.gtotal.download(tot.data(), tot.size(), s);
+            std::vector<long long> ofs((size_t)nw * 8);
+            std::vector<ContractTask> tasks;
+            std::vector<ReduceItem> reds;
+            long long total = 0;
+            int nslots = 0;
+            for (int i = 0; i < nw; i++)
+                for (int q = 0; q < 8; q++) {
+                    const int c = tot[(size_t)i * 8 + q];
+                    ofs[(size_t)i * 8 + q] = total;
+                    jobs[work[i].job].tuples += c;
+                    const int nch = c <= C.chunk ? 1 : (c + C.chunk - 1) / C.chunk;
+                    if (nch == 1) {
+                        tasks.push_back({total, c, i, q, -1});
+                    } else {
+                        reds.push_back({i, q, nslots, nch, 0, 0});
+                        for (int h = 0; h < nch; h++) {
+                            const int cc = std::min(C.chunk, c - h * C.chunk);
+                            tasks.push_back({total + (long long)h * C.chunk, cc, i, q, nslots++});
+                        }
+                    }
+                    total += c;
+                }
+            C.gofs.upload(ofs.data(), ofs.size(), s);
+            C.tuples.alloc((size_t)std::max<long long>(total, 1));
+            HC(launch_enum(s, 1, nw, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, C.gofs.p,
+                           C.tuples.p));
+            std::stable_sort(tasks.begin(), tasks.end(),
+                             [](const ContractTask &a, const ContractTask &b) { return a.count > b.count; });
+            C.tasks.upload(tasks.data(), tasks.size(), s);
+            if (!reds.empty()) C.reds.upload(reds.data(), reds.size(), s);
+            C.partials.alloc((size_t)std::max(nslots, 1) * K3);
+            HC(hipEventRecord(C.ev0, s));
+            HC(launch_contract_fp64(s, K, (int)tasks.size(), C.tasks.p, C.work.p, C.jobs.p, C.tabs.p, C.tuples.p,
+                                    C.genPool.p, C.partials.p));
+            HC(launch_reduce_partials(s, (int)reds.size(), C.reds.p, C.work.p, C.jobs.p, C.partials.p, nullptr, K3));
+            HC(hipEventRecord(C.ev1, s));
+        } else {
+            // mixed: FP64 tuples for the large bounds, FP32 MFMA slices for the rest
+            C.thread_off.alloc((size_t)nw * 9 * 256);
+            C.gtotal.alloc((size_t)nw * 9);
+            HC(launch_enum_mixed(s, 0, nw, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, nullptr,
+                                 nullptr, nullptr));
+            std::vector<int> tot((size_t)nw * 9);
+            C.gtotal.download(tot.data(), tot.size(), s);
+            std::vector<long long> ofs((size_t)nw * 9);
+            std::vector<ContractTask> tasks;
+            std::vector<SliceTask> stasks;
+            std::vector<ReduceItem> reds;
+            long long total = 0, stotal = 0;
+            int nslots = 0, nnslots = 0;
+            for (int i = 0; i < nw; i++) {
+                const int cs = tot[(size_t)i * 9 + 8];
+                ofs[(size_t)i * 9 + 8] = stotal;
+                jobs[work[i].job].slices += cs;
+                const int nn0 = nnslots;
+                for (int h = 0; h * C.slice_chunk < cs; h++) {
+                    const int cc = std::min(C.slice_chunk, cs - h * C.slice_chunk);
+                    stasks.push_back({stotal + (long long)h * C.slice_chunk, cc, i, nnslots++});
+                }
+                stotal += cs;
+                for (int q = 0; q < 8; q++) {
+                    const int c = tot[(size_t)i * 9 + q];
+                    ofs[(size_t)i * 9 + q] = total;
+                    jobs[work[i].job].tuples += c;
+                    const int s0 = nslots;
+                    for (int h = 0; h * C.chunk < c; h++) {
+                        const int cc = std::min(C.chunk, c - h * C.chunk);
+                        tasks.push_back({total + (long long)h * C.chunk, cc, i, q, nslots++});
+                    }
+                    reds.push_back({i, q, s0, nslots - s0, nn0, nnslots - nn0});
+                    total += c;
+                }
+            }
+            C.gofs.upload(ofs.data(), ofs.size(), s);
+            C.tuples.alloc((size_t)std::max<long long>(total, 1));
+            C.slices.alloc((size_t)std::max<long long>(stotal, 1));
+            HC(launch_enum_mixed(s, 1, nw, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, C.gofs.p,
+                                 C.tuples.p, C.slices.p));
+            std::stable_sort(tasks.begin(), tasks.end(),
+                             [](const ContractTask &a, const ContractTask &b) { return a.count > b.count; });
+            if (!tasks.empty()) C.tasks.upload(tasks.data(), tasks.size(), s);
+            if (!stasks.empty()) C.stasks.upload(stasks.data(), stasks.size(), s);
+            C.reds.upload(reds.data(), reds.size(), s);
+            C.partials.alloc((size_t)std::max(nslots, 1) * K3);
+            C.nodeslots.alloc((size_t)std::max(nnslots, 1) * 8 * K3);
+            HC(hipEventRecord(C.ev0, s));
+            HC(launch_contract_fp64(s, K, (int)tasks.size(), C.tasks.p, C.work.p, C.jobs.p, C.tabs.p, C.tuples.p,
+                                    C.genPool.p, C.partials.p));
+            HC(launch_contract_mfma(s, K, (int)stasks.size(), C.stasks.p, C.work.p, C.jobs.p, C.tabs.p, C.slices.p,
+                                    C.genPool.p, C.nodeslots.p));
+            HC(launch_reduce_partials(s, (int)reds.size(), C.reds.p, C.work.p, C.jobs.p, C.partials.p, C.nodeslots.p,
+                                      K3));
+            HC(hipEventRecord(C.ev1, s));
         }
-        C.gofs.upload(ofs.data(), nw, s);
-        C.tuples.alloc((size_t)std::max<long long>(total, 1));
-        HC(launch_enum(s, 1, nw, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, C.gofs.p,
-                       C.tuples.p));
-        // contraction
-        HC(hipEventRecord(C.ev0, s));
-        HC(launch_contract_fp64(s, K, nw, C.work.p, C.jobs.p, C.tabs.p, C.gofs.p, C.gtotal.p, C.tuples.p,
-                                C.genPool.p));
-        HC(hipEventRecord(C.ev1, s));
         // norms and refinement
         C.wnorms.alloc((size_t)nw * 8);
         HC(launch_work_norms(s, nw, C.work.p, C.jobs.p, K3, C.wnorms.p));
@@ -764,12 +861,13 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         st.nodes_out = jobs[j].out->n();
         st.tuples = jobs[j].tuples;
         st.tuples_fp64 = jobs[j].tuples;
+        st.slices = jobs[j].slices;
         st.gen_nodes = genDone;
         st.iters = iter;
         st.predicted_splits = 0;
         st.t_apply = t_total;
         st.t_contract = t_contract_ms * 1e-3;
-        st.flops = fl * (double)jobs[j].tuples;
+        st.flops = fl * (double)jobs[j].tuples + 2.0 * (2 * K) * K * 7.0 * K * K * (double)jobs[j].slices;
     }
 }
 
@@ -1192,7 +1290,7 @@ int mwg_apply_batch(int n, const double *prec, mwg_tree *out, mwg_oper *oper, mw
     Ctx *C = inp[0]->mra->ctx;
     return guarded(C, [&] {
         if (flags & ~(MWG_APPLY_MIXED | MWG_APPLY_PREDICT)) throw Error(MWG_ERR_ARG, "unknown apply flags");
-        if (flags != MWG_APPLY_FP64) throw Error(MWG_ERR_ARG, "only the FP64 apply is available in this build");
+        if (flags & MWG_APPLY_PREDICT) throw Error(MWG_ERR_ARG, "the refinement predictor is not available");
         std::vector<JobH> jobs(n);
         for (int i = 0; i < n; i++) {
             if (!(prec[i] > 0.0)) throw Error(MWG_ERR_ARG, "apply precision must be positive");

@@ -59,4 +59,23 @@ struct Tuple {
 };
 static_assert(sizeof(Tuple) == 16, "tuple record is 16 bytes");
 
+// one (g, f, ft, term) slice of the mixed-precision apply: all 8 output
+// components at once (mask: which gt are accumulated)
+struct Slice {
+    int g;
+    int f;
+    uint8_t term;
+    uint8_t ft;
+    uint8_t mask;
+    uint8_t pad;
+    int16_t dx, dy, dz;
+    int16_t pad2;
+};
+static_assert(sizeof(Slice) == 20, "slice record is 20 bytes");
+
+// a slice is contracted in FP64 (as tuples) when its largest bound exceeds
+// ROUTE_FP64 times the screening threshold: the FP32 rounding of such a slice
+// (~8 * 2^-24 of its bound) would exceed the screening error
+constexpr double ROUTE_FP64 = 131072.0;
+
 }  // namespace mwg

@@ -37,6 +37,7 @@ struct JobTab {
     const double *inCoef;
     const double *inNorm;
     const double *operBlocks;  // this depth's block array of the job's operator
+    const float *operBlocks32; // FP32 copy (mixed precision)
     const double *operNorms;   // this depth's norm array
     double *outCoef;
     double *outNorm;
@@ -76,8 +77,8 @@ hipError_t launch_gen_mark(hipStream_t s, int nw, const WorkItem *work, const Jo
 hipError_t launch_gen_compute(hipStream_t s, int first, int n, const JobTab *jobs, const GenState &gs,
                               const double *W, int K);
 hipError_t launch_enum(hipStream_t s, int mode, int nw, const WorkItem *work, const JobTab *jobs,
-                       const TermTab *tabs, const GenState &gs, int *thread_off, int *gtotal, const long long *gofs,
-                       Tuple *out);
+                       const TermTab *tabs, const GenState &gs, int *thread_off, int *seg_total,
+                       const long long *seg_ofs, Tuple *out);
 hipError_t launch_work_norms(hipStream_t s, int nw, const WorkItem *work, const JobTab *jobs, int K3,
                              double *compact);
 struct TransformItem {
@@ -86,8 +87,35 @@ struct TransformItem {
     int first_child;
 };
 hipError_t launch_transform(hipStream_t s, int mode, int n, const TransformItem *items, const double *M, int K);
-hipError_t launch_contract_fp64(hipStream_t s, int K, int nw, const WorkItem *work, const JobTab *jobs,
-                                const TermTab *tabs, const long long *gofs, const int *gtotal, const Tuple *tuples,
-                                const double *genPool);
+// one wavefront task: a chunk of the tuples of segment (g, gt)
+struct ContractTask {
+    long long start;
+    int count;
+    int g;     // work index
+    int gt;
+    int slot;  // partial slot, -1 = write the output node directly
+};
+struct ReduceItem {
+    int g, gt, slot0, nslots;  // K^3 slots of FP64 tuple tasks
+    int nslot0, nnslots;       // 8 K^3 node slots of mixed-precision slice tasks
+};
+struct SliceTask {
+    long long start;
+    int count;
+    int g;
+    int slot;  // node slot (8 K^3 doubles)
+};
+hipError_t launch_enum_mixed(hipStream_t s, int mode, int nw, const WorkItem *work, const JobTab *jobs,
+                             const TermTab *tabs, const GenState &gs, int *thread_off, int *seg_total,
+                             const long long *seg_ofs, Tuple *out, Slice *sout);
+hipError_t launch_contract_mfma(hipStream_t s, int K, int ntask, const SliceTask *tasks, const WorkItem *work,
+                                const JobTab *jobs, const TermTab *tabs, const Slice *slices, const double *genPool,
+                                double *nodeslots);
+hipError_t launch_to_f32(hipStream_t s, size_t n, const double *in, float *out);
+hipError_t launch_contract_fp64(hipStream_t s, int K, int ntask, const ContractTask *tasks, const WorkItem *work,
+                                const JobTab *jobs, const TermTab *tabs, const Tuple *tuples, const double *genPool,
+                                double *partials);
+hipError_t launch_reduce_partials(hipStream_t s, int n, const ReduceItem *items, const WorkItem *work,
+                                  const JobTab *jobs, const double *partials, const double *nodeslots, int K3);
 
 }  // namespace mwg

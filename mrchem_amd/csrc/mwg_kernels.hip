@@ -543,19 +543,22 @@ __global__ void k_gen_compute(int first, int n, const JobTab *jobs, GenState gs,
 
 // Enumerates, for one output node, every (term, delta, ft, gt) candidate inside the
 // screened boxes and keeps those whose bound |w| prod_d ||X_d|| ||f_ft|| exceeds the
-// job threshold.  mode 0 counts per thread (and writes the block-exclusive offsets),
-// mode 1 writes the tuples at gofs[g] + thread offset.
+// job threshold.  Tuples are laid out by segment (g, gt):
+//   mode 0 counts per (thread, gt), scans over the block and writes per-thread
+//          offsets inside each segment and the segment totals;
+//   mode 1 writes every tuple at seg_ofs[g*8+gt] + thread offset + running index.
 template <int MODE>
 __global__ void __launch_bounds__(256) k_enum(int nw, const WorkItem *work, const JobTab *jobs,
-                                              const TermTab *tabs, GenState gs, int *thread_off, int *gtotal,
-                                              const long long *gofs, Tuple *out) {
+                                              const TermTab *tabs, GenState gs, int *thread_off, int *seg_total,
+                                              const long long *seg_ofs, Tuple *out) {
     const int g = blockIdx.x;
     const WorkItem wi = work[g];
     const JobTab &J = jobs[wi.job];
-    __shared__ int scan[256];
-    int cnt = 0;
-    long long base = 0;
-    if (MODE == 1) base = gofs[g] + thread_off[(size_t)g * 256 + threadIdx.x];
+    __shared__ int scan[8][256];
+    int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    long long base[8];
+    if (MODE == 1)
+        for (int q = 0; q < 8; q++) base[q] = seg_ofs[(size_t)g * 8 + q] + thread_off[((size_t)g * 8 + q) * 256 + threadIdx.x];
     for (int t = J.tab0; t < J.tab0 + J.ntab; t++) {
         const TermTab &tb = tabs[t];
         if (tb.ub[0] < 0) continue;
@@ -604,57 +607,60 @@ __global__ void __launch_bounds__(256) k_enum(int nw, const WorkItem *work, cons
                         tp.dx = (int16_t)dx;
                         tp.dy = (int16_t)dy;
                         tp.dz = (int16_t)dz;
-                        out[base + cnt] = tp;
+                        out[base[gt] + cnt[gt]] = tp;
                     }
-                    cnt++;
+                    cnt[gt]++;
                 }
             }
         }
     }
     if (MODE == 0) {
-        scan[threadIdx.x] = cnt;
+        for (int q = 0; q < 8; q++) scan[q][threadIdx.x] = cnt[q];
         __syncthreads();
-        // inclusive Hillis-Steele scan
         for (int off = 1; off < 256; off <<= 1) {
-            int v = threadIdx.x >= off ? scan[threadIdx.x - off] : 0;
+            int v[8];
+            for (int q = 0; q < 8; q++) v[q] = threadIdx.x >= off ? scan[q][threadIdx.x - off] : 0;
             __syncthreads();
-            scan[threadIdx.x] += v;
+            for (int q = 0; q < 8; q++) scan[q][threadIdx.x] += v[q];
             __syncthreads();
         }
-        thread_off[(size_t)g * 256 + threadIdx.x] = scan[threadIdx.x] - cnt;
-        if (threadIdx.x == 255) gtotal[g] = scan[255];
+        for (int q = 0; q < 8; q++) thread_off[((size_t)g * 8 + q) * 256 + threadIdx.x] = scan[q][threadIdx.x] - cnt[q];
+        if (threadIdx.x == 255)
+            for (int q = 0; q < 8; q++) seg_total[(size_t)g * 8 + q] = scan[q][255];
     }
     (void)nw;
 }
 
 // ----------------------------------------------------------------- apply: FP64 contraction
 
-// One block (8 waves) per output node; wave w owns output component gt = w and
-// walks the node's tuples in order, contracting those with that gt:
+// One wavefront per task = a chunk of the tuples of one segment (g, gt):
 //   out_gt[c][b][a] += w * sum X[a][i] Y[b][j] Z[c][l] f_ft[l][j][i]
-// Pass 1 keeps a row of f per lane, pass 2 and 3 exchange through the wave's LDS slice.
+// Pass 1 keeps a row of f per lane (X by scalar loads), passes 2 and 3 exchange
+// through the wave's LDS slice.  The result goes to the output node when the
+// segment has one chunk, else to a partial slot summed by k_reduce_partials.
 template <int K>
-__global__ void __launch_bounds__(512) k_contract_fp64(int nw, const WorkItem *work, const JobTab *jobs,
-                                                       const TermTab *tabs, const long long *gofs,
-                                                       const int *gtotal, const Tuple *tuples,
-                                                       const double *genPool) {
+__global__ void __launch_bounds__(256) k_contract_fp64(int ntask, const ContractTask *tasks, const WorkItem *work,
+                                                       const JobTab *jobs, const TermTab *tabs,
+                                                       const Tuple *tuples, const double *genPool,
+                                                       double *partials) {
     constexpr int K2 = K * K, K3 = K2 * K;
     constexpr int NP = (K2 + 63) / 64;
-    __shared__ double buf[8][K3];
-    const int g = blockIdx.x;
+    __shared__ double buf[4][K3];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const WorkItem wi = work[g];
+    const int ti = blockIdx.x * 4 + wave;
+    if (ti >= ntask) return;
+    const ContractTask task = tasks[ti];
+    const WorkItem wi = work[task.g];
     const JobTab &J = jobs[wi.job];
+    const int gt = task.gt;
     double *B = buf[wave];
     double acc[NP][K];
+#pragma unroll
     for (int p = 0; p < NP; p++)
+#pragma unroll
         for (int c = 0; c < K; c++) acc[p][c] = 0.0;
-    const long long t0 = gofs[g];
-    const int nt = gtotal[g];
-    for (int it = 0; it < nt; it++) {
-        const Tuple tp = tuples[t0 + it];
-        const int gt = tp.combo >> 3;
-        if (gt != wave) continue;
+    for (int it = 0; it < task.count; it++) {
+        const Tuple tp = tuples[task.start + it];
         const int ft = tp.combo & 7;
         const TermTab &tb = tabs[J.tab0 + tp.term];
         const int tx = 2 * (gt & 1) + (ft & 1);
@@ -665,7 +671,6 @@ __global__ void __launch_bounds__(512) k_contract_fp64(int nw, const WorkItem *w
         const double *Z = J.operBlocks + tb.blk + (size_t)((tp.dz + tb.band) * 4 + tz) * K2;
         const double *f = tp.f >= 0 ? J.inCoef + (size_t)tp.f * 8 * K3 + ft * K3 : genPool + (size_t)(-tp.f - 1) * K3;
         const double w = tb.w;
-        // pass 1: rows r = l*K + j
         for (int r = lane; r < K2; r += 64) {
             double fr[K];
 #pragma unroll
@@ -679,7 +684,6 @@ __global__ void __launch_bounds__(512) k_contract_fp64(int nw, const WorkItem *w
             }
         }
         wave_sync();
-        // pass 2: columns (l, a)
         for (int col = lane; col < K2; col += 64) {
             const int l = col / K, a = col % K;
             double tc[K];
@@ -694,7 +698,6 @@ __global__ void __launch_bounds__(512) k_contract_fp64(int nw, const WorkItem *w
             }
         }
         wave_sync();
-        // pass 3: (b, a) pairs accumulate all c
 #pragma unroll
         for (int p = 0; p < NP; p++) {
             const int ba = lane + 64 * p;
@@ -713,7 +716,8 @@ __global__ void __launch_bounds__(512) k_contract_fp64(int nw, const WorkItem *w
         }
         wave_sync();
     }
-    double *dst = J.outCoef + (size_t)wi.node * 8 * K3 + wave * K3;
+    double *dst = task.slot >= 0 ? partials + (size_t)task.slot * K3
+                                 : J.outCoef + (size_t)wi.node * 8 * K3 + gt * K3;
 #pragma unroll
     for (int p = 0; p < NP; p++) {
         const int ba = lane + 64 * p;
@@ -722,7 +726,267 @@ __global__ void __launch_bounds__(512) k_contract_fp64(int nw, const WorkItem *w
             for (int c = 0; c < K; c++) dst[c * K2 + ba] = acc[p][c];
         }
     }
+}
+
+// sums, per segment (g, gt), the FP64 tuple slots and the gt part of the mixed
+// node slots in slot order (deterministic)
+__global__ void k_reduce_partials(int n, const ReduceItem *items, const WorkItem *work, const JobTab *jobs,
+                                  const double *partials, const double *nodeslots, int K3) {
+    const int i = blockIdx.x;
+    if (i >= n) return;
+    const ReduceItem it = items[i];
+    const WorkItem wi = work[it.g];
+    const JobTab &J = jobs[wi.job];
+    double *dst = J.outCoef + (size_t)wi.node * 8 * K3 + it.gt * K3;
+    for (int e = threadIdx.x; e < K3; e += blockDim.x) {
+        double s = 0.0;
+        for (int q = 0; q < it.nslots; q++) s += partials[(size_t)(it.slot0 + q) * K3 + e];
+        for (int q = 0; q < it.nnslots; q++) s += nodeslots[((size_t)(it.nslot0 + q) * 8 + it.gt) * K3 + e];
+        dst[e] = s;
+    }
+}
+
+
+// ----------------------------------------------------------------- apply: mixed-precision work list
+
+// Same candidates and bounds as k_enum.  A candidate (term, delta, ft) whose
+// surviving combos all stay below ROUTE_FP64 * thr becomes one slice (all allowed
+// gt, FP32 MFMA); otherwise its surviving combos become FP64 tuples.  Counters per
+// thread: 8 tuple segments (per gt) + 1 slice segment; layout of thread_off and
+// seg_total: [g][9][...].
+template <int MODE>
+__global__ void __launch_bounds__(256) k_enum_mixed(int nw, const WorkItem *work, const JobTab *jobs,
+                                                    const TermTab *tabs, GenState gs, int *thread_off,
+                                                    int *seg_total, const long long *seg_ofs, Tuple *out,
+                                                    Slice *sout) {
+    const int g = blockIdx.x;
+    const WorkItem wi = work[g];
+    const JobTab &J = jobs[wi.job];
+    __shared__ int scan[9][256];
+    int cnt[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    long long base[9];
+    if (MODE == 1)
+        for (int q = 0; q < 9; q++) base[q] = seg_ofs[(size_t)g * 9 + q] + thread_off[((size_t)g * 9 + q) * 256 + threadIdx.x];
+    const bool route_all = !(J.thr > 0.0);
+    for (int t = J.tab0; t < J.tab0 + J.ntab; t++) {
+        const TermTab &tb = tabs[t];
+        if (tb.ub[0] < 0) continue;
+        const int ux = tb.ub[0], uy = tb.ub[1], uz = tb.ub[2];
+        const int cells = (2 * ux + 1) * (2 * uy + 1) * (2 * uz + 1);
+        const double aw = fabs(tb.w);
+        const double *nrm = J.operNorms + tb.nrm;
+        for (int c = threadIdx.x; c < cells; c += blockDim.x) {
+            const int dx = c % (2 * ux + 1) - ux;
+            const int dy = (c / (2 * ux + 1)) % (2 * uy + 1) - uy;
+            const int dz = c / ((2 * ux + 1) * (2 * uy + 1)) - uz;
+            const int fx = wi.l[0] - dx, fy = wi.l[1] - dy, fz = wi.l[2] - dz;
+            if (fx < 0 || fy < 0 || fz < 0 || fx >= J.lmax[0] || fy >= J.lmax[1] || fz >= J.lmax[2]) continue;
+            const uint64_t key = dnode_key(J.depth, fx, fy, fz);
+            int f = dhash_find(J.inHash, key);
+            double fn[8];
+            int nft = 8;
+            if (f >= 0) {
+                for (int q = 0; q < 8; q++) fn[q] = J.inNorm[(size_t)f * 8 + q];
+            } else {
+                const int gi = dhash_find(J.genHash, key);
+                if (gi < 0) continue;
+                fn[0] = gs.norm[gi];
+                nft = 1;
+                f = -(gi + 1);
+            }
+            for (int ft = 0; ft < nft; ft++) {
+                const double fv = fn[ft];
+                int surv = 0;
+                double maxb = 0.0;
+                for (int gt = 0; gt < 8; gt++) {
+                    if (!in_box(tb.box[gt * 8 + ft], dx, dy, dz)) continue;
+                    const int tx = 2 * (gt & 1) + (ft & 1);
+                    const int ty = 2 * ((gt >> 1) & 1) + ((ft >> 1) & 1);
+                    const int tz = 2 * ((gt >> 2) & 1) + ((ft >> 2) & 1);
+                    double on = aw;
+                    on *= nrm[(dx + tb.band) * 4 + tx];
+                    on *= nrm[(dy + tb.band) * 4 + ty];
+                    on *= nrm[(dz + tb.band) * 4 + tz];
+                    const double bound = on * fv;
+                    if (!(bound > J.thr)) continue;
+                    surv |= 1 << gt;
+                    maxb = fmax(maxb, bound);
+                }
+                if (!surv) continue;
+                if (route_all || maxb > ROUTE_FP64 * J.thr) {
+                    for (int gt = 0; gt < 8; gt++) {
+                        if (!((surv >> gt) & 1)) continue;
+                        if (MODE == 1) {
+                            Tuple tp;
+                            tp.g = g;
+                            tp.f = f;
+                            tp.term = (uint8_t)(t - J.tab0);
+                            tp.combo = (uint8_t)(gt * 8 + ft);
+                            tp.dx = (int16_t)dx;
+                            tp.dy = (int16_t)dy;
+                            tp.dz = (int16_t)dz;
+                            out[base[gt] + cnt[gt]] = tp;
+                        }
+                        cnt[gt]++;
+                    }
+                } else {
+                    if (MODE == 1) {
+                        Slice sl;
+                        sl.g = g;
+                        sl.f = f;
+                        sl.term = (uint8_t)(t - J.tab0);
+                        sl.ft = (uint8_t)ft;
+                        sl.mask = (uint8_t)((ft == 0 && J.depth > 0) ? 0xFE : 0xFF);
+                        sl.pad = 0;
+                        sl.dx = (int16_t)dx;
+                        sl.dy = (int16_t)dy;
+                        sl.dz = (int16_t)dz;
+                        sl.pad2 = 0;
+                        sout[base[8] + cnt[8]] = sl;
+                    }
+                    cnt[8]++;
+                }
+            }
+        }
+    }
+    if (MODE == 0) {
+        for (int q = 0; q < 9; q++) scan[q][threadIdx.x] = cnt[q];
+        __syncthreads();
+        for (int off = 1; off < 256; off <<= 1) {
+            int v[9];
+            for (int q = 0; q < 9; q++) v[q] = threadIdx.x >= off ? scan[q][threadIdx.x - off] : 0;
+            __syncthreads();
+            for (int q = 0; q < 9; q++) scan[q][threadIdx.x] += v[q];
+            __syncthreads();
+        }
+        for (int q = 0; q < 9; q++) thread_off[((size_t)g * 9 + q) * 256 + threadIdx.x] = scan[q][threadIdx.x] - cnt[q];
+        if (threadIdx.x == 255)
+            for (int q = 0; q < 9; q++) seg_total[(size_t)g * 9 + q] = scan[q][255];
+    }
     (void)nw;
+}
+
+// ----------------------------------------------------------------- apply: FP32 MFMA slice contraction (k = 7)
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// One wavefront per task = a chunk of the slices of one output node.  A slice
+// (f, ft, term, delta) produces all 8 output components with three GEMMs on
+// v_mfma_f32_16x16x4_f32, M = 16 = (gt_d, a) stacked [X_{0,ft_d}; X_{1,ft_d}],
+// K = 8, N = 64, 128, 256 (the dimensions already contracted).  Lane layout of
+// the instruction: A[i][k] from lane i%16 + 16 k, B[k][j] from lane j%16 + 16 k,
+// D[4 (lane/16) + r][lane%16] in register r.  Intermediates rotate through LDS
+// (rows padded to 68 and 132 floats: conflict-free for the access patterns).
+// Each slice result is scaled by the term weight and accumulated in FP64.
+__global__ void __launch_bounds__(256) k_contract_mfma8(int ntask, const SliceTask *tasks, const WorkItem *work,
+                                                        const JobTab *jobs, const TermTab *tabs,
+                                                        const Slice *slices, const double *genPool,
+                                                        double *nodeslots) {
+    constexpr int K = 8, K2 = 64, K3 = 512;
+    constexpr int S1 = 68, S2 = 132;
+    __shared__ float lds1[4][16 * S1];
+    __shared__ float lds2[4][16 * S2];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ti = blockIdx.x * 4 + wave;
+    if (ti >= ntask) return;
+    const SliceTask task = tasks[ti];
+    const WorkItem wi = work[task.g];
+    const JobTab &J = jobs[wi.job];
+    float *L1 = lds1[wave];
+    float *L2 = lds2[wave];
+    const int m = lane & 15, q = lane >> 4;
+    const int gxm = m >> 3, am = m & 7;
+    double acc[16][4];
+#pragma unroll
+    for (int t = 0; t < 16; t++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) acc[t][r] = 0.0;
+    for (int it = 0; it < task.count; it++) {
+        const Slice sl = slices[task.start + it];
+        const TermTab &tb = tabs[J.tab0 + sl.term];
+        const float *B32 = J.operBlocks32 + tb.blk;
+        const int ft = sl.ft;
+        const int fx = ft & 1, fy = (ft >> 1) & 1, fz = (ft >> 2) & 1;
+        // A fragments: row m = (g_d, a), column i = 4 s + q
+        float ax[2], ay[2], az[2];
+        {
+            const float *px = B32 + ((sl.dx + tb.band) * 4 + 2 * gxm + fx) * K2 + am * K;
+            const float *py = B32 + ((sl.dy + tb.band) * 4 + 2 * gxm + fy) * K2 + am * K;
+            const float *pz = B32 + ((sl.dz + tb.band) * 4 + 2 * gxm + fz) * K2 + am * K;
+#pragma unroll
+            for (int s2 = 0; s2 < 2; s2++) {
+                ax[s2] = px[4 * s2 + q];
+                ay[s2] = py[4 * s2 + q];
+                az[s2] = pz[4 * s2 + q];
+            }
+        }
+        const double *f = sl.f >= 0 ? J.inCoef + (size_t)sl.f * 8 * K3 + ft * K3 : genPool + (size_t)(-sl.f - 1) * K3;
+        // pass x: C1[(gx,a)][n = l*8 + j] = sum_i X[a][i] f[n*8 + i]
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            floatx4 c = {0.f, 0.f, 0.f, 0.f};
+            const double *fr = f + (16 * t + m) * K;
+#pragma unroll
+            for (int s2 = 0; s2 < 2; s2++) {
+                const float b = (float)fr[4 * s2 + q];
+                c = __builtin_amdgcn_mfma_f32_16x16x4f32(ax[s2], b, c, 0, 0, 0);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++) L1[(4 * q + r) * S1 + 16 * t + m] = c[r];
+        }
+        wave_sync();
+        // pass y: C2[(gy,b)][n2 = l*16 + (gx,a)] = sum_j Y[b][j] C1[(gx,a)][l*8 + j]
+#pragma unroll
+        for (int t2 = 0; t2 < 8; t2++) {
+            floatx4 c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int s2 = 0; s2 < 2; s2++) {
+                const float b = L1[m * S1 + t2 * 8 + 4 * s2 + q];
+                c = __builtin_amdgcn_mfma_f32_16x16x4f32(ay[s2], b, c, 0, 0, 0);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++) L2[(4 * q + r) * S2 + 16 * t2 + m] = c[r];
+        }
+        wave_sync();
+        // pass z: C3[(gz,c)][n3 = (gy,b)*16 + (gx,a)] = sum_l Z[c][l] C2[(gy,b)][l*16 + (gx,a)]
+        const double w = tb.w;
+        const int mask = sl.mask;
+#pragma unroll
+        for (int t3 = 0; t3 < 16; t3++) {
+            floatx4 c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int s2 = 0; s2 < 2; s2++) {
+                const float b = L2[t3 * S2 + (4 * s2 + q) * 16 + m];
+                c = __builtin_amdgcn_mfma_f32_16x16x4f32(az[s2], b, c, 0, 0, 0);
+            }
+            const int gy = t3 >> 3;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int gz = (4 * q + r) >> 3;
+                const int gt = gxm | (gy << 1) | (gz << 2);
+                const double wm = ((mask >> gt) & 1) ? w : 0.0;
+                acc[t3][r] += wm * (double)c[r];
+            }
+        }
+        wave_sync();
+    }
+    double *dst = nodeslots + (size_t)task.slot * 8 * K3;
+#pragma unroll
+    for (int t3 = 0; t3 < 16; t3++) {
+        const int gy = t3 >> 3, b = t3 & 7;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int m3 = 4 * q + r;
+            const int gz = m3 >> 3, cc = m3 & 7;
+            const int gt = gxm | (gy << 1) | (gz << 2);
+            dst[gt * K3 + (cc * K + b) * K + am] = acc[t3][r];
+        }
+    }
+}
+
+__global__ void k_to_f32(size_t n, const double *in, float *out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (float)in[i];
 }
 
 // norms of the work nodes of all jobs, into their trees and into a compact array
@@ -871,15 +1135,15 @@ hipError_t launch_gen_compute(hipStream_t s, int first, int n, const JobTab *job
 }
 
 hipError_t launch_enum(hipStream_t s, int mode, int nw, const WorkItem *work, const JobTab *jobs,
-                       const TermTab *tabs, const GenState &gs, int *thread_off, int *gtotal, const long long *gofs,
-                       Tuple *out) {
+                       const TermTab *tabs, const GenState &gs, int *thread_off, int *seg_total,
+                       const long long *seg_ofs, Tuple *out) {
     if (nw <= 0) return hipSuccess;
     if (mode == 0)
-        hipLaunchKernelGGL(k_enum<0>, dim3(nw), dim3(256), 0, s, nw, work, jobs, tabs, gs, thread_off, gtotal, gofs,
-                           out);
+        hipLaunchKernelGGL(k_enum<0>, dim3(nw), dim3(256), 0, s, nw, work, jobs, tabs, gs, thread_off, seg_total,
+                           seg_ofs, out);
     else
-        hipLaunchKernelGGL(k_enum<1>, dim3(nw), dim3(256), 0, s, nw, work, jobs, tabs, gs, thread_off, gtotal, gofs,
-                           out);
+        hipLaunchKernelGGL(k_enum<1>, dim3(nw), dim3(256), 0, s, nw, work, jobs, tabs, gs, thread_off, seg_total,
+                           seg_ofs, out);
     return hipGetLastError();
 }
 
@@ -902,12 +1166,49 @@ hipError_t launch_transform(hipStream_t s, int mode, int n, const TransformItem 
     return hipGetLastError();
 }
 
-hipError_t launch_contract_fp64(hipStream_t s, int K, int nw, const WorkItem *work, const JobTab *jobs,
-                                const TermTab *tabs, const long long *gofs, const int *gtotal, const Tuple *tuples,
-                                const double *genPool) {
+hipError_t launch_contract_fp64(hipStream_t s, int K, int ntask, const ContractTask *tasks, const WorkItem *work,
+                                const JobTab *jobs, const TermTab *tabs, const Tuple *tuples, const double *genPool,
+                                double *partials) {
+    if (ntask <= 0) return hipSuccess;
+    const int nb = (ntask + 3) / 4;
+    MWG_DISPATCH_K(K, hipLaunchKernelGGL(k_contract_fp64<KK>, dim3(nb), dim3(256), 0, s, ntask, tasks, work, jobs,
+                                         tabs, tuples, genPool, partials));
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_partials(hipStream_t s, int n, const ReduceItem *items, const WorkItem *work,
+                                  const JobTab *jobs, const double *partials, const double *nodeslots, int K3) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reduce_partials, dim3(n), dim3(256), 0, s, n, items, work, jobs, partials, nodeslots, K3);
+    return hipGetLastError();
+}
+
+hipError_t launch_enum_mixed(hipStream_t s, int mode, int nw, const WorkItem *work, const JobTab *jobs,
+                             const TermTab *tabs, const GenState &gs, int *thread_off, int *seg_total,
+                             const long long *seg_ofs, Tuple *out, Slice *sout) {
     if (nw <= 0) return hipSuccess;
-    MWG_DISPATCH_K(K, hipLaunchKernelGGL(k_contract_fp64<KK>, dim3(nw), dim3(512), 0, s, nw, work, jobs, tabs, gofs,
-                                         gtotal, tuples, genPool));
+    if (mode == 0)
+        hipLaunchKernelGGL(k_enum_mixed<0>, dim3(nw), dim3(256), 0, s, nw, work, jobs, tabs, gs, thread_off,
+                           seg_total, seg_ofs, out, sout);
+    else
+        hipLaunchKernelGGL(k_enum_mixed<1>, dim3(nw), dim3(256), 0, s, nw, work, jobs, tabs, gs, thread_off,
+                           seg_total, seg_ofs, out, sout);
+    return hipGetLastError();
+}
+
+hipError_t launch_contract_mfma(hipStream_t s, int K, int ntask, const SliceTask *tasks, const WorkItem *work,
+                                const JobTab *jobs, const TermTab *tabs, const Slice *slices, const double *genPool,
+                                double *nodeslots) {
+    if (ntask <= 0) return hipSuccess;
+    if (K != 8) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_contract_mfma8, dim3((ntask + 3) / 4), dim3(256), 0, s, ntask, tasks, work, jobs, tabs,
+                       slices, genPool, nodeslots);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_f32(hipStream_t s, size_t n, const double *in, float *out) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, in, out);
     return hipGetLastError();
 }
 

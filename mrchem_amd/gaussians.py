@@ -137,22 +137,65 @@ def integral(g: GaussSum) -> float:
     return s
 
 
+_GH_T, _GH_W = np.polynomial.hermite.hermgauss(8)  # exact for polynomial degree <= 15
+
+
+def overlap_matrix(a: GaussSum, b: GaussSum) -> np.ndarray:
+    """S[i, j] = integral of term i of a times term j of b (exact: Gauss-Hermite in the
+    combined Gaussian, polynomial degree <= 8 + 8 per dimension)."""
+    al = a.alpha[:, None]
+    be = b.alpha[None, :]
+    p = al + be
+    S = np.outer(a.coef, b.coef)
+    for d in range(3):
+        A = a.center[:, d][:, None]
+        B = b.center[:, d][None, :]
+        P = (al * A + be * B) / p
+        pref = np.exp(-al * be / p * (A - B) ** 2) / np.sqrt(p)
+        x = P[..., None] + _GH_T[None, None, :] / np.sqrt(p)[..., None]      # (na, nb, 8)
+        ua = x - A[..., None]
+        ub = x - B[..., None]
+        pa = np.zeros_like(x)
+        pb = np.zeros_like(x)
+        for e in range(MAXDEG, -1, -1):  # Horner
+            pa = pa * ua + a.poly[:, d, e][:, None, None]
+            pb = pb * ub + b.poly[:, d, e][None, :, None]
+        S = S * pref * (pa * pb * _GH_W).sum(-1)
+    return S
+
+
 def overlap(a: GaussSum, b: GaussSum) -> float:
-    return integral(product(a, b))
+    return float(overlap_matrix(a, b).sum())
 
 
 def gram_schmidt(funcs: list[GaussSum]) -> list[GaussSum]:
-    """Orthonormalise with exact overlaps in FP64 (classical Gram-Schmidt, two passes)."""
-    out: list[GaussSum] = []
+    """Orthonormalise with exact overlaps in FP64 (modified Gram-Schmidt, two passes),
+    in coefficient space over the union of all primitives."""
+    prim = GaussSum()
+    rows = []
     for f in funcs:
-        g = f.copy()
+        rows.append((prim.n, f.n))
+        unit = f.copy()
+        unit.coef = np.ones(f.n)
+        prim = prim + unit
+    n = prim.n
+    S = overlap_matrix(prim, prim)
+    C = np.zeros((len(funcs), n))
+    for i, (o, m) in enumerate(rows):
+        C[i, o:o + m] = funcs[i].coef
+    out = np.zeros_like(C)
+    for i in range(len(funcs)):
+        v = C[i].copy()
         for _ in range(2):
-            for q in out:
-                c = overlap(q, g)
-                g = g + q.scaled(-c)
-        nrm = math.sqrt(overlap(g, g))
-        out.append(g.scaled(1.0 / nrm))
-    return out
+            for j in range(i):
+                v -= (out[j] @ S @ v) * out[j]
+        out[i] = v / np.sqrt(v @ S @ v)
+    res = []
+    for i in range(len(funcs)):
+        g = prim.copy()
+        g.coef = out[i].copy()
+        res.append(compact(g, tol=0.0))
+    return res
 
 
 def compact(g: GaussSum, tol: float = 0.0) -> GaussSum:

@@ -1,0 +1,212 @@
+"""GPU parity: the HIP apply (through the C ABI) against the FP64 oracle on the same
+seeded inputs, plus closed forms and size-independent properties."""
+import math
+
+import numpy as np
+import pytest
+
+from mrchem_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def G():
+    from mrchem_amd import mrcpp
+    mrcpp.default_context()
+    return mrcpp
+
+
+@pytest.fixture(scope="module")
+def A():
+    from oracle import analytic
+    return analytic
+
+
+def _same_tree(t_gpu, t_ref, rtol=1e-12):
+    d1, l1, c1 = t_gpu.export()
+    d2, l2, c2 = t_ref.export()
+    assert len(d1) == len(d2)
+    k1 = {(int(d), *map(int, l)): i for i, (d, l) in enumerate(zip(d1, l1))}
+    idx = np.array([k1[(int(d), *map(int, l))] for d, l in zip(d2, l2)])
+    scale = np.abs(c2).max()
+    err = np.abs(c1[idx] - c2).max()
+    assert err <= rtol * scale, (err, scale)
+    return err / scale
+
+
+def test_library_exports_every_symbol():
+    import ctypes
+    from mrchem_amd import _lib
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    for name in _lib.EXPORTED:
+        assert hasattr(L, name)
+
+
+@pytest.mark.parametrize("k,basis", [(5, "L"), (7, "L"), (5, "I")])
+def test_projection_matches_oracle(G, oracle, k, basis):
+    g, _ = configs.density3(1)
+    prec = 1e-5
+    mra = G.mra_from(configs.BOX16, k, basis)
+    omra = oracle.MRA(**configs.BOX16, k=k, basis=basis, max_depth=16)
+    f = G.project(prec, G.FunctionTree(mra), g)
+    of = oracle.Tree(omra).project_gauss(prec, g)
+    _same_tree(f, of, 1e-13)
+    assert f.getSquareNorm() == pytest.approx(of.sqnorm, rel=1e-13)
+
+
+@pytest.mark.parametrize("mu", [0.0, 1.1])
+def test_operator_blocks_match_oracle(G, oracle, mu):
+    mra = G.mra_from(configs.BOX20, 7)
+    omra = oracle.MRA(**configs.BOX20, k=7, max_depth=16)
+    P = G.HelmholtzOperator(mra, mu, 1e-6) if mu > 0 else G.PoissonOperator(mra, 1e-6)
+    OP = oracle.Oper(omra, mu, 1e-6)
+    a1, w1 = P.terms()
+    a2, w2, _ = OP.terms()
+    np.testing.assert_allclose(a1, a2, rtol=1e-14)
+    np.testing.assert_allclose(w1, w2, rtol=1e-14)
+    for term in range(0, P.size(), 7):
+        for depth in (0, 2, 5):
+            m = min(3, (2 << depth) - 1)
+            b1 = P.blocks(term, depth, -m, m)
+            b2 = OP.blocks(term, depth, -m, m)
+            assert np.abs(b1 - b2).max() <= 1e-13 * max(np.abs(b2).max(), 1e-300)
+
+
+def _apply_pair(G, oracle, box, k, prec, g, mu, basis="L"):
+    mra = G.mra_from(box, k, basis)
+    omra = oracle.MRA(**box, k=k, basis=basis, max_depth=16)
+    of = oracle.Tree(omra).project_gauss(prec, g)
+    f = G.FunctionTree(mra).load(*of.export())   # the same input tree on both sides
+    op = G.HelmholtzOperator(mra, mu, prec) if mu > 0 else G.PoissonOperator(mra, prec)
+    oop = oracle.Oper(omra, mu, prec)
+    out = G.FunctionTree(mra)
+    st = G.apply(prec, out, op, f)
+    oout = oracle.Tree(omra)
+    ost = oracle.apply(prec, oout, oop, of)
+    return out, oout, st, ost
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_poisson_apply_matches_oracle_and_erf(G, oracle, A, seed):
+    prec = 1e-4
+    g, parts = configs.density3(seed)
+    out, oout, st, ost = _apply_pair(G, oracle, configs.BOX16, 5, prec, g, 0.0)
+    assert st["tuples"] == ost["tuples"]
+    assert st["nodes_out"] == ost["nodes_out"]
+    _same_tree(out, oout, 1e-12)
+    pts = np.random.default_rng(seed).uniform(-16, 16, (20000, 3))
+    err = out.evalf(pts) - A.erf_potential(parts, pts)
+    assert math.sqrt((err ** 2).mean() * 32 ** 3) < prec * math.sqrt(out.getSquareNorm())
+
+
+def test_helmholtz_apply_matches_oracle_and_erfc(G, oracle, A):
+    prec = 1e-5
+    orbs, nuc, mus = configs.orbitals(0, n_orb=2)
+    out, oout, st, ost = _apply_pair(G, oracle, configs.BOX20, 7, prec, orbs[1], mus[1])
+    assert st["tuples"] == ost["tuples"]
+    assert st["nodes_out"] == ost["nodes_out"]
+    _same_tree(out, oout, 1e-12)
+    pts = np.random.default_rng(3).uniform(-20, 20, (20000, 3))
+    err = out.evalf(pts) - A.potential(orbs[1], pts, mus[1])
+    assert math.sqrt((err ** 2).mean() * 40 ** 3) < prec * math.sqrt(out.getSquareNorm())
+
+
+def test_interpolating_basis_apply_matches_oracle(G, oracle):
+    prec = 1e-4
+    g, _ = configs.density3(2)
+    out, oout, st, ost = _apply_pair(G, oracle, configs.BOX16, 5, prec, g, 0.0, basis="I")
+    assert st["tuples"] == ost["tuples"]
+    _same_tree(out, oout, 1e-12)
+
+
+def test_apply_k9(G, oracle):
+    prec = 1e-4
+    g, _ = configs.density3(3)
+    out, oout, st, ost = _apply_pair(G, oracle, configs.BOX16, 9, prec, g, 0.0)
+    assert st["tuples"] == ost["tuples"]
+    _same_tree(out, oout, 1e-12)
+
+
+def test_batch_equals_single(G, oracle):
+    prec = 1e-4
+    mra = G.mra_from(configs.BOX16, 5)
+    gs = [configs.density3(s)[0] for s in (4, 5, 6)]
+    fs = [G.project(prec, G.FunctionTree(mra), g) for g in gs]
+    P = G.PoissonOperator(mra, prec)
+    H = G.HelmholtzOperator(mra, 0.8, prec)
+    ops = [P, H, P]
+    singles = []
+    for f, op in zip(fs, ops):
+        o = G.FunctionTree(mra)
+        G.apply(prec, o, op, f)
+        singles.append(o)
+    outs = [G.FunctionTree(mra) for _ in fs]
+    G.apply_batch(prec, outs, ops, fs)
+    for a, b in zip(outs, singles):
+        d1, l1, c1 = a.export()
+        d2, l2, c2 = b.export()
+        assert np.array_equal(d1, d2) and np.array_equal(l1, l2)
+        assert np.abs(c1 - c2).max() <= 1e-13 * np.abs(c2).max()
+
+
+def test_reference_coulomb_values_on_gpu(G, A):
+    """tests/qmoperators/coulomb_operator.cpp:76-82 at its settings, through the C ABI."""
+    prec = 1e-3
+    mra = G.mra_from(configs.BOX32, 5, "I", max_depth=20)
+    orbs = {"1s": (1, 0, 0), "2s": (2, 0, 0), "2px": (2, 1, 0), "2py": (2, 1, 1), "2pz": (2, 1, 2)}
+    rho = G.project_hydrogen(prec, G.FunctionTree(mra), [(2.0, q, q) for q in orbs.values()])
+    J = G.FunctionTree(mra)
+    G.apply(prec / math.sqrt(rho.getSquareNorm()), J, G.PoissonOperator(mra, prec), rho)
+    for name, key in [("1s", "1s"), ("2s", "2s"), ("2pz", "2p")]:
+        sq = G.project_hydrogen(prec, G.FunctionTree(mra), [(1.0, orbs[name], orbs[name])])
+        assert G.dot(sq, J) == pytest.approx(A.REFERENCE_E_P[key], rel=3e-5), key
+
+
+def test_errors_are_reported(G):
+    mra = G.mra_from(configs.BOX16, 5)
+    with pytest.raises(G.MwgError):
+        G.HelmholtzOperator(mra, -1.0, 1e-4)
+    with pytest.raises(G.MwgError):
+        G.apply(1e-4, G.FunctionTree(mra), G.PoissonOperator(mra, 1e-4), G.FunctionTree(mra))
+
+
+def _l2_diff(a, b, box_width, n=20000, seed=7):
+    pts = np.random.default_rng(seed).uniform(-box_width / 2, box_width / 2, (n, 3))
+    d = a.evalf(pts) - b.evalf(pts)
+    return math.sqrt((d ** 2).mean() * box_width ** 3)
+
+
+@pytest.mark.parametrize("i", [0, 1])
+def test_mixed_precision_within_prec_of_fp64(G, A, i):
+    """FP32 MFMA slices with FP64 accumulation (k=7) vs the plain FP64 apply: the
+    output differs by less than prec * ||out|| (L2), and both meet the erfc closed form."""
+    prec = 1e-6
+    orbs, nuc, mus = configs.orbitals(0, n_orb=2)
+    mra = G.mra_from(configs.BOX20, 7)
+    f = G.project(prec, G.FunctionTree(mra), orbs[i])
+    H = G.HelmholtzOperator(mra, mus[i], prec)
+    o64 = G.FunctionTree(mra)
+    s64 = G.apply(prec, o64, H, f, flags=G.MWG_APPLY_FP64)
+    o32 = G.FunctionTree(mra)
+    s32 = G.apply(prec, o32, H, f, flags=G.MWG_APPLY_MIXED)
+    assert s32["slices"] > 0
+    norm = math.sqrt(o64.getSquareNorm())
+    assert _l2_diff(o32, o64, 40.0) < prec * norm
+    pts = np.random.default_rng(11).uniform(-20, 20, (20000, 3))
+    err = o32.evalf(pts) - A.potential(orbs[i], pts, mus[i])
+    assert math.sqrt((err ** 2).mean() * 40 ** 3) < prec * norm
+    assert abs(s32["nodes_out"] - s64["nodes_out"]) <= 0.05 * s64["nodes_out"]
+
+
+def test_mixed_poisson_energy(G, oracle):
+    prec = 1e-6
+    g, parts = configs.density3(0)
+    mra = G.mra_from(configs.BOX16, 7)
+    f = G.project(prec, G.FunctionTree(mra), g)
+    P = G.PoissonOperator(mra, prec)
+    o64, o32 = G.FunctionTree(mra), G.FunctionTree(mra)
+    G.apply(prec, o64, P, f)
+    G.apply(prec, o32, P, f, flags=G.MWG_APPLY_MIXED)
+    e64, e32 = G.dot(f, o64), G.dot(f, o32)
+    assert abs(e32 - e64) < prec * abs(e64)
