@@ -100,6 +100,17 @@ int mwg_project_gauss(double prec, mwg_tree out, int n, const double *coef, cons
 int mwg_project_hydrogen(double prec, mwg_tree out, int n, const double *coef, const int *q, double Z,
                          const double *origin);
 
+/* ---- pointwise product: replaces mrcpp::multiply(prec, out, coef, a, b) as used by
+ * qmfunction::multiply (src/qmfunctions/qmfunction_utils.cpp:141-263) for the exchange pair
+ * products (ExchangePotential.cpp:312, 329, 335), V*phi and the density.  out = coef*a*b on
+ * the union grid of a and b, refined by the wavelet-norm split test. */
+int mwg_multiply(double prec, mwg_tree out, double coef, mwg_tree a, mwg_tree b);
+
+/* ---- linear combination: replaces mrcpp::add(prec, out, ca, a, cb, b) as used by
+ * QMFunction::add / qmfunction::linear_combination (src/qmfunctions/QMFunction.cpp:157-200):
+ * out = ca*a + cb*b on the union grid of a and b (exact in the compressed representation). */
+int mwg_add(mwg_tree out, double ca, mwg_tree a, double cb, mwg_tree b);
+
 /* ---- operators: replace new mrcpp::PoissonOperator(MRA, prec) (SCFDriver.cpp:279) and
  * mrcpp::HelmholtzOperator(MRA, mu, prec) (HelmholtzVector.cpp:171).  Kernels 1/r and
  * exp(-mu r)/r (the -1/(2 pi) rescale stays with the caller, HelmholtzVector.cpp:81,135). */

@@ -68,6 +68,8 @@ def _sig():
         "mwg_oper_nterms": [vp, C.POINTER(C.c_int)],
         "mwg_oper_terms": [vp, fd, fd],
         "mwg_oper_blocks": [vp, i, i, i64, i64, fd],
+        "mwg_multiply": [d, vp, d, vp, vp],
+        "mwg_add": [vp, d, vp, d, vp],
         "mwg_apply": [d, vp, vp, vp, i, u, C.POINTER(ApplyStats)],
         "mwg_apply_batch": [i, fd, pvp, pvp, pvp, i, u, C.POINTER(ApplyStats)],
     }
@@ -77,7 +79,7 @@ EXPORTED = ["mwg_ctx_create", "mwg_ctx_destroy", "mwg_last_error", "mwg_sync", "
             "mwg_tree_create", "mwg_tree_destroy", "mwg_tree_nnodes", "mwg_tree_sqnorm", "mwg_tree_maxdepth",
             "mwg_tree_import", "mwg_tree_export", "mwg_tree_rescale", "mwg_tree_dot", "mwg_tree_eval",
             "mwg_project_gauss", "mwg_project_hydrogen", "mwg_oper_poisson", "mwg_oper_helmholtz",
-            "mwg_oper_destroy", "mwg_oper_nterms", "mwg_oper_terms", "mwg_oper_blocks", "mwg_apply",
+            "mwg_oper_destroy", "mwg_oper_nterms", "mwg_oper_terms", "mwg_oper_blocks", "mwg_multiply", "mwg_add", "mwg_apply",
             "mwg_apply_batch"]
 
 

@@ -871,6 +871,130 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
     }
 }
 
+
+// ------------------------------------------------------------------ multiplication
+
+// the node of T at (d, l) (steps 0) or its leaf ancestor and the levels between
+static void locate(const TreeH &T, int d, const int *l, int &src, int &steps) {
+    for (int dd = d; dd >= 0; dd--) {
+        const int sh = d - dd;
+        const int i = T.find(dd, l[0] >> sh, l[1] >> sh, l[2] >> sh);
+        if (i >= 0) {
+            src = i;
+            steps = sh;
+            return;
+        }
+    }
+    throw Error(MWG_ERR_STATE, "input tree does not cover the box");
+}
+
+static void multiply_trees(TreeH &out, double prec, double coef, TreeH &A, TreeH &B) {
+    MraH &M = *A.mra;
+    Ctx &C = *M.ctx;
+    hipStream_t s = C.stream;
+    if (A.n() == 0 || B.n() == 0) throw Error(MWG_ERR_STATE, "multiply: empty input tree");
+    out.make_roots();
+    // union grid
+    std::vector<int> order(out.roots.begin(), out.roots.end());
+    for (size_t w = 0; w < order.size(); w++) {
+        const int i = order[w];
+        const int d = out.depth[i], x = out.lr[3 * i], y = out.lr[3 * i + 1], z = out.lr[3 * i + 2];
+        const int ia = A.find(d, x, y, z), ib = B.find(d, x, y, z);
+        const bool sp = (ia >= 0 && A.child[ia] >= 0) || (ib >= 0 && B.child[ib] >= 0);
+        if (sp && d < M.maxDepth) {
+            out.split(i);
+            for (int a = 0; a < 8; a++) order.push_back(out.child[i] + a);
+        }
+    }
+    std::vector<int> work;
+    for (int i = 0; i < out.n(); i++) work.push_back(i);
+    double sNorm = 0.0, wNorm = 0.0;
+    bool first = true;
+    DBuf<MulItem> ditems;
+    DBuf<int> dwork;
+    while (!work.empty()) {
+        out.ensure_dev(out.n());
+        std::vector<MulItem> items(work.size());
+        for (size_t w = 0; w < work.size(); w++) {
+            const int i = work[w];
+            MulItem &it = items[w];
+            it.node = i;
+            it.depth = out.depth[i];
+            for (int q = 0; q < 3; q++) it.l[q] = out.lr[3 * i + q];
+            locate(A, it.depth, it.l, it.asrc, it.asteps);
+            locate(B, it.depth, it.l, it.bsrc, it.bsteps);
+        }
+        ditems.upload(items.data(), items.size(), s);
+        HC(launch_multiply(s, (int)items.size(), ditems.p, M.geom(), M.dev(), coef, A.coef.p, B.coef.p, out.coef.p));
+        dwork.upload(work.data(), work.size(), s);
+        HC(launch_norms(s, (int)work.size(), dwork.p, out.coef.p, out.norm.p, out.K3()));
+        std::vector<double> nrm(8 * (size_t)out.n());
+        out.norm.download(nrm.data(), nrm.size(), s);
+        for (int i : work) {
+            std::memcpy(&out.cnorm[8 * (size_t)i], &nrm[8 * (size_t)i], 8 * sizeof(double));
+            if (first && out.depth[i] == 0) sNorm += nrm[8 * i] * nrm[8 * i];
+            for (int c = 1; c < 8; c++) wNorm += nrm[8 * i + c] * nrm[8 * i + c];
+        }
+        first = false;
+        const double tnorm = std::sqrt(sNorm + wNorm);
+        std::vector<int> next;
+        for (int i : work) {
+            if (out.child[i] >= 0) continue;
+            const int d = out.depth[i];
+            if (d >= M.maxDepth) continue;
+            double wn = 0.0;
+            for (int c = 1; c < 8; c++) wn += out.cnorm[8 * i + c] * out.cnorm[8 * i + c];
+            if (split_check(M.n0 + d, std::sqrt(wn), prec, tnorm)) {
+                out.split(i);
+                for (int a = 0; a < 8; a++) next.push_back(out.child[i] + a);
+            }
+        }
+        work.swap(next);
+    }
+    auto L = out.internal_by_depth();
+    std::vector<TransformItem> titems;
+    for (int d = (int)L.size() - 1; d >= 0; d--) {
+        titems.clear();
+        for (int p : L[d]) titems.push_back({out.coef.p, p, out.child[p]});
+        C.titems.upload(titems.data(), titems.size(), s);
+        HC(launch_transform(s, 0, (int)titems.size(), C.titems.p, M.W.p, M.B.K));
+    }
+    out.refresh_norms();
+}
+
+// out = ca * a + cb * b on the union grid (mrcpp::add)
+static void add_trees(TreeH &out, double ca, TreeH &A, double cb, TreeH &B) {
+    MraH &M = *A.mra;
+    Ctx &C = *M.ctx;
+    hipStream_t s = C.stream;
+    if (A.n() == 0 || B.n() == 0) throw Error(MWG_ERR_STATE, "add: empty input tree");
+    out.make_roots();
+    std::vector<int> order(out.roots.begin(), out.roots.end());
+    for (size_t w = 0; w < order.size(); w++) {
+        const int i = order[w];
+        const int d = out.depth[i], x = out.lr[3 * i], y = out.lr[3 * i + 1], z = out.lr[3 * i + 2];
+        const int ia = A.find(d, x, y, z), ib = B.find(d, x, y, z);
+        if ((ia >= 0 && A.child[ia] >= 0) || (ib >= 0 && B.child[ib] >= 0)) {
+            out.split(i);
+            for (int a = 0; a < 8; a++) order.push_back(out.child[i] + a);
+        }
+    }
+    out.ensure_dev(out.n());
+    std::vector<MulItem> items(out.n());
+    for (int i = 0; i < out.n(); i++) {
+        MulItem &it = items[i];
+        it.node = i;
+        it.depth = out.depth[i];
+        for (int q = 0; q < 3; q++) it.l[q] = out.lr[3 * i + q];
+        locate(A, it.depth, it.l, it.asrc, it.asteps);
+        locate(B, it.depth, it.l, it.bsrc, it.bsteps);
+    }
+    DBuf<MulItem> ditems;
+    ditems.upload(items.data(), items.size(), s);
+    HC(launch_add(s, (int)items.size(), ditems.p, M.dev(), ca, A.coef.p, cb, B.coef.p, out.coef.p));
+    out.refresh_norms();
+}
+
 }  // namespace mwg
 
 // ================================================================== C ABI
@@ -1279,6 +1403,27 @@ int mwg_oper_blocks(mwg_oper oper, int term, int depth, int64_t dlo, int64_t dhi
         std::vector<long long> rows;
         compute_ns(*oper, depth, {NSRequest{term, dlo, dhi}}, b, nrm, rows);
         b.download(out, (size_t)(dhi - dlo + 1) * 4 * oper->mra->B.K2, C->stream);
+    });
+}
+
+int mwg_multiply(double prec, mwg_tree out, double coef, mwg_tree a, mwg_tree b) {
+    if (!out || !a || !b) return MWG_ERR_ARG;
+    if (a->mra != b->mra || out->mra != a->mra) return MWG_ERR_ARG;
+    Ctx *C = a->mra->ctx;
+    return guarded(C, [&] {
+        if (out == a || out == b) throw Error(MWG_ERR_ARG, "multiply: output must differ from the inputs");
+        if (!(prec > 0.0)) throw Error(MWG_ERR_ARG, "multiply: precision must be positive");
+        multiply_trees(*out, prec, coef, *a, *b);
+    });
+}
+
+int mwg_add(mwg_tree out, double ca, mwg_tree a, double cb, mwg_tree b) {
+    if (!out || !a || !b) return MWG_ERR_ARG;
+    if (a->mra != b->mra || out->mra != a->mra) return MWG_ERR_ARG;
+    Ctx *C = a->mra->ctx;
+    return guarded(C, [&] {
+        if (out == a || out == b) throw Error(MWG_ERR_ARG, "add: output must differ from the inputs");
+        add_trees(*out, ca, *a, cb, *b);
     });
 }
 

@@ -111,6 +111,17 @@ hipError_t launch_enum_mixed(hipStream_t s, int mode, int nw, const WorkItem *wo
 hipError_t launch_contract_mfma(hipStream_t s, int K, int ntask, const SliceTask *tasks, const WorkItem *work,
                                 const JobTab *jobs, const TermTab *tabs, const Slice *slices, const double *genPool,
                                 double *nodeslots);
+struct MulItem {
+    int node;          // output node
+    int depth;
+    int l[3];
+    int asrc, asteps;  // input a: node (steps = 0) or leaf ancestor and levels to refine
+    int bsrc, bsteps;
+};
+hipError_t launch_multiply(hipStream_t s, int n, const MulItem *items, const ProjGeom &geo, const DevBasis &B,
+                           double coef, const double *acoef, const double *bcoef, double *out);
+hipError_t launch_add(hipStream_t s, int n, const MulItem *items, const DevBasis &B, double ca, const double *acoef,
+                      double cb, const double *bcoef, double *out);
 hipError_t launch_to_f32(hipStream_t s, size_t n, const double *in, float *out);
 hipError_t launch_contract_fp64(hipStream_t s, int K, int ntask, const ContractTask *tasks, const WorkItem *work,
                                 const JobTab *jobs, const TermTab *tabs, const Tuple *tuples, const double *genPool,

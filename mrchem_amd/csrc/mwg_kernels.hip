@@ -1043,6 +1043,154 @@ __global__ void k_transform(int mode, int n, const TransformItem *items, const d
     (void)n;
 }
 
+
+// ----------------------------------------------------------------- pointwise multiplication
+
+// Loads the representation of an input tree at an output node into T (interleaved
+// (2K)^3): the node's 8 components when it exists (steps = 0), else the scaling
+// block refined from the leaf ancestor `src` over `steps` levels (path bits from l).
+__device__ void load_input_rep(double *T, const double *coef, int src, int steps, const int *l, const double *W,
+                               int K) {
+    const int N = 2 * K, N2 = N * N, N3 = N2 * N, K3 = K * K * K;
+    const double *p = coef + (size_t)src * 8 * K3;
+    for (int e = threadIdx.x; e < 8 * K3; e += blockDim.x) T[interleaved_index(K, e / K3, e % K3)] = p[e];
+    __syncthreads();
+    if (steps == 0) return;
+    {
+        const int sh = steps - 1;
+        const int ax = (l[0] >> sh) & 1, ay = (l[1] >> sh) & 1, az = (l[2] >> sh) & 1;
+        for (int line = threadIdx.x; line < N2; line += blockDim.x) {
+            const int base = line * N;
+            double in[2 * MAXK];
+            for (int q = 0; q < N; q++) in[q] = T[base + q];
+            for (int j = 0; j < K; j++) {
+                double s = 0.0;
+                for (int q = 0; q < N; q++) s += W[q * N + ax * K + j] * in[q];
+                T[base + j] = s;
+            }
+        }
+        __syncthreads();
+        for (int line = threadIdx.x; line < N * K; line += blockDim.x) {
+            const int z = line / K, x = line % K, base = z * N2 + x;
+            double in[2 * MAXK];
+            for (int q = 0; q < N; q++) in[q] = T[base + q * N];
+            for (int j = 0; j < K; j++) {
+                double s = 0.0;
+                for (int q = 0; q < N; q++) s += W[q * N + ay * K + j] * in[q];
+                T[base + j * N] = s;
+            }
+        }
+        __syncthreads();
+        for (int line = threadIdx.x; line < K * K; line += blockDim.x) {
+            const int y = line / K, x = line % K, base = y * N + x;
+            double in[2 * MAXK];
+            for (int q = 0; q < N; q++) in[q] = T[base + q * N2];
+            for (int j = 0; j < K; j++) {
+                double s = 0.0;
+                for (int q = 0; q < N; q++) s += W[q * N + az * K + j] * in[q];
+                T[base + j * N2] = s;
+            }
+        }
+        __syncthreads();
+    }
+    for (int sh = steps - 2; sh >= 0; sh--) {
+        const int a3[3] = {(l[0] >> sh) & 1, (l[1] >> sh) & 1, (l[2] >> sh) & 1};
+        for (int dim = 0; dim < 3; dim++) {
+            const int stride = dim == 0 ? 1 : (dim == 1 ? N : N2);
+            for (int line = threadIdx.x; line < K * K; line += blockDim.x) {
+                const int u = line % K, v = line / K;
+                const int base = dim == 0 ? v * N2 + u * N : (dim == 1 ? v * N2 + u : v * N + u);
+                double in[MAXK];
+                for (int q = 0; q < K; q++) in[q] = T[base + q * stride];
+                for (int j = 0; j < K; j++) {
+                    double s = 0.0;
+                    for (int q = 0; q < K; q++) s += W[q * N + a3[dim] * K + j] * in[q];
+                    T[base + j * stride] = s;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // keep the scaling block, clear the wavelet components
+    for (int e = threadIdx.x; e < N3; e += blockDim.x) {
+        const int x = e % N, y = (e / N) % N, z = e / N2;
+        if (x >= K || y >= K || z >= K) T[e] = 0.0;
+    }
+    __syncthreads();
+}
+
+// out = coef * a * b at the listed output nodes: both inputs to values at the K^3
+// quadrature points of the 8 children, product, quadrature projection, compression
+__global__ void k_multiply(int n, const MulItem *items, ProjGeom geo, DevBasis B, double coef, const double *acoef,
+                           const double *bcoef, double *out) {
+    extern __shared__ double smem[];
+    const int K = B.K, N = 2 * K, N2 = N * N, N3 = N2 * N, K3 = K * K * K;
+    double *Ta = smem;
+    double *Tb = Ta + N3;
+    double *Ws = Tb + N3;
+    double *WTs = Ws + N2;
+    double *Vs = WTs + N2;
+    double *Ps = Vs + N2;
+    const MulItem it = items[blockIdx.x];
+    for (int e = threadIdx.x; e < N2; e += blockDim.x) {
+        const int r = e / N, c = e % N;
+        Ws[e] = B.W[e];
+        WTs[e] = B.W[c * N + r];
+        const bool same = (r / K) == (c / K);
+        Vs[e] = same ? B.phiq[(r % K) * K + (c % K)] : 0.0;  // values: row = point q, col = function i
+        Ps[e] = same ? B.phiq[(c % K) * K + (r % K)] : 0.0;  // coefficients: row = function i, col = point q
+    }
+    __syncthreads();
+    load_input_rep(Ta, acoef, it.asrc, it.asteps, it.l, Ws, K);
+    load_input_rep(Tb, bcoef, it.bsrc, it.bsteps, it.l, Ws, K);
+    for (int dim = 0; dim < 3; dim++) {
+        line_pass(Ta, WTs, N, dim);
+        line_pass(Tb, WTs, N, dim);
+        __syncthreads();
+    }
+    for (int dim = 0; dim < 3; dim++) {
+        line_pass(Ta, Vs, N, dim);
+        line_pass(Tb, Vs, N, dim);
+        __syncthreads();
+    }
+    const double h = geo.sfac * ldexp(1.0, -(geo.n0 + it.depth + 1));
+    const double scale = coef / (h * sqrt(h));
+    for (int e = threadIdx.x; e < N3; e += blockDim.x) {
+        const int x = e % N, y = (e / N) % N, z = e / N2;
+        Ta[e] = Ta[e] * Tb[e] * scale * B.qw[x % K] * B.qw[y % K] * B.qw[z % K];
+    }
+    __syncthreads();
+    for (int dim = 0; dim < 3; dim++) {
+        line_pass(Ta, Ps, N, dim);
+        __syncthreads();
+    }
+    for (int dim = 0; dim < 3; dim++) {
+        line_pass(Ta, Ws, N, dim);
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < 8 * K3; e += blockDim.x)
+        out[(size_t)it.node * 8 * K3 + e] = Ta[interleaved_index(K, e / K3, e % K3)];
+    (void)n;
+}
+
+// out = ca * a + cb * b at the listed output nodes (compressed coefficients are linear;
+// an input coarser than the node contributes its refined scaling block only)
+__global__ void k_add(int n, const MulItem *items, DevBasis B, double ca, const double *acoef, double cb,
+                      const double *bcoef, double *out) {
+    extern __shared__ double smem[];
+    const int K = B.K, N = 2 * K, N3 = N * N * N, K3 = K * K * K;
+    double *Ta = smem;
+    double *Tb = Ta + N3;
+    const MulItem it = items[blockIdx.x];
+    load_input_rep(Ta, acoef, it.asrc, it.asteps, it.l, B.W, K);
+    load_input_rep(Tb, bcoef, it.bsrc, it.bsteps, it.l, B.W, K);
+    for (int e = threadIdx.x; e < 8 * K3; e += blockDim.x) {
+        const int idx = interleaved_index(K, e / K3, e % K3);
+        out[(size_t)it.node * 8 * K3 + e] = ca * Ta[idx] + cb * Tb[idx];
+    }
+    (void)n;
+}
+
 // ----------------------------------------------------------------- launchers
 
 #define MWG_DISPATCH_K(K, CALL)                                       \
@@ -1209,6 +1357,33 @@ hipError_t launch_contract_mfma(hipStream_t s, int K, int ntask, const SliceTask
 hipError_t launch_to_f32(hipStream_t s, size_t n, const double *in, float *out) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, in, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_multiply(hipStream_t s, int n, const MulItem *items, const ProjGeom &geo, const DevBasis &B,
+                           double coef, const double *acoef, const double *bcoef, double *out) {
+    if (n <= 0) return hipSuccess;
+    const int K = B.K, N = 2 * K;
+    const size_t sm = (size_t)(2 * N * N * N + 4 * N * N) * sizeof(double);
+    if (sm > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_multiply, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)sm);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_multiply, dim3(n), dim3(256), sm, s, n, items, geo, B, coef, acoef, bcoef, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_add(hipStream_t s, int n, const MulItem *items, const DevBasis &B, double ca, const double *acoef,
+                      double cb, const double *bcoef, double *out) {
+    if (n <= 0) return hipSuccess;
+    const int K = B.K, N = 2 * K;
+    const size_t sm = (size_t)(2 * N * N * N) * sizeof(double);
+    if (sm > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_add, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_add, dim3(n), dim3(256), sm, s, n, items, B, ca, acoef, cb, bcoef, out);
     return hipGetLastError();
 }
 

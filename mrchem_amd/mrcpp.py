@@ -27,6 +27,7 @@ from .gaussians import GaussSum
 
 __all__ = ["BoundingBox", "LegendreBasis", "InterpolatingBasis", "MultiResolutionAnalysis", "FunctionTree",
            "PoissonOperator", "HelmholtzOperator", "apply", "apply_batch", "project", "project_hydrogen", "dot",
+           "multiply", "add",
            "MwgError", "MWG_APPLY_FP64", "MWG_APPLY_MIXED", "MWG_APPLY_PREDICT", "Context", "default_context"]
 
 
@@ -241,6 +242,18 @@ def project_hydrogen(prec: float, tree: FunctionTree, terms, Z: float = 1.0, ori
     check(lib().mwg_project_hydrogen(float(prec), tree.h, len(terms), coef, q.reshape(-1), float(Z),
                                      np.array(origin, float)), tree.mra.ctx.h)
     return tree
+
+
+def multiply(prec: float, out: FunctionTree, coef: float, a: FunctionTree, b: FunctionTree) -> FunctionTree:
+    """out = coef * a * b (mrcpp::multiply): union grid of a and b, refined by the split test."""
+    check(lib().mwg_multiply(float(prec), out.h, float(coef), a.h, b.h), a.mra.ctx.h)
+    return out
+
+
+def add(out: FunctionTree, ca: float, a: FunctionTree, cb: float, b: FunctionTree) -> FunctionTree:
+    """out = ca * a + cb * b on the union grid (mrcpp::add)."""
+    check(lib().mwg_add(out.h, float(ca), a.h, float(cb), b.h), a.mra.ctx.h)
+    return out
 
 
 def dot(a: FunctionTree, b: FunctionTree) -> float:

@@ -689,6 +689,197 @@ double tree_dot(const Tree &A, const Tree &Bt) {
     return s;
 }
 
+// ---------------------------------------------------------------- multiplication
+
+// scaling block of tree T at (d, lr): the node's own s when it exists, else the
+// refinement of its leaf ancestor (its wavelets used in the first step); zero outside
+std::vector<double> scaling_at(const Tree &T, int d, const int64_t *lr) {
+    const Basis &B = T.mra->B;
+    int K = B.K, K3 = B.K3, N = 2 * K;
+    std::vector<double> out(K3, 0.0);
+    int self = T.find(d, lr);
+    if (self >= 0) {
+        std::memcpy(out.data(), T.c(self), K3 * sizeof(double));
+        return out;
+    }
+    int anc = -1, ad = -1;
+    for (int dd = d - 1; dd >= 0; dd--) {
+        int64_t la[3] = {lr[0] >> (d - dd), lr[1] >> (d - dd), lr[2] >> (d - dd)};
+        anc = T.find(dd, la);
+        if (anc >= 0) {
+            ad = dd;
+            break;
+        }
+    }
+    if (anc < 0) return out;
+    std::vector<double> ch(8 * K3);
+    reconstruct(B, T.c(anc), ch.data());
+    int sh = d - ad - 1;
+    int a = (int)((lr[0] >> sh) & 1) | (int)(((lr[1] >> sh) & 1) << 1) | (int)(((lr[2] >> sh) & 1) << 2);
+    std::vector<double> cur(ch.begin() + a * K3, ch.begin() + (a + 1) * K3), t1(K3), t2(K3);
+    for (sh = sh - 1; sh >= 0; sh--) {
+        int ax = (int)((lr[0] >> sh) & 1), ay = (int)((lr[1] >> sh) & 1), az = (int)((lr[2] >> sh) & 1);
+        const double *W = B.W.data();
+        for (int zy = 0; zy < K * K; zy++)
+            for (int j = 0; j < K; j++) {
+                double acc = 0.0;
+                for (int i = 0; i < K; i++) acc += W[i * N + ax * K + j] * cur[zy * K + i];
+                t1[zy * K + j] = acc;
+            }
+        for (int z = 0; z < K; z++)
+            for (int j = 0; j < K; j++)
+                for (int x = 0; x < K; x++) {
+                    double acc = 0.0;
+                    for (int i = 0; i < K; i++) acc += W[i * N + ay * K + j] * t1[(z * K + i) * K + x];
+                    t2[(z * K + j) * K + x] = acc;
+                }
+        for (int j = 0; j < K; j++)
+            for (int yx = 0; yx < K * K; yx++) {
+                double acc = 0.0;
+                for (int i = 0; i < K; i++) acc += W[i * N + az * K + j] * t2[i * K * K + yx];
+                cur[j * K * K + yx] = acc;
+            }
+    }
+    return cur;
+}
+
+// the 8 children scaling blocks of tree T under node (d, lr): reconstruct the node's
+// (s, d) when it exists, else refine the scaling block at (d, lr) with zero wavelets
+void children_scaling(const Tree &T, int d, const int64_t *lr, double *ch) {
+    const Basis &B = T.mra->B;
+    int K3 = B.K3;
+    int self = T.find(d, lr);
+    std::vector<double> par(8 * K3, 0.0);
+    if (self >= 0) std::memcpy(par.data(), T.c(self), 8 * K3 * sizeof(double));
+    else {
+        std::vector<double> s = scaling_at(T, d, lr);
+        std::memcpy(par.data(), s.data(), K3 * sizeof(double));
+    }
+    reconstruct(B, par.data(), ch);
+}
+
+// out = coef * a * b on the union grid of a and b, refined by the split rule:
+// each node multiplies the two functions' values at the K^3 quadrature points of its
+// 8 children and projects the product back (the quadrature projection of the product)
+void multiply(Tree &out, double prec, double coef, const Tree &A, const Tree &Bt) {
+    const MRA &mra = *A.mra;
+    const Basis &B = mra.B;
+    int K = B.K, K3 = B.K3;
+    out.make_roots();
+    // grid: union of the inputs
+    std::vector<int> order;
+    for (size_t i = 0; i < out.roots.size(); i++) order.push_back(out.roots[i]);
+    for (size_t w = 0; w < order.size(); w++) {
+        int i = order[w];
+        const int64_t *l = &out.lr[3 * i];
+        int ia = A.find(out.depth[i], l), ib = Bt.find(out.depth[i], l);
+        bool split = (ia >= 0 && A.child[ia] >= 0) || (ib >= 0 && Bt.child[ib] >= 0);
+        if (split && out.depth[i] < mra.maxDepth) {
+            out.split(i);
+            for (int a = 0; a < 8; a++) order.push_back(out.child[i] + a);
+        }
+    }
+    // values <-> coefficients on one child cell (scale of the children): v = phi(x_q) s / sqrt(h)
+    std::vector<int> work;
+    for (int i = 0; i < out.nNodes(); i++) work.push_back(i);
+    double sNorm = 0.0, wNorm = 0.0;
+    bool first = true;
+    while (!work.empty()) {
+        int nw = (int)work.size();
+#pragma omp parallel
+        {
+            std::vector<double> ca(8 * K3), cb(8 * K3), cp(8 * K3), t1(K3), t2(K3);
+#pragma omp for schedule(dynamic, 2)
+            for (int w = 0; w < nw; w++) {
+                int i = work[w];
+                int d = out.depth[i];
+                const int64_t *l = &out.lr[3 * i];
+                children_scaling(A, d, l, ca.data());
+                children_scaling(Bt, d, l, cb.data());
+                for (int c = 0; c < 8; c++) {
+                    // values at the quadrature points: apply phiq along each dimension
+                    double *va = &ca[c * K3], *vb = &cb[c * K3];
+                    for (int pass = 0; pass < 2; pass++) {
+                        double *v = pass == 0 ? va : vb;
+                        // x
+                        for (int zy = 0; zy < K * K; zy++)
+                            for (int q = 0; q < K; q++) {
+                                double acc = 0.0;
+                                for (int i2 = 0; i2 < K; i2++) acc += B.phiq[q * K + i2] * v[zy * K + i2];
+                                t1[zy * K + q] = acc;
+                            }
+                        // y
+                        for (int z = 0; z < K; z++)
+                            for (int q = 0; q < K; q++)
+                                for (int x = 0; x < K; x++) {
+                                    double acc = 0.0;
+                                    for (int i2 = 0; i2 < K; i2++) acc += B.phiq[q * K + i2] * t1[(z * K + i2) * K + x];
+                                    t2[(z * K + q) * K + x] = acc;
+                                }
+                        // z
+                        for (int q = 0; q < K; q++)
+                            for (int yx = 0; yx < K * K; yx++) {
+                                double acc = 0.0;
+                                for (int i2 = 0; i2 < K; i2++) acc += B.phiq[q * K + i2] * t2[i2 * K * K + yx];
+                                v[q * K * K + yx] = acc;
+                            }
+                    }
+                    double h = mra.sfac * std::ldexp(1.0, -(mra.n0 + d + 1));
+                    double scale = coef / (h * h * h);  // two factors h^{-3/2}
+                    double pre = std::pow(h, 1.5);      // back to coefficients
+                    double *vp = &cp[c * K3];
+                    for (int e = 0; e < K3; e++) {
+                        int x = e % K, y = (e / K) % K, z = e / (K * K);
+                        vp[e] = va[e] * vb[e] * scale * B.qw[x] * B.qw[y] * B.qw[z];
+                    }
+                    // coefficients: s_i = pre * sum_q phi_i(x_q) w_q v_q (weights folded above)
+                    for (int zy = 0; zy < K * K; zy++)
+                        for (int i2 = 0; i2 < K; i2++) {
+                            double acc = 0.0;
+                            for (int q = 0; q < K; q++) acc += B.phiq[q * K + i2] * vp[zy * K + q];
+                            t1[zy * K + i2] = acc;
+                        }
+                    for (int z = 0; z < K; z++)
+                        for (int i2 = 0; i2 < K; i2++)
+                            for (int x = 0; x < K; x++) {
+                                double acc = 0.0;
+                                for (int q = 0; q < K; q++) acc += B.phiq[q * K + i2] * t1[(z * K + q) * K + x];
+                                t2[(z * K + i2) * K + x] = acc;
+                            }
+                    for (int i2 = 0; i2 < K; i2++)
+                        for (int yx = 0; yx < K * K; yx++) {
+                            double acc = 0.0;
+                            for (int q = 0; q < K; q++) acc += B.phiq[q * K + i2] * t2[q * K * K + yx];
+                            vp[i2 * K * K + yx] = acc * pre;
+                        }
+                }
+                compress(B, cp.data(), out.c(i));
+                out.calc_norms(i);
+            }
+        }
+        for (int i : work) {
+            if (first && out.depth[i] == 0) sNorm += out.cnorm[8 * i] * out.cnorm[8 * i];
+            for (int cc = 1; cc < 8; cc++) wNorm += out.cnorm[8 * i + cc] * out.cnorm[8 * i + cc];
+        }
+        first = false;
+        double tnorm = std::sqrt(sNorm + wNorm);
+        std::vector<int> next;
+        for (int i : work) {
+            if (out.child[i] >= 0) continue;
+            int d = out.depth[i];
+            if (d >= mra.maxDepth) continue;
+            double wn = 0.0;
+            for (int cc = 1; cc < 8; cc++) wn += out.cnorm[8 * i + cc] * out.cnorm[8 * i + cc];
+            if (split_check(mra, d, std::sqrt(wn), prec, tnorm)) {
+                out.split(i);
+                for (int a = 0; a < 8; a++) next.push_back(out.child[i] + a);
+            }
+        }
+        work.swap(next);
+    }
+    bottom_up(out);
+}
+
 // ---------------------------------------------------------------- kernel fit
 
 struct KernelFit {
@@ -1488,6 +1679,9 @@ int mwref_apply(double prec, void *out, void *oper, void *inp, int max_iter, mwr
     return 0;
 }
 
+void mwref_multiply(void *out, double prec, double coef, void *a, void *b) {
+    multiply(*(Tree *)out, prec, coef, *(Tree *)a, *(Tree *)b);
+}
 void mwref_set_threads(int n) { omp_set_num_threads(n); }
 void mwref_set_screen_factor(double f) { g_screen_factor = f; }
 void mwref_hist(int64_t *h) {

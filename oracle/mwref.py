@@ -89,6 +89,7 @@ def lib():
             "mwref_set_threads": (None, [i]),
             "mwref_get_threads": (i, []),
             "mwref_set_screen_factor": (None, [d]),
+            "mwref_multiply": (None, [vp, d, d, vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -262,6 +263,12 @@ def apply(prec, out: Tree, oper: Oper, inp: Tree, max_iter=-1):
 
 def set_threads(n):
     lib().mwref_set_threads(int(n))
+
+
+def multiply(prec, out: Tree, coef, a: Tree, b: Tree):
+    """out = coef * a * b (pointwise), union grid of a and b refined by the split rule."""
+    lib().mwref_multiply(out.h, float(prec), float(coef), a.h, b.h)
+    return out
 
 
 def set_screen_factor(f):

@@ -210,3 +210,23 @@ def test_mixed_poisson_energy(G, oracle):
     G.apply(prec, o32, P, f, flags=G.MWG_APPLY_MIXED)
     e64, e32 = G.dot(f, o64), G.dot(f, o32)
     assert abs(e32 - e64) < prec * abs(e64)
+
+
+@pytest.mark.parametrize("k", [5, 7])
+def test_multiply_matches_oracle(G, oracle, k):
+    """mrcpp::multiply on the device vs the oracle on the same input trees."""
+    prec = 1e-5
+    orbs, nuc, mus = configs.orbitals(1, n_orb=3)
+    mra = G.mra_from(configs.BOX20, k)
+    omra = oracle.MRA(**configs.BOX20, k=k, max_depth=16)
+    oa = oracle.Tree(omra).project_gauss(prec, orbs[0])
+    ob = oracle.Tree(omra).project_gauss(prec, orbs[2])
+    a = G.FunctionTree(mra).load(*oa.export())
+    b = G.FunctionTree(mra).load(*ob.export())
+    p = G.multiply(prec, G.FunctionTree(mra), 1.5, a, b)
+    op = oracle.multiply(prec, oracle.Tree(omra), 1.5, oa, ob)
+    _same_tree(p, op, 1e-12)
+    from mrchem_amd import gaussians as GS
+    ex = GS.product(orbs[0], orbs[2]).scaled(1.5)
+    pts = np.random.default_rng(5).uniform(-3, 3, (2000, 3))
+    assert np.abs(p.evalf(pts) - ex(pts)).max() < 100 * prec * np.abs(ex(pts)).max()
