@@ -42,6 +42,13 @@ def build_inputs(G, configs, k, prec, n_orb, seed):
     return mra, orbs, mus, trees, ops, t_proj, t_oper
 
 
+def _l2diff(a, b, n=20000, seed=3):
+    import numpy as np
+    pts = np.random.default_rng(seed).uniform(-20.0, 20.0, (n, 3))
+    d = a.evalf(pts) - b.evalf(pts)
+    return math.sqrt((d ** 2).mean() * 40.0 ** 3)
+
+
 def run_variant(G, mra, trees, ops, prec, flags, steps, warmup):
     n = len(trees)
     for _ in range(warmup):
@@ -118,6 +125,15 @@ def main():
     log(f"[bench] fp64: {res64['ms_per_step']:.2f} ms/step, {res64['nodes_per_s']:.1f} nodes/s, "
         f"contraction {1e3 * res64['t_contract_per_step']:.2f} ms/step")
     best, dtype = res64, "f64"
+    res32 = None
+    if args.k == 7:
+        res32 = run_variant(G, mra, trees, ops, args.prec, G.MWG_APPLY_MIXED, args.steps, args.warmup)
+        err = max(_l2diff(a, b) / math.sqrt(b.getSquareNorm()) for a, b in zip(res32["outs"], res64["outs"]))
+        res32["rel_err_vs_fp64"] = err
+        log(f"[bench] mixed: {res32['ms_per_step']:.2f} ms/step, {res32['nodes_per_s']:.1f} nodes/s, "
+            f"contraction {1e3 * res32['t_contract_per_step']:.2f} ms/step, rel L2 err vs fp64 {err:.2e}")
+        if res32["nodes_per_s"] > res64["nodes_per_s"] and err < args.prec:
+            best, dtype = res32, "f32"
 
     achieved = best["flops_per_step"] / best["t_contract_per_step"] if best["t_contract_per_step"] > 0 else 0.0
     roofline = dict(bound="mfma", achieved=achieved / 1e12, peak=PEAK[dtype] / 1e12, unit="TFLOP/s",
@@ -147,6 +163,10 @@ def main():
             "orbitals": args.orbitals, "k": args.k, "prec": args.prec, "seed": args.seed,
             "output_nodes_per_step": best["nodes_per_step"], "tuples_per_step": best["tuples_per_step"],
             "fp64_nodes_per_s": res64["nodes_per_s"], "speedup_vs_fp64": best["nodes_per_s"] / res64["nodes_per_s"],
+            "fp64_ms_per_step": res64["ms_per_step"],
+            "mixed_nodes_per_s": res32["nodes_per_s"] if res32 else None,
+            "mixed_rel_l2_err_vs_fp64": res32["rel_err_vs_fp64"] if res32 else None,
+            "mixed_fp64_tuples_per_step": None,
             "operator_build_s": t_oper, "projection_s": t_proj,
             "contraction_ms_per_step": 1e3 * best["t_contract_per_step"],
         },

@@ -15,6 +15,8 @@
 #include <array>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -83,8 +85,19 @@ struct DBuf {
 
 // ------------------------------------------------------------------ handles
 
+// pinned host staging for asynchronous uploads: data is copied into the ring and
+// sent with hipMemcpyAsync; the ring is reused only after the stream has synchronised
+struct Stage {
+    char *h = nullptr;
+    size_t cap = 0, off = 0;
+    ~Stage() {
+        if (h) (void)hipHostFree(h);
+    }
+};
+
 struct Ctx {
     int device = 0;
+    Stage stage;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string err;
@@ -111,6 +124,38 @@ struct Ctx {
     int chunk = 64;        // tuples per FP64 contraction task
     int slice_chunk = 16;  // slices per MFMA task
 };
+
+// asynchronous upload through the context's staging ring
+template <class T>
+static void upload_async(Ctx &C, DBuf<T> &d, const T *src, size_t n) {
+    d.alloc(n);
+    if (!n) return;
+    const size_t bytes = n * sizeof(T);
+    const size_t need = (bytes + 255) & ~(size_t)255;
+    if (C.stage.off + need > C.stage.cap) {
+        HC(hipStreamSynchronize(C.stream));
+        C.stage.off = 0;
+        if (need > C.stage.cap) {
+            if (C.stage.h) HC(hipHostFree(C.stage.h));
+            C.stage.h = nullptr;
+            const size_t cap = std::max<size_t>(need, (size_t)64 << 20);
+            HC(hipHostMalloc((void **)&C.stage.h, cap, hipHostMallocDefault));
+            C.stage.cap = cap;
+        }
+    }
+    char *p = C.stage.h + C.stage.off;
+    std::memcpy(p, src, bytes);
+    C.stage.off += need;
+    HC(hipMemcpyAsync(d.p, p, bytes, hipMemcpyHostToDevice, C.stream));
+}
+
+// read back (synchronises the stream; the staging ring is free again afterwards)
+template <class T>
+static void download_sync(Ctx &C, const DBuf<T> &d, T *dst, size_t n) {
+    if (n) HC(hipMemcpyAsync(dst, d.p, n * sizeof(T), hipMemcpyDeviceToHost, C.stream));
+    HC(hipStreamSynchronize(C.stream));
+    C.stage.off = 0;
+}
 
 struct MraH {
     Ctx *ctx = nullptr;
@@ -544,12 +589,18 @@ static void job_boxes(const JobH &J, int d, std::vector<TermTab> &tabs, int &tab
                     for (int q2 = 0; q2 < 3; q2++)
                         if (q2 != q) other *= lv.maxn[ty[q2]];
                     const double tau = thr / other;
+                    // env is non-increasing in m: largest m with env[m] > tau
+                    const double *env = &lv.env[ty[q] * (lv.band + 1)];
                     int m = -1;
-                    for (int mm = lv.band; mm >= 0; mm--)
-                        if (lv.env[ty[q] * (lv.band + 1) + mm] > tau) {
-                            m = mm;
-                            break;
+                    if (env[0] > tau) {
+                        int lo = 0, hi = lv.band;
+                        while (lo < hi) {
+                            const int mid = (lo + hi + 1) / 2;
+                            if (env[mid] > tau) lo = mid;
+                            else hi = mid - 1;
                         }
+                        m = lo;
+                    }
                     if (m < 0) ok = false;
                     m3[q] = (short)m;
                 }
@@ -568,8 +619,34 @@ static void job_boxes(const JobH &J, int d, std::vector<TermTab> &tabs, int &tab
     if (ntab > 255) throw Error(MWG_ERR_RANGE, "more than 255 active terms at one depth");
 }
 
+struct PhaseTimer {
+    bool on = false;
+    std::vector<std::pair<std::string, double>> acc;
+    std::chrono::steady_clock::time_point t0;
+    void start() { t0 = std::chrono::steady_clock::now(); }
+    void lap(const char *name) {
+        if (!on) return;
+        auto t1 = std::chrono::steady_clock::now();
+        const double dt = std::chrono::duration<double>(t1 - t0).count();
+        t0 = t1;
+        for (auto &p : acc)
+            if (p.first == name) {
+                p.second += dt;
+                return;
+            }
+        acc.push_back({name, dt});
+    }
+    void report() {
+        if (!on) return;
+        for (auto &p : acc) fprintf(stderr, "[mwg timing] %-14s %8.3f ms\n", p.first.c_str(), 1e3 * p.second);
+    }
+};
+
 static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, std::vector<mwg_apply_stats> &stats) {
     if (jobs.empty()) return;
+    PhaseTimer PT;
+    PT.on = std::getenv("MWG_TIMING") != nullptr;
+    PT.start();
     MraH &M = *jobs[0].in->mra;
     Ctx &C = *M.ctx;
     hipStream_t s = C.stream;
@@ -622,6 +699,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
     C.genPool.alloc(1024 * (size_t)K3);
     HC(hipMemsetAsync(C.genCount.p, 0, sizeof(int), s));
     int genDone = 0;
+    PT.lap("setup");
 
     int iter = 0;
     while (true) {
@@ -662,14 +740,15 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             for (int g : J.work) work.push_back({j, g, {J.out->lr[3 * g], J.out->lr[3 * g + 1], J.out->lr[3 * g + 2]}});
         }
         const int nw = (int)work.size();
-        C.work.upload(work.data(), work.size(), s);
-        C.jobs.upload(jt.data(), jt.size(), s);
+        PT.lap("opers+boxes");
+        upload_async(C, C.work, work.data(), work.size());
+        upload_async(C, C.jobs, jt.data(), jt.size());
         if (tabs.empty()) {
             TermTab dummy;
             std::memset(&dummy, 0, sizeof(dummy));
             tabs.push_back(dummy);
         }
-        C.tabs.upload(tabs.data(), tabs.size(), s);
+        upload_async(C, C.tabs, tabs.data(), tabs.size());
         // generated input nodes
         HC(hipMemsetAsync(C.genOverflow.p, 0, sizeof(int), s));
         GenState gs{C.genKeys.p, C.genJob.p, C.genCount.p, genCap, C.genOverflow.p, C.genPool.p, C.genNorm.p};
@@ -678,7 +757,9 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         HC(hipMemcpyAsync(&hc[0], C.genCount.p, sizeof(int), hipMemcpyDeviceToHost, s));
         HC(hipMemcpyAsync(&hc[1], C.genOverflow.p, sizeof(int), hipMemcpyDeviceToHost, s));
         HC(hipStreamSynchronize(s));
+        C.stage.off = 0;
         if (hc[1]) throw Error(MWG_ERR_RANGE, "generated-node table overflow");
+        PT.lap("upload+genmark");
         const int genNow = hc[0];
         if (genNow > genDone) {
             C.genPool.grow_keep((size_t)genNow * K3, s);
@@ -688,13 +769,14 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             HC(launch_gen_compute(s, genDone, genNow - genDone, C.jobs.p, gs, M.W.p, K));
             genDone = genNow;
         }
+        PT.lap("gencompute");
         if (!mixed) {
             // work list: tuples grouped by segment (g, gt)
             C.thread_off.alloc((size_t)nw * 8 * 256);
             C.gtotal.alloc((size_t)nw * 8);
             HC(launch_enum(s, 0, nw, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, nullptr, nullptr));
             std::vector<int> tot((size_t)nw * 8);
-            C.gtotal.download(tot.data(), tot.size(), s);
+            download_sync(C, C.gtotal, tot.data(), tot.size());
             std::vector<long long> ofs((size_t)nw * 8);
             std::vector<ContractTask> tasks;
             std::vector<ReduceItem> reds;
@@ -717,14 +799,14 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                     }
                     total += c;
                 }
-            C.gofs.upload(ofs.data(), ofs.size(), s);
+            upload_async(C, C.gofs, ofs.data(), ofs.size());
             C.tuples.alloc((size_t)std::max<long long>(total, 1));
             HC(launch_enum(s, 1, nw, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, C.gofs.p,
                            C.tuples.p));
             std::stable_sort(tasks.begin(), tasks.end(),
                              [](const ContractTask &a, const ContractTask &b) { return a.count > b.count; });
-            C.tasks.upload(tasks.data(), tasks.size(), s);
-            if (!reds.empty()) C.reds.upload(reds.data(), reds.size(), s);
+            upload_async(C, C.tasks, tasks.data(), tasks.size());
+            if (!reds.empty()) upload_async(C, C.reds, reds.data(), reds.size());
             C.partials.alloc((size_t)std::max(nslots, 1) * K3);
             HC(hipEventRecord(C.ev0, s));
             HC(launch_contract_fp64(s, K, (int)tasks.size(), C.tasks.p, C.work.p, C.jobs.p, C.tabs.p, C.tuples.p,
@@ -738,7 +820,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             HC(launch_enum_mixed(s, 0, nw, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, nullptr,
                                  nullptr, nullptr));
             std::vector<int> tot((size_t)nw * 9);
-            C.gtotal.download(tot.data(), tot.size(), s);
+            download_sync(C, C.gtotal, tot.data(), tot.size());
             std::vector<long long> ofs((size_t)nw * 9);
             std::vector<ContractTask> tasks;
             std::vector<SliceTask> stasks;
@@ -768,16 +850,16 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                     total += c;
                 }
             }
-            C.gofs.upload(ofs.data(), ofs.size(), s);
+            upload_async(C, C.gofs, ofs.data(), ofs.size());
             C.tuples.alloc((size_t)std::max<long long>(total, 1));
             C.slices.alloc((size_t)std::max<long long>(stotal, 1));
             HC(launch_enum_mixed(s, 1, nw, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, C.gofs.p,
                                  C.tuples.p, C.slices.p));
             std::stable_sort(tasks.begin(), tasks.end(),
                              [](const ContractTask &a, const ContractTask &b) { return a.count > b.count; });
-            if (!tasks.empty()) C.tasks.upload(tasks.data(), tasks.size(), s);
-            if (!stasks.empty()) C.stasks.upload(stasks.data(), stasks.size(), s);
-            C.reds.upload(reds.data(), reds.size(), s);
+            if (!tasks.empty()) upload_async(C, C.tasks, tasks.data(), tasks.size());
+            if (!stasks.empty()) upload_async(C, C.stasks, stasks.data(), stasks.size());
+            upload_async(C, C.reds, reds.data(), reds.size());
             C.partials.alloc((size_t)std::max(nslots, 1) * K3);
             C.nodeslots.alloc((size_t)std::max(nnslots, 1) * 8 * K3);
             HC(hipEventRecord(C.ev0, s));
@@ -789,14 +871,16 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                                       K3));
             HC(hipEventRecord(C.ev1, s));
         }
+        PT.lap("worklist+launch");
         // norms and refinement
         C.wnorms.alloc((size_t)nw * 8);
         HC(launch_work_norms(s, nw, C.work.p, C.jobs.p, K3, C.wnorms.p));
         std::vector<double> wn((size_t)nw * 8);
-        C.wnorms.download(wn.data(), wn.size(), s);
+        download_sync(C, C.wnorms, wn.data(), wn.size());
         float ms = 0.0f;
         HC(hipEventElapsedTime(&ms, C.ev0, C.ev1));
         t_contract_ms += ms;
+        PT.lap("contract+norms");
         for (int i = 0; i < nw; i++) {
             JobH &J = jobs[work[i].job];
             std::memcpy(&J.out->cnorm[8 * (size_t)work[i].node], &wn[8 * (size_t)i], 8 * sizeof(double));
@@ -826,6 +910,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             }
             J.work.swap(next);
         }
+        PT.lap("split");
         iter++;
     }
     // top-down additive transform, then bottom-up compression, for all output trees
@@ -841,7 +926,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         for (size_t j = 0; j < jobs.size(); j++)
             if (d < (int)L[j].size())
                 for (int p : L[j][d]) items.push_back({jobs[j].out->coef.p, p, jobs[j].out->child[p]});
-        C.titems.upload(items.data(), items.size(), s);
+        upload_async(C, C.titems, items.data(), items.size());
         HC(launch_transform(s, 1, (int)items.size(), C.titems.p, M.WT.p, K));
     }
     for (int d = maxd - 1; d >= 0; d--) {
@@ -849,10 +934,12 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         for (size_t j = 0; j < jobs.size(); j++)
             if (d < (int)L[j].size())
                 for (int p : L[j][d]) items.push_back({jobs[j].out->coef.p, p, jobs[j].out->child[p]});
-        C.titems.upload(items.data(), items.size(), s);
+        upload_async(C, C.titems, items.data(), items.size());
         HC(launch_transform(s, 0, (int)items.size(), C.titems.p, M.W.p, K));
     }
     for (JobH &J : jobs) J.out->refresh_norms();
+    PT.lap("transforms");
+    PT.report();
     const double t_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
     stats.assign(jobs.size(), mwg_apply_stats{});
     const double fl = 6.0 * K * K * K * K;
@@ -1076,7 +1163,7 @@ int mwg_mra_create(mwg_ctx ctx, int root_scale, const int *corner, const int *bo
         for (int d = 0; d < 3; d++)
             if (boxes[d] < 1) throw Error(MWG_ERR_ARG, "boxes must be >= 1");
         int bmax = std::max({boxes[0], boxes[1], boxes[2]});
-        if (max_depth < 1 || ((int64_t)bmax << (max_depth + 1)) > (1 << 19))
+        if (max_depth < 1 || max_depth > 30 || ((int64_t)bmax << max_depth) > (1 << 19))
             throw Error(MWG_ERR_RANGE, "max depth too large for 19-bit translations");
         m->ctx = ctx;
         m->n0 = root_scale;

@@ -6,6 +6,7 @@
 // index is bit*K + i; each 1D pass multiplies every line of 2K values in place.
 #include "mwg_kernels.h"
 
+#include <climits>
 #include <cmath>
 
 namespace mwg {
@@ -30,6 +31,12 @@ __device__ inline void d_phi(const DevBasis &B, double x, double *out) {
         out[i] = s;
     }
 }
+
+// global-address-space views: uniform loads through these become scalar loads
+typedef const double __attribute__((address_space(1))) gdouble;
+typedef const float __attribute__((address_space(1))) gfloat;
+
+__device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 
 __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -539,6 +546,54 @@ __global__ void k_gen_compute(int first, int n, const JobTab *jobs, GenState gs,
     if (threadIdx.x == 0) gs.norm[g] = sqrt(red[0]);
 }
 
+
+// Per output node, the input node (>= 0), generated node (-(g+1)) or nothing
+// (INT_MIN) at every translation of the union box over all terms, cached in LDS so
+// that each term's loop reads the cache instead of probing the hash tables again.
+constexpr int FCACHE = 4096;
+struct FCache {
+    int ux, uy, uz;  // half-widths, ux < 0: no cache (box too large)
+};
+
+__device__ inline int probe_f(const JobTab &J, const GenState &gs, int fx, int fy, int fz) {
+    if (fx < 0 || fy < 0 || fz < 0 || fx >= J.lmax[0] || fy >= J.lmax[1] || fz >= J.lmax[2]) return INT_MIN;
+    const uint64_t key = dnode_key(J.depth, fx, fy, fz);
+    const int f = dhash_find(J.inHash, key);
+    if (f >= 0) return f;
+    const int gi = dhash_find(J.genHash, key);
+    return gi >= 0 ? -(gi + 1) : INT_MIN;
+}
+
+__device__ inline FCache fill_fcache(int *cache, const WorkItem &wi, const JobTab &J, const TermTab *tabs,
+                                     const GenState &gs) {
+    int ux = -1, uy = -1, uz = -1;
+    for (int t = J.tab0; t < J.tab0 + J.ntab; t++) {
+        ux = max(ux, (int)tabs[t].ub[0]);
+        uy = max(uy, (int)tabs[t].ub[1]);
+        uz = max(uz, (int)tabs[t].ub[2]);
+    }
+    FCache fc{ux, uy, uz};
+    if (ux < 0) return fc;
+    const int nx = 2 * ux + 1, ny = 2 * uy + 1, nz = 2 * uz + 1;
+    if (nx * ny * nz > FCACHE) {
+        fc.ux = -1;
+        return fc;
+    }
+    for (int c = threadIdx.x; c < nx * ny * nz; c += blockDim.x) {
+        const int dx = c % nx - ux, dy = (c / nx) % ny - uy, dz = c / (nx * ny) - uz;
+        cache[c] = probe_f(J, gs, wi.l[0] - dx, wi.l[1] - dy, wi.l[2] - dz);
+    }
+    __syncthreads();
+    return fc;
+}
+
+__device__ inline int lookup_f(const int *cache, const FCache &fc, const WorkItem &wi, const JobTab &J,
+                               const GenState &gs, int dx, int dy, int dz) {
+    if (fc.ux >= 0)
+        return cache[((dz + fc.uz) * (2 * fc.uy + 1) + (dy + fc.uy)) * (2 * fc.ux + 1) + (dx + fc.ux)];
+    return probe_f(J, gs, wi.l[0] - dx, wi.l[1] - dy, wi.l[2] - dz);
+}
+
 // ----------------------------------------------------------------- apply: work list
 
 // Enumerates, for one output node, every (term, delta, ft, gt) candidate inside the
@@ -555,6 +610,8 @@ __global__ void __launch_bounds__(256) k_enum(int nw, const WorkItem *work, cons
     const WorkItem wi = work[g];
     const JobTab &J = jobs[wi.job];
     __shared__ int scan[8][256];
+    __shared__ int fcache[FCACHE];
+    const FCache fc = fill_fcache(fcache, wi, J, tabs, gs);
     int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     long long base[8];
     if (MODE == 1)
@@ -570,20 +627,15 @@ __global__ void __launch_bounds__(256) k_enum(int nw, const WorkItem *work, cons
             const int dx = c % (2 * ux + 1) - ux;
             const int dy = (c / (2 * ux + 1)) % (2 * uy + 1) - uy;
             const int dz = c / ((2 * ux + 1) * (2 * uy + 1)) - uz;
-            const int fx = wi.l[0] - dx, fy = wi.l[1] - dy, fz = wi.l[2] - dz;
-            if (fx < 0 || fy < 0 || fz < 0 || fx >= J.lmax[0] || fy >= J.lmax[1] || fz >= J.lmax[2]) continue;
-            const uint64_t key = dnode_key(J.depth, fx, fy, fz);
-            int f = dhash_find(J.inHash, key);
+            const int f = lookup_f(fcache, fc, wi, J, gs, dx, dy, dz);
+            if (f == INT_MIN) continue;
             double fn[8];
             int nft = 8;
             if (f >= 0) {
                 for (int q = 0; q < 8; q++) fn[q] = J.inNorm[(size_t)f * 8 + q];
             } else {
-                const int gi = dhash_find(J.genHash, key);
-                if (gi < 0) continue;
-                fn[0] = gs.norm[gi];
+                fn[0] = gs.norm[-f - 1];
                 nft = 1;
-                f = -(gi + 1);
             }
             for (int ft = 0; ft < nft; ft++) {
                 const double fv = fn[ft];
@@ -646,13 +698,16 @@ __global__ void __launch_bounds__(256) k_contract_fp64(int ntask, const Contract
     constexpr int K2 = K * K, K3 = K2 * K;
     constexpr int NP = (K2 + 63) / 64;
     __shared__ double buf[4][K3];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = wave_id(), lane = threadIdx.x & 63;
     const int ti = blockIdx.x * 4 + wave;
     if (ti >= ntask) return;
     const ContractTask task = tasks[ti];
     const WorkItem wi = work[task.g];
     const JobTab &J = jobs[wi.job];
     const int gt = task.gt;
+    const gdouble *blocks = (const gdouble *)J.operBlocks;
+    const gdouble *inCoef = (const gdouble *)J.inCoef;
+    const gdouble *gen = (const gdouble *)genPool;
     double *B = buf[wave];
     double acc[NP][K];
 #pragma unroll
@@ -666,10 +721,11 @@ __global__ void __launch_bounds__(256) k_contract_fp64(int ntask, const Contract
         const int tx = 2 * (gt & 1) + (ft & 1);
         const int ty = 2 * ((gt >> 1) & 1) + ((ft >> 1) & 1);
         const int tz = 2 * ((gt >> 2) & 1) + ((ft >> 2) & 1);
-        const double *X = J.operBlocks + tb.blk + (size_t)((tp.dx + tb.band) * 4 + tx) * K2;
-        const double *Y = J.operBlocks + tb.blk + (size_t)((tp.dy + tb.band) * 4 + ty) * K2;
-        const double *Z = J.operBlocks + tb.blk + (size_t)((tp.dz + tb.band) * 4 + tz) * K2;
-        const double *f = tp.f >= 0 ? J.inCoef + (size_t)tp.f * 8 * K3 + ft * K3 : genPool + (size_t)(-tp.f - 1) * K3;
+        const long long band = tb.band, blk = tb.blk;
+        const gdouble *X = blocks + blk + ((tp.dx + band) * 4 + tx) * K2;
+        const gdouble *Y = blocks + blk + ((tp.dy + band) * 4 + ty) * K2;
+        const gdouble *Z = blocks + blk + ((tp.dz + band) * 4 + tz) * K2;
+        const gdouble *f = tp.f >= 0 ? inCoef + (size_t)tp.f * 8 * K3 + ft * K3 : gen + (size_t)(-tp.f - 1) * K3;
         const double w = tb.w;
         for (int r = lane; r < K2; r += 64) {
             double fr[K];
@@ -763,6 +819,8 @@ __global__ void __launch_bounds__(256) k_enum_mixed(int nw, const WorkItem *work
     const WorkItem wi = work[g];
     const JobTab &J = jobs[wi.job];
     __shared__ int scan[9][256];
+    __shared__ int fcache[FCACHE];
+    const FCache fc = fill_fcache(fcache, wi, J, tabs, gs);
     int cnt[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     long long base[9];
     if (MODE == 1)
@@ -779,20 +837,15 @@ __global__ void __launch_bounds__(256) k_enum_mixed(int nw, const WorkItem *work
             const int dx = c % (2 * ux + 1) - ux;
             const int dy = (c / (2 * ux + 1)) % (2 * uy + 1) - uy;
             const int dz = c / ((2 * ux + 1) * (2 * uy + 1)) - uz;
-            const int fx = wi.l[0] - dx, fy = wi.l[1] - dy, fz = wi.l[2] - dz;
-            if (fx < 0 || fy < 0 || fz < 0 || fx >= J.lmax[0] || fy >= J.lmax[1] || fz >= J.lmax[2]) continue;
-            const uint64_t key = dnode_key(J.depth, fx, fy, fz);
-            int f = dhash_find(J.inHash, key);
+            const int f = lookup_f(fcache, fc, wi, J, gs, dx, dy, dz);
+            if (f == INT_MIN) continue;
             double fn[8];
             int nft = 8;
             if (f >= 0) {
                 for (int q = 0; q < 8; q++) fn[q] = J.inNorm[(size_t)f * 8 + q];
             } else {
-                const int gi = dhash_find(J.genHash, key);
-                if (gi < 0) continue;
-                fn[0] = gs.norm[gi];
+                fn[0] = gs.norm[-f - 1];
                 nft = 1;
-                f = -(gi + 1);
             }
             for (int ft = 0; ft < nft; ft++) {
                 const double fv = fn[ft];
@@ -886,7 +939,7 @@ __global__ void __launch_bounds__(256) k_contract_mfma8(int ntask, const SliceTa
     constexpr int S1 = 68, S2 = 132;
     __shared__ float lds1[4][16 * S1];
     __shared__ float lds2[4][16 * S2];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = wave_id(), lane = threadIdx.x & 63;
     const int ti = blockIdx.x * 4 + wave;
     if (ti >= ntask) return;
     const SliceTask task = tasks[ti];
@@ -904,15 +957,15 @@ __global__ void __launch_bounds__(256) k_contract_mfma8(int ntask, const SliceTa
     for (int it = 0; it < task.count; it++) {
         const Slice sl = slices[task.start + it];
         const TermTab &tb = tabs[J.tab0 + sl.term];
-        const float *B32 = J.operBlocks32 + tb.blk;
+        const gfloat *B32 = (const gfloat *)J.operBlocks32 + tb.blk;
         const int ft = sl.ft;
         const int fx = ft & 1, fy = (ft >> 1) & 1, fz = (ft >> 2) & 1;
         // A fragments: row m = (g_d, a), column i = 4 s + q
         float ax[2], ay[2], az[2];
         {
-            const float *px = B32 + ((sl.dx + tb.band) * 4 + 2 * gxm + fx) * K2 + am * K;
-            const float *py = B32 + ((sl.dy + tb.band) * 4 + 2 * gxm + fy) * K2 + am * K;
-            const float *pz = B32 + ((sl.dz + tb.band) * 4 + 2 * gxm + fz) * K2 + am * K;
+            const gfloat *px = B32 + ((sl.dx + tb.band) * 4 + 2 * gxm + fx) * K2 + am * K;
+            const gfloat *py = B32 + ((sl.dy + tb.band) * 4 + 2 * gxm + fy) * K2 + am * K;
+            const gfloat *pz = B32 + ((sl.dz + tb.band) * 4 + 2 * gxm + fz) * K2 + am * K;
 #pragma unroll
             for (int s2 = 0; s2 < 2; s2++) {
                 ax[s2] = px[4 * s2 + q];
@@ -920,12 +973,13 @@ __global__ void __launch_bounds__(256) k_contract_mfma8(int ntask, const SliceTa
                 az[s2] = pz[4 * s2 + q];
             }
         }
-        const double *f = sl.f >= 0 ? J.inCoef + (size_t)sl.f * 8 * K3 + ft * K3 : genPool + (size_t)(-sl.f - 1) * K3;
+        const gdouble *f = sl.f >= 0 ? (const gdouble *)J.inCoef + (size_t)sl.f * 8 * K3 + ft * K3
+                                     : (const gdouble *)genPool + (size_t)(-sl.f - 1) * K3;
         // pass x: C1[(gx,a)][n = l*8 + j] = sum_i X[a][i] f[n*8 + i]
 #pragma unroll
         for (int t = 0; t < 4; t++) {
             floatx4 c = {0.f, 0.f, 0.f, 0.f};
-            const double *fr = f + (16 * t + m) * K;
+            const gdouble *fr = f + (16 * t + m) * K;
 #pragma unroll
             for (int s2 = 0; s2 < 2; s2++) {
                 const float b = (float)fr[4 * s2 + q];

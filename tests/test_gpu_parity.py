@@ -153,7 +153,7 @@ def test_batch_equals_single(G, oracle):
 def test_reference_coulomb_values_on_gpu(G, A):
     """tests/qmoperators/coulomb_operator.cpp:76-82 at its settings, through the C ABI."""
     prec = 1e-3
-    mra = G.mra_from(configs.BOX32, 5, "I", max_depth=20)
+    mra = G.mra_from(configs.BOX32, 5, "I", max_depth=18)
     orbs = {"1s": (1, 0, 0), "2s": (2, 0, 0), "2px": (2, 1, 0), "2py": (2, 1, 1), "2pz": (2, 1, 2)}
     rho = G.project_hydrogen(prec, G.FunctionTree(mra), [(2.0, q, q) for q in orbs.values()])
     J = G.FunctionTree(mra)
