@@ -59,6 +59,7 @@ def run_variant(G, mra, trees, ops, prec, flags, steps, warmup):
     flops = 0.0
     t_con = 0.0
     tuples = 0
+    slices = 0
     t0 = time.perf_counter()
     for _ in range(steps):
         outs = [G.FunctionTree(mra) for _ in range(n)]
@@ -66,12 +67,36 @@ def run_variant(G, mra, trees, ops, prec, flags, steps, warmup):
         nodes += sum(s["nodes_out"] for s in st)
         flops += sum(s["flops"] for s in st)
         tuples += sum(s["tuples"] for s in st)
+        slices += sum(s["slices"] for s in st)
         t_con += st[0]["t_contract"]
     mra.ctx.sync()
     dt = time.perf_counter() - t0
     return dict(ms_per_step=1e3 * dt / steps, nodes_per_step=nodes / steps, nodes_per_s=nodes / dt,
-                flops_per_step=flops / steps, tuples_per_step=tuples / steps, t_contract_per_step=t_con / steps,
-                outs=outs)
+                flops_per_step=flops / steps, tuples_per_step=tuples / steps, slices_per_step=slices / steps,
+                t_contract_per_step=t_con / steps, iters=st[0]["iters"],
+                predicted=st[0]["predicted_splits"], wasted=st[0]["predicted_wasted"], outs=outs)
+
+
+def train_predictor(G, configs, mra, args):
+    """Refinement traces from plain applies on other seeded orbitals (Helmholtz, as the
+    workload, and Poisson on pair products), then GPU training of the MLP."""
+    t0 = time.perf_counter()
+    G.trace_enable(True)
+    for seed in (101, 102):
+        orbs, nuc, mus = configs.orbitals(seed, n_orb=args.orbitals)
+        trees = [G.project(args.prec, G.FunctionTree(mra), o) for o in orbs]
+        ops = [G.HelmholtzOperator(mra, mu, args.prec) for mu in mus]
+        G.apply_batch(args.prec, [G.FunctionTree(mra) for _ in trees], ops, trees)
+        pairs = [G.multiply(args.prec, G.FunctionTree(mra), 1.0, trees[i], trees[j])
+                 for i in range(len(trees)) for j in range(i, min(i + 2, len(trees)))]
+        P = G.PoissonOperator(mra, args.prec)
+        G.apply_batch(args.prec, [G.FunctionTree(mra) for _ in pairs], [P] * len(pairs), pairs)
+    X, y = G.trace_get()
+    G.trace_enable(False)
+    pred = G.Predictor(seed=0)
+    pred.train(X, y, epochs=150, batch=512, lr=3e-3, seed=0)
+    pred.install(args.pred_threshold)
+    return pred, len(y), time.perf_counter() - t0
 
 
 def cpu_baseline(orbs, mus, k, prec, budget_s=15.0):
@@ -107,6 +132,8 @@ def main():
     ap.add_argument("--orbitals", type=int, default=5)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-predict", action="store_true", help="skip the learned-predictor variant")
+    ap.add_argument("--pred-threshold", type=float, default=0.5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -134,6 +161,20 @@ def main():
             f"contraction {1e3 * res32['t_contract_per_step']:.2f} ms/step, rel L2 err vs fp64 {err:.2e}")
         if res32["nodes_per_s"] > res64["nodes_per_s"] and err < args.prec:
             best, dtype = res32, "f32"
+    resP = None
+    if not args.no_predict:
+        pred, ntrain, t_train = train_predictor(G, configs, mra, args)
+        flags = (G.MWG_APPLY_MIXED if args.k == 7 else G.MWG_APPLY_FP64) | G.MWG_APPLY_PREDICT
+        resP = run_variant(G, mra, trees, ops, args.prec, flags, args.steps, args.warmup)
+        errP = max(_l2diff(a, b) / math.sqrt(b.getSquareNorm()) for a, b in zip(resP["outs"], res64["outs"]))
+        resP["rel_err_vs_fp64"] = errP
+        resP["train_samples"] = ntrain
+        resP["train_s"] = t_train
+        log(f"[bench] mixed+predictor: {resP['ms_per_step']:.2f} ms/step, {resP['nodes_per_s']:.1f} nodes/s, "
+            f"iters {resP['iters']} (fp64 {res64['iters']}), rel L2 err vs fp64 {errP:.2e}, "
+            f"trained on {ntrain} trace samples in {t_train:.1f} s")
+        if resP["nodes_per_s"] > best["nodes_per_s"] and errP < args.prec:
+            best, dtype = resP, ("f32" if args.k == 7 else "f64")
 
     achieved = best["flops_per_step"] / best["t_contract_per_step"] if best["t_contract_per_step"] > 0 else 0.0
     roofline = dict(bound="mfma", achieved=achieved / 1e12, peak=PEAK[dtype] / 1e12, unit="TFLOP/s",
@@ -166,7 +207,14 @@ def main():
             "fp64_ms_per_step": res64["ms_per_step"],
             "mixed_nodes_per_s": res32["nodes_per_s"] if res32 else None,
             "mixed_rel_l2_err_vs_fp64": res32["rel_err_vs_fp64"] if res32 else None,
-            "mixed_fp64_tuples_per_step": None,
+            "mixed_fp64_tuples_per_step": res32["tuples_per_step"] if res32 else None,
+            "mixed_slices_per_step": res32["slices_per_step"] if res32 else None,
+            "predict_nodes_per_s": resP["nodes_per_s"] if resP else None,
+            "predict_rel_l2_err_vs_fp64": resP["rel_err_vs_fp64"] if resP else None,
+            "predict_iters": resP["iters"] if resP else None, "fp64_iters": res64["iters"],
+            "predict_speculative_nodes": resP["predicted"] if resP else None,
+            "predict_speculative_wasted": resP["wasted"] if resP else None,
+            "predict_train_samples": resP["train_samples"] if resP else None,
             "operator_build_s": t_oper, "projection_s": t_proj,
             "contraction_ms_per_step": 1e3 * best["t_contract_per_step"],
         },

@@ -36,6 +36,7 @@ typedef struct mwg_ctx_s *mwg_ctx;
 typedef struct mwg_mra_s *mwg_mra;
 typedef struct mwg_tree_s *mwg_tree;
 typedef struct mwg_oper_s *mwg_oper;
+typedef struct mwg_predictor_s *mwg_predictor;
 
 /* status codes */
 #define MWG_OK 0
@@ -60,7 +61,8 @@ typedef struct mwg_apply_stats {
     int64_t gen_nodes;       /* generated (scaling-only) input nodes */
     int64_t iters;           /* refinement iterations */
     int64_t slices;          /* FP32 MFMA slices (g, f, ft, term: all 8 gt at once), mixed mode */
-    int64_t predicted_splits;/* nodes split ahead of the norm test by the predictor */
+    int64_t predicted_splits;/* output nodes computed ahead of their parent's norm test (predictor) */
+    int64_t predicted_wasted;/* of which discarded because the norm test did not split the parent */
     double t_apply;          /* seconds: work-list build + contraction + refine + transforms */
     double t_contract;       /* seconds spent in contraction kernels (HIP events) */
     double flops;            /* flops issued: FP64 tuples * 6 (k+1)^4 + slices * 2*2(k+1)*(k+1)*7(k+1)^2 */
@@ -132,6 +134,28 @@ int mwg_apply(double prec, mwg_tree out, mwg_oper oper, mwg_tree inp, int max_it
  * one work list and one contraction launch per refinement level for all of them. */
 int mwg_apply_batch(int n, const double *prec, mwg_tree *out, mwg_oper *oper, mwg_tree *inp, int max_iter,
                     unsigned flags, mwg_apply_stats *stats);
+
+/* ---- learned refinement predictor (no counterpart in the reference: MRCPP refines one level
+ * per iteration, TreeBuilder/WaveletAdaptor).  A 2 x 64 tanh MLP (FP32) trained on refinement
+ * traces predicts which freshly created output nodes the norm test will split; with
+ * MWG_APPLY_PREDICT their children are computed in the same iteration and kept only when the
+ * norm test (still run on every node) splits the parent.  Features: see mrchem_amd/csrc/mwg_predict.h. */
+int mwg_predictor_create(mwg_ctx ctx, uint64_t seed, mwg_predictor *out);
+int mwg_predictor_destroy(mwg_predictor pred);
+int mwg_predictor_sizes(int *nparams, int *nfeatures);
+int mwg_predictor_get(mwg_predictor pred, float *params, float *mean, float *sdev);
+int mwg_predictor_set(mwg_predictor pred, const float *params, const float *mean, const float *sdev);
+/* mini-batch Adam on (X[n][nfeatures], y[n] in {0,1}); standardisation from X; mean BCE before/after */
+int mwg_predictor_train(mwg_predictor pred, int64_t n, const float *X, const float *y, int epochs, int batch,
+                        float lr, uint64_t seed, double *loss_before, double *loss_after);
+/* gradient of the mean BCE over all n samples with the current parameters and standardisation */
+int mwg_predictor_gradient(mwg_predictor pred, int64_t n, const float *X, const float *y, float *grad);
+int mwg_predictor_infer(mwg_predictor pred, int64_t n, const float *X, float *prob);
+int mwg_ctx_set_predictor(mwg_ctx ctx, mwg_predictor pred, float threshold);
+/* refinement traces: features of every node created by a split and its own split decision */
+int mwg_trace_enable(mwg_ctx ctx, int on);
+int mwg_trace_size(mwg_ctx ctx, int64_t *n);
+int mwg_trace_get(mwg_ctx ctx, float *X, float *y);
 
 #ifdef __cplusplus
 }

@@ -21,7 +21,7 @@ MWG_APPLY_PREDICT = 2
 class ApplyStats(C.Structure):
     _fields_ = [("nodes_out", C.c_int64), ("tuples", C.c_int64), ("tuples_fp64", C.c_int64),
                 ("gen_nodes", C.c_int64), ("iters", C.c_int64), ("slices", C.c_int64),
-                ("predicted_splits", C.c_int64),
+                ("predicted_splits", C.c_int64), ("predicted_wasted", C.c_int64),
                 ("t_apply", C.c_double), ("t_contract", C.c_double), ("flops", C.c_double)]
 
     def as_dict(self):
@@ -43,6 +43,7 @@ def _sig():
     fd = np.ctypeslib.ndpointer(dtype=np.float64, flags="C")
     fi = np.ctypeslib.ndpointer(dtype=np.int32, flags="C")
     fl = np.ctypeslib.ndpointer(dtype=np.int64, flags="C")
+    ff = np.ctypeslib.ndpointer(dtype=np.float32, flags="C")
     pvp = C.POINTER(C.c_void_p)
     return {
         "mwg_ctx_create": [i, pvp],
@@ -72,6 +73,19 @@ def _sig():
         "mwg_add": [vp, d, vp, d, vp],
         "mwg_apply": [d, vp, vp, vp, i, u, C.POINTER(ApplyStats)],
         "mwg_apply_batch": [i, fd, pvp, pvp, pvp, i, u, C.POINTER(ApplyStats)],
+        "mwg_predictor_create": [vp, C.c_uint64, pvp],
+        "mwg_predictor_destroy": [vp],
+        "mwg_predictor_sizes": [C.POINTER(C.c_int), C.POINTER(C.c_int)],
+        "mwg_predictor_get": [vp, ff, ff, ff],
+        "mwg_predictor_set": [vp, ff, ff, ff],
+        "mwg_predictor_train": [vp, i64, ff, ff, i, i, C.c_float, C.c_uint64, C.POINTER(C.c_double),
+                                C.POINTER(C.c_double)],
+        "mwg_predictor_gradient": [vp, i64, ff, ff, ff],
+        "mwg_predictor_infer": [vp, i64, ff, ff],
+        "mwg_ctx_set_predictor": [vp, vp, C.c_float],
+        "mwg_trace_enable": [vp, i],
+        "mwg_trace_size": [vp, C.POINTER(C.c_int64)],
+        "mwg_trace_get": [vp, ff, ff],
     }
 
 
@@ -80,7 +94,9 @@ EXPORTED = ["mwg_ctx_create", "mwg_ctx_destroy", "mwg_last_error", "mwg_sync", "
             "mwg_tree_import", "mwg_tree_export", "mwg_tree_rescale", "mwg_tree_dot", "mwg_tree_eval",
             "mwg_project_gauss", "mwg_project_hydrogen", "mwg_oper_poisson", "mwg_oper_helmholtz",
             "mwg_oper_destroy", "mwg_oper_nterms", "mwg_oper_terms", "mwg_oper_blocks", "mwg_multiply", "mwg_add", "mwg_apply",
-            "mwg_apply_batch"]
+            "mwg_apply_batch", "mwg_predictor_create", "mwg_predictor_destroy", "mwg_predictor_sizes",
+            "mwg_predictor_get", "mwg_predictor_set", "mwg_predictor_train", "mwg_predictor_gradient",
+            "mwg_predictor_infer", "mwg_ctx_set_predictor", "mwg_trace_enable", "mwg_trace_size", "mwg_trace_get"]
 
 
 def lib():

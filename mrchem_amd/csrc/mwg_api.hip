@@ -27,6 +27,7 @@
 #include "../../include/mwg.h"
 #include "mwg_core.h"
 #include "mwg_kernels.h"
+#include "mwg_predict.h"
 
 namespace mwg {
 
@@ -121,6 +122,11 @@ struct Ctx {
     DBuf<double> partials, nodeslots;
     DBuf<Slice> slices;
     DBuf<SliceTask> stasks;
+    // refinement traces (features, split label) and the learned predictor
+    bool tracing = false;
+    std::vector<float> trace_X, trace_y;
+    struct PredH *pred = nullptr;
+    float pred_thr = 0.5f;
     int chunk = 64;        // tuples per FP64 contraction task
     int slice_chunk = 16;  // slices per MFMA task
 };
@@ -544,7 +550,49 @@ struct JobH {
     int64_t tuples = 0, slices = 0;
     size_t genOff = 0;
     uint32_t genMask = 0;
+    std::vector<char> spec, alive, has_feat;  // per output node
+    std::vector<float> feat;                  // per output node, MLP_FEATURES each
 };
+
+// features of a freshly created child c of node p (see mwg_predict.h)
+static void child_features(const JobH &J, const MraH &M, int p, int c, double tnorm, float *f) {
+    const TreeH &T = *J.out;
+    const double *pn = &T.cnorm[8 * (size_t)p];
+    double w2 = 0.0;
+    for (int q = 1; q < 8; q++) w2 += pn[q] * pn[q];
+    const double wn = std::sqrt(w2), sn = pn[0];
+    const int np = M.n0 + T.depth[p];
+    const double thr = std::max(2.0 * 2.220446049250313e-16, J.prec * tnorm * std::pow(2.0, -0.5 * (np + 1)));
+    auto lg = [](double v) { return (float)std::max(-20.0, std::min(20.0, std::log10(std::max(v, 1e-300)))); };
+    f[0] = lg(wn / thr);
+    f[1] = lg(wn / tnorm);
+    f[2] = lg(sn / tnorm);
+    const int d = T.depth[c];
+    f[3] = (float)(M.n0 + d);
+    const double inn = std::sqrt(std::max(J.in->sqnorm, 1e-300));
+    const int x = T.lr[3 * c], y = T.lr[3 * c + 1], z = T.lr[3 * c + 2];
+    auto in_w = [&](int i) {
+        double a = 0.0;
+        for (int q = 1; q < 8; q++) a += J.in->cnorm[8 * (size_t)i + q] * J.in->cnorm[8 * (size_t)i + q];
+        return std::sqrt(a);
+    };
+    const int i0 = J.in->find(d, x, y, z);
+    f[4] = i0 >= 0 ? 1.0f : 0.0f;
+    f[5] = i0 >= 0 ? lg(in_w(i0) / inn) : -20.0f;
+    f[6] = (i0 >= 0 && J.in->child[i0] >= 0) ? 1.0f : 0.0f;
+    double nb = 0.0;
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                if (!dx && !dy && !dz) continue;
+                if (x + dx < 0 || y + dy < 0 || z + dz < 0) continue;
+                const int i = J.in->find(d, x + dx, y + dy, z + dz);
+                if (i >= 0) nb = std::max(nb, in_w(i));
+            }
+    f[7] = nb > 0.0 ? lg(nb / inn) : -20.0f;
+    f[8] = (float)std::log10(J.op->mu + 1e-3);
+    f[9] = (float)std::log10(J.prec);
+}
 
 struct Box3 {
     short m[3];
@@ -617,6 +665,117 @@ static void job_boxes(const JobH &J, int d, std::vector<TermTab> &tabs, int &tab
     }
     ntab = (int)tabs.size() - tab0;
     if (ntab > 255) throw Error(MWG_ERR_RANGE, "more than 255 active terms at one depth");
+}
+
+
+// ------------------------------------------------------------------ learned predictor
+
+struct PredH {
+    Ctx *ctx = nullptr;
+    DBuf<float> P, m, v, G, mean, sdev;
+    DBuf<float> X, Y, prob, Xn, H1, H2, dZ1, dZ2, dZ3;
+    DBuf<int> idx;
+    int adam_t = 0;
+};
+
+static void predictor_infer(PredH &Pr, int n, const float *X, float *prob) {
+    Ctx &C = *Pr.ctx;
+    Pr.X.alloc((size_t)n * MLP_FEATURES);
+    HC(hipMemcpyAsync(Pr.X.p, X, (size_t)n * MLP_FEATURES * sizeof(float), hipMemcpyHostToDevice, C.stream));
+    Pr.prob.alloc(n);
+    HC(launch_mlp_forward(C.stream, n, nullptr, Pr.X.p, Pr.mean.p, Pr.sdev.p, Pr.P.p, Pr.prob.p, nullptr, nullptr,
+                          nullptr));
+    HC(hipMemcpyAsync(prob, Pr.prob.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, C.stream));
+    HC(hipStreamSynchronize(C.stream));
+    C.stage.off = 0;
+}
+
+// forward + backward over samples idx[0..n) of the resident set (idx null: all); gradient in Pr.G
+static float predictor_grad(PredH &Pr, int n, const int *didx) {
+    Ctx &C = *Pr.ctx;
+    hipStream_t s = C.stream;
+    Pr.prob.alloc(n);
+    Pr.Xn.alloc((size_t)n * MLP_FEATURES);
+    Pr.H1.alloc((size_t)n * MLP_HIDDEN);
+    Pr.H2.alloc((size_t)n * MLP_HIDDEN);
+    Pr.dZ1.alloc((size_t)n * MLP_HIDDEN);
+    Pr.dZ2.alloc((size_t)n * MLP_HIDDEN);
+    Pr.dZ3.alloc(n);
+    HC(launch_mlp_forward(s, n, didx, Pr.X.p, Pr.mean.p, Pr.sdev.p, Pr.P.p, Pr.prob.p, Pr.Xn.p, Pr.H1.p, Pr.H2.p));
+    HC(launch_mlp_backward(s, n, didx, Pr.Y.p, Pr.P.p, Pr.prob.p, Pr.H1.p, Pr.H2.p, Pr.dZ1.p, Pr.dZ2.p, Pr.dZ3.p));
+    HC(launch_mlp_grad(s, n, Pr.Xn.p, Pr.H1.p, Pr.H2.p, Pr.dZ1.p, Pr.dZ2.p, Pr.dZ3.p, Pr.G.p));
+    return 0.0f;
+}
+
+// mean binary cross-entropy of the resident set (host-side from the device probabilities)
+static double predictor_loss(PredH &Pr, int n, const float *y) {
+    Ctx &C = *Pr.ctx;
+    Pr.prob.alloc(n);
+    HC(launch_mlp_forward(C.stream, n, nullptr, Pr.X.p, Pr.mean.p, Pr.sdev.p, Pr.P.p, Pr.prob.p, nullptr, nullptr,
+                          nullptr));
+    std::vector<float> pr(n);
+    Pr.prob.download(pr.data(), n, C.stream);
+    double L = 0.0;
+    for (int i = 0; i < n; i++) {
+        const double p = std::min(std::max((double)pr[i], 1e-7), 1.0 - 1e-7);
+        L -= y[i] > 0.5f ? std::log(p) : std::log(1.0 - p);
+    }
+    return L / std::max(n, 1);
+}
+
+// ------------------------------------------------------------------ compaction
+
+__global__ void k_gather_nodes(int n, const int *src, const double *in, double *out, int per) {
+    const int i = blockIdx.x;
+    if (i >= n) return;
+    const double *a = in + (size_t)src[i] * per;
+    double *b = out + (size_t)i * per;
+    for (int e = threadIdx.x; e < per; e += blockDim.x) b[e] = a[e];
+}
+
+// removes the nodes with alive == 0 (whole sibling groups already unlinked from their parents)
+static void compact_tree(TreeH &T, const std::vector<char> &alive) {
+    Ctx &C = *T.mra->ctx;
+    const int n = T.n();
+    std::vector<int> newid(n, -1), src;
+    for (int i = 0; i < n; i++)
+        if (alive[i]) {
+            newid[i] = (int)src.size();
+            src.push_back(i);
+        }
+    const int m = (int)src.size();
+    std::vector<int> depth(m), parent(m), child(m), lr(3 * (size_t)m);
+    std::vector<double> cn(8 * (size_t)m);
+    for (int k = 0; k < m; k++) {
+        const int i = src[k];
+        depth[k] = T.depth[i];
+        parent[k] = T.parent[i] >= 0 ? newid[T.parent[i]] : -1;
+        child[k] = T.child[i] >= 0 ? newid[T.child[i]] : -1;
+        for (int q = 0; q < 3; q++) lr[3 * k + q] = T.lr[3 * i + q];
+        std::memcpy(&cn[8 * (size_t)k], &T.cnorm[8 * (size_t)i], 8 * sizeof(double));
+    }
+    DBuf<int> dsrc;
+    dsrc.upload(src.data(), src.size(), C.stream);
+    DBuf<double> nc, nn;
+    nc.alloc((size_t)m * 8 * T.K3());
+    nn.alloc((size_t)m * 8);
+    hipLaunchKernelGGL(k_gather_nodes, dim3(m), dim3(256), 0, C.stream, m, dsrc.p, T.coef.p, nc.p, 8 * T.K3());
+    hipLaunchKernelGGL(k_gather_nodes, dim3(m), dim3(8), 0, C.stream, m, dsrc.p, T.norm.p, nn.p, 8);
+    HC(hipGetLastError());
+    HC(hipStreamSynchronize(C.stream));
+    std::swap(T.coef.p, nc.p);
+    std::swap(T.coef.n, nc.n);
+    std::swap(T.norm.p, nn.p);
+    std::swap(T.norm.n, nn.n);
+    T.depth.swap(depth);
+    T.parent.swap(parent);
+    T.child.swap(child);
+    T.lr.swap(lr);
+    T.cnorm.swap(cn);
+    T.index.clear();
+    for (int k = 0; k < m; k++) T.index[node_key(T.depth[k], T.lr[3 * k], T.lr[3 * k + 1], T.lr[3 * k + 2])] = k;
+    for (int &r : T.roots) r = newid[r];
+    T.hvalid = false;
 }
 
 struct PhaseTimer {
@@ -702,42 +861,62 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
     PT.lap("setup");
 
     int iter = 0;
+    int64_t predicted = 0, wasted = 0;
+    const bool use_pred = (flags & MWG_APPLY_PREDICT) && C.pred != nullptr;
+    for (JobH &J : jobs) {
+        J.spec.assign(J.out->n(), 0);
+        J.alive.assign(J.out->n(), 1);
+        if (C.tracing || use_pred) J.feat.assign((size_t)J.out->n() * MLP_FEATURES, 0.0f);
+        J.has_feat.assign(J.out->n(), 0);
+    }
     while (true) {
         std::vector<int> active;
         for (int j = 0; j < (int)jobs.size(); j++)
             if (!jobs[j].work.empty()) active.push_back(j);
         if (active.empty()) break;
-        const int d = iter;
+        // one job table per (job, depth) present in the frontier
         std::vector<TermTab> tabs;
-        std::vector<JobTab> jt(jobs.size());
+        std::vector<JobTab> jt;
+        std::vector<int> jt_job;
         std::vector<WorkItem> work;
         for (int j : active) {
             JobH &J = jobs[j];
-            J.op->ensure(d);
-            JobTab &T = jt[j];
-            std::memset(&T, 0, sizeof(T));
-            job_boxes(J, d, tabs, T.tab0, T.ntab, T.thr);
             J.out->ensure_dev(J.out->n());
-            T.inHash = J.in->dhash();
-            T.genHash = DHash{C.genHashKeys.p + J.genOff, C.genHashVals.p + J.genOff, J.genMask};
-            T.inCoef = J.in->coef.p;
-            T.inNorm = J.in->norm.p;
-            T.operBlocks = J.op->lev[d]->blocks.p;
-            if (mixed) {
+            std::vector<int> depths;
+            for (int g : J.work) depths.push_back(J.out->depth[g]);
+            std::sort(depths.begin(), depths.end());
+            depths.erase(std::unique(depths.begin(), depths.end()), depths.end());
+            std::vector<int> tab_of(M.maxDepth + 2, -1);
+            for (int d : depths) {
+                J.op->ensure(d);
+                JobTab T;
+                std::memset(&T, 0, sizeof(T));
+                job_boxes(J, d, tabs, T.tab0, T.ntab, T.thr);
+                T.inHash = J.in->dhash();
+                T.genHash = DHash{C.genHashKeys.p + J.genOff, C.genHashVals.p + J.genOff, J.genMask};
+                T.inCoef = J.in->coef.p;
+                T.inNorm = J.in->norm.p;
                 LevelH &LV = *J.op->lev[d];
-                if (!LV.has32) {
-                    LV.blocks32.alloc(LV.blocks.n);
-                    HC(launch_to_f32(s, LV.blocks.n, LV.blocks.p, LV.blocks32.p));
-                    LV.has32 = true;
+                T.operBlocks = LV.blocks.p;
+                if (mixed) {
+                    if (!LV.has32) {
+                        LV.blocks32.alloc(LV.blocks.n);
+                        HC(launch_to_f32(s, LV.blocks.n, LV.blocks.p, LV.blocks32.p));
+                        LV.has32 = true;
+                    }
+                    T.operBlocks32 = LV.blocks32.p;
                 }
-                T.operBlocks32 = LV.blocks32.p;
+                T.operNorms = LV.norms.p;
+                T.outCoef = J.out->coef.p;
+                T.outNorm = J.out->norm.p;
+                for (int q = 0; q < 3; q++) T.lmax[q] = M.boxes[q] << d;
+                T.depth = d;
+                tab_of[d] = (int)jt.size();
+                jt.push_back(T);
+                jt_job.push_back(j);
             }
-            T.operNorms = J.op->lev[d]->norms.p;
-            T.outCoef = J.out->coef.p;
-            T.outNorm = J.out->norm.p;
-            for (int q = 0; q < 3; q++) T.lmax[q] = M.boxes[q] << d;
-            T.depth = d;
-            for (int g : J.work) work.push_back({j, g, {J.out->lr[3 * g], J.out->lr[3 * g + 1], J.out->lr[3 * g + 2]}});
+            for (int g : J.work)
+                work.push_back({tab_of[J.out->depth[g]], g, {J.out->lr[3 * g], J.out->lr[3 * g + 1], J.out->lr[3 * g + 2]}});
         }
         const int nw = (int)work.size();
         PT.lap("opers+boxes");
@@ -786,7 +965,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                 for (int q = 0; q < 8; q++) {
                     const int c = tot[(size_t)i * 8 + q];
                     ofs[(size_t)i * 8 + q] = total;
-                    jobs[work[i].job].tuples += c;
+                    jobs[jt_job[work[i].job]].tuples += c;
                     const int nch = c <= C.chunk ? 1 : (c + C.chunk - 1) / C.chunk;
                     if (nch == 1) {
                         tasks.push_back({total, c, i, q, -1});
@@ -830,7 +1009,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             for (int i = 0; i < nw; i++) {
                 const int cs = tot[(size_t)i * 9 + 8];
                 ofs[(size_t)i * 9 + 8] = stotal;
-                jobs[work[i].job].slices += cs;
+                jobs[jt_job[work[i].job]].slices += cs;
                 const int nn0 = nnslots;
                 for (int h = 0; h * C.slice_chunk < cs; h++) {
                     const int cc = std::min(C.slice_chunk, cs - h * C.slice_chunk);
@@ -840,7 +1019,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                 for (int q = 0; q < 8; q++) {
                     const int c = tot[(size_t)i * 9 + q];
                     ofs[(size_t)i * 9 + q] = total;
-                    jobs[work[i].job].tuples += c;
+                    jobs[jt_job[work[i].job]].tuples += c;
                     const int s0 = nslots;
                     for (int h = 0; h * C.chunk < c; h++) {
                         const int cc = std::min(C.chunk, c - h * C.chunk);
@@ -882,36 +1061,111 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         t_contract_ms += ms;
         PT.lap("contract+norms");
         for (int i = 0; i < nw; i++) {
-            JobH &J = jobs[work[i].job];
+            JobH &J = jobs[jt_job[work[i].job]];
             std::memcpy(&J.out->cnorm[8 * (size_t)work[i].node], &wn[8 * (size_t)i], 8 * sizeof(double));
         }
+        // split tests level by level (the plain method's order): the running norm of a
+        // level includes every computed node of that level and above; speculative
+        // children are kept only when the norm test splits their parent
+        std::vector<int> cand_job, cand_node;
         for (int j : active) {
             JobH &J = jobs[j];
-            for (int g : J.work) {
-                const double *p = &J.out->cnorm[8 * (size_t)g];
-                if (iter == 0) J.sNorm += p[0] * p[0];
-                for (int c = 1; c < 8; c++) J.wNorm += p[c] * p[c];
-            }
-            J.gsq = J.sNorm + J.wNorm;
-        }
-        for (int j : active) {
-            JobH &J = jobs[j];
-            const double tnorm = std::sqrt(J.gsq);
+            std::vector<int> cur = J.work;
+            std::stable_sort(cur.begin(), cur.end(), [&](int a, int b) { return J.out->depth[a] < J.out->depth[b]; });
             std::vector<int> next;
-            for (int g : J.work) {
-                if (d >= M.maxDepth || (max_iter >= 0 && iter >= max_iter)) continue;
-                double w2 = 0.0;
-                const double *p = &J.out->cnorm[8 * (size_t)g];
-                for (int c = 1; c < 8; c++) w2 += p[c] * p[c];
-                if (split_check(M.n0 + d, std::sqrt(w2), J.prec, tnorm)) {
-                    J.out->split(g);
-                    for (int a = 0; a < 8; a++) next.push_back(J.out->child[g] + a);
+            size_t q0 = 0;
+            while (q0 < cur.size()) {
+                const int d = J.out->depth[cur[q0]];
+                size_t q1 = q0;
+                while (q1 < cur.size() && J.out->depth[cur[q1]] == d) q1++;
+                for (size_t q = q0; q < q1; q++) {
+                    const int g = cur[q];
+                    if (!J.alive[g]) continue;
+                    const double *p = &J.out->cnorm[8 * (size_t)g];
+                    if (d == 0) J.sNorm += p[0] * p[0];
+                    for (int c = 1; c < 8; c++) J.wNorm += p[c] * p[c];
                 }
+                J.gsq = J.sNorm + J.wNorm;
+                const double tnorm = std::sqrt(J.gsq);
+                for (size_t q = q0; q < q1; q++) {
+                    const int g = cur[q];
+                    if (!J.alive[g]) continue;
+                    double w2 = 0.0;
+                    const double *p = &J.out->cnorm[8 * (size_t)g];
+                    for (int c = 1; c < 8; c++) w2 += p[c] * p[c];
+                    const bool can = d < M.maxDepth && !(max_iter >= 0 && d >= max_iter);
+                    const bool split = can && split_check(M.n0 + d, std::sqrt(w2), J.prec, tnorm);
+                    if (C.tracing && J.has_feat[g]) {
+                        C.trace_X.insert(C.trace_X.end(), &J.feat[(size_t)g * MLP_FEATURES],
+                                         &J.feat[(size_t)g * MLP_FEATURES] + MLP_FEATURES);
+                        C.trace_y.push_back(split ? 1.0f : 0.0f);
+                    }
+                    if (J.spec[g]) {
+                        J.spec[g] = 0;
+                        if (!split) {
+                            for (int a = 0; a < 8; a++) J.alive[J.out->child[g] + a] = 0;
+                            J.out->child[g] = -1;
+                            wasted += 8;
+                        }
+                        continue;
+                    }
+                    if (split) {
+                        J.out->split(g);
+                        J.spec.resize(J.out->n(), 0);
+                        J.alive.resize(J.out->n(), 1);
+                        J.has_feat.resize(J.out->n(), 0);
+                        if (C.tracing || use_pred) {
+                            J.feat.resize((size_t)J.out->n() * MLP_FEATURES, 0.0f);
+                            for (int a = 0; a < 8; a++) {
+                                const int c = J.out->child[g] + a;
+                                child_features(J, M, g, c, tnorm, &J.feat[(size_t)c * MLP_FEATURES]);
+                                J.has_feat[c] = 1;
+                            }
+                        }
+                        for (int a = 0; a < 8; a++) {
+                            next.push_back(J.out->child[g] + a);
+                            cand_job.push_back(j);
+                            cand_node.push_back(J.out->child[g] + a);
+                        }
+                    }
+                }
+                q0 = q1;
             }
             J.work.swap(next);
         }
+        // learned lookahead: nodes predicted to split get their children computed now
+        if (use_pred && !cand_node.empty()) {
+            const int nc = (int)cand_node.size();
+            std::vector<float> X((size_t)nc * MLP_FEATURES);
+            for (int i = 0; i < nc; i++)
+                std::memcpy(&X[(size_t)i * MLP_FEATURES], &jobs[cand_job[i]].feat[(size_t)cand_node[i] * MLP_FEATURES],
+                            MLP_FEATURES * sizeof(float));
+            std::vector<float> prob(nc);
+            predictor_infer(*C.pred, nc, X.data(), prob.data());
+            for (int i = 0; i < nc; i++) {
+                if (prob[i] < C.pred_thr) continue;
+                JobH &J = jobs[cand_job[i]];
+                const int c = cand_node[i];
+                const int d = J.out->depth[c];
+                if (d >= M.maxDepth || (max_iter >= 0 && d >= max_iter)) continue;
+                J.out->split(c);
+                J.spec.resize(J.out->n(), 0);
+                J.alive.resize(J.out->n(), 1);
+                J.has_feat.resize(J.out->n(), 0);
+                J.spec[c] = 1;
+                for (int a = 0; a < 8; a++) J.work.push_back(J.out->child[c] + a);
+                predicted += 8;
+            }
+        }
         PT.lap("split");
         iter++;
+    }
+    // drop the speculative children that the norm test rejected
+    for (JobH &J : jobs) {
+        bool any_dead = false;
+        for (char a : J.alive)
+            if (!a) any_dead = true;
+        if (any_dead) compact_tree(*J.out, J.alive);
     }
     // top-down additive transform, then bottom-up compression, for all output trees
     std::vector<std::vector<std::vector<int>>> L(jobs.size());
@@ -951,7 +1205,8 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         st.slices = jobs[j].slices;
         st.gen_nodes = genDone;
         st.iters = iter;
-        st.predicted_splits = 0;
+        st.predicted_splits = predicted;
+        st.predicted_wasted = wasted;
         st.t_apply = t_total;
         st.t_contract = t_contract_ms * 1e-3;
         st.flops = fl * (double)jobs[j].tuples + 2.0 * (2 * K) * K * 7.0 * K * K * (double)jobs[j].slices;
@@ -1092,6 +1347,7 @@ struct mwg_ctx_s : Ctx {};
 struct mwg_mra_s : MraH {};
 struct mwg_tree_s : TreeH {};
 struct mwg_oper_s : OperH {};
+struct mwg_predictor_s : PredH {};
 
 static thread_local std::string g_err;
 
@@ -1544,6 +1800,159 @@ int mwg_apply_batch(int n, const double *prec, mwg_tree *out, mwg_oper *oper, mw
 int mwg_apply(double prec, mwg_tree out, mwg_oper oper, mwg_tree inp, int max_iter, unsigned flags,
               mwg_apply_stats *stats) {
     return mwg_apply_batch(1, &prec, &out, &oper, &inp, max_iter, flags, stats);
+}
+
+
+static uint64_t splitmix(uint64_t &st) {
+    st += 0x9E3779B97F4A7C15ull;
+    uint64_t z = st;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+static float urand(uint64_t &st) { return (float)((splitmix(st) >> 40) * (1.0 / 16777216.0)); }
+
+int mwg_predictor_sizes(int *nparams, int *nfeatures) {
+    if (nparams) *nparams = MLP_PARAMS;
+    if (nfeatures) *nfeatures = MLP_FEATURES;
+    return MWG_OK;
+}
+
+int mwg_predictor_create(mwg_ctx ctx, uint64_t seed, mwg_predictor *out) {
+    if (!ctx || !out) return MWG_ERR_ARG;
+    auto *p = new mwg_predictor_s();
+    int rc = guarded(ctx, [&] {
+        p->ctx = ctx;
+        std::vector<float> P(MLP_PARAMS, 0.0f);
+        uint64_t st = seed;
+        const int H = MLP_HIDDEN, NF = MLP_FEATURES;
+        const float a1 = std::sqrt(6.0f / (NF + H)), a2 = std::sqrt(6.0f / (2 * H)), a3 = std::sqrt(6.0f / (H + 1));
+        size_t o = 0;
+        for (int i = 0; i < H * NF; i++) P[o++] = a1 * (2.0f * urand(st) - 1.0f);
+        o += H;
+        for (int i = 0; i < H * H; i++) P[o++] = a2 * (2.0f * urand(st) - 1.0f);
+        o += H;
+        for (int i = 0; i < H; i++) P[o++] = a3 * (2.0f * urand(st) - 1.0f);
+        std::vector<float> mean(NF, 0.0f), sdev(NF, 1.0f), zero(MLP_PARAMS, 0.0f);
+        p->P.upload(P.data(), P.size(), ctx->stream);
+        p->m.upload(zero.data(), zero.size(), ctx->stream);
+        p->v.upload(zero.data(), zero.size(), ctx->stream);
+        p->G.alloc(MLP_PARAMS);
+        p->mean.upload(mean.data(), NF, ctx->stream);
+        p->sdev.upload(sdev.data(), NF, ctx->stream);
+    });
+    if (rc) {
+        delete p;
+        return rc;
+    }
+    *out = p;
+    return MWG_OK;
+}
+
+int mwg_predictor_destroy(mwg_predictor pred) {
+    if (pred && pred->ctx && pred->ctx->pred == pred) pred->ctx->pred = nullptr;
+    delete pred;
+    return MWG_OK;
+}
+
+int mwg_predictor_get(mwg_predictor pred, float *params, float *mean, float *sdev) {
+    if (!pred) return MWG_ERR_ARG;
+    return guarded(pred->ctx, [&] {
+        if (params) pred->P.download(params, MLP_PARAMS, pred->ctx->stream);
+        if (mean) pred->mean.download(mean, MLP_FEATURES, pred->ctx->stream);
+        if (sdev) pred->sdev.download(sdev, MLP_FEATURES, pred->ctx->stream);
+    });
+}
+
+int mwg_predictor_set(mwg_predictor pred, const float *params, const float *mean, const float *sdev) {
+    if (!pred) return MWG_ERR_ARG;
+    return guarded(pred->ctx, [&] {
+        if (params) pred->P.upload(params, MLP_PARAMS, pred->ctx->stream);
+        if (mean) pred->mean.upload(mean, MLP_FEATURES, pred->ctx->stream);
+        if (sdev) pred->sdev.upload(sdev, MLP_FEATURES, pred->ctx->stream);
+    });
+}
+
+int mwg_predictor_gradient(mwg_predictor pred, int64_t n, const float *X, const float *y, float *grad) {
+    if (!pred || n <= 0 || !X || !y || !grad || n > (1 << 26)) return MWG_ERR_ARG;
+    return guarded(pred->ctx, [&] {
+        hipStream_t s = pred->ctx->stream;
+        pred->X.upload(X, (size_t)n * MLP_FEATURES, s);
+        pred->Y.upload(y, (size_t)n, s);
+        predictor_grad(*pred, (int)n, nullptr);
+        pred->G.download(grad, MLP_PARAMS, s);
+    });
+}
+
+int mwg_predictor_infer(mwg_predictor pred, int64_t n, const float *X, float *prob) {
+    if (!pred || n < 0 || (n > 0 && (!X || !prob)) || n > (1 << 26)) return MWG_ERR_ARG;
+    return guarded(pred->ctx, [&] {
+        if (n) predictor_infer(*pred, (int)n, X, prob);
+    });
+}
+
+int mwg_predictor_train(mwg_predictor pred, int64_t n, const float *X, const float *y, int epochs, int batch,
+                        float lr, uint64_t seed, double *loss_before, double *loss_after) {
+    if (!pred || n <= 0 || !X || !y || epochs < 0 || batch <= 0 || n > (1 << 26)) return MWG_ERR_ARG;
+    return guarded(pred->ctx, [&] {
+        hipStream_t s = pred->ctx->stream;
+        const int NF = MLP_FEATURES;
+        std::vector<float> mean(NF, 0.0f), sdev(NF, 0.0f);
+        for (int64_t i = 0; i < n; i++)
+            for (int f = 0; f < NF; f++) mean[f] += X[i * NF + f];
+        for (int f = 0; f < NF; f++) mean[f] /= (float)n;
+        for (int64_t i = 0; i < n; i++)
+            for (int f = 0; f < NF; f++) sdev[f] += (X[i * NF + f] - mean[f]) * (X[i * NF + f] - mean[f]);
+        for (int f = 0; f < NF; f++) sdev[f] = std::max(std::sqrt(sdev[f] / (float)n), 1e-3f);
+        pred->mean.upload(mean.data(), NF, s);
+        pred->sdev.upload(sdev.data(), NF, s);
+        pred->X.upload(X, (size_t)n * NF, s);
+        pred->Y.upload(y, (size_t)n, s);
+        if (loss_before) *loss_before = predictor_loss(*pred, (int)n, y);
+        std::vector<int> order(n);
+        for (int64_t i = 0; i < n; i++) order[i] = (int)i;
+        uint64_t st = seed;
+        for (int e = 0; e < epochs; e++) {
+            for (int64_t i = n - 1; i > 0; i--) std::swap(order[i], order[splitmix(st) % (uint64_t)(i + 1)]);
+            pred->idx.upload(order.data(), order.size(), s);
+            for (int64_t b0 = 0; b0 < n; b0 += batch) {
+                const int nb = (int)std::min<int64_t>(batch, n - b0);
+                predictor_grad(*pred, nb, pred->idx.p + b0);
+                HC(launch_adam(s, pred->P.p, pred->G.p, pred->m.p, pred->v.p, lr, 0.9f, 0.999f, 1e-8f, ++pred->adam_t));
+            }
+        }
+        HC(hipStreamSynchronize(s));
+        if (loss_after) *loss_after = predictor_loss(*pred, (int)n, y);
+    });
+}
+
+int mwg_ctx_set_predictor(mwg_ctx ctx, mwg_predictor pred, float threshold) {
+    if (!ctx) return MWG_ERR_ARG;
+    if (pred && pred->ctx != ctx) return MWG_ERR_ARG;
+    ctx->pred = pred;
+    ctx->pred_thr = threshold;
+    return MWG_OK;
+}
+
+int mwg_trace_enable(mwg_ctx ctx, int on) {
+    if (!ctx) return MWG_ERR_ARG;
+    ctx->tracing = on != 0;
+    ctx->trace_X.clear();
+    ctx->trace_y.clear();
+    return MWG_OK;
+}
+
+int mwg_trace_size(mwg_ctx ctx, int64_t *n) {
+    if (!ctx || !n) return MWG_ERR_ARG;
+    *n = (int64_t)ctx->trace_y.size();
+    return MWG_OK;
+}
+
+int mwg_trace_get(mwg_ctx ctx, float *X, float *y) {
+    if (!ctx) return MWG_ERR_ARG;
+    if (X) std::memcpy(X, ctx->trace_X.data(), ctx->trace_X.size() * sizeof(float));
+    if (y) std::memcpy(y, ctx->trace_y.data(), ctx->trace_y.size() * sizeof(float));
+    return MWG_OK;
 }
 
 }  // extern "C"

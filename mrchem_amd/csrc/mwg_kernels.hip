@@ -594,6 +594,67 @@ __device__ inline int lookup_f(const int *cache, const FCache &fc, const WorkIte
     return probe_f(J, gs, wi.l[0] - dx, wi.l[1] - dy, wi.l[2] - dz);
 }
 
+
+// Flattened candidate walk: every (term, delta) cell of the job's screened boxes is
+// one candidate, spread over all threads of the block (term prefix sums in LDS).
+// For each candidate with an input node the visitor gets the 12 operator norms
+// (4 block types per dimension) and the input component norms.
+struct CandTerms {
+    int start[257];
+};
+
+__device__ inline void term_prefix(CandTerms &ct, const JobTab &J, const TermTab *tabs) {
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int t = 0; t < J.ntab; t++) {
+            ct.start[t] = acc;
+            const TermTab &tb = tabs[J.tab0 + t];
+            if (tb.ub[0] >= 0) acc += (2 * tb.ub[0] + 1) * (2 * tb.ub[1] + 1) * (2 * tb.ub[2] + 1);
+        }
+        ct.start[J.ntab] = acc;
+    }
+    __syncthreads();
+}
+
+template <class Visit>
+__device__ inline void walk_candidates(const CandTerms &ct, const int *fcache, const FCache &fc, const WorkItem &wi,
+                                       const JobTab &J, const TermTab *tabs, const GenState &gs, Visit &&visit) {
+    const int total = ct.start[J.ntab];
+    for (int k = threadIdx.x; k < total; k += blockDim.x) {
+        int lo = 0, hi = J.ntab - 1;  // last t with start[t] <= k
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (ct.start[mid] <= k) lo = mid;
+            else hi = mid - 1;
+        }
+        const int t = lo;
+        const TermTab &tb = tabs[J.tab0 + t];
+        const int c = k - ct.start[t];
+        const int ux = tb.ub[0], uy = tb.ub[1], uz = tb.ub[2];
+        const int dx = c % (2 * ux + 1) - ux;
+        const int dy = (c / (2 * ux + 1)) % (2 * uy + 1) - uy;
+        const int dz = c / ((2 * ux + 1) * (2 * uy + 1)) - uz;
+        const int f = lookup_f(fcache, fc, wi, J, gs, dx, dy, dz);
+        if (f == INT_MIN) continue;
+        double fn[8];
+        int nft = 8;
+        if (f >= 0) {
+            for (int q = 0; q < 8; q++) fn[q] = J.inNorm[(size_t)f * 8 + q];
+        } else {
+            fn[0] = gs.norm[-f - 1];
+            nft = 1;
+        }
+        const double *nrm = J.operNorms + tb.nrm;
+        double nx[4], ny[4], nz[4];
+        for (int q = 0; q < 4; q++) {
+            nx[q] = nrm[(dx + tb.band) * 4 + q];
+            ny[q] = nrm[(dy + tb.band) * 4 + q];
+            nz[q] = nrm[(dz + tb.band) * 4 + q];
+        }
+        visit(t, tb, dx, dy, dz, f, fn, nft, nx, ny, nz);
+    }
+}
+
 // ----------------------------------------------------------------- apply: work list
 
 // Enumerates, for one output node, every (term, delta, ft, gt) candidate inside the
@@ -611,61 +672,45 @@ __global__ void __launch_bounds__(256) k_enum(int nw, const WorkItem *work, cons
     const JobTab &J = jobs[wi.job];
     __shared__ int scan[8][256];
     __shared__ int fcache[FCACHE];
+    __shared__ CandTerms ct;
     const FCache fc = fill_fcache(fcache, wi, J, tabs, gs);
+    term_prefix(ct, J, tabs);
     int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     long long base[8];
     if (MODE == 1)
         for (int q = 0; q < 8; q++) base[q] = seg_ofs[(size_t)g * 8 + q] + thread_off[((size_t)g * 8 + q) * 256 + threadIdx.x];
-    for (int t = J.tab0; t < J.tab0 + J.ntab; t++) {
-        const TermTab &tb = tabs[t];
-        if (tb.ub[0] < 0) continue;
-        const int ux = tb.ub[0], uy = tb.ub[1], uz = tb.ub[2];
-        const int cells = (2 * ux + 1) * (2 * uy + 1) * (2 * uz + 1);
-        const double aw = fabs(tb.w);
-        const double *nrm = J.operNorms + tb.nrm;
-        for (int c = threadIdx.x; c < cells; c += blockDim.x) {
-            const int dx = c % (2 * ux + 1) - ux;
-            const int dy = (c / (2 * ux + 1)) % (2 * uy + 1) - uy;
-            const int dz = c / ((2 * ux + 1) * (2 * uy + 1)) - uz;
-            const int f = lookup_f(fcache, fc, wi, J, gs, dx, dy, dz);
-            if (f == INT_MIN) continue;
-            double fn[8];
-            int nft = 8;
-            if (f >= 0) {
-                for (int q = 0; q < 8; q++) fn[q] = J.inNorm[(size_t)f * 8 + q];
-            } else {
-                fn[0] = gs.norm[-f - 1];
-                nft = 1;
-            }
-            for (int ft = 0; ft < nft; ft++) {
-                const double fv = fn[ft];
-                for (int gt = 0; gt < 8; gt++) {
-                    if (!in_box(tb.box[gt * 8 + ft], dx, dy, dz)) continue;
-                    const int tx = 2 * (gt & 1) + (ft & 1);
-                    const int ty = 2 * ((gt >> 1) & 1) + ((ft >> 1) & 1);
-                    const int tz = 2 * ((gt >> 2) & 1) + ((ft >> 2) & 1);
-                    double on = aw;
-                    on *= nrm[(dx + tb.band) * 4 + tx];
-                    on *= nrm[(dy + tb.band) * 4 + ty];
-                    on *= nrm[(dz + tb.band) * 4 + tz];
-                    const double bound = on * fv;
-                    if (!(bound > J.thr)) continue;
-                    if (MODE == 1) {
-                        Tuple tp;
-                        tp.g = g;
-                        tp.f = f;
-                        tp.term = (uint8_t)(t - J.tab0);
-                        tp.combo = (uint8_t)(gt * 8 + ft);
-                        tp.dx = (int16_t)dx;
-                        tp.dy = (int16_t)dy;
-                        tp.dz = (int16_t)dz;
-                        out[base[gt] + cnt[gt]] = tp;
-                    }
-                    cnt[gt]++;
-                }
-            }
-        }
-    }
+    walk_candidates(ct, fcache, fc, wi, J, tabs, gs,
+                    [&](int t, const TermTab &tb, int dx, int dy, int dz, int f, const double *fn, int nft,
+                        const double *nx, const double *ny, const double *nz) {
+                        const double aw = fabs(tb.w);
+                        for (int ft = 0; ft < nft; ft++) {
+                            const double fv = fn[ft];
+                            for (int gt = 0; gt < 8; gt++) {
+                                if (!in_box(tb.box[gt * 8 + ft], dx, dy, dz)) continue;
+                                const int tx = 2 * (gt & 1) + (ft & 1);
+                                const int ty = 2 * ((gt >> 1) & 1) + ((ft >> 1) & 1);
+                                const int tz = 2 * ((gt >> 2) & 1) + ((ft >> 2) & 1);
+                                double on = aw;
+                                on *= nx[tx];
+                                on *= ny[ty];
+                                on *= nz[tz];
+                                const double bound = on * fv;
+                                if (!(bound > J.thr)) continue;
+                                if (MODE == 1) {
+                                    Tuple tp;
+                                    tp.g = g;
+                                    tp.f = f;
+                                    tp.term = (uint8_t)t;
+                                    tp.combo = (uint8_t)(gt * 8 + ft);
+                                    tp.dx = (int16_t)dx;
+                                    tp.dy = (int16_t)dy;
+                                    tp.dz = (int16_t)dz;
+                                    out[base[gt] + cnt[gt]] = tp;
+                                }
+                                cnt[gt]++;
+                            }
+                        }
+                    });
     if (MODE == 0) {
         for (int q = 0; q < 8; q++) scan[q][threadIdx.x] = cnt[q];
         __syncthreads();
@@ -820,88 +865,72 @@ __global__ void __launch_bounds__(256) k_enum_mixed(int nw, const WorkItem *work
     const JobTab &J = jobs[wi.job];
     __shared__ int scan[9][256];
     __shared__ int fcache[FCACHE];
+    __shared__ CandTerms ct;
     const FCache fc = fill_fcache(fcache, wi, J, tabs, gs);
+    term_prefix(ct, J, tabs);
     int cnt[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     long long base[9];
     if (MODE == 1)
         for (int q = 0; q < 9; q++) base[q] = seg_ofs[(size_t)g * 9 + q] + thread_off[((size_t)g * 9 + q) * 256 + threadIdx.x];
     const bool route_all = !(J.thr > 0.0);
-    for (int t = J.tab0; t < J.tab0 + J.ntab; t++) {
-        const TermTab &tb = tabs[t];
-        if (tb.ub[0] < 0) continue;
-        const int ux = tb.ub[0], uy = tb.ub[1], uz = tb.ub[2];
-        const int cells = (2 * ux + 1) * (2 * uy + 1) * (2 * uz + 1);
-        const double aw = fabs(tb.w);
-        const double *nrm = J.operNorms + tb.nrm;
-        for (int c = threadIdx.x; c < cells; c += blockDim.x) {
-            const int dx = c % (2 * ux + 1) - ux;
-            const int dy = (c / (2 * ux + 1)) % (2 * uy + 1) - uy;
-            const int dz = c / ((2 * ux + 1) * (2 * uy + 1)) - uz;
-            const int f = lookup_f(fcache, fc, wi, J, gs, dx, dy, dz);
-            if (f == INT_MIN) continue;
-            double fn[8];
-            int nft = 8;
-            if (f >= 0) {
-                for (int q = 0; q < 8; q++) fn[q] = J.inNorm[(size_t)f * 8 + q];
-            } else {
-                fn[0] = gs.norm[-f - 1];
-                nft = 1;
-            }
-            for (int ft = 0; ft < nft; ft++) {
-                const double fv = fn[ft];
-                int surv = 0;
-                double maxb = 0.0;
-                for (int gt = 0; gt < 8; gt++) {
-                    if (!in_box(tb.box[gt * 8 + ft], dx, dy, dz)) continue;
-                    const int tx = 2 * (gt & 1) + (ft & 1);
-                    const int ty = 2 * ((gt >> 1) & 1) + ((ft >> 1) & 1);
-                    const int tz = 2 * ((gt >> 2) & 1) + ((ft >> 2) & 1);
-                    double on = aw;
-                    on *= nrm[(dx + tb.band) * 4 + tx];
-                    on *= nrm[(dy + tb.band) * 4 + ty];
-                    on *= nrm[(dz + tb.band) * 4 + tz];
-                    const double bound = on * fv;
-                    if (!(bound > J.thr)) continue;
-                    surv |= 1 << gt;
-                    maxb = fmax(maxb, bound);
-                }
-                if (!surv) continue;
-                if (route_all || maxb > ROUTE_FP64 * J.thr) {
-                    for (int gt = 0; gt < 8; gt++) {
-                        if (!((surv >> gt) & 1)) continue;
-                        if (MODE == 1) {
-                            Tuple tp;
-                            tp.g = g;
-                            tp.f = f;
-                            tp.term = (uint8_t)(t - J.tab0);
-                            tp.combo = (uint8_t)(gt * 8 + ft);
-                            tp.dx = (int16_t)dx;
-                            tp.dy = (int16_t)dy;
-                            tp.dz = (int16_t)dz;
-                            out[base[gt] + cnt[gt]] = tp;
+    walk_candidates(ct, fcache, fc, wi, J, tabs, gs,
+                    [&](int t, const TermTab &tb, int dx, int dy, int dz, int f, const double *fn, int nft,
+                        const double *nx, const double *ny, const double *nz) {
+                        const double aw = fabs(tb.w);
+                        for (int ft = 0; ft < nft; ft++) {
+                            const double fv = fn[ft];
+                            int surv = 0;
+                            double maxb = 0.0;
+                            for (int gt = 0; gt < 8; gt++) {
+                                if (!in_box(tb.box[gt * 8 + ft], dx, dy, dz)) continue;
+                                const int tx = 2 * (gt & 1) + (ft & 1);
+                                const int ty = 2 * ((gt >> 1) & 1) + ((ft >> 1) & 1);
+                                const int tz = 2 * ((gt >> 2) & 1) + ((ft >> 2) & 1);
+                                double on = aw;
+                                on *= nx[tx];
+                                on *= ny[ty];
+                                on *= nz[tz];
+                                const double bound = on * fv;
+                                if (!(bound > J.thr)) continue;
+                                surv |= 1 << gt;
+                                maxb = fmax(maxb, bound);
+                            }
+                            if (!surv) continue;
+                            if (route_all || maxb > ROUTE_FP64 * J.thr) {
+                                for (int gt = 0; gt < 8; gt++) {
+                                    if (!((surv >> gt) & 1)) continue;
+                                    if (MODE == 1) {
+                                        Tuple tp;
+                                        tp.g = g;
+                                        tp.f = f;
+                                        tp.term = (uint8_t)t;
+                                        tp.combo = (uint8_t)(gt * 8 + ft);
+                                        tp.dx = (int16_t)dx;
+                                        tp.dy = (int16_t)dy;
+                                        tp.dz = (int16_t)dz;
+                                        out[base[gt] + cnt[gt]] = tp;
+                                    }
+                                    cnt[gt]++;
+                                }
+                            } else {
+                                if (MODE == 1) {
+                                    Slice sl;
+                                    sl.g = g;
+                                    sl.f = f;
+                                    sl.term = (uint8_t)t;
+                                    sl.ft = (uint8_t)ft;
+                                    sl.mask = (uint8_t)((ft == 0 && J.depth > 0) ? 0xFE : 0xFF);
+                                    sl.pad = 0;
+                                    sl.dx = (int16_t)dx;
+                                    sl.dy = (int16_t)dy;
+                                    sl.dz = (int16_t)dz;
+                                    sl.pad2 = 0;
+                                    sout[base[8] + cnt[8]] = sl;
+                                }
+                                cnt[8]++;
+                            }
                         }
-                        cnt[gt]++;
-                    }
-                } else {
-                    if (MODE == 1) {
-                        Slice sl;
-                        sl.g = g;
-                        sl.f = f;
-                        sl.term = (uint8_t)(t - J.tab0);
-                        sl.ft = (uint8_t)ft;
-                        sl.mask = (uint8_t)((ft == 0 && J.depth > 0) ? 0xFE : 0xFF);
-                        sl.pad = 0;
-                        sl.dx = (int16_t)dx;
-                        sl.dy = (int16_t)dy;
-                        sl.dz = (int16_t)dz;
-                        sl.pad2 = 0;
-                        sout[base[8] + cnt[8]] = sl;
-                    }
-                    cnt[8]++;
-                }
-            }
-        }
-    }
+                    });
     if (MODE == 0) {
         for (int q = 0; q < 9; q++) scan[q][threadIdx.x] = cnt[q];
         __syncthreads();

@@ -27,7 +27,7 @@ from .gaussians import GaussSum
 
 __all__ = ["BoundingBox", "LegendreBasis", "InterpolatingBasis", "MultiResolutionAnalysis", "FunctionTree",
            "PoissonOperator", "HelmholtzOperator", "apply", "apply_batch", "project", "project_hydrogen", "dot",
-           "multiply", "add",
+           "multiply", "add", "Predictor", "trace_enable", "trace_get",
            "MwgError", "MWG_APPLY_FP64", "MWG_APPLY_MIXED", "MWG_APPLY_PREDICT", "Context", "default_context"]
 
 
@@ -260,6 +260,79 @@ def dot(a: FunctionTree, b: FunctionTree) -> float:
     v = C.c_double()
     check(lib().mwg_tree_dot(a.h, b.h, C.byref(v)), a.mra.ctx.h)
     return float(v.value)
+
+
+class Predictor:
+    """Learned refinement predictor (2 x 64 tanh MLP, FP32) trained and run by HIP kernels."""
+
+    def __init__(self, ctx: Context | None = None, seed: int = 0):
+        self.ctx = ctx or default_context()
+        np_, nf = C.c_int(), C.c_int()
+        lib().mwg_predictor_sizes(C.byref(np_), C.byref(nf))
+        self.nparams, self.nfeatures = np_.value, nf.value
+        h = C.c_void_p()
+        check(lib().mwg_predictor_create(self.ctx.h, int(seed), C.byref(h)), self.ctx.h)
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib._LIB is not None:
+            _lib._LIB.mwg_predictor_destroy(self.h)
+            self.h = None
+
+    def params(self):
+        P = np.zeros(self.nparams, np.float32)
+        m = np.zeros(self.nfeatures, np.float32)
+        s = np.zeros(self.nfeatures, np.float32)
+        check(lib().mwg_predictor_get(self.h, P, m, s), self.ctx.h)
+        return P, m, s
+
+    def set_params(self, P, mean, sdev):
+        check(lib().mwg_predictor_set(self.h, np.ascontiguousarray(P, np.float32), np.ascontiguousarray(mean, np.float32),
+                                      np.ascontiguousarray(sdev, np.float32)), self.ctx.h)
+
+    def train(self, X, y, epochs=200, batch=512, lr=3e-3, seed=0):
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        lb, la = C.c_double(), C.c_double()
+        check(lib().mwg_predictor_train(self.h, len(y), X.reshape(-1), y, int(epochs), int(batch), float(lr), int(seed),
+                                        C.byref(lb), C.byref(la)), self.ctx.h)
+        return lb.value, la.value
+
+    def gradient(self, X, y):
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        g = np.zeros(self.nparams, np.float32)
+        check(lib().mwg_predictor_gradient(self.h, len(y), X.reshape(-1), y, g), self.ctx.h)
+        return g
+
+    def infer(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        p = np.zeros(len(X), np.float32)
+        check(lib().mwg_predictor_infer(self.h, len(X), X.reshape(-1), p), self.ctx.h)
+        return p
+
+    def install(self, threshold: float = 0.5):
+        """Use this predictor for applies with MWG_APPLY_PREDICT on its context."""
+        check(lib().mwg_ctx_set_predictor(self.ctx.h, self.h, float(threshold)), self.ctx.h)
+
+
+def trace_enable(on: bool = True, ctx: Context | None = None):
+    ctx = ctx or default_context()
+    check(lib().mwg_trace_enable(ctx.h, int(bool(on))), ctx.h)
+
+
+def trace_get(ctx: Context | None = None):
+    """(features [n, nfeatures], labels [n]) recorded since trace_enable."""
+    ctx = ctx or default_context()
+    n = C.c_int64()
+    check(lib().mwg_trace_size(ctx.h, C.byref(n)), ctx.h)
+    nf = C.c_int()
+    lib().mwg_predictor_sizes(None, C.byref(nf))
+    X = np.zeros(n.value * nf.value, np.float32)
+    y = np.zeros(n.value, np.float32)
+    if n.value:
+        check(lib().mwg_trace_get(ctx.h, X, y), ctx.h)
+    return X.reshape(n.value, nf.value), y
 
 
 def mra_from(box: dict, order: int, basis: str = "L", max_depth: int = 16, ctx: Context | None = None):
