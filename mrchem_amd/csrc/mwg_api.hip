@@ -168,7 +168,7 @@ struct MraH {
     int n0 = 0, corner[3] = {0, 0, 0}, boxes[3] = {1, 1, 1}, maxDepth = 16;
     double sfac = 1.0;
     Basis B;
-    DBuf<double> U, qx, qw, phiq, W, WT;
+    DBuf<double> U, qx, qw, phiq, W, WT, ccoef;
     double lo(int d) const { return sfac * std::ldexp((double)corner[d], -n0); }
     double width(int d) const { return sfac * std::ldexp((double)boxes[d], -n0); }
     DevBasis dev() const { return DevBasis{B.K, U.p, qx.p, qw.p, phiq.p, W.p}; }
@@ -353,7 +353,7 @@ static void compute_ns(OperH &O, int d, const std::vector<NSRequest> &req, DBuf<
     O.iterm.upload(tterm.data(), tterm.size(), s);
     O.idelta.upload(tdelta.data(), tdelta.size(), s);
     O.t1buf.alloc((size_t)std::max(nt1, 1) * K2);
-    HC(launch_tblock(s, nt1, O.iterm.p, O.idelta.p, O.bvals.p, pre1, M.dev(), C.gx.p, C.gw.p, O.t1buf.p));
+    HC(launch_tblock_mom(s, nt1, O.iterm.p, O.idelta.p, O.bvals.p, pre1, B.K, M.ccoef.p, C.gx.p, C.gw.p, O.t1buf.p));
     // NS transform per delta
     std::vector<long long> ndelta;
     std::vector<int> nrow0, ntlo, nthi;
@@ -598,12 +598,14 @@ struct Box3 {
     short m[3];
 };
 
-static void job_boxes(const JobH &J, int d, std::vector<TermTab> &tabs, int &tab0, int &ntab, double &thr) {
+static void job_boxes(const JobH &J, int d, std::vector<TermTab> &tabs, int &tab0, int &ntab, double &thr,
+                      double &route) {
     const OperH &O = *J.op;
     const LevelH &L = *O.lev[d];
     const int nterms = O.nterms();
     thr = J.gsq > 0.0 ? SCREEN_REL * J.prec * std::sqrt(J.gsq / nterms) : 0.0;
     tab0 = (int)tabs.size();
+    double bmax = 0.0;
     for (int t = 0; t < nterms; t++) {
         const TermLevel &lv = L.t[t];
         if (!lv.active) continue;
@@ -630,6 +632,7 @@ static void job_boxes(const JobH &J, int d, std::vector<TermTab> &tabs, int &tab
                 const double base = std::fabs(O.weff[t]) * F;
                 const double full = base * lv.maxn[ty[0]] * lv.maxn[ty[1]] * lv.maxn[ty[2]];
                 if (!(full > thr)) continue;
+                bmax = std::max(bmax, full);
                 short m3[3];
                 bool ok = true;
                 for (int q = 0; q < 3; q++) {
@@ -665,6 +668,7 @@ static void job_boxes(const JobH &J, int d, std::vector<TermTab> &tabs, int &tab
     }
     ntab = (int)tabs.size() - tab0;
     if (ntab > 255) throw Error(MWG_ERR_RANGE, "more than 255 active terms at one depth");
+    route = thr > 0.0 ? ROUTE_FP64 * thr : ROUTE_ROOT * bmax;
 }
 
 
@@ -891,7 +895,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                 J.op->ensure(d);
                 JobTab T;
                 std::memset(&T, 0, sizeof(T));
-                job_boxes(J, d, tabs, T.tab0, T.ntab, T.thr);
+                job_boxes(J, d, tabs, T.tab0, T.ntab, T.thr, T.route);
                 T.inHash = J.in->dhash();
                 T.genHash = DHash{C.genHashKeys.p + J.genOff, C.genHashVals.p + J.genOff, J.genMask};
                 T.inCoef = J.in->coef.p;
@@ -1441,6 +1445,7 @@ int mwg_mra_create(mwg_ctx ctx, int root_scale, const int *corner, const int *bo
         for (int i = 0; i < N; i++)
             for (int j = 0; j < N; j++) WT[i * N + j] = m->B.W[j * N + i];
         m->WT.upload(WT.data(), WT.size(), s);
+        m->ccoef.upload(m->B.ccoef.data(), m->B.ccoef.size(), s);
     });
     if (rc) {
         delete m;

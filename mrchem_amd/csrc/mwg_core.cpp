@@ -119,6 +119,44 @@ void build_basis(Basis &B, int k, int type) {
     }
     B.W.assign(N * N, 0.0);
     for (int i = 0; i < N * N; i++) B.W[i] = (double)Wl[i];
+
+    // cross-correlation in shifted Legendre polynomials (exact: degree <= 2K-1 per half)
+    const int P = 2 * K;
+    std::vector<double> tx(P), tw(P);
+    gauss_legendre01(P, tx.data(), tw.data());
+    B.ccoef.assign((size_t)2 * K * K * P, 0.0);
+    std::vector<long double> cz(K * K);
+    double pu[MAXK], pv[MAXK], L[2 * MAXK];
+    for (int h = 0; h < 2; h++)
+        for (int q = 0; q < P; q++) {
+            const double t = tx[q];
+            const double z = t - 1.0 + h;
+            // C_ij(z) by the K-point rule on the overlap of the two cells
+            const double lo = std::max(0.0, z), hi = std::min(1.0, 1.0 + z);
+            std::fill(cz.begin(), cz.end(), 0.0L);
+            for (int r = 0; r < K; r++) {
+                const double u = lo + (hi - lo) * B.qx[r];
+                const long double wr = (long double)(hi - lo) * B.qw[r];
+                B.phi(u, pu);
+                B.phi(u - z, pv);
+                for (int i = 0; i < K; i++)
+                    for (int j = 0; j < K; j++) cz[i * K + j] += wr * pu[i] * pv[j];
+            }
+            // orthonormal shifted Legendre values at t
+            const double x = 2.0 * t - 1.0;
+            double p0 = 1.0, p1 = x;
+            L[0] = 1.0;
+            if (P > 1) L[1] = std::sqrt(3.0) * x;
+            for (int n = 2; n < P; n++) {
+                const double p2 = ((2.0 * n - 1.0) * x * p1 - (n - 1.0) * p0) / n;
+                p0 = p1;
+                p1 = p2;
+                L[n] = std::sqrt(2.0 * n + 1.0) * p2;
+            }
+            for (int ij = 0; ij < K * K; ij++)
+                for (int n = 0; n < P; n++)
+                    B.ccoef[((size_t)h * K * K + ij) * P + n] += (double)(tw[q] * cz[ij]) * L[n];
+        }
 }
 
 static double fit_error(const KernelFit &F, double mu) {

@@ -20,6 +20,10 @@ struct Basis {
     std::vector<double> qx, qw;  // K-point rule on [0,1]
     std::vector<double> phiq;    // [q][i] = phi^B_i(qx[q])
     std::vector<double> W;       // 2K x 2K two-scale matrix [[H0 H1][G0 G1]]
+    // cross-correlation C_ij(z) = int phi_i(u) phi_j(u - z) du on the halves z in [-1,0]
+    // (h = 0) and [0,1] (h = 1), expanded in orthonormal shifted Legendre polynomials
+    // L_p(t), t = z + 1 - h in [0,1], p < 2K: ccoef[(h*K*K + i*K + j)*2K + p]
+    std::vector<double> ccoef;
     void phi(double x, double *out) const;
 };
 void build_basis(Basis &B, int k, int type);

@@ -77,5 +77,8 @@ static_assert(sizeof(Slice) == 20, "slice record is 20 bytes");
 // ROUTE_FP64 times the screening threshold: the FP32 rounding of such a slice
 // (~8 * 2^-24 of its bound) would exceed the screening error
 constexpr double ROUTE_FP64 = 131072.0;
+// on the root level (no screening threshold yet) slices above ROUTE_ROOT times the
+// largest possible bound run in FP64; FP32 rounding of the others stays ~1e-10 of it
+constexpr double ROUTE_ROOT = 1e-3;
 
 }  // namespace mwg

@@ -42,6 +42,7 @@ struct JobTab {
     double *outCoef;
     double *outNorm;
     double thr;
+    double route;  // mixed mode: slices with a bound above this run in FP64
     int tab0, ntab;
     int lmax[3];
     int depth;
@@ -59,6 +60,9 @@ struct GenState {
 
 hipError_t launch_tblock(hipStream_t s, int n, const int *item_term, const long long *item_delta, const double *b,
                          double pre, const DevBasis &B, const double *gx, const double *gw, double *out);
+hipError_t launch_tblock_mom(hipStream_t s, int n, const int *item_term, const long long *item_delta,
+                             const double *b, double pre, int K, const double *ccoef, const double *gx,
+                             const double *gw, double *out);
 hipError_t launch_nsblocks(hipStream_t s, int n, const long long *item_delta, const int *item_row0,
                            const int *item_tlo, const int *item_thi, const double *t1, const double *W, int K,
                            double *blocks, double *norms);

@@ -101,6 +101,69 @@ __global__ void k_tblock(int n, const int *item_term, const long long *item_delt
     if (tid < K2) out[(size_t)it * K2 + tid] = acc * pre;
 }
 
+
+// Same block by moments: t[i][j] = pre * sum_h sum_p ccoef[h][i][j][p] M_h[p] with
+// M_h[p] = int_half exp(-b (delta+z)^2) L_p(t(z)) dz on the same Gauss-Legendre panels.
+// Threads take quadrature points independently; the 4K moments are reduced at the end.
+__global__ void __launch_bounds__(256) k_tblock_mom(int n, const int *item_term, const long long *item_delta,
+                                                    const double *bvals, double pre, int K, const double *ccoef,
+                                                    const double *gx, const double *gw, double *out) {
+    const int it = blockIdx.x;
+    if (it >= n) return;
+    const int P = 2 * K, K2 = K * K;
+    __shared__ double red[4][4 * MAXK];
+    __shared__ double Ms[4 * MAXK];
+    const double b = bvals[item_term[it]];
+    const double delta = (double)item_delta[it];
+    const double zc = -delta;
+    const double R = sqrt(GAUSS_CUT_D / b);
+    const double pw = (b > 4.0) ? 0.5 / sqrt(b) : 0.25;
+    double M[2][2 * MAXK];
+    for (int h = 0; h < 2; h++)
+        for (int p = 0; p < P; p++) M[h][p] = 0.0;
+    for (int half = 0; half < 2; half++) {
+        const double lo = half == 0 ? -1.0 : 0.0, hi = half == 0 ? 0.0 : 1.0;
+        const double a0 = fmax(lo, zc - R), a1 = fmin(hi, zc + R);
+        if (a1 <= a0) continue;
+        int np = (int)ceil((a1 - a0) / pw);
+        if (np < 1) np = 1;
+        const double ph = (a1 - a0) / np;
+        for (int e = threadIdx.x; e < np * 24; e += blockDim.x) {
+            const int pnl = e / 24, q = e % 24;
+            const double z = a0 + pnl * ph + ph * gx[q];
+            const double g = exp(-b * (delta + z) * (delta + z)) * ph * gw[q];
+            const double x = 2.0 * (z - lo) - 1.0;
+            double p0 = 1.0, p1 = x;
+            M[half][0] += g;
+            if (P > 1) M[half][1] += g * sqrt(3.0) * x;
+            for (int k = 2; k < P; k++) {
+                const double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+                M[half][k] += g * sqrt(2.0 * k + 1.0) * p2;
+            }
+        }
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int h = 0; h < 2; h++)
+        for (int p = 0; p < P; p++) {
+            double v = M[h][p];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+            if (lane == 0) red[wave][h * P + p] = v;
+        }
+    __syncthreads();
+    if (threadIdx.x < 2 * P) Ms[threadIdx.x] = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x < K2) {
+        double t = 0.0;
+        for (int h = 0; h < 2; h++) {
+            const double *c = ccoef + ((size_t)h * K2 + threadIdx.x) * P;
+            for (int p = 0; p < P; p++) t += c[p] * Ms[h * P + p];
+        }
+        out[(size_t)it * K2 + threadIdx.x] = t * pre;
+    }
+}
+
 // NS blocks from the child-scale scaling blocks: R = W M W^T with
 // M = [[t(2d), t(2d-1)], [t(2d+1), t(2d)]]; type = 2*gt_bit + ft_bit.
 __global__ void k_nsblocks(int n, const long long *item_delta, const int *item_row0, const int *item_tlo,
@@ -872,7 +935,6 @@ __global__ void __launch_bounds__(256) k_enum_mixed(int nw, const WorkItem *work
     long long base[9];
     if (MODE == 1)
         for (int q = 0; q < 9; q++) base[q] = seg_ofs[(size_t)g * 9 + q] + thread_off[((size_t)g * 9 + q) * 256 + threadIdx.x];
-    const bool route_all = !(J.thr > 0.0);
     walk_candidates(ct, fcache, fc, wi, J, tabs, gs,
                     [&](int t, const TermTab &tb, int dx, int dy, int dz, int f, const double *fn, int nft,
                         const double *nx, const double *ny, const double *nz) {
@@ -896,7 +958,7 @@ __global__ void __launch_bounds__(256) k_enum_mixed(int nw, const WorkItem *work
                                 maxb = fmax(maxb, bound);
                             }
                             if (!surv) continue;
-                            if (route_all || maxb > ROUTE_FP64 * J.thr) {
+                            if (maxb > J.route) {
                                 for (int gt = 0; gt < 8; gt++) {
                                     if (!((surv >> gt) & 1)) continue;
                                     if (MODE == 1) {
@@ -1292,6 +1354,14 @@ hipError_t launch_tblock(hipStream_t s, int n, const int *item_term, const long 
                          double pre, const DevBasis &B, const double *gx, const double *gw, double *out) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_tblock, dim3(n), dim3(128), 0, s, n, item_term, item_delta, b, pre, B, gx, gw, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_tblock_mom(hipStream_t s, int n, const int *item_term, const long long *item_delta,
+                             const double *b, double pre, int K, const double *ccoef, const double *gx,
+                             const double *gw, double *out) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tblock_mom, dim3(n), dim3(256), 0, s, n, item_term, item_delta, b, pre, K, ccoef, gx, gw, out);
     return hipGetLastError();
 }
 
