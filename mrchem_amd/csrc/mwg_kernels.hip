@@ -447,47 +447,64 @@ __device__ inline bool in_box(const short *b, int dx, int dy, int dz) {
 
 // mark the input nodes that the output nodes need at this depth but the input
 // tree does not have (scaling-only "generated" nodes)
-__global__ void k_gen_mark(int nw, const WorkItem *work, const JobTab *jobs, const TermTab *tabs, GenState gs) {
+__global__ void __launch_bounds__(256) k_gen_mark(int nw, const WorkItem *work, const JobTab *jobs,
+                                                  const TermTab *tabs, GenState gs) {
     const WorkItem wi = work[blockIdx.x];
     const JobTab &J = jobs[wi.job];
-    for (int t = J.tab0; t < J.tab0 + J.ntab; t++) {
-        const TermTab &tb = tabs[t];
-        if (tb.ub0[0] < 0) continue;
+    __shared__ int start[257];
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int t = 0; t < J.ntab; t++) {
+            start[t] = acc;
+            const TermTab &tb = tabs[J.tab0 + t];
+            if (tb.ub0[0] >= 0) acc += (2 * tb.ub0[0] + 1) * (2 * tb.ub0[1] + 1) * (2 * tb.ub0[2] + 1);
+        }
+        start[J.ntab] = acc;
+    }
+    __syncthreads();
+    const int total = start[J.ntab];
+    for (int k = threadIdx.x; k < total; k += blockDim.x) {
+        int lo = 0, hi = J.ntab - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (start[mid] <= k) lo = mid;
+            else hi = mid - 1;
+        }
+        const TermTab &tb = tabs[J.tab0 + lo];
+        const int c = k - start[lo];
         const int ux = tb.ub0[0], uy = tb.ub0[1], uz = tb.ub0[2];
-        const int cells = (2 * ux + 1) * (2 * uy + 1) * (2 * uz + 1);
-        for (int c = threadIdx.x; c < cells; c += blockDim.x) {
-            const int dx = c % (2 * ux + 1) - ux;
-            const int dy = (c / (2 * ux + 1)) % (2 * uy + 1) - uy;
-            const int dz = c / ((2 * ux + 1) * (2 * uy + 1)) - uz;
-            const int fx = wi.l[0] - dx, fy = wi.l[1] - dy, fz = wi.l[2] - dz;
-            if (fx < 0 || fy < 0 || fz < 0 || fx >= J.lmax[0] || fy >= J.lmax[1] || fz >= J.lmax[2]) continue;
-            bool need = false;
-            for (int gt = 0; gt < 8 && !need; gt++) need = in_box(tb.box[gt * 8], dx, dy, dz);
-            if (!need) continue;
-            const uint64_t key = dnode_key(J.depth, fx, fy, fz);
-            if (dhash_find(J.inHash, key) >= 0) continue;
-            // insert into this job's generated-node table
-            const unsigned long long kk = (unsigned long long)key + 1ull;
-            uint32_t i = dkey_hash(key) & J.genHash.mask;
-            for (uint32_t probe = 0; probe <= J.genHash.mask; probe++) {
-                unsigned long long old = atomicCAS(&J.genHash.keys[i], 0ull, kk);
-                if (old == 0ull) {
-                    const int idx = atomicAdd(gs.count, 1);
-                    if (idx < gs.cap) {
-                        gs.keys[idx] = key;
-                        gs.job[idx] = wi.job;
-                    } else {
-                        atomicOr(gs.overflow, 1);
-                    }
-                    J.genHash.vals[i] = idx;
-                    break;
+        const int dx = c % (2 * ux + 1) - ux;
+        const int dy = (c / (2 * ux + 1)) % (2 * uy + 1) - uy;
+        const int dz = c / ((2 * ux + 1) * (2 * uy + 1)) - uz;
+        const int fx = wi.l[0] - dx, fy = wi.l[1] - dy, fz = wi.l[2] - dz;
+        if (fx < 0 || fy < 0 || fz < 0 || fx >= J.lmax[0] || fy >= J.lmax[1] || fz >= J.lmax[2]) continue;
+        bool need = false;
+        for (int gt = 0; gt < 8 && !need; gt++) need = in_box(tb.box[gt * 8], dx, dy, dz);
+        if (!need) continue;
+        const uint64_t key = dnode_key(J.depth, fx, fy, fz);
+        if (dhash_find(J.inHash, key) >= 0) continue;
+        // insert into this job's generated-node table
+        const unsigned long long kk = (unsigned long long)key + 1ull;
+        uint32_t i = dkey_hash(key) & J.genHash.mask;
+        for (uint32_t probe = 0; probe <= J.genHash.mask; probe++) {
+            unsigned long long old = atomicCAS(&J.genHash.keys[i], 0ull, kk);
+            if (old == 0ull) {
+                const int idx = atomicAdd(gs.count, 1);
+                if (idx < gs.cap) {
+                    gs.keys[idx] = key;
+                    gs.job[idx] = wi.job;
+                } else {
+                    atomicOr(gs.overflow, 1);
                 }
-                if (old == kk) break;
-                i = (i + 1) & J.genHash.mask;
-                if (probe == J.genHash.mask) atomicOr(gs.overflow, 2);
+                J.genHash.vals[i] = idx;
+                break;
             }
+            if (old == kk) break;
+            i = (i + 1) & J.genHash.mask;
+            if (probe == J.genHash.mask) atomicOr(gs.overflow, 2);
         }
     }
+    (void)nw;
 }
 
 // scaling block of a generated node: reconstruct from the leaf ancestor (using its
