@@ -99,11 +99,7 @@ def train_predictor(G, configs, mra, args):
     return pred, len(y), time.perf_counter() - t0
 
 
-def cpu_baseline(orbs, mus, k, prec, budget_s=15.0):
-    from mrchem_amd import configs
-    from oracle import mwref as M
-    threads = M.get_threads()
-    mra = M.MRA(**configs.BOX20, k=k, basis="L", max_depth=16)
+def _cpu_sample(M, mra, orbs, mus, prec, budget_s):
     nodes = 0
     t_apply = 0.0
     done = 0
@@ -117,9 +113,38 @@ def cpu_baseline(orbs, mus, k, prec, budget_s=15.0):
         done += 1
         if t_apply > budget_s:
             break
+    return nodes, t_apply, done
+
+
+def cpu_baseline(orbs, mus, k, prec, budget_s=15.0, budget_1t_s=6.0):
+    """FP64 OpenMP oracle on all host threads it is given, and on one thread (the
+    reference's published logs are single-threaded)."""
+    from mrchem_amd import configs
+    from oracle import mwref as M
+    threads = M.get_threads()
+    mra = M.MRA(**configs.BOX20, k=k, basis="L", max_depth=16)
+    nodes, t_apply, done = _cpu_sample(M, mra, orbs, mus, prec, budget_s)
+    M.set_threads(1)
+    try:
+        n1, t1, d1 = _cpu_sample(M, mra, orbs, mus, prec, budget_1t_s)
+    finally:
+        M.set_threads(threads)
     return dict(value=nodes / t_apply, unit="output nodes/s", cores=threads, kind="port",
                 sample=f"FP64 OpenMP oracle (oracle/libmwref.so), Helmholtz apply of {done} of {len(orbs)} "
-                       f"orbitals of the same workload, {t_apply:.2f} s of apply time")
+                       f"orbitals of the same workload, {t_apply:.2f} s of apply time",
+                value_1_thread=n1 / t1,
+                sample_1_thread=f"same oracle on 1 thread, {d1} orbital(s), {t1:.2f} s of apply time")
+
+
+def copy_probe(G, tree):
+    """One in-place rescale by 1.0 of a tree's coefficients (k_scale): a streaming kernel
+    of known size (reads and writes nnodes * 8 (k+1)^3 doubles) that calibrates HBM
+    byte counters against the contraction's."""
+    n = tree.getNNodes()
+    tree.rescale(1.0)
+    tree.mra.ctx.sync()
+    K = tree.mra.order + 1
+    log(f"[bench] copy probe: k_scale over {n} nodes, {n * 8 * K ** 3 * 8} bytes read and written")
 
 
 def exchange_main(args, G, configs):
@@ -142,7 +167,8 @@ def exchange_main(args, G, configs):
     log(f"[bench] mixed: {res32['ms_per_step']:.1f} ms/step, {res32['nodes_per_s']:.0f} nodes/s, "
         f"max per-pair rel L2 err {max(errs):.2e}")
     best, dtype = (res32, "f32") if res32["nodes_per_s"] > res64["nodes_per_s"] and max(errs) < args.prec else (res64, "f64")
-    achieved = best["flops_per_step"] / best["t_contract_per_step"] if best["t_contract_per_step"] > 0 else 0.0
+    tc = best["t_contract_per_step"]
+    achieved = res64["tuples_per_step"] * 6.0 * (args.k + 1) ** 4 / tc if tc > 0 else 0.0
     line = {
         "metric": "output nodes/s for Poisson/Helmholtz apply at prec 1e-6 vs plain FP64 GPU",
         "value": best["nodes_per_s"], "unit": "output nodes/s", "n_gpus": 1, "steps": args.steps,
@@ -154,8 +180,56 @@ def exchange_main(args, G, configs):
                    "per_pair_rel_l2_err_max": max(errs), "per_pair_rel_l2_err_mean": sum(errs) / len(errs),
                    "output_nodes_per_step": best["nodes_per_step"], "multiply_s": t_mul},
         "roofline": dict(bound="mfma", achieved=achieved / 1e12, peak=PEAK[dtype] / 1e12, unit="TFLOP/s",
-                         frac=achieved / PEAK[dtype], traffic=None),
+                         frac=achieved / PEAK[dtype], traffic=None, contraction_ms_per_step=1e3 * tc),
         "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def predictor_main(args, G, configs):
+    """Learned refinement predictor, trained on refinement traces of k=7, prec 1e-6 applies
+    (Helmholtz on 5-orbital sets and Poisson on their pair products, seeds 101 and 102),
+    evaluated on 8 fresh orbitals (seed 200) at k=9, prec 1e-7: Helmholtz and Poisson
+    applies, plain FP64 against FP64 + predictor (refinement iterations, output nodes,
+    time, error of the predicted apply against the plain one)."""
+    mra7 = G.mra_from(configs.BOX20, 7, "L", max_depth=16)
+    targs = argparse.Namespace(**vars(args))
+    targs.prec = 1e-6
+    targs.orbitals = 5
+    pred, ntrain, t_train = train_predictor(G, configs, mra7, targs)
+    log(f"[bench] predictor trained on {ntrain} trace samples in {t_train:.1f} s")
+    k, prec = 9, 1e-7
+    mra = G.mra_from(configs.BOX20, k, "L", max_depth=16)
+    orbs, nuc, mus = configs.orbitals(200, n_orb=8)
+    trees = [G.project(prec, G.FunctionTree(mra), o) for o in orbs]
+    P = G.PoissonOperator(mra, prec)
+    cases = {"helmholtz": [G.HelmholtzOperator(mra, mu, prec) for mu in mus], "poisson": [P] * len(trees)}
+    report = {}
+    best = None
+    for name, ops in cases.items():
+        res0 = run_variant(G, mra, trees, ops, prec, G.MWG_APPLY_FP64, args.steps, args.warmup)
+        res1 = run_variant(G, mra, trees, ops, prec, G.MWG_APPLY_FP64 | G.MWG_APPLY_PREDICT, args.steps, args.warmup)
+        errs = [_l2diff(a, b) / math.sqrt(b.getSquareNorm()) for a, b in zip(res1["outs"], res0["outs"])]
+        report[name] = dict(plain_iters=res0["iters"], predict_iters=res1["iters"],
+                            plain_nodes_per_s=res0["nodes_per_s"], predict_nodes_per_s=res1["nodes_per_s"],
+                            plain_ms=res0["ms_per_step"], predict_ms=res1["ms_per_step"],
+                            plain_nodes=res0["nodes_per_step"], predict_nodes=res1["nodes_per_step"],
+                            speculative=res1["predicted"], wasted=res1["wasted"],
+                            rel_l2_err_max=max(errs))
+        log(f"[bench] {name}: iters {res0['iters']} -> {res1['iters']}, {res0['ms_per_step']:.1f} -> "
+            f"{res1['ms_per_step']:.1f} ms/step, nodes {res0['nodes_per_step']:.0f} -> {res1['nodes_per_step']:.0f}, "
+            f"max rel L2 err {max(errs):.2e}")
+        if best is None or name == "helmholtz":
+            best = (res0, res1)
+    res0, res1 = best
+    line = {
+        "metric": "output nodes/s for Poisson/Helmholtz apply at prec 1e-6 vs plain FP64 GPU",
+        "value": res1["nodes_per_s"], "unit": "output nodes/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": res1["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded Gaussian orbitals)",
+        "config": {"workload": f"learned predictor: 8 orbitals, k={k}, prec={prec}, Helmholtz (value) and Poisson",
+                   "train_samples": ntrain, "train_s": t_train, "cases": report},
+        "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
 
@@ -172,7 +246,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-predict", action="store_true", help="skip the learned-predictor variant")
     ap.add_argument("--pred-threshold", type=float, default=0.5)
-    ap.add_argument("--workload", default="helmholtz", choices=["helmholtz", "exchange"])
+    ap.add_argument("--workload", default="helmholtz", choices=["helmholtz", "exchange", "predictor"])
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default contraction task sizes")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after timing, rescale one output tree in place (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -184,18 +261,33 @@ def main():
 
     if args.workload == "exchange":
         return exchange_main(args, G, configs)
+    if args.workload == "predictor":
+        return predictor_main(args, G, configs)
 
     log(f"[bench] building inputs: {args.orbitals} orbitals, k={args.k}, prec={args.prec}")
     mra, orbs, mus, trees, ops, t_proj, t_oper = build_inputs(G, configs, args.k, args.prec, args.orbitals,
                                                               args.seed)
     log(f"[bench] input nodes {[t.getNNodes() for t in trees]}, projection {t_proj:.3f} s, operators {t_oper:.3f} s")
 
+    from mrchem_amd.autotune import autotune
+    tuning = {}
+
+    def tune(name, flags):
+        if args.no_autotune:
+            return
+        r = autotune(mra, trees, ops, args.prec, flags)
+        tuning[name] = dict(fp64_chunk=r["fp64_chunk"], slice_chunk=r["slice_chunk"])
+        log(f"[bench] autotune {name}: fp64_chunk {r['fp64_chunk']}, slice_chunk {r['slice_chunk']}, "
+            f"contraction {1e3 * r['t_contract']:.2f} ms ({len(r['table'])} candidates, {r['seconds']:.1f} s)")
+
+    tune("fp64", G.MWG_APPLY_FP64)
     res64 = run_variant(G, mra, trees, ops, args.prec, G.MWG_APPLY_FP64, args.steps, args.warmup)
     log(f"[bench] fp64: {res64['ms_per_step']:.2f} ms/step, {res64['nodes_per_s']:.1f} nodes/s, "
         f"contraction {1e3 * res64['t_contract_per_step']:.2f} ms/step")
     best, dtype = res64, "f64"
     res32 = None
     if args.k == 7:
+        tune("mixed", G.MWG_APPLY_MIXED)
         res32 = run_variant(G, mra, trees, ops, args.prec, G.MWG_APPLY_MIXED, args.steps, args.warmup)
         err = max(_l2diff(a, b) / math.sqrt(b.getSquareNorm()) for a, b in zip(res32["outs"], res64["outs"]))
         res32["rel_err_vs_fp64"] = err
@@ -229,6 +321,9 @@ def main():
                     else "k_contract_fp64 + k_reduce_partials",
                     issued_tflops=(best["flops_per_step"] / tc / 1e12) if tc > 0 else None,
                     contraction_ms_per_step=1e3 * tc)
+
+    if args.copy_probe:
+        copy_probe(G, res64["outs"][0])
 
     cpu = None
     if not args.no_cpu:
@@ -267,6 +362,7 @@ def main():
             "predict_train_samples": resP["train_samples"] if resP else None,
             "operator_build_s": t_oper, "projection_s": t_proj,
             "contraction_ms_per_step": 1e3 * best["t_contract_per_step"],
+            "tuning": tuning or None,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

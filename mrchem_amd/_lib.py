@@ -49,6 +49,8 @@ def _sig():
         "mwg_ctx_create": [i, pvp],
         "mwg_ctx_destroy": [vp],
         "mwg_sync": [vp],
+        "mwg_ctx_set_tuning": [vp, i, i],
+        "mwg_ctx_get_tuning": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)],
         "mwg_mra_create": [vp, i, fi, fi, d, i, i, i, pvp],
         "mwg_mra_destroy": [vp],
         "mwg_tree_create": [vp, pvp],
@@ -89,7 +91,8 @@ def _sig():
     }
 
 
-EXPORTED = ["mwg_ctx_create", "mwg_ctx_destroy", "mwg_last_error", "mwg_sync", "mwg_mra_create", "mwg_mra_destroy",
+EXPORTED = ["mwg_ctx_create", "mwg_ctx_destroy", "mwg_last_error", "mwg_sync", "mwg_ctx_set_tuning",
+            "mwg_ctx_get_tuning", "mwg_mra_create", "mwg_mra_destroy",
             "mwg_tree_create", "mwg_tree_destroy", "mwg_tree_nnodes", "mwg_tree_sqnorm", "mwg_tree_maxdepth",
             "mwg_tree_import", "mwg_tree_export", "mwg_tree_rescale", "mwg_tree_dot", "mwg_tree_eval",
             "mwg_project_gauss", "mwg_project_hydrogen", "mwg_oper_poisson", "mwg_oper_helmholtz",

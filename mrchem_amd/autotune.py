@@ -1,0 +1,74 @@
+"""Autotuner for the contraction task granularity of the batched apply.
+
+The device apply splits each output node's tuple list into FP64 wave tasks of
+``fp64_chunk`` tuples and its FP32 MFMA slice list into wave tasks of
+``slice_chunk`` slices; partial results of a node's tasks are summed in a fixed
+order afterwards.  Small tasks spread a node over more waves (better occupancy
+for few, heavy nodes); large tasks cut partial-result traffic and reduction work.
+The best point depends on k, the precision and the tree shapes, so it is measured:
+``autotune`` times the contraction kernels (HIP events, ``t_contract``) of the
+caller's own batch for each candidate and installs the fastest on the context.
+
+Candidates are applied to fresh output trees each time; the results agree to
+rounding for every setting (only the FP64 summation grouping changes).
+"""
+from __future__ import annotations
+
+import itertools
+import time
+
+from . import mrcpp as G
+
+FP64_CHUNKS = (16, 32, 64, 128, 256)
+SLICE_CHUNKS = (4, 8, 16, 32, 64)
+
+
+def _measure(mra, trees, ops, prec, flags, reps):
+    best = float("inf")
+    for _ in range(reps):
+        outs = [G.FunctionTree(mra) for _ in trees]
+        st = G.apply_batch(prec, outs, ops, trees, flags=flags)
+        best = min(best, st[0]["t_contract"])
+    return best
+
+
+def autotune(mra, trees, ops, prec, flags=G.MWG_APPLY_MIXED, fp64_chunks=FP64_CHUNKS,
+             slice_chunks=SLICE_CHUNKS, reps=2, install=True):
+    """Sweep (fp64_chunk, slice_chunk) on ``apply_batch(prec, ., ops, trees, flags)``.
+
+    The FP64 chunk is tuned first with the slice chunk held at its current value, then
+    the slice chunk (only when ``flags`` selects the mixed path, which has slices).
+    Returns {"fp64_chunk", "slice_chunk", "t_contract", "table": [(fc, sc, t), ...],
+    "seconds"}; installs the winner on ``mra.ctx`` when ``install``.
+    """
+    ctx = mra.ctx
+    t0 = time.perf_counter()
+    fc0, sc0 = ctx.get_tuning()
+    table = []
+    # one untimed apply so that operator levels and scratch pools exist
+    G.apply_batch(prec, [G.FunctionTree(mra) for _ in trees], ops, trees, flags=flags)
+    best = (float("inf"), fc0, sc0)
+    for fc in fp64_chunks:
+        ctx.set_tuning(fc, sc0)
+        t = _measure(mra, trees, ops, prec, flags, reps)
+        table.append((fc, sc0, t))
+        if t < best[0]:
+            best = (t, fc, sc0)
+    if flags & G.MWG_APPLY_MIXED:
+        for sc in slice_chunks:
+            if sc == sc0:
+                continue
+            ctx.set_tuning(best[1], sc)
+            t = _measure(mra, trees, ops, prec, flags, reps)
+            table.append((best[1], sc, t))
+            if t < best[0]:
+                best = (t, best[1], sc)
+    t_best, fc, sc = best
+    ctx.set_tuning(fc, sc) if install else ctx.set_tuning(fc0, sc0)
+    return dict(fp64_chunk=fc, slice_chunk=sc, t_contract=t_best, table=table,
+                seconds=time.perf_counter() - t0)
+
+
+def grid(fp64_chunks=FP64_CHUNKS, slice_chunks=SLICE_CHUNKS):
+    """Full candidate grid (for exhaustive sweeps)."""
+    return list(itertools.product(fp64_chunks, slice_chunks))

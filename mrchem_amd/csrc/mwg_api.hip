@@ -107,6 +107,8 @@ struct Ctx {
     DBuf<WorkItem> work;
     DBuf<JobTab> jobs;
     DBuf<TermTab> tabs;
+    DBuf<BoxJob> bjobs;
+    DBuf<double> tbmax;
     DBuf<int> thread_off, gtotal;
     DBuf<long long> gofs;
     DBuf<Tuple> tuples;
@@ -290,6 +292,8 @@ struct TermLevel {
 struct LevelH {
     std::vector<TermLevel> t;
     DBuf<double> blocks, norms;
+    DBuf<LevTerm> dterms;  // per-term screening data for the device box builder
+    DBuf<double> denv;     // concatenated norm envelopes of the active terms
     DBuf<float> blocks32;  // FP32 copy for the mixed-precision contraction (made on first use)
     bool has32 = false;
 };
@@ -442,6 +446,25 @@ void OperH::build_level(int d) {
             }
         }
     }
+    std::vector<LevTerm> lts(nt);
+    std::vector<double> envs;
+    for (int t = 0; t < nt; t++) {
+        const TermLevel &tl = L->t[t];
+        LevTerm &x = lts[t];
+        std::memset(&x, 0, sizeof(x));
+        x.w = weff[t];
+        x.active = tl.active ? 1 : 0;
+        x.band = tl.band;
+        if (!tl.active) continue;
+        x.blk = tl.blk;
+        x.nrm = tl.nrm;
+        x.env = (long long)envs.size();
+        for (int q = 0; q < 4; q++) x.maxn[q] = tl.maxn[q];
+        envs.insert(envs.end(), tl.env.begin(), tl.env.end());
+    }
+    if (envs.empty()) envs.push_back(0.0);
+    L->dterms.upload(lts.data(), lts.size(), C.stream);
+    L->denv.upload(envs.data(), envs.size(), C.stream);
     lev[d] = std::move(L);
 }
 
@@ -592,83 +615,6 @@ static void child_features(const JobH &J, const MraH &M, int p, int c, double tn
     f[7] = nb > 0.0 ? lg(nb / inn) : -20.0f;
     f[8] = (float)std::log10(J.op->mu + 1e-3);
     f[9] = (float)std::log10(J.prec);
-}
-
-struct Box3 {
-    short m[3];
-};
-
-static void job_boxes(const JobH &J, int d, std::vector<TermTab> &tabs, int &tab0, int &ntab, double &thr,
-                      double &route) {
-    const OperH &O = *J.op;
-    const LevelH &L = *O.lev[d];
-    const int nterms = O.nterms();
-    thr = J.gsq > 0.0 ? SCREEN_REL * J.prec * std::sqrt(J.gsq / nterms) : 0.0;
-    tab0 = (int)tabs.size();
-    double bmax = 0.0;
-    for (int t = 0; t < nterms; t++) {
-        const TermLevel &lv = L.t[t];
-        if (!lv.active) continue;
-        TermTab tb;
-        std::memset(&tb, 0, sizeof(tb));
-        tb.w = O.weff[t];
-        tb.band = lv.band;
-        tb.blk = lv.blk;
-        tb.nrm = lv.nrm;
-        for (int q = 0; q < 3; q++) {
-            tb.ub[q] = -1;
-            tb.ub0[q] = -1;
-        }
-        for (int combo = 0; combo < 64; combo++)
-            for (int q = 0; q < 3; q++) tb.box[combo][q] = -1;
-        bool any = false;
-        for (int gt = 0; gt < 8; gt++)
-            for (int ft = 0; ft < 8; ft++) {
-                if (d > 0 && gt == 0 && ft == 0) continue;
-                const double F = J.fmax[d][ft];
-                if (F <= 0.0) continue;
-                int ty[3];
-                for (int q = 0; q < 3; q++) ty[q] = 2 * ((gt >> q) & 1) + ((ft >> q) & 1);
-                const double base = std::fabs(O.weff[t]) * F;
-                const double full = base * lv.maxn[ty[0]] * lv.maxn[ty[1]] * lv.maxn[ty[2]];
-                if (!(full > thr)) continue;
-                bmax = std::max(bmax, full);
-                short m3[3];
-                bool ok = true;
-                for (int q = 0; q < 3; q++) {
-                    double other = base;
-                    for (int q2 = 0; q2 < 3; q2++)
-                        if (q2 != q) other *= lv.maxn[ty[q2]];
-                    const double tau = thr / other;
-                    // env is non-increasing in m: largest m with env[m] > tau
-                    const double *env = &lv.env[ty[q] * (lv.band + 1)];
-                    int m = -1;
-                    if (env[0] > tau) {
-                        int lo = 0, hi = lv.band;
-                        while (lo < hi) {
-                            const int mid = (lo + hi + 1) / 2;
-                            if (env[mid] > tau) lo = mid;
-                            else hi = mid - 1;
-                        }
-                        m = lo;
-                    }
-                    if (m < 0) ok = false;
-                    m3[q] = (short)m;
-                }
-                if (!ok) continue;
-                any = true;
-                for (int q = 0; q < 3; q++) {
-                    tb.box[gt * 8 + ft][q] = m3[q];
-                    tb.ub[q] = std::max(tb.ub[q], m3[q]);
-                    if (ft == 0) tb.ub0[q] = std::max(tb.ub0[q], m3[q]);
-                }
-            }
-        if (!any) continue;
-        tabs.push_back(tb);
-    }
-    ntab = (int)tabs.size() - tab0;
-    if (ntab > 255) throw Error(MWG_ERR_RANGE, "more than 255 active terms at one depth");
-    route = thr > 0.0 ? ROUTE_FP64 * thr : ROUTE_ROOT * bmax;
 }
 
 
@@ -841,7 +787,9 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             run = std::max(run, leafmax[d]);
         }
         J.out->make_roots();
-        J.out->ensure_dev(std::max(64, J.out->n()));
+        // output trees usually end near the input's size: reserve once instead of
+        // growing (each growth is an allocation, a copy and a free)
+        J.out->ensure_dev(std::max<size_t>(64, (size_t)J.out->n() + J.in->n() + J.in->n() / 4));
         J.work = J.out->roots;
         uint32_t cap = 1u << 14;
         while (cap < 4u * (uint32_t)J.in->n()) cap <<= 1;
@@ -879,13 +827,15 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             if (!jobs[j].work.empty()) active.push_back(j);
         if (active.empty()) break;
         // one job table per (job, depth) present in the frontier
-        std::vector<TermTab> tabs;
+        std::vector<BoxJob> bjobs;
+        int ntabs = 0, max_terms = 0;
         std::vector<JobTab> jt;
         std::vector<int> jt_job;
         std::vector<WorkItem> work;
+        for (int j : active) jobs[j].out->ensure_dev(jobs[j].out->n());
+        PT.lap("ensure_dev");
         for (int j : active) {
             JobH &J = jobs[j];
-            J.out->ensure_dev(J.out->n());
             std::vector<int> depths;
             for (int g : J.work) depths.push_back(J.out->depth[g]);
             std::sort(depths.begin(), depths.end());
@@ -895,7 +845,26 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                 J.op->ensure(d);
                 JobTab T;
                 std::memset(&T, 0, sizeof(T));
-                job_boxes(J, d, tabs, T.tab0, T.ntab, T.thr, T.route);
+                {
+                    const LevelH &LV = *J.op->lev[d];
+                    const int nt = J.op->nterms();
+                    T.thr = J.gsq > 0.0 ? SCREEN_REL * J.prec * std::sqrt(J.gsq / nt) : 0.0;
+                    T.tab0 = ntabs;
+                    T.ntab = nt;
+                    BoxJob B;
+                    std::memset(&B, 0, sizeof(B));
+                    B.terms = LV.dterms.p;
+                    B.env = LV.denv.p;
+                    B.thr = T.thr;
+                    for (int q = 0; q < 8; q++) B.F[q] = J.fmax[d][q];
+                    B.nterms = nt;
+                    B.depth = d;
+                    B.tab0 = ntabs;
+                    B.jt = (int)jt.size();
+                    bjobs.push_back(B);
+                    ntabs += nt;
+                    max_terms = std::max(max_terms, nt);
+                }
                 T.inHash = J.in->dhash();
                 T.genHash = DHash{C.genHashKeys.p + J.genOff, C.genHashVals.p + J.genOff, J.genMask};
                 T.inCoef = J.in->coef.p;
@@ -926,12 +895,10 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         PT.lap("opers+boxes");
         upload_async(C, C.work, work.data(), work.size());
         upload_async(C, C.jobs, jt.data(), jt.size());
-        if (tabs.empty()) {
-            TermTab dummy;
-            std::memset(&dummy, 0, sizeof(dummy));
-            tabs.push_back(dummy);
-        }
-        upload_async(C, C.tabs, tabs.data(), tabs.size());
+        upload_async(C, C.bjobs, bjobs.data(), bjobs.size());
+        C.tabs.alloc((size_t)std::max(ntabs, 1));
+        C.tbmax.alloc((size_t)std::max<size_t>(bjobs.size() * max_terms, 1));
+        HC(launch_job_boxes(s, (int)bjobs.size(), max_terms, C.bjobs.p, C.tabs.p, C.tbmax.p, C.jobs.p));
         // generated input nodes
         HC(hipMemsetAsync(C.genOverflow.p, 0, sizeof(int), s));
         GenState gs{C.genKeys.p, C.genJob.p, C.genCount.p, genCap, C.genOverflow.p, C.genPool.p, C.genNorm.p};
@@ -960,6 +927,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             HC(launch_enum(s, 0, nw, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, nullptr, nullptr));
             std::vector<int> tot((size_t)nw * 8);
             download_sync(C, C.gtotal, tot.data(), tot.size());
+            PT.lap("enum_count");
             std::vector<long long> ofs((size_t)nw * 8);
             std::vector<ContractTask> tasks;
             std::vector<ReduceItem> reds;
@@ -1004,6 +972,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                                  nullptr, nullptr));
             std::vector<int> tot((size_t)nw * 9);
             download_sync(C, C.gtotal, tot.data(), tot.size());
+            PT.lap("enum_count");
             std::vector<long long> ofs((size_t)nw * 9);
             std::vector<ContractTask> tasks;
             std::vector<SliceTask> stasks;
@@ -1783,7 +1752,8 @@ int mwg_apply_batch(int n, const double *prec, mwg_tree *out, mwg_oper *oper, mw
     Ctx *C = inp[0]->mra->ctx;
     return guarded(C, [&] {
         if (flags & ~(MWG_APPLY_MIXED | MWG_APPLY_PREDICT)) throw Error(MWG_ERR_ARG, "unknown apply flags");
-        if (flags & MWG_APPLY_PREDICT) throw Error(MWG_ERR_ARG, "the refinement predictor is not available");
+        if ((flags & MWG_APPLY_PREDICT) && !C->pred)
+            throw Error(MWG_ERR_ARG, "MWG_APPLY_PREDICT needs a predictor installed with mwg_ctx_set_predictor");
         std::vector<JobH> jobs(n);
         for (int i = 0; i < n; i++) {
             if (!(prec[i] > 0.0)) throw Error(MWG_ERR_ARG, "apply precision must be positive");
@@ -1929,6 +1899,24 @@ int mwg_predictor_train(mwg_predictor pred, int64_t n, const float *X, const flo
         HC(hipStreamSynchronize(s));
         if (loss_after) *loss_after = predictor_loss(*pred, (int)n, y);
     });
+}
+
+int mwg_ctx_set_tuning(mwg_ctx ctx, int fp64_chunk, int slice_chunk) {
+    if (!ctx) return MWG_ERR_ARG;
+    if (fp64_chunk < 1 || fp64_chunk > 4096 || slice_chunk < 1 || slice_chunk > 1024) {
+        ctx->err = "mwg_ctx_set_tuning: fp64_chunk must be in [1, 4096] and slice_chunk in [1, 1024]";
+        return MWG_ERR_ARG;
+    }
+    ctx->chunk = fp64_chunk;
+    ctx->slice_chunk = slice_chunk;
+    return MWG_OK;
+}
+
+int mwg_ctx_get_tuning(mwg_ctx ctx, int *fp64_chunk, int *slice_chunk) {
+    if (!ctx || !fp64_chunk || !slice_chunk) return MWG_ERR_ARG;
+    *fp64_chunk = ctx->chunk;
+    *slice_chunk = ctx->slice_chunk;
+    return MWG_OK;
 }
 
 int mwg_ctx_set_predictor(mwg_ctx ctx, mwg_predictor pred, float threshold) {

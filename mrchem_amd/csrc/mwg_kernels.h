@@ -48,6 +48,29 @@ struct JobTab {
     int depth;
 };
 
+// one Gaussian term of one operator level, as the screening-box kernel reads it
+struct LevTerm {
+    double w;        // weight (unit coordinates)
+    long long blk;   // block offset (doubles) in the level block array
+    long long nrm;   // norm offset (doubles) in the level norm array
+    long long env;   // offset of the term's 4 x (band + 1) norm envelopes in the level envelope array
+    double maxn[4];  // largest block norm per type over the band
+    int band;
+    int active;
+};
+
+// screening-box request for one (job, depth) table
+struct BoxJob {
+    const LevTerm *terms;
+    const double *env;
+    double thr;   // tuple threshold of the job at this depth (0 on the roots)
+    double F[8];  // largest input component norm per component at this depth
+    int nterms;
+    int depth;
+    int tab0;     // first TermTab of this table
+    int jt;       // index of the JobTab that receives the routing bound
+};
+
 struct GenState {
     uint64_t *keys;
     int *job;
@@ -76,6 +99,10 @@ hipError_t launch_project_gauss(hipStream_t s, int nw, const int *work, const in
 hipError_t launch_project_hyd(hipStream_t s, int nw, const int *work, const int *depth, const int *lr,
                               const ProjGeom &geo, const DevBasis &B, int nterm, const double *coef, const int *q,
                               double Z, const double *origin, double *out);
+// TermTab[tab0 + t] for every (table, term): per-combination screening boxes from the
+// monotone norm envelopes, union boxes, and the table's FP64 routing bound
+hipError_t launch_job_boxes(hipStream_t s, int nbj, int max_terms, const BoxJob *bj, TermTab *tabs, double *tbmax,
+                            JobTab *jobs);
 hipError_t launch_gen_mark(hipStream_t s, int nw, const WorkItem *work, const JobTab *jobs, const TermTab *tabs,
                            const GenState &gs);
 hipError_t launch_gen_compute(hipStream_t s, int first, int n, const JobTab *jobs, const GenState &gs,

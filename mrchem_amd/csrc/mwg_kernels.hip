@@ -735,6 +735,103 @@ __device__ inline void walk_candidates(const CandTerms &ct, const int *fcache, c
     }
 }
 
+// ----------------------------------------------------------------- apply: screening boxes
+
+// One wavefront per (table, term); lane = combination gt*8+ft.  For the bound
+// |w| F_ft prod_d ||X_d|| of the combination to exceed thr somewhere, each dimension's
+// translation must stay inside the largest m with env_type[m] > thr / (|w| F_ft prod of
+// the other two dimensions' largest norms); env is non-increasing in m.
+__device__ inline int wave_max_int(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ inline double wave_max_double(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__global__ void __launch_bounds__(64) k_job_boxes(int nbj, const BoxJob *bjs, TermTab *tabs, double *tbmax,
+                                                  int max_terms) {
+    const int jb = blockIdx.x, t = blockIdx.y;
+    if (jb >= nbj) return;
+    const BoxJob &B = bjs[jb];
+    if (t >= B.nterms) return;
+    const LevTerm lt = B.terms[t];
+    TermTab *tb = tabs + B.tab0 + t;
+    const int c = threadIdx.x, gt = c >> 3, ft = c & 7;
+    int m3[3] = {-1, -1, -1};
+    double full = 0.0;
+    bool ok = false;
+    if (lt.active && !(B.depth > 0 && c == 0)) {
+        const double F = B.F[ft];
+        if (F > 0.0) {
+            int ty[3];
+            for (int q = 0; q < 3; q++) ty[q] = 2 * ((gt >> q) & 1) + ((ft >> q) & 1);
+            const double base = fabs(lt.w) * F;
+            const double f3 = base * lt.maxn[ty[0]] * lt.maxn[ty[1]] * lt.maxn[ty[2]];
+            if (f3 > B.thr) {
+                full = f3;
+                ok = true;
+                for (int q = 0; q < 3; q++) {
+                    double other = base;
+                    for (int q2 = 0; q2 < 3; q2++)
+                        if (q2 != q) other *= lt.maxn[ty[q2]];
+                    const double tau = B.thr / other;
+                    const double *env = B.env + lt.env + (long long)ty[q] * (lt.band + 1);
+                    int m = -1;
+                    if (env[0] > tau) {
+                        int lo = 0, hi = lt.band;
+                        while (lo < hi) {
+                            const int mid = (lo + hi + 1) / 2;
+                            if (env[mid] > tau) lo = mid;
+                            else hi = mid - 1;
+                        }
+                        m = lo;
+                    }
+                    if (m < 0) ok = false;
+                    m3[q] = m;
+                }
+            }
+        }
+    }
+    int u[3], u0[3];
+    for (int q = 0; q < 3; q++) {
+        const short v = ok ? (short)m3[q] : (short)-1;
+        tb->box[c][q] = v;
+        u[q] = wave_max_int(v);
+        u0[q] = wave_max_int(ft == 0 ? v : -1);
+    }
+    const double bmax = wave_max_double(full);
+    if (c == 0) {
+        tb->w = lt.w;
+        tb->blk = lt.blk;
+        tb->nrm = lt.nrm;
+        tb->band = lt.band;
+        for (int q = 0; q < 3; q++) {
+            tb->ub[q] = (short)u[q];
+            tb->ub0[q] = (short)u0[q];
+        }
+        tbmax[(size_t)jb * max_terms + t] = bmax;
+    }
+}
+
+// FP64 routing bound of each table (mixed mode): slices whose bound exceeds it run in
+// FP64.  Above the roots it is a fixed multiple of the screening threshold; on the
+// roots (no screening) a fraction of the largest bound of the table.
+__global__ void __launch_bounds__(64) k_job_route(int nbj, const BoxJob *bjs, const double *tbmax, int max_terms,
+                                                  JobTab *jobs) {
+    const int jb = blockIdx.x;
+    if (jb >= nbj) return;
+    const BoxJob &B = bjs[jb];
+    double m = 0.0;
+    for (int t = threadIdx.x; t < B.nterms; t += 64) m = fmax(m, tbmax[(size_t)jb * max_terms + t]);
+    m = wave_max_double(m);
+    if (threadIdx.x == 0) jobs[B.jt].route = B.thr > 0.0 ? ROUTE_FP64 * B.thr : ROUTE_ROOT * m;
+}
+
 // ----------------------------------------------------------------- apply: work list
 
 // Enumerates, for one output node, every (term, delta, ft, gt) candidate inside the
@@ -1434,6 +1531,14 @@ hipError_t launch_project_hyd(hipStream_t s, int nw, const int *work, const int 
     }
     hipLaunchKernelGGL(k_project_hyd, dim3(nw), dim3(256), sm, s, nw, work, depth, lr, geo, B, nterm, coef, q, Z,
                        origin, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_job_boxes(hipStream_t s, int nbj, int max_terms, const BoxJob *bj, TermTab *tabs, double *tbmax,
+                            JobTab *jobs) {
+    if (nbj <= 0 || max_terms <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_job_boxes, dim3(nbj, max_terms), dim3(64), 0, s, nbj, bj, tabs, tbmax, max_terms);
+    hipLaunchKernelGGL(k_job_route, dim3(nbj), dim3(64), 0, s, nbj, bj, tbmax, max_terms, jobs);
     return hipGetLastError();
 }
 

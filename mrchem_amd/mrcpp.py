@@ -43,6 +43,15 @@ class Context:
     def sync(self):
         check(lib().mwg_sync(self.h), self.h)
 
+    def set_tuning(self, fp64_chunk: int, slice_chunk: int):
+        """Contraction task granularity: tuples per FP64 wave task, MFMA slices per wave task."""
+        check(lib().mwg_ctx_set_tuning(self.h, int(fp64_chunk), int(slice_chunk)), self.h)
+
+    def get_tuning(self) -> tuple[int, int]:
+        a, b = C.c_int(), C.c_int()
+        check(lib().mwg_ctx_get_tuning(self.h, C.byref(a), C.byref(b)), self.h)
+        return int(a.value), int(b.value)
+
     def __del__(self):
         if getattr(self, "h", None) and _lib._LIB is not None:
             _lib._LIB.mwg_ctx_destroy(self.h)
