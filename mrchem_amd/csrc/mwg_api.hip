@@ -262,6 +262,9 @@ struct TreeH {
         HC(launch_norms(s, n(), nullptr, coef.p, norm.p, K3()));
         cnorm.resize(8 * (size_t)n());
         norm.download(cnorm.data(), 8 * (size_t)n(), s);
+        sqnorm_from_cnorm();
+    }
+    void sqnorm_from_cnorm() {  // compressed tree: root scaling parts + every wavelet part
         double sq = 0.0;
         for (int r : roots) sq += cnorm[8 * r] * cnorm[8 * r];
         for (int i = 0; i < n(); i++)
@@ -553,7 +556,7 @@ static void project_adaptive(TreeH &T, double prec, LaunchFn launch, const std::
     std::vector<TransformItem> items;
     for (int d = (int)L.size() - 1; d >= 0; d--) {
         items.clear();
-        for (int p : L[d]) items.push_back({T.coef.p, p, T.child[p]});
+        for (int p : L[d]) items.push_back({T.coef.p, nullptr, p, T.child[p]});
         C.titems.upload(items.data(), items.size(), s);
         HC(launch_transform(s, 0, (int)items.size(), C.titems.p, M.W.p, M.B.K));
     }
@@ -578,7 +581,10 @@ struct JobH {
 };
 
 // features of a freshly created child c of node p (see mwg_predict.h)
-static void child_features(const JobH &J, const MraH &M, int p, int c, double tnorm, float *f) {
+// Features of the 8 children of a node that was just split (one row of MLP_FEATURES
+// each, at f + a * MLP_FEATURES for child a).  The input-tree neighbourhoods of the
+// 8 siblings lie in one 4 x 4 x 4 block of translations, looked up once.
+static void children_features(const JobH &J, const MraH &M, int p, double tnorm, float *f) {
     const TreeH &T = *J.out;
     const double *pn = &T.cnorm[8 * (size_t)p];
     double w2 = 0.0;
@@ -587,36 +593,50 @@ static void child_features(const JobH &J, const MraH &M, int p, int c, double tn
     const int np = M.n0 + T.depth[p];
     const double thr = std::max(2.0 * 2.220446049250313e-16, J.prec * tnorm * std::pow(2.0, -0.5 * (np + 1)));
     auto lg = [](double v) { return (float)std::max(-20.0, std::min(20.0, std::log10(std::max(v, 1e-300)))); };
-    f[0] = lg(wn / thr);
-    f[1] = lg(wn / tnorm);
-    f[2] = lg(sn / tnorm);
-    const int d = T.depth[c];
-    f[3] = (float)(M.n0 + d);
+    const float f0 = lg(wn / thr), f1 = lg(wn / tnorm), f2 = lg(sn / tnorm);
+    const int c0 = T.child[p];
+    const int d = T.depth[c0];
     const double inn = std::sqrt(std::max(J.in->sqnorm, 1e-300));
-    const int x = T.lr[3 * c], y = T.lr[3 * c + 1], z = T.lr[3 * c + 2];
-    auto in_w = [&](int i) {
+    const float f8 = (float)std::log10(J.op->mu + 1e-3), f9 = (float)std::log10(J.prec);
+    const int bx = 2 * T.lr[3 * p] - 1, by = 2 * T.lr[3 * p + 1] - 1, bz = 2 * T.lr[3 * p + 2] - 1;
+    int idx[64];
+    double wv[64];
+    for (int k = 0; k < 64; k++) {
+        const int x = bx + (k & 3), y = by + ((k >> 2) & 3), z = bz + (k >> 4);
+        idx[k] = (x < 0 || y < 0 || z < 0) ? -1 : J.in->find(d, x, y, z);
         double a = 0.0;
-        for (int q = 1; q < 8; q++) a += J.in->cnorm[8 * (size_t)i + q] * J.in->cnorm[8 * (size_t)i + q];
-        return std::sqrt(a);
-    };
-    const int i0 = J.in->find(d, x, y, z);
-    f[4] = i0 >= 0 ? 1.0f : 0.0f;
-    f[5] = i0 >= 0 ? lg(in_w(i0) / inn) : -20.0f;
-    f[6] = (i0 >= 0 && J.in->child[i0] >= 0) ? 1.0f : 0.0f;
-    double nb = 0.0;
-    for (int dz = -1; dz <= 1; dz++)
-        for (int dy = -1; dy <= 1; dy++)
-            for (int dx = -1; dx <= 1; dx++) {
-                if (!dx && !dy && !dz) continue;
-                if (x + dx < 0 || y + dy < 0 || z + dz < 0) continue;
-                const int i = J.in->find(d, x + dx, y + dy, z + dz);
-                if (i >= 0) nb = std::max(nb, in_w(i));
-            }
-    f[7] = nb > 0.0 ? lg(nb / inn) : -20.0f;
-    f[8] = (float)std::log10(J.op->mu + 1e-3);
-    f[9] = (float)std::log10(J.prec);
+        if (idx[k] >= 0)
+            for (int q = 1; q < 8; q++) a += J.in->cnorm[8 * (size_t)idx[k] + q] * J.in->cnorm[8 * (size_t)idx[k] + q];
+        wv[k] = std::sqrt(a);
+    }
+    for (int a = 0; a < 8; a++) {
+        const int c = c0 + a;
+        float *fc = f + (size_t)a * MLP_FEATURES;
+        // child position inside the block: offset 1 + its bit per dimension
+        const int cx = 1 + (int)(T.lr[3 * c] - 2 * T.lr[3 * p]), cy = 1 + (int)(T.lr[3 * c + 1] - 2 * T.lr[3 * p + 1]),
+                  cz = 1 + (int)(T.lr[3 * c + 2] - 2 * T.lr[3 * p + 2]);
+        const int k0 = cx + 4 * cy + 16 * cz;
+        fc[0] = f0;
+        fc[1] = f1;
+        fc[2] = f2;
+        fc[3] = (float)(M.n0 + d);
+        const int i0 = idx[k0];
+        fc[4] = i0 >= 0 ? 1.0f : 0.0f;
+        fc[5] = i0 >= 0 ? lg(wv[k0] / inn) : -20.0f;
+        fc[6] = (i0 >= 0 && J.in->child[i0] >= 0) ? 1.0f : 0.0f;
+        double nb = 0.0;
+        for (int dz = -1; dz <= 1; dz++)
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dx = -1; dx <= 1; dx++) {
+                    if (!dx && !dy && !dz) continue;
+                    const int k = (cx + dx) + 4 * (cy + dy) + 16 * (cz + dz);
+                    if (idx[k] >= 0) nb = std::max(nb, wv[k]);
+                }
+        fc[7] = nb > 0.0 ? lg(nb / inn) : -20.0f;
+        fc[8] = f8;
+        fc[9] = f9;
+    }
 }
-
 
 // ------------------------------------------------------------------ learned predictor
 
@@ -806,10 +826,8 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
     C.genJob.alloc(genCap);
     C.genCount.alloc(1);
     C.genOverflow.alloc(1);
-    C.genNorm.alloc(1024);
-    C.genPool.alloc(1024 * (size_t)K3);
     HC(hipMemsetAsync(C.genCount.p, 0, sizeof(int), s));
-    int genDone = 0;
+    int genDone = 0, genPoolCap = 0, genLastNew = 0;
     PT.lap("setup");
 
     int iter = 0;
@@ -899,34 +917,62 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         C.tabs.alloc((size_t)std::max(ntabs, 1));
         C.tbmax.alloc((size_t)std::max<size_t>(bjobs.size() * max_terms, 1));
         HC(launch_job_boxes(s, (int)bjobs.size(), max_terms, C.bjobs.p, C.tabs.p, C.tbmax.p, C.jobs.p));
-        // generated input nodes
+        // generated input nodes: marked, then computed for the range the device counter
+        // reached (read back with the work-list counts below, not before)
         HC(hipMemsetAsync(C.genOverflow.p, 0, sizeof(int), s));
-        GenState gs{C.genKeys.p, C.genJob.p, C.genCount.p, genCap, C.genOverflow.p, C.genPool.p, C.genNorm.p};
-        HC(launch_gen_mark(s, nw, C.work.p, C.jobs.p, C.tabs.p, gs));
-        int hc[2];
-        HC(hipMemcpyAsync(&hc[0], C.genCount.p, sizeof(int), hipMemcpyDeviceToHost, s));
-        HC(hipMemcpyAsync(&hc[1], C.genOverflow.p, sizeof(int), hipMemcpyDeviceToHost, s));
-        HC(hipStreamSynchronize(s));
-        C.stage.off = 0;
-        if (hc[1]) throw Error(MWG_ERR_RANGE, "generated-node table overflow");
-        PT.lap("upload+genmark");
-        const int genNow = hc[0];
-        if (genNow > genDone) {
-            C.genPool.grow_keep((size_t)genNow * K3, s);
-            C.genNorm.grow_keep((size_t)genNow, s);
-            gs.pool = C.genPool.p;
-            gs.norm = C.genNorm.p;
-            HC(launch_gen_compute(s, genDone, genNow - genDone, C.jobs.p, gs, M.W.p, K));
-            genDone = genNow;
+        {
+            const int want = std::min(genCap, genDone + std::max(4096, 2 * genLastNew + 1024));
+            if (genPoolCap < want) {
+                C.genPool.grow_keep((size_t)want * K3, s);
+                C.genNorm.grow_keep((size_t)want, s);
+                genPoolCap = want;
+            }
         }
-        PT.lap("gencompute");
+        GenState gs{C.genKeys.p, C.genJob.p, C.genCount.p, genCap, C.genOverflow.p, C.genPool.p, C.genNorm.p,
+                    genPoolCap};
+        HC(launch_gen_mark(s, nw, C.work.p, C.jobs.p, C.tabs.p, gs));
+        HC(launch_gen_compute(s, genDone, genPoolCap, C.jobs.p, gs, M.W.p, K));
+        PT.lap("upload+genmark");
+        // work-list counts per segment (nseg segments per output node), with the generated
+        // node count; when the pool was too small, grow it, finish the generated nodes and
+        // count again
+        auto count_segments = [&](int nseg, std::vector<int> &tot) {
+            C.thread_off.alloc((size_t)nw * nseg * 256);
+            C.gtotal.alloc((size_t)nw * nseg);
+            tot.assign((size_t)nw * nseg, 0);
+            for (int attempt = 0;; attempt++) {
+                if (nseg == 8)
+                    HC(launch_enum(s, 0, nw, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, nullptr,
+                                   nullptr));
+                else
+                    HC(launch_enum_mixed(s, 0, nw, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p,
+                                         nullptr, nullptr, nullptr));
+                int hc[2];
+                HC(hipMemcpyAsync(&hc[0], C.genCount.p, sizeof(int), hipMemcpyDeviceToHost, s));
+                HC(hipMemcpyAsync(&hc[1], C.genOverflow.p, sizeof(int), hipMemcpyDeviceToHost, s));
+                download_sync(C, C.gtotal, tot.data(), tot.size());
+                if (hc[1]) throw Error(MWG_ERR_RANGE, "generated-node table overflow");
+                const int genNow = hc[0];
+                if (genNow <= genPoolCap) {
+                    genLastNew = genNow - genDone;
+                    genDone = genNow;
+                    return;
+                }
+                if (attempt > 0) throw Error(MWG_ERR_RANGE, "generated-node pool sizing failed");
+                const int want = std::min(genCap, genNow + std::max(4096, genNow / 2));
+                C.genPool.grow_keep((size_t)want * K3, s);
+                C.genNorm.grow_keep((size_t)want, s);
+                gs.pool = C.genPool.p;
+                gs.norm = C.genNorm.p;
+                gs.pcap = want;
+                HC(launch_gen_compute(s, genPoolCap, want, C.jobs.p, gs, M.W.p, K));
+                genPoolCap = want;
+            }
+        };
         if (!mixed) {
             // work list: tuples grouped by segment (g, gt)
-            C.thread_off.alloc((size_t)nw * 8 * 256);
-            C.gtotal.alloc((size_t)nw * 8);
-            HC(launch_enum(s, 0, nw, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, nullptr, nullptr));
-            std::vector<int> tot((size_t)nw * 8);
-            download_sync(C, C.gtotal, tot.data(), tot.size());
+            std::vector<int> tot;
+            count_segments(8, tot);
             PT.lap("enum_count");
             std::vector<long long> ofs((size_t)nw * 8);
             std::vector<ContractTask> tasks;
@@ -966,12 +1012,8 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             HC(hipEventRecord(C.ev1, s));
         } else {
             // mixed: FP64 tuples for the large bounds, FP32 MFMA slices for the rest
-            C.thread_off.alloc((size_t)nw * 9 * 256);
-            C.gtotal.alloc((size_t)nw * 9);
-            HC(launch_enum_mixed(s, 0, nw, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, nullptr,
-                                 nullptr, nullptr));
-            std::vector<int> tot((size_t)nw * 9);
-            download_sync(C, C.gtotal, tot.data(), tot.size());
+            std::vector<int> tot;
+            count_segments(9, tot);
             PT.lap("enum_count");
             std::vector<long long> ofs((size_t)nw * 9);
             std::vector<ContractTask> tasks;
@@ -984,9 +1026,11 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                 ofs[(size_t)i * 9 + 8] = stotal;
                 jobs[jt_job[work[i].job]].slices += cs;
                 const int nn0 = nnslots;
-                for (int h = 0; h * C.slice_chunk < cs; h++) {
-                    const int cc = std::min(C.slice_chunk, cs - h * C.slice_chunk);
-                    stasks.push_back({stotal + (long long)h * C.slice_chunk, cc, i, nnslots++});
+                // one workgroup task = up to 4 waves x slice_chunk slices of this node
+                const int wchunk = 4 * C.slice_chunk;
+                for (int h = 0; h * wchunk < cs; h++) {
+                    const int cc = std::min(wchunk, cs - h * wchunk);
+                    stasks.push_back({stotal + (long long)h * wchunk, cc, i, nnslots++});
                 }
                 stotal += cs;
                 for (int q = 0; q < 8; q++) {
@@ -1089,11 +1133,9 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                         J.has_feat.resize(J.out->n(), 0);
                         if (C.tracing || use_pred) {
                             J.feat.resize((size_t)J.out->n() * MLP_FEATURES, 0.0f);
-                            for (int a = 0; a < 8; a++) {
-                                const int c = J.out->child[g] + a;
-                                child_features(J, M, g, c, tnorm, &J.feat[(size_t)c * MLP_FEATURES]);
-                                J.has_feat[c] = 1;
-                            }
+                            const int c0 = J.out->child[g];
+                            children_features(J, M, g, tnorm, &J.feat[(size_t)c0 * MLP_FEATURES]);
+                            for (int a = 0; a < 8; a++) J.has_feat[c0 + a] = 1;
                         }
                         for (int a = 0; a < 8; a++) {
                             next.push_back(J.out->child[g] + a);
@@ -1147,24 +1189,35 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         L[j] = jobs[j].out->internal_by_depth();
         maxd = std::max(maxd, (int)L[j].size());
     }
+    // one item list for all levels (uploaded once); the same items serve both directions.
+    // The transforms also write the norms they change: the children's scaling norms
+    // (top-down; final for leaves) and all 8 norms of every parent (bottom-up)
     std::vector<TransformItem> items;
+    std::vector<size_t> lofs(maxd + 1, 0);
     for (int d = 0; d < maxd; d++) {
-        items.clear();
+        lofs[d] = items.size();
         for (size_t j = 0; j < jobs.size(); j++)
             if (d < (int)L[j].size())
-                for (int p : L[j][d]) items.push_back({jobs[j].out->coef.p, p, jobs[j].out->child[p]});
-        upload_async(C, C.titems, items.data(), items.size());
-        HC(launch_transform(s, 1, (int)items.size(), C.titems.p, M.WT.p, K));
+                for (int p : L[j][d])
+                    items.push_back({jobs[j].out->coef.p, jobs[j].out->norm.p, p, jobs[j].out->child[p]});
     }
-    for (int d = maxd - 1; d >= 0; d--) {
-        items.clear();
-        for (size_t j = 0; j < jobs.size(); j++)
-            if (d < (int)L[j].size())
-                for (int p : L[j][d]) items.push_back({jobs[j].out->coef.p, p, jobs[j].out->child[p]});
+    lofs[maxd] = items.size();
+    if (!items.empty()) {
         upload_async(C, C.titems, items.data(), items.size());
-        HC(launch_transform(s, 0, (int)items.size(), C.titems.p, M.W.p, K));
+        for (int d = 0; d < maxd; d++)
+            HC(launch_transform(s, 1, (int)(lofs[d + 1] - lofs[d]), C.titems.p + lofs[d], M.WT.p, K));
+        for (int d = maxd - 1; d >= 0; d--)
+            HC(launch_transform(s, 0, (int)(lofs[d + 1] - lofs[d]), C.titems.p + lofs[d], M.W.p, K));
     }
-    for (JobH &J : jobs) J.out->refresh_norms();
+    for (JobH &J : jobs) {
+        J.out->cnorm.resize(8 * (size_t)J.out->n());
+        if (J.out->n())
+            HC(hipMemcpyAsync(J.out->cnorm.data(), J.out->norm.p, 8 * (size_t)J.out->n() * sizeof(double),
+                              hipMemcpyDeviceToHost, s));
+    }
+    HC(hipStreamSynchronize(s));
+    C.stage.off = 0;
+    for (JobH &J : jobs) J.out->sqnorm_from_cnorm();
     PT.lap("transforms");
     PT.report();
     const double t_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
@@ -1270,7 +1323,7 @@ static void multiply_trees(TreeH &out, double prec, double coef, TreeH &A, TreeH
     std::vector<TransformItem> titems;
     for (int d = (int)L.size() - 1; d >= 0; d--) {
         titems.clear();
-        for (int p : L[d]) titems.push_back({out.coef.p, p, out.child[p]});
+        for (int p : L[d]) titems.push_back({out.coef.p, nullptr, p, out.child[p]});
         C.titems.upload(titems.data(), titems.size(), s);
         HC(launch_transform(s, 0, (int)titems.size(), C.titems.p, M.W.p, M.B.K));
     }

@@ -77,8 +77,9 @@ struct GenState {
     int *count;
     int cap;
     int *overflow;
-    double *pool;  // cap x K^3
-    double *norm;  // cap
+    double *pool;  // pcap x K^3
+    double *norm;  // pcap
+    int pcap;      // entries allocated in pool and norm (<= cap)
 };
 
 hipError_t launch_tblock(hipStream_t s, int n, const int *item_term, const long long *item_delta, const double *b,
@@ -105,7 +106,8 @@ hipError_t launch_job_boxes(hipStream_t s, int nbj, int max_terms, const BoxJob 
                             JobTab *jobs);
 hipError_t launch_gen_mark(hipStream_t s, int nw, const WorkItem *work, const JobTab *jobs, const TermTab *tabs,
                            const GenState &gs);
-hipError_t launch_gen_compute(hipStream_t s, int first, int n, const JobTab *jobs, const GenState &gs,
+// generated nodes first .. min(device count, cap) - 1 (cap = entries allocated in the pool)
+hipError_t launch_gen_compute(hipStream_t s, int first, int cap, const JobTab *jobs, const GenState &gs,
                               const double *W, int K);
 hipError_t launch_enum(hipStream_t s, int mode, int nw, const WorkItem *work, const JobTab *jobs,
                        const TermTab *tabs, const GenState &gs, int *thread_off, int *seg_total,
@@ -114,6 +116,7 @@ hipError_t launch_work_norms(hipStream_t s, int nw, const WorkItem *work, const 
                              double *compact);
 struct TransformItem {
     double *coef;
+    double *norm;  // component norms of the tree (null: not updated)
     int parent;
     int first_child;
 };

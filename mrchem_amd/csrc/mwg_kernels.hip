@@ -6,6 +6,7 @@
 // index is bit*K + i; each 1D pass multiplies every line of 2K values in place.
 #include "mwg_kernels.h"
 
+#include <algorithm>
 #include <climits>
 #include <cmath>
 
@@ -241,6 +242,51 @@ __device__ inline int interleaved_index(int K, int c, int e) {
     const int x = e % K, y = (e / K) % K, z = e / (K * K);
     const int bx = c & 1, by = (c >> 1) & 1, bz = (c >> 2) & 1;
     return ((bz * K + z) * N + (by * K + y)) * N + bx * K + x;
+}
+
+// (2K)^3 node tensor in LDS with the x rows padded to N + 1 doubles: x-lines (stride 1)
+// of neighbouring lanes then start 2(N+1) banks apart instead of 2N (no conflicts)
+template <int K>
+struct PadT {
+    static constexpr int N = 2 * K, P = N + 1, PL = N * P, SIZE = N * PL;
+};
+
+template <int K>
+__device__ inline int padded_index(int c, int e) {
+    constexpr int N = 2 * K, P = N + 1;
+    const int x = e % K, y = (e / K) % K, z = e / (K * K);
+    const int bx = c & 1, by = (c >> 1) & 1, bz = (c >> 2) & 1;
+    return ((bz * K + z) * N + (by * K + y)) * P + bx * K + x;
+}
+
+// in-place N x N filter on every line of the padded tensor along dim (0 = x)
+template <int N>
+__device__ inline void line_pass_padded(double *T, const double *M, int dim) {
+    constexpr int P = N + 1, N2 = N * N;
+    for (int line = threadIdx.x; line < N2; line += blockDim.x) {
+        const int u = line % N, v = line / N;
+        int base, stride;
+        if (dim == 0) {
+            base = line * P;
+            stride = 1;
+        } else if (dim == 1) {
+            base = v * N * P + u;
+            stride = P;
+        } else {
+            base = v * P + u;
+            stride = N * P;
+        }
+        double in[N];
+#pragma unroll
+        for (int q = 0; q < N; q++) in[q] = T[base + q * stride];
+#pragma unroll
+        for (int p = 0; p < N; p++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int q = 0; q < N; q++) acc += M[p * N + q] * in[q];
+            T[base + p * stride] = acc;
+        }
+    }
 }
 
 // component norms of listed nodes (one wave per node)
@@ -509,12 +555,10 @@ __global__ void __launch_bounds__(256) k_gen_mark(int nw, const WorkItem *work, 
 
 // scaling block of a generated node: reconstruct from the leaf ancestor (using its
 // wavelets for the first step), then refine with the low-pass filters only
-__global__ void k_gen_compute(int first, int n, const JobTab *jobs, GenState gs, const double *W, int K) {
-    extern __shared__ double smem[];
-    const int g = first + blockIdx.x;
-    if (g >= first + n) return;
-    const int N = 2 * K, N2 = N * N, K3 = K * K * K;
-    double *T = smem;
+template <int K>
+__device__ void gen_one(int g, const JobTab *jobs, const GenState &gs, double *T, const double *Ws) {
+    using L = PadT<K>;
+    constexpr int N = L::N, P = L::P, K3 = K * K * K;
     const uint64_t key = gs.keys[g];
     const JobTab &J = jobs[gs.job[g]];
     const int d = (int)(key >> 57);
@@ -539,10 +583,11 @@ __global__ void k_gen_compute(int first, int n, const JobTab *jobs, GenState gs,
     if (anc < 0) {
         for (int e = threadIdx.x; e < K3; e += blockDim.x) outp[e] = 0.0;
         if (threadIdx.x == 0) gs.norm[g] = 0.0;
+        __syncthreads();
         return;
     }
     const double *src = J.inCoef + (size_t)anc * 8 * K3;
-    for (int e = threadIdx.x; e < 8 * K3; e += blockDim.x) T[interleaved_index(K, e / K3, e % K3)] = src[e];
+    for (int e = threadIdx.x; e < 8 * K3; e += blockDim.x) T[padded_index<K>(e / K3, e % K3)] = src[e];
     __syncthreads();
     const int steps = d - ad;
     // first step: child a of the (s, d) parent; line (2K) -> K values, s_j = sum_p W[p][a*K+j] in_p
@@ -550,60 +595,79 @@ __global__ void k_gen_compute(int first, int n, const JobTab *jobs, GenState gs,
         const int sh = steps - 1;
         const int ax = (int)((l[0] >> sh) & 1), ay = (int)((l[1] >> sh) & 1), az = (int)((l[2] >> sh) & 1);
         // x: lines (z, y) over all N x N
-        for (int line = threadIdx.x; line < N2; line += blockDim.x) {
-            const int base = line * N;
-            double in[2 * MAXK];
+        for (int line = threadIdx.x; line < N * N; line += blockDim.x) {
+            const int base = line * P;
+            double in[N];
+#pragma unroll
             for (int p = 0; p < N; p++) in[p] = T[base + p];
+#pragma unroll
             for (int j = 0; j < K; j++) {
-                double s = 0.0;
-                for (int p = 0; p < N; p++) s += W[p * N + ax * K + j] * in[p];
-                T[base + j] = s;
+                double acc = 0.0;
+#pragma unroll
+                for (int p = 0; p < N; p++) acc += Ws[p * N + ax * K + j] * in[p];
+                T[base + j] = acc;
             }
         }
         __syncthreads();
-        // y: lines (z, x<K)
+        // y: lines (z, x < K)
         for (int line = threadIdx.x; line < N * K; line += blockDim.x) {
             const int z = line / K, x = line % K;
-            const int base = z * N2 + x;
-            double in[2 * MAXK];
-            for (int p = 0; p < N; p++) in[p] = T[base + p * N];
+            const int base = z * N * P + x;
+            double in[N];
+#pragma unroll
+            for (int p = 0; p < N; p++) in[p] = T[base + p * P];
+#pragma unroll
             for (int j = 0; j < K; j++) {
-                double s = 0.0;
-                for (int p = 0; p < N; p++) s += W[p * N + ay * K + j] * in[p];
-                T[base + j * N] = s;
+                double acc = 0.0;
+#pragma unroll
+                for (int p = 0; p < N; p++) acc += Ws[p * N + ay * K + j] * in[p];
+                T[base + j * P] = acc;
             }
         }
         __syncthreads();
-        // z: lines (y<K, x<K)
+        // z: lines (y < K, x < K)
         for (int line = threadIdx.x; line < K * K; line += blockDim.x) {
             const int y = line / K, x = line % K;
-            const int base = y * N + x;
-            double in[2 * MAXK];
-            for (int p = 0; p < N; p++) in[p] = T[base + p * N2];
+            const int base = y * P + x;
+            double in[N];
+#pragma unroll
+            for (int p = 0; p < N; p++) in[p] = T[base + p * N * P];
+#pragma unroll
             for (int j = 0; j < K; j++) {
-                double s = 0.0;
-                for (int p = 0; p < N; p++) s += W[p * N + az * K + j] * in[p];
-                T[base + j * N2] = s;
+                double acc = 0.0;
+#pragma unroll
+                for (int p = 0; p < N; p++) acc += Ws[p * N + az * K + j] * in[p];
+                T[base + j * N * P] = acc;
             }
         }
         __syncthreads();
     }
+    // further steps: scaling-only descent, K x K filter per dimension
     for (int sh = steps - 2; sh >= 0; sh--) {
         const int a3[3] = {(int)((l[0] >> sh) & 1), (int)((l[1] >> sh) & 1), (int)((l[2] >> sh) & 1)};
         for (int dim = 0; dim < 3; dim++) {
-            const int stride = dim == 0 ? 1 : (dim == 1 ? N : N2);
             for (int line = threadIdx.x; line < K * K; line += blockDim.x) {
                 const int u = line % K, v = line / K;
-                int base;
-                if (dim == 0) base = v * N2 + u * N;
-                else if (dim == 1) base = v * N2 + u;
-                else base = v * N + u;
-                double in[MAXK];
+                int base, stride;
+                if (dim == 0) {
+                    base = (v * N + u) * P;
+                    stride = 1;
+                } else if (dim == 1) {
+                    base = v * N * P + u;
+                    stride = P;
+                } else {
+                    base = v * P + u;
+                    stride = N * P;
+                }
+                double in[K];
+#pragma unroll
                 for (int p = 0; p < K; p++) in[p] = T[base + p * stride];
+#pragma unroll
                 for (int j = 0; j < K; j++) {
-                    double s = 0.0;
-                    for (int p = 0; p < K; p++) s += W[p * N + a3[dim] * K + j] * in[p];
-                    T[base + j * stride] = s;
+                    double acc = 0.0;
+#pragma unroll
+                    for (int p = 0; p < K; p++) acc += Ws[p * N + a3[dim] * K + j] * in[p];
+                    T[base + j * stride] = acc;
                 }
             }
             __syncthreads();
@@ -613,17 +677,35 @@ __global__ void k_gen_compute(int first, int n, const JobTab *jobs, GenState gs,
     double s2 = 0.0;
     for (int e = threadIdx.x; e < K3; e += blockDim.x) {
         const int x = e % K, y = (e / K) % K, z = e / (K * K);
-        const double v = T[(z * N + y) * N + x];
+        const double v = T[(z * N + y) * P + x];
         outp[e] = v;
         s2 += v * v;
     }
     red[threadIdx.x] = s2;
     __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
         __syncthreads();
     }
     if (threadIdx.x == 0) gs.norm[g] = sqrt(red[0]);
+    __syncthreads();
+}
+
+// Generated nodes first .. min(count, cap) - 1, count read from the device counter
+// that k_gen_mark advanced (no host round trip); grid-stride over the range.
+template <int K>
+__global__ void __launch_bounds__(256) k_gen_compute(int first, int cap, const JobTab *jobs, GenState gs,
+                                                     const double *W) {
+    using L = PadT<K>;
+    constexpr int N = L::N;
+    extern __shared__ double smem[];
+    double *T = smem;
+    double *Ws = smem + L::SIZE;
+    const int end = min(*(volatile const int *)gs.count, cap);
+    if (first + (int)blockIdx.x >= end) return;
+    for (int e = threadIdx.x; e < N * N; e += blockDim.x) Ws[e] = W[e];
+    __syncthreads();
+    for (int g = first + blockIdx.x; g < end; g += gridDim.x) gen_one<K>(g, jobs, gs, T, Ws);
 }
 
 
@@ -721,7 +803,8 @@ __device__ inline void walk_candidates(const CandTerms &ct, const int *fcache, c
         if (f >= 0) {
             for (int q = 0; q < 8; q++) fn[q] = J.inNorm[(size_t)f * 8 + q];
         } else {
-            fn[0] = gs.norm[-f - 1];
+            // an entry past the allocated pool (its count is redone after the pool grows)
+            fn[0] = -f - 1 < gs.pcap ? gs.norm[-f - 1] : 0.0;
             nft = 1;
         }
         const double *nrm = J.operNorms + tb.nrm;
@@ -1142,16 +1225,15 @@ __global__ void __launch_bounds__(256) k_contract_mfma8(int ntask, const SliceTa
                                                         double *nodeslots) {
     constexpr int K = 8, K2 = 64, K3 = 512;
     constexpr int S1 = 68, S2 = 132;
-    __shared__ float lds1[4][16 * S1];
-    __shared__ float lds2[4][16 * S2];
+    // per-wave operand staging; reused for the cross-wave reduction at the end
+    __shared__ __align__(16) float lds[4 * 16 * S1 + 4 * 16 * S2];
     const int wave = wave_id(), lane = threadIdx.x & 63;
-    const int ti = blockIdx.x * 4 + wave;
-    if (ti >= ntask) return;
-    const SliceTask task = tasks[ti];
+    if ((int)blockIdx.x >= ntask) return;
+    const SliceTask task = tasks[blockIdx.x];
     const WorkItem wi = work[task.g];
     const JobTab &J = jobs[wi.job];
-    float *L1 = lds1[wave];
-    float *L2 = lds2[wave];
+    float *L1 = lds + wave * 16 * S1;
+    float *L2 = lds + 4 * 16 * S1 + wave * 16 * S2;
     const int m = lane & 15, q = lane >> 4;
     const int gxm = m >> 3, am = m & 7;
     double acc[16][4];
@@ -1159,7 +1241,8 @@ __global__ void __launch_bounds__(256) k_contract_mfma8(int ntask, const SliceTa
     for (int t = 0; t < 16; t++)
 #pragma unroll
         for (int r = 0; r < 4; r++) acc[t][r] = 0.0;
-    for (int it = 0; it < task.count; it++) {
+    // the four waves take every fourth slice of the task
+    for (int it = wave; it < task.count; it += 4) {
         const Slice sl = slices[task.start + it];
         const TermTab &tb = tabs[J.tab0 + sl.term];
         const gfloat *B32 = (const gfloat *)J.operBlocks32 + tb.blk;
@@ -1229,6 +1312,26 @@ __global__ void __launch_bounds__(256) k_contract_mfma8(int ntask, const SliceTa
         }
         wave_sync();
     }
+    // waves 1..3 add their accumulators into wave 0's, in that order (one 32 KB LDS buffer)
+    double *R = (double *)lds;
+    __syncthreads();
+    for (int src = 1; src < 4; src++) {
+        if (wave == src) {
+#pragma unroll
+            for (int t3 = 0; t3 < 16; t3++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) R[(t3 * 4 + r) * 64 + lane] = acc[t3][r];
+        }
+        __syncthreads();
+        if (wave == 0) {
+#pragma unroll
+            for (int t3 = 0; t3 < 16; t3++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) acc[t3][r] += R[(t3 * 4 + r) * 64 + lane];
+        }
+        __syncthreads();
+    }
+    if (wave != 0) return;
     double *dst = nodeslots + (size_t)task.slot * 8 * K3;
 #pragma unroll
     for (int t3 = 0; t3 < 16; t3++) {
@@ -1267,36 +1370,64 @@ __global__ void k_work_norms(int nw, const WorkItem *work, const JobTab *jobs, i
     }
 }
 
+// sum over the block of one value per thread, same order every call (all threads get it)
+__device__ inline double block_sum_256(double v, double *wsum) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
+}
+
 // batched two-scale transforms over (tree, parent) items:
-// mode 0 compress children -> parent, mode 1 add reconstruct(parent) to the children
-__global__ void k_transform(int mode, int n, const TransformItem *items, const double *M, int K) {
+// mode 0 compress children -> parent (and the parent's 8 component norms),
+// mode 1 add reconstruct(parent) to the children's scaling parts (and their norms)
+template <int K>
+__global__ void __launch_bounds__(256) k_transform(int mode, int n, const TransformItem *items, const double *M) {
+    using L = PadT<K>;
+    constexpr int N = L::N, K3 = K * K * K;
     extern __shared__ double smem[];
-    const int N = 2 * K, K3 = K * K * K, N3 = N * N * N;
+    __shared__ double wsum[4];
     double *T = smem;
-    double *Ms = smem + N3;
+    double *Ms = smem + L::SIZE;
     const TransformItem it = items[blockIdx.x];
     for (int e = threadIdx.x; e < N * N; e += blockDim.x) Ms[e] = M[e];
     if (mode == 0) {
         for (int e = threadIdx.x; e < 8 * K3; e += blockDim.x) {
             const int a = e / K3, r = e % K3;
-            T[interleaved_index(K, a, r)] = it.coef[(size_t)(it.first_child + a) * 8 * K3 + r];
+            T[padded_index<K>(a, r)] = it.coef[(size_t)(it.first_child + a) * 8 * K3 + r];
         }
     } else {
         for (int e = threadIdx.x; e < 8 * K3; e += blockDim.x)
-            T[interleaved_index(K, e / K3, e % K3)] = it.coef[(size_t)it.parent * 8 * K3 + e];
+            T[padded_index<K>(e / K3, e % K3)] = it.coef[(size_t)it.parent * 8 * K3 + e];
     }
     __syncthreads();
     for (int dim = 0; dim < 3; dim++) {
-        line_pass(T, Ms, N, dim);
+        line_pass_padded<N>(T, Ms, dim);
         __syncthreads();
     }
-    if (mode == 0) {
-        for (int e = threadIdx.x; e < 8 * K3; e += blockDim.x)
-            it.coef[(size_t)it.parent * 8 * K3 + e] = T[interleaved_index(K, e / K3, e % K3)];
-    } else {
-        for (int e = threadIdx.x; e < 8 * K3; e += blockDim.x) {
-            const int a = e / K3, r = e % K3;
-            it.coef[(size_t)(it.first_child + a) * 8 * K3 + r] += T[interleaved_index(K, a, r)];
+    for (int c = 0; c < 8; c++) {
+        double ss = 0.0;
+        if (mode == 0) {
+            double *dst = it.coef + (size_t)it.parent * 8 * K3 + c * K3;
+            for (int r = threadIdx.x; r < K3; r += blockDim.x) {
+                const double v = T[padded_index<K>(c, r)];
+                dst[r] = v;
+                ss += v * v;
+            }
+        } else {
+            double *dst = it.coef + (size_t)(it.first_child + c) * 8 * K3;
+            for (int r = threadIdx.x; r < K3; r += blockDim.x) {
+                const double v = dst[r] + T[padded_index<K>(c, r)];
+                dst[r] = v;
+                ss += v * v;
+            }
+        }
+        ss = block_sum_256(ss, wsum);
+        if (threadIdx.x == 0 && it.norm) {
+            if (mode == 0) it.norm[(size_t)it.parent * 8 + c] = sqrt(ss);
+            else it.norm[(size_t)(it.first_child + c) * 8] = sqrt(ss);
         }
     }
     (void)n;
@@ -1549,11 +1680,25 @@ hipError_t launch_gen_mark(hipStream_t s, int nw, const WorkItem *work, const Jo
     return hipGetLastError();
 }
 
-hipError_t launch_gen_compute(hipStream_t s, int first, int n, const JobTab *jobs, const GenState &gs,
+// padded node tensor + one (2K)^2 filter matrix
+static size_t padded_smem(int K) { return (size_t)(2 * K * 2 * K * (2 * K + 1) + 4 * K * K) * sizeof(double); }
+
+template <class Kern>
+static hipError_t allow_smem(Kern *k, size_t sm) {
+    if (sm <= 65536) return hipSuccess;
+    return hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+}
+
+hipError_t launch_gen_compute(hipStream_t s, int first, int cap, const JobTab *jobs, const GenState &gs,
                               const double *W, int K) {
-    if (n <= 0) return hipSuccess;
-    const size_t sm = (size_t)(8 * K * K * K) * sizeof(double);
-    hipLaunchKernelGGL(k_gen_compute, dim3(n), dim3(256), sm, s, first, n, jobs, gs, W, K);
+    if (cap <= first) return hipSuccess;
+    const size_t sm = padded_smem(K);
+    const int nb = std::min(cap - first, 2048);
+    MWG_DISPATCH_K(K, {
+        hipError_t e = allow_smem(k_gen_compute<KK>, sm);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_gen_compute<KK>, dim3(nb), dim3(256), sm, s, first, cap, jobs, gs, W);
+    });
     return hipGetLastError();
 }
 
@@ -1579,13 +1724,12 @@ hipError_t launch_work_norms(hipStream_t s, int nw, const WorkItem *work, const 
 
 hipError_t launch_transform(hipStream_t s, int mode, int n, const TransformItem *items, const double *M, int K) {
     if (n <= 0) return hipSuccess;
-    const size_t sm = tensor_smem(K);
-    if (sm > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_transform, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)sm);
+    const size_t sm = padded_smem(K);
+    MWG_DISPATCH_K(K, {
+        hipError_t e = allow_smem(k_transform<KK>, sm);
         if (e != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL(k_transform, dim3(n), dim3(256), sm, s, mode, n, items, M, K);
+        hipLaunchKernelGGL(k_transform<KK>, dim3(n), dim3(256), sm, s, mode, n, items, M);
+    });
     return hipGetLastError();
 }
 
@@ -1624,7 +1768,7 @@ hipError_t launch_contract_mfma(hipStream_t s, int K, int ntask, const SliceTask
                                 double *nodeslots) {
     if (ntask <= 0) return hipSuccess;
     if (K != 8) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_contract_mfma8, dim3((ntask + 3) / 4), dim3(256), 0, s, ntask, tasks, work, jobs, tabs,
+    hipLaunchKernelGGL(k_contract_mfma8, dim3(ntask), dim3(256), 0, s, ntask, tasks, work, jobs, tabs,
                        slices, genPool, nodeslots);
     return hipGetLastError();
 }
