@@ -109,6 +109,7 @@ struct Ctx {
     DBuf<TermTab> tabs;
     DBuf<BoxJob> bjobs;
     DBuf<double> tbmax;
+    DBuf<int> tpre, tpre0;
     DBuf<int> thread_off, gtotal;
     DBuf<long long> gofs;
     DBuf<Tuple> tuples;
@@ -748,6 +749,19 @@ static void compact_tree(TreeH &T, const std::vector<char> &alive) {
     T.hvalid = false;
 }
 
+// stable order by task size, largest first (counts are bounded by the chunk size)
+static void sort_by_count_desc(std::vector<ContractTask> &v) {
+    if (v.size() < 2) return;
+    int maxc = 0;
+    for (const ContractTask &t : v) maxc = std::max(maxc, t.count);
+    std::vector<int> start((size_t)maxc + 2, 0);
+    for (const ContractTask &t : v) start[(size_t)(maxc - t.count) + 1]++;
+    for (size_t i = 1; i < start.size(); i++) start[i] += start[i - 1];
+    std::vector<ContractTask> out(v.size());
+    for (const ContractTask &t : v) out[start[(size_t)(maxc - t.count)]++] = t;
+    v.swap(out);
+}
+
 struct PhaseTimer {
     bool on = false;
     std::vector<std::pair<std::string, double>> acc;
@@ -879,6 +893,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                     B.depth = d;
                     B.tab0 = ntabs;
                     B.jt = (int)jt.size();
+                    B.pre_off = ntabs + (int)jt.size();
                     bjobs.push_back(B);
                     ntabs += nt;
                     max_terms = std::max(max_terms, nt);
@@ -911,12 +926,20 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         }
         const int nw = (int)work.size();
         PT.lap("opers+boxes");
+        // per-table prefix arrays (ntab + 1 entries each), filled on the device
+        C.tpre.alloc((size_t)ntabs + jt.size() + 1);
+        C.tpre0.alloc((size_t)ntabs + jt.size() + 1);
+        for (size_t q = 0; q < jt.size(); q++) {
+            jt[q].pre = C.tpre.p + jt[q].tab0 + q;
+            jt[q].pre0 = C.tpre0.p + jt[q].tab0 + q;
+        }
         upload_async(C, C.work, work.data(), work.size());
         upload_async(C, C.jobs, jt.data(), jt.size());
         upload_async(C, C.bjobs, bjobs.data(), bjobs.size());
         C.tabs.alloc((size_t)std::max(ntabs, 1));
         C.tbmax.alloc((size_t)std::max<size_t>(bjobs.size() * max_terms, 1));
-        HC(launch_job_boxes(s, (int)bjobs.size(), max_terms, C.bjobs.p, C.tabs.p, C.tbmax.p, C.jobs.p));
+        HC(launch_job_boxes(s, (int)bjobs.size(), max_terms, C.bjobs.p, C.tabs.p, C.tbmax.p, C.jobs.p, C.tpre.p,
+                            C.tpre0.p));
         // generated input nodes: marked, then computed for the range the device counter
         // reached (read back with the work-list counts below, not before)
         HC(hipMemsetAsync(C.genOverflow.p, 0, sizeof(int), s));
@@ -1000,8 +1023,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             C.tuples.alloc((size_t)std::max<long long>(total, 1));
             HC(launch_enum(s, 1, nw, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, C.gofs.p,
                            C.tuples.p));
-            std::stable_sort(tasks.begin(), tasks.end(),
-                             [](const ContractTask &a, const ContractTask &b) { return a.count > b.count; });
+            sort_by_count_desc(tasks);
             upload_async(C, C.tasks, tasks.data(), tasks.size());
             if (!reds.empty()) upload_async(C, C.reds, reds.data(), reds.size());
             C.partials.alloc((size_t)std::max(nslots, 1) * K3);
@@ -1051,8 +1073,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             C.slices.alloc((size_t)std::max<long long>(stotal, 1));
             HC(launch_enum_mixed(s, 1, nw, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, C.gofs.p,
                                  C.tuples.p, C.slices.p));
-            std::stable_sort(tasks.begin(), tasks.end(),
-                             [](const ContractTask &a, const ContractTask &b) { return a.count > b.count; });
+            sort_by_count_desc(tasks);
             if (!tasks.empty()) upload_async(C, C.tasks, tasks.data(), tasks.size());
             if (!stasks.empty()) upload_async(C, C.stasks, stasks.data(), stasks.size());
             upload_async(C, C.reds, reds.data(), reds.size());

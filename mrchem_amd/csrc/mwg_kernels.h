@@ -43,9 +43,12 @@ struct JobTab {
     double *outNorm;
     double thr;
     double route;  // mixed mode: slices with a bound above this run in FP64
+    const int *pre;   // [ntab + 1] candidate prefix over terms (union boxes ub), device-built
+    const int *pre0;  // [ntab + 1] same over the ft = 0 union boxes ub0
     int tab0, ntab;
     int lmax[3];
     int depth;
+    int ubox[3];  // union of ub over the table's terms (device-built)
 };
 
 // one Gaussian term of one operator level, as the screening-box kernel reads it
@@ -69,6 +72,7 @@ struct BoxJob {
     int depth;
     int tab0;     // first TermTab of this table
     int jt;       // index of the JobTab that receives the routing bound
+    int pre_off;  // first entry of this table's prefix arrays (tab0 + jt)
 };
 
 struct GenState {
@@ -103,7 +107,7 @@ hipError_t launch_project_hyd(hipStream_t s, int nw, const int *work, const int 
 // TermTab[tab0 + t] for every (table, term): per-combination screening boxes from the
 // monotone norm envelopes, union boxes, and the table's FP64 routing bound
 hipError_t launch_job_boxes(hipStream_t s, int nbj, int max_terms, const BoxJob *bj, TermTab *tabs, double *tbmax,
-                            JobTab *jobs);
+                            JobTab *jobs, int *pre, int *pre0);
 hipError_t launch_gen_mark(hipStream_t s, int nw, const WorkItem *work, const JobTab *jobs, const TermTab *tabs,
                            const GenState &gs);
 // generated nodes first .. min(device count, cap) - 1 (cap = entries allocated in the pool)

@@ -498,15 +498,7 @@ __global__ void __launch_bounds__(256) k_gen_mark(int nw, const WorkItem *work, 
     const WorkItem wi = work[blockIdx.x];
     const JobTab &J = jobs[wi.job];
     __shared__ int start[257];
-    if (threadIdx.x == 0) {
-        int acc = 0;
-        for (int t = 0; t < J.ntab; t++) {
-            start[t] = acc;
-            const TermTab &tb = tabs[J.tab0 + t];
-            if (tb.ub0[0] >= 0) acc += (2 * tb.ub0[0] + 1) * (2 * tb.ub0[1] + 1) * (2 * tb.ub0[2] + 1);
-        }
-        start[J.ntab] = acc;
-    }
+    for (int t = threadIdx.x; t <= J.ntab; t += blockDim.x) start[t] = J.pre0[t];
     __syncthreads();
     const int total = start[J.ntab];
     for (int k = threadIdx.x; k < total; k += blockDim.x) {
@@ -728,13 +720,9 @@ __device__ inline int probe_f(const JobTab &J, const GenState &gs, int fx, int f
 
 __device__ inline FCache fill_fcache(int *cache, const WorkItem &wi, const JobTab &J, const TermTab *tabs,
                                      const GenState &gs) {
-    int ux = -1, uy = -1, uz = -1;
-    for (int t = J.tab0; t < J.tab0 + J.ntab; t++) {
-        ux = max(ux, (int)tabs[t].ub[0]);
-        uy = max(uy, (int)tabs[t].ub[1]);
-        uz = max(uz, (int)tabs[t].ub[2]);
-    }
+    const int ux = J.ubox[0], uy = J.ubox[1], uz = J.ubox[2];
     FCache fc{ux, uy, uz};
+    (void)tabs;
     if (ux < 0) return fc;
     const int nx = 2 * ux + 1, ny = 2 * uy + 1, nz = 2 * uz + 1;
     if (nx * ny * nz > FCACHE) {
@@ -766,16 +754,9 @@ struct CandTerms {
 };
 
 __device__ inline void term_prefix(CandTerms &ct, const JobTab &J, const TermTab *tabs) {
-    if (threadIdx.x == 0) {
-        int acc = 0;
-        for (int t = 0; t < J.ntab; t++) {
-            ct.start[t] = acc;
-            const TermTab &tb = tabs[J.tab0 + t];
-            if (tb.ub[0] >= 0) acc += (2 * tb.ub[0] + 1) * (2 * tb.ub[1] + 1) * (2 * tb.ub[2] + 1);
-        }
-        ct.start[J.ntab] = acc;
-    }
+    for (int t = threadIdx.x; t <= J.ntab; t += blockDim.x) ct.start[t] = J.pre[t];
     __syncthreads();
+    (void)tabs;
 }
 
 template <class Visit>
@@ -901,18 +882,65 @@ __global__ void __launch_bounds__(64) k_job_boxes(int nbj, const BoxJob *bjs, Te
     }
 }
 
-// FP64 routing bound of each table (mixed mode): slices whose bound exceeds it run in
-// FP64.  Above the roots it is a fixed multiple of the screening threshold; on the
-// roots (no screening) a fraction of the largest bound of the table.
-__global__ void __launch_bounds__(64) k_job_route(int nbj, const BoxJob *bjs, const double *tbmax, int max_terms,
-                                                  JobTab *jobs) {
+// Per table, one wave: the FP64 routing bound (mixed mode: slices whose bound exceeds
+// it run in FP64; above the roots a fixed multiple of the screening threshold, on the
+// roots (no screening) a fraction of the largest bound of the table), the union box of
+// the terms, and the prefix sums over terms of the candidate counts of the union boxes
+// ub and ub0 (the work-list and generated-node kernels split their walks by these).
+__device__ inline int wave_incl_scan(int v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
+    }
+    return v;
+}
+
+__global__ void __launch_bounds__(64) k_job_route(int nbj, const BoxJob *bjs, const TermTab *tabs,
+                                                  const double *tbmax, int max_terms, JobTab *jobs, int *pre,
+                                                  int *pre0) {
     const int jb = blockIdx.x;
     if (jb >= nbj) return;
     const BoxJob &B = bjs[jb];
+    const int lane = threadIdx.x;
     double m = 0.0;
-    for (int t = threadIdx.x; t < B.nterms; t += 64) m = fmax(m, tbmax[(size_t)jb * max_terms + t]);
+    int ux = -1, uy = -1, uz = -1, carry = 0, carry0 = 0;
+    for (int base = 0; base < B.nterms; base += 64) {
+        const int t = base + lane;
+        int v = 0, v0 = 0;
+        if (t < B.nterms) {
+            m = fmax(m, tbmax[(size_t)jb * max_terms + t]);
+            const TermTab &tb = tabs[B.tab0 + t];
+            if (tb.ub[0] >= 0) {
+                v = (2 * tb.ub[0] + 1) * (2 * tb.ub[1] + 1) * (2 * tb.ub[2] + 1);
+                ux = max(ux, (int)tb.ub[0]);
+                uy = max(uy, (int)tb.ub[1]);
+                uz = max(uz, (int)tb.ub[2]);
+            }
+            if (tb.ub0[0] >= 0) v0 = (2 * tb.ub0[0] + 1) * (2 * tb.ub0[1] + 1) * (2 * tb.ub0[2] + 1);
+        }
+        const int incl = wave_incl_scan(v), incl0 = wave_incl_scan(v0);
+        if (t < B.nterms) {
+            pre[B.pre_off + t] = carry + incl - v;
+            pre0[B.pre_off + t] = carry0 + incl0 - v0;
+        }
+        carry += __shfl(incl, 63, 64);
+        carry0 += __shfl(incl0, 63, 64);
+    }
     m = wave_max_double(m);
-    if (threadIdx.x == 0) jobs[B.jt].route = B.thr > 0.0 ? ROUTE_FP64 * B.thr : ROUTE_ROOT * m;
+    ux = wave_max_int(ux);
+    uy = wave_max_int(uy);
+    uz = wave_max_int(uz);
+    if (lane == 0) {
+        pre[B.pre_off + B.nterms] = carry;
+        pre0[B.pre_off + B.nterms] = carry0;
+        JobTab &J = jobs[B.jt];
+        J.route = B.thr > 0.0 ? ROUTE_FP64 * B.thr : ROUTE_ROOT * m;
+        J.ubox[0] = ux;
+        J.ubox[1] = uy;
+        J.ubox[2] = uz;
+    }
 }
 
 // ----------------------------------------------------------------- apply: work list
@@ -1666,10 +1694,10 @@ hipError_t launch_project_hyd(hipStream_t s, int nw, const int *work, const int 
 }
 
 hipError_t launch_job_boxes(hipStream_t s, int nbj, int max_terms, const BoxJob *bj, TermTab *tabs, double *tbmax,
-                            JobTab *jobs) {
+                            JobTab *jobs, int *pre, int *pre0) {
     if (nbj <= 0 || max_terms <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_job_boxes, dim3(nbj, max_terms), dim3(64), 0, s, nbj, bj, tabs, tbmax, max_terms);
-    hipLaunchKernelGGL(k_job_route, dim3(nbj), dim3(64), 0, s, nbj, bj, tbmax, max_terms, jobs);
+    hipLaunchKernelGGL(k_job_route, dim3(nbj), dim3(64), 0, s, nbj, bj, tabs, tbmax, max_terms, jobs, pre, pre0);
     return hipGetLastError();
 }
 

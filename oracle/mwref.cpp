@@ -39,7 +39,9 @@
 //   1/r and exp(-mu r)/r; operator blocks: pinned by direct quadrature against
 //   the two-scale NS construction and the constant-kernel closed form; apply:
 //   pinned by the erf / erfc closed forms for Gaussian charges, the p-type
-//   derivative closed forms and the hydrogenic Slater-Condon integrals.
+//   derivative closed forms and the hydrogenic Slater-Condon integrals;
+//   multiply: pinned by the Gaussian product theorem (pointwise, analytic norm
+//   and overlap of the product); dot: pinned by analytic Gaussian overlaps.
 
 #include <algorithm>
 #include <array>

@@ -277,3 +277,33 @@ def test_closed_forms_self_consistent(A):
     assert ex["1s"] == pytest.approx(3.126543, abs=1e-6)
     assert ex["2s"] == pytest.approx(1.693191, abs=1e-6)
     assert ex["2p"] == pytest.approx(1.899659, abs=1e-6)
+
+
+def test_multiply_matches_gaussian_product_theorem(M):
+    """oracle multiply(a, b) against the analytic product a*b (Gaussian product theorem):
+    pointwise, in L2 (analytic norm of the product) and through its integral with a
+    third function (analytic overlap)."""
+    orbs, nuc, mus = configs.orbitals(7, n_orb=3)
+    mra = M.MRA(**configs.BOX20, k=7, max_depth=16)
+    prec = 1e-6
+    a, b, c = orbs
+    ta, tb, tc = (M.Tree(mra).project_gauss(prec, f) for f in (a, b, c))
+    ab = G.product(a, b)
+    tab = M.multiply(prec, M.Tree(mra), 1.0, ta, tb)
+    pts = np.random.default_rng(4).uniform(-4, 4, (2000, 3))
+    ref = ab(pts)
+    assert np.abs(tab.eval(pts) - ref).max() < 1e-4 * np.abs(ref).max()
+    assert tab.sqnorm == pytest.approx(G.overlap(ab, ab), rel=1e-5)
+    assert tab.dot(tc) == pytest.approx(G.overlap(ab, c), abs=1e-5 * math.sqrt(G.overlap(ab, ab)))
+    # the coefficient scales the product linearly
+    t2 = M.multiply(prec, M.Tree(mra), -2.5, ta, tb)
+    assert t2.dot(tc) == pytest.approx(-2.5 * tab.dot(tc), rel=1e-12)
+
+
+def test_dot_matches_analytic_overlap(M):
+    orbs, nuc, mus = configs.orbitals(8, n_orb=3)
+    mra = M.MRA(**configs.BOX20, k=7, max_depth=16)
+    t = [M.Tree(mra).project_gauss(1e-7, f) for f in orbs]
+    for i in range(3):
+        for j in range(3):
+            assert t[i].dot(t[j]) == pytest.approx(G.overlap(orbs[i], orbs[j]), abs=1e-6)
