@@ -110,6 +110,8 @@ struct Ctx {
     DBuf<BoxJob> bjobs;
     DBuf<double> tbmax;
     DBuf<int> tpre, tpre0;
+    DBuf<TaskSummary> tsum;
+    TaskCaps caps{1 << 20, 1 << 18, 1 << 15, 1 << 13, 1 << 15, 1 << 15, 1 << 13};
     DBuf<int> thread_off, gtotal;
     DBuf<long long> gofs;
     DBuf<Tuple> tuples;
@@ -430,6 +432,8 @@ void OperH::build_level(int d) {
     // phase 2: full bands of the active terms
     if (!full.empty()) {
         compute_ns(*this, d, full, L->blocks, L->norms, rows);
+        // slice records address the level's FP32 block array with 32-bit offsets
+        if (L->blocks.n >= ((size_t)1 << 31)) throw Error(MWG_ERR_RANGE, "operator level too large");
         std::vector<double> hn(L->norms.n);
         L->norms.download(hn.data(), hn.size(), C.stream);
         for (size_t r = 0; r < full.size(); r++) {
@@ -749,19 +753,6 @@ static void compact_tree(TreeH &T, const std::vector<char> &alive) {
     T.hvalid = false;
 }
 
-// stable order by task size, largest first (counts are bounded by the chunk size)
-static void sort_by_count_desc(std::vector<ContractTask> &v) {
-    if (v.size() < 2) return;
-    int maxc = 0;
-    for (const ContractTask &t : v) maxc = std::max(maxc, t.count);
-    std::vector<int> start((size_t)maxc + 2, 0);
-    for (const ContractTask &t : v) start[(size_t)(maxc - t.count) + 1]++;
-    for (size_t i = 1; i < start.size(); i++) start[i] += start[i - 1];
-    std::vector<ContractTask> out(v.size());
-    for (const ContractTask &t : v) out[start[(size_t)(maxc - t.count)]++] = t;
-    v.swap(out);
-}
-
 struct PhaseTimer {
     bool on = false;
     std::vector<std::pair<std::string, double>> acc;
@@ -956,32 +947,91 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         HC(launch_gen_mark(s, nw, C.work.p, C.jobs.p, C.tabs.p, gs));
         HC(launch_gen_compute(s, genDone, genPoolCap, C.jobs.p, gs, M.W.p, K));
         PT.lap("upload+genmark");
-        // work-list counts per segment (nseg segments per output node), with the generated
-        // node count; when the pool was too small, grow it, finish the generated nodes and
-        // count again
-        auto count_segments = [&](int nseg, std::vector<int> &tot) {
-            C.thread_off.alloc((size_t)nw * nseg * 256);
-            C.gtotal.alloc((size_t)nw * nseg);
-            tot.assign((size_t)nw * nseg, 0);
-            for (int attempt = 0;; attempt++) {
-                if (nseg == 8)
-                    HC(launch_enum(s, 0, nw, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, nullptr,
-                                   nullptr));
-                else
-                    HC(launch_enum_mixed(s, 0, nw, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p,
-                                         nullptr, nullptr, nullptr));
-                int hc[2];
-                HC(hipMemcpyAsync(&hc[0], C.genCount.p, sizeof(int), hipMemcpyDeviceToHost, s));
-                HC(hipMemcpyAsync(&hc[1], C.genOverflow.p, sizeof(int), hipMemcpyDeviceToHost, s));
-                download_sync(C, C.gtotal, tot.data(), tot.size());
-                if (hc[1]) throw Error(MWG_ERR_RANGE, "generated-node table overflow");
-                const int genNow = hc[0];
-                if (genNow <= genPoolCap) {
-                    genLastNew = genNow - genDone;
-                    genDone = genNow;
-                    return;
-                }
-                if (attempt > 0) throw Error(MWG_ERR_RANGE, "generated-node pool sizing failed");
+        // Work list and contraction without a host round trip: counts per segment
+        // (nseg per output node), a device-built schedule (offsets, tasks, reduce items),
+        // the tuple/slice fill, the contractions and the output norms; the host reads
+        // the norms, the schedule sizes and the counts back once.  When a capacity (or
+        // the generated-node pool) was too small, nothing of the schedule ran: grow and
+        // run it again.
+        const int nseg = mixed ? 9 : 8;
+        C.thread_off.alloc((size_t)nw * nseg * 256);
+        C.gtotal.alloc((size_t)nw * nseg);
+        C.gofs.alloc((size_t)nw * nseg);
+        C.wnorms.alloc((size_t)nw * 8);
+        C.tsum.alloc(1);
+        auto launch_count = [&]() {
+            if (!mixed)
+                HC(launch_enum(s, 0, nw, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, nullptr,
+                               nullptr, nullptr));
+            else
+                HC(launch_enum_mixed(s, 0, nw, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p,
+                                     nullptr, nullptr, nullptr, nullptr));
+        };
+        auto ensure_caps = [&](const TaskSummary *need) {
+            auto grow = [](long long have, long long want) {
+                return want <= have ? have : std::max<long long>(want + want / 4, 1024);
+            };
+            if (need) {
+                C.caps.tuples = grow(C.caps.tuples, need->ntuples);
+                C.caps.slices = grow(C.caps.slices, need->nslices);
+                C.caps.tasks = (int)grow(C.caps.tasks, need->ntasks);
+                C.caps.stasks = (int)grow(C.caps.stasks, need->nstasks);
+                C.caps.reds = (int)grow(C.caps.reds, need->nreds);
+                C.caps.slots = (int)grow(C.caps.slots, need->nslots);
+                C.caps.nnslots = (int)grow(C.caps.nnslots, need->nnslots);
+            }
+            C.tuples.alloc((size_t)std::max<long long>(C.caps.tuples, 1));
+            C.tasks.alloc((size_t)std::max(C.caps.tasks, 1));
+            C.reds.alloc((size_t)std::max(C.caps.reds, 1));
+            C.partials.alloc((size_t)std::max(C.caps.slots, 1) * K3);
+            if (mixed) {
+                C.slices.alloc((size_t)std::max<long long>(C.caps.slices, 1));
+                C.stasks.alloc((size_t)std::max(C.caps.stasks, 1));
+                C.nodeslots.alloc((size_t)std::max(C.caps.nnslots, 1) * 8 * K3);
+            }
+        };
+        std::vector<double> wn((size_t)nw * 8);
+        std::vector<int> tot((size_t)nw * nseg);
+        TaskSummary hsum;
+        int hgen[2];
+        auto launch_schedule = [&]() {
+            ensure_caps(nullptr);
+            HC(launch_build_tasks(s, nw, nseg, C.chunk, 4 * C.slice_chunk, C.gtotal.p, C.caps, C.genCount.p,
+                                  genPoolCap, C.gofs.p, C.tasks.p, C.stasks.p, C.reds.p, C.tsum.p));
+            if (!mixed)
+                HC(launch_enum(s, 1, nw, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, C.gofs.p,
+                               C.tuples.p, C.tsum.p));
+            else
+                HC(launch_enum_mixed(s, 1, nw, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p,
+                                     C.gofs.p, C.tuples.p, C.slices.p, C.tsum.p));
+            const int grid64 = std::max(1, (C.caps.tasks + 3) / 4);
+            const int gridred = std::min(4096, std::max(1, C.caps.reds));
+            HC(hipEventRecord(C.ev0, s));
+            HC(launch_contract_fp64(s, K, 0, C.tsum.p, grid64, C.tasks.p, C.work.p, C.jobs.p, C.tabs.p, C.tuples.p,
+                                    C.genPool.p, C.partials.p));
+            if (mixed) {
+                const int gridmf = std::max(1, C.caps.stasks);
+                HC(launch_contract_mfma(s, K, 0, C.tsum.p, gridmf, C.stasks.p, C.work.p, C.jobs.p, C.tabs.p,
+                                        C.slices.p, C.genPool.p, C.nodeslots.p));
+            }
+            HC(launch_reduce_partials(s, 0, C.tsum.p, gridred, C.reds.p, C.work.p, C.jobs.p, C.partials.p,
+                                      mixed ? C.nodeslots.p : nullptr, K3));
+            HC(hipEventRecord(C.ev1, s));
+            HC(launch_work_norms(s, nw, C.work.p, C.jobs.p, K3, C.wnorms.p));
+            HC(hipMemcpyAsync(&hsum, C.tsum.p, sizeof(TaskSummary), hipMemcpyDeviceToHost, s));
+            HC(hipMemcpyAsync(&hgen[0], C.genCount.p, sizeof(int), hipMemcpyDeviceToHost, s));
+            HC(hipMemcpyAsync(&hgen[1], C.genOverflow.p, sizeof(int), hipMemcpyDeviceToHost, s));
+            HC(hipMemcpyAsync(tot.data(), C.gtotal.p, tot.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+            download_sync(C, C.wnorms, wn.data(), wn.size());
+            if (hgen[1]) throw Error(MWG_ERR_RANGE, "generated-node table overflow");
+        };
+        launch_count();
+        PT.lap("enum_count");
+        launch_schedule();
+        if (hsum.overflow) {
+            if (hsum.overflow & 2) {
+                // finish the generated nodes past the pool, then count again
+                const int genNow = hgen[0];
                 const int want = std::min(genCap, genNow + std::max(4096, genNow / 2));
                 C.genPool.grow_keep((size_t)want * K3, s);
                 C.genNorm.grow_keep((size_t)want, s);
@@ -990,110 +1040,20 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                 gs.pcap = want;
                 HC(launch_gen_compute(s, genPoolCap, want, C.jobs.p, gs, M.W.p, K));
                 genPoolCap = want;
+                launch_count();
             }
-        };
-        if (!mixed) {
-            // work list: tuples grouped by segment (g, gt)
-            std::vector<int> tot;
-            count_segments(8, tot);
-            PT.lap("enum_count");
-            std::vector<long long> ofs((size_t)nw * 8);
-            std::vector<ContractTask> tasks;
-            std::vector<ReduceItem> reds;
-            long long total = 0;
-            int nslots = 0;
-            for (int i = 0; i < nw; i++)
-                for (int q = 0; q < 8; q++) {
-                    const int c = tot[(size_t)i * 8 + q];
-                    ofs[(size_t)i * 8 + q] = total;
-                    jobs[jt_job[work[i].job]].tuples += c;
-                    const int nch = c <= C.chunk ? 1 : (c + C.chunk - 1) / C.chunk;
-                    if (nch == 1) {
-                        tasks.push_back({total, c, i, q, -1});
-                    } else {
-                        reds.push_back({i, q, nslots, nch, 0, 0});
-                        for (int h = 0; h < nch; h++) {
-                            const int cc = std::min(C.chunk, c - h * C.chunk);
-                            tasks.push_back({total + (long long)h * C.chunk, cc, i, q, nslots++});
-                        }
-                    }
-                    total += c;
-                }
-            upload_async(C, C.gofs, ofs.data(), ofs.size());
-            C.tuples.alloc((size_t)std::max<long long>(total, 1));
-            HC(launch_enum(s, 1, nw, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, C.gofs.p,
-                           C.tuples.p));
-            sort_by_count_desc(tasks);
-            upload_async(C, C.tasks, tasks.data(), tasks.size());
-            if (!reds.empty()) upload_async(C, C.reds, reds.data(), reds.size());
-            C.partials.alloc((size_t)std::max(nslots, 1) * K3);
-            HC(hipEventRecord(C.ev0, s));
-            HC(launch_contract_fp64(s, K, (int)tasks.size(), C.tasks.p, C.work.p, C.jobs.p, C.tabs.p, C.tuples.p,
-                                    C.genPool.p, C.partials.p));
-            HC(launch_reduce_partials(s, (int)reds.size(), C.reds.p, C.work.p, C.jobs.p, C.partials.p, nullptr, K3));
-            HC(hipEventRecord(C.ev1, s));
-        } else {
-            // mixed: FP64 tuples for the large bounds, FP32 MFMA slices for the rest
-            std::vector<int> tot;
-            count_segments(9, tot);
-            PT.lap("enum_count");
-            std::vector<long long> ofs((size_t)nw * 9);
-            std::vector<ContractTask> tasks;
-            std::vector<SliceTask> stasks;
-            std::vector<ReduceItem> reds;
-            long long total = 0, stotal = 0;
-            int nslots = 0, nnslots = 0;
-            for (int i = 0; i < nw; i++) {
-                const int cs = tot[(size_t)i * 9 + 8];
-                ofs[(size_t)i * 9 + 8] = stotal;
-                jobs[jt_job[work[i].job]].slices += cs;
-                const int nn0 = nnslots;
-                // one workgroup task = up to 4 waves x slice_chunk slices of this node
-                const int wchunk = 4 * C.slice_chunk;
-                for (int h = 0; h * wchunk < cs; h++) {
-                    const int cc = std::min(wchunk, cs - h * wchunk);
-                    stasks.push_back({stotal + (long long)h * wchunk, cc, i, nnslots++});
-                }
-                stotal += cs;
-                for (int q = 0; q < 8; q++) {
-                    const int c = tot[(size_t)i * 9 + q];
-                    ofs[(size_t)i * 9 + q] = total;
-                    jobs[jt_job[work[i].job]].tuples += c;
-                    const int s0 = nslots;
-                    for (int h = 0; h * C.chunk < c; h++) {
-                        const int cc = std::min(C.chunk, c - h * C.chunk);
-                        tasks.push_back({total + (long long)h * C.chunk, cc, i, q, nslots++});
-                    }
-                    reds.push_back({i, q, s0, nslots - s0, nn0, nnslots - nn0});
-                    total += c;
-                }
-            }
-            upload_async(C, C.gofs, ofs.data(), ofs.size());
-            C.tuples.alloc((size_t)std::max<long long>(total, 1));
-            C.slices.alloc((size_t)std::max<long long>(stotal, 1));
-            HC(launch_enum_mixed(s, 1, nw, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, C.gofs.p,
-                                 C.tuples.p, C.slices.p));
-            sort_by_count_desc(tasks);
-            if (!tasks.empty()) upload_async(C, C.tasks, tasks.data(), tasks.size());
-            if (!stasks.empty()) upload_async(C, C.stasks, stasks.data(), stasks.size());
-            upload_async(C, C.reds, reds.data(), reds.size());
-            C.partials.alloc((size_t)std::max(nslots, 1) * K3);
-            C.nodeslots.alloc((size_t)std::max(nnslots, 1) * 8 * K3);
-            HC(hipEventRecord(C.ev0, s));
-            HC(launch_contract_fp64(s, K, (int)tasks.size(), C.tasks.p, C.work.p, C.jobs.p, C.tabs.p, C.tuples.p,
-                                    C.genPool.p, C.partials.p));
-            HC(launch_contract_mfma(s, K, (int)stasks.size(), C.stasks.p, C.work.p, C.jobs.p, C.tabs.p, C.slices.p,
-                                    C.genPool.p, C.nodeslots.p));
-            HC(launch_reduce_partials(s, (int)reds.size(), C.reds.p, C.work.p, C.jobs.p, C.partials.p, C.nodeslots.p,
-                                      K3));
-            HC(hipEventRecord(C.ev1, s));
+            ensure_caps(&hsum);
+            launch_schedule();
+            if (hsum.overflow) throw Error(MWG_ERR_RANGE, "work-list schedule sizing failed");
+        }
+        genLastNew = hgen[0] - genDone;
+        genDone = hgen[0];
+        for (int i = 0; i < nw; i++) {
+            JobH &J = jobs[jt_job[work[i].job]];
+            for (int q = 0; q < 8; q++) J.tuples += tot[(size_t)i * nseg + q];
+            if (mixed) J.slices += tot[(size_t)i * 9 + 8];
         }
         PT.lap("worklist+launch");
-        // norms and refinement
-        C.wnorms.alloc((size_t)nw * 8);
-        HC(launch_work_norms(s, nw, C.work.p, C.jobs.p, K3, C.wnorms.p));
-        std::vector<double> wn((size_t)nw * 8);
-        download_sync(C, C.wnorms, wn.data(), wn.size());
         float ms = 0.0f;
         HC(hipEventElapsedTime(&ms, C.ev0, C.ev1));
         t_contract_ms += ms;

@@ -61,17 +61,19 @@ static_assert(sizeof(Tuple) == 16, "tuple record is 16 bytes");
 
 // one (g, f, ft, term) slice of the mixed-precision apply: all 8 output
 // components at once (mask: which gt are accumulated)
+// one FP32 MFMA slice (g, f, ft, term): every output component gt of node g from
+// component ft of input node f, with the operator blocks already resolved: xo, yo, zo
+// are the FP32 level-array offsets of the (gt_d = 0, ft_d) block of each dimension
+// (the gt_d = 1 block follows 2 (k+1)^2 floats later)
 struct Slice {
-    int g;
-    int f;
-    uint8_t term;
+    double w;      // term weight
+    int f;         // input node index (>= 0) or -(generated index + 1)
+    int xo, yo, zo;
     uint8_t ft;
-    uint8_t mask;
-    uint8_t pad;
-    int16_t dx, dy, dz;
-    int16_t pad2;
+    uint8_t mask;  // output components to accumulate (bit gt)
+    uint8_t pad[6];
 };
-static_assert(sizeof(Slice) == 20, "slice record is 20 bytes");
+static_assert(sizeof(Slice) == 32, "slice record is 32 bytes");
 
 // a slice is contracted in FP64 (as tuples) when its largest bound exceeds
 // ROUTE_FP64 times the screening threshold: the FP32 rounding of such a slice

@@ -51,6 +51,20 @@ struct JobTab {
     int ubox[3];  // union of ub over the table's terms (device-built)
 };
 
+// Device-built contraction schedule of one refinement iteration (sizes the host reads
+// back with the output norms; overflow set when a buffer capacity was too small or the
+// generated-node pool did not hold every generated node, and then no kernel of the
+// schedule runs).
+struct TaskSummary {
+    long long ntuples, nslices;
+    int ntasks, nstasks, nreds, nslots, nnslots;
+    int overflow;  // bit 0: a capacity below, bit 1: generated-node pool
+};
+struct TaskCaps {
+    long long tuples, slices;
+    int tasks, stasks, reds, slots, nnslots;
+};
+
 // one Gaussian term of one operator level, as the screening-box kernel reads it
 struct LevTerm {
     double w;        // weight (unit coordinates)
@@ -115,7 +129,7 @@ hipError_t launch_gen_compute(hipStream_t s, int first, int cap, const JobTab *j
                               const double *W, int K);
 hipError_t launch_enum(hipStream_t s, int mode, int nw, const WorkItem *work, const JobTab *jobs,
                        const TermTab *tabs, const GenState &gs, int *thread_off, int *seg_total,
-                       const long long *seg_ofs, Tuple *out);
+                       const long long *seg_ofs, Tuple *out, const TaskSummary *sum);
 hipError_t launch_work_norms(hipStream_t s, int nw, const WorkItem *work, const JobTab *jobs, int K3,
                              double *compact);
 struct TransformItem {
@@ -145,10 +159,10 @@ struct SliceTask {
 };
 hipError_t launch_enum_mixed(hipStream_t s, int mode, int nw, const WorkItem *work, const JobTab *jobs,
                              const TermTab *tabs, const GenState &gs, int *thread_off, int *seg_total,
-                             const long long *seg_ofs, Tuple *out, Slice *sout);
-hipError_t launch_contract_mfma(hipStream_t s, int K, int ntask, const SliceTask *tasks, const WorkItem *work,
-                                const JobTab *jobs, const TermTab *tabs, const Slice *slices, const double *genPool,
-                                double *nodeslots);
+                             const long long *seg_ofs, Tuple *out, Slice *sout, const TaskSummary *sum);
+hipError_t launch_contract_mfma(hipStream_t s, int K, int ntask, const TaskSummary *sum, int grid,
+                                const SliceTask *tasks, const WorkItem *work, const JobTab *jobs,
+                                const TermTab *tabs, const Slice *slices, const double *genPool, double *nodeslots);
 struct MulItem {
     int node;          // output node
     int depth;
@@ -161,10 +175,17 @@ hipError_t launch_multiply(hipStream_t s, int n, const MulItem *items, const Pro
 hipError_t launch_add(hipStream_t s, int n, const MulItem *items, const DevBasis &B, double ca, const double *acoef,
                       double cb, const double *bcoef, double *out);
 hipError_t launch_to_f32(hipStream_t s, size_t n, const double *in, float *out);
-hipError_t launch_contract_fp64(hipStream_t s, int K, int ntask, const ContractTask *tasks, const WorkItem *work,
-                                const JobTab *jobs, const TermTab *tabs, const Tuple *tuples, const double *genPool,
-                                double *partials);
-hipError_t launch_reduce_partials(hipStream_t s, int n, const ReduceItem *items, const WorkItem *work,
-                                  const JobTab *jobs, const double *partials, const double *nodeslots, int K3);
+// from the per-segment counts (nseg = 8: plain FP64, 9: mixed): segment offsets, FP64
+// tasks (full chunks first), MFMA slice tasks, reduce items, slot numbering
+hipError_t launch_build_tasks(hipStream_t s, int nw, int nseg, int chunk, int wchunk, const int *tot, TaskCaps caps,
+                              const int *gen_count, int gen_pcap, long long *ofs, ContractTask *tasks,
+                              SliceTask *stasks, ReduceItem *reds, TaskSummary *sum);
+// with sum: the task count is read on the device; grid (blocks) covers the capacity
+hipError_t launch_contract_fp64(hipStream_t s, int K, int ntask, const TaskSummary *sum, int grid,
+                                const ContractTask *tasks, const WorkItem *work, const JobTab *jobs,
+                                const TermTab *tabs, const Tuple *tuples, const double *genPool, double *partials);
+hipError_t launch_reduce_partials(hipStream_t s, int n, const TaskSummary *sum, int grid, const ReduceItem *items,
+                                  const WorkItem *work, const JobTab *jobs, const double *partials,
+                                  const double *nodeslots, int K3);
 
 }  // namespace mwg

@@ -954,7 +954,8 @@ __global__ void __launch_bounds__(64) k_job_route(int nbj, const BoxJob *bjs, co
 template <int MODE>
 __global__ void __launch_bounds__(256) k_enum(int nw, const WorkItem *work, const JobTab *jobs,
                                               const TermTab *tabs, GenState gs, int *thread_off, int *seg_total,
-                                              const long long *seg_ofs, Tuple *out) {
+                                              const long long *seg_ofs, Tuple *out, const TaskSummary *sum) {
+    if (MODE == 1 && sum->overflow) return;
     const int g = blockIdx.x;
     const WorkItem wi = work[g];
     const JobTab &J = jobs[wi.job];
@@ -1016,6 +1017,150 @@ __global__ void __launch_bounds__(256) k_enum(int nw, const WorkItem *work, cons
     (void)nw;
 }
 
+// ----------------------------------------------------------------- apply: contraction schedule
+
+__device__ inline long long wave_incl_scan_ll(long long v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const long long u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
+    }
+    return v;
+}
+
+// exclusive prefix over the 1024 threads of the block; *total gets the sum
+__device__ inline long long block_excl_scan_1024(long long v, long long *wsum, long long *total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const long long incl = wave_incl_scan_ll(v);
+    __syncthreads();
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long acc = 0;
+        for (int w = 0; w < 16; w++) {
+            const long long x = wsum[w];
+            wsum[w] = acc;
+            acc += x;
+        }
+        wsum[16] = acc;
+    }
+    __syncthreads();
+    *total = wsum[16];
+    return wsum[wave] + incl - v;
+}
+
+__global__ void __launch_bounds__(1024) k_build_tasks(int nw, int nseg, int chunk, int wchunk, const int *tot,
+                                                      TaskCaps caps, const int *gen_count, int gen_pcap,
+                                                      long long *ofs, ContractTask *tasks, SliceTask *stasks,
+                                                      ReduceItem *reds, TaskSummary *sum) {
+    __shared__ long long wsum[17];
+    const int g0 = (int)((long long)nw * threadIdx.x / 1024), g1 = (int)((long long)nw * (threadIdx.x + 1) / 1024);
+    const bool mixed = nseg == 9;
+    // pass 1: this thread's totals
+    long long tup = 0, sl = 0, full = 0, part = 0, nst = 0, slots = 0, nred = 0;
+    for (int g = g0; g < g1; g++) {
+        if (mixed) {
+            const int cs = tot[(size_t)g * 9 + 8];
+            sl += cs;
+            nst += (cs + wchunk - 1) / wchunk;
+            for (int q = 0; q < 8; q++) {
+                const int c = tot[(size_t)g * 9 + q];
+                tup += c;
+                full += c / chunk;
+                part += (c % chunk) != 0;
+                slots += (c + chunk - 1) / chunk;
+            }
+        } else {
+            for (int q = 0; q < 8; q++) {
+                const int c = tot[(size_t)g * 8 + q];
+                tup += c;
+                if (c <= chunk) {
+                    if (c == chunk) full++;
+                    else part++;
+                } else {
+                    const int nch = (c + chunk - 1) / chunk;
+                    full += c / chunk;
+                    part += (c % chunk) != 0;
+                    slots += nch;
+                    nred++;
+                }
+            }
+        }
+    }
+    long long Ttup, Tsl, Tfull, Tpart, Tnst, Tslots, Tnred;
+    long long btup = block_excl_scan_1024(tup, wsum, &Ttup);
+    long long bsl = block_excl_scan_1024(sl, wsum, &Tsl);
+    long long bfull = block_excl_scan_1024(full, wsum, &Tfull);
+    long long bpart = block_excl_scan_1024(part, wsum, &Tpart);
+    long long bnst = block_excl_scan_1024(nst, wsum, &Tnst);
+    long long bslots = block_excl_scan_1024(slots, wsum, &Tslots);
+    long long bnred = block_excl_scan_1024(nred, wsum, &Tnred);
+    if (mixed) Tnred = 8LL * nw;
+    int overflow = 0;
+    if (Ttup > caps.tuples || Tsl > caps.slices || Tfull + Tpart > caps.tasks || Tnst > caps.stasks ||
+        Tnred > caps.reds || Tslots > caps.slots || Tnst > caps.nnslots)
+        overflow |= 1;
+    if (*(volatile const int *)gen_count > gen_pcap) overflow |= 2;
+    if (threadIdx.x == 0) {
+        sum->ntuples = Ttup;
+        sum->nslices = Tsl;
+        sum->ntasks = (int)(Tfull + Tpart);
+        sum->nstasks = (int)Tnst;
+        sum->nreds = (int)Tnred;
+        sum->nslots = (int)Tslots;
+        sum->nnslots = (int)Tnst;
+        sum->overflow = overflow;
+    }
+    if (overflow) return;
+    // pass 2: write the schedule
+    long long fpos = bfull, ppos = Tfull + bpart;
+    auto put = [&](const ContractTask &t) { tasks[t.count == chunk ? fpos++ : ppos++] = t; };
+    for (int g = g0; g < g1; g++) {
+        if (mixed) {
+            const int cs = tot[(size_t)g * 9 + 8];
+            ofs[(size_t)g * 9 + 8] = bsl;
+            const int nn0 = (int)bnst;
+            for (int h = 0; h * wchunk < cs; h++) {
+                const int cc = min(wchunk, cs - h * wchunk);
+                stasks[bnst] = SliceTask{bsl + (long long)h * wchunk, cc, g, (int)bnst};
+                bnst++;
+            }
+            const int nns = (int)bnst - nn0;
+            bsl += cs;
+            for (int q = 0; q < 8; q++) {
+                const int c = tot[(size_t)g * 9 + q];
+                ofs[(size_t)g * 9 + q] = btup;
+                const int s0 = (int)bslots;
+                for (int h = 0; h * chunk < c; h++) {
+                    const int cc = min(chunk, c - h * chunk);
+                    put(ContractTask{btup + (long long)h * chunk, cc, g, q, (int)bslots});
+                    bslots++;
+                }
+                reds[(size_t)g * 8 + q] = ReduceItem{g, q, s0, (int)bslots - s0, nn0, nns};
+                btup += c;
+            }
+        } else {
+            for (int q = 0; q < 8; q++) {
+                const int c = tot[(size_t)g * 8 + q];
+                ofs[(size_t)g * 8 + q] = btup;
+                if (c <= chunk) {
+                    put(ContractTask{btup, c, g, q, -1});
+                } else {
+                    const int nch = (c + chunk - 1) / chunk;
+                    reds[bnred++] = ReduceItem{g, q, (int)bslots, nch, 0, 0};
+                    for (int h = 0; h < nch; h++) {
+                        const int cc = min(chunk, c - h * chunk);
+                        put(ContractTask{btup + (long long)h * chunk, cc, g, q, (int)bslots});
+                        bslots++;
+                    }
+                }
+                btup += c;
+            }
+        }
+    }
+}
+
 // ----------------------------------------------------------------- apply: FP64 contraction
 
 // One wavefront per task = a chunk of the tuples of one segment (g, gt):
@@ -1024,24 +1169,17 @@ __global__ void __launch_bounds__(256) k_enum(int nw, const WorkItem *work, cons
 // through the wave's LDS slice.  The result goes to the output node when the
 // segment has one chunk, else to a partial slot summed by k_reduce_partials.
 template <int K>
-__global__ void __launch_bounds__(256) k_contract_fp64(int ntask, const ContractTask *tasks, const WorkItem *work,
-                                                       const JobTab *jobs, const TermTab *tabs,
-                                                       const Tuple *tuples, const double *genPool,
-                                                       double *partials) {
+__device__ inline void contract_fp64_task(const ContractTask task, const WorkItem *work, const JobTab *jobs,
+                                          const TermTab *tabs, const Tuple *tuples, const double *genPool,
+                                          double *partials, double *B, int lane) {
     constexpr int K2 = K * K, K3 = K2 * K;
     constexpr int NP = (K2 + 63) / 64;
-    __shared__ double buf[4][K3];
-    const int wave = wave_id(), lane = threadIdx.x & 63;
-    const int ti = blockIdx.x * 4 + wave;
-    if (ti >= ntask) return;
-    const ContractTask task = tasks[ti];
     const WorkItem wi = work[task.g];
     const JobTab &J = jobs[wi.job];
     const int gt = task.gt;
     const gdouble *blocks = (const gdouble *)J.operBlocks;
     const gdouble *inCoef = (const gdouble *)J.inCoef;
     const gdouble *gen = (const gdouble *)genPool;
-    double *B = buf[wave];
     double acc[NP][K];
 #pragma unroll
     for (int p = 0; p < NP; p++)
@@ -1117,21 +1255,40 @@ __global__ void __launch_bounds__(256) k_contract_fp64(int ntask, const Contract
     }
 }
 
+// The task count comes from the device-built schedule (sum) when given; the grid
+// covers the schedule's capacity and waves past the count exit.
+template <int K>
+__global__ void __launch_bounds__(256) k_contract_fp64(int ntask_arg, const TaskSummary *sum,
+                                                       const ContractTask *tasks, const WorkItem *work,
+                                                       const JobTab *jobs, const TermTab *tabs,
+                                                       const Tuple *tuples, const double *genPool,
+                                                       double *partials) {
+    constexpr int K3 = K * K * K;
+    __shared__ double buf[4][K3];
+    const int wave = wave_id(), lane = threadIdx.x & 63;
+    const int ntask = sum ? (sum->overflow ? 0 : sum->ntasks) : ntask_arg;
+    const int ti = blockIdx.x * 4 + wave;
+    if (ti >= ntask) return;
+    contract_fp64_task<K>(tasks[ti], work, jobs, tabs, tuples, genPool, partials, buf[wave], lane);
+}
+
+
 // sums, per segment (g, gt), the FP64 tuple slots and the gt part of the mixed
 // node slots in slot order (deterministic)
-__global__ void k_reduce_partials(int n, const ReduceItem *items, const WorkItem *work, const JobTab *jobs,
-                                  const double *partials, const double *nodeslots, int K3) {
-    const int i = blockIdx.x;
-    if (i >= n) return;
-    const ReduceItem it = items[i];
-    const WorkItem wi = work[it.g];
-    const JobTab &J = jobs[wi.job];
-    double *dst = J.outCoef + (size_t)wi.node * 8 * K3 + it.gt * K3;
-    for (int e = threadIdx.x; e < K3; e += blockDim.x) {
-        double s = 0.0;
-        for (int q = 0; q < it.nslots; q++) s += partials[(size_t)(it.slot0 + q) * K3 + e];
-        for (int q = 0; q < it.nnslots; q++) s += nodeslots[((size_t)(it.nslot0 + q) * 8 + it.gt) * K3 + e];
-        dst[e] = s;
+__global__ void k_reduce_partials(int n_arg, const TaskSummary *sum, const ReduceItem *items, const WorkItem *work,
+                                  const JobTab *jobs, const double *partials, const double *nodeslots, int K3) {
+    const int n = sum ? (sum->overflow ? 0 : sum->nreds) : n_arg;
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        const ReduceItem it = items[i];
+        const WorkItem wi = work[it.g];
+        const JobTab &J = jobs[wi.job];
+        double *dst = J.outCoef + (size_t)wi.node * 8 * K3 + it.gt * K3;
+        for (int e = threadIdx.x; e < K3; e += blockDim.x) {
+            double s = 0.0;
+            for (int q = 0; q < it.nslots; q++) s += partials[(size_t)(it.slot0 + q) * K3 + e];
+            for (int q = 0; q < it.nnslots; q++) s += nodeslots[((size_t)(it.nslot0 + q) * 8 + it.gt) * K3 + e];
+            dst[e] = s;
+        }
     }
 }
 
@@ -1147,7 +1304,8 @@ template <int MODE>
 __global__ void __launch_bounds__(256) k_enum_mixed(int nw, const WorkItem *work, const JobTab *jobs,
                                                     const TermTab *tabs, GenState gs, int *thread_off,
                                                     int *seg_total, const long long *seg_ofs, Tuple *out,
-                                                    Slice *sout) {
+                                                    Slice *sout, const TaskSummary *sum) {
+    if (MODE == 1 && sum->overflow) return;
     const int g = blockIdx.x;
     const WorkItem wi = work[g];
     const JobTab &J = jobs[wi.job];
@@ -1202,16 +1360,16 @@ __global__ void __launch_bounds__(256) k_enum_mixed(int nw, const WorkItem *work
                             } else {
                                 if (MODE == 1) {
                                     Slice sl;
-                                    sl.g = g;
+                                    sl.w = tb.w;
                                     sl.f = f;
-                                    sl.term = (uint8_t)t;
+                                    sl.xo = (int)(tb.blk + ((dx + tb.band) * 4 + (ft & 1)) * 64);
+                                    sl.yo = (int)(tb.blk + ((dy + tb.band) * 4 + ((ft >> 1) & 1)) * 64);
+                                    sl.zo = (int)(tb.blk + ((dz + tb.band) * 4 + ((ft >> 2) & 1)) * 64);
                                     sl.ft = (uint8_t)ft;
+                                    // every gt except (gt, ft) = (0, 0) below the roots (the
+                                    // contraction kernel relies on this being the only mask)
                                     sl.mask = (uint8_t)((ft == 0 && J.depth > 0) ? 0xFE : 0xFF);
-                                    sl.pad = 0;
-                                    sl.dx = (int16_t)dx;
-                                    sl.dy = (int16_t)dy;
-                                    sl.dz = (int16_t)dz;
-                                    sl.pad2 = 0;
+                                    for (int b = 0; b < 6; b++) sl.pad[b] = 0;
                                     sout[base[8] + cnt[8]] = sl;
                                 }
                                 cnt[8]++;
@@ -1247,17 +1405,45 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 // D[4 (lane/16) + r][lane%16] in register r.  Intermediates rotate through LDS
 // (rows padded to 68 and 132 floats: conflict-free for the access patterns).
 // Each slice result is scaled by the term weight and accumulated in FP64.
-__global__ void __launch_bounds__(256) k_contract_mfma8(int ntask, const SliceTask *tasks, const WorkItem *work,
-                                                        const JobTab *jobs, const TermTab *tabs,
-                                                        const Slice *slices, const double *genPool,
-                                                        double *nodeslots) {
-    constexpr int K = 8, K2 = 64, K3 = 512;
+// operands of one slice in registers: A fragments of the three passes and the
+// pass-x B fragments of f (converted to FP32)
+struct SliceOps {
+    float ax[2], ay[2], az[2];
+    float fb[4][2];
+    double w;
+    int mask;
+};
+
+__device__ inline SliceOps load_slice_ops(const Slice &sl, const float *B32, const double *inCoef,
+                                          const double *genPool, int m, int q) {
+    constexpr int K = 8, K3 = 512;
+    const int gxm = m >> 3, am = m & 7;
+    SliceOps o;
+    const gfloat *px = (const gfloat *)B32 + sl.xo + gxm * 128 + am * K;
+    const gfloat *py = (const gfloat *)B32 + sl.yo + gxm * 128 + am * K;
+    const gfloat *pz = (const gfloat *)B32 + sl.zo + gxm * 128 + am * K;
+#pragma unroll
+    for (int s2 = 0; s2 < 2; s2++) {
+        o.ax[s2] = px[4 * s2 + q];
+        o.ay[s2] = py[4 * s2 + q];
+        o.az[s2] = pz[4 * s2 + q];
+    }
+    const gdouble *f = sl.f >= 0 ? (const gdouble *)inCoef + (size_t)sl.f * 8 * K3 + sl.ft * K3
+                                 : (const gdouble *)genPool + (size_t)(-sl.f - 1) * K3;
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int s2 = 0; s2 < 2; s2++) o.fb[t][s2] = (float)f[(16 * t + m) * K + 4 * s2 + q];
+    o.w = sl.w;
+    o.mask = sl.mask;
+    return o;
+}
+
+__device__ inline void mfma8_task(const SliceTask task, const WorkItem *work, const JobTab *jobs, const Slice *slices,
+                                  const double *genPool, double *nodeslots, float *lds) {
+    constexpr int K = 8, K3 = 512;
     constexpr int S1 = 68, S2 = 132;
-    // per-wave operand staging; reused for the cross-wave reduction at the end
-    __shared__ __align__(16) float lds[4 * 16 * S1 + 4 * 16 * S2];
     const int wave = wave_id(), lane = threadIdx.x & 63;
-    if ((int)blockIdx.x >= ntask) return;
-    const SliceTask task = tasks[blockIdx.x];
     const WorkItem wi = work[task.g];
     const JobTab &J = jobs[wi.job];
     float *L1 = lds + wave * 16 * S1;
@@ -1269,38 +1455,20 @@ __global__ void __launch_bounds__(256) k_contract_mfma8(int ntask, const SliceTa
     for (int t = 0; t < 16; t++)
 #pragma unroll
         for (int r = 0; r < 4; r++) acc[t][r] = 0.0;
-    // the four waves take every fourth slice of the task
+    // the four waves take every fourth slice of the task; the next slice's operands
+    // are loaded while the current one is contracted
+    SliceOps cur;
+    if (wave < task.count) cur = load_slice_ops(slices[task.start + wave], J.operBlocks32, J.inCoef, genPool, m, q);
     for (int it = wave; it < task.count; it += 4) {
-        const Slice sl = slices[task.start + it];
-        const TermTab &tb = tabs[J.tab0 + sl.term];
-        const gfloat *B32 = (const gfloat *)J.operBlocks32 + tb.blk;
-        const int ft = sl.ft;
-        const int fx = ft & 1, fy = (ft >> 1) & 1, fz = (ft >> 2) & 1;
-        // A fragments: row m = (g_d, a), column i = 4 s + q
-        float ax[2], ay[2], az[2];
-        {
-            const gfloat *px = B32 + ((sl.dx + tb.band) * 4 + 2 * gxm + fx) * K2 + am * K;
-            const gfloat *py = B32 + ((sl.dy + tb.band) * 4 + 2 * gxm + fy) * K2 + am * K;
-            const gfloat *pz = B32 + ((sl.dz + tb.band) * 4 + 2 * gxm + fz) * K2 + am * K;
-#pragma unroll
-            for (int s2 = 0; s2 < 2; s2++) {
-                ax[s2] = px[4 * s2 + q];
-                ay[s2] = py[4 * s2 + q];
-                az[s2] = pz[4 * s2 + q];
-            }
-        }
-        const gdouble *f = sl.f >= 0 ? (const gdouble *)J.inCoef + (size_t)sl.f * 8 * K3 + ft * K3
-                                     : (const gdouble *)genPool + (size_t)(-sl.f - 1) * K3;
+        SliceOps nxt;
+        if (it + 4 < task.count)
+            nxt = load_slice_ops(slices[task.start + it + 4], J.operBlocks32, J.inCoef, genPool, m, q);
         // pass x: C1[(gx,a)][n = l*8 + j] = sum_i X[a][i] f[n*8 + i]
 #pragma unroll
         for (int t = 0; t < 4; t++) {
             floatx4 c = {0.f, 0.f, 0.f, 0.f};
-            const gdouble *fr = f + (16 * t + m) * K;
 #pragma unroll
-            for (int s2 = 0; s2 < 2; s2++) {
-                const float b = (float)fr[4 * s2 + q];
-                c = __builtin_amdgcn_mfma_f32_16x16x4f32(ax[s2], b, c, 0, 0, 0);
-            }
+            for (int s2 = 0; s2 < 2; s2++) c = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.ax[s2], cur.fb[t][s2], c, 0, 0, 0);
 #pragma unroll
             for (int r = 0; r < 4; r++) L1[(4 * q + r) * S1 + 16 * t + m] = c[r];
         }
@@ -1312,33 +1480,31 @@ __global__ void __launch_bounds__(256) k_contract_mfma8(int ntask, const SliceTa
 #pragma unroll
             for (int s2 = 0; s2 < 2; s2++) {
                 const float b = L1[m * S1 + t2 * 8 + 4 * s2 + q];
-                c = __builtin_amdgcn_mfma_f32_16x16x4f32(ay[s2], b, c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.ay[s2], b, c, 0, 0, 0);
             }
 #pragma unroll
             for (int r = 0; r < 4; r++) L2[(4 * q + r) * S2 + 16 * t2 + m] = c[r];
         }
         wave_sync();
         // pass z: C3[(gz,c)][n3 = (gy,b)*16 + (gx,a)] = sum_l Z[c][l] C2[(gy,b)][l*16 + (gx,a)]
-        const double w = tb.w;
-        const int mask = sl.mask;
+        // Output component gt = gx | gy << 1 | gz << 2; only gt = 0 can be masked out
+        // (ft = 0 below the roots), and gt = 0 means gx = 0 (m < 8), gy = 0 (t3 < 8) and
+        // gz = 0 (q < 2): one weight per lane for tiles t3 < 8, the plain weight above.
+        const double wlow = (gxm == 0 && q < 2 && !(cur.mask & 1)) ? 0.0 : cur.w;
 #pragma unroll
         for (int t3 = 0; t3 < 16; t3++) {
             floatx4 c = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int s2 = 0; s2 < 2; s2++) {
                 const float b = L2[t3 * S2 + (4 * s2 + q) * 16 + m];
-                c = __builtin_amdgcn_mfma_f32_16x16x4f32(az[s2], b, c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.az[s2], b, c, 0, 0, 0);
             }
-            const int gy = t3 >> 3;
+            const double wm = t3 < 8 ? wlow : cur.w;
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int gz = (4 * q + r) >> 3;
-                const int gt = gxm | (gy << 1) | (gz << 2);
-                const double wm = ((mask >> gt) & 1) ? w : 0.0;
-                acc[t3][r] += wm * (double)c[r];
-            }
+            for (int r = 0; r < 4; r++) acc[t3][r] += wm * (double)c[r];
         }
         wave_sync();
+        cur = nxt;
     }
     // waves 1..3 add their accumulators into wave 0's, in that order (one 32 KB LDS buffer)
     double *R = (double *)lds;
@@ -1359,20 +1525,35 @@ __global__ void __launch_bounds__(256) k_contract_mfma8(int ntask, const SliceTa
         }
         __syncthreads();
     }
-    if (wave != 0) return;
-    double *dst = nodeslots + (size_t)task.slot * 8 * K3;
+    if (wave == 0) {
+        double *dst = nodeslots + (size_t)task.slot * 8 * K3;
 #pragma unroll
-    for (int t3 = 0; t3 < 16; t3++) {
-        const int gy = t3 >> 3, b = t3 & 7;
+        for (int t3 = 0; t3 < 16; t3++) {
+            const int gy = t3 >> 3, b = t3 & 7;
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int m3 = 4 * q + r;
-            const int gz = m3 >> 3, cc = m3 & 7;
-            const int gt = gxm | (gy << 1) | (gz << 2);
-            dst[gt * K3 + (cc * K + b) * K + am] = acc[t3][r];
+            for (int r = 0; r < 4; r++) {
+                const int m3 = 4 * q + r;
+                const int gz = m3 >> 3, cc = m3 & 7;
+                const int gt = gxm | (gy << 1) | (gz << 2);
+                dst[gt * K3 + (cc * K + b) * K + am] = acc[t3][r];
+            }
         }
     }
 }
+
+__global__ void __launch_bounds__(256) k_contract_mfma8(int ntask_arg, const TaskSummary *sum,
+                                                        const SliceTask *tasks, const WorkItem *work,
+                                                        const JobTab *jobs, const TermTab *tabs,
+                                                        const Slice *slices, const double *genPool,
+                                                        double *nodeslots) {
+    constexpr int S1 = 68, S2 = 132;
+    // per-wave operand staging; reused for the cross-wave reduction at the end
+    __shared__ __align__(16) float lds[4 * 16 * S1 + 4 * 16 * S2];
+    const int ntask = sum ? (sum->overflow ? 0 : sum->nstasks) : ntask_arg;
+    if ((int)blockIdx.x >= ntask) return;
+    mfma8_task(tasks[blockIdx.x], work, jobs, slices, genPool, nodeslots, lds);
+}
+
 
 __global__ void k_to_f32(size_t n, const double *in, float *out) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1732,14 +1913,14 @@ hipError_t launch_gen_compute(hipStream_t s, int first, int cap, const JobTab *j
 
 hipError_t launch_enum(hipStream_t s, int mode, int nw, const WorkItem *work, const JobTab *jobs,
                        const TermTab *tabs, const GenState &gs, int *thread_off, int *seg_total,
-                       const long long *seg_ofs, Tuple *out) {
+                       const long long *seg_ofs, Tuple *out, const TaskSummary *sum) {
     if (nw <= 0) return hipSuccess;
     if (mode == 0)
         hipLaunchKernelGGL(k_enum<0>, dim3(nw), dim3(256), 0, s, nw, work, jobs, tabs, gs, thread_off, seg_total,
-                           seg_ofs, out);
+                           seg_ofs, out, sum);
     else
         hipLaunchKernelGGL(k_enum<1>, dim3(nw), dim3(256), 0, s, nw, work, jobs, tabs, gs, thread_off, seg_total,
-                           seg_ofs, out);
+                           seg_ofs, out, sum);
     return hipGetLastError();
 }
 
@@ -1761,43 +1942,55 @@ hipError_t launch_transform(hipStream_t s, int mode, int n, const TransformItem 
     return hipGetLastError();
 }
 
-hipError_t launch_contract_fp64(hipStream_t s, int K, int ntask, const ContractTask *tasks, const WorkItem *work,
-                                const JobTab *jobs, const TermTab *tabs, const Tuple *tuples, const double *genPool,
-                                double *partials) {
-    if (ntask <= 0) return hipSuccess;
-    const int nb = (ntask + 3) / 4;
-    MWG_DISPATCH_K(K, hipLaunchKernelGGL(k_contract_fp64<KK>, dim3(nb), dim3(256), 0, s, ntask, tasks, work, jobs,
-                                         tabs, tuples, genPool, partials));
+hipError_t launch_build_tasks(hipStream_t s, int nw, int nseg, int chunk, int wchunk, const int *tot, TaskCaps caps,
+                              const int *gen_count, int gen_pcap, long long *ofs, ContractTask *tasks,
+                              SliceTask *stasks, ReduceItem *reds, TaskSummary *sum) {
+    hipLaunchKernelGGL(k_build_tasks, dim3(1), dim3(1024), 0, s, nw, nseg, chunk, wchunk, tot, caps, gen_count,
+                       gen_pcap, ofs, tasks, stasks, reds, sum);
     return hipGetLastError();
 }
 
-hipError_t launch_reduce_partials(hipStream_t s, int n, const ReduceItem *items, const WorkItem *work,
-                                  const JobTab *jobs, const double *partials, const double *nodeslots, int K3) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_reduce_partials, dim3(n), dim3(256), 0, s, n, items, work, jobs, partials, nodeslots, K3);
+hipError_t launch_contract_fp64(hipStream_t s, int K, int ntask, const TaskSummary *sum, int grid,
+                                const ContractTask *tasks, const WorkItem *work, const JobTab *jobs,
+                                const TermTab *tabs, const Tuple *tuples, const double *genPool, double *partials) {
+    const int nb = sum ? grid : (ntask + 3) / 4;
+    if (nb <= 0) return hipSuccess;
+    MWG_DISPATCH_K(K, hipLaunchKernelGGL(k_contract_fp64<KK>, dim3(nb), dim3(256), 0, s, ntask, sum, tasks, work,
+                                         jobs, tabs, tuples, genPool, partials));
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_partials(hipStream_t s, int n, const TaskSummary *sum, int grid, const ReduceItem *items,
+                                  const WorkItem *work, const JobTab *jobs, const double *partials,
+                                  const double *nodeslots, int K3) {
+    const int nb = sum ? grid : n;
+    if (nb <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reduce_partials, dim3(nb), dim3(256), 0, s, n, sum, items, work, jobs, partials, nodeslots,
+                       K3);
     return hipGetLastError();
 }
 
 hipError_t launch_enum_mixed(hipStream_t s, int mode, int nw, const WorkItem *work, const JobTab *jobs,
                              const TermTab *tabs, const GenState &gs, int *thread_off, int *seg_total,
-                             const long long *seg_ofs, Tuple *out, Slice *sout) {
+                             const long long *seg_ofs, Tuple *out, Slice *sout, const TaskSummary *sum) {
     if (nw <= 0) return hipSuccess;
     if (mode == 0)
         hipLaunchKernelGGL(k_enum_mixed<0>, dim3(nw), dim3(256), 0, s, nw, work, jobs, tabs, gs, thread_off,
-                           seg_total, seg_ofs, out, sout);
+                           seg_total, seg_ofs, out, sout, sum);
     else
         hipLaunchKernelGGL(k_enum_mixed<1>, dim3(nw), dim3(256), 0, s, nw, work, jobs, tabs, gs, thread_off,
-                           seg_total, seg_ofs, out, sout);
+                           seg_total, seg_ofs, out, sout, sum);
     return hipGetLastError();
 }
 
-hipError_t launch_contract_mfma(hipStream_t s, int K, int ntask, const SliceTask *tasks, const WorkItem *work,
-                                const JobTab *jobs, const TermTab *tabs, const Slice *slices, const double *genPool,
-                                double *nodeslots) {
-    if (ntask <= 0) return hipSuccess;
+hipError_t launch_contract_mfma(hipStream_t s, int K, int ntask, const TaskSummary *sum, int grid,
+                                const SliceTask *tasks, const WorkItem *work, const JobTab *jobs,
+                                const TermTab *tabs, const Slice *slices, const double *genPool, double *nodeslots) {
     if (K != 8) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_contract_mfma8, dim3(ntask), dim3(256), 0, s, ntask, tasks, work, jobs, tabs,
-                       slices, genPool, nodeslots);
+    const int nb = sum ? grid : ntask;
+    if (nb <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_contract_mfma8, dim3(nb), dim3(256), 0, s, ntask, sum, tasks, work, jobs, tabs, slices,
+                       genPool, nodeslots);
     return hipGetLastError();
 }
 
