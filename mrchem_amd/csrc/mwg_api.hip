@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -73,6 +74,18 @@ struct DBuf {
         p = q;
         n = std::max<size_t>(nn, 1);
     }
+    // ownership transfer (device block caches)
+    T *release() {
+        T *q = p;
+        p = nullptr;
+        n = 0;
+        return q;
+    }
+    void adopt(T *q, size_t m) {
+        if (p) HC(hipFree(p));
+        p = q;
+        n = m;
+    }
     void upload(const T *h, size_t m, hipStream_t s) {
         alloc(m);
         if (m) HC(hipMemcpyAsync(p, h, m * sizeof(T), hipMemcpyHostToDevice, s));
@@ -96,8 +109,63 @@ struct Stage {
     }
 };
 
+// Device blocks released by trees (coefficients, norms), kept for reuse by later trees
+// of the same context.  All work of a context is ordered on its one stream, so a block
+// handed out again is only touched after every earlier use of it.
+struct DevCache {
+    std::multimap<size_t, void *> blocks;  // bytes -> block
+    size_t bytes = 0;
+    static constexpr size_t LIMIT = (size_t)16 << 30;
+    void *get(size_t want, size_t &got) {
+        auto it = blocks.lower_bound(want);
+        if (it != blocks.end() && it->first <= 2 * want + ((size_t)1 << 20)) {
+            got = it->first;
+            void *q = it->second;
+            bytes -= it->first;
+            blocks.erase(it);
+            return q;
+        }
+        void *q = nullptr;
+        HC(hipMalloc(&q, std::max<size_t>(want, 8)));
+        got = std::max<size_t>(want, 8);
+        return q;
+    }
+    void put(void *q, size_t nbytes) {
+        if (!q) return;
+        blocks.insert({nbytes, q});
+        bytes += nbytes;
+        while (bytes > LIMIT && !blocks.empty()) {  // drop the largest blocks first
+            auto it = std::prev(blocks.end());
+            bytes -= it->first;
+            (void)hipFree(it->second);
+            blocks.erase(it);
+        }
+    }
+    ~DevCache() {
+        for (auto &b : blocks) (void)hipFree(b.second);
+    }
+};
+
+// pinned host area for device-to-host reads (plain DMA, no staging through pageable memory)
+struct Readback {
+    char *h = nullptr;
+    size_t cap = 0, off = 0;
+    ~Readback() {
+        if (h) (void)hipHostFree(h);
+    }
+    void reserve(size_t bytes) {
+        if (bytes <= cap) return;
+        if (h) HC(hipHostFree(h));
+        h = nullptr;
+        cap = std::max(bytes, (size_t)1 << 20);
+        HC(hipHostMalloc((void **)&h, cap, hipHostMallocDefault));
+    }
+};
+
 struct Ctx {
     int device = 0;
+    DevCache cache;
+    Readback rb;
     Stage stage;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -111,6 +179,7 @@ struct Ctx {
     DBuf<double> tbmax;
     DBuf<int> tpre, tpre0;
     DBuf<TaskSummary> tsum;
+    DBuf<int> compact_src;
     TaskCaps caps{1 << 20, 1 << 18, 1 << 15, 1 << 13, 1 << 15, 1 << 15, 1 << 13};
     DBuf<int> thread_off, gtotal;
     DBuf<long long> gofs;
@@ -235,10 +304,31 @@ struct TreeH {
         auto it = index.find(node_key(d, x, y, z));
         return it == index.end() ? -1 : it->second;
     }
+    // grow the coefficient and norm arrays through the context's block cache (the old
+    // contents are copied on the stream; no synchronisation)
+    void grow_cached(DBuf<double> &b, size_t m) {
+        if (m <= b.n && b.p) return;
+        Ctx &C = *mra->ctx;
+        const size_t want = std::max<size_t>(m, 2 * b.n);
+        size_t got = 0;
+        double *q = (double *)C.cache.get(want * sizeof(double), got);
+        if (b.p) {
+            HC(hipMemcpyAsync(q, b.p, b.n * sizeof(double), hipMemcpyDeviceToDevice, C.stream));
+            const size_t old = b.n * sizeof(double);
+            C.cache.put(b.release(), old);
+        }
+        b.adopt(q, got / sizeof(double));
+    }
     void ensure_dev(size_t nodes) {
-        hipStream_t s = mra->ctx->stream;
-        coef.grow_keep(nodes * 8 * (size_t)K3(), s);
-        norm.grow_keep(nodes * 8, s);
+        grow_cached(coef, nodes * 8 * (size_t)K3());
+        grow_cached(norm, nodes * 8);
+    }
+    ~TreeH() {
+        if (!mra) return;
+        Ctx &C = *mra->ctx;
+        const size_t cb = coef.n * sizeof(double), nb = norm.n * sizeof(double);
+        C.cache.put(coef.release(), cb);
+        C.cache.put(norm.release(), nb);
     }
     void build_hash() {
         if (hvalid) return;
@@ -729,19 +819,19 @@ static void compact_tree(TreeH &T, const std::vector<char> &alive) {
         for (int q = 0; q < 3; q++) lr[3 * k + q] = T.lr[3 * i + q];
         std::memcpy(&cn[8 * (size_t)k], &T.cnorm[8 * (size_t)i], 8 * sizeof(double));
     }
-    DBuf<int> dsrc;
-    dsrc.upload(src.data(), src.size(), C.stream);
-    DBuf<double> nc, nn;
-    nc.alloc((size_t)m * 8 * T.K3());
-    nn.alloc((size_t)m * 8);
-    hipLaunchKernelGGL(k_gather_nodes, dim3(m), dim3(256), 0, C.stream, m, dsrc.p, T.coef.p, nc.p, 8 * T.K3());
-    hipLaunchKernelGGL(k_gather_nodes, dim3(m), dim3(8), 0, C.stream, m, dsrc.p, T.norm.p, nn.p, 8);
+    upload_async(C, C.compact_src, src.data(), src.size());
+    size_t gc = 0, gn = 0;
+    double *nc = (double *)C.cache.get((size_t)m * 8 * T.K3() * sizeof(double), gc);
+    double *nn = (double *)C.cache.get((size_t)m * 8 * sizeof(double), gn);
+    hipLaunchKernelGGL(k_gather_nodes, dim3(m), dim3(256), 0, C.stream, m, C.compact_src.p, T.coef.p, nc,
+                       8 * T.K3());
+    hipLaunchKernelGGL(k_gather_nodes, dim3(m), dim3(8), 0, C.stream, m, C.compact_src.p, T.norm.p, nn, 8);
     HC(hipGetLastError());
-    HC(hipStreamSynchronize(C.stream));
-    std::swap(T.coef.p, nc.p);
-    std::swap(T.coef.n, nc.n);
-    std::swap(T.norm.p, nn.p);
-    std::swap(T.norm.n, nn.n);
+    const size_t oc = T.coef.n * sizeof(double), on = T.norm.n * sizeof(double);
+    C.cache.put(T.coef.release(), oc);
+    C.cache.put(T.norm.release(), on);
+    T.coef.adopt(nc, gc / sizeof(double));
+    T.norm.adopt(nn, gn / sizeof(double));
     T.depth.swap(depth);
     T.parent.swap(parent);
     T.child.swap(child);
@@ -1018,11 +1108,22 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                                       mixed ? C.nodeslots.p : nullptr, K3));
             HC(hipEventRecord(C.ev1, s));
             HC(launch_work_norms(s, nw, C.work.p, C.jobs.p, K3, C.wnorms.p));
-            HC(hipMemcpyAsync(&hsum, C.tsum.p, sizeof(TaskSummary), hipMemcpyDeviceToHost, s));
-            HC(hipMemcpyAsync(&hgen[0], C.genCount.p, sizeof(int), hipMemcpyDeviceToHost, s));
-            HC(hipMemcpyAsync(&hgen[1], C.genOverflow.p, sizeof(int), hipMemcpyDeviceToHost, s));
-            HC(hipMemcpyAsync(tot.data(), C.gtotal.p, tot.size() * sizeof(int), hipMemcpyDeviceToHost, s));
-            download_sync(C, C.wnorms, wn.data(), wn.size());
+            // one pinned read-back area: summary | generated count, overflow | counts | norms
+            const size_t o_sum = 0, o_gen = 64, o_tot = 128;
+            const size_t o_wn = (o_tot + tot.size() * sizeof(int) + 255) & ~(size_t)255;
+            C.rb.reserve(o_wn + wn.size() * sizeof(double));
+            char *h = C.rb.h;
+            HC(hipMemcpyAsync(h + o_sum, C.tsum.p, sizeof(TaskSummary), hipMemcpyDeviceToHost, s));
+            HC(hipMemcpyAsync(h + o_gen, C.genCount.p, sizeof(int), hipMemcpyDeviceToHost, s));
+            HC(hipMemcpyAsync(h + o_gen + 4, C.genOverflow.p, sizeof(int), hipMemcpyDeviceToHost, s));
+            HC(hipMemcpyAsync(h + o_tot, C.gtotal.p, tot.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+            HC(hipMemcpyAsync(h + o_wn, C.wnorms.p, wn.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+            HC(hipStreamSynchronize(s));
+            C.stage.off = 0;
+            std::memcpy(&hsum, h + o_sum, sizeof(TaskSummary));
+            std::memcpy(hgen, h + o_gen, 2 * sizeof(int));
+            std::memcpy(tot.data(), h + o_tot, tot.size() * sizeof(int));
+            std::memcpy(wn.data(), h + o_wn, wn.size() * sizeof(double));
             if (hgen[1]) throw Error(MWG_ERR_RANGE, "generated-node table overflow");
         };
         launch_count();
@@ -1190,15 +1291,27 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         for (int d = maxd - 1; d >= 0; d--)
             HC(launch_transform(s, 0, (int)(lofs[d + 1] - lofs[d]), C.titems.p + lofs[d], M.W.p, K));
     }
-    for (JobH &J : jobs) {
-        J.out->cnorm.resize(8 * (size_t)J.out->n());
-        if (J.out->n())
-            HC(hipMemcpyAsync(J.out->cnorm.data(), J.out->norm.p, 8 * (size_t)J.out->n() * sizeof(double),
-                              hipMemcpyDeviceToHost, s));
+    {
+        size_t total = 0;
+        for (JobH &J : jobs) total += 8 * (size_t)J.out->n();
+        C.rb.reserve(total * sizeof(double));
+        double *h = (double *)C.rb.h;
+        size_t off = 0;
+        for (JobH &J : jobs) {
+            const size_t m = 8 * (size_t)J.out->n();
+            if (m) HC(hipMemcpyAsync(h + off, J.out->norm.p, m * sizeof(double), hipMemcpyDeviceToHost, s));
+            off += m;
+        }
+        HC(hipStreamSynchronize(s));
+        C.stage.off = 0;
+        off = 0;
+        for (JobH &J : jobs) {
+            const size_t m = 8 * (size_t)J.out->n();
+            J.out->cnorm.assign(h + off, h + off + m);
+            off += m;
+            J.out->sqnorm_from_cnorm();
+        }
     }
-    HC(hipStreamSynchronize(s));
-    C.stage.off = 0;
-    for (JobH &J : jobs) J.out->sqnorm_from_cnorm();
     PT.lap("transforms");
     PT.report();
     const double t_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();

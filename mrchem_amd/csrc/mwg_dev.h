@@ -80,7 +80,8 @@ static_assert(sizeof(Slice) == 32, "slice record is 32 bytes");
 // (~8 * 2^-24 of its bound) would exceed the screening error
 constexpr double ROUTE_FP64 = 131072.0;
 // on the root level (no screening threshold yet) slices above ROUTE_ROOT times the
-// largest possible bound run in FP64; FP32 rounding of the others stays ~1e-10 of it
-constexpr double ROUTE_ROOT = 1e-3;
+// largest possible bound run in FP64; FP32 rounding of the others stays ~1e-12 of it,
+// small also for outputs far below that bound (pair densities of orthogonal orbitals)
+constexpr double ROUTE_ROOT = 1e-5;
 
 }  // namespace mwg

@@ -1366,9 +1366,9 @@ __global__ void __launch_bounds__(256) k_enum_mixed(int nw, const WorkItem *work
                                     sl.yo = (int)(tb.blk + ((dy + tb.band) * 4 + ((ft >> 1) & 1)) * 64);
                                     sl.zo = (int)(tb.blk + ((dz + tb.band) * 4 + ((ft >> 2) & 1)) * 64);
                                     sl.ft = (uint8_t)ft;
-                                    // every gt except (gt, ft) = (0, 0) below the roots (the
-                                    // contraction kernel relies on this being the only mask)
-                                    sl.mask = (uint8_t)((ft == 0 && J.depth > 0) ? 0xFE : 0xFF);
+                                    // the surviving components only: the same tuples as the
+                                    // plain work list, FP32 in place of FP64
+                                    sl.mask = (uint8_t)surv;
                                     for (int b = 0; b < 6; b++) sl.pad[b] = 0;
                                     sout[base[8] + cnt[8]] = sl;
                                 }
@@ -1487,10 +1487,12 @@ __device__ inline void mfma8_task(const SliceTask task, const WorkItem *work, co
         }
         wave_sync();
         // pass z: C3[(gz,c)][n3 = (gy,b)*16 + (gx,a)] = sum_l Z[c][l] C2[(gy,b)][l*16 + (gx,a)]
-        // Output component gt = gx | gy << 1 | gz << 2; only gt = 0 can be masked out
-        // (ft = 0 below the roots), and gt = 0 means gx = 0 (m < 8), gy = 0 (t3 < 8) and
-        // gz = 0 (q < 2): one weight per lane for tiles t3 < 8, the plain weight above.
-        const double wlow = (gxm == 0 && q < 2 && !(cur.mask & 1)) ? 0.0 : cur.w;
+        // Output component gt = gx | gy << 1 | gz << 2 with gx = m >> 3 (the lane), gy = t3 >> 3
+        // and gz = (4 q + r) >> 3 = q >> 1 (the lane): two weights per lane and slice, masked
+        // by the slice's surviving components.
+        const int gtl = gxm | ((q >> 1) << 2), gth = gtl | 2;
+        const double wlow = ((cur.mask >> gtl) & 1) ? cur.w : 0.0;
+        const double whigh = ((cur.mask >> gth) & 1) ? cur.w : 0.0;
 #pragma unroll
         for (int t3 = 0; t3 < 16; t3++) {
             floatx4 c = {0.f, 0.f, 0.f, 0.f};
@@ -1499,7 +1501,7 @@ __device__ inline void mfma8_task(const SliceTask task, const WorkItem *work, co
                 const float b = L2[t3 * S2 + (4 * s2 + q) * 16 + m];
                 c = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.az[s2], b, c, 0, 0, 0);
             }
-            const double wm = t3 < 8 ? wlow : cur.w;
+            const double wm = t3 < 8 ? wlow : whigh;
 #pragma unroll
             for (int r = 0; r < 4; r++) acc[t3][r] += wm * (double)c[r];
         }
