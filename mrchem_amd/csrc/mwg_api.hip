@@ -259,6 +259,8 @@ struct TreeH {
     DBuf<double> coef, norm;
     DBuf<unsigned long long> hkeys;
     DBuf<int> hvals;
+    std::vector<unsigned long long> hkeys_h;  // host copies of the hash table
+    std::vector<int> hvals_h;
     uint32_t hmask = 0;
     bool hvalid = false;
 
@@ -347,7 +349,22 @@ struct TreeH {
         hkeys.upload(keys.data(), cap, s);
         hvals.upload(vals.data(), cap, s);
         hmask = cap - 1;
+        hkeys_h.swap(keys);
+        hvals_h.swap(vals);
         hvalid = true;
+    }
+    // lookup in the host copy of the open-addressing table (valid after build_hash)
+    int find_hashed(int d, int64_t x, int64_t y, int64_t z) const {
+        const uint64_t k = node_key(d, x, y, z);
+        const unsigned long long kk = (unsigned long long)k + 1ull;
+        uint32_t h = dkey_hash(k) & hmask;
+        for (uint32_t probe = 0; probe <= hmask; probe++) {
+            const unsigned long long v = hkeys_h[h];
+            if (v == kk) return hvals_h[h];
+            if (v == 0ull) return -1;
+            h = (h + 1) & hmask;
+        }
+        return -1;
     }
     DHash dhash() const { return DHash{hkeys.p, hvals.p, hmask}; }
     void refresh_norms() {  // device norms -> host, square norm
@@ -698,7 +715,7 @@ static void children_features(const JobH &J, const MraH &M, int p, double tnorm,
     double wv[64];
     for (int k = 0; k < 64; k++) {
         const int x = bx + (k & 3), y = by + ((k >> 2) & 3), z = bz + (k >> 4);
-        idx[k] = (x < 0 || y < 0 || z < 0) ? -1 : J.in->find(d, x, y, z);
+        idx[k] = (x < 0 || y < 0 || z < 0) ? -1 : J.in->find_hashed(d, x, y, z);
         double a = 0.0;
         if (idx[k] >= 0)
             for (int q = 1; q < 8; q++) a += J.in->cnorm[8 * (size_t)idx[k] + q] * J.in->cnorm[8 * (size_t)idx[k] + q];
@@ -1044,17 +1061,19 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         // the generated-node pool) was too small, nothing of the schedule ran: grow and
         // run it again.
         const int nseg = mixed ? 9 : 8;
-        C.thread_off.alloc((size_t)nw * nseg * 256);
-        C.gtotal.alloc((size_t)nw * nseg);
-        C.gofs.alloc((size_t)nw * nseg);
+        // few output nodes (the roots): several workgroups share each node's candidates
+        const int nparts = std::max(1, std::min(16, (1024 + nw - 1) / nw));
+        C.thread_off.alloc((size_t)nw * nseg * nparts * 256);
+        C.gtotal.alloc((size_t)nw * nseg * nparts);
+        C.gofs.alloc((size_t)nw * nseg * nparts);
         C.wnorms.alloc((size_t)nw * 8);
         C.tsum.alloc(1);
         auto launch_count = [&]() {
             if (!mixed)
-                HC(launch_enum(s, 0, nw, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, nullptr,
+                HC(launch_enum(s, 0, nw, nparts, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, nullptr,
                                nullptr, nullptr));
             else
-                HC(launch_enum_mixed(s, 0, nw, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p,
+                HC(launch_enum_mixed(s, 0, nw, nparts, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p,
                                      nullptr, nullptr, nullptr, nullptr));
         };
         auto ensure_caps = [&](const TaskSummary *need) {
@@ -1081,18 +1100,18 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             }
         };
         std::vector<double> wn((size_t)nw * 8);
-        std::vector<int> tot((size_t)nw * nseg);
+        std::vector<int> tot((size_t)nw * nseg * nparts);
         TaskSummary hsum;
         int hgen[2];
         auto launch_schedule = [&]() {
             ensure_caps(nullptr);
-            HC(launch_build_tasks(s, nw, nseg, C.chunk, 4 * C.slice_chunk, C.gtotal.p, C.caps, C.genCount.p,
+            HC(launch_build_tasks(s, nw, nseg, nparts, C.chunk, 4 * C.slice_chunk, C.gtotal.p, C.caps, C.genCount.p,
                                   genPoolCap, C.gofs.p, C.tasks.p, C.stasks.p, C.reds.p, C.tsum.p));
             if (!mixed)
-                HC(launch_enum(s, 1, nw, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, C.gofs.p,
+                HC(launch_enum(s, 1, nw, nparts, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, C.gofs.p,
                                C.tuples.p, C.tsum.p));
             else
-                HC(launch_enum_mixed(s, 1, nw, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p,
+                HC(launch_enum_mixed(s, 1, nw, nparts, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p,
                                      C.gofs.p, C.tuples.p, C.slices.p, C.tsum.p));
             const int grid64 = std::max(1, (C.caps.tasks + 3) / 4);
             const int gridred = std::min(4096, std::max(1, C.caps.reds));
@@ -1151,8 +1170,10 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         genDone = hgen[0];
         for (int i = 0; i < nw; i++) {
             JobH &J = jobs[jt_job[work[i].job]];
-            for (int q = 0; q < 8; q++) J.tuples += tot[(size_t)i * nseg + q];
-            if (mixed) J.slices += tot[(size_t)i * 9 + 8];
+            for (int p = 0; p < nparts; p++) {
+                for (int q = 0; q < 8; q++) J.tuples += tot[((size_t)i * nseg + q) * nparts + p];
+                if (mixed) J.slices += tot[((size_t)i * 9 + 8) * nparts + p];
+            }
         }
         PT.lap("worklist+launch");
         float ms = 0.0f;

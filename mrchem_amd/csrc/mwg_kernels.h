@@ -127,7 +127,9 @@ hipError_t launch_gen_mark(hipStream_t s, int nw, const WorkItem *work, const Jo
 // generated nodes first .. min(device count, cap) - 1 (cap = entries allocated in the pool)
 hipError_t launch_gen_compute(hipStream_t s, int first, int cap, const JobTab *jobs, const GenState &gs,
                               const double *W, int K);
-hipError_t launch_enum(hipStream_t s, int mode, int nw, const WorkItem *work, const JobTab *jobs,
+// nparts workgroups share each output node's candidates (counts and offsets per
+// (node, segment, part))
+hipError_t launch_enum(hipStream_t s, int mode, int nw, int nparts, const WorkItem *work, const JobTab *jobs,
                        const TermTab *tabs, const GenState &gs, int *thread_off, int *seg_total,
                        const long long *seg_ofs, Tuple *out, const TaskSummary *sum);
 hipError_t launch_work_norms(hipStream_t s, int nw, const WorkItem *work, const JobTab *jobs, int K3,
@@ -157,7 +159,7 @@ struct SliceTask {
     int g;
     int slot;  // node slot (8 K^3 doubles)
 };
-hipError_t launch_enum_mixed(hipStream_t s, int mode, int nw, const WorkItem *work, const JobTab *jobs,
+hipError_t launch_enum_mixed(hipStream_t s, int mode, int nw, int nparts, const WorkItem *work, const JobTab *jobs,
                              const TermTab *tabs, const GenState &gs, int *thread_off, int *seg_total,
                              const long long *seg_ofs, Tuple *out, Slice *sout, const TaskSummary *sum);
 hipError_t launch_contract_mfma(hipStream_t s, int K, int ntask, const TaskSummary *sum, int grid,
@@ -177,8 +179,8 @@ hipError_t launch_add(hipStream_t s, int n, const MulItem *items, const DevBasis
 hipError_t launch_to_f32(hipStream_t s, size_t n, const double *in, float *out);
 // from the per-segment counts (nseg = 8: plain FP64, 9: mixed): segment offsets, FP64
 // tasks (full chunks first), MFMA slice tasks, reduce items, slot numbering
-hipError_t launch_build_tasks(hipStream_t s, int nw, int nseg, int chunk, int wchunk, const int *tot, TaskCaps caps,
-                              const int *gen_count, int gen_pcap, long long *ofs, ContractTask *tasks,
+hipError_t launch_build_tasks(hipStream_t s, int nw, int nseg, int nparts, int chunk, int wchunk, const int *tot,
+                              TaskCaps caps, const int *gen_count, int gen_pcap, long long *ofs, ContractTask *tasks,
                               SliceTask *stasks, ReduceItem *reds, TaskSummary *sum);
 // with sum: the task count is read on the device; grid (blocks) covers the capacity
 hipError_t launch_contract_fp64(hipStream_t s, int K, int ntask, const TaskSummary *sum, int grid,

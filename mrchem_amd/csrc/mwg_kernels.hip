@@ -761,9 +761,12 @@ __device__ inline void term_prefix(CandTerms &ct, const JobTab &J, const TermTab
 
 template <class Visit>
 __device__ inline void walk_candidates(const CandTerms &ct, const int *fcache, const FCache &fc, const WorkItem &wi,
-                                       const JobTab &J, const TermTab *tabs, const GenState &gs, Visit &&visit) {
+                                       const JobTab &J, const TermTab *tabs, const GenState &gs, int part, int nparts,
+                                       Visit &&visit) {
     const int total = ct.start[J.ntab];
-    for (int k = threadIdx.x; k < total; k += blockDim.x) {
+    // part of nparts workgroups that share this output node: a contiguous candidate range
+    const int k0 = (int)((long long)total * part / nparts), k1 = (int)((long long)total * (part + 1) / nparts);
+    for (int k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
         int lo = 0, hi = J.ntab - 1;  // last t with start[t] <= k
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -952,11 +955,11 @@ __global__ void __launch_bounds__(64) k_job_route(int nbj, const BoxJob *bjs, co
 //          offsets inside each segment and the segment totals;
 //   mode 1 writes every tuple at seg_ofs[g*8+gt] + thread offset + running index.
 template <int MODE>
-__global__ void __launch_bounds__(256) k_enum(int nw, const WorkItem *work, const JobTab *jobs,
+__global__ void __launch_bounds__(256) k_enum(int nw, int nparts, const WorkItem *work, const JobTab *jobs,
                                               const TermTab *tabs, GenState gs, int *thread_off, int *seg_total,
                                               const long long *seg_ofs, Tuple *out, const TaskSummary *sum) {
     if (MODE == 1 && sum->overflow) return;
-    const int g = blockIdx.x;
+    const int g = blockIdx.x / nparts, part = blockIdx.x % nparts;
     const WorkItem wi = work[g];
     const JobTab &J = jobs[wi.job];
     __shared__ int scan[8][256];
@@ -967,8 +970,11 @@ __global__ void __launch_bounds__(256) k_enum(int nw, const WorkItem *work, cons
     int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     long long base[8];
     if (MODE == 1)
-        for (int q = 0; q < 8; q++) base[q] = seg_ofs[(size_t)g * 8 + q] + thread_off[((size_t)g * 8 + q) * 256 + threadIdx.x];
-    walk_candidates(ct, fcache, fc, wi, J, tabs, gs,
+        for (int q = 0; q < 8; q++) {
+            const size_t sp = ((size_t)g * 8 + q) * nparts + part;  // (segment, part)
+            base[q] = seg_ofs[sp] + thread_off[sp * 256 + threadIdx.x];
+        }
+    walk_candidates(ct, fcache, fc, wi, J, tabs, gs, part, nparts,
                     [&](int t, const TermTab &tb, int dx, int dy, int dz, int f, const double *fn, int nft,
                         const double *nx, const double *ny, const double *nz) {
                         const double aw = fabs(tb.w);
@@ -1010,9 +1016,10 @@ __global__ void __launch_bounds__(256) k_enum(int nw, const WorkItem *work, cons
             for (int q = 0; q < 8; q++) scan[q][threadIdx.x] += v[q];
             __syncthreads();
         }
-        for (int q = 0; q < 8; q++) thread_off[((size_t)g * 8 + q) * 256 + threadIdx.x] = scan[q][threadIdx.x] - cnt[q];
+        for (int q = 0; q < 8; q++)
+            thread_off[(((size_t)g * 8 + q) * nparts + part) * 256 + threadIdx.x] = scan[q][threadIdx.x] - cnt[q];
         if (threadIdx.x == 255)
-            for (int q = 0; q < 8; q++) seg_total[(size_t)g * 8 + q] = scan[q][255];
+            for (int q = 0; q < 8; q++) seg_total[((size_t)g * 8 + q) * nparts + part] = scan[q][255];
     }
     (void)nw;
 }
@@ -1050,22 +1057,35 @@ __device__ inline long long block_excl_scan_1024(long long v, long long *wsum, l
     return wsum[wave] + incl - v;
 }
 
-__global__ void __launch_bounds__(1024) k_build_tasks(int nw, int nseg, int chunk, int wchunk, const int *tot,
+__global__ void __launch_bounds__(1024) k_build_tasks(int nw, int nseg, int nparts, int chunk, int wchunk,
+                                                      const int *tot,
                                                       TaskCaps caps, const int *gen_count, int gen_pcap,
                                                       long long *ofs, ContractTask *tasks, SliceTask *stasks,
                                                       ReduceItem *reds, TaskSummary *sum) {
     __shared__ long long wsum[17];
+    // count of segment sg (all parts), and the per-part offsets inside it
+    auto segc = [&](int sg) {
+        int c = 0;
+        for (int p = 0; p < nparts; p++) c += tot[(size_t)sg * nparts + p];
+        return c;
+    };
+    auto put_ofs = [&](int sg, long long start) {
+        for (int p = 0; p < nparts; p++) {
+            ofs[(size_t)sg * nparts + p] = start;
+            start += tot[(size_t)sg * nparts + p];
+        }
+    };
     const int g0 = (int)((long long)nw * threadIdx.x / 1024), g1 = (int)((long long)nw * (threadIdx.x + 1) / 1024);
     const bool mixed = nseg == 9;
     // pass 1: this thread's totals
     long long tup = 0, sl = 0, full = 0, part = 0, nst = 0, slots = 0, nred = 0;
     for (int g = g0; g < g1; g++) {
         if (mixed) {
-            const int cs = tot[(size_t)g * 9 + 8];
+            const int cs = segc(g * 9 + 8);
             sl += cs;
             nst += (cs + wchunk - 1) / wchunk;
             for (int q = 0; q < 8; q++) {
-                const int c = tot[(size_t)g * 9 + q];
+                const int c = segc(g * 9 + q);
                 tup += c;
                 full += c / chunk;
                 part += (c % chunk) != 0;
@@ -1073,7 +1093,7 @@ __global__ void __launch_bounds__(1024) k_build_tasks(int nw, int nseg, int chun
             }
         } else {
             for (int q = 0; q < 8; q++) {
-                const int c = tot[(size_t)g * 8 + q];
+                const int c = segc(g * 8 + q);
                 tup += c;
                 if (c <= chunk) {
                     if (c == chunk) full++;
@@ -1118,8 +1138,8 @@ __global__ void __launch_bounds__(1024) k_build_tasks(int nw, int nseg, int chun
     auto put = [&](const ContractTask &t) { tasks[t.count == chunk ? fpos++ : ppos++] = t; };
     for (int g = g0; g < g1; g++) {
         if (mixed) {
-            const int cs = tot[(size_t)g * 9 + 8];
-            ofs[(size_t)g * 9 + 8] = bsl;
+            const int cs = segc(g * 9 + 8);
+            put_ofs(g * 9 + 8, bsl);
             const int nn0 = (int)bnst;
             for (int h = 0; h * wchunk < cs; h++) {
                 const int cc = min(wchunk, cs - h * wchunk);
@@ -1129,8 +1149,8 @@ __global__ void __launch_bounds__(1024) k_build_tasks(int nw, int nseg, int chun
             const int nns = (int)bnst - nn0;
             bsl += cs;
             for (int q = 0; q < 8; q++) {
-                const int c = tot[(size_t)g * 9 + q];
-                ofs[(size_t)g * 9 + q] = btup;
+                const int c = segc(g * 9 + q);
+                put_ofs(g * 9 + q, btup);
                 const int s0 = (int)bslots;
                 for (int h = 0; h * chunk < c; h++) {
                     const int cc = min(chunk, c - h * chunk);
@@ -1142,8 +1162,8 @@ __global__ void __launch_bounds__(1024) k_build_tasks(int nw, int nseg, int chun
             }
         } else {
             for (int q = 0; q < 8; q++) {
-                const int c = tot[(size_t)g * 8 + q];
-                ofs[(size_t)g * 8 + q] = btup;
+                const int c = segc(g * 8 + q);
+                put_ofs(g * 8 + q, btup);
                 if (c <= chunk) {
                     put(ContractTask{btup, c, g, q, -1});
                 } else {
@@ -1301,12 +1321,12 @@ __global__ void k_reduce_partials(int n_arg, const TaskSummary *sum, const Reduc
 // thread: 8 tuple segments (per gt) + 1 slice segment; layout of thread_off and
 // seg_total: [g][9][...].
 template <int MODE>
-__global__ void __launch_bounds__(256) k_enum_mixed(int nw, const WorkItem *work, const JobTab *jobs,
+__global__ void __launch_bounds__(256) k_enum_mixed(int nw, int nparts, const WorkItem *work, const JobTab *jobs,
                                                     const TermTab *tabs, GenState gs, int *thread_off,
                                                     int *seg_total, const long long *seg_ofs, Tuple *out,
                                                     Slice *sout, const TaskSummary *sum) {
     if (MODE == 1 && sum->overflow) return;
-    const int g = blockIdx.x;
+    const int g = blockIdx.x / nparts, part = blockIdx.x % nparts;
     const WorkItem wi = work[g];
     const JobTab &J = jobs[wi.job];
     __shared__ int scan[9][256];
@@ -1317,8 +1337,11 @@ __global__ void __launch_bounds__(256) k_enum_mixed(int nw, const WorkItem *work
     int cnt[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     long long base[9];
     if (MODE == 1)
-        for (int q = 0; q < 9; q++) base[q] = seg_ofs[(size_t)g * 9 + q] + thread_off[((size_t)g * 9 + q) * 256 + threadIdx.x];
-    walk_candidates(ct, fcache, fc, wi, J, tabs, gs,
+        for (int q = 0; q < 9; q++) {
+            const size_t sp = ((size_t)g * 9 + q) * nparts + part;  // (segment, part)
+            base[q] = seg_ofs[sp] + thread_off[sp * 256 + threadIdx.x];
+        }
+    walk_candidates(ct, fcache, fc, wi, J, tabs, gs, part, nparts,
                     [&](int t, const TermTab &tb, int dx, int dy, int dz, int f, const double *fn, int nft,
                         const double *nx, const double *ny, const double *nz) {
                         const double aw = fabs(tb.w);
@@ -1386,9 +1409,10 @@ __global__ void __launch_bounds__(256) k_enum_mixed(int nw, const WorkItem *work
             for (int q = 0; q < 9; q++) scan[q][threadIdx.x] += v[q];
             __syncthreads();
         }
-        for (int q = 0; q < 9; q++) thread_off[((size_t)g * 9 + q) * 256 + threadIdx.x] = scan[q][threadIdx.x] - cnt[q];
+        for (int q = 0; q < 9; q++)
+            thread_off[(((size_t)g * 9 + q) * nparts + part) * 256 + threadIdx.x] = scan[q][threadIdx.x] - cnt[q];
         if (threadIdx.x == 255)
-            for (int q = 0; q < 9; q++) seg_total[(size_t)g * 9 + q] = scan[q][255];
+            for (int q = 0; q < 9; q++) seg_total[((size_t)g * 9 + q) * nparts + part] = scan[q][255];
     }
     (void)nw;
 }
@@ -1913,16 +1937,17 @@ hipError_t launch_gen_compute(hipStream_t s, int first, int cap, const JobTab *j
     return hipGetLastError();
 }
 
-hipError_t launch_enum(hipStream_t s, int mode, int nw, const WorkItem *work, const JobTab *jobs,
+hipError_t launch_enum(hipStream_t s, int mode, int nw, int nparts, const WorkItem *work, const JobTab *jobs,
                        const TermTab *tabs, const GenState &gs, int *thread_off, int *seg_total,
                        const long long *seg_ofs, Tuple *out, const TaskSummary *sum) {
     if (nw <= 0) return hipSuccess;
+    const unsigned nb = (unsigned)nw * (unsigned)nparts;
     if (mode == 0)
-        hipLaunchKernelGGL(k_enum<0>, dim3(nw), dim3(256), 0, s, nw, work, jobs, tabs, gs, thread_off, seg_total,
-                           seg_ofs, out, sum);
+        hipLaunchKernelGGL(k_enum<0>, dim3(nb), dim3(256), 0, s, nw, nparts, work, jobs, tabs, gs, thread_off,
+                           seg_total, seg_ofs, out, sum);
     else
-        hipLaunchKernelGGL(k_enum<1>, dim3(nw), dim3(256), 0, s, nw, work, jobs, tabs, gs, thread_off, seg_total,
-                           seg_ofs, out, sum);
+        hipLaunchKernelGGL(k_enum<1>, dim3(nb), dim3(256), 0, s, nw, nparts, work, jobs, tabs, gs, thread_off,
+                           seg_total, seg_ofs, out, sum);
     return hipGetLastError();
 }
 
@@ -1944,11 +1969,11 @@ hipError_t launch_transform(hipStream_t s, int mode, int n, const TransformItem 
     return hipGetLastError();
 }
 
-hipError_t launch_build_tasks(hipStream_t s, int nw, int nseg, int chunk, int wchunk, const int *tot, TaskCaps caps,
-                              const int *gen_count, int gen_pcap, long long *ofs, ContractTask *tasks,
+hipError_t launch_build_tasks(hipStream_t s, int nw, int nseg, int nparts, int chunk, int wchunk, const int *tot,
+                              TaskCaps caps, const int *gen_count, int gen_pcap, long long *ofs, ContractTask *tasks,
                               SliceTask *stasks, ReduceItem *reds, TaskSummary *sum) {
-    hipLaunchKernelGGL(k_build_tasks, dim3(1), dim3(1024), 0, s, nw, nseg, chunk, wchunk, tot, caps, gen_count,
-                       gen_pcap, ofs, tasks, stasks, reds, sum);
+    hipLaunchKernelGGL(k_build_tasks, dim3(1), dim3(1024), 0, s, nw, nseg, nparts, chunk, wchunk, tot, caps,
+                       gen_count, gen_pcap, ofs, tasks, stasks, reds, sum);
     return hipGetLastError();
 }
 
@@ -1972,15 +1997,16 @@ hipError_t launch_reduce_partials(hipStream_t s, int n, const TaskSummary *sum, 
     return hipGetLastError();
 }
 
-hipError_t launch_enum_mixed(hipStream_t s, int mode, int nw, const WorkItem *work, const JobTab *jobs,
+hipError_t launch_enum_mixed(hipStream_t s, int mode, int nw, int nparts, const WorkItem *work, const JobTab *jobs,
                              const TermTab *tabs, const GenState &gs, int *thread_off, int *seg_total,
                              const long long *seg_ofs, Tuple *out, Slice *sout, const TaskSummary *sum) {
     if (nw <= 0) return hipSuccess;
+    const unsigned nb = (unsigned)nw * (unsigned)nparts;
     if (mode == 0)
-        hipLaunchKernelGGL(k_enum_mixed<0>, dim3(nw), dim3(256), 0, s, nw, work, jobs, tabs, gs, thread_off,
+        hipLaunchKernelGGL(k_enum_mixed<0>, dim3(nb), dim3(256), 0, s, nw, nparts, work, jobs, tabs, gs, thread_off,
                            seg_total, seg_ofs, out, sout, sum);
     else
-        hipLaunchKernelGGL(k_enum_mixed<1>, dim3(nw), dim3(256), 0, s, nw, work, jobs, tabs, gs, thread_off,
+        hipLaunchKernelGGL(k_enum_mixed<1>, dim3(nb), dim3(256), 0, s, nw, nparts, work, jobs, tabs, gs, thread_off,
                            seg_total, seg_ofs, out, sout, sum);
     return hipGetLastError();
 }
