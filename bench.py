@@ -49,6 +49,24 @@ def _l2diff(a, b, n=20000, seed=3):
     return math.sqrt((d ** 2).mean() * 40.0 ** 3)
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 counter summary
+    (profiles/hbm_traffic_<kernel>_<workload>.txt: FETCH_SIZE and WRITE_SIZE runs of this
+    benchmark, each scaled by the same counter on a copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{kernel}_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if ":" in line and not line.startswith("#"):
+            k, v = line.split(":", 1)
+            vals[k.strip()] = v.strip()
+    try:
+        return float(vals["traffic_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
 def run_variant(G, mra, trees, ops, prec, flags, steps, warmup):
     n = len(trees)
     for _ in range(warmup):
@@ -316,7 +334,9 @@ def main():
     tc = best["t_contract_per_step"]
     achieved = alg / tc if tc > 0 else 0.0
     roofline = dict(bound="mfma", achieved=achieved / 1e12, peak=PEAK[dtype] / 1e12, unit="TFLOP/s",
-                    frac=achieved / PEAK[dtype], traffic=None,
+                    frac=achieved / PEAK[dtype],
+                    traffic=measured_traffic("k_contract_mfma8" if dtype == "f32" else "k_contract_fp64",
+                                             f"helmholtz{args.orbitals}_k{args.k}_prec{args.prec:g}"),
                     kernel="k_contract_mfma8 + k_contract_fp64 + k_reduce_partials" if dtype == "f32"
                     else "k_contract_fp64 + k_reduce_partials",
                     issued_tflops=(best["flops_per_step"] / tc / 1e12) if tc > 0 else None,

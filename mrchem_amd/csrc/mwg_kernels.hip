@@ -1488,26 +1488,38 @@ __device__ inline void mfma8_task(const SliceTask task, const WorkItem *work, co
         if (it + 4 < task.count)
             nxt = load_slice_ops(slices[task.start + it + 4], J.operBlocks32, J.inCoef, genPool, m, q);
         // pass x: C1[(gx,a)][n = l*8 + j] = sum_i X[a][i] f[n*8 + i]
+        // (tiles in groups of 4 independent MFMA chains, operands read before the MFMAs)
+        {
+            floatx4 c[4];
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
-            floatx4 c = {0.f, 0.f, 0.f, 0.f};
+            for (int t = 0; t < 4; t++)
+                c[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.ax[0], cur.fb[t][0], floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
 #pragma unroll
-            for (int s2 = 0; s2 < 2; s2++) c = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.ax[s2], cur.fb[t][s2], c, 0, 0, 0);
+            for (int t = 0; t < 4; t++) c[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.ax[1], cur.fb[t][1], c[t], 0, 0, 0);
 #pragma unroll
-            for (int r = 0; r < 4; r++) L1[(4 * q + r) * S1 + 16 * t + m] = c[r];
+            for (int t = 0; t < 4; t++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) L1[(4 * q + r) * S1 + 16 * t + m] = c[t][r];
         }
         wave_sync();
         // pass y: C2[(gy,b)][n2 = l*16 + (gx,a)] = sum_j Y[b][j] C1[(gx,a)][l*8 + j]
 #pragma unroll
-        for (int t2 = 0; t2 < 8; t2++) {
-            floatx4 c = {0.f, 0.f, 0.f, 0.f};
+        for (int g4 = 0; g4 < 2; g4++) {
+            float b[4][2];
 #pragma unroll
-            for (int s2 = 0; s2 < 2; s2++) {
-                const float b = L1[m * S1 + t2 * 8 + 4 * s2 + q];
-                c = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.ay[s2], b, c, 0, 0, 0);
-            }
+            for (int t = 0; t < 4; t++)
 #pragma unroll
-            for (int r = 0; r < 4; r++) L2[(4 * q + r) * S2 + 16 * t2 + m] = c[r];
+                for (int s2 = 0; s2 < 2; s2++) b[t][s2] = L1[m * S1 + (4 * g4 + t) * 8 + 4 * s2 + q];
+            floatx4 c[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+                c[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.ay[0], b[t][0], floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < 4; t++) c[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.ay[1], b[t][1], c[t], 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) L2[(4 * q + r) * S2 + 16 * (4 * g4 + t) + m] = c[t][r];
         }
         wave_sync();
         // pass z: C3[(gz,c)][n3 = (gy,b)*16 + (gx,a)] = sum_l Z[c][l] C2[(gy,b)][l*16 + (gx,a)]
@@ -1518,16 +1530,23 @@ __device__ inline void mfma8_task(const SliceTask task, const WorkItem *work, co
         const double wlow = ((cur.mask >> gtl) & 1) ? cur.w : 0.0;
         const double whigh = ((cur.mask >> gth) & 1) ? cur.w : 0.0;
 #pragma unroll
-        for (int t3 = 0; t3 < 16; t3++) {
-            floatx4 c = {0.f, 0.f, 0.f, 0.f};
+        for (int g4 = 0; g4 < 4; g4++) {
+            float b[4][2];
 #pragma unroll
-            for (int s2 = 0; s2 < 2; s2++) {
-                const float b = L2[t3 * S2 + (4 * s2 + q) * 16 + m];
-                c = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.az[s2], b, c, 0, 0, 0);
-            }
-            const double wm = t3 < 8 ? wlow : whigh;
+            for (int t = 0; t < 4; t++)
 #pragma unroll
-            for (int r = 0; r < 4; r++) acc[t3][r] += wm * (double)c[r];
+                for (int s2 = 0; s2 < 2; s2++) b[t][s2] = L2[(4 * g4 + t) * S2 + (4 * s2 + q) * 16 + m];
+            floatx4 c[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+                c[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.az[0], b[t][0], floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < 4; t++) c[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.az[1], b[t][1], c[t], 0, 0, 0);
+            const double wm = g4 < 2 ? wlow : whigh;  // t3 = 4 g4 + t < 8 iff g4 < 2
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) acc[4 * g4 + t][r] += wm * (double)c[t][r];
         }
         wave_sync();
         cur = nxt;
