@@ -179,6 +179,7 @@ struct Ctx {
     DBuf<double> tbmax;
     DBuf<int> tpre, tpre0;
     DBuf<TaskSummary> tsum;
+    DBuf<SegRec> segs;
     DBuf<int> compact_src;
     TaskCaps caps{1 << 20, 1 << 18, 1 << 15, 1 << 13, 1 << 15, 1 << 15, 1 << 13};
     DBuf<int> thread_off, gtotal;
@@ -1066,6 +1067,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         C.thread_off.alloc((size_t)nw * nseg * nparts * 256);
         C.gtotal.alloc((size_t)nw * nseg * nparts);
         C.gofs.alloc((size_t)nw * nseg * nparts);
+        C.segs.alloc((size_t)nw * 8);
         C.wnorms.alloc((size_t)nw * 8);
         C.tsum.alloc(1);
         auto launch_count = [&]() {
@@ -1106,7 +1108,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         auto launch_schedule = [&]() {
             ensure_caps(nullptr);
             HC(launch_build_tasks(s, nw, nseg, nparts, C.chunk, 4 * C.slice_chunk, C.gtotal.p, C.caps, C.genCount.p,
-                                  genPoolCap, C.gofs.p, C.tasks.p, C.stasks.p, C.reds.p, C.tsum.p));
+                                  genPoolCap, C.gofs.p, C.tasks.p, C.stasks.p, C.reds.p, C.tsum.p, C.segs.p));
             if (!mixed)
                 HC(launch_enum(s, 1, nw, nparts, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, C.gofs.p,
                                C.tuples.p, C.tsum.p));

@@ -179,9 +179,17 @@ hipError_t launch_add(hipStream_t s, int n, const MulItem *items, const DevBasis
 hipError_t launch_to_f32(hipStream_t s, size_t n, const double *in, float *out);
 // from the per-segment counts (nseg = 8: plain FP64, 9: mixed): segment offsets, FP64
 // tasks (full chunks first), MFMA slice tasks, reduce items, slot numbering
+// per tuple segment (output node, component): first tuple, count, first FP64 task, first
+// partial slot (-1: its single task writes the output directly)
+struct SegRec {
+    long long start;
+    int count;
+    int task0;
+    int slot0;
+};
 hipError_t launch_build_tasks(hipStream_t s, int nw, int nseg, int nparts, int chunk, int wchunk, const int *tot,
                               TaskCaps caps, const int *gen_count, int gen_pcap, long long *ofs, ContractTask *tasks,
-                              SliceTask *stasks, ReduceItem *reds, TaskSummary *sum);
+                              SliceTask *stasks, ReduceItem *reds, TaskSummary *sum, SegRec *segs);
 // with sum: the task count is read on the device; grid (blocks) covers the capacity
 hipError_t launch_contract_fp64(hipStream_t s, int K, int ntask, const TaskSummary *sum, int grid,
                                 const ContractTask *tasks, const WorkItem *work, const JobTab *jobs,

@@ -1057,75 +1057,61 @@ __device__ inline long long block_excl_scan_1024(long long v, long long *wsum, l
     return wsum[wave] + incl - v;
 }
 
+// One workgroup.  Phase 1: per thread (a range of output nodes) the totals of tuples,
+// slices, FP64 tasks, slots, slice tasks and reduce items; block scans give each
+// thread's offsets.  Phase 2: per segment (node, component) its record (tuple offset,
+// count, first task, first slot) and the per-part offsets the fill kernels use; per node
+// the slice segment and its slice tasks; reduce items.  Phase 3: the FP64 tasks, spread
+// evenly over the threads (each thread walks the segments from its first task on).
 __global__ void __launch_bounds__(1024) k_build_tasks(int nw, int nseg, int nparts, int chunk, int wchunk,
-                                                      const int *tot,
-                                                      TaskCaps caps, const int *gen_count, int gen_pcap,
-                                                      long long *ofs, ContractTask *tasks, SliceTask *stasks,
-                                                      ReduceItem *reds, TaskSummary *sum) {
+                                                      const int *tot, TaskCaps caps, const int *gen_count,
+                                                      int gen_pcap, long long *ofs, ContractTask *tasks,
+                                                      SliceTask *stasks, ReduceItem *reds, TaskSummary *sum,
+                                                      SegRec *segs) {
     __shared__ long long wsum[17];
-    // count of segment sg (all parts), and the per-part offsets inside it
     auto segc = [&](int sg) {
         int c = 0;
         for (int p = 0; p < nparts; p++) c += tot[(size_t)sg * nparts + p];
         return c;
     };
-    auto put_ofs = [&](int sg, long long start) {
-        for (int p = 0; p < nparts; p++) {
-            ofs[(size_t)sg * nparts + p] = start;
-            start += tot[(size_t)sg * nparts + p];
-        }
-    };
-    const int g0 = (int)((long long)nw * threadIdx.x / 1024), g1 = (int)((long long)nw * (threadIdx.x + 1) / 1024);
     const bool mixed = nseg == 9;
-    // pass 1: this thread's totals
-    long long tup = 0, sl = 0, full = 0, part = 0, nst = 0, slots = 0, nred = 0;
+    // FP64 tasks of a tuple segment with c tuples (plain: always one, even empty, so that
+    // every output component is written; mixed: the reduce item writes the component)
+    auto ntasks_of = [&](int c) { return mixed ? (c + chunk - 1) / chunk : (c <= chunk ? 1 : (c + chunk - 1) / chunk); };
+    auto nslots_of = [&](int c) { return mixed ? (c + chunk - 1) / chunk : (c <= chunk ? 0 : (c + chunk - 1) / chunk); };
+    const int g0 = (int)((long long)nw * threadIdx.x / 1024), g1 = (int)((long long)nw * (threadIdx.x + 1) / 1024);
+    long long tup = 0, sl = 0, nt = 0, nst = 0, slots = 0, nred = 0;
     for (int g = g0; g < g1; g++) {
         if (mixed) {
             const int cs = segc(g * 9 + 8);
             sl += cs;
             nst += (cs + wchunk - 1) / wchunk;
-            for (int q = 0; q < 8; q++) {
-                const int c = segc(g * 9 + q);
-                tup += c;
-                full += c / chunk;
-                part += (c % chunk) != 0;
-                slots += (c + chunk - 1) / chunk;
-            }
-        } else {
-            for (int q = 0; q < 8; q++) {
-                const int c = segc(g * 8 + q);
-                tup += c;
-                if (c <= chunk) {
-                    if (c == chunk) full++;
-                    else part++;
-                } else {
-                    const int nch = (c + chunk - 1) / chunk;
-                    full += c / chunk;
-                    part += (c % chunk) != 0;
-                    slots += nch;
-                    nred++;
-                }
-            }
+        }
+        for (int q = 0; q < 8; q++) {
+            const int c = segc(g * nseg + q);
+            tup += c;
+            nt += ntasks_of(c);
+            slots += nslots_of(c);
+            if (!mixed && c > chunk) nred++;
         }
     }
-    long long Ttup, Tsl, Tfull, Tpart, Tnst, Tslots, Tnred;
+    long long Ttup, Tsl, Tnt, Tnst, Tslots, Tnred;
     long long btup = block_excl_scan_1024(tup, wsum, &Ttup);
     long long bsl = block_excl_scan_1024(sl, wsum, &Tsl);
-    long long bfull = block_excl_scan_1024(full, wsum, &Tfull);
-    long long bpart = block_excl_scan_1024(part, wsum, &Tpart);
+    long long bnt = block_excl_scan_1024(nt, wsum, &Tnt);
     long long bnst = block_excl_scan_1024(nst, wsum, &Tnst);
     long long bslots = block_excl_scan_1024(slots, wsum, &Tslots);
     long long bnred = block_excl_scan_1024(nred, wsum, &Tnred);
     if (mixed) Tnred = 8LL * nw;
     int overflow = 0;
-    if (Ttup > caps.tuples || Tsl > caps.slices || Tfull + Tpart > caps.tasks || Tnst > caps.stasks ||
-        Tnred > caps.reds || Tslots > caps.slots || Tnst > caps.nnslots)
+    if (Ttup > caps.tuples || Tsl > caps.slices || Tnt > caps.tasks || Tnst > caps.stasks || Tnred > caps.reds ||
+        Tslots > caps.slots || Tnst > caps.nnslots)
         overflow |= 1;
     if (*(volatile const int *)gen_count > gen_pcap) overflow |= 2;
     if (threadIdx.x == 0) {
         sum->ntuples = Ttup;
         sum->nslices = Tsl;
-        sum->ntasks = (int)(Tfull + Tpart);
+        sum->ntasks = (int)Tnt;
         sum->nstasks = (int)Tnst;
         sum->nreds = (int)Tnred;
         sum->nslots = (int)Tslots;
@@ -1133,51 +1119,63 @@ __global__ void __launch_bounds__(1024) k_build_tasks(int nw, int nseg, int npar
         sum->overflow = overflow;
     }
     if (overflow) return;
-    // pass 2: write the schedule
-    long long fpos = bfull, ppos = Tfull + bpart;
-    auto put = [&](const ContractTask &t) { tasks[t.count == chunk ? fpos++ : ppos++] = t; };
+    // phase 2
     for (int g = g0; g < g1; g++) {
+        int nn0 = (int)bnst, nns = 0;
         if (mixed) {
-            const int cs = segc(g * 9 + 8);
-            put_ofs(g * 9 + 8, bsl);
-            const int nn0 = (int)bnst;
+            const int sg = g * 9 + 8;
+            long long o = bsl;
+            for (int p = 0; p < nparts; p++) {
+                ofs[(size_t)sg * nparts + p] = o;
+                o += tot[(size_t)sg * nparts + p];
+            }
+            const int cs = (int)(o - bsl);
             for (int h = 0; h * wchunk < cs; h++) {
-                const int cc = min(wchunk, cs - h * wchunk);
-                stasks[bnst] = SliceTask{bsl + (long long)h * wchunk, cc, g, (int)bnst};
+                stasks[bnst] = SliceTask{bsl + (long long)h * wchunk, min(wchunk, cs - h * wchunk), g, (int)bnst};
                 bnst++;
             }
-            const int nns = (int)bnst - nn0;
+            nns = (int)bnst - nn0;
             bsl += cs;
-            for (int q = 0; q < 8; q++) {
-                const int c = segc(g * 9 + q);
-                put_ofs(g * 9 + q, btup);
-                const int s0 = (int)bslots;
-                for (int h = 0; h * chunk < c; h++) {
-                    const int cc = min(chunk, c - h * chunk);
-                    put(ContractTask{btup + (long long)h * chunk, cc, g, q, (int)bslots});
-                    bslots++;
-                }
-                reds[(size_t)g * 8 + q] = ReduceItem{g, q, s0, (int)bslots - s0, nn0, nns};
-                btup += c;
-            }
-        } else {
-            for (int q = 0; q < 8; q++) {
-                const int c = segc(g * 8 + q);
-                put_ofs(g * 8 + q, btup);
-                if (c <= chunk) {
-                    put(ContractTask{btup, c, g, q, -1});
-                } else {
-                    const int nch = (c + chunk - 1) / chunk;
-                    reds[bnred++] = ReduceItem{g, q, (int)bslots, nch, 0, 0};
-                    for (int h = 0; h < nch; h++) {
-                        const int cc = min(chunk, c - h * chunk);
-                        put(ContractTask{btup + (long long)h * chunk, cc, g, q, (int)bslots});
-                        bslots++;
-                    }
-                }
-                btup += c;
-            }
         }
+        for (int q = 0; q < 8; q++) {
+            const int sg = g * nseg + q;
+            long long o = btup;
+            for (int p = 0; p < nparts; p++) {
+                ofs[(size_t)sg * nparts + p] = o;
+                o += tot[(size_t)sg * nparts + p];
+            }
+            const int c = (int)(o - btup);
+            const int k = ntasks_of(c), ns = nslots_of(c);
+            segs[(size_t)g * 8 + q] = SegRec{btup, c, (int)bnt, ns ? (int)bslots : -1};
+            if (mixed) reds[(size_t)g * 8 + q] = ReduceItem{g, q, (int)bslots, ns, nn0, nns};
+            else if (ns) reds[bnred++] = ReduceItem{g, q, (int)bslots, ns, 0, 0};
+            bnt += k;
+            bslots += ns;
+            btup = o;
+        }
+    }
+    __syncthreads();
+    // phase 3: tasks [i0, i1) of this thread; find the segment of i0 by binary search
+    // over the segment records (task starts are non-decreasing), then walk forward
+    const long long i0 = Tnt * threadIdx.x / 1024, i1 = Tnt * (threadIdx.x + 1) / 1024;
+    if (i0 >= i1) return;
+    int lo = 0, hi = 8 * nw - 1;
+    while (lo < hi) {  // last segment whose first task is <= i0 and that has tasks
+        const int mid = (lo + hi + 1) >> 1;
+        if (segs[mid].task0 <= i0) lo = mid;
+        else hi = mid - 1;
+    }
+    int sgi = lo;
+    for (long long i = i0; i < i1; i++) {
+        while (true) {
+            const SegRec &r = segs[sgi];
+            if (i < r.task0 + ntasks_of(r.count)) break;
+            sgi++;
+        }
+        const SegRec r = segs[sgi];
+        const int h = (int)(i - r.task0);
+        const int cnt = r.slot0 < 0 ? r.count : min(chunk, r.count - h * chunk);
+        tasks[i] = ContractTask{r.start + (long long)h * chunk, cnt, sgi >> 3, sgi & 7, r.slot0 < 0 ? -1 : r.slot0 + h};
     }
 }
 
@@ -1990,9 +1988,9 @@ hipError_t launch_transform(hipStream_t s, int mode, int n, const TransformItem 
 
 hipError_t launch_build_tasks(hipStream_t s, int nw, int nseg, int nparts, int chunk, int wchunk, const int *tot,
                               TaskCaps caps, const int *gen_count, int gen_pcap, long long *ofs, ContractTask *tasks,
-                              SliceTask *stasks, ReduceItem *reds, TaskSummary *sum) {
+                              SliceTask *stasks, ReduceItem *reds, TaskSummary *sum, SegRec *segs) {
     hipLaunchKernelGGL(k_build_tasks, dim3(1), dim3(1024), 0, s, nw, nseg, nparts, chunk, wchunk, tot, caps,
-                       gen_count, gen_pcap, ofs, tasks, stasks, reds, sum);
+                       gen_count, gen_pcap, ofs, tasks, stasks, reds, sum, segs);
     return hipGetLastError();
 }
 
