@@ -172,13 +172,11 @@ struct Ctx {
     std::string err;
     DBuf<double> gx, gw;  // 24-point Gauss-Legendre (operator quadrature)
     // apply scratch
-    DBuf<WorkItem> work;
-    DBuf<JobTab> jobs;
     DBuf<TermTab> tabs;
-    DBuf<BoxJob> bjobs;
     DBuf<double> tbmax;
     DBuf<int> tpre, tpre0;
     DBuf<TaskSummary> tsum;
+    DBuf<char> iterbuf;  // per-iteration tables (work items, job tables, box requests)
     DBuf<SegRec> segs;
     DBuf<int> compact_src;
     TaskCaps caps{1 << 20, 1 << 18, 1 << 15, 1 << 13, 1 << 15, 1 << 15, 1 << 13};
@@ -205,6 +203,17 @@ struct Ctx {
     int chunk = 64;        // tuples per FP64 contraction task
     int slice_chunk = 16;  // slices per MFMA task
 };
+
+// Wait for the context's stream by polling: the refinement loop waits once per
+// iteration for a few hundred microseconds of GPU work, and a blocking wait adds its
+// wake-up latency to every iteration.
+static void stream_wait(Ctx &C) {
+    while (true) {
+        const hipError_t e = hipStreamQuery(C.stream);
+        if (e == hipSuccess) return;
+        if (e != hipErrorNotReady) HC(e);
+    }
+}
 
 // asynchronous upload through the context's staging ring
 template <class T>
@@ -234,7 +243,7 @@ static void upload_async(Ctx &C, DBuf<T> &d, const T *src, size_t n) {
 template <class T>
 static void download_sync(Ctx &C, const DBuf<T> &d, T *dst, size_t n) {
     if (n) HC(hipMemcpyAsync(dst, d.p, n * sizeof(T), hipMemcpyDeviceToHost, C.stream));
-    HC(hipStreamSynchronize(C.stream));
+    stream_wait(C);
     C.stage.off = 0;
 }
 
@@ -769,7 +778,7 @@ static void predictor_infer(PredH &Pr, int n, const float *X, float *prob) {
     HC(launch_mlp_forward(C.stream, n, nullptr, Pr.X.p, Pr.mean.p, Pr.sdev.p, Pr.P.p, Pr.prob.p, nullptr, nullptr,
                           nullptr));
     HC(hipMemcpyAsync(prob, Pr.prob.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, C.stream));
-    HC(hipStreamSynchronize(C.stream));
+    stream_wait(C);
     C.stage.off = 0;
 }
 
@@ -1032,12 +1041,23 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             jt[q].pre = C.tpre.p + jt[q].tab0 + q;
             jt[q].pre0 = C.tpre0.p + jt[q].tab0 + q;
         }
-        upload_async(C, C.work, work.data(), work.size());
-        upload_async(C, C.jobs, jt.data(), jt.size());
-        upload_async(C, C.bjobs, bjobs.data(), bjobs.size());
+        // the iteration's tables in one upload: work items | job tables | box requests
+        const size_t o_jobs = (work.size() * sizeof(WorkItem) + 255) & ~(size_t)255;
+        const size_t o_bj = o_jobs + ((jt.size() * sizeof(JobTab) + 255) & ~(size_t)255);
+        const size_t tbytes = o_bj + bjobs.size() * sizeof(BoxJob);
+        {
+            std::vector<char> blob(tbytes);
+            std::memcpy(blob.data(), work.data(), work.size() * sizeof(WorkItem));
+            std::memcpy(blob.data() + o_jobs, jt.data(), jt.size() * sizeof(JobTab));
+            std::memcpy(blob.data() + o_bj, bjobs.data(), bjobs.size() * sizeof(BoxJob));
+            upload_async(C, C.iterbuf, blob.data(), blob.size());
+        }
+        WorkItem *dwork = (WorkItem *)C.iterbuf.p;
+        JobTab *djobs = (JobTab *)(C.iterbuf.p + o_jobs);
+        BoxJob *dbjobs = (BoxJob *)(C.iterbuf.p + o_bj);
         C.tabs.alloc((size_t)std::max(ntabs, 1));
         C.tbmax.alloc((size_t)std::max<size_t>(bjobs.size() * max_terms, 1));
-        HC(launch_job_boxes(s, (int)bjobs.size(), max_terms, C.bjobs.p, C.tabs.p, C.tbmax.p, C.jobs.p, C.tpre.p,
+        HC(launch_job_boxes(s, (int)bjobs.size(), max_terms, dbjobs, C.tabs.p, C.tbmax.p, djobs, C.tpre.p,
                             C.tpre0.p));
         // generated input nodes: marked, then computed for the range the device counter
         // reached (read back with the work-list counts below, not before)
@@ -1052,8 +1072,8 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         }
         GenState gs{C.genKeys.p, C.genJob.p, C.genCount.p, genCap, C.genOverflow.p, C.genPool.p, C.genNorm.p,
                     genPoolCap};
-        HC(launch_gen_mark(s, nw, C.work.p, C.jobs.p, C.tabs.p, gs));
-        HC(launch_gen_compute(s, genDone, genPoolCap, C.jobs.p, gs, M.W.p, K));
+        HC(launch_gen_mark(s, nw, dwork, djobs, C.tabs.p, gs));
+        HC(launch_gen_compute(s, genDone, genPoolCap, djobs, gs, M.W.p, K));
         PT.lap("upload+genmark");
         // Work list and contraction without a host round trip: counts per segment
         // (nseg per output node), a device-built schedule (offsets, tasks, reduce items),
@@ -1072,10 +1092,10 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         C.tsum.alloc(1);
         auto launch_count = [&]() {
             if (!mixed)
-                HC(launch_enum(s, 0, nw, nparts, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, nullptr,
+                HC(launch_enum(s, 0, nw, nparts, dwork, djobs, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, nullptr,
                                nullptr, nullptr));
             else
-                HC(launch_enum_mixed(s, 0, nw, nparts, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p,
+                HC(launch_enum_mixed(s, 0, nw, nparts, dwork, djobs, C.tabs.p, gs, C.thread_off.p, C.gtotal.p,
                                      nullptr, nullptr, nullptr, nullptr));
         };
         auto ensure_caps = [&](const TaskSummary *need) {
@@ -1110,25 +1130,25 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             HC(launch_build_tasks(s, nw, nseg, nparts, C.chunk, 4 * C.slice_chunk, C.gtotal.p, C.caps, C.genCount.p,
                                   genPoolCap, C.gofs.p, C.tasks.p, C.stasks.p, C.reds.p, C.tsum.p, C.segs.p));
             if (!mixed)
-                HC(launch_enum(s, 1, nw, nparts, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, C.gofs.p,
+                HC(launch_enum(s, 1, nw, nparts, dwork, djobs, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, C.gofs.p,
                                C.tuples.p, C.tsum.p));
             else
-                HC(launch_enum_mixed(s, 1, nw, nparts, C.work.p, C.jobs.p, C.tabs.p, gs, C.thread_off.p, C.gtotal.p,
+                HC(launch_enum_mixed(s, 1, nw, nparts, dwork, djobs, C.tabs.p, gs, C.thread_off.p, C.gtotal.p,
                                      C.gofs.p, C.tuples.p, C.slices.p, C.tsum.p));
             const int grid64 = std::max(1, (C.caps.tasks + 3) / 4);
             const int gridred = std::min(4096, std::max(1, C.caps.reds));
             HC(hipEventRecord(C.ev0, s));
-            HC(launch_contract_fp64(s, K, 0, C.tsum.p, grid64, C.tasks.p, C.work.p, C.jobs.p, C.tabs.p, C.tuples.p,
+            HC(launch_contract_fp64(s, K, 0, C.tsum.p, grid64, C.tasks.p, dwork, djobs, C.tabs.p, C.tuples.p,
                                     C.genPool.p, C.partials.p));
             if (mixed) {
                 const int gridmf = std::max(1, C.caps.stasks);
-                HC(launch_contract_mfma(s, K, 0, C.tsum.p, gridmf, C.stasks.p, C.work.p, C.jobs.p, C.tabs.p,
+                HC(launch_contract_mfma(s, K, 0, C.tsum.p, gridmf, C.stasks.p, dwork, djobs, C.tabs.p,
                                         C.slices.p, C.genPool.p, C.nodeslots.p));
             }
-            HC(launch_reduce_partials(s, 0, C.tsum.p, gridred, C.reds.p, C.work.p, C.jobs.p, C.partials.p,
+            HC(launch_reduce_partials(s, 0, C.tsum.p, gridred, C.reds.p, dwork, djobs, C.partials.p,
                                       mixed ? C.nodeslots.p : nullptr, K3));
             HC(hipEventRecord(C.ev1, s));
-            HC(launch_work_norms(s, nw, C.work.p, C.jobs.p, K3, C.wnorms.p));
+            HC(launch_work_norms(s, nw, dwork, djobs, K3, C.wnorms.p));
             // one pinned read-back area: summary | generated count, overflow | counts | norms
             const size_t o_sum = 0, o_gen = 64, o_tot = 128;
             const size_t o_wn = (o_tot + tot.size() * sizeof(int) + 255) & ~(size_t)255;
@@ -1139,7 +1159,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             HC(hipMemcpyAsync(h + o_gen + 4, C.genOverflow.p, sizeof(int), hipMemcpyDeviceToHost, s));
             HC(hipMemcpyAsync(h + o_tot, C.gtotal.p, tot.size() * sizeof(int), hipMemcpyDeviceToHost, s));
             HC(hipMemcpyAsync(h + o_wn, C.wnorms.p, wn.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-            HC(hipStreamSynchronize(s));
+            stream_wait(C);
             C.stage.off = 0;
             std::memcpy(&hsum, h + o_sum, sizeof(TaskSummary));
             std::memcpy(hgen, h + o_gen, 2 * sizeof(int));
@@ -1160,7 +1180,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                 gs.pool = C.genPool.p;
                 gs.norm = C.genNorm.p;
                 gs.pcap = want;
-                HC(launch_gen_compute(s, genPoolCap, want, C.jobs.p, gs, M.W.p, K));
+                HC(launch_gen_compute(s, genPoolCap, want, djobs, gs, M.W.p, K));
                 genPoolCap = want;
                 launch_count();
             }
@@ -1325,7 +1345,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             if (m) HC(hipMemcpyAsync(h + off, J.out->norm.p, m * sizeof(double), hipMemcpyDeviceToHost, s));
             off += m;
         }
-        HC(hipStreamSynchronize(s));
+        stream_wait(C);
         C.stage.off = 0;
         off = 0;
         for (JobH &J : jobs) {
