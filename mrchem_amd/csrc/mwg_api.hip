@@ -713,13 +713,26 @@ static void children_features(const JobH &J, const MraH &M, int p, double tnorm,
     for (int q = 1; q < 8; q++) w2 += pn[q] * pn[q];
     const double wn = std::sqrt(w2), sn = pn[0];
     const int np = M.n0 + T.depth[p];
-    const double thr = std::max(2.0 * 2.220446049250313e-16, J.prec * tnorm * std::pow(2.0, -0.5 * (np + 1)));
+    const double eps2 = 2.0 * 2.220446049250313e-16;
+    const double thr = std::max(eps2, J.prec * tnorm * std::pow(2.0, -0.5 * (np + 1)));
+    const double thr_c = std::max(eps2, J.prec * tnorm * std::pow(2.0, -0.5 * (np + 2)));
     auto lg = [](double v) { return (float)std::max(-20.0, std::min(20.0, std::log10(std::max(v, 1e-300)))); };
-    const float f0 = lg(wn / thr), f1 = lg(wn / tnorm), f2 = lg(sn / tnorm);
+    // decay of the wavelet norm from the grandparent to the parent (0 on the roots' children)
+    double wpp = 0.0;
+    if (T.parent[p] >= 0) {
+        const double *gn = &T.cnorm[8 * (size_t)T.parent[p]];
+        for (int q = 1; q < 8; q++) wpp += gn[q] * gn[q];
+        wpp = std::sqrt(wpp);
+    }
+    const float f0 = lg(wn / thr), f1 = lg(sn / thr_c), f2 = wpp > 0.0 ? lg(wn / wpp) : 0.0f;
+    const float f3 = lg(wn / std::max(std::fabs(sn), 1e-300));
     const int c0 = T.child[p];
     const int d = T.depth[c0];
     const double inn = std::sqrt(std::max(J.in->sqnorm, 1e-300));
-    const float f8 = (float)std::log10(J.op->mu + 1e-3), f9 = (float)std::log10(J.prec);
+    // input norms mapped to the output's scale, against the child's split threshold
+    const double in_to_out = tnorm / inn;
+    const float f8 = (float)std::log10(J.op->mu + 1e-3);
+    const float f9 = (float)std::log10(M.width(0) * std::ldexp(1.0, -d) / M.boxes[0]);
     const int bx = 2 * T.lr[3 * p] - 1, by = 2 * T.lr[3 * p + 1] - 1, bz = 2 * T.lr[3 * p + 2] - 1;
     int idx[64];
     double wv[64];
@@ -741,10 +754,10 @@ static void children_features(const JobH &J, const MraH &M, int p, double tnorm,
         fc[0] = f0;
         fc[1] = f1;
         fc[2] = f2;
-        fc[3] = (float)(M.n0 + d);
+        fc[3] = f3;
         const int i0 = idx[k0];
         fc[4] = i0 >= 0 ? 1.0f : 0.0f;
-        fc[5] = i0 >= 0 ? lg(wv[k0] / inn) : -20.0f;
+        fc[5] = i0 >= 0 ? lg(wv[k0] * in_to_out / thr_c) : -20.0f;
         fc[6] = (i0 >= 0 && J.in->child[i0] >= 0) ? 1.0f : 0.0f;
         double nb = 0.0;
         for (int dz = -1; dz <= 1; dz++)
@@ -754,7 +767,7 @@ static void children_features(const JobH &J, const MraH &M, int p, double tnorm,
                     const int k = (cx + dx) + 4 * (cy + dy) + 16 * (cz + dz);
                     if (idx[k] >= 0) nb = std::max(nb, wv[k]);
                 }
-        fc[7] = nb > 0.0 ? lg(nb / inn) : -20.0f;
+        fc[7] = nb > 0.0 ? lg(nb * in_to_out / thr_c) : -20.0f;
         fc[8] = f8;
         fc[9] = f9;
     }
@@ -2068,7 +2081,8 @@ int mwg_predictor_train(mwg_predictor pred, int64_t n, const float *X, const flo
         for (int f = 0; f < NF; f++) mean[f] /= (float)n;
         for (int64_t i = 0; i < n; i++)
             for (int f = 0; f < NF; f++) sdev[f] += (X[i * NF + f] - mean[f]) * (X[i * NF + f] - mean[f]);
-        for (int f = 0; f < NF; f++) sdev[f] = std::max(std::sqrt(sdev[f] / (float)n), 1e-3f);
+        // floor: a feature (nearly) constant in the training set must not blow up on new data
+        for (int f = 0; f < NF; f++) sdev[f] = std::max(std::sqrt(sdev[f] / (float)n), 0.1f);
         pred->mean.upload(mean.data(), NF, s);
         pred->sdev.upload(sdev.data(), NF, s);
         pred->X.upload(X, (size_t)n * NF, s);
