@@ -113,7 +113,7 @@ def train_predictor(G, configs, mra, args):
     G.trace_enable(False)
     pred = G.Predictor(seed=0)
     pred.train(X, y, epochs=150, batch=512, lr=3e-3, seed=0)
-    pred.install(args.pred_threshold)
+    pred.install(args.pred_threshold if args.pred_threshold > 0 else 0.5)
     return pred, len(y), time.perf_counter() - t0
 
 
@@ -226,13 +226,25 @@ def predictor_main(args, G, configs):
     best = None
     for name, ops in cases.items():
         res0 = run_variant(G, mra, trees, ops, prec, G.MWG_APPLY_FP64, args.steps, args.warmup)
-        res1 = run_variant(G, mra, trees, ops, prec, G.MWG_APPLY_FP64 | G.MWG_APPLY_PREDICT, args.steps, args.warmup)
+        cand = {}
+        for thr in ([args.pred_threshold] if args.pred_threshold > 0 else [0.5, 0.7, 0.85, 0.95]):
+            pred.install(thr)
+            cand[thr] = run_variant(G, mra, trees, ops, prec, G.MWG_APPLY_FP64 | G.MWG_APPLY_PREDICT, args.steps,
+                                    args.warmup)
+            log(f"[bench] {name} threshold {thr}: iters {cand[thr]['iters']}, "
+                f"{cand[thr]['ms_per_step']:.1f} ms/step, speculative {cand[thr]['predicted']}, "
+                f"discarded {cand[thr]['wasted']}")
+        pthr = max(cand, key=lambda t: cand[t]["nodes_per_s"])
+        res1 = cand[pthr]
         errs = [_l2diff(a, b) / math.sqrt(b.getSquareNorm()) for a, b in zip(res1["outs"], res0["outs"])]
         report[name] = dict(plain_iters=res0["iters"], predict_iters=res1["iters"],
                             plain_nodes_per_s=res0["nodes_per_s"], predict_nodes_per_s=res1["nodes_per_s"],
                             plain_ms=res0["ms_per_step"], predict_ms=res1["ms_per_step"],
                             plain_nodes=res0["nodes_per_step"], predict_nodes=res1["nodes_per_step"],
-                            speculative=res1["predicted"], wasted=res1["wasted"],
+                            speculative=res1["predicted"], wasted=res1["wasted"], threshold=pthr,
+                            threshold_sweep={str(t): dict(ms=r["ms_per_step"], iters=r["iters"],
+                                                          speculative=r["predicted"], discarded=r["wasted"])
+                                             for t, r in cand.items()},
                             rel_l2_err_max=max(errs))
         log(f"[bench] {name}: iters {res0['iters']} -> {res1['iters']}, {res0['ms_per_step']:.1f} -> "
             f"{res1['ms_per_step']:.1f} ms/step, nodes {res0['nodes_per_step']:.0f} -> {res1['nodes_per_step']:.0f}, "
@@ -263,7 +275,8 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-predict", action="store_true", help="skip the learned-predictor variant")
-    ap.add_argument("--pred-threshold", type=float, default=0.5)
+    ap.add_argument("--pred-threshold", type=float, default=0.0,
+                    help="predictor decision threshold (0: the fastest of 0.5, 0.7, 0.85, 0.95)")
     ap.add_argument("--workload", default="helmholtz", choices=["helmholtz", "exchange", "predictor"])
     ap.add_argument("--no-autotune", action="store_true", help="keep the default contraction task sizes")
     ap.add_argument("--copy-probe", action="store_true",
@@ -317,13 +330,26 @@ def main():
     if not args.no_predict:
         pred, ntrain, t_train = train_predictor(G, configs, mra, args)
         flags = (G.MWG_APPLY_MIXED if args.k == 7 else G.MWG_APPLY_FP64) | G.MWG_APPLY_PREDICT
-        resP = run_variant(G, mra, trees, ops, args.prec, flags, args.steps, args.warmup)
+        # decision threshold of the predictor: the given one, or the fastest of a few
+        sweep = {}
+        for thr in ([args.pred_threshold] if args.pred_threshold > 0 else [0.5, 0.7, 0.85, 0.95]):
+            pred.install(thr)
+            r = run_variant(G, mra, trees, ops, args.prec, flags, args.steps, args.warmup)
+            sweep[thr] = r
+            log(f"[bench] mixed+predictor (threshold {thr}): {r['ms_per_step']:.2f} ms/step, iters {r['iters']}, "
+                f"speculative {r['predicted']}, discarded {r['wasted']}")
+        pthr = max(sweep, key=lambda t: sweep[t]["nodes_per_s"])
+        resP = sweep[pthr]
         errP = max(_l2diff(a, b) / math.sqrt(b.getSquareNorm()) for a, b in zip(resP["outs"], res64["outs"]))
         resP["rel_err_vs_fp64"] = errP
         resP["train_samples"] = ntrain
         resP["train_s"] = t_train
+        resP["threshold"] = pthr
+        resP["threshold_sweep"] = {str(t): dict(ms_per_step=r["ms_per_step"], iters=r["iters"],
+                                                speculative=r["predicted"], discarded=r["wasted"])
+                                   for t, r in sweep.items()}
         log(f"[bench] mixed+predictor: {resP['ms_per_step']:.2f} ms/step, {resP['nodes_per_s']:.1f} nodes/s, "
-            f"iters {resP['iters']} (fp64 {res64['iters']}), rel L2 err vs fp64 {errP:.2e}, "
+            f"iters {resP['iters']} (fp64 {res64['iters']}), threshold {pthr}, rel L2 err vs fp64 {errP:.2e}, "
             f"trained on {ntrain} trace samples in {t_train:.1f} s")
         if resP["nodes_per_s"] > best["nodes_per_s"] and errP < args.prec:
             best, dtype = resP, ("f32" if args.k == 7 else "f64")
@@ -380,6 +406,8 @@ def main():
             "predict_speculative_nodes": resP["predicted"] if resP else None,
             "predict_speculative_wasted": resP["wasted"] if resP else None,
             "predict_train_samples": resP["train_samples"] if resP else None,
+            "predict_threshold": resP["threshold"] if resP else None,
+            "predict_threshold_sweep": resP["threshold_sweep"] if resP else None,
             "operator_build_s": t_oper, "projection_s": t_proj,
             "contraction_ms_per_step": 1e3 * best["t_contract_per_step"],
             "tuning": tuning or None,
