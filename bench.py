@@ -264,6 +264,78 @@ def predictor_main(args, G, configs):
     print(json.dumps(line), flush=True)
 
 
+def scf_main(args, G, configs):
+    """SCF operator step: 32 orbitals on 16 nuclei (10 bohr cube), k=9, prec 1e-6; one step =
+    the 32 Helmholtz applies (mu_i = sqrt(-2 eps_i), eps_i ~ U[-2, -0.2]) and the 528 pair
+    Poissons phi_i phi_j (i <= j) as one batch.  The FP32 MFMA path covers k=7 only, so the
+    step runs in FP64, plain and with the learned predictor; the CPU oracle is timed on a
+    sample of the same applies."""
+    k = 9
+    mra = G.mra_from(configs.BOX20, k, "L", max_depth=16)
+    orbs, nuc, _ = configs.orbitals(args.seed, n_orb=32, n_nuc=16, nuc_box=10.0)
+    rng = configs.SplitMix64(args.seed + 1000)
+    eps = [-rng.uniform(0.2, 2.0) for _ in range(32)]
+    mus = [math.sqrt(-2.0 * e) for e in eps]
+    t0 = time.perf_counter()
+    trees = [G.project(args.prec, G.FunctionTree(mra), o) for o in orbs]
+    pairs = [(i, j) for i in range(32) for j in range(i, 32)]
+    rho = [G.multiply(args.prec, G.FunctionTree(mra), 1.0, trees[i], trees[j]) for i, j in pairs]
+    t_in = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    P = G.PoissonOperator(mra, args.prec)
+    Hs = [G.HelmholtzOperator(mra, mu, args.prec) for mu in mus]
+    t_op = time.perf_counter() - t0
+    inps = trees + rho
+    ops = Hs + [P] * len(rho)
+    log(f"[bench] scf step: {len(inps)} applies, input nodes {sum(t.getNNodes() for t in inps)}, "
+        f"inputs {t_in:.1f} s, operators {t_op:.1f} s")
+    res64 = run_variant(G, mra, inps, ops, args.prec, G.MWG_APPLY_FP64, args.steps, args.warmup)
+    log(f"[bench] fp64: {res64['ms_per_step']:.0f} ms/step, {res64['nodes_per_s']:.0f} nodes/s, "
+        f"iters {res64['iters']}")
+    best = res64
+    resP = None
+    if not args.no_predict:
+        pred, ntrain, t_train = train_predictor(G, configs, G.mra_from(configs.BOX20, 7, "L", max_depth=16),
+                                                argparse.Namespace(**dict(vars(args), prec=1e-6, orbitals=5)))
+        pred.install(args.pred_threshold if args.pred_threshold > 0 else 0.85)
+        resP = run_variant(G, mra, inps, ops, args.prec, G.MWG_APPLY_FP64 | G.MWG_APPLY_PREDICT, args.steps,
+                           args.warmup)
+        errP = max(_l2diff(a, b) / math.sqrt(b.getSquareNorm()) for a, b in zip(resP["outs"], res64["outs"]))
+        resP["rel_err_vs_fp64"] = errP
+        log(f"[bench] fp64+predictor: {resP['ms_per_step']:.0f} ms/step, {resP['nodes_per_s']:.0f} nodes/s, "
+            f"iters {resP['iters']}, speculative {resP['predicted']}, discarded {resP['wasted']}, "
+            f"max rel L2 err vs fp64 {errP:.2e}")
+        if resP["nodes_per_s"] > best["nodes_per_s"] and errP < args.prec:
+            best = resP
+    cpu = None
+    if not args.no_cpu:
+        from oracle import mwref as M
+        omra = M.MRA(**configs.BOX20, k=k, basis="L", max_depth=16)
+        nodes, t_apply = 0, 0.0
+        for i in range(2):
+            f = M.Tree(omra).project_gauss(args.prec, orbs[i])
+            st = M.apply(args.prec, M.Tree(omra), M.helmholtz(omra, mus[i], args.prec), f)
+            nodes += st["nodes_out"]
+            t_apply += st["t_total"]
+        cpu = dict(value=nodes / t_apply, unit="output nodes/s", cores=M.get_threads(), kind="port",
+                   sample=f"FP64 OpenMP oracle, 2 of the 32 Helmholtz applies of the step, {t_apply:.1f} s")
+    line = {
+        "metric": "output nodes/s for Poisson/Helmholtz apply at prec 1e-6 vs plain FP64 GPU",
+        "value": best["nodes_per_s"], "unit": "output nodes/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": best["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded Gaussian orbitals)",
+        "config": {"workload": f"SCF operator step: 32 Helmholtz + 528 pair Poissons, k={k}, prec={args.prec}",
+                   "fp64_nodes_per_s": res64["nodes_per_s"], "fp64_ms_per_step": res64["ms_per_step"],
+                   "fp64_iters": res64["iters"],
+                   "predict_nodes_per_s": resP["nodes_per_s"] if resP else None,
+                   "predict_iters": resP["iters"] if resP else None,
+                   "predict_rel_l2_err_vs_fp64": resP["rel_err_vs_fp64"] if resP else None,
+                   "output_nodes_per_step": best["nodes_per_step"], "inputs_s": t_in, "operators_s": t_op},
+        "roofline": None, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -277,7 +349,7 @@ def main():
     ap.add_argument("--no-predict", action="store_true", help="skip the learned-predictor variant")
     ap.add_argument("--pred-threshold", type=float, default=0.0,
                     help="predictor decision threshold (0: the fastest of 0.5, 0.7, 0.85, 0.95)")
-    ap.add_argument("--workload", default="helmholtz", choices=["helmholtz", "exchange", "predictor"])
+    ap.add_argument("--workload", default="helmholtz", choices=["helmholtz", "exchange", "predictor", "scf"])
     ap.add_argument("--no-autotune", action="store_true", help="keep the default contraction task sizes")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after timing, rescale one output tree in place (HBM counter calibration)")
@@ -294,6 +366,8 @@ def main():
         return exchange_main(args, G, configs)
     if args.workload == "predictor":
         return predictor_main(args, G, configs)
+    if args.workload == "scf":
+        return scf_main(args, G, configs)
 
     log(f"[bench] building inputs: {args.orbitals} orbitals, k={args.k}, prec={args.prec}")
     mra, orbs, mus, trees, ops, t_proj, t_oper = build_inputs(G, configs, args.k, args.prec, args.orbitals,

@@ -178,7 +178,6 @@ struct Ctx {
     DBuf<TaskSummary> tsum;
     DBuf<char> iterbuf;  // per-iteration tables (work items, job tables, box requests)
     DBuf<SegRec> segs;
-    DBuf<int> compact_src;
     TaskCaps caps{1 << 20, 1 << 18, 1 << 15, 1 << 13, 1 << 15, 1 << 15, 1 << 13};
     DBuf<int> thread_off, gtotal;
     DBuf<long long> gofs;
@@ -779,20 +778,23 @@ struct PredH {
     Ctx *ctx = nullptr;
     DBuf<float> P, m, v, G, mean, sdev;
     DBuf<float> X, Y, prob, Xn, H1, H2, dZ1, dZ2, dZ3;
+    DBuf<float> Xq;  // feature rows of inference queries
     DBuf<int> idx;
     int adam_t = 0;
 };
 
+// probabilities for n feature rows (pinned staging both ways, one polling wait)
 static void predictor_infer(PredH &Pr, int n, const float *X, float *prob) {
     Ctx &C = *Pr.ctx;
-    Pr.X.alloc((size_t)n * MLP_FEATURES);
-    HC(hipMemcpyAsync(Pr.X.p, X, (size_t)n * MLP_FEATURES * sizeof(float), hipMemcpyHostToDevice, C.stream));
+    upload_async(C, Pr.Xq, X, (size_t)n * MLP_FEATURES);
     Pr.prob.alloc(n);
-    HC(launch_mlp_forward(C.stream, n, nullptr, Pr.X.p, Pr.mean.p, Pr.sdev.p, Pr.P.p, Pr.prob.p, nullptr, nullptr,
+    HC(launch_mlp_forward(C.stream, n, nullptr, Pr.Xq.p, Pr.mean.p, Pr.sdev.p, Pr.P.p, Pr.prob.p, nullptr, nullptr,
                           nullptr));
-    HC(hipMemcpyAsync(prob, Pr.prob.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, C.stream));
+    C.rb.reserve((size_t)n * sizeof(float));
+    HC(hipMemcpyAsync(C.rb.h, Pr.prob.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, C.stream));
     stream_wait(C);
     C.stage.off = 0;
+    std::memcpy(prob, C.rb.h, (size_t)n * sizeof(float));
 }
 
 // forward + backward over samples idx[0..n) of the resident set (idx null: all); gradient in Pr.G
@@ -830,56 +832,60 @@ static double predictor_loss(PredH &Pr, int n, const float *y) {
 
 // ------------------------------------------------------------------ compaction
 
-__global__ void k_gather_nodes(int n, const int *src, const double *in, double *out, int per) {
-    const int i = blockIdx.x;
-    if (i >= n) return;
-    const double *a = in + (size_t)src[i] * per;
-    double *b = out + (size_t)i * per;
-    for (int e = threadIdx.x; e < per; e += blockDim.x) b[e] = a[e];
-}
 
 // removes the nodes with alive == 0 (whole sibling groups already unlinked from their parents)
+// Remove discarded speculative children.  Below the roots the node array is a sequence
+// of sibling groups of 8 (one per split), and a discarded group is a group of leaves, so
+// the array is compacted by moving the last live group into each hole: O(discarded)
+// host work and one 8-node device copy per move.
 static void compact_tree(TreeH &T, const std::vector<char> &alive) {
     Ctx &C = *T.mra->ctx;
-    const int n = T.n();
-    std::vector<int> newid(n, -1), src;
-    for (int i = 0; i < n; i++)
-        if (alive[i]) {
-            newid[i] = (int)src.size();
-            src.push_back(i);
+    const size_t nb = (size_t)8 * T.K3();  // doubles per node
+    const int nroots = (int)T.roots.size();
+    int n = T.n();
+    auto key_of = [&](int i) { return node_key(T.depth[i], T.lr[3 * i], T.lr[3 * i + 1], T.lr[3 * i + 2]); };
+    std::vector<int> holes;
+    for (int i = nroots; i < n; i += 8)
+        if (!alive[i]) {
+            holes.push_back(i);
+            for (int a = 0; a < 8; a++) T.index.erase(key_of(i + a));
         }
-    const int m = (int)src.size();
-    std::vector<int> depth(m), parent(m), child(m), lr(3 * (size_t)m);
-    std::vector<double> cn(8 * (size_t)m);
-    for (int k = 0; k < m; k++) {
-        const int i = src[k];
-        depth[k] = T.depth[i];
-        parent[k] = T.parent[i] >= 0 ? newid[T.parent[i]] : -1;
-        child[k] = T.child[i] >= 0 ? newid[T.child[i]] : -1;
-        for (int q = 0; q < 3; q++) lr[3 * k + q] = T.lr[3 * i + q];
-        std::memcpy(&cn[8 * (size_t)k], &T.cnorm[8 * (size_t)i], 8 * sizeof(double));
+    auto move_group = [&](int from, int to) {
+        const int p = T.parent[from];
+        if (p >= 0) T.child[p] = to;
+        for (int a = 0; a < 8; a++) {
+            const int f = from + a, t = to + a;
+            T.depth[t] = T.depth[f];
+            T.parent[t] = T.parent[f];
+            T.child[t] = T.child[f];
+            for (int q = 0; q < 3; q++) T.lr[3 * t + q] = T.lr[3 * f + q];
+            std::memcpy(&T.cnorm[8 * (size_t)t], &T.cnorm[8 * (size_t)f], 8 * sizeof(double));
+            if (T.child[t] >= 0)
+                for (int b = 0; b < 8; b++) T.parent[T.child[t] + b] = t;
+            T.index[key_of(t)] = t;
+        }
+        HC(hipMemcpyAsync(T.coef.p + (size_t)to * nb, T.coef.p + (size_t)from * nb, 8 * nb * sizeof(double),
+                          hipMemcpyDeviceToDevice, C.stream));
+        HC(hipMemcpyAsync(T.norm.p + (size_t)to * 8, T.norm.p + (size_t)from * 8, 64 * sizeof(double),
+                          hipMemcpyDeviceToDevice, C.stream));
+    };
+    std::vector<char> dead_at((size_t)n, 0);
+    for (int h : holes) dead_at[h] = 1;
+    size_t hi = 0;
+    while (hi < holes.size()) {
+        while (n > nroots && dead_at[n - 8]) n -= 8;  // discarded groups at the end just go
+        const int h = holes[hi];
+        if (h >= n) break;
+        move_group(n - 8, h);
+        dead_at[h] = 0;
+        n -= 8;
+        hi++;
     }
-    upload_async(C, C.compact_src, src.data(), src.size());
-    size_t gc = 0, gn = 0;
-    double *nc = (double *)C.cache.get((size_t)m * 8 * T.K3() * sizeof(double), gc);
-    double *nn = (double *)C.cache.get((size_t)m * 8 * sizeof(double), gn);
-    hipLaunchKernelGGL(k_gather_nodes, dim3(m), dim3(256), 0, C.stream, m, C.compact_src.p, T.coef.p, nc,
-                       8 * T.K3());
-    hipLaunchKernelGGL(k_gather_nodes, dim3(m), dim3(8), 0, C.stream, m, C.compact_src.p, T.norm.p, nn, 8);
-    HC(hipGetLastError());
-    const size_t oc = T.coef.n * sizeof(double), on = T.norm.n * sizeof(double);
-    C.cache.put(T.coef.release(), oc);
-    C.cache.put(T.norm.release(), on);
-    T.coef.adopt(nc, gc / sizeof(double));
-    T.norm.adopt(nn, gn / sizeof(double));
-    T.depth.swap(depth);
-    T.parent.swap(parent);
-    T.child.swap(child);
-    T.lr.swap(lr);
-    T.cnorm.swap(cn);
-    T.index.clear();
-    for (int k = 0; k < m; k++) T.index[node_key(T.depth[k], T.lr[3 * k], T.lr[3 * k + 1], T.lr[3 * k + 2])] = k;
-    for (int &r : T.roots) r = newid[r];
+    T.depth.resize(n);
+    T.parent.resize(n);
+    T.child.resize(n);
+    T.lr.resize(3 * (size_t)n);
+    T.cnorm.resize(8 * (size_t)n);
     T.hvalid = false;
 }
 
