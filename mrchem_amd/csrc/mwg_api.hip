@@ -20,6 +20,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -43,6 +44,18 @@ struct Error : std::runtime_error {
         if (e_ != hipSuccess) throw Error(MWG_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// device allocation with one retry after the block caches have been emptied
+void flush_device_caches();
+inline hipError_t dev_malloc(void **p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        flush_device_caches();
+        e = hipMalloc(p, bytes);
+    }
+    return e;
+}
+
 template <class T>
 struct DBuf {
     T *p = nullptr;
@@ -58,14 +71,14 @@ struct DBuf {
         if (p) HC(hipFree(p));
         p = nullptr;
         n = 0;
-        HC(hipMalloc(&p, std::max<size_t>(m, 1) * sizeof(T)));
+        HC(dev_malloc((void **)&p, std::max<size_t>(m, 1) * sizeof(T)));
         n = std::max<size_t>(m, 1);
     }
     void grow_keep(size_t m, hipStream_t s) {
         if (m <= n && p) return;
         size_t nn = std::max<size_t>(m, 2 * n);
         T *q = nullptr;
-        HC(hipMalloc(&q, std::max<size_t>(nn, 1) * sizeof(T)));
+        HC(dev_malloc((void **)&q, std::max<size_t>(nn, 1) * sizeof(T)));
         if (p) {
             HC(hipMemcpyAsync(q, p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
             HC(hipStreamSynchronize(s));
@@ -112,10 +125,23 @@ struct Stage {
 // Device blocks released by trees (coefficients, norms), kept for reuse by later trees
 // of the same context.  All work of a context is ordered on its one stream, so a block
 // handed out again is only touched after every earlier use of it.
+struct DevCache;
+static std::mutex g_cache_mu;
+static std::vector<DevCache *> g_caches;
+
 struct DevCache {
     std::multimap<size_t, void *> blocks;  // bytes -> block
     size_t bytes = 0;
-    static constexpr size_t LIMIT = (size_t)16 << 30;
+    static constexpr size_t LIMIT = (size_t)8 << 30;
+    DevCache() {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        g_caches.push_back(this);
+    }
+    void flush() {
+        for (auto &b : blocks) (void)hipFree(b.second);
+        blocks.clear();
+        bytes = 0;
+    }
     void *get(size_t want, size_t &got) {
         auto it = blocks.lower_bound(want);
         if (it != blocks.end() && it->first <= 2 * want + ((size_t)1 << 20)) {
@@ -126,7 +152,7 @@ struct DevCache {
             return q;
         }
         void *q = nullptr;
-        HC(hipMalloc(&q, std::max<size_t>(want, 8)));
+        HC(dev_malloc(&q, std::max<size_t>(want, 8)));
         got = std::max<size_t>(want, 8);
         return q;
     }
@@ -142,9 +168,21 @@ struct DevCache {
         }
     }
     ~DevCache() {
-        for (auto &b : blocks) (void)hipFree(b.second);
+        {
+            std::lock_guard<std::mutex> lk(g_cache_mu);
+            g_caches.erase(std::remove(g_caches.begin(), g_caches.end(), this), g_caches.end());
+        }
+        flush();
     }
 };
+
+// cached blocks are only reused on their own context's stream; emptying a cache first
+// waits for the device so that no queued work still uses a block being freed
+void flush_device_caches() {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    (void)hipDeviceSynchronize();
+    for (DevCache *c : g_caches) c->flush();
+}
 
 // pinned host area for device-to-host reads (plain DMA, no staging through pageable memory)
 struct Readback {
