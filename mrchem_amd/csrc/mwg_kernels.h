@@ -186,6 +186,7 @@ struct SegRec {
     int count;
     int task0;
     int slot0;
+    int pad_stask0;  // on component 0 of a node: its first slice task (mixed)
 };
 hipError_t launch_build_tasks(hipStream_t s, int nw, int nseg, int nparts, int chunk, int wchunk, const int *tot,
                               TaskCaps caps, const int *gen_count, int gen_pcap, long long *ofs, ContractTask *tasks,

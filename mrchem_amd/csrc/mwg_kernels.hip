@@ -1059,16 +1059,22 @@ __device__ inline long long block_excl_scan_1024(long long v, long long *wsum, l
 
 // One workgroup.  Phase 1: per thread (a range of output nodes) the totals of tuples,
 // slices, FP64 tasks, slots, slice tasks and reduce items; block scans give each
-// thread's offsets.  Phase 2: per segment (node, component) its record (tuple offset,
-// count, first task, first slot) and the per-part offsets the fill kernels use; per node
-// the slice segment and its slice tasks; reduce items.  Phase 3: the FP64 tasks, spread
-// evenly over the threads (each thread walks the segments from its first task on).
+// thread's offsets.  Phase 2: per segment (node, component) the per-part offsets the fill
+// kernels use, reduce items, and the tasks of segments with few tasks; segments (and
+// nodes' slice runs) with many tasks go to a list in LDS.  Phase 3: all threads write the
+// tasks of the listed segments.  Every task has a fixed position, so the schedule does
+// not depend on the order of the list.
+constexpr int BT_FEW = 16, BT_LIST = 2048;
+
 __global__ void __launch_bounds__(1024) k_build_tasks(int nw, int nseg, int nparts, int chunk, int wchunk,
                                                       const int *tot, TaskCaps caps, const int *gen_count,
                                                       int gen_pcap, long long *ofs, ContractTask *tasks,
                                                       SliceTask *stasks, ReduceItem *reds, TaskSummary *sum,
                                                       SegRec *segs) {
     __shared__ long long wsum[17];
+    __shared__ int big[BT_LIST];     // segment ids (>= 0) or -(node + 1) for slice runs
+    __shared__ long long bigs[BT_LIST];
+    __shared__ int nbig;
     auto segc = [&](int sg) {
         int c = 0;
         for (int p = 0; p < nparts; p++) c += tot[(size_t)sg * nparts + p];
@@ -1079,6 +1085,7 @@ __global__ void __launch_bounds__(1024) k_build_tasks(int nw, int nseg, int npar
     // every output component is written; mixed: the reduce item writes the component)
     auto ntasks_of = [&](int c) { return mixed ? (c + chunk - 1) / chunk : (c <= chunk ? 1 : (c + chunk - 1) / chunk); };
     auto nslots_of = [&](int c) { return mixed ? (c + chunk - 1) / chunk : (c <= chunk ? 0 : (c + chunk - 1) / chunk); };
+    if (threadIdx.x == 0) nbig = 0;
     const int g0 = (int)((long long)nw * threadIdx.x / 1024), g1 = (int)((long long)nw * (threadIdx.x + 1) / 1024);
     long long tup = 0, sl = 0, nt = 0, nst = 0, slots = 0, nred = 0;
     for (int g = g0; g < g1; g++) {
@@ -1119,6 +1126,11 @@ __global__ void __launch_bounds__(1024) k_build_tasks(int nw, int nseg, int npar
         sum->overflow = overflow;
     }
     if (overflow) return;
+    auto emit_task = [&](const SegRec &r, int sgi, int h) {
+        const int cnt = r.slot0 < 0 ? r.count : min(chunk, r.count - h * chunk);
+        tasks[r.task0 + h] = ContractTask{r.start + (long long)h * chunk, cnt, sgi >> 3, sgi & 7,
+                                          r.slot0 < 0 ? -1 : r.slot0 + h};
+    };
     // phase 2
     for (int g = g0; g < g1; g++) {
         int nn0 = (int)bnst, nns = 0;
@@ -1130,11 +1142,20 @@ __global__ void __launch_bounds__(1024) k_build_tasks(int nw, int nseg, int npar
                 o += tot[(size_t)sg * nparts + p];
             }
             const int cs = (int)(o - bsl);
-            for (int h = 0; h * wchunk < cs; h++) {
-                stasks[bnst] = SliceTask{bsl + (long long)h * wchunk, min(wchunk, cs - h * wchunk), g, (int)bnst};
-                bnst++;
+            nns = (cs + wchunk - 1) / wchunk;
+            int slot_ = -1;
+            if (nns > BT_FEW) {
+                slot_ = atomicAdd(&nbig, 1);
+                if (slot_ < BT_LIST) {
+                    big[slot_] = -(g + 1);
+                    bigs[slot_] = bsl;
+                }
             }
-            nns = (int)bnst - nn0;
+            if (slot_ < 0 || slot_ >= BT_LIST)
+                for (int h = 0; h < nns; h++)
+                    stasks[nn0 + h] = SliceTask{bsl + (long long)h * wchunk, min(wchunk, cs - h * wchunk), g, nn0 + h};
+            segs[(size_t)g * 8].pad_stask0 = nn0;  // first slice task of node g
+            bnst += nns;
             bsl += cs;
         }
         for (int q = 0; q < 8; q++) {
@@ -1146,36 +1167,44 @@ __global__ void __launch_bounds__(1024) k_build_tasks(int nw, int nseg, int npar
             }
             const int c = (int)(o - btup);
             const int k = ntasks_of(c), ns = nslots_of(c);
-            segs[(size_t)g * 8 + q] = SegRec{btup, c, (int)bnt, ns ? (int)bslots : -1};
+            const int sgi = g * 8 + q;
+            const SegRec r{btup, c, (int)bnt, ns ? (int)bslots : -1, 0};
+            segs[sgi].start = r.start;
+            segs[sgi].count = r.count;
+            segs[sgi].task0 = r.task0;
+            segs[sgi].slot0 = r.slot0;
             if (mixed) reds[(size_t)g * 8 + q] = ReduceItem{g, q, (int)bslots, ns, nn0, nns};
             else if (ns) reds[bnred++] = ReduceItem{g, q, (int)bslots, ns, 0, 0};
+            int slot_ = -1;
+            if (k > BT_FEW) {
+                slot_ = atomicAdd(&nbig, 1);
+                if (slot_ < BT_LIST) big[slot_] = sgi;
+            }
+            if (slot_ < 0 || slot_ >= BT_LIST)
+                for (int h = 0; h < k; h++) emit_task(r, sgi, h);
             bnt += k;
             bslots += ns;
             btup = o;
         }
     }
     __syncthreads();
-    // phase 3: tasks [i0, i1) of this thread; find the segment of i0 by binary search
-    // over the segment records (task starts are non-decreasing), then walk forward
-    const long long i0 = Tnt * threadIdx.x / 1024, i1 = Tnt * (threadIdx.x + 1) / 1024;
-    if (i0 >= i1) return;
-    int lo = 0, hi = 8 * nw - 1;
-    while (lo < hi) {  // last segment whose first task is <= i0 and that has tasks
-        const int mid = (lo + hi + 1) >> 1;
-        if (segs[mid].task0 <= i0) lo = mid;
-        else hi = mid - 1;
-    }
-    int sgi = lo;
-    for (long long i = i0; i < i1; i++) {
-        while (true) {
-            const SegRec &r = segs[sgi];
-            if (i < r.task0 + ntasks_of(r.count)) break;
-            sgi++;
+    // phase 3: the listed segments, all threads
+    const int nb = min(nbig, BT_LIST);
+    for (int b = 0; b < nb; b++) {
+        const int id = big[b];
+        if (id >= 0) {
+            const SegRec r = segs[id];
+            const int k = ntasks_of(r.count);
+            for (int h = threadIdx.x; h < k; h += blockDim.x) emit_task(r, id, h);
+        } else {
+            const int g = -id - 1;
+            const int nn0 = segs[(size_t)g * 8].pad_stask0;
+            const long long s0 = bigs[b];
+            const int cs = segc(g * 9 + 8);
+            const int nns = (cs + wchunk - 1) / wchunk;
+            for (int h = threadIdx.x; h < nns; h += blockDim.x)
+                stasks[nn0 + h] = SliceTask{s0 + (long long)h * wchunk, min(wchunk, cs - h * wchunk), g, nn0 + h};
         }
-        const SegRec r = segs[sgi];
-        const int h = (int)(i - r.task0);
-        const int cnt = r.slot0 < 0 ? r.count : min(chunk, r.count - h * chunk);
-        tasks[i] = ContractTask{r.start + (long long)h * chunk, cnt, sgi >> 3, sgi & 7, r.slot0 < 0 ? -1 : r.slot0 + h};
     }
 }
 
@@ -1527,8 +1556,12 @@ __device__ inline void mfma8_task(const SliceTask task, const WorkItem *work, co
         const int gtl = gxm | ((q >> 1) << 2), gth = gtl | 2;
         const double wlow = ((cur.mask >> gtl) & 1) ? cur.w : 0.0;
         const double whigh = ((cur.mask >> gth) & 1) ? cur.w : 0.0;
+        // tiles t3 < 8 carry gy = 0 (gt in {0,1,4,5}: mask bits 0x33), t3 >= 8 gy = 1 (0xCC);
+        // a half without a surviving component is skipped (the mask is uniform per wave)
+        const bool need_lo = (cur.mask & 0x33) != 0, need_hi = (cur.mask & 0xCC) != 0;
 #pragma unroll
         for (int g4 = 0; g4 < 4; g4++) {
+            if (g4 < 2 ? !need_lo : !need_hi) continue;
             float b[4][2];
 #pragma unroll
             for (int t = 0; t < 4; t++)
