@@ -214,8 +214,11 @@ struct Ctx {
     DBuf<double> tbmax;
     DBuf<int> tpre, tpre0;
     DBuf<TaskSummary> tsum;
+    DBuf<float> cfeat, cprob;  // predictor features and probabilities of the work nodes' children
     DBuf<char> iterbuf;  // per-iteration tables (work items, job tables, box requests)
     DBuf<SegRec> segs;
+    DBuf<int> segcnt;        // per work-list segment: count over all parts
+    DBuf<long long> gslice;  // per output node: first slice
     TaskCaps caps{1 << 20, 1 << 18, 1 << 15, 1 << 13, 1 << 15, 1 << 15, 1 << 13};
     DBuf<int> thread_off, gtotal;
     DBuf<long long> gofs;
@@ -732,6 +735,7 @@ struct JobH {
     std::vector<std::array<double, 8>> fmax;
     double sNorm = 0.0, wNorm = 0.0, gsq = -1.0;
     std::vector<int> work;
+    std::vector<int> wpos;  // work-list index of each node of this iteration (-1: none)
     int64_t tuples = 0, slices = 0;
     size_t genOff = 0;
     uint32_t genMask = 0;
@@ -740,76 +744,6 @@ struct JobH {
 };
 
 // features of a freshly created child c of node p (see mwg_predict.h)
-// Features of the 8 children of a node that was just split (one row of MLP_FEATURES
-// each, at f + a * MLP_FEATURES for child a).  The input-tree neighbourhoods of the
-// 8 siblings lie in one 4 x 4 x 4 block of translations, looked up once.
-static void children_features(const JobH &J, const MraH &M, int p, double tnorm, float *f) {
-    const TreeH &T = *J.out;
-    const double *pn = &T.cnorm[8 * (size_t)p];
-    double w2 = 0.0;
-    for (int q = 1; q < 8; q++) w2 += pn[q] * pn[q];
-    const double wn = std::sqrt(w2), sn = pn[0];
-    const int np = M.n0 + T.depth[p];
-    const double eps2 = 2.0 * 2.220446049250313e-16;
-    const double thr = std::max(eps2, J.prec * tnorm * std::pow(2.0, -0.5 * (np + 1)));
-    const double thr_c = std::max(eps2, J.prec * tnorm * std::pow(2.0, -0.5 * (np + 2)));
-    auto lg = [](double v) { return (float)std::max(-20.0, std::min(20.0, std::log10(std::max(v, 1e-300)))); };
-    // decay of the wavelet norm from the grandparent to the parent (0 on the roots' children)
-    double wpp = 0.0;
-    if (T.parent[p] >= 0) {
-        const double *gn = &T.cnorm[8 * (size_t)T.parent[p]];
-        for (int q = 1; q < 8; q++) wpp += gn[q] * gn[q];
-        wpp = std::sqrt(wpp);
-    }
-    const float f0 = lg(wn / thr), f1 = lg(sn / thr_c), f2 = wpp > 0.0 ? lg(wn / wpp) : 0.0f;
-    const float f3 = lg(wn / std::max(std::fabs(sn), 1e-300));
-    const int c0 = T.child[p];
-    const int d = T.depth[c0];
-    const double inn = std::sqrt(std::max(J.in->sqnorm, 1e-300));
-    // input norms mapped to the output's scale, against the child's split threshold
-    const double in_to_out = tnorm / inn;
-    const float f8 = (float)std::log10(J.op->mu + 1e-3);
-    const float f9 = (float)std::log10(M.width(0) * std::ldexp(1.0, -d) / M.boxes[0]);
-    const int bx = 2 * T.lr[3 * p] - 1, by = 2 * T.lr[3 * p + 1] - 1, bz = 2 * T.lr[3 * p + 2] - 1;
-    int idx[64];
-    double wv[64];
-    for (int k = 0; k < 64; k++) {
-        const int x = bx + (k & 3), y = by + ((k >> 2) & 3), z = bz + (k >> 4);
-        idx[k] = (x < 0 || y < 0 || z < 0) ? -1 : J.in->find_hashed(d, x, y, z);
-        double a = 0.0;
-        if (idx[k] >= 0)
-            for (int q = 1; q < 8; q++) a += J.in->cnorm[8 * (size_t)idx[k] + q] * J.in->cnorm[8 * (size_t)idx[k] + q];
-        wv[k] = std::sqrt(a);
-    }
-    for (int a = 0; a < 8; a++) {
-        const int c = c0 + a;
-        float *fc = f + (size_t)a * MLP_FEATURES;
-        // child position inside the block: offset 1 + its bit per dimension
-        const int cx = 1 + (int)(T.lr[3 * c] - 2 * T.lr[3 * p]), cy = 1 + (int)(T.lr[3 * c + 1] - 2 * T.lr[3 * p + 1]),
-                  cz = 1 + (int)(T.lr[3 * c + 2] - 2 * T.lr[3 * p + 2]);
-        const int k0 = cx + 4 * cy + 16 * cz;
-        fc[0] = f0;
-        fc[1] = f1;
-        fc[2] = f2;
-        fc[3] = f3;
-        const int i0 = idx[k0];
-        fc[4] = i0 >= 0 ? 1.0f : 0.0f;
-        fc[5] = i0 >= 0 ? lg(wv[k0] * in_to_out / thr_c) : -20.0f;
-        fc[6] = (i0 >= 0 && J.in->child[i0] >= 0) ? 1.0f : 0.0f;
-        double nb = 0.0;
-        for (int dz = -1; dz <= 1; dz++)
-            for (int dy = -1; dy <= 1; dy++)
-                for (int dx = -1; dx <= 1; dx++) {
-                    if (!dx && !dy && !dz) continue;
-                    const int k = (cx + dx) + 4 * (cy + dy) + 16 * (cz + dz);
-                    if (idx[k] >= 0) nb = std::max(nb, wv[k]);
-                }
-        fc[7] = nb > 0.0 ? lg(nb * in_to_out / thr_c) : -20.0f;
-        fc[8] = f8;
-        fc[9] = f9;
-    }
-}
-
 // ------------------------------------------------------------------ learned predictor
 
 struct PredH {
@@ -1082,12 +1016,22 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                 T.outNorm = J.out->norm.p;
                 for (int q = 0; q < 3; q++) T.lmax[q] = M.boxes[q] << d;
                 T.depth = d;
+                T.feat_innorm = std::sqrt(std::max(J.in->sqnorm, 0.0));
+                T.feat_tnorm = J.gsq > 0.0 ? std::sqrt(J.gsq) : T.feat_innorm;  // first iteration: input norm
+                T.prec = J.prec;
+                T.feat_mu = (float)std::log10(J.op->mu + 1e-3);
+                T.feat_cell = (float)std::log10(M.width(0) / M.boxes[0] * std::ldexp(1.0, -(d + 1)));
+                T.n0 = M.n0;
                 tab_of[d] = (int)jt.size();
                 jt.push_back(T);
                 jt_job.push_back(j);
             }
-            for (int g : J.work)
-                work.push_back({tab_of[J.out->depth[g]], g, {J.out->lr[3 * g], J.out->lr[3 * g + 1], J.out->lr[3 * g + 2]}});
+            J.wpos.assign(J.out->n(), -1);
+            for (int g : J.work) {
+                J.wpos[g] = (int)work.size();
+                work.push_back({tab_of[J.out->depth[g]], g, {J.out->lr[3 * g], J.out->lr[3 * g + 1], J.out->lr[3 * g + 2]},
+                                J.out->parent[g]});
+            }
         }
         const int nw = (int)work.size();
         PT.lap("opers+boxes");
@@ -1145,6 +1089,8 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         C.gtotal.alloc((size_t)nw * nseg * nparts);
         C.gofs.alloc((size_t)nw * nseg * nparts);
         C.segs.alloc((size_t)nw * 8);
+        C.segcnt.alloc((size_t)nw * nseg);
+        C.gslice.alloc((size_t)nw);
         C.wnorms.alloc((size_t)nw * 8);
         C.tsum.alloc(1);
         auto launch_count = [&]() {
@@ -1180,12 +1126,16 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         };
         std::vector<double> wn((size_t)nw * 8);
         std::vector<int> tot((size_t)nw * nseg * nparts);
+        const bool want_feat = use_pred || C.tracing;
+        std::vector<float> cprob(use_pred ? (size_t)nw * 8 : 0);
+        std::vector<float> cfeat(C.tracing ? (size_t)nw * 8 * MLP_FEATURES : 0);
         TaskSummary hsum;
         int hgen[2];
         auto launch_schedule = [&]() {
             ensure_caps(nullptr);
             HC(launch_build_tasks(s, nw, nseg, nparts, C.chunk, 4 * C.slice_chunk, C.gtotal.p, C.caps, C.genCount.p,
-                                  genPoolCap, C.gofs.p, C.tasks.p, C.stasks.p, C.reds.p, C.tsum.p, C.segs.p));
+                                  genPoolCap, C.gofs.p, C.tasks.p, C.stasks.p, C.reds.p, C.tsum.p, C.segs.p,
+                                  C.segcnt.p, C.gslice.p));
             if (!mixed)
                 HC(launch_enum(s, 1, nw, nparts, dwork, djobs, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, C.gofs.p,
                                C.tuples.p, C.tsum.p));
@@ -1206,11 +1156,29 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                                       mixed ? C.nodeslots.p : nullptr, K3));
             HC(hipEventRecord(C.ev1, s));
             HC(launch_work_norms(s, nw, dwork, djobs, K3, C.wnorms.p));
+            // refinement predictor: features of every work node's children and their split
+            // probabilities, read back with the norms (no extra round trip)
+            if (want_feat) {
+                C.cfeat.alloc((size_t)nw * 8 * MLP_FEATURES);
+                HC(launch_child_features(s, nw, dwork, djobs, C.wnorms.p, C.cfeat.p));
+                if (use_pred) {
+                    C.cprob.alloc((size_t)nw * 8);
+                    HC(launch_mlp_forward(s, nw * 8, nullptr, C.cfeat.p, C.pred->mean.p, C.pred->sdev.p,
+                                          C.pred->P.p, C.cprob.p, nullptr, nullptr, nullptr));
+                }
+            }
             // one pinned read-back area: summary | generated count, overflow | counts | norms
+            // | child probabilities | child features
             const size_t o_sum = 0, o_gen = 64, o_tot = 128;
             const size_t o_wn = (o_tot + tot.size() * sizeof(int) + 255) & ~(size_t)255;
-            C.rb.reserve(o_wn + wn.size() * sizeof(double));
+            const size_t o_pr = (o_wn + wn.size() * sizeof(double) + 255) & ~(size_t)255;
+            const size_t o_ft = (o_pr + cprob.size() * sizeof(float) + 255) & ~(size_t)255;
+            C.rb.reserve(o_ft + cfeat.size() * sizeof(float));
             char *h = C.rb.h;
+            if (use_pred)
+                HC(hipMemcpyAsync(h + o_pr, C.cprob.p, cprob.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+            if (C.tracing)
+                HC(hipMemcpyAsync(h + o_ft, C.cfeat.p, cfeat.size() * sizeof(float), hipMemcpyDeviceToHost, s));
             HC(hipMemcpyAsync(h + o_sum, C.tsum.p, sizeof(TaskSummary), hipMemcpyDeviceToHost, s));
             HC(hipMemcpyAsync(h + o_gen, C.genCount.p, sizeof(int), hipMemcpyDeviceToHost, s));
             HC(hipMemcpyAsync(h + o_gen + 4, C.genOverflow.p, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1222,6 +1190,8 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             std::memcpy(hgen, h + o_gen, 2 * sizeof(int));
             std::memcpy(tot.data(), h + o_tot, tot.size() * sizeof(int));
             std::memcpy(wn.data(), h + o_wn, wn.size() * sizeof(double));
+            if (use_pred) std::memcpy(cprob.data(), h + o_pr, cprob.size() * sizeof(float));
+            if (C.tracing) std::memcpy(cfeat.data(), h + o_ft, cfeat.size() * sizeof(float));
             if (hgen[1]) throw Error(MWG_ERR_RANGE, "generated-node table overflow");
         };
         launch_count();
@@ -1267,6 +1237,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         // level includes every computed node of that level and above; speculative
         // children are kept only when the norm test splits their parent
         std::vector<int> cand_job, cand_node;
+        std::vector<float> cand_prob;
         for (int j : active) {
             JobH &J = jobs[j];
             std::vector<int> cur = J.work;
@@ -1313,16 +1284,20 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                         J.spec.resize(J.out->n(), 0);
                         J.alive.resize(J.out->n(), 1);
                         J.has_feat.resize(J.out->n(), 0);
-                        if (C.tracing || use_pred) {
+                        const int c0 = J.out->child[g], wi = J.wpos[g];
+                        if (C.tracing) {
                             J.feat.resize((size_t)J.out->n() * MLP_FEATURES, 0.0f);
-                            const int c0 = J.out->child[g];
-                            children_features(J, M, g, tnorm, &J.feat[(size_t)c0 * MLP_FEATURES]);
+                            std::memcpy(&J.feat[(size_t)c0 * MLP_FEATURES], &cfeat[(size_t)wi * 8 * MLP_FEATURES],
+                                        8 * MLP_FEATURES * sizeof(float));
                             for (int a = 0; a < 8; a++) J.has_feat[c0 + a] = 1;
                         }
                         for (int a = 0; a < 8; a++) {
-                            next.push_back(J.out->child[g] + a);
-                            cand_job.push_back(j);
-                            cand_node.push_back(J.out->child[g] + a);
+                            next.push_back(c0 + a);
+                            if (use_pred) {
+                                cand_job.push_back(j);
+                                cand_node.push_back(c0 + a);
+                                cand_prob.push_back(cprob[(size_t)wi * 8 + a]);
+                            }
                         }
                     }
                 }
@@ -1333,14 +1308,8 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         // learned lookahead: nodes predicted to split get their children computed now
         if (use_pred && !cand_node.empty()) {
             const int nc = (int)cand_node.size();
-            std::vector<float> X((size_t)nc * MLP_FEATURES);
-            for (int i = 0; i < nc; i++)
-                std::memcpy(&X[(size_t)i * MLP_FEATURES], &jobs[cand_job[i]].feat[(size_t)cand_node[i] * MLP_FEATURES],
-                            MLP_FEATURES * sizeof(float));
-            std::vector<float> prob(nc);
-            predictor_infer(*C.pred, nc, X.data(), prob.data());
             for (int i = 0; i < nc; i++) {
-                if (prob[i] < C.pred_thr) continue;
+                if (cand_prob[i] < C.pred_thr) continue;
                 JobH &J = jobs[cand_job[i]];
                 const int c = cand_node[i];
                 const int d = J.out->depth[c];

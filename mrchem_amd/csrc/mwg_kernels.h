@@ -27,8 +27,9 @@ struct ProjGeom {
 
 struct WorkItem {
     int job;
-    int node;  // index in the job's output tree
-    int l[3];  // translation relative to the root corner
+    int node;    // index in the job's output tree
+    int l[3];    // translation relative to the root corner
+    int parent;  // the node's parent in the output tree (-1 on the roots)
 };
 
 struct JobTab {
@@ -49,6 +50,13 @@ struct JobTab {
     int lmax[3];
     int depth;
     int ubox[3];  // union of ub over the table's terms (device-built)
+    // refinement-predictor features of the nodes' children (see k_child_features)
+    double feat_tnorm;   // output norm estimate at the start of the iteration
+    double feat_innorm;  // input tree norm
+    double prec;
+    float feat_mu;       // log10(mu + 1e-3)
+    float feat_cell;     // log10 of the children's cell width
+    int n0;              // root scale
 };
 
 // Device-built contraction schedule of one refinement iteration (sizes the host reads
@@ -190,7 +198,13 @@ struct SegRec {
 };
 hipError_t launch_build_tasks(hipStream_t s, int nw, int nseg, int nparts, int chunk, int wchunk, const int *tot,
                               TaskCaps caps, const int *gen_count, int gen_pcap, long long *ofs, ContractTask *tasks,
-                              SliceTask *stasks, ReduceItem *reds, TaskSummary *sum, SegRec *segs);
+                              SliceTask *stasks, ReduceItem *reds, TaskSummary *sum, SegRec *segs, int *segcnt,
+                              long long *gslice);
+// refinement-predictor features of the 8 children of every work node (row i*8 + a, a the
+// child index), from the work nodes' norms just computed, their parents' norms and the
+// input tree around each child
+hipError_t launch_child_features(hipStream_t s, int nw, const WorkItem *work, const JobTab *jobs,
+                                 const double *wnorms, float *feat);
 // with sum: the task count is read on the device; grid (blocks) covers the capacity
 hipError_t launch_contract_fp64(hipStream_t s, int K, int ntask, const TaskSummary *sum, int grid,
                                 const ContractTask *tasks, const WorkItem *work, const JobTab *jobs,

@@ -1024,6 +1024,88 @@ __global__ void __launch_bounds__(256) k_enum(int nw, int nparts, const WorkItem
     (void)nw;
 }
 
+// ----------------------------------------------------------------- apply: predictor features
+
+// One thread per (work node, child).  Features (norms relative to split thresholds, so
+// that one network serves several precisions; thresholds use the output-norm estimate
+// at the start of the iteration):
+//  0 log10(||w_parent|| / split threshold of the parent)
+//  1 log10(||s_parent|| / split threshold of the child)
+//  2 log10(||w_parent|| / ||w_grandparent||) (0 below the roots)
+//  3 log10(||w_parent|| / ||s_parent||)    4 input tree has the child's cell (0/1)
+//  5 log10(||w_input(cell)|| ||g|| / ||input|| / split threshold of the child), -20 absent
+//  6 the input node of the cell has children (0/1)
+//  7 as 5 for the largest of the 26 neighbouring cells, -20 when none
+//  8 log10(mu + 1e-3)                      9 log10(cell width of the child)
+__device__ inline float feat_lg(double v) {
+    return (float)fmin(20.0, fmax(-20.0, log10(fmax(v, 1e-300))));
+}
+
+__device__ inline double input_wnorm(const JobTab &J, int i) {
+    double a = 0.0;
+    for (int q = 1; q < 8; q++) a += J.inNorm[(size_t)i * 8 + q] * J.inNorm[(size_t)i * 8 + q];
+    return sqrt(a);
+}
+
+__global__ void k_child_features(int nw, const WorkItem *work, const JobTab *jobs, const double *wnorms,
+                                 float *feat) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nw * 8) return;
+    const int i = t >> 3, a = t & 7;
+    const WorkItem wi = work[i];
+    const JobTab &J = jobs[wi.job];
+    const double *pn = wnorms + (size_t)i * 8;
+    double w2 = 0.0;
+    for (int q = 1; q < 8; q++) w2 += pn[q] * pn[q];
+    const double wn = sqrt(w2), sn = pn[0];
+    const int np = J.n0 + J.depth;
+    const double eps2 = 2.0 * 2.220446049250313e-16;
+    const double thr = fmax(eps2, J.prec * J.feat_tnorm * exp2(-0.5 * (np + 1)));
+    const double thr_c = fmax(eps2, J.prec * J.feat_tnorm * exp2(-0.5 * (np + 2)));
+    double wpp = 0.0;
+    if (wi.parent >= 0) {
+        for (int q = 1; q < 8; q++) {
+            const double v = J.outNorm[(size_t)wi.parent * 8 + q];
+            wpp += v * v;
+        }
+        wpp = sqrt(wpp);
+    }
+    float *f = feat + (size_t)t * 10;
+    f[0] = feat_lg(wn / thr);
+    f[1] = feat_lg(sn / thr_c);
+    f[2] = wpp > 0.0 ? feat_lg(wn / wpp) : 0.0f;
+    f[3] = feat_lg(wn / fmax(fabs(sn), 1e-300));
+    const int d = J.depth + 1;
+    const int cx = 2 * wi.l[0] + (a & 1), cy = 2 * wi.l[1] + ((a >> 1) & 1), cz = 2 * wi.l[2] + ((a >> 2) & 1);
+    const int lim[3] = {2 * J.lmax[0], 2 * J.lmax[1], 2 * J.lmax[2]};
+    const double in_to_out = J.feat_tnorm / fmax(J.feat_innorm, 1e-300);
+    const int i0 = dhash_find(J.inHash, dnode_key(d, cx, cy, cz));
+    f[4] = i0 >= 0 ? 1.0f : 0.0f;
+    f[5] = i0 >= 0 ? feat_lg(input_wnorm(J, i0) * in_to_out / thr_c) : -20.0f;
+    f[6] = (i0 >= 0 && dhash_find(J.inHash, dnode_key(d + 1, 2 * cx, 2 * cy, 2 * cz)) >= 0) ? 1.0f : 0.0f;
+    double nb = 0.0;
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                if (!dx && !dy && !dz) continue;
+                const int x = cx + dx, y = cy + dy, z = cz + dz;
+                if (x < 0 || y < 0 || z < 0 || x >= lim[0] || y >= lim[1] || z >= lim[2]) continue;
+                const int j = dhash_find(J.inHash, dnode_key(d, x, y, z));
+                if (j >= 0) nb = fmax(nb, input_wnorm(J, j));
+            }
+    f[7] = nb > 0.0 ? feat_lg(nb * in_to_out / thr_c) : -20.0f;
+    f[8] = J.feat_mu;
+    f[9] = J.feat_cell;
+}
+
+hipError_t launch_child_features(hipStream_t s, int nw, const WorkItem *work, const JobTab *jobs,
+                                 const double *wnorms, float *feat) {
+    if (nw <= 0) return hipSuccess;
+    const int n = nw * 8;
+    hipLaunchKernelGGL(k_child_features, dim3((n + 255) / 256), dim3(256), 0, s, nw, work, jobs, wnorms, feat);
+    return hipGetLastError();
+}
+
 // ----------------------------------------------------------------- apply: contraction schedule
 
 __device__ inline long long wave_incl_scan_ll(long long v) {
@@ -1064,38 +1146,44 @@ __device__ inline long long block_excl_scan_1024(long long v, long long *wsum, l
 // nodes' slice runs) with many tasks go to a list in LDS.  Phase 3: all threads write the
 // tasks of the listed segments.  Every task has a fixed position, so the schedule does
 // not depend on the order of the list.
-constexpr int BT_FEW = 16, BT_LIST = 2048;
+constexpr int BT_FEW = 32, BT_LIST = 2048;
 
 __global__ void __launch_bounds__(1024) k_build_tasks(int nw, int nseg, int nparts, int chunk, int wchunk,
                                                       const int *tot, TaskCaps caps, const int *gen_count,
                                                       int gen_pcap, long long *ofs, ContractTask *tasks,
                                                       SliceTask *stasks, ReduceItem *reds, TaskSummary *sum,
-                                                      SegRec *segs) {
+                                                      SegRec *segs, int *segcnt, long long *gslice) {
     __shared__ long long wsum[17];
     __shared__ int big[BT_LIST];     // segment ids (>= 0) or -(node + 1) for slice runs
-    __shared__ long long bigs[BT_LIST];
     __shared__ int nbig;
-    auto segc = [&](int sg) {
-        int c = 0;
-        for (int p = 0; p < nparts; p++) c += tot[(size_t)sg * nparts + p];
-        return c;
-    };
     const bool mixed = nseg == 9;
     // FP64 tasks of a tuple segment with c tuples (plain: always one, even empty, so that
     // every output component is written; mixed: the reduce item writes the component)
     auto ntasks_of = [&](int c) { return mixed ? (c + chunk - 1) / chunk : (c <= chunk ? 1 : (c + chunk - 1) / chunk); };
     auto nslots_of = [&](int c) { return mixed ? (c + chunk - 1) / chunk : (c <= chunk ? 0 : (c + chunk - 1) / chunk); };
     if (threadIdx.x == 0) nbig = 0;
+    // phase 0: segment counts (sum over the parts) and per-part offsets inside the segment
+    const long long nsg = (long long)nw * nseg;
+    for (long long sg = threadIdx.x; sg < nsg; sg += blockDim.x) {
+        int c = 0;
+        for (int p = 0; p < nparts; p++) {
+            ofs[sg * nparts + p] = c;
+            c += tot[sg * nparts + p];
+        }
+        segcnt[sg] = c;
+    }
+    __syncthreads();
+    // phase 1: this thread's totals (a contiguous range of output nodes)
     const int g0 = (int)((long long)nw * threadIdx.x / 1024), g1 = (int)((long long)nw * (threadIdx.x + 1) / 1024);
     long long tup = 0, sl = 0, nt = 0, nst = 0, slots = 0, nred = 0;
     for (int g = g0; g < g1; g++) {
         if (mixed) {
-            const int cs = segc(g * 9 + 8);
+            const int cs = segcnt[(size_t)g * 9 + 8];
             sl += cs;
             nst += (cs + wchunk - 1) / wchunk;
         }
         for (int q = 0; q < 8; q++) {
-            const int c = segc(g * nseg + q);
+            const int c = segcnt[(size_t)g * nseg + q];
             tup += c;
             nt += ntasks_of(c);
             slots += nslots_of(c);
@@ -1131,41 +1219,31 @@ __global__ void __launch_bounds__(1024) k_build_tasks(int nw, int nseg, int npar
         tasks[r.task0 + h] = ContractTask{r.start + (long long)h * chunk, cnt, sgi >> 3, sgi & 7,
                                           r.slot0 < 0 ? -1 : r.slot0 + h};
     };
-    // phase 2
+    auto emit_stask = [&](int g, int h) {
+        const int cs = segcnt[(size_t)g * 9 + 8], nn0 = segs[(size_t)g * 8].pad_stask0;
+        stasks[nn0 + h] = SliceTask{gslice[g] + (long long)h * wchunk, min(wchunk, cs - h * wchunk), g, nn0 + h};
+    };
+    // phase 2: segment records, reduce items, tasks of small segments
     for (int g = g0; g < g1; g++) {
         int nn0 = (int)bnst, nns = 0;
         if (mixed) {
-            const int sg = g * 9 + 8;
-            long long o = bsl;
-            for (int p = 0; p < nparts; p++) {
-                ofs[(size_t)sg * nparts + p] = o;
-                o += tot[(size_t)sg * nparts + p];
-            }
-            const int cs = (int)(o - bsl);
+            const int cs = segcnt[(size_t)g * 9 + 8];
             nns = (cs + wchunk - 1) / wchunk;
+            gslice[g] = bsl;
+            segs[(size_t)g * 8].pad_stask0 = nn0;  // first slice task of node g
             int slot_ = -1;
             if (nns > BT_FEW) {
                 slot_ = atomicAdd(&nbig, 1);
-                if (slot_ < BT_LIST) {
-                    big[slot_] = -(g + 1);
-                    bigs[slot_] = bsl;
-                }
+                if (slot_ < BT_LIST) big[slot_] = -(g + 1);
             }
             if (slot_ < 0 || slot_ >= BT_LIST)
                 for (int h = 0; h < nns; h++)
                     stasks[nn0 + h] = SliceTask{bsl + (long long)h * wchunk, min(wchunk, cs - h * wchunk), g, nn0 + h};
-            segs[(size_t)g * 8].pad_stask0 = nn0;  // first slice task of node g
             bnst += nns;
             bsl += cs;
         }
         for (int q = 0; q < 8; q++) {
-            const int sg = g * nseg + q;
-            long long o = btup;
-            for (int p = 0; p < nparts; p++) {
-                ofs[(size_t)sg * nparts + p] = o;
-                o += tot[(size_t)sg * nparts + p];
-            }
-            const int c = (int)(o - btup);
+            const int c = segcnt[(size_t)g * nseg + q];
             const int k = ntasks_of(c), ns = nslots_of(c);
             const int sgi = g * 8 + q;
             const SegRec r{btup, c, (int)bnt, ns ? (int)bslots : -1, 0};
@@ -1184,26 +1262,29 @@ __global__ void __launch_bounds__(1024) k_build_tasks(int nw, int nseg, int npar
                 for (int h = 0; h < k; h++) emit_task(r, sgi, h);
             bnt += k;
             bslots += ns;
-            btup = o;
+            btup += c;
         }
     }
     __syncthreads();
-    // phase 3: the listed segments, all threads
+    // phase 3: absolute per-part offsets (segment base + relative), then the listed
+    // segments, one wave per segment with the lanes over its tasks
+    for (long long sg = threadIdx.x; sg < nsg; sg += blockDim.x) {
+        const int g = (int)(sg / nseg), q = (int)(sg % nseg);
+        const long long base = q < 8 ? segs[(size_t)g * 8 + q].start : gslice[g];
+        for (int p = 0; p < nparts; p++) ofs[sg * nparts + p] += base;
+    }
     const int nb = min(nbig, BT_LIST);
-    for (int b = 0; b < nb; b++) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int b = wave; b < nb; b += 16) {
         const int id = big[b];
         if (id >= 0) {
             const SegRec r = segs[id];
             const int k = ntasks_of(r.count);
-            for (int h = threadIdx.x; h < k; h += blockDim.x) emit_task(r, id, h);
+            for (int h = lane; h < k; h += 64) emit_task(r, id, h);
         } else {
             const int g = -id - 1;
-            const int nn0 = segs[(size_t)g * 8].pad_stask0;
-            const long long s0 = bigs[b];
-            const int cs = segc(g * 9 + 8);
-            const int nns = (cs + wchunk - 1) / wchunk;
-            for (int h = threadIdx.x; h < nns; h += blockDim.x)
-                stasks[nn0 + h] = SliceTask{s0 + (long long)h * wchunk, min(wchunk, cs - h * wchunk), g, nn0 + h};
+            const int nns = (segcnt[(size_t)g * 9 + 8] + wchunk - 1) / wchunk;
+            for (int h = lane; h < nns; h += 64) emit_stask(g, h);
         }
     }
 }
@@ -2021,9 +2102,10 @@ hipError_t launch_transform(hipStream_t s, int mode, int n, const TransformItem 
 
 hipError_t launch_build_tasks(hipStream_t s, int nw, int nseg, int nparts, int chunk, int wchunk, const int *tot,
                               TaskCaps caps, const int *gen_count, int gen_pcap, long long *ofs, ContractTask *tasks,
-                              SliceTask *stasks, ReduceItem *reds, TaskSummary *sum, SegRec *segs) {
+                              SliceTask *stasks, ReduceItem *reds, TaskSummary *sum, SegRec *segs, int *segcnt,
+                              long long *gslice) {
     hipLaunchKernelGGL(k_build_tasks, dim3(1), dim3(1024), 0, s, nw, nseg, nparts, chunk, wchunk, tot, caps,
-                       gen_count, gen_pcap, ofs, tasks, stasks, reds, sum, segs);
+                       gen_count, gen_pcap, ofs, tasks, stasks, reds, sum, segs, segcnt, gslice);
     return hipGetLastError();
 }
 

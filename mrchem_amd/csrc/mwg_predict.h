@@ -5,16 +5,8 @@
 
 namespace mwg {
 
-// per-node features (computed on the host when a node is created by a split); norms are
-// taken relative to split thresholds so that one network serves several precisions:
-//  0 log10(||w_parent|| / split threshold of the parent)
-//  1 log10(||s_parent|| / split threshold of the node)
-//  2 log10(||w_parent|| / ||w_grandparent||) (0 below the roots)
-//  3 log10(||w_parent|| / ||s_parent||)  4 input tree has the node (0/1)
-//  5 log10(||w_input(node)|| ||g|| / ||input|| / split threshold of the node), -20 absent
-//  6 input tree is refined below the node (0/1)
-//  7 as 5 for the largest of the 26 neighbours, -20 when none
-//  8 log10(mu + 1e-3)                    9 log10(cell width of the node)
+// per-node features: 10 values computed on the device for the children of every node of
+// a refinement iteration (k_child_features in mwg_kernels.hip lists them)
 constexpr int MLP_FEATURES = 10;
 constexpr int MLP_HIDDEN = 64;
 constexpr int MLP_PARAMS = MLP_HIDDEN * MLP_FEATURES + MLP_HIDDEN + MLP_HIDDEN * MLP_HIDDEN + MLP_HIDDEN + MLP_HIDDEN + 1;
