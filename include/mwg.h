@@ -75,9 +75,11 @@ const char *mwg_last_error(mwg_ctx ctx);
 int mwg_sync(mwg_ctx ctx);
 /* contraction task granularity (no reference counterpart: MRCPP threads over output
  * nodes with OpenMP): tuples per FP64 wave task (1..4096) and FP32 MFMA slices per wave
- * (1..1024; a workgroup task takes 4 x slice_chunk slices of one output node).  Results
- * agree to rounding for any setting; mrchem_amd.autotune picks the fastest on a sample
- * workload. */
+ * (1..16; a workgroup task takes 4 x slice_chunk slices of one output node, and each
+ * wave sums its slice_chunk results in FP32 before the FP64 sum over waves and tasks).
+ * Results agree to FP64 rounding for any setting on the FP64 path and to FP32 rounding
+ * of the FP32-routed slices on the mixed path; mrchem_amd.autotune picks the fastest on
+ * a sample workload. */
 int mwg_ctx_set_tuning(mwg_ctx ctx, int fp64_chunk, int slice_chunk);
 int mwg_ctx_get_tuning(mwg_ctx ctx, int *fp64_chunk, int *slice_chunk);
 

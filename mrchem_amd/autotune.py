@@ -10,7 +10,8 @@ The best point depends on k, the precision and the tree shapes, so it is measure
 caller's own batch for each candidate and installs the fastest on the context.
 
 Candidates are applied to fresh output trees each time; the results agree to
-rounding for every setting (only the FP64 summation grouping changes).
+rounding for every setting (only the summation grouping changes: FP64 on the FP64
+path, and the FP32 per-wave runs of FP32-routed slices on the mixed path).
 """
 from __future__ import annotations
 
@@ -20,7 +21,7 @@ import time
 from . import mrcpp as G
 
 FP64_CHUNKS = (16, 32, 64, 128, 256)
-SLICE_CHUNKS = (4, 8, 16, 32, 64)
+SLICE_CHUNKS = (2, 4, 8, 16)
 
 
 def _measure(mra, trees, ops, prec, flags, reps):

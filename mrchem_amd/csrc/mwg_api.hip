@@ -241,7 +241,7 @@ struct Ctx {
     struct PredH *pred = nullptr;
     float pred_thr = 0.5f;
     int chunk = 64;        // tuples per FP64 contraction task
-    int slice_chunk = 16;  // slices per MFMA task
+    int slice_chunk = 16;  // slices per wave of an MFMA task (at most MAX_SLICE_CHUNK)
 };
 
 // Wait for the context's stream by polling: the refinement loop waits once per
@@ -2120,8 +2120,8 @@ int mwg_predictor_train(mwg_predictor pred, int64_t n, const float *X, const flo
 
 int mwg_ctx_set_tuning(mwg_ctx ctx, int fp64_chunk, int slice_chunk) {
     if (!ctx) return MWG_ERR_ARG;
-    if (fp64_chunk < 1 || fp64_chunk > 4096 || slice_chunk < 1 || slice_chunk > 1024) {
-        ctx->err = "mwg_ctx_set_tuning: fp64_chunk must be in [1, 4096] and slice_chunk in [1, 1024]";
+    if (fp64_chunk < 1 || fp64_chunk > 4096 || slice_chunk < 1 || slice_chunk > MAX_SLICE_CHUNK) {
+        ctx->err = "mwg_ctx_set_tuning: fp64_chunk must be in [1, 4096] and slice_chunk in [1, 16]";
         return MWG_ERR_ARG;
     }
     ctx->chunk = fp64_chunk;

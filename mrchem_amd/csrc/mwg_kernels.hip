@@ -1529,16 +1529,27 @@ __global__ void __launch_bounds__(256) k_enum_mixed(int nw, int nparts, const Wo
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-// One wavefront per task = a chunk of the slices of one output node.  A slice
+// One workgroup of 4 waves per task = a chunk of the slices of one output node, the
+// waves taking every fourth slice.  A slice
 // (f, ft, term, delta) produces all 8 output components with three GEMMs on
 // v_mfma_f32_16x16x4_f32, M = 16 = (gt_d, a) stacked [X_{0,ft_d}; X_{1,ft_d}],
 // K = 8, N = 64, 128, 256 (the dimensions already contracted).  Lane layout of
 // the instruction: A[i][k] from lane i%16 + 16 k, B[k][j] from lane j%16 + 16 k,
 // D[4 (lane/16) + r][lane%16] in register r.  Intermediates rotate through LDS
 // (rows padded to 68 and 132 floats: conflict-free for the access patterns).
-// Each slice result is scaled by the term weight and accumulated in FP64.
+// Each slice result is scaled by the term weight (zero for components the slice
+// does not reach) and summed in FP32 over the wave's slices (at most
+// MAX_SLICE_CHUNK), then in FP64 over the waves and over the node's tasks.
 // operands of one slice in registers: A fragments of the three passes and the
-// pass-x B fragments of f (converted to FP32)
+// pass-x B fragments of f, as loaded (FP64: converted to FP32 only when the slice
+// is contracted, so that the loads for a later slice stay in flight meanwhile)
+struct SliceRaw {
+    float ax[2], ay[2], az[2];
+    double fb[4][2];
+    double w;
+    int mask;
+};
+
 struct SliceOps {
     float ax[2], ay[2], az[2];
     float fb[4][2];
@@ -1546,11 +1557,26 @@ struct SliceOps {
     int mask;
 };
 
-__device__ inline SliceOps load_slice_ops(const Slice &sl, const float *B32, const double *inCoef,
+// a slice record through a wave-uniform pointer (scalar loads of its two 16-byte halves)
+__device__ inline Slice read_slice(const Slice *p) {
+    const uint4 *q = (const uint4 *)p;
+    const uint4 a = q[0], b = q[1];
+    Slice s;
+    s.w = __hiloint2double((int)a.y, (int)a.x);
+    s.f = (int)a.z;
+    s.xo = (int)a.w;
+    s.yo = (int)b.x;
+    s.zo = (int)b.y;
+    s.ft = (uint8_t)(b.z & 0xffu);
+    s.mask = (uint8_t)((b.z >> 8) & 0xffu);
+    return s;
+}
+
+__device__ inline SliceRaw load_slice_raw(const Slice &sl, const float *B32, const double *inCoef,
                                           const double *genPool, int m, int q) {
     constexpr int K = 8, K3 = 512;
     const int gxm = m >> 3, am = m & 7;
-    SliceOps o;
+    SliceRaw o;
     const gfloat *px = (const gfloat *)B32 + sl.xo + gxm * 128 + am * K;
     const gfloat *py = (const gfloat *)B32 + sl.yo + gxm * 128 + am * K;
     const gfloat *pz = (const gfloat *)B32 + sl.zo + gxm * 128 + am * K;
@@ -1565,9 +1591,26 @@ __device__ inline SliceOps load_slice_ops(const Slice &sl, const float *B32, con
 #pragma unroll
     for (int t = 0; t < 4; t++)
 #pragma unroll
-        for (int s2 = 0; s2 < 2; s2++) o.fb[t][s2] = (float)f[(16 * t + m) * K + 4 * s2 + q];
+        for (int s2 = 0; s2 < 2; s2++) o.fb[t][s2] = f[(16 * t + m) * K + 4 * s2 + q];
     o.w = sl.w;
     o.mask = sl.mask;
+    return o;
+}
+
+__device__ inline SliceOps slice_ops(const SliceRaw &r) {
+    SliceOps o;
+#pragma unroll
+    for (int s2 = 0; s2 < 2; s2++) {
+        o.ax[s2] = r.ax[s2];
+        o.ay[s2] = r.ay[s2];
+        o.az[s2] = r.az[s2];
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int s2 = 0; s2 < 2; s2++) o.fb[t][s2] = (float)r.fb[t][s2];
+    o.w = r.w;
+    o.mask = r.mask;
     return o;
 }
 
@@ -1582,19 +1625,25 @@ __device__ inline void mfma8_task(const SliceTask task, const WorkItem *work, co
     float *L2 = lds + 4 * 16 * S1 + wave * 16 * S2;
     const int m = lane & 15, q = lane >> 4;
     const int gxm = m >> 3, am = m & 7;
-    double acc[16][4];
+    float acc[16][4];
 #pragma unroll
     for (int t = 0; t < 16; t++)
 #pragma unroll
-        for (int r = 0; r < 4; r++) acc[t][r] = 0.0;
+        for (int r = 0; r < 4; r++) acc[t][r] = 0.f;
     // the four waves take every fourth slice of the task; the next slice's operands
     // are loaded while the current one is contracted
-    SliceOps cur;
-    if (wave < task.count) cur = load_slice_ops(slices[task.start + wave], J.operBlocks32, J.inCoef, genPool, m, q);
+    // software pipeline over the wave's slices it = wave, wave + 4, ...: the record of
+    // slice it + 8 and the operands of slice it + 4 are loaded while slice it is contracted
+    // (indices past the end are clamped to the last slice: the loads stay unconditional,
+    // so that the waits for them fall where their data are used)
+    const Slice *srec = slices + task.start;
+    const int last = task.count - 1;  // a task holds at least one slice
+    SliceRaw craw = load_slice_raw(read_slice(srec + min(wave, last)), J.operBlocks32, J.inCoef, genPool, m, q);
+    Slice rnext = read_slice(srec + min(wave + 4, last));
     for (int it = wave; it < task.count; it += 4) {
-        SliceOps nxt;
-        if (it + 4 < task.count)
-            nxt = load_slice_ops(slices[task.start + it + 4], J.operBlocks32, J.inCoef, genPool, m, q);
+        const SliceRaw nraw = load_slice_raw(rnext, J.operBlocks32, J.inCoef, genPool, m, q);
+        rnext = read_slice(srec + min(it + 8, last));
+        const SliceOps cur = slice_ops(craw);
         // pass x: C1[(gx,a)][n = l*8 + j] = sum_i X[a][i] f[n*8 + i]
         // (tiles in groups of 4 independent MFMA chains, operands read before the MFMAs)
         {
@@ -1635,8 +1684,8 @@ __device__ inline void mfma8_task(const SliceTask task, const WorkItem *work, co
         // and gz = (4 q + r) >> 3 = q >> 1 (the lane): two weights per lane and slice, masked
         // by the slice's surviving components.
         const int gtl = gxm | ((q >> 1) << 2), gth = gtl | 2;
-        const double wlow = ((cur.mask >> gtl) & 1) ? cur.w : 0.0;
-        const double whigh = ((cur.mask >> gth) & 1) ? cur.w : 0.0;
+        const float wlow = ((cur.mask >> gtl) & 1) ? (float)cur.w : 0.f;
+        const float whigh = ((cur.mask >> gth) & 1) ? (float)cur.w : 0.f;
         // tiles t3 < 8 carry gy = 0 (gt in {0,1,4,5}: mask bits 0x33), t3 >= 8 gy = 1 (0xCC);
         // a half without a surviving component is skipped (the mask is uniform per wave)
         const bool need_lo = (cur.mask & 0x33) != 0, need_hi = (cur.mask & 0xCC) != 0;
@@ -1654,51 +1703,49 @@ __device__ inline void mfma8_task(const SliceTask task, const WorkItem *work, co
                 c[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.az[0], b[t][0], floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
 #pragma unroll
             for (int t = 0; t < 4; t++) c[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.az[1], b[t][1], c[t], 0, 0, 0);
-            const double wm = g4 < 2 ? wlow : whigh;  // t3 = 4 g4 + t < 8 iff g4 < 2
+            const float wm = g4 < 2 ? wlow : whigh;
 #pragma unroll
             for (int t = 0; t < 4; t++)
 #pragma unroll
-                for (int r = 0; r < 4; r++) acc[4 * g4 + t][r] += wm * (double)c[t][r];
+                for (int r = 0; r < 4; r++) acc[4 * g4 + t][r] = fmaf(wm, c[t][r], acc[4 * g4 + t][r]);
         }
         wave_sync();
-        cur = nxt;
+        craw = nraw;
     }
-    // waves 1..3 add their accumulators into wave 0's, in that order (one 32 KB LDS buffer)
+    // FP64 sum over the waves in LDS, wave 0 first, then 1, 2, 3 (one 32 KB buffer)
     double *R = (double *)lds;
     __syncthreads();
-    for (int src = 1; src < 4; src++) {
+    for (int src = 0; src < 4; src++) {
         if (wave == src) {
 #pragma unroll
             for (int t3 = 0; t3 < 16; t3++)
 #pragma unroll
-                for (int r = 0; r < 4; r++) R[(t3 * 4 + r) * 64 + lane] = acc[t3][r];
-        }
-        __syncthreads();
-        if (wave == 0) {
-#pragma unroll
-            for (int t3 = 0; t3 < 16; t3++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) acc[t3][r] += R[(t3 * 4 + r) * 64 + lane];
+                for (int r = 0; r < 4; r++) {
+                    double *e = &R[(t3 * 4 + r) * 64 + lane];
+                    *e = src == 0 ? (double)acc[t3][r] : *e + (double)acc[t3][r];
+                }
         }
         __syncthreads();
     }
-    if (wave == 0) {
+    {
         double *dst = nodeslots + (size_t)task.slot * 8 * K3;
 #pragma unroll
-        for (int t3 = 0; t3 < 16; t3++) {
+        for (int u = 0; u < 4; u++) {
+            const int t3 = 4 * wave + u;
             const int gy = t3 >> 3, b = t3 & 7;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int m3 = 4 * q + r;
                 const int gz = m3 >> 3, cc = m3 & 7;
                 const int gt = gxm | (gy << 1) | (gz << 2);
-                dst[gt * K3 + (cc * K + b) * K + am] = acc[t3][r];
+                dst[gt * K3 + (cc * K + b) * K + am] = R[(t3 * 4 + r) * 64 + lane];
             }
         }
     }
 }
 
-__global__ void __launch_bounds__(256) k_contract_mfma8(int ntask_arg, const TaskSummary *sum,
+__global__ void __launch_bounds__(256)
+k_contract_mfma8(int ntask_arg, const TaskSummary *sum,
                                                         const SliceTask *tasks, const WorkItem *work,
                                                         const JobTab *jobs, const TermTab *tabs,
                                                         const Slice *slices, const double *genPool,

@@ -1,6 +1,7 @@
 """Contraction task granularity: every setting gives the same tree and coefficients
-(to FP64 rounding), invalid settings are refused, and the autotuner installs one of
-its candidates."""
+(to FP64 rounding on the FP64 path; on the mixed path to the FP32 rounding of the
+slices routed to FP32, far below prec), invalid settings are refused, and the
+autotuner installs one of its candidates."""
 import numpy as np
 import pytest
 
@@ -19,7 +20,7 @@ def test_tuning_does_not_change_results():
     default = ctx.get_tuning()
     ref = None
     try:
-        for fc, sc in [(64, 16), (1, 1), (7, 3), (4096, 1024)]:
+        for fc, sc in [(64, 16), (1, 1), (7, 3), (4096, 16)]:
             ctx.set_tuning(fc, sc)
             for flags in (G.MWG_APPLY_FP64, G.MWG_APPLY_MIXED):
                 outs = [G.FunctionTree(mra) for _ in trees]
@@ -32,7 +33,8 @@ def test_tuning_does_not_change_results():
                     continue
                 for (d0, l0, c0), (d1, l1, c1) in zip(ref[flags], ex):
                     assert np.array_equal(d0, d1) and np.array_equal(l0, l1)
-                    assert np.abs(c0 - c1).max() <= 1e-12 * np.abs(c0).max()
+                    tol = 1e-12 if flags == G.MWG_APPLY_FP64 else 1e-3 * prec
+                    assert np.abs(c0 - c1).max() <= tol * np.abs(c0).max()
     finally:
         ctx.set_tuning(*default)
 
@@ -42,7 +44,7 @@ def test_invalid_tuning_is_refused():
     from mrchem_amd import mrcpp as G
     ctx = G.default_context()
     before = ctx.get_tuning()
-    for fc, sc in [(0, 16), (64, 0), (5000, 16), (64, 2000)]:
+    for fc, sc in [(0, 16), (64, 0), (5000, 16), (64, 17)]:
         with pytest.raises(G.MwgError):
             ctx.set_tuning(fc, sc)
     assert ctx.get_tuning() == before
