@@ -23,7 +23,6 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
-#include <unordered_map>
 #include <vector>
 
 #include "../../include/mwg.h"
@@ -213,8 +212,10 @@ struct Ctx {
     DBuf<TermTab> tabs;
     DBuf<double> tbmax;
     DBuf<int> tpre, tpre0;
-    DBuf<TaskSummary> tsum;
-    DBuf<float> cfeat, cprob;  // predictor features and probabilities of the work nodes' children
+    // per-iteration read-back block, written by the kernels and copied to the host in
+    // one transfer: schedule summary | per-table counts | work norms | child
+    // probabilities | child features
+    DBuf<char> rbd;
     DBuf<char> iterbuf;  // per-iteration tables (work items, job tables, box requests)
     DBuf<SegRec> segs;
     DBuf<int> segcnt;        // per work-list segment: count over all parts
@@ -223,7 +224,6 @@ struct Ctx {
     DBuf<int> thread_off, gtotal;
     DBuf<long long> gofs;
     DBuf<Tuple> tuples;
-    DBuf<double> wnorms;
     DBuf<uint64_t> genKeys;
     DBuf<int> genJob, genCount, genOverflow;
     DBuf<double> genPool, genNorm;
@@ -299,18 +299,63 @@ struct MraH {
     ProjGeom geom() const { return ProjGeom{{lo(0), lo(1), lo(2)}, sfac, n0}; }
 };
 
+// open-addressing node index (slot key = node key + 1, 0 = empty; load at most 1/2):
+// the layout of the device tables, so that build_hash uploads it as it is
+struct NodeIndex {
+    std::vector<unsigned long long> keys;
+    std::vector<int> vals;
+    size_t count = 0;
+    void clear() {
+        keys.clear();
+        vals.clear();
+        count = 0;
+    }
+    uint32_t mask() const { return (uint32_t)keys.size() - 1u; }
+    void put(uint64_t k, int v) {
+        if (2 * (count + 1) > keys.size()) rehash(std::max<size_t>(1024, 2 * keys.size()));
+        const unsigned long long kk = (unsigned long long)k + 1ull;
+        uint32_t h = dkey_hash(k) & mask();
+        while (keys[h] != 0ull && keys[h] != kk) h = (h + 1) & mask();
+        if (keys[h] == 0ull) {
+            keys[h] = kk;
+            count++;
+        }
+        vals[h] = v;
+    }
+    int get(uint64_t k) const {
+        if (keys.empty()) return -1;
+        const unsigned long long kk = (unsigned long long)k + 1ull;
+        uint32_t h = dkey_hash(k) & mask();
+        while (true) {  // an empty slot always exists (load <= 1/2)
+            const unsigned long long v = keys[h];
+            if (v == kk) return vals[h];
+            if (v == 0ull) return -1;
+            h = (h + 1) & mask();
+        }
+    }
+    void rehash(size_t cap) {
+        std::vector<unsigned long long> ok;
+        std::vector<int> ov;
+        ok.swap(keys);
+        ov.swap(vals);
+        keys.assign(cap, 0ull);
+        vals.assign(cap, -1);
+        count = 0;
+        for (size_t i = 0; i < ok.size(); i++)
+            if (ok[i]) put(ok[i] - 1ull, ov[i]);
+    }
+};
+
 struct TreeH {
     MraH *mra = nullptr;
     std::vector<int> depth, parent, child, lr;  // lr: 3 per node
-    std::unordered_map<uint64_t, int> index;
+    NodeIndex index;
     std::vector<int> roots;
     std::vector<double> cnorm;  // 8 per node (host copy)
     double sqnorm = 0.0;
     DBuf<double> coef, norm;
     DBuf<unsigned long long> hkeys;
     DBuf<int> hvals;
-    std::vector<unsigned long long> hkeys_h;  // host copies of the hash table
-    std::vector<int> hvals_h;
     uint32_t hmask = 0;
     bool hvalid = false;
 
@@ -336,7 +381,7 @@ struct TreeH {
         lr.push_back(y);
         lr.push_back(z);
         cnorm.resize(8 * (size_t)(id + 1), 0.0);
-        index[node_key(d, x, y, z)] = id;
+        index.put(node_key(d, x, y, z), id);
         hvalid = false;
         return id;
     }
@@ -353,8 +398,7 @@ struct TreeH {
         child[i] = first;
     }
     int find(int d, int x, int y, int z) const {
-        auto it = index.find(node_key(d, x, y, z));
-        return it == index.end() ? -1 : it->second;
+        return index.get(node_key(d, x, y, z));
     }
     // grow the coefficient and norm arrays through the context's block cache (the old
     // contents are copied on the stream; no synchronisation)
@@ -384,37 +428,12 @@ struct TreeH {
     }
     void build_hash() {
         if (hvalid) return;
-        uint32_t cap = 1024;
-        while (cap < 2u * (uint32_t)n() + 16) cap <<= 1;
-        std::vector<unsigned long long> keys(cap, 0ull);
-        std::vector<int> vals(cap, -1);
-        for (int i = 0; i < n(); i++) {
-            const uint64_t k = node_key(depth[i], lr[3 * i], lr[3 * i + 1], lr[3 * i + 2]);
-            uint32_t h = dkey_hash(k) & (cap - 1);
-            while (keys[h] != 0ull) h = (h + 1) & (cap - 1);
-            keys[h] = k + 1ull;
-            vals[h] = i;
-        }
+        if (index.keys.empty()) index.rehash(1024);
         hipStream_t s = mra->ctx->stream;
-        hkeys.upload(keys.data(), cap, s);
-        hvals.upload(vals.data(), cap, s);
-        hmask = cap - 1;
-        hkeys_h.swap(keys);
-        hvals_h.swap(vals);
+        hkeys.upload(index.keys.data(), index.keys.size(), s);
+        hvals.upload(index.vals.data(), index.vals.size(), s);
+        hmask = index.mask();
         hvalid = true;
-    }
-    // lookup in the host copy of the open-addressing table (valid after build_hash)
-    int find_hashed(int d, int64_t x, int64_t y, int64_t z) const {
-        const uint64_t k = node_key(d, x, y, z);
-        const unsigned long long kk = (unsigned long long)k + 1ull;
-        uint32_t h = dkey_hash(k) & hmask;
-        for (uint32_t probe = 0; probe <= hmask; probe++) {
-            const unsigned long long v = hkeys_h[h];
-            if (v == kk) return hvals_h[h];
-            if (v == 0ull) return -1;
-            h = (h + 1) & hmask;
-        }
-        return -1;
     }
     DHash dhash() const { return DHash{hkeys.p, hvals.p, hmask}; }
     void refresh_norms() {  // device norms -> host, square norm
@@ -818,10 +837,7 @@ static void compact_tree(TreeH &T, const std::vector<char> &alive) {
     auto key_of = [&](int i) { return node_key(T.depth[i], T.lr[3 * i], T.lr[3 * i + 1], T.lr[3 * i + 2]); };
     std::vector<int> holes;
     for (int i = nroots; i < n; i += 8)
-        if (!alive[i]) {
-            holes.push_back(i);
-            for (int a = 0; a < 8; a++) T.index.erase(key_of(i + a));
-        }
+        if (!alive[i]) holes.push_back(i);
     auto move_group = [&](int from, int to) {
         const int p = T.parent[from];
         if (p >= 0) T.child[p] = to;
@@ -834,7 +850,6 @@ static void compact_tree(TreeH &T, const std::vector<char> &alive) {
             std::memcpy(&T.cnorm[8 * (size_t)t], &T.cnorm[8 * (size_t)f], 8 * sizeof(double));
             if (T.child[t] >= 0)
                 for (int b = 0; b < 8; b++) T.parent[T.child[t] + b] = t;
-            T.index[key_of(t)] = t;
         }
         HC(hipMemcpyAsync(T.coef.p + (size_t)to * nb, T.coef.p + (size_t)from * nb, 8 * nb * sizeof(double),
                           hipMemcpyDeviceToDevice, C.stream));
@@ -858,6 +873,8 @@ static void compact_tree(TreeH &T, const std::vector<char> &alive) {
     T.child.resize(n);
     T.lr.resize(3 * (size_t)n);
     T.cnorm.resize(8 * (size_t)n);
+    T.index.clear();
+    for (int i = 0; i < n; i++) T.index.put(key_of(i), i);
     T.hvalid = false;
 }
 
@@ -1091,8 +1108,18 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         C.segs.alloc((size_t)nw * 8);
         C.segcnt.alloc((size_t)nw * nseg);
         C.gslice.alloc((size_t)nw);
-        C.wnorms.alloc((size_t)nw * 8);
-        C.tsum.alloc(1);
+        const bool want_feat = use_pred || C.tracing;
+        const size_t o_tab = 256;
+        const size_t o_wn = (o_tab + 2 * jt.size() * sizeof(long long) + 255) & ~(size_t)255;
+        const size_t o_pr = (o_wn + (size_t)nw * 8 * sizeof(double) + 255) & ~(size_t)255;
+        const size_t o_ft = (o_pr + (use_pred ? (size_t)nw * 8 * sizeof(float) : 0) + 255) & ~(size_t)255;
+        const size_t rb_dev = o_ft + (want_feat ? (size_t)nw * 8 * MLP_FEATURES * sizeof(float) : 0);
+        const size_t rb_host = C.tracing ? rb_dev : o_ft;  // the features only when tracing
+        C.rbd.alloc(rb_dev);
+        TaskSummary *dsum = (TaskSummary *)C.rbd.p;
+        long long *dtab = (long long *)(C.rbd.p + o_tab);
+        double *dwn = (double *)(C.rbd.p + o_wn);
+        float *dprob = (float *)(C.rbd.p + o_pr), *dfeat = (float *)(C.rbd.p + o_ft);
         auto launch_count = [&]() {
             if (!mixed)
                 HC(launch_enum(s, 0, nw, nparts, dwork, djobs, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, nullptr,
@@ -1125,74 +1152,54 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             }
         };
         std::vector<double> wn((size_t)nw * 8);
-        std::vector<int> tot((size_t)nw * nseg * nparts);
-        const bool want_feat = use_pred || C.tracing;
+        std::vector<long long> tabcnt(2 * jt.size());
         std::vector<float> cprob(use_pred ? (size_t)nw * 8 : 0);
         std::vector<float> cfeat(C.tracing ? (size_t)nw * 8 * MLP_FEATURES : 0);
         TaskSummary hsum;
-        int hgen[2];
         auto launch_schedule = [&]() {
             ensure_caps(nullptr);
             HC(launch_build_tasks(s, nw, nseg, nparts, C.chunk, 4 * C.slice_chunk, C.gtotal.p, C.caps, C.genCount.p,
-                                  genPoolCap, C.gofs.p, C.tasks.p, C.stasks.p, C.reds.p, C.tsum.p, C.segs.p,
-                                  C.segcnt.p, C.gslice.p));
+                                  genPoolCap, C.gofs.p, C.tasks.p, C.stasks.p, C.reds.p, dsum, C.segs.p,
+                                  C.segcnt.p, C.gslice.p, C.genOverflow.p, dwork, dtab, (int)jt.size()));
             if (!mixed)
                 HC(launch_enum(s, 1, nw, nparts, dwork, djobs, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, C.gofs.p,
-                               C.tuples.p, C.tsum.p));
+                               C.tuples.p, dsum));
             else
                 HC(launch_enum_mixed(s, 1, nw, nparts, dwork, djobs, C.tabs.p, gs, C.thread_off.p, C.gtotal.p,
-                                     C.gofs.p, C.tuples.p, C.slices.p, C.tsum.p));
+                                     C.gofs.p, C.tuples.p, C.slices.p, dsum));
             const int grid64 = std::max(1, (C.caps.tasks + 3) / 4);
             const int gridred = std::min(4096, std::max(1, C.caps.reds));
             HC(hipEventRecord(C.ev0, s));
-            HC(launch_contract_fp64(s, K, 0, C.tsum.p, grid64, C.tasks.p, dwork, djobs, C.tabs.p, C.tuples.p,
+            HC(launch_contract_fp64(s, K, 0, dsum, grid64, C.tasks.p, dwork, djobs, C.tabs.p, C.tuples.p,
                                     C.genPool.p, C.partials.p));
             if (mixed) {
                 const int gridmf = std::max(1, C.caps.stasks);
-                HC(launch_contract_mfma(s, K, 0, C.tsum.p, gridmf, C.stasks.p, dwork, djobs, C.tabs.p,
+                HC(launch_contract_mfma(s, K, 0, dsum, gridmf, C.stasks.p, dwork, djobs, C.tabs.p,
                                         C.slices.p, C.genPool.p, C.nodeslots.p));
             }
-            HC(launch_reduce_partials(s, 0, C.tsum.p, gridred, C.reds.p, dwork, djobs, C.partials.p,
+            HC(launch_reduce_partials(s, 0, dsum, gridred, C.reds.p, dwork, djobs, C.partials.p,
                                       mixed ? C.nodeslots.p : nullptr, K3));
             HC(hipEventRecord(C.ev1, s));
-            HC(launch_work_norms(s, nw, dwork, djobs, K3, C.wnorms.p));
+            HC(launch_work_norms(s, nw, dwork, djobs, K3, dwn));
             // refinement predictor: features of every work node's children and their split
             // probabilities, read back with the norms (no extra round trip)
             if (want_feat) {
-                C.cfeat.alloc((size_t)nw * 8 * MLP_FEATURES);
-                HC(launch_child_features(s, nw, dwork, djobs, C.wnorms.p, C.cfeat.p));
-                if (use_pred) {
-                    C.cprob.alloc((size_t)nw * 8);
-                    HC(launch_mlp_forward(s, nw * 8, nullptr, C.cfeat.p, C.pred->mean.p, C.pred->sdev.p,
-                                          C.pred->P.p, C.cprob.p, nullptr, nullptr, nullptr));
-                }
+                HC(launch_child_features(s, nw, dwork, djobs, dwn, dfeat));
+                if (use_pred)
+                    HC(launch_mlp_forward(s, nw * 8, nullptr, dfeat, C.pred->mean.p, C.pred->sdev.p, C.pred->P.p,
+                                          dprob, nullptr, nullptr, nullptr));
             }
-            // one pinned read-back area: summary | generated count, overflow | counts | norms
-            // | child probabilities | child features
-            const size_t o_sum = 0, o_gen = 64, o_tot = 128;
-            const size_t o_wn = (o_tot + tot.size() * sizeof(int) + 255) & ~(size_t)255;
-            const size_t o_pr = (o_wn + wn.size() * sizeof(double) + 255) & ~(size_t)255;
-            const size_t o_ft = (o_pr + cprob.size() * sizeof(float) + 255) & ~(size_t)255;
-            C.rb.reserve(o_ft + cfeat.size() * sizeof(float));
+            C.rb.reserve(rb_host);
             char *h = C.rb.h;
-            if (use_pred)
-                HC(hipMemcpyAsync(h + o_pr, C.cprob.p, cprob.size() * sizeof(float), hipMemcpyDeviceToHost, s));
-            if (C.tracing)
-                HC(hipMemcpyAsync(h + o_ft, C.cfeat.p, cfeat.size() * sizeof(float), hipMemcpyDeviceToHost, s));
-            HC(hipMemcpyAsync(h + o_sum, C.tsum.p, sizeof(TaskSummary), hipMemcpyDeviceToHost, s));
-            HC(hipMemcpyAsync(h + o_gen, C.genCount.p, sizeof(int), hipMemcpyDeviceToHost, s));
-            HC(hipMemcpyAsync(h + o_gen + 4, C.genOverflow.p, sizeof(int), hipMemcpyDeviceToHost, s));
-            HC(hipMemcpyAsync(h + o_tot, C.gtotal.p, tot.size() * sizeof(int), hipMemcpyDeviceToHost, s));
-            HC(hipMemcpyAsync(h + o_wn, C.wnorms.p, wn.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+            HC(hipMemcpyAsync(h, C.rbd.p, rb_host, hipMemcpyDeviceToHost, s));
             stream_wait(C);
             C.stage.off = 0;
-            std::memcpy(&hsum, h + o_sum, sizeof(TaskSummary));
-            std::memcpy(hgen, h + o_gen, 2 * sizeof(int));
-            std::memcpy(tot.data(), h + o_tot, tot.size() * sizeof(int));
+            std::memcpy(&hsum, h, sizeof(TaskSummary));
+            std::memcpy(tabcnt.data(), h + o_tab, tabcnt.size() * sizeof(long long));
             std::memcpy(wn.data(), h + o_wn, wn.size() * sizeof(double));
             if (use_pred) std::memcpy(cprob.data(), h + o_pr, cprob.size() * sizeof(float));
             if (C.tracing) std::memcpy(cfeat.data(), h + o_ft, cfeat.size() * sizeof(float));
-            if (hgen[1]) throw Error(MWG_ERR_RANGE, "generated-node table overflow");
+            if (hsum.gen_overflow) throw Error(MWG_ERR_RANGE, "generated-node table overflow");
         };
         launch_count();
         PT.lap("enum_count");
@@ -1200,7 +1207,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         if (hsum.overflow) {
             if (hsum.overflow & 2) {
                 // finish the generated nodes past the pool, then count again
-                const int genNow = hgen[0];
+                const int genNow = hsum.gen_count;
                 const int want = std::min(genCap, genNow + std::max(4096, genNow / 2));
                 C.genPool.grow_keep((size_t)want * K3, s);
                 C.genNorm.grow_keep((size_t)want, s);
@@ -1215,14 +1222,11 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             launch_schedule();
             if (hsum.overflow) throw Error(MWG_ERR_RANGE, "work-list schedule sizing failed");
         }
-        genLastNew = hgen[0] - genDone;
-        genDone = hgen[0];
-        for (int i = 0; i < nw; i++) {
-            JobH &J = jobs[jt_job[work[i].job]];
-            for (int p = 0; p < nparts; p++) {
-                for (int q = 0; q < 8; q++) J.tuples += tot[((size_t)i * nseg + q) * nparts + p];
-                if (mixed) J.slices += tot[((size_t)i * 9 + 8) * nparts + p];
-            }
+        genLastNew = hsum.gen_count - genDone;
+        genDone = hsum.gen_count;
+        for (size_t q = 0; q < jt.size(); q++) {
+            jobs[jt_job[q]].tuples += tabcnt[2 * q];
+            jobs[jt_job[q]].slices += tabcnt[2 * q + 1];
         }
         PT.lap("worklist+launch");
         float ms = 0.0f;

@@ -67,6 +67,7 @@ struct TaskSummary {
     long long ntuples, nslices;
     int ntasks, nstasks, nreds, nslots, nnslots;
     int overflow;  // bit 0: a capacity below, bit 1: generated-node pool
+    int gen_count, gen_overflow;  // generated-node counter and table-overflow flag, for the host
 };
 struct TaskCaps {
     long long tuples, slices;
@@ -199,7 +200,8 @@ struct SegRec {
 hipError_t launch_build_tasks(hipStream_t s, int nw, int nseg, int nparts, int chunk, int wchunk, const int *tot,
                               TaskCaps caps, const int *gen_count, int gen_pcap, long long *ofs, ContractTask *tasks,
                               SliceTask *stasks, ReduceItem *reds, TaskSummary *sum, SegRec *segs, int *segcnt,
-                              long long *gslice);
+                              long long *gslice, const int *gen_overflow, const WorkItem *work, long long *tabcnt,
+                              int ntab);
 // refinement-predictor features of the 8 children of every work node (row i*8 + a, a the
 // child index), from the work nodes' norms just computed, their parents' norms and the
 // input tree around each child

@@ -1047,13 +1047,49 @@ __device__ inline double input_wnorm(const JobTab &J, int i) {
     return sqrt(a);
 }
 
-__global__ void k_child_features(int nw, const WorkItem *work, const JobTab *jobs, const double *wnorms,
-                                 float *feat) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nw * 8) return;
+// 32 lanes per child: lanes 0..25 look up the 26 same-level neighbours of the child
+// in the input tree, lane 26 the child itself, lane 27 its first child, lane 28 the
+// parent's wavelet norm; the features are then gathered with shuffles in the half-wave
+__global__ void __launch_bounds__(256) k_child_features(int nw, const WorkItem *work, const JobTab *jobs,
+                                                        const double *wnorms, float *feat) {
+    const int tt = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 5);
+    const int l = threadIdx.x & 31;
+    const bool valid = tt < nw * 8;
+    const int t = valid ? tt : nw * 8 - 1;
     const int i = t >> 3, a = t & 7;
     const WorkItem wi = work[i];
     const JobTab &J = jobs[wi.job];
+    const int d = J.depth + 1;
+    const int cx = 2 * wi.l[0] + (a & 1), cy = 2 * wi.l[1] + ((a >> 1) & 1), cz = 2 * wi.l[2] + ((a >> 2) & 1);
+    double v = 0.0;  // lanes < 26: neighbour wavelet norm; 26: own norm (-1: absent); 28: parent's
+    int found = 0;   // lanes 26 and 27: node present in the input tree
+    if (l < 26) {
+        const int c = l < 13 ? l : l + 1;  // 27 cells without the centre
+        const int x = cx + c % 3 - 1, y = cy + (c / 3) % 3 - 1, z = cz + c / 9 - 1;
+        if (x >= 0 && y >= 0 && z >= 0 && x < 2 * J.lmax[0] && y < 2 * J.lmax[1] && z < 2 * J.lmax[2]) {
+            const int j = dhash_find(J.inHash, dnode_key(d, x, y, z));
+            if (j >= 0) v = input_wnorm(J, j);
+        }
+    } else if (l == 26) {
+        const int i0 = dhash_find(J.inHash, dnode_key(d, cx, cy, cz));
+        found = i0 >= 0;
+        v = found ? input_wnorm(J, i0) : 0.0;
+    } else if (l == 27) {
+        found = dhash_find(J.inHash, dnode_key(d + 1, 2 * cx, 2 * cy, 2 * cz)) >= 0;
+    } else if (l == 28 && wi.parent >= 0) {
+        double wpp = 0.0;
+        for (int q = 1; q < 8; q++) {
+            const double u = J.outNorm[(size_t)wi.parent * 8 + q];
+            wpp += u * u;
+        }
+        v = sqrt(wpp);
+    }
+    double nb = l < 26 ? v : 0.0;
+#pragma unroll
+    for (int o = 16; o > 0; o >>= 1) nb = fmax(nb, __shfl_xor(nb, o, 32));
+    const double own = __shfl(v, 26, 32), wpp = __shfl(v, 28, 32);
+    const int has = __shfl(found, 26, 32), has2 = __shfl(found, 27, 32);
+    if (!valid || l != 0) return;
     const double *pn = wnorms + (size_t)i * 8;
     double w2 = 0.0;
     for (int q = 1; q < 8; q++) w2 += pn[q] * pn[q];
@@ -1062,37 +1098,15 @@ __global__ void k_child_features(int nw, const WorkItem *work, const JobTab *job
     const double eps2 = 2.0 * 2.220446049250313e-16;
     const double thr = fmax(eps2, J.prec * J.feat_tnorm * exp2(-0.5 * (np + 1)));
     const double thr_c = fmax(eps2, J.prec * J.feat_tnorm * exp2(-0.5 * (np + 2)));
-    double wpp = 0.0;
-    if (wi.parent >= 0) {
-        for (int q = 1; q < 8; q++) {
-            const double v = J.outNorm[(size_t)wi.parent * 8 + q];
-            wpp += v * v;
-        }
-        wpp = sqrt(wpp);
-    }
+    const double in_to_out = J.feat_tnorm / fmax(J.feat_innorm, 1e-300);
     float *f = feat + (size_t)t * 10;
     f[0] = feat_lg(wn / thr);
     f[1] = feat_lg(sn / thr_c);
     f[2] = wpp > 0.0 ? feat_lg(wn / wpp) : 0.0f;
     f[3] = feat_lg(wn / fmax(fabs(sn), 1e-300));
-    const int d = J.depth + 1;
-    const int cx = 2 * wi.l[0] + (a & 1), cy = 2 * wi.l[1] + ((a >> 1) & 1), cz = 2 * wi.l[2] + ((a >> 2) & 1);
-    const int lim[3] = {2 * J.lmax[0], 2 * J.lmax[1], 2 * J.lmax[2]};
-    const double in_to_out = J.feat_tnorm / fmax(J.feat_innorm, 1e-300);
-    const int i0 = dhash_find(J.inHash, dnode_key(d, cx, cy, cz));
-    f[4] = i0 >= 0 ? 1.0f : 0.0f;
-    f[5] = i0 >= 0 ? feat_lg(input_wnorm(J, i0) * in_to_out / thr_c) : -20.0f;
-    f[6] = (i0 >= 0 && dhash_find(J.inHash, dnode_key(d + 1, 2 * cx, 2 * cy, 2 * cz)) >= 0) ? 1.0f : 0.0f;
-    double nb = 0.0;
-    for (int dz = -1; dz <= 1; dz++)
-        for (int dy = -1; dy <= 1; dy++)
-            for (int dx = -1; dx <= 1; dx++) {
-                if (!dx && !dy && !dz) continue;
-                const int x = cx + dx, y = cy + dy, z = cz + dz;
-                if (x < 0 || y < 0 || z < 0 || x >= lim[0] || y >= lim[1] || z >= lim[2]) continue;
-                const int j = dhash_find(J.inHash, dnode_key(d, x, y, z));
-                if (j >= 0) nb = fmax(nb, input_wnorm(J, j));
-            }
+    f[4] = has ? 1.0f : 0.0f;
+    f[5] = has ? feat_lg(own * in_to_out / thr_c) : -20.0f;
+    f[6] = (has && has2) ? 1.0f : 0.0f;
     f[7] = nb > 0.0 ? feat_lg(nb * in_to_out / thr_c) : -20.0f;
     f[8] = J.feat_mu;
     f[9] = J.feat_cell;
@@ -1102,7 +1116,7 @@ hipError_t launch_child_features(hipStream_t s, int nw, const WorkItem *work, co
                                  const double *wnorms, float *feat) {
     if (nw <= 0) return hipSuccess;
     const int n = nw * 8;
-    hipLaunchKernelGGL(k_child_features, dim3((n + 255) / 256), dim3(256), 0, s, nw, work, jobs, wnorms, feat);
+    hipLaunchKernelGGL(k_child_features, dim3((n + 7) / 8), dim3(256), 0, s, nw, work, jobs, wnorms, feat);
     return hipGetLastError();
 }
 
@@ -1152,7 +1166,9 @@ __global__ void __launch_bounds__(1024) k_build_tasks(int nw, int nseg, int npar
                                                       const int *tot, TaskCaps caps, const int *gen_count,
                                                       int gen_pcap, long long *ofs, ContractTask *tasks,
                                                       SliceTask *stasks, ReduceItem *reds, TaskSummary *sum,
-                                                      SegRec *segs, int *segcnt, long long *gslice) {
+                                                      SegRec *segs, int *segcnt, long long *gslice,
+                                                      const int *gen_overflow, const WorkItem *work,
+                                                      long long *tabcnt, int ntab) {
     __shared__ long long wsum[17];
     __shared__ int big[BT_LIST];     // segment ids (>= 0) or -(node + 1) for slice runs
     __shared__ int nbig;
@@ -1162,6 +1178,8 @@ __global__ void __launch_bounds__(1024) k_build_tasks(int nw, int nseg, int npar
     auto ntasks_of = [&](int c) { return mixed ? (c + chunk - 1) / chunk : (c <= chunk ? 1 : (c + chunk - 1) / chunk); };
     auto nslots_of = [&](int c) { return mixed ? (c + chunk - 1) / chunk : (c <= chunk ? 0 : (c + chunk - 1) / chunk); };
     if (threadIdx.x == 0) nbig = 0;
+    // per job table: tuples and slices of its output nodes (statistics for the host)
+    for (int q = threadIdx.x; q < 2 * ntab; q += blockDim.x) tabcnt[q] = 0;
     // phase 0: segment counts (sum over the parts) and per-part offsets inside the segment
     const long long nsg = (long long)nw * nseg;
     for (long long sg = threadIdx.x; sg < nsg; sg += blockDim.x) {
@@ -1212,6 +1230,8 @@ __global__ void __launch_bounds__(1024) k_build_tasks(int nw, int nseg, int npar
         sum->nslots = (int)Tslots;
         sum->nnslots = (int)Tnst;
         sum->overflow = overflow;
+        sum->gen_count = *(volatile const int *)gen_count;
+        sum->gen_overflow = *(volatile const int *)gen_overflow;
     }
     if (overflow) return;
     auto emit_task = [&](const SegRec &r, int sgi, int h) {
@@ -1225,6 +1245,13 @@ __global__ void __launch_bounds__(1024) k_build_tasks(int nw, int nseg, int npar
     };
     // phase 2: segment records, reduce items, tasks of small segments
     for (int g = g0; g < g1; g++) {
+        {
+            long long nt_g = 0;
+            for (int q = 0; q < 8; q++) nt_g += segcnt[(size_t)g * nseg + q];
+            const int jt = work[g].job;
+            atomicAdd((unsigned long long *)&tabcnt[2 * jt], (unsigned long long)nt_g);
+            if (mixed) atomicAdd((unsigned long long *)&tabcnt[2 * jt + 1], (unsigned long long)segcnt[(size_t)g * 9 + 8]);
+        }
         int nn0 = (int)bnst, nns = 0;
         if (mixed) {
             const int cs = segcnt[(size_t)g * 9 + 8];
@@ -2150,9 +2177,11 @@ hipError_t launch_transform(hipStream_t s, int mode, int n, const TransformItem 
 hipError_t launch_build_tasks(hipStream_t s, int nw, int nseg, int nparts, int chunk, int wchunk, const int *tot,
                               TaskCaps caps, const int *gen_count, int gen_pcap, long long *ofs, ContractTask *tasks,
                               SliceTask *stasks, ReduceItem *reds, TaskSummary *sum, SegRec *segs, int *segcnt,
-                              long long *gslice) {
+                              long long *gslice, const int *gen_overflow, const WorkItem *work, long long *tabcnt,
+                              int ntab) {
     hipLaunchKernelGGL(k_build_tasks, dim3(1), dim3(1024), 0, s, nw, nseg, nparts, chunk, wchunk, tot, caps,
-                       gen_count, gen_pcap, ofs, tasks, stasks, reds, sum, segs, segcnt, gslice);
+                       gen_count, gen_pcap, ofs, tasks, stasks, reds, sum, segs, segcnt, gslice, gen_overflow, work,
+                       tabcnt, ntab);
     return hipGetLastError();
 }
 

@@ -6,6 +6,7 @@
 // Parameter vector layout: W1[64][NF] | b1[64] | W2[64][64] | b2[64] | w3[64] | b3.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "mwg_predict.h"
@@ -18,37 +19,59 @@ constexpr int OW1 = 0, OB1 = OW1 + H * NF, OW2 = OB1 + H, OB2 = OW2 + H * H, OW3
 static_assert(OB3 + 1 == MLP_PARAMS, "parameter layout");
 
 // forward for the samples idx[0..n) (idx null: 0..n); stores normalised inputs and
-// both hidden activations when the buffers are given (training)
-__global__ void k_mlp_forward(int n, const int *idx, const float *X, const float *mean, const float *sdev,
-                              const float *P, float *prob, float *Xn, float *H1, float *H2) {
+// both hidden activations when the buffers are given (training).  One wave per
+// sample: lane o computes hidden unit o of both layers (the layer inputs through the
+// wave's LDS row), lane 0 the output; every sum runs over its inputs in index order.
+constexpr int FWD_WAVES = 4;
+__global__ void __launch_bounds__(64 * FWD_WAVES) k_mlp_forward(int n, const int *idx, const float *X,
+                                                                const float *mean, const float *sdev,
+                                                                const float *P, float *prob, float *Xn, float *H1,
+                                                                float *H2) {
+    static_assert(H == 64, "one lane per hidden unit");
+    // parameters with the weight matrices transposed (input-major: the lanes of a wave
+    // read consecutive words)
     __shared__ float sp[MLP_PARAMS];
-    for (int e = threadIdx.x; e < MLP_PARAMS; e += blockDim.x) sp[e] = P[e];
+    __shared__ float sx[FWD_WAVES][NF], sh[FWD_WAVES][H];
+    for (int e = threadIdx.x; e < MLP_PARAMS; e += blockDim.x) {
+        int src = e;
+        if (e < OB1) src = OW1 + (e % H) * NF + e / H;                   // W1T[i][o] = W1[o][i]
+        else if (e >= OW2 && e < OB2) src = OW2 + ((e - OW2) % H) * H + (e - OW2) / H;  // W2T[i][o]
+        sp[e] = P[src];
+    }
     __syncthreads();
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const int r = idx ? idx[s] : s;
-    float x[NF];
-    for (int i = 0; i < NF; i++) x[i] = (X[(size_t)r * NF + i] - mean[i]) / sdev[i];
-    float h1[H], h2[H];
-    for (int o = 0; o < H; o++) {
-        float z = sp[OB1 + o];
-        for (int i = 0; i < NF; i++) z += sp[OW1 + o * NF + i] * x[i];
-        h1[o] = tanhf(z);
-    }
-    for (int o = 0; o < H; o++) {
-        float z = sp[OB2 + o];
-        for (int i = 0; i < H; i++) z += sp[OW2 + o * H + i] * h1[i];
-        h2[o] = tanhf(z);
-    }
-    float z = sp[OB3];
-    for (int i = 0; i < H; i++) z += sp[OW3 + i] * h2[i];
-    prob[s] = 1.0f / (1.0f + expf(-z));
-    if (Xn) {
-        for (int i = 0; i < NF; i++) Xn[(size_t)s * NF + i] = x[i];
-        for (int i = 0; i < H; i++) {
-            H1[(size_t)s * H + i] = h1[i];
-            H2[(size_t)s * H + i] = h2[i];
+    const int wave = threadIdx.x >> 6, o = threadIdx.x & 63;
+    for (int s = blockIdx.x * FWD_WAVES + wave; s < n; s += gridDim.x * FWD_WAVES) {
+        const int r = idx ? idx[s] : s;
+        if (o < NF) {
+            const float xv = (X[(size_t)r * NF + o] - mean[o]) / sdev[o];
+            sx[wave][o] = xv;
+            if (Xn) Xn[(size_t)s * NF + o] = xv;
         }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        float z = sp[OB1 + o];
+        for (int i = 0; i < NF; i++) z += sp[OW1 + i * H + o] * sx[wave][i];
+        const float h1 = tanhf(z);
+        sh[wave][o] = h1;
+        if (H1) H1[(size_t)s * H + o] = h1;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        z = sp[OB2 + o];
+        for (int i = 0; i < H; i++) z += sp[OW2 + i * H + o] * sh[wave][i];
+        const float h2 = tanhf(z);
+        if (H2) H2[(size_t)s * H + o] = h2;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        sh[wave][o] = h2;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (o == 0) {
+            float z3 = sp[OB3];
+            for (int i = 0; i < H; i++) z3 += sp[OW3 + i] * sh[wave][i];
+            prob[s] = 1.0f / (1.0f + expf(-z3));
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
 }
 
@@ -119,8 +142,9 @@ __global__ void k_adam(float *P, const float *G, float *m, float *v, float lr, f
 hipError_t launch_mlp_forward(hipStream_t s, int n, const int *idx, const float *X, const float *mean,
                               const float *sdev, const float *P, float *prob, float *Xn, float *H1, float *H2) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_mlp_forward, dim3((n + 127) / 128), dim3(128), 0, s, n, idx, X, mean, sdev, P, prob, Xn, H1,
-                       H2);
+    const int blocks = std::min((n + FWD_WAVES - 1) / FWD_WAVES, 512);
+    hipLaunchKernelGGL(k_mlp_forward, dim3(blocks), dim3(64 * FWD_WAVES), 0, s, n, idx, X, mean, sdev, P, prob, Xn,
+                       H1, H2);
     return hipGetLastError();
 }
 
