@@ -1177,10 +1177,11 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                 HC(launch_contract_mfma(s, K, 0, dsum, gridmf, C.stasks.p, dwork, djobs, C.tabs.p,
                                         C.slices.p, C.genPool.p, C.nodeslots.p));
             }
+            // (mixed: the reduce items cover every output component and write the norms)
             HC(launch_reduce_partials(s, 0, dsum, gridred, C.reds.p, dwork, djobs, C.partials.p,
-                                      mixed ? C.nodeslots.p : nullptr, K3));
+                                      mixed ? C.nodeslots.p : nullptr, K3, mixed ? dwn : nullptr));
             HC(hipEventRecord(C.ev1, s));
-            HC(launch_work_norms(s, nw, dwork, djobs, K3, dwn));
+            if (!mixed) HC(launch_work_norms(s, nw, dwork, djobs, K3, dwn));
             // refinement predictor: features of every work node's children and their split
             // probabilities, read back with the norms (no extra round trip)
             if (want_feat) {

@@ -39,6 +39,7 @@ __device__ inline int dhash_find(const DHash &h, uint64_t key) {
 }
 
 // one active Gaussian term at the current depth, as the work-list build sees it
+constexpr int BM = 16;  // translations |d| < BM have precomputed combination masks
 struct TermTab {
     double w;          // weight (unit-coordinate, includes scaling factor^3)
     long long blk;     // offset (doubles) of delta=-band, type 0 block in the level block array
@@ -47,6 +48,9 @@ struct TermTab {
     short ub[3];       // union box half-widths over all combos (-1: term unused)
     short ub0[3];      // union box over combos with ft = 0
     short box[64][3];  // per combo (gt*8+ft) half-widths, -1 = combo screened out entirely
+    // bm[q][m]: bit c set iff combo c is active and box[c][q] >= m; the combos whose box
+    // holds delta are bm[0][|dx|] & bm[1][|dy|] & bm[2][|dz|]
+    unsigned long long bm[3][BM];
 };
 
 // one surviving (g, f, ft, gt, term) tuple

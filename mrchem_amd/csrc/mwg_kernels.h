@@ -213,6 +213,6 @@ hipError_t launch_contract_fp64(hipStream_t s, int K, int ntask, const TaskSumma
                                 const TermTab *tabs, const Tuple *tuples, const double *genPool, double *partials);
 hipError_t launch_reduce_partials(hipStream_t s, int n, const TaskSummary *sum, int grid, const ReduceItem *items,
                                   const WorkItem *work, const JobTab *jobs, const double *partials,
-                                  const double *nodeslots, int K3);
+                                  const double *nodeslots, int K3, double *wnorm = nullptr);
 
 }  // namespace mwg

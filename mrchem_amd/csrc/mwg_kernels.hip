@@ -491,6 +491,17 @@ __device__ inline bool in_box(const short *b, int dx, int dy, int dz) {
     return b[0] >= 0 && abs(dx) <= b[0] && abs(dy) <= b[1] && abs(dz) <= b[2];
 }
 
+// bit c (= gt * 8 + ft) set iff the screening box of combo c holds (dx, dy, dz)
+__device__ inline unsigned long long combo_mask(const TermTab &tb, int dx, int dy, int dz) {
+    const int ax = abs(dx), ay = abs(dy), az = abs(dz);
+    if (ax < BM && ay < BM && az < BM) return tb.bm[0][ax] & tb.bm[1][ay] & tb.bm[2][az];
+    unsigned long long m = 0;
+    for (int c = 0; c < 64; c++)
+        if (in_box(tb.box[c], dx, dy, dz)) m |= 1ull << c;
+    return m;
+}
+constexpr unsigned long long FT0_COMBOS = 0x0101010101010101ull;  // the combos with ft = 0
+
 // mark the input nodes that the output nodes need at this depth but the input
 // tree does not have (scaling-only "generated" nodes)
 __global__ void __launch_bounds__(256) k_gen_mark(int nw, const WorkItem *work, const JobTab *jobs,
@@ -516,9 +527,7 @@ __global__ void __launch_bounds__(256) k_gen_mark(int nw, const WorkItem *work, 
         const int dz = c / ((2 * ux + 1) * (2 * uy + 1)) - uz;
         const int fx = wi.l[0] - dx, fy = wi.l[1] - dy, fz = wi.l[2] - dz;
         if (fx < 0 || fy < 0 || fz < 0 || fx >= J.lmax[0] || fy >= J.lmax[1] || fz >= J.lmax[2]) continue;
-        bool need = false;
-        for (int gt = 0; gt < 8 && !need; gt++) need = in_box(tb.box[gt * 8], dx, dy, dz);
-        if (!need) continue;
+        if (!(combo_mask(tb, dx, dy, dz) & FT0_COMBOS)) continue;
         const uint64_t key = dnode_key(J.depth, fx, fy, fz);
         if (dhash_find(J.inHash, key) >= 0) continue;
         // insert into this job's generated-node table
@@ -870,6 +879,10 @@ __global__ void __launch_bounds__(64) k_job_boxes(int nbj, const BoxJob *bjs, Te
         tb->box[c][q] = v;
         u[q] = wave_max_int(v);
         u0[q] = wave_max_int(ft == 0 ? v : -1);
+        for (int m = 0; m < BM; m++) {
+            const unsigned long long b = __ballot(ok && m3[q] >= m);
+            if (c == 0) tb->bm[q][m] = b;
+        }
     }
     const double bmax = wave_max_double(full);
     if (c == 0) {
@@ -978,10 +991,13 @@ __global__ void __launch_bounds__(256) k_enum(int nw, int nparts, const WorkItem
                     [&](int t, const TermTab &tb, int dx, int dy, int dz, int f, const double *fn, int nft,
                         const double *nx, const double *ny, const double *nz) {
                         const double aw = fabs(tb.w);
+                        const unsigned long long cm = combo_mask(tb, dx, dy, dz);
                         for (int ft = 0; ft < nft; ft++) {
                             const double fv = fn[ft];
+                            const unsigned long long cf = (cm >> ft) & FT0_COMBOS;  // bit 8 gt
+                            if (!cf) continue;
                             for (int gt = 0; gt < 8; gt++) {
-                                if (!in_box(tb.box[gt * 8 + ft], dx, dy, dz)) continue;
+                                if (!((cf >> (8 * gt)) & 1ull)) continue;
                                 const int tx = 2 * (gt & 1) + (ft & 1);
                                 const int ty = 2 * ((gt >> 1) & 1) + ((ft >> 1) & 1);
                                 const int tz = 2 * ((gt >> 2) & 1) + ((ft >> 2) & 1);
@@ -1160,7 +1176,7 @@ __device__ inline long long block_excl_scan_1024(long long v, long long *wsum, l
 // nodes' slice runs) with many tasks go to a list in LDS.  Phase 3: all threads write the
 // tasks of the listed segments.  Every task has a fixed position, so the schedule does
 // not depend on the order of the list.
-constexpr int BT_FEW = 32, BT_LIST = 2048;
+constexpr int BT_FEW = 32, BT_LIST = 2048, BT_TABS = 256;
 
 __global__ void __launch_bounds__(1024) k_build_tasks(int nw, int nseg, int nparts, int chunk, int wchunk,
                                                       const int *tot, TaskCaps caps, const int *gen_count,
@@ -1178,8 +1194,11 @@ __global__ void __launch_bounds__(1024) k_build_tasks(int nw, int nseg, int npar
     auto ntasks_of = [&](int c) { return mixed ? (c + chunk - 1) / chunk : (c <= chunk ? 1 : (c + chunk - 1) / chunk); };
     auto nslots_of = [&](int c) { return mixed ? (c + chunk - 1) / chunk : (c <= chunk ? 0 : (c + chunk - 1) / chunk); };
     if (threadIdx.x == 0) nbig = 0;
-    // per job table: tuples and slices of its output nodes (statistics for the host)
+    // per job table: tuples and slices of its output nodes (statistics for the host),
+    // summed in LDS for the first BT_TABS tables, in global memory beyond
+    __shared__ unsigned long long lcnt[2 * BT_TABS];
     for (int q = threadIdx.x; q < 2 * ntab; q += blockDim.x) tabcnt[q] = 0;
+    for (int q = threadIdx.x; q < 2 * BT_TABS; q += blockDim.x) lcnt[q] = 0;
     // phase 0: segment counts (sum over the parts) and per-part offsets inside the segment
     const long long nsg = (long long)nw * nseg;
     for (long long sg = threadIdx.x; sg < nsg; sg += blockDim.x) {
@@ -1244,13 +1263,28 @@ __global__ void __launch_bounds__(1024) k_build_tasks(int nw, int nseg, int npar
         stasks[nn0 + h] = SliceTask{gslice[g] + (long long)h * wchunk, min(wchunk, cs - h * wchunk), g, nn0 + h};
     };
     // phase 2: segment records, reduce items, tasks of small segments
+    int run_jt = -1;  // table statistics: one add per run of nodes of the same table
+    unsigned long long run_t = 0, run_s = 0;
+    auto flush_run = [&]() {
+        if (run_jt < 0) return;
+        if (run_jt < BT_TABS) {
+            atomicAdd(&lcnt[2 * run_jt], run_t);
+            atomicAdd(&lcnt[2 * run_jt + 1], run_s);
+        } else {
+            atomicAdd((unsigned long long *)&tabcnt[2 * run_jt], run_t);
+            atomicAdd((unsigned long long *)&tabcnt[2 * run_jt + 1], run_s);
+        }
+    };
     for (int g = g0; g < g1; g++) {
         {
-            long long nt_g = 0;
-            for (int q = 0; q < 8; q++) nt_g += segcnt[(size_t)g * nseg + q];
             const int jt = work[g].job;
-            atomicAdd((unsigned long long *)&tabcnt[2 * jt], (unsigned long long)nt_g);
-            if (mixed) atomicAdd((unsigned long long *)&tabcnt[2 * jt + 1], (unsigned long long)segcnt[(size_t)g * 9 + 8]);
+            if (jt != run_jt) {
+                flush_run();
+                run_jt = jt;
+                run_t = run_s = 0;
+            }
+            for (int q = 0; q < 8; q++) run_t += (unsigned long long)segcnt[(size_t)g * nseg + q];
+            if (mixed) run_s += (unsigned long long)segcnt[(size_t)g * 9 + 8];
         }
         int nn0 = (int)bnst, nns = 0;
         if (mixed) {
@@ -1292,7 +1326,9 @@ __global__ void __launch_bounds__(1024) k_build_tasks(int nw, int nseg, int npar
             btup += c;
         }
     }
+    flush_run();
     __syncthreads();
+    for (int q = threadIdx.x; q < 2 * min(ntab, BT_TABS); q += blockDim.x) tabcnt[q] = (long long)lcnt[q];
     // phase 3: absolute per-part offsets (segment base + relative), then the listed
     // segments, one wave per segment with the lanes over its tasks
     for (long long sg = threadIdx.x; sg < nsg; sg += blockDim.x) {
@@ -1430,19 +1466,45 @@ __global__ void __launch_bounds__(256) k_contract_fp64(int ntask_arg, const Task
 
 // sums, per segment (g, gt), the FP64 tuple slots and the gt part of the mixed
 // node slots in slot order (deterministic)
-__global__ void k_reduce_partials(int n_arg, const TaskSummary *sum, const ReduceItem *items, const WorkItem *work,
-                                  const JobTab *jobs, const double *partials, const double *nodeslots, int K3) {
+// sum over the block of one value per thread, same order every call (all threads get it)
+__device__ inline double block_sum_256(double v, double *wsum) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
+}
+
+// Sums the partial results of an output component in a fixed order.  With wnorm
+// (mixed path: every component of every work node has an item) the component's norm
+// goes to the tree and to the compact per-work-node array as well.
+__global__ void __launch_bounds__(256) k_reduce_partials(int n_arg, const TaskSummary *sum, const ReduceItem *items,
+                                                         const WorkItem *work, const JobTab *jobs,
+                                                         const double *partials, const double *nodeslots, int K3,
+                                                         double *wnorm) {
+    __shared__ double wsum[4];
     const int n = sum ? (sum->overflow ? 0 : sum->nreds) : n_arg;
     for (int i = blockIdx.x; i < n; i += gridDim.x) {
         const ReduceItem it = items[i];
         const WorkItem wi = work[it.g];
         const JobTab &J = jobs[wi.job];
         double *dst = J.outCoef + (size_t)wi.node * 8 * K3 + it.gt * K3;
+        double ss = 0.0;
         for (int e = threadIdx.x; e < K3; e += blockDim.x) {
             double s = 0.0;
             for (int q = 0; q < it.nslots; q++) s += partials[(size_t)(it.slot0 + q) * K3 + e];
             for (int q = 0; q < it.nnslots; q++) s += nodeslots[((size_t)(it.nslot0 + q) * 8 + it.gt) * K3 + e];
             dst[e] = s;
+            ss += s * s;
+        }
+        if (wnorm) {
+            ss = block_sum_256(ss, wsum);
+            if (threadIdx.x == 0) {
+                const double v = sqrt(ss);
+                J.outNorm[(size_t)wi.node * 8 + it.gt] = v;
+                wnorm[(size_t)it.g * 8 + it.gt] = v;
+            }
         }
     }
 }
@@ -1480,12 +1542,15 @@ __global__ void __launch_bounds__(256) k_enum_mixed(int nw, int nparts, const Wo
                     [&](int t, const TermTab &tb, int dx, int dy, int dz, int f, const double *fn, int nft,
                         const double *nx, const double *ny, const double *nz) {
                         const double aw = fabs(tb.w);
+                        const unsigned long long cm = combo_mask(tb, dx, dy, dz);
                         for (int ft = 0; ft < nft; ft++) {
                             const double fv = fn[ft];
                             int surv = 0;
                             double maxb = 0.0;
+                            const unsigned long long cf = (cm >> ft) & FT0_COMBOS;  // bit 8 gt
+                            if (!cf) continue;
                             for (int gt = 0; gt < 8; gt++) {
-                                if (!in_box(tb.box[gt * 8 + ft], dx, dy, dz)) continue;
+                                if (!((cf >> (8 * gt)) & 1ull)) continue;
                                 const int tx = 2 * (gt & 1) + (ft & 1);
                                 const int ty = 2 * ((gt >> 1) & 1) + ((ft >> 1) & 1);
                                 const int tz = 2 * ((gt >> 2) & 1) + ((ft >> 2) & 1);
@@ -1808,16 +1873,6 @@ __global__ void k_work_norms(int nw, const WorkItem *work, const JobTab *jobs, i
             compact[(size_t)i * 8 + c] = v;
         }
     }
-}
-
-// sum over the block of one value per thread, same order every call (all threads get it)
-__device__ inline double block_sum_256(double v, double *wsum) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
-    __syncthreads();
-    return (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
 }
 
 // batched two-scale transforms over (tree, parent) items:
@@ -2197,11 +2252,11 @@ hipError_t launch_contract_fp64(hipStream_t s, int K, int ntask, const TaskSumma
 
 hipError_t launch_reduce_partials(hipStream_t s, int n, const TaskSummary *sum, int grid, const ReduceItem *items,
                                   const WorkItem *work, const JobTab *jobs, const double *partials,
-                                  const double *nodeslots, int K3) {
+                                  const double *nodeslots, int K3, double *wnorm) {
     const int nb = sum ? grid : n;
     if (nb <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_reduce_partials, dim3(nb), dim3(256), 0, s, n, sum, items, work, jobs, partials, nodeslots,
-                       K3);
+                       K3, wnorm);
     return hipGetLastError();
 }
 
