@@ -216,6 +216,7 @@ struct Ctx {
     // one transfer: schedule summary | per-table counts | work norms | child
     // probabilities | child features
     DBuf<char> rbd;
+    DBuf<int> moves;  // tree compaction: (from, to) node-group pairs
     DBuf<char> iterbuf;  // per-iteration tables (work items, job tables, box requests)
     DBuf<SegRec> segs;
     DBuf<int> segcnt;        // per work-list segment: count over all parts
@@ -838,6 +839,7 @@ static void compact_tree(TreeH &T, const std::vector<char> &alive) {
     std::vector<int> holes;
     for (int i = nroots; i < n; i += 8)
         if (!alive[i]) holes.push_back(i);
+    std::vector<int> moves;  // (from, to) pairs, copied on the device by one launch
     auto move_group = [&](int from, int to) {
         const int p = T.parent[from];
         if (p >= 0) T.child[p] = to;
@@ -851,10 +853,8 @@ static void compact_tree(TreeH &T, const std::vector<char> &alive) {
             if (T.child[t] >= 0)
                 for (int b = 0; b < 8; b++) T.parent[T.child[t] + b] = t;
         }
-        HC(hipMemcpyAsync(T.coef.p + (size_t)to * nb, T.coef.p + (size_t)from * nb, 8 * nb * sizeof(double),
-                          hipMemcpyDeviceToDevice, C.stream));
-        HC(hipMemcpyAsync(T.norm.p + (size_t)to * 8, T.norm.p + (size_t)from * 8, 64 * sizeof(double),
-                          hipMemcpyDeviceToDevice, C.stream));
+        moves.push_back(from);
+        moves.push_back(to);
     };
     std::vector<char> dead_at((size_t)n, 0);
     for (int h : holes) dead_at[h] = 1;
@@ -867,6 +867,10 @@ static void compact_tree(TreeH &T, const std::vector<char> &alive) {
         dead_at[h] = 0;
         n -= 8;
         hi++;
+    }
+    if (!moves.empty()) {
+        upload_async(C, C.moves, moves.data(), moves.size());
+        HC(launch_move_groups(C.stream, (int)moves.size() / 2, C.moves.p, T.coef.p, T.norm.p, (int)nb));
     }
     T.depth.resize(n);
     T.parent.resize(n);

@@ -211,6 +211,7 @@ hipError_t launch_child_features(hipStream_t s, int nw, const WorkItem *work, co
 hipError_t launch_contract_fp64(hipStream_t s, int K, int ntask, const TaskSummary *sum, int grid,
                                 const ContractTask *tasks, const WorkItem *work, const JobTab *jobs,
                                 const TermTab *tabs, const Tuple *tuples, const double *genPool, double *partials);
+hipError_t launch_move_groups(hipStream_t s, int npairs, const int *from_to, double *coef, double *norm, int nb);
 hipError_t launch_reduce_partials(hipStream_t s, int n, const TaskSummary *sum, int grid, const ReduceItem *items,
                                   const WorkItem *work, const JobTab *jobs, const double *partials,
                                   const double *nodeslots, int K3, double *wnorm = nullptr);

@@ -1929,6 +1929,26 @@ __global__ void __launch_bounds__(256) k_transform(int mode, int n, const Transf
 }
 
 
+// ----------------------------------------------------------------- tree compaction
+
+// node groups (8 siblings) moved inside a tree: pair i copies the group at node
+// from_to[2i] to node from_to[2i+1], one block per node of the group (nb coefficient
+// doubles and 8 norms per node); no source of a pair is the destination of another
+__global__ void __launch_bounds__(256) k_move_groups(int npairs, const int *from_to, double *coef, double *norm,
+                                                     int nb) {
+    const int i = blockIdx.x, a = blockIdx.y;
+    if (i >= npairs) return;
+    const size_t f = (size_t)from_to[2 * i] + a, t = (size_t)from_to[2 * i + 1] + a;
+    for (int e = threadIdx.x; e < nb; e += blockDim.x) coef[t * nb + e] = coef[f * nb + e];
+    if (threadIdx.x < 8) norm[t * 8 + threadIdx.x] = norm[f * 8 + threadIdx.x];
+}
+
+hipError_t launch_move_groups(hipStream_t s, int npairs, const int *from_to, double *coef, double *norm, int nb) {
+    if (npairs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_move_groups, dim3(npairs, 8), dim3(256), 0, s, npairs, from_to, coef, norm, nb);
+    return hipGetLastError();
+}
+
 // ----------------------------------------------------------------- pointwise multiplication
 
 // Loads the representation of an input tree at an output node into T (interleaved
