@@ -1178,13 +1178,15 @@ __device__ inline long long block_excl_scan_1024(long long v, long long *wsum, l
 // not depend on the order of the list.
 constexpr int BT_FEW = 32, BT_LIST = 2048, BT_TABS = 256;
 
-__global__ void __launch_bounds__(1024) k_build_tasks(int nw, int nseg, int nparts, int chunk, int wchunk,
-                                                      const int *tot, TaskCaps caps, const int *gen_count,
-                                                      int gen_pcap, long long *ofs, ContractTask *tasks,
-                                                      SliceTask *stasks, ReduceItem *reds, TaskSummary *sum,
-                                                      SegRec *segs, int *segcnt, long long *gslice,
-                                                      const int *gen_overflow, const WorkItem *work,
-                                                      long long *tabcnt, int ntab) {
+// (one workgroup: the pointers never alias, so that loads of later iterations can be
+// issued ahead of the stores of earlier ones)
+__global__ void __launch_bounds__(1024)
+    k_build_tasks(int nw, int nseg, int nparts, int chunk, int wchunk, const int *__restrict__ tot, TaskCaps caps,
+                  const int *__restrict__ gen_count, int gen_pcap, long long *__restrict__ ofs,
+                  ContractTask *__restrict__ tasks, SliceTask *__restrict__ stasks, ReduceItem *__restrict__ reds,
+                  TaskSummary *__restrict__ sum, SegRec *__restrict__ segs, int *__restrict__ segcnt,
+                  long long *__restrict__ gslice, const int *__restrict__ gen_overflow,
+                  const WorkItem *__restrict__ work, long long *__restrict__ tabcnt, int ntab) {
     __shared__ long long wsum[17];
     __shared__ int big[BT_LIST];     // segment ids (>= 0) or -(node + 1) for slice runs
     __shared__ int nbig;
@@ -1201,6 +1203,7 @@ __global__ void __launch_bounds__(1024) k_build_tasks(int nw, int nseg, int npar
     for (int q = threadIdx.x; q < 2 * BT_TABS; q += blockDim.x) lcnt[q] = 0;
     // phase 0: segment counts (sum over the parts) and per-part offsets inside the segment
     const long long nsg = (long long)nw * nseg;
+#pragma unroll 4
     for (long long sg = threadIdx.x; sg < nsg; sg += blockDim.x) {
         int c = 0;
         for (int p = 0; p < nparts; p++) {
@@ -1331,6 +1334,7 @@ __global__ void __launch_bounds__(1024) k_build_tasks(int nw, int nseg, int npar
     for (int q = threadIdx.x; q < 2 * min(ntab, BT_TABS); q += blockDim.x) tabcnt[q] = (long long)lcnt[q];
     // phase 3: absolute per-part offsets (segment base + relative), then the listed
     // segments, one wave per segment with the lanes over its tasks
+#pragma unroll 4
     for (long long sg = threadIdx.x; sg < nsg; sg += blockDim.x) {
         const int g = (int)(sg / nseg), q = (int)(sg % nseg);
         const long long base = q < 8 ? segs[(size_t)g * 8 + q].start : gslice[g];
@@ -1883,7 +1887,7 @@ __global__ void __launch_bounds__(256) k_transform(int mode, int n, const Transf
     using L = PadT<K>;
     constexpr int N = L::N, K3 = K * K * K;
     extern __shared__ double smem[];
-    __shared__ double wsum[4];
+    __shared__ double wsum[4][8];
     double *T = smem;
     double *Ms = smem + L::SIZE;
     const TransformItem it = items[blockIdx.x];
@@ -1902,28 +1906,40 @@ __global__ void __launch_bounds__(256) k_transform(int mode, int n, const Transf
         line_pass_padded<N>(T, Ms, dim);
         __syncthreads();
     }
+    double ss[8];
+#pragma unroll
     for (int c = 0; c < 8; c++) {
-        double ss = 0.0;
+        ss[c] = 0.0;
         if (mode == 0) {
             double *dst = it.coef + (size_t)it.parent * 8 * K3 + c * K3;
             for (int r = threadIdx.x; r < K3; r += blockDim.x) {
                 const double v = T[padded_index<K>(c, r)];
                 dst[r] = v;
-                ss += v * v;
+                ss[c] += v * v;
             }
         } else {
             double *dst = it.coef + (size_t)(it.first_child + c) * 8 * K3;
             for (int r = threadIdx.x; r < K3; r += blockDim.x) {
                 const double v = dst[r] + T[padded_index<K>(c, r)];
                 dst[r] = v;
-                ss += v * v;
+                ss[c] += v * v;
             }
         }
-        ss = block_sum_256(ss, wsum);
-        if (threadIdx.x == 0 && it.norm) {
-            if (mode == 0) it.norm[(size_t)it.parent * 8 + c] = sqrt(ss);
-            else it.norm[(size_t)(it.first_child + c) * 8] = sqrt(ss);
-        }
+    }
+    // the 8 sums over the block at once (fixed order: lanes by butterfly, then waves 0..3)
+#pragma unroll
+    for (int c = 0; c < 8; c++)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) ss[c] += __shfl_xor(ss[c], o, 64);
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int c = 0; c < 8; c++) wsum[threadIdx.x >> 6][c] = ss[c];
+    __syncthreads();
+    if (threadIdx.x < 8 && it.norm) {
+        const int c = threadIdx.x;
+        const double tot = (wsum[0][c] + wsum[1][c]) + (wsum[2][c] + wsum[3][c]);
+        if (mode == 0) it.norm[(size_t)it.parent * 8 + c] = sqrt(tot);
+        else it.norm[(size_t)(it.first_child + c) * 8] = sqrt(tot);
     }
     (void)n;
 }
