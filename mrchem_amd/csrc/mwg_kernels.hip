@@ -504,12 +504,45 @@ constexpr unsigned long long FT0_COMBOS = 0x0101010101010101ull;  // the combos 
 
 // mark the input nodes that the output nodes need at this depth but the input
 // tree does not have (scaling-only "generated" nodes)
+// translations cached per output node in LDS (the union box over all terms, cells)
+constexpr int FCACHE = 4096;
+
+__device__ inline void gen_insert(const JobTab &J, const GenState &gs, int job, uint64_t key) {
+    const unsigned long long kk = (unsigned long long)key + 1ull;
+    uint32_t i = dkey_hash(key) & J.genHash.mask;
+    for (uint32_t probe = 0; probe <= J.genHash.mask; probe++) {
+        unsigned long long old = atomicCAS(&J.genHash.keys[i], 0ull, kk);
+        if (old == 0ull) {
+            const int idx = atomicAdd(gs.count, 1);
+            if (idx < gs.cap) {
+                gs.keys[idx] = key;
+                gs.job[idx] = job;
+            } else {
+                atomicOr(gs.overflow, 1);
+            }
+            J.genHash.vals[i] = idx;
+            return;
+        }
+        if (old == kk) return;
+        i = (i + 1) & J.genHash.mask;
+        if (probe == J.genHash.mask) atomicOr(gs.overflow, 2);
+    }
+}
+
+// Walks the (term, delta) candidates of the ft = 0 boxes; the translations that some
+// term needs are first collected in an LDS bitmap over the job's union box, so that
+// each input position is probed once (candidates of different terms share them).
 __global__ void __launch_bounds__(256) k_gen_mark(int nw, const WorkItem *work, const JobTab *jobs,
                                                   const TermTab *tabs, GenState gs) {
     const WorkItem wi = work[blockIdx.x];
     const JobTab &J = jobs[wi.job];
     __shared__ int start[257];
+    __shared__ unsigned need[FCACHE / 32];
     for (int t = threadIdx.x; t <= J.ntab; t += blockDim.x) start[t] = J.pre0[t];
+    const int ux = J.ubox[0], uy = J.ubox[1], uz = J.ubox[2];
+    const int nx = 2 * ux + 1, ny = 2 * uy + 1, nz = 2 * uz + 1;
+    const bool cached = ux >= 0 && nx * ny * nz <= FCACHE;
+    for (int q = threadIdx.x; q < FCACHE / 32; q += blockDim.x) need[q] = 0u;
     __syncthreads();
     const int total = start[J.ntab];
     for (int k = threadIdx.x; k < total; k += blockDim.x) {
@@ -521,35 +554,28 @@ __global__ void __launch_bounds__(256) k_gen_mark(int nw, const WorkItem *work, 
         }
         const TermTab &tb = tabs[J.tab0 + lo];
         const int c = k - start[lo];
-        const int ux = tb.ub0[0], uy = tb.ub0[1], uz = tb.ub0[2];
-        const int dx = c % (2 * ux + 1) - ux;
-        const int dy = (c / (2 * ux + 1)) % (2 * uy + 1) - uy;
-        const int dz = c / ((2 * ux + 1) * (2 * uy + 1)) - uz;
+        const int vx = tb.ub0[0], vy = tb.ub0[1], vz = tb.ub0[2];
+        const int dx = c % (2 * vx + 1) - vx;
+        const int dy = (c / (2 * vx + 1)) % (2 * vy + 1) - vy;
+        const int dz = c / ((2 * vx + 1) * (2 * vy + 1)) - vz;
         const int fx = wi.l[0] - dx, fy = wi.l[1] - dy, fz = wi.l[2] - dz;
         if (fx < 0 || fy < 0 || fz < 0 || fx >= J.lmax[0] || fy >= J.lmax[1] || fz >= J.lmax[2]) continue;
         if (!(combo_mask(tb, dx, dy, dz) & FT0_COMBOS)) continue;
-        const uint64_t key = dnode_key(J.depth, fx, fy, fz);
-        if (dhash_find(J.inHash, key) >= 0) continue;
-        // insert into this job's generated-node table
-        const unsigned long long kk = (unsigned long long)key + 1ull;
-        uint32_t i = dkey_hash(key) & J.genHash.mask;
-        for (uint32_t probe = 0; probe <= J.genHash.mask; probe++) {
-            unsigned long long old = atomicCAS(&J.genHash.keys[i], 0ull, kk);
-            if (old == 0ull) {
-                const int idx = atomicAdd(gs.count, 1);
-                if (idx < gs.cap) {
-                    gs.keys[idx] = key;
-                    gs.job[idx] = wi.job;
-                } else {
-                    atomicOr(gs.overflow, 1);
-                }
-                J.genHash.vals[i] = idx;
-                break;
-            }
-            if (old == kk) break;
-            i = (i + 1) & J.genHash.mask;
-            if (probe == J.genHash.mask) atomicOr(gs.overflow, 2);
+        if (cached) {
+            const int cell = ((dz + uz) * ny + (dy + uy)) * nx + (dx + ux);
+            atomicOr(&need[cell >> 5], 1u << (cell & 31));
+            continue;
         }
+        const uint64_t key = dnode_key(J.depth, fx, fy, fz);
+        if (dhash_find(J.inHash, key) < 0) gen_insert(J, gs, wi.job, key);
+    }
+    if (!cached) return;
+    __syncthreads();
+    for (int cell = threadIdx.x; cell < nx * ny * nz; cell += blockDim.x) {
+        if (!((need[cell >> 5] >> (cell & 31)) & 1u)) continue;
+        const int dx = cell % nx - ux, dy = (cell / nx) % ny - uy, dz = cell / (nx * ny) - uz;
+        const uint64_t key = dnode_key(J.depth, wi.l[0] - dx, wi.l[1] - dy, wi.l[2] - dz);
+        if (dhash_find(J.inHash, key) < 0) gen_insert(J, gs, wi.job, key);
     }
     (void)nw;
 }
@@ -713,7 +739,6 @@ __global__ void __launch_bounds__(256) k_gen_compute(int first, int cap, const J
 // Per output node, the input node (>= 0), generated node (-(g+1)) or nothing
 // (INT_MIN) at every translation of the union box over all terms, cached in LDS so
 // that each term's loop reads the cache instead of probing the hash tables again.
-constexpr int FCACHE = 4096;
 struct FCache {
     int ux, uy, uz;  // half-widths, ux < 0: no cache (box too large)
 };
