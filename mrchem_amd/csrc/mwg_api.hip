@@ -386,6 +386,17 @@ struct TreeH {
         hvalid = false;
         return id;
     }
+    // host capacity for m nodes (no reallocation while the tree grows to m)
+    void reserve_host(size_t m) {
+        depth.reserve(m);
+        parent.reserve(m);
+        child.reserve(m);
+        lr.reserve(3 * m);
+        cnorm.reserve(8 * m);
+        size_t cap = 1024;
+        while (cap < 2 * m + 2) cap <<= 1;
+        if (index.keys.size() < cap) index.rehash(cap);
+    }
     void make_roots() {
         clear();
         for (int z = 0; z < mra->boxes[2]; z++)
@@ -943,7 +954,9 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         J.out->make_roots();
         // output trees usually end near the input's size: reserve once instead of
         // growing (each growth is an allocation, a copy and a free)
-        J.out->ensure_dev(std::max<size_t>(64, (size_t)J.out->n() + J.in->n() + J.in->n() / 4));
+        const size_t est = std::max<size_t>(64, (size_t)J.out->n() + J.in->n() + J.in->n() / 4);
+        J.out->ensure_dev(est);
+        J.out->reserve_host(est);
         J.work = J.out->roots;
         uint32_t cap = 1u << 14;
         while (cap < 4u * (uint32_t)J.in->n()) cap <<= 1;
@@ -1080,10 +1093,9 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         C.tabs.alloc((size_t)std::max(ntabs, 1));
         C.tbmax.alloc((size_t)std::max<size_t>(bjobs.size() * max_terms, 1));
         HC(launch_job_boxes(s, (int)bjobs.size(), max_terms, dbjobs, C.tabs.p, C.tbmax.p, djobs, C.tpre.p,
-                            C.tpre0.p));
+                            C.tpre0.p, C.genOverflow.p));
         // generated input nodes: marked, then computed for the range the device counter
         // reached (read back with the work-list counts below, not before)
-        HC(hipMemsetAsync(C.genOverflow.p, 0, sizeof(int), s));
         {
             const int want = std::min(genCap, genDone + std::max(4096, 2 * genLastNew + 1024));
             if (genPoolCap < want) {
@@ -1173,9 +1185,10 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                                      C.gofs.p, C.tuples.p, C.slices.p, dsum));
             const int grid64 = std::max(1, (C.caps.tasks + 3) / 4);
             const int gridred = std::min(4096, std::max(1, C.caps.reds));
-            HC(hipEventRecord(C.ev0, s));
+            // contraction time: ev0 and ev1 are stamped by the dispatches of the first and
+            // the last contraction kernel (no separate event packets on the stream)
             HC(launch_contract_fp64(s, K, 0, dsum, grid64, C.tasks.p, dwork, djobs, C.tabs.p, C.tuples.p,
-                                    C.genPool.p, C.partials.p));
+                                    C.genPool.p, C.partials.p, C.ev0));
             if (mixed) {
                 const int gridmf = std::max(1, C.caps.stasks);
                 HC(launch_contract_mfma(s, K, 0, dsum, gridmf, C.stasks.p, dwork, djobs, C.tabs.p,
@@ -1183,8 +1196,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             }
             // (mixed: the reduce items cover every output component and write the norms)
             HC(launch_reduce_partials(s, 0, dsum, gridred, C.reds.p, dwork, djobs, C.partials.p,
-                                      mixed ? C.nodeslots.p : nullptr, K3, mixed ? dwn : nullptr));
-            HC(hipEventRecord(C.ev1, s));
+                                      mixed ? C.nodeslots.p : nullptr, K3, mixed ? dwn : nullptr, C.ev1));
             if (!mixed) HC(launch_work_norms(s, nw, dwork, djobs, K3, dwn));
             // refinement predictor: features of every work node's children and their split
             // probabilities, read back with the norms (no extra round trip)
@@ -1250,7 +1262,12 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         for (int j : active) {
             JobH &J = jobs[j];
             std::vector<int> cur = J.work;
-            std::stable_sort(cur.begin(), cur.end(), [&](int a, int b) { return J.out->depth[a] < J.out->depth[b]; });
+            bool one_depth = true;
+            for (size_t q = 1; q < cur.size() && one_depth; q++)
+                one_depth = J.out->depth[cur[q]] == J.out->depth[cur[0]];
+            if (!one_depth)
+                std::stable_sort(cur.begin(), cur.end(),
+                                 [&](int a, int b) { return J.out->depth[a] < J.out->depth[b]; });
             std::vector<int> next;
             size_t q0 = 0;
             while (q0 < cur.size()) {

@@ -130,7 +130,7 @@ hipError_t launch_project_hyd(hipStream_t s, int nw, const int *work, const int 
 // TermTab[tab0 + t] for every (table, term): per-combination screening boxes from the
 // monotone norm envelopes, union boxes, and the table's FP64 routing bound
 hipError_t launch_job_boxes(hipStream_t s, int nbj, int max_terms, const BoxJob *bj, TermTab *tabs, double *tbmax,
-                            JobTab *jobs, int *pre, int *pre0);
+                            JobTab *jobs, int *pre, int *pre0, int *gen_overflow);
 hipError_t launch_gen_mark(hipStream_t s, int nw, const WorkItem *work, const JobTab *jobs, const TermTab *tabs,
                            const GenState &gs);
 // generated nodes first .. min(device count, cap) - 1 (cap = entries allocated in the pool)
@@ -210,10 +210,12 @@ hipError_t launch_child_features(hipStream_t s, int nw, const WorkItem *work, co
 // with sum: the task count is read on the device; grid (blocks) covers the capacity
 hipError_t launch_contract_fp64(hipStream_t s, int K, int ntask, const TaskSummary *sum, int grid,
                                 const ContractTask *tasks, const WorkItem *work, const JobTab *jobs,
-                                const TermTab *tabs, const Tuple *tuples, const double *genPool, double *partials);
+                                const TermTab *tabs, const Tuple *tuples, const double *genPool, double *partials,
+                                hipEvent_t start = nullptr);
 hipError_t launch_move_groups(hipStream_t s, int npairs, const int *from_to, double *coef, double *norm, int nb);
 hipError_t launch_reduce_partials(hipStream_t s, int n, const TaskSummary *sum, int grid, const ReduceItem *items,
                                   const WorkItem *work, const JobTab *jobs, const double *partials,
-                                  const double *nodeslots, int K3, double *wnorm = nullptr);
+                                  const double *nodeslots, int K3, double *wnorm = nullptr,
+                                  hipEvent_t stop = nullptr);
 
 }  // namespace mwg

@@ -6,6 +6,8 @@
 // index is bit*K + i; each 1D pass multiplies every line of 2K values in place.
 #include "mwg_kernels.h"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -590,20 +592,22 @@ __device__ void gen_one(int g, const JobTab *jobs, const GenState &gs, double *T
     const JobTab &J = jobs[gs.job[g]];
     const int d = (int)(key >> 57);
     const int64_t l[3] = {(int64_t)((key >> 38) & 0x7FFFF), (int64_t)((key >> 19) & 0x7FFFF), (int64_t)(key & 0x7FFFF)};
+    // the deepest ancestor present in the input tree: one thread per level probes it
     __shared__ int s_anc, s_ad;
     if (threadIdx.x == 0) {
         s_anc = -1;
         s_ad = -1;
-        for (int dd = d - 1; dd >= 0; dd--) {
-            const int sh = d - dd;
-            const int a = dhash_find(J.inHash, dnode_key(dd, l[0] >> sh, l[1] >> sh, l[2] >> sh));
-            if (a >= 0) {
-                s_anc = a;
-                s_ad = dd;
-                break;
-            }
-        }
     }
+    __syncthreads();
+    const int my_dd = d - 1 - (int)threadIdx.x;
+    int my_a = -1;
+    if (my_dd >= 0) {
+        const int sh = d - my_dd;
+        my_a = dhash_find(J.inHash, dnode_key(my_dd, l[0] >> sh, l[1] >> sh, l[2] >> sh));
+        if (my_a >= 0) atomicMax(&s_ad, my_dd);
+    }
+    __syncthreads();
+    if (my_a >= 0 && my_dd == s_ad) s_anc = my_a;
     __syncthreads();
     const int anc = s_anc, ad = s_ad;
     double *outp = gs.pool + (size_t)g * K3;
@@ -940,9 +944,10 @@ __device__ inline int wave_incl_scan(int v) {
 
 __global__ void __launch_bounds__(64) k_job_route(int nbj, const BoxJob *bjs, const TermTab *tabs,
                                                   const double *tbmax, int max_terms, JobTab *jobs, int *pre,
-                                                  int *pre0) {
+                                                  int *pre0, int *gen_overflow) {
     const int jb = blockIdx.x;
     if (jb >= nbj) return;
+    if (jb == 0 && threadIdx.x == 0) *gen_overflow = 0;  // the iteration's generated-table flag
     const BoxJob &B = bjs[jb];
     const int lane = threadIdx.x;
     double m = 0.0;
@@ -2222,10 +2227,11 @@ hipError_t launch_project_hyd(hipStream_t s, int nw, const int *work, const int 
 }
 
 hipError_t launch_job_boxes(hipStream_t s, int nbj, int max_terms, const BoxJob *bj, TermTab *tabs, double *tbmax,
-                            JobTab *jobs, int *pre, int *pre0) {
+                            JobTab *jobs, int *pre, int *pre0, int *gen_overflow) {
     if (nbj <= 0 || max_terms <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_job_boxes, dim3(nbj, max_terms), dim3(64), 0, s, nbj, bj, tabs, tbmax, max_terms);
-    hipLaunchKernelGGL(k_job_route, dim3(nbj), dim3(64), 0, s, nbj, bj, tabs, tbmax, max_terms, jobs, pre, pre0);
+    hipLaunchKernelGGL(k_job_route, dim3(nbj), dim3(64), 0, s, nbj, bj, tabs, tbmax, max_terms, jobs, pre, pre0,
+                       gen_overflow);
     return hipGetLastError();
 }
 
@@ -2303,21 +2309,23 @@ hipError_t launch_build_tasks(hipStream_t s, int nw, int nseg, int nparts, int c
 
 hipError_t launch_contract_fp64(hipStream_t s, int K, int ntask, const TaskSummary *sum, int grid,
                                 const ContractTask *tasks, const WorkItem *work, const JobTab *jobs,
-                                const TermTab *tabs, const Tuple *tuples, const double *genPool, double *partials) {
+                                const TermTab *tabs, const Tuple *tuples, const double *genPool, double *partials,
+                                hipEvent_t start) {
     const int nb = sum ? grid : (ntask + 3) / 4;
     if (nb <= 0) return hipSuccess;
-    MWG_DISPATCH_K(K, hipLaunchKernelGGL(k_contract_fp64<KK>, dim3(nb), dim3(256), 0, s, ntask, sum, tasks, work,
-                                         jobs, tabs, tuples, genPool, partials));
+    // the start event (if any) is stamped by the dispatch packet itself
+    MWG_DISPATCH_K(K, hipExtLaunchKernelGGL(k_contract_fp64<KK>, dim3(nb), dim3(256), 0, s, start, nullptr, 0,
+                                            ntask, sum, tasks, work, jobs, tabs, tuples, genPool, partials));
     return hipGetLastError();
 }
 
 hipError_t launch_reduce_partials(hipStream_t s, int n, const TaskSummary *sum, int grid, const ReduceItem *items,
                                   const WorkItem *work, const JobTab *jobs, const double *partials,
-                                  const double *nodeslots, int K3, double *wnorm) {
+                                  const double *nodeslots, int K3, double *wnorm, hipEvent_t stop) {
     const int nb = sum ? grid : n;
     if (nb <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_reduce_partials, dim3(nb), dim3(256), 0, s, n, sum, items, work, jobs, partials, nodeslots,
-                       K3, wnorm);
+    hipExtLaunchKernelGGL(k_reduce_partials, dim3(nb), dim3(256), 0, s, nullptr, stop, 0, n, sum, items, work, jobs,
+                          partials, nodeslots, K3, wnorm);
     return hipGetLastError();
 }
 
