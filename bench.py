@@ -49,6 +49,11 @@ def _l2diff(a, b, n=20000, seed=3):
     return math.sqrt((d ** 2).mean() * 40.0 ** 3)
 
 
+def prec_tag(p):
+    """1e-06 -> '1e-6' (the precision as the profile file names spell it)."""
+    return f"{p:g}".replace("e-0", "e-").replace("e+0", "e+")
+
+
 def measured_traffic(kernel, workload):
     """HBM bytes per launch of a kernel from the committed rocprofv3 counter summary
     (profiles/hbm_traffic_<kernel>_<workload>.txt: FETCH_SIZE and WRITE_SIZE runs of this
@@ -439,7 +444,7 @@ def main():
     roofline = dict(bound="mfma", achieved=achieved / 1e12, peak=PEAK[dtype] / 1e12, unit="TFLOP/s",
                     frac=achieved / PEAK[dtype],
                     traffic=measured_traffic("k_contract_mfma8" if dtype == "f32" else "k_contract_fp64",
-                                             f"helmholtz{args.orbitals}_k{args.k}_prec{args.prec:g}"),
+                                             f"helmholtz{args.orbitals}_k{args.k}_prec{prec_tag(args.prec)}"),
                     kernel="k_contract_mfma8 + k_contract_fp64 + k_reduce_partials" if dtype == "f32"
                     else "k_contract_fp64 + k_reduce_partials",
                     issued_tflops=(best["flops_per_step"] / tc / 1e12) if tc > 0 else None,

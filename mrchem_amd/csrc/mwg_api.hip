@@ -52,6 +52,14 @@ inline hipError_t dev_malloc(void **p, size_t bytes) {
         flush_device_caches();
         e = hipMalloc(p, bytes);
     }
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        size_t fr = 0, tot = 0;
+        (void)hipMemGetInfo(&fr, &tot);
+        throw Error(MWG_ERR_HIP, "device allocation of " + std::to_string(bytes >> 20) + " MiB failed (" +
+                                     std::to_string(fr >> 20) + " of " + std::to_string(tot >> 20) +
+                                     " MiB free after emptying the block caches)");
+    }
     return e;
 }
 
@@ -916,6 +924,9 @@ struct PhaseTimer {
     }
 };
 
+// device memory for the FP64 partial slots of one refinement iteration
+constexpr double SLOT_BUDGET = 8.0 * (1ull << 30);
+
 static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, std::vector<mwg_apply_stats> &stats) {
     if (jobs.empty()) return;
     PhaseTimer PT;
@@ -1172,9 +1183,10 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         std::vector<float> cprob(use_pred ? (size_t)nw * 8 : 0);
         std::vector<float> cfeat(C.tracing ? (size_t)nw * 8 * MLP_FEATURES : 0);
         TaskSummary hsum;
+        int chunk_it = C.chunk;  // FP64 task size of this iteration (raised when partial slots would not fit)
         auto launch_schedule = [&]() {
             ensure_caps(nullptr);
-            HC(launch_build_tasks(s, nw, nseg, nparts, C.chunk, 4 * C.slice_chunk, C.gtotal.p, C.caps, C.genCount.p,
+            HC(launch_build_tasks(s, nw, nseg, nparts, chunk_it, 4 * C.slice_chunk, C.gtotal.p, C.caps, C.genCount.p,
                                   genPoolCap, C.gofs.p, C.tasks.p, C.stasks.p, C.reds.p, dsum, C.segs.p,
                                   C.segcnt.p, C.gslice.p, C.genOverflow.p, dwork, dtab, (int)jt.size()));
             if (!mixed)
@@ -1221,7 +1233,11 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         launch_count();
         PT.lap("enum_count");
         launch_schedule();
-        if (hsum.overflow) {
+        // A capacity (or the generated-node pool) was too small: grow and run again.  A
+        // partial-slot need beyond SLOT_BUDGET raises this iteration's FP64 task size
+        // instead (fewer, larger tasks: only the summation grouping changes).
+        for (int attempt = 0; hsum.overflow; attempt++) {
+            if (attempt == 12) throw Error(MWG_ERR_RANGE, "work-list schedule sizing failed");
             if (hsum.overflow & 2) {
                 // finish the generated nodes past the pool, then count again
                 const int genNow = hsum.gen_count;
@@ -1235,12 +1251,20 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                 genPoolCap = want;
                 launch_count();
             }
-            ensure_caps(&hsum);
+            TaskSummary need = hsum;
+            if ((double)need.nslots * K3 * sizeof(double) > SLOT_BUDGET && chunk_it < 4096) {
+                // twice the task size; the slot capacity stays, the next schedule tells
+                chunk_it *= 2;
+                need.nslots = 0;
+            }
+            ensure_caps(&need);
             launch_schedule();
-            if (hsum.overflow) throw Error(MWG_ERR_RANGE, "work-list schedule sizing failed");
         }
         genLastNew = hsum.gen_count - genDone;
         genDone = hsum.gen_count;
+        if (PT.on)
+            fprintf(stderr, "[mwg timing] iteration %d: %d work nodes, %lld tuples, %lld slices, %d generated (pool %d)\n",
+                    iter, nw, hsum.ntuples, hsum.nslices, genLastNew, genPoolCap);
         for (size_t q = 0; q < jt.size(); q++) {
             jobs[jt_job[q]].tuples += tabcnt[2 * q];
             jobs[jt_job[q]].slices += tabcnt[2 * q + 1];
