@@ -1423,11 +1423,16 @@ __device__ inline void contract_fp64_task(const ContractTask task, const WorkIte
         const gdouble *Z = blocks + blk + ((tp.dz + band) * 4 + tz) * K2;
         const gdouble *f = tp.f >= 0 ? inCoef + (size_t)tp.f * 8 * K3 + ft * K3 : gen + (size_t)(-tp.f - 1) * K3;
         const double w = tb.w;
+        // The operator rows come through scalar loads.  Up to K = 8 a whole K x K block fits
+        // the scalar registers and the loops unroll fully; beyond, one row (pass 3: one
+        // column) is live at a time — the same sums in the same order.
+        constexpr bool ROWWISE = K > 8;
+        constexpr int RU = ROWWISE ? 1 : K;
         for (int r = lane; r < K2; r += 64) {
             double fr[K];
 #pragma unroll
             for (int i = 0; i < K; i++) fr[i] = f[r * K + i];
-#pragma unroll
+#pragma unroll RU
             for (int a = 0; a < K; a++) {
                 double s = 0.0;
 #pragma unroll
@@ -1441,7 +1446,7 @@ __device__ inline void contract_fp64_task(const ContractTask task, const WorkIte
             double tc[K];
 #pragma unroll
             for (int j = 0; j < K; j++) tc[j] = B[(l * K + j) * K + a];
-#pragma unroll
+#pragma unroll RU
             for (int b = 0; b < K; b++) {
                 double s = 0.0;
 #pragma unroll
@@ -1454,15 +1459,29 @@ __device__ inline void contract_fp64_task(const ContractTask task, const WorkIte
         for (int p = 0; p < NP; p++) {
             const int ba = lane + 64 * p;
             if (ba < K2) {
-                double tc[K];
+                if constexpr (ROWWISE) {
+                    double s[K];
 #pragma unroll
-                for (int l = 0; l < K; l++) tc[l] = B[l * K2 + ba];
+                    for (int c = 0; c < K; c++) s[c] = 0.0;
+#pragma unroll 1
+                    for (int l = 0; l < K; l++) {
+                        const double t = B[l * K2 + ba];
 #pragma unroll
-                for (int c = 0; c < K; c++) {
-                    double s = 0.0;
+                        for (int c = 0; c < K; c++) s[c] += Z[c * K + l] * t;
+                    }
 #pragma unroll
-                    for (int l = 0; l < K; l++) s += Z[c * K + l] * tc[l];
-                    acc[p][c] += w * s;
+                    for (int c = 0; c < K; c++) acc[p][c] += w * s[c];
+                } else {
+                    double tc[K];
+#pragma unroll
+                    for (int l = 0; l < K; l++) tc[l] = B[l * K2 + ba];
+#pragma unroll
+                    for (int c = 0; c < K; c++) {
+                        double s = 0.0;
+#pragma unroll
+                        for (int l = 0; l < K; l++) s += Z[c * K + l] * tc[l];
+                        acc[p][c] += w * s;
+                    }
                 }
             }
         }
