@@ -47,7 +47,7 @@ typedef struct mwg_predictor_s *mwg_predictor;
 
 /* apply flags */
 #define MWG_APPLY_FP64 0u     /* plain method: FP64 contractions, norm-test refinement */
-#define MWG_APPLY_MIXED 1u    /* FP32 MFMA contractions with FP64 accumulation */
+#define MWG_APPLY_MIXED 1u    /* FP32 MFMA slice contractions (FP32 sums over <= 16 slices, FP64 beyond) */
 #define MWG_APPLY_PREDICT 2u  /* learned refinement predictor (norm test stays the judge) */
 
 /* basis types (src/mrenv.cpp:62-69: "I" Interpolating, "L" Legendre) */
