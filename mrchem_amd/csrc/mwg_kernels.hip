@@ -858,6 +858,7 @@ __device__ inline double wave_max_double(double v) {
     return v;
 }
 
+constexpr int ENV_LDS = 512;  // envelope entries per type staged in LDS
 __global__ void __launch_bounds__(64) k_job_boxes(int nbj, const BoxJob *bjs, TermTab *tabs, double *tbmax,
                                                   int max_terms) {
     const int jb = blockIdx.x, t = blockIdx.y;
@@ -867,6 +868,14 @@ __global__ void __launch_bounds__(64) k_job_boxes(int nbj, const BoxJob *bjs, Te
     const LevTerm lt = B.terms[t];
     TermTab *tb = tabs + B.tab0 + t;
     const int c = threadIdx.x, gt = c >> 3, ft = c & 7;
+    // the term's norm envelopes (4 types x (band + 1)) staged in LDS for the searches
+    __shared__ double senv[4 * ENV_LDS];
+    const int ne = 4 * (lt.band + 1);
+    const bool env_lds = ne <= 4 * ENV_LDS;
+    if (env_lds)
+        for (int e = c; e < ne; e += 64) senv[e] = B.env[lt.env + e];
+    __syncthreads();
+    const double *envb = env_lds ? senv : B.env + lt.env;
     int m3[3] = {-1, -1, -1};
     double full = 0.0;
     bool ok = false;
@@ -885,7 +894,7 @@ __global__ void __launch_bounds__(64) k_job_boxes(int nbj, const BoxJob *bjs, Te
                     for (int q2 = 0; q2 < 3; q2++)
                         if (q2 != q) other *= lt.maxn[ty[q2]];
                     const double tau = B.thr / other;
-                    const double *env = B.env + lt.env + (long long)ty[q] * (lt.band + 1);
+                    const double *env = envb + (long long)ty[q] * (lt.band + 1);
                     int m = -1;
                     if (env[0] > tau) {
                         int lo = 0, hi = lt.band;
@@ -1178,6 +1187,37 @@ __device__ inline long long wave_incl_scan_ll(long long v) {
     return v;
 }
 
+// exclusive prefixes of NV values at once over the 1024 threads of the block (one
+// exchange through LDS for all of them); tot[k] gets the sums
+template <int NV>
+__device__ inline void block_excl_scan_1024_n(long long (&v)[NV], long long (*wsum)[NV], long long (&tot)[NV]) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    long long incl[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) incl[k] = wave_incl_scan_ll(v[k]);
+    __syncthreads();
+    if (lane == 63)
+#pragma unroll
+        for (int k = 0; k < NV; k++) wsum[wave][k] = incl[k];
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        const int k = threadIdx.x;
+        long long acc = 0;
+        for (int w = 0; w < 16; w++) {
+            const long long x = wsum[w][k];
+            wsum[w][k] = acc;
+            acc += x;
+        }
+        wsum[16][k] = acc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        tot[k] = wsum[16][k];
+        v[k] = wsum[wave][k] + incl[k] - v[k];
+    }
+}
+
 // exclusive prefix over the 1024 threads of the block; *total gets the sum
 __device__ inline long long block_excl_scan_1024(long long v, long long *wsum, long long *total) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1217,7 +1257,7 @@ __global__ void __launch_bounds__(1024)
                   TaskSummary *__restrict__ sum, SegRec *__restrict__ segs, int *__restrict__ segcnt,
                   long long *__restrict__ gslice, const int *__restrict__ gen_overflow,
                   const WorkItem *__restrict__ work, long long *__restrict__ tabcnt, int ntab) {
-    __shared__ long long wsum[17];
+    __shared__ long long wsum6[17][6];
     __shared__ int big[BT_LIST];     // segment ids (>= 0) or -(node + 1) for slice runs
     __shared__ int nbig;
     const bool mixed = nseg == 9;
@@ -1232,9 +1272,9 @@ __global__ void __launch_bounds__(1024)
     for (int q = threadIdx.x; q < 2 * ntab; q += blockDim.x) tabcnt[q] = 0;
     for (int q = threadIdx.x; q < 2 * BT_TABS; q += blockDim.x) lcnt[q] = 0;
     // phase 0: segment counts (sum over the parts) and per-part offsets inside the segment
-    const long long nsg = (long long)nw * nseg;
+    const int nsg = nw * nseg;  // (work lists stay far below 2^31 / 9 nodes)
 #pragma unroll 4
-    for (long long sg = threadIdx.x; sg < nsg; sg += blockDim.x) {
+    for (int sg = threadIdx.x; sg < nsg; sg += blockDim.x) {
         int c = 0;
         for (int p = 0; p < nparts; p++) {
             ofs[sg * nparts + p] = c;
@@ -1260,13 +1300,10 @@ __global__ void __launch_bounds__(1024)
             if (!mixed && c > chunk) nred++;
         }
     }
-    long long Ttup, Tsl, Tnt, Tnst, Tslots, Tnred;
-    long long btup = block_excl_scan_1024(tup, wsum, &Ttup);
-    long long bsl = block_excl_scan_1024(sl, wsum, &Tsl);
-    long long bnt = block_excl_scan_1024(nt, wsum, &Tnt);
-    long long bnst = block_excl_scan_1024(nst, wsum, &Tnst);
-    long long bslots = block_excl_scan_1024(slots, wsum, &Tslots);
-    long long bnred = block_excl_scan_1024(nred, wsum, &Tnred);
+    long long pv[6] = {tup, sl, nt, nst, slots, nred}, pt[6];
+    block_excl_scan_1024_n<6>(pv, wsum6, pt);
+    long long btup = pv[0], bsl = pv[1], bnt = pv[2], bnst = pv[3], bslots = pv[4], bnred = pv[5];
+    long long Ttup = pt[0], Tsl = pt[1], Tnt = pt[2], Tnst = pt[3], Tslots = pt[4], Tnred = pt[5];
     if (mixed) Tnred = 8LL * nw;
     int overflow = 0;
     if (Ttup > caps.tuples || Tsl > caps.slices || Tnt > caps.tasks || Tnst > caps.stasks || Tnred > caps.reds ||
@@ -1365,8 +1402,8 @@ __global__ void __launch_bounds__(1024)
     // phase 3: absolute per-part offsets (segment base + relative), then the listed
     // segments, one wave per segment with the lanes over its tasks
 #pragma unroll 4
-    for (long long sg = threadIdx.x; sg < nsg; sg += blockDim.x) {
-        const int g = (int)(sg / nseg), q = (int)(sg % nseg);
+    for (int sg = threadIdx.x; sg < nsg; sg += blockDim.x) {
+        const int g = sg / nseg, q = sg - g * nseg;
         const long long base = q < 8 ? segs[(size_t)g * 8 + q].start : gslice[g];
         for (int p = 0; p < nparts; p++) ofs[sg * nparts + p] += base;
     }
