@@ -47,6 +47,16 @@ __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// sum over the block of one value per thread, same order every call (all threads get it)
+__device__ inline double block_sum_256(double v, double *wsum) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
+}
+
 // ----------------------------------------------------------------- operator blocks
 
 // Scaling-scaling block of one unit Gaussian exp(-b (delta + u - v)^2) at cell scale m:
@@ -704,7 +714,7 @@ __device__ void gen_one(int g, const JobTab *jobs, const GenState &gs, double *T
             __syncthreads();
         }
     }
-    __shared__ double red[256];
+    __shared__ double wsum[4];
     double s2 = 0.0;
     for (int e = threadIdx.x; e < K3; e += blockDim.x) {
         const int x = e % K, y = (e / K) % K, z = e / (K * K);
@@ -712,13 +722,8 @@ __device__ void gen_one(int g, const JobTab *jobs, const GenState &gs, double *T
         outp[e] = v;
         s2 += v * v;
     }
-    red[threadIdx.x] = s2;
-    __syncthreads();
-    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) gs.norm[g] = sqrt(red[0]);
+    s2 = block_sum_256(s2, wsum);
+    if (threadIdx.x == 0) gs.norm[g] = sqrt(s2);
     __syncthreads();
 }
 
@@ -1556,16 +1561,6 @@ __global__ void __launch_bounds__(256) k_contract_fp64(int ntask_arg, const Task
 
 // sums, per segment (g, gt), the FP64 tuple slots and the gt part of the mixed
 // node slots in slot order (deterministic)
-// sum over the block of one value per thread, same order every call (all threads get it)
-__device__ inline double block_sum_256(double v, double *wsum) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
-    __syncthreads();
-    return (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
-}
-
 // Sums the partial results of an output component in a fixed order.  With wnorm
 // (mixed path: every component of every work node has an item) the component's norm
 // goes to the tree and to the compact per-work-node array as well.
