@@ -291,7 +291,8 @@ __device__ inline void line_pass_padded(double *T, const double *M, int dim) {
         double in[N];
 #pragma unroll
         for (int q = 0; q < N; q++) in[q] = T[base + q * stride];
-#pragma unroll
+        // (output rows two at a time: a full unroll would keep all N^2 matrix entries live)
+#pragma unroll 2
         for (int p = 0; p < N; p++) {
             double acc = 0.0;
 #pragma unroll
@@ -592,45 +593,14 @@ __global__ void __launch_bounds__(256) k_gen_mark(int nw, const WorkItem *work, 
     (void)nw;
 }
 
-// scaling block of a generated node: reconstruct from the leaf ancestor (using its
-// wavelets for the first step), then refine with the low-pass filters only
-template <int K>
-__device__ void gen_one(int g, const JobTab *jobs, const GenState &gs, double *T, const double *Ws) {
+// Refines the node tensor in T (padded (2K)^3, all 8 components of an ancestor) down
+// `steps` levels along the path bits of l (bit steps-1 first): the first step uses
+// the scaling and wavelet parts (2K -> K per line), the others the scaling part only.
+// The scaling block of the descendant is left at T[(z N + y) P + x], x, y, z < K.
+template <int K, class IT>
+__device__ void descend_padded(double *T, const double *Ws, const IT *l, int steps) {
     using L = PadT<K>;
-    constexpr int N = L::N, P = L::P, K3 = K * K * K;
-    const uint64_t key = gs.keys[g];
-    const JobTab &J = jobs[gs.job[g]];
-    const int d = (int)(key >> 57);
-    const int64_t l[3] = {(int64_t)((key >> 38) & 0x7FFFF), (int64_t)((key >> 19) & 0x7FFFF), (int64_t)(key & 0x7FFFF)};
-    // the deepest ancestor present in the input tree: one thread per level probes it
-    __shared__ int s_anc, s_ad;
-    if (threadIdx.x == 0) {
-        s_anc = -1;
-        s_ad = -1;
-    }
-    __syncthreads();
-    const int my_dd = d - 1 - (int)threadIdx.x;
-    int my_a = -1;
-    if (my_dd >= 0) {
-        const int sh = d - my_dd;
-        my_a = dhash_find(J.inHash, dnode_key(my_dd, l[0] >> sh, l[1] >> sh, l[2] >> sh));
-        if (my_a >= 0) atomicMax(&s_ad, my_dd);
-    }
-    __syncthreads();
-    if (my_a >= 0 && my_dd == s_ad) s_anc = my_a;
-    __syncthreads();
-    const int anc = s_anc, ad = s_ad;
-    double *outp = gs.pool + (size_t)g * K3;
-    if (anc < 0) {
-        for (int e = threadIdx.x; e < K3; e += blockDim.x) outp[e] = 0.0;
-        if (threadIdx.x == 0) gs.norm[g] = 0.0;
-        __syncthreads();
-        return;
-    }
-    const double *src = J.inCoef + (size_t)anc * 8 * K3;
-    for (int e = threadIdx.x; e < 8 * K3; e += blockDim.x) T[padded_index<K>(e / K3, e % K3)] = src[e];
-    __syncthreads();
-    const int steps = d - ad;
+    constexpr int N = L::N, P = L::P;
     // first step: child a of the (s, d) parent; line (2K) -> K values, s_j = sum_p W[p][a*K+j] in_p
     {
         const int sh = steps - 1;
@@ -714,6 +684,47 @@ __device__ void gen_one(int g, const JobTab *jobs, const GenState &gs, double *T
             __syncthreads();
         }
     }
+}
+
+// scaling block of a generated node: reconstruct from the leaf ancestor (using its
+// wavelets for the first step), then refine with the low-pass filters only
+template <int K>
+__device__ void gen_one(int g, const JobTab *jobs, const GenState &gs, double *T, const double *Ws) {
+    using L = PadT<K>;
+    constexpr int N = L::N, P = L::P, K3 = K * K * K;
+    const uint64_t key = gs.keys[g];
+    const JobTab &J = jobs[gs.job[g]];
+    const int d = (int)(key >> 57);
+    const int64_t l[3] = {(int64_t)((key >> 38) & 0x7FFFF), (int64_t)((key >> 19) & 0x7FFFF), (int64_t)(key & 0x7FFFF)};
+    // the deepest ancestor present in the input tree: one thread per level probes it
+    __shared__ int s_anc, s_ad;
+    if (threadIdx.x == 0) {
+        s_anc = -1;
+        s_ad = -1;
+    }
+    __syncthreads();
+    const int my_dd = d - 1 - (int)threadIdx.x;
+    int my_a = -1;
+    if (my_dd >= 0) {
+        const int sh = d - my_dd;
+        my_a = dhash_find(J.inHash, dnode_key(my_dd, l[0] >> sh, l[1] >> sh, l[2] >> sh));
+        if (my_a >= 0) atomicMax(&s_ad, my_dd);
+    }
+    __syncthreads();
+    if (my_a >= 0 && my_dd == s_ad) s_anc = my_a;
+    __syncthreads();
+    const int anc = s_anc, ad = s_ad;
+    double *outp = gs.pool + (size_t)g * K3;
+    if (anc < 0) {
+        for (int e = threadIdx.x; e < K3; e += blockDim.x) outp[e] = 0.0;
+        if (threadIdx.x == 0) gs.norm[g] = 0.0;
+        __syncthreads();
+        return;
+    }
+    const double *src = J.inCoef + (size_t)anc * 8 * K3;
+    for (int e = threadIdx.x; e < 8 * K3; e += blockDim.x) T[padded_index<K>(e / K3, e % K3)] = src[e];
+    __syncthreads();
+    descend_padded<K>(T, Ws, l, d - ad);
     __shared__ double wsum[4];
     double s2 = 0.0;
     for (int e = threadIdx.x; e < K3; e += blockDim.x) {
@@ -2048,88 +2059,37 @@ hipError_t launch_move_groups(hipStream_t s, int npairs, const int *from_to, dou
 
 // ----------------------------------------------------------------- pointwise multiplication
 
-// Loads the representation of an input tree at an output node into T (interleaved
-// (2K)^3): the node's 8 components when it exists (steps = 0), else the scaling
-// block refined from the leaf ancestor `src` over `steps` levels (path bits from l).
-__device__ void load_input_rep(double *T, const double *coef, int src, int steps, const int *l, const double *W,
-                               int K) {
-    const int N = 2 * K, N2 = N * N, N3 = N2 * N, K3 = K * K * K;
+// Loads the representation of an input tree at an output node into the padded node
+// tensor T: the node's 8 components when it exists (steps = 0), else the scaling block
+// refined from the leaf ancestor `src` over `steps` levels (path bits from l) with zero
+// wavelet components.
+template <int K>
+__device__ void load_rep_padded(double *T, const double *coef, int src, int steps, const int *l, const double *Ws) {
+    using L = PadT<K>;
+    constexpr int N = L::N, P = L::P, K3 = K * K * K;
     const double *p = coef + (size_t)src * 8 * K3;
-    for (int e = threadIdx.x; e < 8 * K3; e += blockDim.x) T[interleaved_index(K, e / K3, e % K3)] = p[e];
+    for (int e = threadIdx.x; e < 8 * K3; e += blockDim.x) T[padded_index<K>(e / K3, e % K3)] = p[e];
     __syncthreads();
     if (steps == 0) return;
-    {
-        const int sh = steps - 1;
-        const int ax = (l[0] >> sh) & 1, ay = (l[1] >> sh) & 1, az = (l[2] >> sh) & 1;
-        for (int line = threadIdx.x; line < N2; line += blockDim.x) {
-            const int base = line * N;
-            double in[2 * MAXK];
-            for (int q = 0; q < N; q++) in[q] = T[base + q];
-            for (int j = 0; j < K; j++) {
-                double s = 0.0;
-                for (int q = 0; q < N; q++) s += W[q * N + ax * K + j] * in[q];
-                T[base + j] = s;
-            }
-        }
-        __syncthreads();
-        for (int line = threadIdx.x; line < N * K; line += blockDim.x) {
-            const int z = line / K, x = line % K, base = z * N2 + x;
-            double in[2 * MAXK];
-            for (int q = 0; q < N; q++) in[q] = T[base + q * N];
-            for (int j = 0; j < K; j++) {
-                double s = 0.0;
-                for (int q = 0; q < N; q++) s += W[q * N + ay * K + j] * in[q];
-                T[base + j * N] = s;
-            }
-        }
-        __syncthreads();
-        for (int line = threadIdx.x; line < K * K; line += blockDim.x) {
-            const int y = line / K, x = line % K, base = y * N + x;
-            double in[2 * MAXK];
-            for (int q = 0; q < N; q++) in[q] = T[base + q * N2];
-            for (int j = 0; j < K; j++) {
-                double s = 0.0;
-                for (int q = 0; q < N; q++) s += W[q * N + az * K + j] * in[q];
-                T[base + j * N2] = s;
-            }
-        }
-        __syncthreads();
-    }
-    for (int sh = steps - 2; sh >= 0; sh--) {
-        const int a3[3] = {(l[0] >> sh) & 1, (l[1] >> sh) & 1, (l[2] >> sh) & 1};
-        for (int dim = 0; dim < 3; dim++) {
-            const int stride = dim == 0 ? 1 : (dim == 1 ? N : N2);
-            for (int line = threadIdx.x; line < K * K; line += blockDim.x) {
-                const int u = line % K, v = line / K;
-                const int base = dim == 0 ? v * N2 + u * N : (dim == 1 ? v * N2 + u : v * N + u);
-                double in[MAXK];
-                for (int q = 0; q < K; q++) in[q] = T[base + q * stride];
-                for (int j = 0; j < K; j++) {
-                    double s = 0.0;
-                    for (int q = 0; q < K; q++) s += W[q * N + a3[dim] * K + j] * in[q];
-                    T[base + j * stride] = s;
-                }
-            }
-            __syncthreads();
-        }
-    }
-    // keep the scaling block, clear the wavelet components
-    for (int e = threadIdx.x; e < N3; e += blockDim.x) {
-        const int x = e % N, y = (e / N) % N, z = e / N2;
-        if (x >= K || y >= K || z >= K) T[e] = 0.0;
+    descend_padded<K>(T, Ws, l, steps);
+    for (int e = threadIdx.x; e < N * N * N; e += blockDim.x) {
+        const int x = e % N, y = (e / N) % N, z = e / (N * N);
+        if (x >= K || y >= K || z >= K) T[(z * N + y) * P + x] = 0.0;
     }
     __syncthreads();
 }
 
 // out = coef * a * b at the listed output nodes: both inputs to values at the K^3
 // quadrature points of the 8 children, product, quadrature projection, compression
-__global__ void k_multiply(int n, const MulItem *items, ProjGeom geo, DevBasis B, double coef, const double *acoef,
-                           const double *bcoef, double *out) {
+template <int K>
+__global__ void __launch_bounds__(256) k_multiply(int n, const MulItem *items, ProjGeom geo, DevBasis B, double coef,
+                                                  const double *acoef, const double *bcoef, double *out) {
+    using L = PadT<K>;
+    constexpr int N = L::N, P = L::P, N2 = N * N, K3 = K * K * K;
     extern __shared__ double smem[];
-    const int K = B.K, N = 2 * K, N2 = N * N, N3 = N2 * N, K3 = K * K * K;
     double *Ta = smem;
-    double *Tb = Ta + N3;
-    double *Ws = Tb + N3;
+    double *Tb = Ta + L::SIZE;
+    double *Ws = Tb + L::SIZE;
     double *WTs = Ws + N2;
     double *Vs = WTs + N2;
     double *Ps = Vs + N2;
@@ -2143,51 +2103,56 @@ __global__ void k_multiply(int n, const MulItem *items, ProjGeom geo, DevBasis B
         Ps[e] = same ? B.phiq[(c % K) * K + (r % K)] : 0.0;  // coefficients: row = function i, col = point q
     }
     __syncthreads();
-    load_input_rep(Ta, acoef, it.asrc, it.asteps, it.l, Ws, K);
-    load_input_rep(Tb, bcoef, it.bsrc, it.bsteps, it.l, Ws, K);
-    for (int dim = 0; dim < 3; dim++) {
-        line_pass(Ta, WTs, N, dim);
-        line_pass(Tb, WTs, N, dim);
+    load_rep_padded<K>(Ta, acoef, it.asrc, it.asteps, it.l, Ws);
+    load_rep_padded<K>(Tb, bcoef, it.bsrc, it.bsteps, it.l, Ws);
+    for (int dim = 0; dim < 3; dim++) {  // reconstruct to the children's scaling coefficients
+        line_pass_padded<N>(Ta, WTs, dim);
+        line_pass_padded<N>(Tb, WTs, dim);
         __syncthreads();
     }
-    for (int dim = 0; dim < 3; dim++) {
-        line_pass(Ta, Vs, N, dim);
-        line_pass(Tb, Vs, N, dim);
+    for (int dim = 0; dim < 3; dim++) {  // values at the quadrature points
+        line_pass_padded<N>(Ta, Vs, dim);
+        line_pass_padded<N>(Tb, Vs, dim);
         __syncthreads();
     }
     const double h = geo.sfac * ldexp(1.0, -(geo.n0 + it.depth + 1));
     const double scale = coef / (h * sqrt(h));
-    for (int e = threadIdx.x; e < N3; e += blockDim.x) {
+    for (int e = threadIdx.x; e < N * N2; e += blockDim.x) {
         const int x = e % N, y = (e / N) % N, z = e / N2;
-        Ta[e] = Ta[e] * Tb[e] * scale * B.qw[x % K] * B.qw[y % K] * B.qw[z % K];
+        const int i = (z * N + y) * P + x;
+        Ta[i] = Ta[i] * Tb[i] * scale * B.qw[x % K] * B.qw[y % K] * B.qw[z % K];
     }
     __syncthreads();
-    for (int dim = 0; dim < 3; dim++) {
-        line_pass(Ta, Ps, N, dim);
+    for (int dim = 0; dim < 3; dim++) {  // projection onto the children's scaling functions
+        line_pass_padded<N>(Ta, Ps, dim);
         __syncthreads();
     }
-    for (int dim = 0; dim < 3; dim++) {
-        line_pass(Ta, Ws, N, dim);
+    for (int dim = 0; dim < 3; dim++) {  // compression to the node's 8 components
+        line_pass_padded<N>(Ta, Ws, dim);
         __syncthreads();
     }
     for (int e = threadIdx.x; e < 8 * K3; e += blockDim.x)
-        out[(size_t)it.node * 8 * K3 + e] = Ta[interleaved_index(K, e / K3, e % K3)];
+        out[(size_t)it.node * 8 * K3 + e] = Ta[padded_index<K>(e / K3, e % K3)];
     (void)n;
 }
 
-// out = ca * a + cb * b at the listed output nodes (compressed coefficients are linear;
-// an input coarser than the node contributes its refined scaling block only)
-__global__ void k_add(int n, const MulItem *items, DevBasis B, double ca, const double *acoef, double cb,
-                      const double *bcoef, double *out) {
+// out = ca * a + cb * b at the listed output nodes
+template <int K>
+__global__ void __launch_bounds__(256) k_add(int n, const MulItem *items, DevBasis B, double ca,
+                                             const double *acoef, double cb, const double *bcoef, double *out) {
+    using L = PadT<K>;
+    constexpr int N = L::N, K3 = K * K * K;
     extern __shared__ double smem[];
-    const int K = B.K, N = 2 * K, N3 = N * N * N, K3 = K * K * K;
     double *Ta = smem;
-    double *Tb = Ta + N3;
+    double *Tb = Ta + L::SIZE;
+    double *Ws = Tb + L::SIZE;
+    for (int e = threadIdx.x; e < N * N; e += blockDim.x) Ws[e] = B.W[e];
+    __syncthreads();
     const MulItem it = items[blockIdx.x];
-    load_input_rep(Ta, acoef, it.asrc, it.asteps, it.l, B.W, K);
-    load_input_rep(Tb, bcoef, it.bsrc, it.bsteps, it.l, B.W, K);
+    load_rep_padded<K>(Ta, acoef, it.asrc, it.asteps, it.l, Ws);
+    load_rep_padded<K>(Tb, bcoef, it.bsrc, it.bsteps, it.l, Ws);
     for (int e = threadIdx.x; e < 8 * K3; e += blockDim.x) {
-        const int idx = interleaved_index(K, e / K3, e % K3);
+        const int idx = padded_index<K>(e / K3, e % K3);
         out[(size_t)it.node * 8 * K3 + e] = ca * Ta[idx] + cb * Tb[idx];
     }
     (void)n;
@@ -2415,13 +2380,12 @@ hipError_t launch_multiply(hipStream_t s, int n, const MulItem *items, const Pro
                            double coef, const double *acoef, const double *bcoef, double *out) {
     if (n <= 0) return hipSuccess;
     const int K = B.K, N = 2 * K;
-    const size_t sm = (size_t)(2 * N * N * N + 4 * N * N) * sizeof(double);
-    if (sm > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_multiply, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)sm);
+    const size_t sm = (size_t)(2 * N * N * (N + 1) + 4 * N * N) * sizeof(double);
+    MWG_DISPATCH_K(K, {
+        hipError_t e = allow_smem(k_multiply<KK>, sm);
         if (e != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL(k_multiply, dim3(n), dim3(256), sm, s, n, items, geo, B, coef, acoef, bcoef, out);
+        hipLaunchKernelGGL(k_multiply<KK>, dim3(n), dim3(256), sm, s, n, items, geo, B, coef, acoef, bcoef, out);
+    });
     return hipGetLastError();
 }
 
@@ -2429,12 +2393,12 @@ hipError_t launch_add(hipStream_t s, int n, const MulItem *items, const DevBasis
                       double cb, const double *bcoef, double *out) {
     if (n <= 0) return hipSuccess;
     const int K = B.K, N = 2 * K;
-    const size_t sm = (size_t)(2 * N * N * N) * sizeof(double);
-    if (sm > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_add, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    const size_t sm = (size_t)(2 * N * N * (N + 1) + N * N) * sizeof(double);
+    MWG_DISPATCH_K(K, {
+        hipError_t e = allow_smem(k_add<KK>, sm);
         if (e != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL(k_add, dim3(n), dim3(256), sm, s, n, items, B, ca, acoef, cb, bcoef, out);
+        hipLaunchKernelGGL(k_add<KK>, dim3(n), dim3(256), sm, s, n, items, B, ca, acoef, cb, bcoef, out);
+    });
     return hipGetLastError();
 }
 
