@@ -228,34 +228,6 @@ __global__ void k_nsblocks(int n, const long long *item_delta, const int *item_r
 
 // ----------------------------------------------------------------- tensor line passes
 
-// in-place pass along dimension `dim` of an N^3 LDS tensor: line <- M line (M is N x N)
-__device__ inline void line_pass(double *T, const double *M, int N, int dim) {
-    const int N2 = N * N;
-    const int stride = dim == 0 ? 1 : (dim == 1 ? N : N2);
-    for (int line = threadIdx.x; line < N2; line += blockDim.x) {
-        int base;
-        const int u = line % N, v = line / N;
-        if (dim == 0) base = line * N;
-        else if (dim == 1) base = v * N2 + u;
-        else base = line;
-        double in[2 * MAXK];
-        for (int q = 0; q < N; q++) in[q] = T[base + q * stride];
-        for (int p = 0; p < N; p++) {
-            double s = 0.0;
-            for (int q = 0; q < N; q++) s += M[p * N + q] * in[q];
-            T[base + p * stride] = s;
-        }
-    }
-}
-
-__device__ inline int interleaved_index(int K, int c, int e) {
-    // component c, in-component index e = (z*K + y)*K + x -> (2K)^3 index
-    const int N = 2 * K;
-    const int x = e % K, y = (e / K) % K, z = e / (K * K);
-    const int bx = c & 1, by = (c >> 1) & 1, bz = (c >> 2) & 1;
-    return ((bz * K + z) * N + (by * K + y)) * N + bx * K + x;
-}
-
 // (2K)^3 node tensor in LDS with the x rows padded to N + 1 doubles: x-lines (stride 1)
 // of neighbouring lanes then start 2(N+1) banks apart instead of 2N (no conflicts)
 template <int K>
@@ -343,8 +315,8 @@ __global__ void __launch_bounds__(256) k_project_gauss(int nw, const int *work, 
                                                        ProjGeom geo, DevBasis B, int nterm, const double *coef,
                                                        const double *alpha, const double *center,
                                                        const double *poly, double *out) {
-    constexpr int N = 2 * K, N3 = N * N * N, K3 = K * K * K;
-    constexpr int PER = (N3 + 255) / 256;
+    constexpr int N = 2 * K, K3 = K * K * K;
+    constexpr int LPER = (N * N + 255) / 256;  // x-lines (y, z) per thread
     constexpr int CH = 8;
     __shared__ double P[CH][3][N], WP[CH][3][N], Ws[N * N], cf[CH];
     const int node = work[blockIdx.x];
@@ -353,8 +325,11 @@ __global__ void __launch_bounds__(256) k_project_gauss(int nw, const int *work, 
     const double pre = sqrt(hc);
     const int tid = threadIdx.x;
     for (int e = tid; e < N * N; e += 256) Ws[e] = B.W[e];
-    double acc[PER];
-    for (int r = 0; r < PER; r++) acc[r] = 0.0;
+    double acc[LPER][N];
+#pragma unroll
+    for (int r = 0; r < LPER; r++)
+#pragma unroll
+        for (int x = 0; x < N; x++) acc[r][x] = 0.0;
     for (int t0 = 0; t0 < nterm; t0 += CH) {
         __syncthreads();
         for (int e = tid; e < CH * 3 * N; e += 256) {
@@ -387,23 +362,32 @@ __global__ void __launch_bounds__(256) k_project_gauss(int nw, const int *work, 
             WP[tt][dim][p] = s;
         }
         __syncthreads();
-        for (int r = 0; r < PER; r++) {
-            const int e = tid + 256 * r;
-            if (e < N3) {
-                const int px = e % N, py = (e / N) % N, pz = e / (N * N);
-                double s = acc[r];
-                for (int tt = 0; tt < CH; tt++) s += cf[tt] * WP[tt][2][pz] * WP[tt][1][py] * WP[tt][0][px];
-                acc[r] = s;
+        // the thread's x-lines accumulate the outer products term by term
+#pragma unroll 1
+        for (int tt = 0; tt < CH; tt++) {
+#pragma unroll
+            for (int r = 0; r < LPER; r++) {
+                const int line = tid + 256 * r;
+                if (line < N * N) {
+                    const int py = line % N, pz = line / N;
+                    const double yz = cf[tt] * WP[tt][2][pz] * WP[tt][1][py];
+#pragma unroll
+                    for (int x = 0; x < N; x++) acc[r][x] += yz * WP[tt][0][x];
+                }
             }
         }
     }
-    for (int r = 0; r < PER; r++) {
-        const int e = tid + 256 * r;
-        if (e < N3) {
-            const int px = e % N, py = (e / N) % N, pz = e / (N * N);
-            const int c = (px / K) | ((py / K) << 1) | ((pz / K) << 2);
-            const int idx = ((pz % K) * K + (py % K)) * K + (px % K);
-            out[(size_t)node * 8 * K3 + c * K3 + idx] = acc[r];
+#pragma unroll
+    for (int r = 0; r < LPER; r++) {
+        const int line = tid + 256 * r;
+        if (line < N * N) {
+            const int py = line % N, pz = line / N;
+#pragma unroll
+            for (int px = 0; px < N; px++) {
+                const int c = (px / K) | ((py / K) << 1) | ((pz / K) << 2);
+                const int idx = ((pz % K) * K + (py % K)) * K + (px % K);
+                out[(size_t)node * 8 * K3 + c * K3 + idx] = acc[r][px];
+            }
         }
     }
     (void)nw;
@@ -452,13 +436,15 @@ __device__ double d_hydrogen_orb(int N, int L, int M, double Z, double x, double
 
 // hydrogen functions are not separable: values at the K^3 quadrature points of
 // each child, then three 1D quadrature passes, then compression
-__global__ void k_project_hyd(int nw, const int *work, const int *depth, const int *lr, ProjGeom geo, DevBasis B,
-                              int nterm, const double *coef, const int *q, double Z, const double *origin,
-                              double *out) {
+template <int K>
+__global__ void __launch_bounds__(256) k_project_hyd(int nw, const int *work, const int *depth, const int *lr,
+                                                     ProjGeom geo, DevBasis B, int nterm, const double *coef,
+                                                     const int *q, double Z, const double *origin, double *out) {
+    using L = PadT<K>;
+    constexpr int N = L::N, P = L::P, N3 = N * N * N, K3 = K * K * K;
     extern __shared__ double smem[];
-    const int K = B.K, N = 2 * K, N3 = N * N * N, K3 = K * K * K;
     double *T = smem;
-    double *Ws = smem + N3;
+    double *Ws = smem + L::SIZE;
     double *Qm = Ws + N * N;  // N x N block-diagonal quadrature matrix: phi_i(x_q) w_q * pre
     const int node = work[blockIdx.x];
     const int d = depth[node];
@@ -480,21 +466,19 @@ __global__ void k_project_hyd(int nw, const int *work, const int *depth, const i
             if (q[6 * t + 3] > 0) a *= d_hydrogen_orb(q[6 * t + 3], q[6 * t + 4], q[6 * t + 5], Z, x, y, z);
             v += coef[t] * a;
         }
-        T[e] = v;
+        T[(pz * N + py) * P + px] = v;
     }
     __syncthreads();
     for (int dim = 0; dim < 3; dim++) {
-        line_pass(T, Qm, N, dim);
+        line_pass_padded<N>(T, Qm, dim);
         __syncthreads();
     }
     for (int dim = 0; dim < 3; dim++) {
-        line_pass(T, Ws, N, dim);
+        line_pass_padded<N>(T, Ws, dim);
         __syncthreads();
     }
-    for (int e = threadIdx.x; e < 8 * K3; e += blockDim.x) {
-        const int c = e / K3, r = e % K3;
-        out[(size_t)node * 8 * K3 + e] = T[interleaved_index(K, c, r)];
-    }
+    for (int e = threadIdx.x; e < 8 * K3; e += blockDim.x)
+        out[(size_t)node * 8 * K3 + e] = T[padded_index<K>(e / K3, e % K3)];
     (void)nw;
 }
 
@@ -2160,6 +2144,13 @@ __global__ void __launch_bounds__(256) k_add(int n, const MulItem *items, DevBas
 
 // ----------------------------------------------------------------- launchers
 
+// kernels with more than 64 KiB of dynamic LDS must say so before their launch
+template <class Kern>
+static hipError_t allow_smem(Kern *k, size_t sm) {
+    if (sm <= 65536) return hipSuccess;
+    return hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+}
+
 #define MWG_DISPATCH_K(K, CALL)                                       \
     switch (K) {                                                      \
         case 4: { constexpr int KK = 4; CALL; } break;                \
@@ -2230,15 +2221,14 @@ hipError_t launch_project_hyd(hipStream_t s, int nw, const int *work, const int 
                               const ProjGeom &geo, const DevBasis &B, int nterm, const double *coef, const int *q,
                               double Z, const double *origin, double *out) {
     if (nw <= 0) return hipSuccess;
-    const int K = B.K;
-    const size_t sm = (size_t)(8 * K * K * K + 8 * K * K) * sizeof(double);
-    if (sm > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_project_hyd, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)sm);
+    const int K = B.K, N = 2 * K;
+    const size_t sm = (size_t)(N * N * (N + 1) + 2 * N * N) * sizeof(double);
+    MWG_DISPATCH_K(K, {
+        hipError_t e = allow_smem(k_project_hyd<KK>, sm);
         if (e != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL(k_project_hyd, dim3(nw), dim3(256), sm, s, nw, work, depth, lr, geo, B, nterm, coef, q, Z,
-                       origin, out);
+        hipLaunchKernelGGL(k_project_hyd<KK>, dim3(nw), dim3(256), sm, s, nw, work, depth, lr, geo, B, nterm, coef,
+                           q, Z, origin, out);
+    });
     return hipGetLastError();
 }
 
@@ -2260,12 +2250,6 @@ hipError_t launch_gen_mark(hipStream_t s, int nw, const WorkItem *work, const Jo
 
 // padded node tensor + one (2K)^2 filter matrix
 static size_t padded_smem(int K) { return (size_t)(2 * K * 2 * K * (2 * K + 1) + 4 * K * K) * sizeof(double); }
-
-template <class Kern>
-static hipError_t allow_smem(Kern *k, size_t sm) {
-    if (sm <= 65536) return hipSuccess;
-    return hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
-}
 
 hipError_t launch_gen_compute(hipStream_t s, int first, int cap, const JobTab *jobs, const GenState &gs,
                               const double *W, int K) {
