@@ -865,7 +865,7 @@ __global__ void __launch_bounds__(64) k_job_boxes(int nbj, const BoxJob *bjs, Te
     if (jb >= nbj) return;
     const BoxJob &B = bjs[jb];
     if (t >= B.nterms) return;
-    const LevTerm lt = B.terms[t];
+    const LevTerm &lt = B.terms[t];  // (read in place: maxn is indexed by the lane's types)
     TermTab *tb = tabs + B.tab0 + t;
     const int c = threadIdx.x, gt = c >> 3, ft = c & 7;
     // the term's norm envelopes (4 types x (band + 1)) staged in LDS for the searches
@@ -883,14 +883,17 @@ __global__ void __launch_bounds__(64) k_job_boxes(int nbj, const BoxJob *bjs, Te
         const double F = B.F[ft];
         if (F > 0.0) {
             int ty[3];
+#pragma unroll
             for (int q = 0; q < 3; q++) ty[q] = 2 * ((gt >> q) & 1) + ((ft >> q) & 1);
             const double base = fabs(lt.w) * F;
             const double f3 = base * lt.maxn[ty[0]] * lt.maxn[ty[1]] * lt.maxn[ty[2]];
             if (f3 > B.thr) {
                 full = f3;
                 ok = true;
+#pragma unroll
                 for (int q = 0; q < 3; q++) {
                     double other = base;
+#pragma unroll
                     for (int q2 = 0; q2 < 3; q2++)
                         if (q2 != q) other *= lt.maxn[ty[q2]];
                     const double tau = B.thr / other;
@@ -912,6 +915,7 @@ __global__ void __launch_bounds__(64) k_job_boxes(int nbj, const BoxJob *bjs, Te
         }
     }
     int u[3], u0[3];
+#pragma unroll
     for (int q = 0; q < 3; q++) {
         const short v = ok ? (short)m3[q] : (short)-1;
         tb->box[c][q] = v;
@@ -1645,6 +1649,7 @@ __global__ void __launch_bounds__(256) k_enum_mixed(int nw, int nparts, const Wo
                             }
                             if (!surv) continue;
                             if (maxb > J.route) {
+#pragma unroll
                                 for (int gt = 0; gt < 8; gt++) {
                                     if (!((surv >> gt) & 1)) continue;
                                     if (MODE == 1) {
@@ -1662,18 +1667,17 @@ __global__ void __launch_bounds__(256) k_enum_mixed(int nw, int nparts, const Wo
                                 }
                             } else {
                                 if (MODE == 1) {
-                                    Slice sl;
-                                    sl.w = tb.w;
-                                    sl.f = f;
-                                    sl.xo = (int)(tb.blk + ((dx + tb.band) * 4 + (ft & 1)) * 64);
-                                    sl.yo = (int)(tb.blk + ((dy + tb.band) * 4 + ((ft >> 1) & 1)) * 64);
-                                    sl.zo = (int)(tb.blk + ((dz + tb.band) * 4 + ((ft >> 2) & 1)) * 64);
-                                    sl.ft = (uint8_t)ft;
-                                    // the surviving components only: the same tuples as the
-                                    // plain work list, FP32 in place of FP64
-                                    sl.mask = (uint8_t)surv;
-                                    for (int b = 0; b < 6; b++) sl.pad[b] = 0;
-                                    sout[base[8] + cnt[8]] = sl;
+                                    // the record as two 16-byte stores (layout of Slice): weight,
+                                    // f, FP32 block offsets of the three dimensions, ft, and the
+                                    // surviving components only (the same tuples as the plain
+                                    // work list, FP32 in place of FP64)
+                                    const unsigned long long wb = (unsigned long long)__double_as_longlong(tb.w);
+                                    const int xo = (int)(tb.blk + ((dx + tb.band) * 4 + (ft & 1)) * 64);
+                                    const int yo = (int)(tb.blk + ((dy + tb.band) * 4 + ((ft >> 1) & 1)) * 64);
+                                    const int zo = (int)(tb.blk + ((dz + tb.band) * 4 + ((ft >> 2) & 1)) * 64);
+                                    uint4 *dst = (uint4 *)(sout + base[8] + cnt[8]);
+                                    dst[0] = make_uint4((unsigned)wb, (unsigned)(wb >> 32), (unsigned)f, (unsigned)xo);
+                                    dst[1] = make_uint4((unsigned)yo, (unsigned)zo, (unsigned)ft | ((unsigned)surv << 8), 0u);
                                 }
                                 cnt[8]++;
                             }
