@@ -139,7 +139,11 @@ static std::vector<DevCache *> g_caches;
 struct DevCache {
     std::multimap<size_t, void *> blocks;  // bytes -> block
     size_t bytes = 0;
-    static constexpr size_t LIMIT = (size_t)8 << 30;
+    // bytes kept for reuse: a quarter of the device memory (at least 8 GiB), set at the
+    // first return (with the context's device current).  Freed trees come back here
+    // instead of hipFree (which waits for the device); an allocation that fails empties
+    // every cache first (dev_malloc).
+    size_t LIMIT = 0;
     DevCache() {
         std::lock_guard<std::mutex> lk(g_cache_mu);
         g_caches.push_back(this);
@@ -165,6 +169,12 @@ struct DevCache {
     }
     void put(void *q, size_t nbytes) {
         if (!q) return;
+        if (!LIMIT) {
+            size_t fr = 0, tot = 0;
+            LIMIT = (size_t)8 << 30;
+            if (hipMemGetInfo(&fr, &tot) == hipSuccess) LIMIT = std::max(LIMIT, tot / 4);
+            else (void)hipGetLastError();
+        }
         blocks.insert({nbytes, q});
         bytes += nbytes;
         while (bytes > LIMIT && !blocks.empty()) {  // drop the largest blocks first

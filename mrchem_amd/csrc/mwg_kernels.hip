@@ -1465,10 +1465,10 @@ __device__ inline void contract_fp64_task(const ContractTask task, const WorkIte
         const gdouble *f = tp.f >= 0 ? inCoef + (size_t)tp.f * 8 * K3 + ft * K3 : gen + (size_t)(-tp.f - 1) * K3;
         const double w = tb.w;
         // The operator rows come through scalar loads.  Up to K = 8 a whole K x K block fits
-        // the scalar registers and the loops unroll fully; beyond, one row (pass 3: one
-        // column) is live at a time — the same sums in the same order.
+        // the scalar registers and the loops unroll fully; beyond, two rows (pass 3: two
+        // columns) are live at a time — the same sums in the same order.
         constexpr bool ROWWISE = K > 8;
-        constexpr int RU = ROWWISE ? 1 : K;
+        constexpr int RU = ROWWISE ? 2 : K;
         for (int r = lane; r < K2; r += 64) {
             double fr[K];
 #pragma unroll
@@ -1504,7 +1504,7 @@ __device__ inline void contract_fp64_task(const ContractTask task, const WorkIte
                     double s[K];
 #pragma unroll
                     for (int c = 0; c < K; c++) s[c] = 0.0;
-#pragma unroll 1
+#pragma unroll 2
                     for (int l = 0; l < K; l++) {
                         const double t = B[l * K2 + ba];
 #pragma unroll
