@@ -1160,10 +1160,10 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         auto launch_count = [&]() {
             if (!mixed)
                 HC(launch_enum(s, 0, nw, nparts, dwork, djobs, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, nullptr,
-                               nullptr, nullptr));
+                               nullptr, nullptr, nullptr));
             else
                 HC(launch_enum_mixed(s, 0, nw, nparts, dwork, djobs, C.tabs.p, gs, C.thread_off.p, C.gtotal.p,
-                                     nullptr, nullptr, nullptr, nullptr));
+                                     nullptr, nullptr, nullptr, nullptr, nullptr, nullptr));
         };
         auto ensure_caps = [&](const TaskSummary *need) {
             auto grow = [](long long have, long long want) {
@@ -1201,10 +1201,10 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                                   C.segcnt.p, C.gslice.p, C.genOverflow.p, dwork, dtab, (int)jt.size()));
             if (!mixed)
                 HC(launch_enum(s, 1, nw, nparts, dwork, djobs, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, C.gofs.p,
-                               C.tuples.p, dsum));
+                               C.tuples.p, dsum, C.segs.p));
             else
                 HC(launch_enum_mixed(s, 1, nw, nparts, dwork, djobs, C.tabs.p, gs, C.thread_off.p, C.gtotal.p,
-                                     C.gofs.p, C.tuples.p, C.slices.p, dsum));
+                                     C.gofs.p, C.tuples.p, C.slices.p, dsum, C.segs.p, C.gslice.p));
             const int grid64 = std::max(1, (C.caps.tasks + 3) / 4);
             const int gridred = std::min(4096, std::max(1, C.caps.reds));
             // contraction time: ev0 and ev1 are stamped by the dispatches of the first and

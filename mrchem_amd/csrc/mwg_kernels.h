@@ -136,11 +136,20 @@ hipError_t launch_gen_mark(hipStream_t s, int nw, const WorkItem *work, const Jo
 // generated nodes first .. min(device count, cap) - 1 (cap = entries allocated in the pool)
 hipError_t launch_gen_compute(hipStream_t s, int first, int cap, const JobTab *jobs, const GenState &gs,
                               const double *W, int K);
+// per tuple segment (output node, component): first tuple, count, first FP64 task, first
+// partial slot (-1: its single task writes the output directly)
+struct SegRec {
+    long long start;
+    int count;
+    int task0;
+    int slot0;
+    int pad_stask0;  // on component 0 of a node: its first slice task (mixed)
+};
 // nparts workgroups share each output node's candidates (counts and offsets per
 // (node, segment, part))
 hipError_t launch_enum(hipStream_t s, int mode, int nw, int nparts, const WorkItem *work, const JobTab *jobs,
                        const TermTab *tabs, const GenState &gs, int *thread_off, int *seg_total,
-                       const long long *seg_ofs, Tuple *out, const TaskSummary *sum);
+                       const long long *seg_ofs, Tuple *out, const TaskSummary *sum, const SegRec *segs);
 hipError_t launch_work_norms(hipStream_t s, int nw, const WorkItem *work, const JobTab *jobs, int K3,
                              double *compact);
 struct TransformItem {
@@ -170,7 +179,8 @@ struct SliceTask {
 };
 hipError_t launch_enum_mixed(hipStream_t s, int mode, int nw, int nparts, const WorkItem *work, const JobTab *jobs,
                              const TermTab *tabs, const GenState &gs, int *thread_off, int *seg_total,
-                             const long long *seg_ofs, Tuple *out, Slice *sout, const TaskSummary *sum);
+                             const long long *seg_ofs, Tuple *out, Slice *sout, const TaskSummary *sum,
+                             const SegRec *segs, const long long *gslice);
 hipError_t launch_contract_mfma(hipStream_t s, int K, int ntask, const TaskSummary *sum, int grid,
                                 const SliceTask *tasks, const WorkItem *work, const JobTab *jobs,
                                 const TermTab *tabs, const Slice *slices, const double *genPool, double *nodeslots);
@@ -188,15 +198,6 @@ hipError_t launch_add(hipStream_t s, int n, const MulItem *items, const DevBasis
 hipError_t launch_to_f32(hipStream_t s, size_t n, const double *in, float *out);
 // from the per-segment counts (nseg = 8: plain FP64, 9: mixed): segment offsets, FP64
 // tasks (full chunks first), MFMA slice tasks, reduce items, slot numbering
-// per tuple segment (output node, component): first tuple, count, first FP64 task, first
-// partial slot (-1: its single task writes the output directly)
-struct SegRec {
-    long long start;
-    int count;
-    int task0;
-    int slot0;
-    int pad_stask0;  // on component 0 of a node: its first slice task (mixed)
-};
 hipError_t launch_build_tasks(hipStream_t s, int nw, int nseg, int nparts, int chunk, int wchunk, const int *tot,
                               TaskCaps caps, const int *gen_count, int gen_pcap, long long *ofs, ContractTask *tasks,
                               SliceTask *stasks, ReduceItem *reds, TaskSummary *sum, SegRec *segs, int *segcnt,

@@ -1009,11 +1009,13 @@ __global__ void __launch_bounds__(64) k_job_route(int nbj, const BoxJob *bjs, co
 // job threshold.  Tuples are laid out by segment (g, gt):
 //   mode 0 counts per (thread, gt), scans over the block and writes per-thread
 //          offsets inside each segment and the segment totals;
-//   mode 1 writes every tuple at seg_ofs[g*8+gt] + thread offset + running index.
+//   mode 1 writes every tuple at the segment's start (from the schedule) + the part's
+//          offset inside the segment + thread offset + running index.
 template <int MODE>
 __global__ void __launch_bounds__(256) k_enum(int nw, int nparts, const WorkItem *work, const JobTab *jobs,
                                               const TermTab *tabs, GenState gs, int *thread_off, int *seg_total,
-                                              const long long *seg_ofs, Tuple *out, const TaskSummary *sum) {
+                                              const long long *seg_ofs, Tuple *out, const TaskSummary *sum,
+                                              const SegRec *segs) {
     if (MODE == 1 && sum->overflow) return;
     const int g = blockIdx.x / nparts, part = blockIdx.x % nparts;
     const WorkItem wi = work[g];
@@ -1028,7 +1030,7 @@ __global__ void __launch_bounds__(256) k_enum(int nw, int nparts, const WorkItem
     if (MODE == 1)
         for (int q = 0; q < 8; q++) {
             const size_t sp = ((size_t)g * 8 + q) * nparts + part;  // (segment, part)
-            base[q] = seg_ofs[sp] + thread_off[sp * 256 + threadIdx.x];
+            base[q] = segs[(size_t)g * 8 + q].start + seg_ofs[sp] + thread_off[sp * 256 + threadIdx.x];
         }
     walk_candidates(ct, fcache, fc, wi, J, tabs, gs, part, nparts,
                     [&](int t, const TermTab &tb, int dx, int dy, int dz, int f, const double *fn, int nft,
@@ -1276,6 +1278,7 @@ __global__ void __launch_bounds__(1024)
     for (int q = threadIdx.x; q < 2 * ntab; q += blockDim.x) tabcnt[q] = 0;
     for (int q = threadIdx.x; q < 2 * BT_TABS; q += blockDim.x) lcnt[q] = 0;
     // phase 0: segment counts (sum over the parts) and per-part offsets inside the segment
+    // (relative to the segment's start)
     const int nsg = nw * nseg;  // (work lists stay far below 2^31 / 9 nodes)
 #pragma unroll 4
     for (int sg = threadIdx.x; sg < nsg; sg += blockDim.x) {
@@ -1403,14 +1406,8 @@ __global__ void __launch_bounds__(1024)
     flush_run();
     __syncthreads();
     for (int q = threadIdx.x; q < 2 * min(ntab, BT_TABS); q += blockDim.x) tabcnt[q] = (long long)lcnt[q];
-    // phase 3: absolute per-part offsets (segment base + relative), then the listed
-    // segments, one wave per segment with the lanes over its tasks
-#pragma unroll 4
-    for (int sg = threadIdx.x; sg < nsg; sg += blockDim.x) {
-        const int g = sg / nseg, q = sg - g * nseg;
-        const long long base = q < 8 ? segs[(size_t)g * 8 + q].start : gslice[g];
-        for (int p = 0; p < nparts; p++) ofs[sg * nparts + p] += base;
-    }
+    // phase 3: the listed segments, one wave per segment with the lanes over its tasks
+    // (the fill kernels add the segment starts to the per-part offsets themselves)
     const int nb = min(nbig, BT_LIST);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int b = wave; b < nb; b += 16) {
@@ -1605,7 +1602,8 @@ template <int MODE>
 __global__ void __launch_bounds__(256) k_enum_mixed(int nw, int nparts, const WorkItem *work, const JobTab *jobs,
                                                     const TermTab *tabs, GenState gs, int *thread_off,
                                                     int *seg_total, const long long *seg_ofs, Tuple *out,
-                                                    Slice *sout, const TaskSummary *sum) {
+                                                    Slice *sout, const TaskSummary *sum, const SegRec *segs,
+                                                    const long long *gslice) {
     if (MODE == 1 && sum->overflow) return;
     const int g = blockIdx.x / nparts, part = blockIdx.x % nparts;
     const WorkItem wi = work[g];
@@ -1620,7 +1618,8 @@ __global__ void __launch_bounds__(256) k_enum_mixed(int nw, int nparts, const Wo
     if (MODE == 1)
         for (int q = 0; q < 9; q++) {
             const size_t sp = ((size_t)g * 9 + q) * nparts + part;  // (segment, part)
-            base[q] = seg_ofs[sp] + thread_off[sp * 256 + threadIdx.x];
+            base[q] = (q < 8 ? segs[(size_t)g * 8 + q].start : gslice[g]) + seg_ofs[sp] +
+                      thread_off[sp * 256 + threadIdx.x];
         }
     walk_candidates(ct, fcache, fc, wi, J, tabs, gs, part, nparts,
                     [&](int t, const TermTab &tb, int dx, int dy, int dz, int f, const double *fn, int nft,
@@ -2270,15 +2269,15 @@ hipError_t launch_gen_compute(hipStream_t s, int first, int cap, const JobTab *j
 
 hipError_t launch_enum(hipStream_t s, int mode, int nw, int nparts, const WorkItem *work, const JobTab *jobs,
                        const TermTab *tabs, const GenState &gs, int *thread_off, int *seg_total,
-                       const long long *seg_ofs, Tuple *out, const TaskSummary *sum) {
+                       const long long *seg_ofs, Tuple *out, const TaskSummary *sum, const SegRec *segs) {
     if (nw <= 0) return hipSuccess;
     const unsigned nb = (unsigned)nw * (unsigned)nparts;
     if (mode == 0)
         hipLaunchKernelGGL(k_enum<0>, dim3(nb), dim3(256), 0, s, nw, nparts, work, jobs, tabs, gs, thread_off,
-                           seg_total, seg_ofs, out, sum);
+                           seg_total, seg_ofs, out, sum, segs);
     else
         hipLaunchKernelGGL(k_enum<1>, dim3(nb), dim3(256), 0, s, nw, nparts, work, jobs, tabs, gs, thread_off,
-                           seg_total, seg_ofs, out, sum);
+                           seg_total, seg_ofs, out, sum, segs);
     return hipGetLastError();
 }
 
@@ -2335,15 +2334,16 @@ hipError_t launch_reduce_partials(hipStream_t s, int n, const TaskSummary *sum, 
 
 hipError_t launch_enum_mixed(hipStream_t s, int mode, int nw, int nparts, const WorkItem *work, const JobTab *jobs,
                              const TermTab *tabs, const GenState &gs, int *thread_off, int *seg_total,
-                             const long long *seg_ofs, Tuple *out, Slice *sout, const TaskSummary *sum) {
+                             const long long *seg_ofs, Tuple *out, Slice *sout, const TaskSummary *sum,
+                             const SegRec *segs, const long long *gslice) {
     if (nw <= 0) return hipSuccess;
     const unsigned nb = (unsigned)nw * (unsigned)nparts;
     if (mode == 0)
         hipLaunchKernelGGL(k_enum_mixed<0>, dim3(nb), dim3(256), 0, s, nw, nparts, work, jobs, tabs, gs, thread_off,
-                           seg_total, seg_ofs, out, sout, sum);
+                           seg_total, seg_ofs, out, sout, sum, segs, gslice);
     else
         hipLaunchKernelGGL(k_enum_mixed<1>, dim3(nb), dim3(256), 0, s, nw, nparts, work, jobs, tabs, gs, thread_off,
-                           seg_total, seg_ofs, out, sout, sum);
+                           seg_total, seg_ofs, out, sout, sum, segs, gslice);
     return hipGetLastError();
 }
 
