@@ -1030,7 +1030,8 @@ __global__ void __launch_bounds__(256) k_enum(int nw, int nparts, const WorkItem
     if (MODE == 1)
         for (int q = 0; q < 8; q++) {
             const size_t sp = ((size_t)g * 8 + q) * nparts + part;  // (segment, part)
-            base[q] = segs[(size_t)g * 8 + q].start + seg_ofs[sp] + thread_off[sp * 256 + threadIdx.x];
+            base[q] = segs[(size_t)g * 8 + q].start + (nparts > 1 ? seg_ofs[sp] : 0) +
+                      thread_off[sp * 256 + threadIdx.x];
         }
     walk_candidates(ct, fcache, fc, wi, J, tabs, gs, part, nparts,
                     [&](int t, const TermTab &tb, int dx, int dy, int dz, int f, const double *fn, int nft,
@@ -1279,28 +1280,33 @@ __global__ void __launch_bounds__(1024)
     for (int q = threadIdx.x; q < 2 * BT_TABS; q += blockDim.x) lcnt[q] = 0;
     // phase 0: segment counts (sum over the parts) and per-part offsets inside the segment
     // (relative to the segment's start)
+    // (one part per node: the counts are the totals and every part offset is 0, which
+    // the fill kernels know)
     const int nsg = nw * nseg;  // (work lists stay far below 2^31 / 9 nodes)
+    if (nparts > 1) {
 #pragma unroll 4
-    for (int sg = threadIdx.x; sg < nsg; sg += blockDim.x) {
-        int c = 0;
-        for (int p = 0; p < nparts; p++) {
-            ofs[sg * nparts + p] = c;
-            c += tot[sg * nparts + p];
+        for (int sg = threadIdx.x; sg < nsg; sg += blockDim.x) {
+            int c = 0;
+            for (int p = 0; p < nparts; p++) {
+                ofs[sg * nparts + p] = c;
+                c += tot[sg * nparts + p];
+            }
+            segcnt[sg] = c;
         }
-        segcnt[sg] = c;
     }
+    const int *sc = nparts > 1 ? segcnt : tot;
     __syncthreads();
     // phase 1: this thread's totals (a contiguous range of output nodes)
     const int g0 = (int)((long long)nw * threadIdx.x / 1024), g1 = (int)((long long)nw * (threadIdx.x + 1) / 1024);
     long long tup = 0, sl = 0, nt = 0, nst = 0, slots = 0, nred = 0;
     for (int g = g0; g < g1; g++) {
         if (mixed) {
-            const int cs = segcnt[(size_t)g * 9 + 8];
+            const int cs = sc[(size_t)g * 9 + 8];
             sl += cs;
             nst += (cs + wchunk - 1) / wchunk;
         }
         for (int q = 0; q < 8; q++) {
-            const int c = segcnt[(size_t)g * nseg + q];
+            const int c = sc[(size_t)g * nseg + q];
             tup += c;
             nt += ntasks_of(c);
             slots += nslots_of(c);
@@ -1336,7 +1342,7 @@ __global__ void __launch_bounds__(1024)
                                           r.slot0 < 0 ? -1 : r.slot0 + h};
     };
     auto emit_stask = [&](int g, int h) {
-        const int cs = segcnt[(size_t)g * 9 + 8], nn0 = segs[(size_t)g * 8].pad_stask0;
+        const int cs = sc[(size_t)g * 9 + 8], nn0 = segs[(size_t)g * 8].pad_stask0;
         stasks[nn0 + h] = SliceTask{gslice[g] + (long long)h * wchunk, min(wchunk, cs - h * wchunk), g, nn0 + h};
     };
     // phase 2: segment records, reduce items, tasks of small segments
@@ -1360,12 +1366,12 @@ __global__ void __launch_bounds__(1024)
                 run_jt = jt;
                 run_t = run_s = 0;
             }
-            for (int q = 0; q < 8; q++) run_t += (unsigned long long)segcnt[(size_t)g * nseg + q];
-            if (mixed) run_s += (unsigned long long)segcnt[(size_t)g * 9 + 8];
+            for (int q = 0; q < 8; q++) run_t += (unsigned long long)sc[(size_t)g * nseg + q];
+            if (mixed) run_s += (unsigned long long)sc[(size_t)g * 9 + 8];
         }
         int nn0 = (int)bnst, nns = 0;
         if (mixed) {
-            const int cs = segcnt[(size_t)g * 9 + 8];
+            const int cs = sc[(size_t)g * 9 + 8];
             nns = (cs + wchunk - 1) / wchunk;
             gslice[g] = bsl;
             segs[(size_t)g * 8].pad_stask0 = nn0;  // first slice task of node g
@@ -1381,7 +1387,7 @@ __global__ void __launch_bounds__(1024)
             bsl += cs;
         }
         for (int q = 0; q < 8; q++) {
-            const int c = segcnt[(size_t)g * nseg + q];
+            const int c = sc[(size_t)g * nseg + q];
             const int k = ntasks_of(c), ns = nslots_of(c);
             const int sgi = g * 8 + q;
             const SegRec r{btup, c, (int)bnt, ns ? (int)bslots : -1, 0};
@@ -1418,7 +1424,7 @@ __global__ void __launch_bounds__(1024)
             for (int h = lane; h < k; h += 64) emit_task(r, id, h);
         } else {
             const int g = -id - 1;
-            const int nns = (segcnt[(size_t)g * 9 + 8] + wchunk - 1) / wchunk;
+            const int nns = (sc[(size_t)g * 9 + 8] + wchunk - 1) / wchunk;
             for (int h = lane; h < nns; h += 64) emit_stask(g, h);
         }
     }
@@ -1618,7 +1624,7 @@ __global__ void __launch_bounds__(256) k_enum_mixed(int nw, int nparts, const Wo
     if (MODE == 1)
         for (int q = 0; q < 9; q++) {
             const size_t sp = ((size_t)g * 9 + q) * nparts + part;  // (segment, part)
-            base[q] = (q < 8 ? segs[(size_t)g * 8 + q].start : gslice[g]) + seg_ofs[sp] +
+            base[q] = (q < 8 ? segs[(size_t)g * 8 + q].start : gslice[g]) + (nparts > 1 ? seg_ofs[sp] : 0) +
                       thread_off[sp * 256 + threadIdx.x];
         }
     walk_candidates(ct, fcache, fc, wi, J, tabs, gs, part, nparts,
