@@ -1225,34 +1225,15 @@ __device__ inline void block_excl_scan_1024_n(long long (&v)[NV], long long (*ws
     }
 }
 
-// exclusive prefix over the 1024 threads of the block; *total gets the sum
-__device__ inline long long block_excl_scan_1024(long long v, long long *wsum, long long *total) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const long long incl = wave_incl_scan_ll(v);
-    __syncthreads();
-    if (lane == 63) wsum[wave] = incl;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        long long acc = 0;
-        for (int w = 0; w < 16; w++) {
-            const long long x = wsum[w];
-            wsum[w] = acc;
-            acc += x;
-        }
-        wsum[16] = acc;
-    }
-    __syncthreads();
-    *total = wsum[16];
-    return wsum[wave] + incl - v;
-}
 
-// One workgroup.  Phase 1: per thread (a range of output nodes) the totals of tuples,
-// slices, FP64 tasks, slots, slice tasks and reduce items; block scans give each
-// thread's offsets.  Phase 2: per segment (node, component) the per-part offsets the fill
-// kernels use, reduce items, and the tasks of segments with few tasks; segments (and
-// nodes' slice runs) with many tasks go to a list in LDS.  Phase 3: all threads write the
-// tasks of the listed segments.  Every task has a fixed position, so the schedule does
-// not depend on the order of the list.
+// One workgroup.  Phase 0 (several parts per node): segment counts and the parts'
+// offsets inside their segment.  Phase 1: per thread (a range of output nodes) the
+// totals of tuples, slices, FP64 tasks, slots, slice tasks and reduce items; one block
+// scan of the six gives each thread's offsets.  Phase 2: per segment (node, component)
+// its record (start, count, first task, first slot), reduce items, and the tasks of
+// segments with few tasks; segments (and nodes' slice runs) with many tasks go to a list
+// in LDS.  Phase 3: one wave per listed segment writes its tasks.  Every task has a fixed
+// position, so the schedule does not depend on the order of the list.
 constexpr int BT_FEW = 32, BT_LIST = 2048, BT_TABS = 256;
 
 // (one workgroup: the pointers never alias, so that loads of later iterations can be
