@@ -69,6 +69,8 @@ typedef struct mwg_apply_stats {
 } mwg_apply_stats;
 
 /* ---- context (mrenv / device) ------------------------------------------------ */
+/* Handles may be destroyed in any order: an MRA or predictor keeps its context, and a
+ * tree or operator its MRA, until the last of them is destroyed. */
 int mwg_ctx_create(int device, mwg_ctx *out);
 int mwg_ctx_destroy(mwg_ctx ctx);
 const char *mwg_last_error(mwg_ctx ctx);

@@ -13,6 +13,7 @@
 // level by level as the output trees reach new depths.
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -219,6 +220,7 @@ struct Readback {
 
 struct Ctx {
     int device = 0;
+    std::atomic<int> refs{1};  // the owner's handle, plus one per MRA and predictor made on it
     DevCache cache;
     Readback rb;
     Stage stage;
@@ -308,6 +310,7 @@ static void download_sync(Ctx &C, const DBuf<T> &d, T *dst, size_t n) {
 
 struct MraH {
     Ctx *ctx = nullptr;
+    std::atomic<int> refs{1};  // the owner's handle, plus one per tree and operator made on it
     int n0 = 0, corner[3] = {0, 0, 0}, boxes[3] = {1, 1, 1}, maxDepth = 16;
     double sfac = 1.0;
     Basis B;
@@ -1598,6 +1601,25 @@ struct mwg_tree_s : TreeH {};
 struct mwg_oper_s : OperH {};
 struct mwg_predictor_s : PredH {};
 
+// Handles may be destroyed in any order (a garbage collector finalises them in no
+// particular order): a tree or operator holds a reference to its MRA, an MRA or a
+// predictor one to its context, and each object is freed with its last reference.
+static void release_ctx(Ctx *c) {
+    if (!c || c->refs.fetch_sub(1) != 1) return;
+    (void)hipStreamSynchronize(c->stream);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    hipStream_t s = c->stream;
+    delete static_cast<mwg_ctx_s *>(c);
+    if (s) (void)hipStreamDestroy(s);
+}
+static void release_mra(MraH *m) {
+    if (!m || m->refs.fetch_sub(1) != 1) return;
+    Ctx *c = m->ctx;
+    delete static_cast<mwg_mra_s *>(m);
+    release_ctx(c);
+}
+
 static thread_local std::string g_err;
 
 template <class F>
@@ -1639,13 +1661,7 @@ int mwg_ctx_create(int device, mwg_ctx *out) {
 }
 
 int mwg_ctx_destroy(mwg_ctx ctx) {
-    if (!ctx) return MWG_OK;
-    (void)hipStreamSynchronize(ctx->stream);
-    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
-    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
-    hipStream_t s = ctx->stream;
-    delete ctx;
-    if (s) (void)hipStreamDestroy(s);
+    release_ctx(ctx);
     return MWG_OK;
 }
 
@@ -1696,12 +1712,13 @@ int mwg_mra_create(mwg_ctx ctx, int root_scale, const int *corner, const int *bo
         delete m;
         return rc;
     }
+    ctx->refs++;
     *out = m;
     return MWG_OK;
 }
 
 int mwg_mra_destroy(mwg_mra mra) {
-    delete mra;
+    release_mra(mra);
     return MWG_OK;
 }
 
@@ -1709,11 +1726,14 @@ int mwg_tree_create(mwg_mra mra, mwg_tree *out) {
     if (!mra || !out) return MWG_ERR_ARG;
     auto *t = new mwg_tree_s();
     t->mra = mra;
+    mra->refs++;
     *out = t;
     return MWG_OK;
 }
 int mwg_tree_destroy(mwg_tree tree) {
+    MraH *m = tree ? tree->mra : nullptr;
     delete tree;
+    release_mra(m);
     return MWG_OK;
 }
 int mwg_tree_nnodes(mwg_tree tree, int64_t *n) {
@@ -1959,6 +1979,7 @@ static int make_oper(mwg_mra mra, double mu, double prec, mwg_oper *out) {
         delete o;
         return rc;
     }
+    mra->refs++;
     *out = o;
     return MWG_OK;
 }
@@ -1972,7 +1993,9 @@ int mwg_oper_helmholtz(mwg_mra mra, double mu, double prec, mwg_oper *out) {
     return make_oper(mra, mu, prec, out);
 }
 int mwg_oper_destroy(mwg_oper oper) {
+    MraH *m = oper ? oper->mra : nullptr;
     delete oper;
+    release_mra(m);
     return MWG_OK;
 }
 int mwg_oper_nterms(mwg_oper oper, int *n) {
@@ -2096,13 +2119,16 @@ int mwg_predictor_create(mwg_ctx ctx, uint64_t seed, mwg_predictor *out) {
         delete p;
         return rc;
     }
+    ctx->refs++;
     *out = p;
     return MWG_OK;
 }
 
 int mwg_predictor_destroy(mwg_predictor pred) {
-    if (pred && pred->ctx && pred->ctx->pred == pred) pred->ctx->pred = nullptr;
+    Ctx *c = pred ? pred->ctx : nullptr;
+    if (c && c->pred == pred) c->pred = nullptr;
     delete pred;
+    release_ctx(c);
     return MWG_OK;
 }
 
