@@ -14,7 +14,9 @@
 
 namespace mwg {
 
-void gauss_legendre01(int q, double *x, double *w) {
+// Gauss-Legendre rule on [0,1] in extended precision: Newton on P_q from the asymptotic
+// guess for each root, ascending nodes.
+static void gauss_legendre01_ld(int q, long double *x, long double *w) {
     for (int i = 0; i < q; i++) {
         long double t = std::cos(M_PI * (i + 0.75L) / (q + 0.5L));
         long double p0 = 1, p1 = t, dp = 1;
@@ -39,8 +41,30 @@ void gauss_legendre01(int q, double *x, double *w) {
             p1 = p2;
         }
         dp = q * (t * p1 - p0) / (t * t - 1.0L);
-        x[q - 1 - i] = (double)(0.5L * (1.0L + t));
-        w[q - 1 - i] = (double)(1.0L / ((1.0L - t * t) * dp * dp));
+        x[q - 1 - i] = 0.5L * (1.0L + t);
+        w[q - 1 - i] = 1.0L / ((1.0L - t * t) * dp * dp);
+    }
+}
+
+void gauss_legendre01(int q, double *x, double *w) {
+    std::vector<long double> xl(q), wl(q);
+    gauss_legendre01_ld(q, xl.data(), wl.data());
+    for (int i = 0; i < q; i++) {
+        x[i] = (double)xl[i];
+        w[i] = (double)wl[i];
+    }
+}
+
+// sqrt(2n+1) P_n(2x-1), n < K, in extended precision
+static void legendre_scaled_ld(int K, long double x, long double *out) {
+    long double t = 2.0L * x - 1.0L, p0 = 1.0L, p1 = t;
+    out[0] = 1.0L;
+    if (K > 1) out[1] = std::sqrt(3.0L) * t;
+    for (int n = 2; n < K; n++) {
+        long double p2 = ((2.0L * n - 1.0L) * t * p1 - (n - 1.0L) * p0) / n;
+        p0 = p1;
+        p1 = p2;
+        out[n] = std::sqrt(2.0L * n + 1.0L) * p2;
     }
 }
 
@@ -73,31 +97,52 @@ void build_basis(Basis &B, int k, int type) {
     B.K3 = B.K2 * B.K;
     B.type = type;
     const int K = B.K, N = 2 * K;
-    B.qx.assign(K, 0.0);
-    B.qw.assign(K, 0.0);
-    gauss_legendre01(K, B.qx.data(), B.qw.data());
-    B.U.assign(K * K, 0.0);
+    // The basis tables and the two-scale matrix are built in extended precision and
+    // rounded once: the wavelet rows below amplify errors in the scaling rows by up to
+    // ~1e6 at k = 9 (the flipped rows are nearly dependent modulo the scaling rows).
+    std::vector<long double> xl(K), wl(K), Ul(K * K, 0.0L);
+    gauss_legendre01_ld(K, xl.data(), wl.data());
     if (type == 0) {
-        for (int i = 0; i < K; i++) B.U[i * K + i] = 1.0;
+        for (int i = 0; i < K; i++) Ul[i * K + i] = 1.0L;
     } else {
-        double l[MAXK];
+        long double l[MAXK];
         for (int j = 0; j < K; j++) {
-            legendre_scaled(K, B.qx[j], l);
-            for (int i = 0; i < K; i++) B.U[j * K + i] = std::sqrt(B.qw[j]) * l[i];
+            legendre_scaled_ld(K, xl[j], l);
+            for (int i = 0; i < K; i++) Ul[j * K + i] = std::sqrt(wl[j]) * l[i];
         }
     }
+    auto phil = [&](long double x, long double *out) {
+        long double l[MAXK];
+        legendre_scaled_ld(K, x, l);
+        for (int i = 0; i < K; i++) {
+            long double acc = 0.0L;
+            for (int j = 0; j < K; j++) acc += Ul[i * K + j] * l[j];
+            out[i] = acc;
+        }
+    };
+    B.qx.assign(K, 0.0);
+    B.qw.assign(K, 0.0);
+    B.U.assign(K * K, 0.0);
     B.phiq.assign(K * K, 0.0);
-    for (int q = 0; q < K; q++) B.phi(B.qx[q], &B.phiq[q * K]);
+    for (int q = 0; q < K; q++) {
+        B.qx[q] = (double)xl[q];
+        B.qw[q] = (double)wl[q];
+    }
+    for (int i = 0; i < K * K; i++) B.U[i] = (double)Ul[i];
+    for (int q = 0; q < K; q++) {
+        long double v[MAXK];
+        phil(xl[q], v);
+        for (int i = 0; i < K; i++) B.phiq[q * K + i] = (double)v[i];
+    }
 
     std::vector<long double> Wl(N * N, 0.0L);
-    double pa[MAXK], pb[MAXK];
+    long double pa[MAXK], pb[MAXK];
     for (int a = 0; a < 2; a++)
         for (int q = 0; q < K; q++) {
-            B.phi(0.5 * (B.qx[q] + a), pa);
-            B.phi(B.qx[q], pb);
+            phil(0.5L * (xl[q] + a), pa);
+            phil(xl[q], pb);
             for (int i = 0; i < K; i++)
-                for (int j = 0; j < K; j++)
-                    Wl[i * N + a * K + j] += (long double)B.qw[q] * pa[i] * pb[j] / std::sqrt(2.0L);
+                for (int j = 0; j < K; j++) Wl[i * N + a * K + j] += wl[q] * pa[i] * pb[j] / std::sqrt(2.0L);
         }
     // wavelet rows: orthonormal complement built from the sign-flipped scaling rows
     for (int j = 0; j < K; j++) {

@@ -62,8 +62,8 @@ namespace {
 
 // ---------------------------------------------------------------- quadrature
 
-// Gauss-Legendre rule with q points mapped to [0,1].
-void gl01(int q, double *x, double *w) {
+// Gauss-Legendre rule with q points mapped to [0,1], extended precision.
+void gl01_ld(int q, long double *x, long double *w) {
     for (int i = 0; i < q; i++) {
         long double t = std::cos(M_PI * (i + 0.75L) / (q + 0.5L));
         for (int it = 0; it < 100; it++) {
@@ -87,8 +87,31 @@ void gl01(int q, double *x, double *w) {
         long double dp = q * (t * p1 - p0) / (t * t - 1.0L);
         long double ww = 2.0L / ((1.0L - t * t) * dp * dp);
         // ascending order on [0,1]
-        x[q - 1 - i] = (double)(0.5L * (1.0L + t));
-        w[q - 1 - i] = (double)(0.5L * ww);
+        x[q - 1 - i] = 0.5L * (1.0L + t);
+        w[q - 1 - i] = 0.5L * ww;
+    }
+}
+
+void gl01(int q, double *x, double *w) {
+    std::vector<long double> xl(q), wl(q);
+    gl01_ld(q, xl.data(), wl.data());
+    for (int i = 0; i < q; i++) {
+        x[i] = (double)xl[i];
+        w[i] = (double)wl[i];
+    }
+}
+
+// legendre_phi in extended precision
+void legendre_phi_ld(int K, long double x, long double *out) {
+    long double t = 2.0L * x - 1.0L;
+    long double p0 = 1.0L, p1 = t;
+    out[0] = 1.0L;
+    if (K > 1) out[1] = std::sqrt(3.0L) * t;
+    for (int n = 2; n < K; n++) {
+        long double p2 = ((2.0L * n - 1.0L) * t * p1 - (n - 1.0L) * p0) / n;
+        p0 = p1;
+        p1 = p2;
+        out[n] = std::sqrt(2.0L * n + 1.0L) * p2;
     }
 }
 
@@ -132,35 +155,55 @@ void build_basis(Basis &B, int k, int type) {
     B.K3 = B.K2 * B.K;
     B.type = type;
     int K = B.K;
-    B.qx.resize(K);
-    B.qw.resize(K);
-    gl01(K, B.qx.data(), B.qw.data());
-    B.U.assign(K * K, 0.0);
+    // tables and filters in extended precision, rounded once (the Gram-Schmidt below
+    // magnifies errors of the scaling rows by up to ~1e6 at k = 9)
+    std::vector<long double> xl(K), wl(K), Ul(K * K, 0.0L);
+    gl01_ld(K, xl.data(), wl.data());
     if (type == 0) {
-        for (int i = 0; i < K; i++) B.U[i * K + i] = 1.0;
+        for (int i = 0; i < K; i++) Ul[i * K + i] = 1.0L;
     } else {
         // interpolating scaling functions: phi^I_j = sqrt(w_j) sum_i phi^L_i(x_j) phi^L_i
-        double tmp[32];
+        long double tmp[32];
         for (int j = 0; j < K; j++) {
-            legendre_phi(K, B.qx[j], tmp);
-            for (int i = 0; i < K; i++) B.U[j * K + i] = std::sqrt(B.qw[j]) * tmp[i];
+            legendre_phi_ld(K, xl[j], tmp);
+            for (int i = 0; i < K; i++) Ul[j * K + i] = std::sqrt(wl[j]) * tmp[i];
         }
     }
+    auto phi_ld = [&](long double x, long double *out) {
+        long double tmp[32];
+        legendre_phi_ld(K, x, tmp);
+        for (int i = 0; i < K; i++) {
+            long double s = 0.0L;
+            for (int j = 0; j < K; j++) s += Ul[i * K + j] * tmp[j];
+            out[i] = s;
+        }
+    };
+    B.qx.resize(K);
+    B.qw.resize(K);
+    B.U.resize(K * K);
     B.phiq.resize(K * K);
-    for (int q = 0; q < K; q++) B.phi(B.qx[q], &B.phiq[q * K]);
+    for (int q = 0; q < K; q++) {
+        B.qx[q] = (double)xl[q];
+        B.qw[q] = (double)wl[q];
+    }
+    for (int i = 0; i < K * K; i++) B.U[i] = (double)Ul[i];
+    for (int q = 0; q < K; q++) {
+        long double v[32];
+        phi_ld(xl[q], v);
+        for (int i = 0; i < K; i++) B.phiq[q * K + i] = (double)v[i];
+    }
 
     // H_a[i][j] = <phi^0_i, phi^1_{a,j}> = 2^{-1/2} int_0^1 phi_i((u+a)/2) phi_j(u) du
     int N = 2 * K;
     std::vector<long double> Wl(N * N, 0.0L);
-    double pi_[32], pj[32];
+    long double pi_[32], pj[32];
     for (int a = 0; a < 2; a++) {
         for (int q = 0; q < K; q++) {
-            double u = B.qx[q];
-            B.phi(0.5 * (u + a), pi_);
-            B.phi(u, pj);
+            long double u = xl[q];
+            phi_ld(0.5L * (u + a), pi_);
+            phi_ld(u, pj);
             for (int i = 0; i < K; i++)
-                for (int j = 0; j < K; j++)
-                    Wl[i * N + a * K + j] += (long double)B.qw[q] * pi_[i] * pj[j] / std::sqrt(2.0L);
+                for (int j = 0; j < K; j++) Wl[i * N + a * K + j] += wl[q] * pi_[i] * pj[j] / std::sqrt(2.0L);
         }
     }
     // wavelet rows: Gram-Schmidt of [H0_j, -H1_j] against the scaling rows (two passes)

@@ -43,7 +43,7 @@ def test_library_exports_every_symbol():
         assert hasattr(L, name)
 
 
-@pytest.mark.parametrize("k,basis", [(5, "L"), (7, "L"), (5, "I")])
+@pytest.mark.parametrize("k,basis", [(5, "L"), (7, "L"), (9, "L"), (5, "I")])
 def test_projection_matches_oracle(G, oracle, k, basis):
     g, _ = configs.density3(1)
     prec = 1e-5
@@ -212,7 +212,7 @@ def test_mixed_poisson_energy(G, oracle):
     assert abs(e32 - e64) < prec * abs(e64)
 
 
-@pytest.mark.parametrize("k", [5, 7])
+@pytest.mark.parametrize("k", [5, 7, 9])
 def test_multiply_matches_oracle(G, oracle, k):
     """mrcpp::multiply on the device vs the oracle on the same input trees."""
     prec = 1e-5
@@ -230,3 +230,42 @@ def test_multiply_matches_oracle(G, oracle, k):
     ex = GS.product(orbs[0], orbs[2]).scaled(1.5)
     pts = np.random.default_rng(5).uniform(-3, 3, (2000, 3))
     assert np.abs(p.evalf(pts) - ex(pts)).max() < 100 * prec * np.abs(ex(pts)).max()
+
+
+@pytest.mark.parametrize("k", [5, 9])
+def test_add_is_the_exact_linear_combination(G, oracle, k):
+    """mrcpp::add on the device: on the union of the two grids the refinement of the
+    coarser tree is exact, so the sum evaluates to ca a(x) + cb b(x) to rounding."""
+    prec = 1e-5
+    orbs, nuc, mus = configs.orbitals(3, n_orb=2)
+    mra = G.mra_from(configs.BOX20, k)
+    omra = oracle.MRA(**configs.BOX20, k=k, max_depth=16)
+    oa = oracle.Tree(omra).project_gauss(prec, orbs[0])
+    ob = oracle.Tree(omra).project_gauss(prec * 10, orbs[1])
+    a = G.FunctionTree(mra).load(*oa.export())
+    b = G.FunctionTree(mra).load(*ob.export())
+    s = G.add(G.FunctionTree(mra), 0.7, a, -1.3, b)
+    assert s.getNNodes() >= max(a.getNNodes(), b.getNNodes())
+    pts = np.random.default_rng(11).uniform(-6, 6, (3000, 3))
+    ex = 0.7 * a.evalf(pts) - 1.3 * b.evalf(pts)
+    assert np.abs(s.evalf(pts) - ex).max() <= 1e-12 * np.abs(ex).max()
+
+
+def test_handles_destroyed_in_any_order(G):
+    """A garbage collector may finalise the context and the MRA before the trees made on
+    them; every handle keeps what it was made on, so the tree still works and frees last."""
+    g, _ = configs.density3(1)
+    ctx = G.Context(0)
+    mra = G.mra_from(configs.BOX16, 5, ctx=ctx)
+    t = G.project(1e-4, G.FunctionTree(mra), g)
+    ref = G.project(1e-4, G.FunctionTree(G.mra_from(configs.BOX16, 5)), g)
+    n, nrm = t.getNNodes(), t.getSquareNorm()
+    t2 = G.FunctionTree(mra)
+    ctx.__del__()
+    mra.__del__()
+    assert t.getNNodes() == n == ref.getNNodes()
+    pts = np.random.default_rng(5).uniform(-4, 4, (500, 3))
+    assert np.array_equal(t.evalf(pts), ref.evalf(pts))
+    G.add(t2, 1.0, t, 1.0, t)
+    assert t2.getSquareNorm() == pytest.approx(4.0 * nrm, rel=1e-12)
+    del t, t2
