@@ -218,6 +218,15 @@ struct Readback {
     }
 };
 
+// the candidate walk of an output node is split over up to ENUM_MAX_PARTS workgroups so
+// that a work list launches about ENUM_BLOCKS of them
+#ifndef ENUM_BLOCKS
+#define ENUM_BLOCKS 1024
+#endif
+#ifndef ENUM_MAX_PARTS
+#define ENUM_MAX_PARTS 16
+#endif
+
 struct Ctx {
     int device = 0;
     std::atomic<int> refs{1};  // the owner's handle, plus one per MRA and predictor made on it
@@ -1141,7 +1150,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         // run it again.
         const int nseg = mixed ? 9 : 8;
         // few output nodes (the roots): several workgroups share each node's candidates
-        const int nparts = std::max(1, std::min(16, (1024 + nw - 1) / nw));
+        const int nparts = std::max(1, std::min(ENUM_MAX_PARTS, (ENUM_BLOCKS + nw - 1) / nw));
         C.thread_off.alloc((size_t)nw * nseg * nparts * 256);
         C.gtotal.alloc((size_t)nw * nseg * nparts);
         C.gofs.alloc((size_t)nw * nseg * nparts);
