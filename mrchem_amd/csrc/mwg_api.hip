@@ -221,7 +221,7 @@ struct Readback {
 // the candidate walk of an output node is split over up to ENUM_MAX_PARTS workgroups so
 // that a work list launches about ENUM_BLOCKS of them
 #ifndef ENUM_BLOCKS
-#define ENUM_BLOCKS 1024
+#define ENUM_BLOCKS 512
 #endif
 #ifndef ENUM_MAX_PARTS
 #define ENUM_MAX_PARTS 16
