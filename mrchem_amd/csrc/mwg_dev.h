@@ -80,13 +80,17 @@ struct Slice {
 static_assert(sizeof(Slice) == 32, "slice record is 32 bytes");
 
 // a slice is contracted in FP64 (as tuples) when its largest bound exceeds
-// ROUTE_FP64 = 2^17 times the screening threshold thr: the FP32 rounding of a slice
-// (~8 * 2^-24 of its bound) then stays below thr / 16
-constexpr double ROUTE_FP64 = 131072.0;
+// ROUTE_FP64 = 2^21 times the screening threshold thr: the FP32 rounding of a slice
+// (~8 * 2^-24 of its bound) then stays below thr, the size of every tuple the
+// screening drops
+#ifndef MWG_ROUTE_FP64
+#define MWG_ROUTE_FP64 2097152.0
+#endif
+constexpr double ROUTE_FP64 = MWG_ROUTE_FP64;
 // slices per wave of an MFMA task: a wave sums its slices' results in FP32 before
 // the FP64 sum over waves and tasks, so its run is kept short: the FP32 running sum
 // of n <= 16 slices adds at most (n - 1) 2^-24 of their summed bounds, i.e. below
-// 2 thr for a wave whose 16 slices all sit at the routing bound (the plain method
+// 30 thr for a wave whose 16 slices all sit at the routing bound (the plain method
 // drops every tuple below thr, hundreds per node)
 constexpr int MAX_SLICE_CHUNK = 16;
 // on the root level (no screening threshold yet) slices above ROUTE_ROOT times the
