@@ -295,8 +295,11 @@ def scf_main(args, G, configs):
     log(f"[bench] scf step: {len(inps)} applies, input nodes {sum(t.getNNodes() for t in inps)}, "
         f"inputs {t_in:.1f} s, operators {t_op:.1f} s")
     res64 = run_variant(G, mra, inps, ops, args.prec, G.MWG_APPLY_FP64, args.steps, args.warmup)
+    K = k + 1
+    scf_tflops = res64["tuples_per_step"] * 6.0 * K ** 4 / res64["t_contract_per_step"] / 1e12
     log(f"[bench] fp64: {res64['ms_per_step']:.0f} ms/step, {res64['nodes_per_s']:.0f} nodes/s, "
-        f"iters {res64['iters']}")
+        f"iters {res64['iters']}, {res64['tuples_per_step']:.0f} tuples, contraction "
+        f"{1e3 * res64['t_contract_per_step']:.0f} ms/step = {scf_tflops:.1f} TFLOP/s FP64")
     best = res64
     resP = None
     # keep the Helmholtz outputs and a few pair potentials for the error check, free the rest
@@ -338,8 +341,13 @@ def scf_main(args, G, configs):
                    "predict_nodes_per_s": resP["nodes_per_s"] if resP else None,
                    "predict_iters": resP["iters"] if resP else None,
                    "predict_rel_l2_err_vs_fp64": resP["rel_err_vs_fp64"] if resP else None,
-                   "output_nodes_per_step": best["nodes_per_step"], "inputs_s": t_in, "operators_s": t_op},
-        "roofline": None, "cpu_baseline": cpu,
+                   "output_nodes_per_step": best["nodes_per_step"], "inputs_s": t_in, "operators_s": t_op,
+                   "fp64_tuples_per_step": res64["tuples_per_step"],
+                   "fp64_contraction_ms_per_step": 1e3 * res64["t_contract_per_step"]},
+        # FP64 contraction (plain variant): 6 (k+1)^4 flops per tuple over the contraction time
+        "roofline": {"bound": "mfma", "achieved": scf_tflops, "peak": 78.6, "unit": "TFLOP/s",
+                     "frac": scf_tflops / 78.6, "traffic": None, "kernel": f"k_contract_fp64<{K}> + k_reduce_partials"},
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 
