@@ -182,7 +182,7 @@ def exchange_main(args, G, configs):
     t_mul = time.perf_counter() - t0
     P = G.PoissonOperator(mra, args.prec)
     ops = [P] * len(rho)
-    log(f"[bench] exchange: {len(rho)} pair densities ({sum(r.getNNodes() for r in rho)} nodes) in {t_mul:.1f} s")
+    log(f"[bench] exchange: {len(rho)} pair densities ({sum(r.getNNodes() for r in rho)} nodes) in {t_mul:.3f} s")
     res64 = run_variant(G, mra, rho, ops, args.prec, G.MWG_APPLY_FP64, args.steps, args.warmup)
     log(f"[bench] fp64: {res64['ms_per_step']:.1f} ms/step, {res64['nodes_per_s']:.0f} nodes/s")
     res32 = run_variant(G, mra, rho, ops, args.prec, G.MWG_APPLY_MIXED, args.steps, args.warmup)
@@ -293,7 +293,7 @@ def scf_main(args, G, configs):
     inps = trees + rho
     ops = Hs + [P] * len(rho)
     log(f"[bench] scf step: {len(inps)} applies, input nodes {sum(t.getNNodes() for t in inps)}, "
-        f"inputs {t_in:.1f} s, operators {t_op:.1f} s")
+        f"inputs {t_in:.2f} s, operators {t_op:.2f} s")
     res64 = run_variant(G, mra, inps, ops, args.prec, G.MWG_APPLY_FP64, args.steps, args.warmup)
     K = k + 1
     scf_tflops = res64["tuples_per_step"] * 6.0 * K ** 4 / res64["t_contract_per_step"] / 1e12
