@@ -176,13 +176,21 @@ def exchange_main(args, G, configs):
     mra = G.mra_from(configs.BOX20, args.k, "L", max_depth=16)
     orbs, nuc, mus = configs.orbitals(args.seed, n_orb=16, n_nuc=8, nuc_box=6.0)
     trees = [G.project(args.prec, G.FunctionTree(mra), o) for o in orbs]
-    t0 = time.perf_counter()
     pairs = [(i, j) for i in range(16) for j in range(i, 16)]
-    rho = [G.multiply(args.prec, G.FunctionTree(mra), 1.0, trees[i], trees[j]) for i, j in pairs]
+    t0 = time.perf_counter()
+    rho1 = [G.multiply(args.prec, G.FunctionTree(mra), 1.0, trees[i], trees[j]) for i, j in pairs]
+    t_mul1 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rho = G.multiply_batch(args.prec, [G.FunctionTree(mra) for _ in pairs], 1.0, [trees[i] for i, _ in pairs],
+                           [trees[j] for _, j in pairs])
     t_mul = time.perf_counter() - t0
+    if any(a.getSquareNorm() != b.getSquareNorm() or a.getNNodes() != b.getNNodes() for a, b in zip(rho, rho1)):
+        raise RuntimeError("batched pair products differ from the single products")
+    del rho1
     P = G.PoissonOperator(mra, args.prec)
     ops = [P] * len(rho)
-    log(f"[bench] exchange: {len(rho)} pair densities ({sum(r.getNNodes() for r in rho)} nodes) in {t_mul:.3f} s")
+    log(f"[bench] exchange: {len(rho)} pair densities ({sum(r.getNNodes() for r in rho)} nodes) in {t_mul:.3f} s "
+        f"batched ({t_mul1:.3f} s one by one)")
     res64 = run_variant(G, mra, rho, ops, args.prec, G.MWG_APPLY_FP64, args.steps, args.warmup)
     log(f"[bench] fp64: {res64['ms_per_step']:.1f} ms/step, {res64['nodes_per_s']:.0f} nodes/s")
     res32 = run_variant(G, mra, rho, ops, args.prec, G.MWG_APPLY_MIXED, args.steps, args.warmup)
@@ -201,7 +209,8 @@ def exchange_main(args, G, configs):
                    "fp64_nodes_per_s": res64["nodes_per_s"], "mixed_nodes_per_s": res32["nodes_per_s"],
                    "speedup_vs_fp64": best["nodes_per_s"] / res64["nodes_per_s"],
                    "per_pair_rel_l2_err_max": max(errs), "per_pair_rel_l2_err_mean": sum(errs) / len(errs),
-                   "output_nodes_per_step": best["nodes_per_step"], "multiply_s": t_mul},
+                   "output_nodes_per_step": best["nodes_per_step"], "multiply_s": t_mul,
+                   "multiply_one_by_one_s": t_mul1},
         "roofline": dict(bound="mfma", achieved=achieved / 1e12, peak=PEAK[dtype] / 1e12, unit="TFLOP/s",
                          frac=achieved / PEAK[dtype], traffic=None, contraction_ms_per_step=1e3 * tc),
         "cpu_baseline": None,
@@ -284,7 +293,8 @@ def scf_main(args, G, configs):
     t0 = time.perf_counter()
     trees = [G.project(args.prec, G.FunctionTree(mra), o) for o in orbs]
     pairs = [(i, j) for i in range(32) for j in range(i, 32)]
-    rho = [G.multiply(args.prec, G.FunctionTree(mra), 1.0, trees[i], trees[j]) for i, j in pairs]
+    rho = G.multiply_batch(args.prec, [G.FunctionTree(mra) for _ in pairs], 1.0, [trees[i] for i, _ in pairs],
+                           [trees[j] for _, j in pairs])
     t_in = time.perf_counter() - t0
     t0 = time.perf_counter()
     P = G.PoissonOperator(mra, args.prec)

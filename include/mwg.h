@@ -118,6 +118,11 @@ int mwg_project_hydrogen(double prec, mwg_tree out, int n, const double *coef, c
  * products (ExchangePotential.cpp:312, 329, 335), V*phi and the density.  out = coef*a*b on
  * the union grid of a and b, refined by the wavelet-norm split test. */
 int mwg_multiply(double prec, mwg_tree out, double coef, mwg_tree a, mwg_tree b);
+/* n independent products out[i] = coef[i]*a[i]*b[i] at prec[i]: no single reference
+ * counterpart (MRChem loops over qmfunction::multiply for the pairs of
+ * ExchangePotential.cpp:312-335); each output is identical to its mwg_multiply, with the
+ * launches of all products merged and one host round trip per refinement level. */
+int mwg_multiply_batch(int n, const double *prec, mwg_tree *out, const double *coef, mwg_tree *a, mwg_tree *b);
 
 /* ---- linear combination: replaces mrcpp::add(prec, out, ca, a, cb, b) as used by
  * QMFunction::add / qmfunction::linear_combination (src/qmfunctions/QMFunction.cpp:157-200):

@@ -72,6 +72,7 @@ def _sig():
         "mwg_oper_terms": [vp, fd, fd],
         "mwg_oper_blocks": [vp, i, i, i64, i64, fd],
         "mwg_multiply": [d, vp, d, vp, vp],
+        "mwg_multiply_batch": [i, fd, pvp, fd, pvp, pvp],
         "mwg_add": [vp, d, vp, d, vp],
         "mwg_apply": [d, vp, vp, vp, i, u, C.POINTER(ApplyStats)],
         "mwg_apply_batch": [i, fd, pvp, pvp, pvp, i, u, C.POINTER(ApplyStats)],
@@ -96,7 +97,7 @@ EXPORTED = ["mwg_ctx_create", "mwg_ctx_destroy", "mwg_last_error", "mwg_sync", "
             "mwg_tree_create", "mwg_tree_destroy", "mwg_tree_nnodes", "mwg_tree_sqnorm", "mwg_tree_maxdepth",
             "mwg_tree_import", "mwg_tree_export", "mwg_tree_rescale", "mwg_tree_dot", "mwg_tree_eval",
             "mwg_project_gauss", "mwg_project_hydrogen", "mwg_oper_poisson", "mwg_oper_helmholtz",
-            "mwg_oper_destroy", "mwg_oper_nterms", "mwg_oper_terms", "mwg_oper_blocks", "mwg_multiply", "mwg_add", "mwg_apply",
+            "mwg_oper_destroy", "mwg_oper_nterms", "mwg_oper_terms", "mwg_oper_blocks", "mwg_multiply", "mwg_multiply_batch", "mwg_add", "mwg_apply",
             "mwg_apply_batch", "mwg_predictor_create", "mwg_predictor_destroy", "mwg_predictor_sizes",
             "mwg_predictor_get", "mwg_predictor_set", "mwg_predictor_train", "mwg_predictor_gradient",
             "mwg_predictor_infer", "mwg_ctx_set_predictor", "mwg_trace_enable", "mwg_trace_size", "mwg_trace_get"]

@@ -260,6 +260,8 @@ struct Ctx {
     DBuf<unsigned long long> genHashKeys;
     DBuf<int> genHashVals;
     DBuf<TransformItem> titems;
+    DBuf<MulItem> mitems;  // batched products: items and work lists of all jobs
+    DBuf<int> mwork;
     DBuf<ContractTask> tasks;
     DBuf<ReduceItem> reds;
     DBuf<double> partials, nodeslots;
@@ -1565,6 +1567,163 @@ static void multiply_trees(TreeH &out, double prec, double coef, TreeH &A, TreeH
     out.refresh_norms();
 }
 
+// Many independent products as one: the same items, kernels and split tests as
+// multiply_trees for every job, with the launches of all jobs queued back to back and
+// one host round trip per refinement iteration (uploads through the staging ring, one
+// read-back of all jobs' norms).  Each output is identical to its single product.
+struct MulJob {
+    TreeH *out, *a, *b;
+    double prec, coef;
+    std::vector<int> work;
+    double sNorm = 0.0, wNorm = 0.0;
+    bool first = true;
+};
+
+static void multiply_batch(std::vector<MulJob> &jobs) {
+    if (jobs.empty()) return;
+    MraH &M = *jobs[0].a->mra;
+    Ctx &C = *M.ctx;
+    hipStream_t s = C.stream;
+    for (MulJob &J : jobs) {
+        TreeH &out = *J.out, &A = *J.a, &B = *J.b;
+        if (A.n() == 0 || B.n() == 0) throw Error(MWG_ERR_STATE, "multiply: empty input tree");
+        out.make_roots();
+        std::vector<int> order(out.roots.begin(), out.roots.end());
+        for (size_t w = 0; w < order.size(); w++) {
+            const int i = order[w];
+            const int d = out.depth[i], x = out.lr[3 * i], y = out.lr[3 * i + 1], z = out.lr[3 * i + 2];
+            const int ia = A.find(d, x, y, z), ib = B.find(d, x, y, z);
+            const bool sp = (ia >= 0 && A.child[ia] >= 0) || (ib >= 0 && B.child[ib] >= 0);
+            if (sp && d < M.maxDepth) {
+                out.split(i);
+                for (int a = 0; a < 8; a++) order.push_back(out.child[i] + a);
+            }
+        }
+        J.work.resize(out.n());
+        for (int i = 0; i < out.n(); i++) J.work[i] = i;
+    }
+    std::vector<MulItem> items;
+    std::vector<int> wl;
+    std::vector<size_t> ofs(jobs.size() + 1);
+    while (true) {
+        items.clear();
+        wl.clear();
+        for (size_t j = 0; j < jobs.size(); j++) {
+            MulJob &J = jobs[j];
+            ofs[j] = items.size();
+            if (J.work.empty()) continue;
+            J.out->ensure_dev(J.out->n());
+            for (int i : J.work) {
+                MulItem it;
+                it.node = i;
+                it.depth = J.out->depth[i];
+                for (int q = 0; q < 3; q++) it.l[q] = J.out->lr[3 * i + q];
+                locate(*J.a, it.depth, it.l, it.asrc, it.asteps);
+                locate(*J.b, it.depth, it.l, it.bsrc, it.bsteps);
+                items.push_back(it);
+                wl.push_back(i);
+            }
+        }
+        ofs[jobs.size()] = items.size();
+        if (items.empty()) break;
+        upload_async(C, C.mitems, items.data(), items.size());
+        upload_async(C, C.mwork, wl.data(), wl.size());
+        size_t total = 0;
+        for (size_t j = 0; j < jobs.size(); j++) {
+            const int nj = (int)(ofs[j + 1] - ofs[j]);
+            if (!nj) continue;
+            MulJob &J = jobs[j];
+            HC(launch_multiply(s, nj, C.mitems.p + ofs[j], M.geom(), M.dev(), J.coef, J.a->coef.p, J.b->coef.p,
+                               J.out->coef.p));
+            HC(launch_norms(s, nj, C.mwork.p + ofs[j], J.out->coef.p, J.out->norm.p, J.out->K3()));
+            total += 8 * (size_t)J.out->n();
+        }
+        C.rb.reserve(total * sizeof(double));
+        double *h = (double *)C.rb.h;
+        size_t off = 0;
+        for (size_t j = 0; j < jobs.size(); j++) {
+            if (ofs[j + 1] == ofs[j]) continue;
+            const size_t m = 8 * (size_t)jobs[j].out->n();
+            HC(hipMemcpyAsync(h + off, jobs[j].out->norm.p, m * sizeof(double), hipMemcpyDeviceToHost, s));
+            off += m;
+        }
+        stream_wait(C);
+        C.stage.off = 0;
+        off = 0;
+        for (size_t j = 0; j < jobs.size(); j++) {
+            if (ofs[j + 1] == ofs[j]) continue;
+            MulJob &J = jobs[j];
+            TreeH &out = *J.out;
+            const double *nrm = h + off;
+            off += 8 * (size_t)out.n();
+            for (int i : J.work) {
+                std::memcpy(&out.cnorm[8 * (size_t)i], &nrm[8 * (size_t)i], 8 * sizeof(double));
+                if (J.first && out.depth[i] == 0) J.sNorm += nrm[8 * i] * nrm[8 * i];
+                for (int c = 1; c < 8; c++) J.wNorm += nrm[8 * i + c] * nrm[8 * i + c];
+            }
+            J.first = false;
+            const double tnorm = std::sqrt(J.sNorm + J.wNorm);
+            std::vector<int> next;
+            for (int i : J.work) {
+                if (out.child[i] >= 0) continue;
+                const int d = out.depth[i];
+                if (d >= M.maxDepth) continue;
+                double wn = 0.0;
+                for (int c = 1; c < 8; c++) wn += out.cnorm[8 * i + c] * out.cnorm[8 * i + c];
+                if (split_check(M.n0 + d, std::sqrt(wn), J.prec, tnorm)) {
+                    out.split(i);
+                    for (int a = 0; a < 8; a++) next.push_back(out.child[i] + a);
+                }
+            }
+            J.work.swap(next);
+        }
+    }
+    // bottom-up compression, all jobs' parents of a depth in one launch
+    std::vector<std::vector<std::vector<int>>> L(jobs.size());
+    int maxd = 0;
+    for (size_t j = 0; j < jobs.size(); j++) {
+        L[j] = jobs[j].out->internal_by_depth();
+        maxd = std::max(maxd, (int)L[j].size());
+    }
+    std::vector<TransformItem> titems;
+    std::vector<size_t> lofs(maxd + 1, 0);
+    for (int d = 0; d < maxd; d++) {
+        lofs[d] = titems.size();
+        for (size_t j = 0; j < jobs.size(); j++)
+            if (d < (int)L[j].size())
+                for (int p : L[j][d]) titems.push_back({jobs[j].out->coef.p, nullptr, p, jobs[j].out->child[p]});
+    }
+    lofs[maxd] = titems.size();
+    if (!titems.empty()) {
+        upload_async(C, C.titems, titems.data(), titems.size());
+        for (int d = maxd - 1; d >= 0; d--)
+            HC(launch_transform(s, 0, (int)(lofs[d + 1] - lofs[d]), C.titems.p + lofs[d], M.W.p, M.B.K));
+    }
+    // every node's norms (as refresh_norms), one read-back for all jobs
+    size_t total = 0;
+    for (MulJob &J : jobs) {
+        HC(launch_norms(s, J.out->n(), nullptr, J.out->coef.p, J.out->norm.p, J.out->K3()));
+        total += 8 * (size_t)J.out->n();
+    }
+    C.rb.reserve(total * sizeof(double));
+    double *h = (double *)C.rb.h;
+    size_t off = 0;
+    for (MulJob &J : jobs) {
+        const size_t m = 8 * (size_t)J.out->n();
+        HC(hipMemcpyAsync(h + off, J.out->norm.p, m * sizeof(double), hipMemcpyDeviceToHost, s));
+        off += m;
+    }
+    stream_wait(C);
+    C.stage.off = 0;
+    off = 0;
+    for (MulJob &J : jobs) {
+        const size_t m = 8 * (size_t)J.out->n();
+        J.out->cnorm.assign(h + off, h + off + m);
+        off += m;
+        J.out->sqnorm_from_cnorm();
+    }
+}
+
 // out = ca * a + cb * b on the union grid (mrcpp::add)
 static void add_trees(TreeH &out, double ca, TreeH &A, double cb, TreeH &B) {
     MraH &M = *A.mra;
@@ -2039,6 +2198,31 @@ int mwg_multiply(double prec, mwg_tree out, double coef, mwg_tree a, mwg_tree b)
         if (out == a || out == b) throw Error(MWG_ERR_ARG, "multiply: output must differ from the inputs");
         if (!(prec > 0.0)) throw Error(MWG_ERR_ARG, "multiply: precision must be positive");
         multiply_trees(*out, prec, coef, *a, *b);
+    });
+}
+
+int mwg_multiply_batch(int n, const double *prec, mwg_tree *out, const double *coef, mwg_tree *a, mwg_tree *b) {
+    if (n <= 0 || !prec || !out || !coef || !a || !b) return MWG_ERR_ARG;
+    for (int i = 0; i < n; i++)
+        if (!out[i] || !a[i] || !b[i]) return MWG_ERR_ARG;
+    MraH *M = a[0]->mra;
+    for (int i = 0; i < n; i++)
+        if (a[i]->mra != M || b[i]->mra != M || out[i]->mra != M) return MWG_ERR_ARG;
+    Ctx *C = M->ctx;
+    return guarded(C, [&] {
+        std::vector<MulJob> jobs(n);
+        for (int i = 0; i < n; i++) {
+            if (!(prec[i] > 0.0)) throw Error(MWG_ERR_ARG, "multiply: precision must be positive");
+            for (int j = 0; j < n; j++)
+                if (out[i] == a[j] || out[i] == b[j] || (j != i && out[i] == out[j]))
+                    throw Error(MWG_ERR_ARG, "multiply: outputs must differ from the inputs and each other");
+            jobs[i].out = out[i];
+            jobs[i].a = a[i];
+            jobs[i].b = b[i];
+            jobs[i].prec = prec[i];
+            jobs[i].coef = coef[i];
+        }
+        multiply_batch(jobs);
     });
 }
 

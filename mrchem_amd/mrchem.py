@@ -173,8 +173,11 @@ class ExchangePotential:
         return pairs
 
     def pair_densities(self, pairs):
-        return [M.multiply(p, M.FunctionTree(self.mra), 1.0, self.orbitals[i].tree, self.orbitals[j].tree)
-                for i, j, p in pairs]
+        """phi_i phi_j for every pair, as one batched multiply (the reference multiplies the
+        pairs one at a time, ExchangePotential.cpp:312-335; each product is the same)."""
+        outs = [M.FunctionTree(self.mra) for _ in pairs]
+        return M.multiply_batch([p for _, _, p in pairs], outs, 1.0, [self.orbitals[i].tree for i, _, _ in pairs],
+                                [self.orbitals[j].tree for _, j, _ in pairs])
 
     def setup(self, prec: float, flags: int = M.MWG_APPLY_FP64, pair_densities=None):
         if self.exchange is not None:
@@ -195,18 +198,29 @@ class ExchangePotential:
             else:
                 ex[k] = M.add(M.FunctionTree(self.mra), 1.0, ex[k], f, t)
 
+        # the products V_ij phi_j (and V_ij phi_i) of all pairs in one batched multiply, then
+        # summed into the exchange functions in pair order
+        spec = []
         for (i, j, p), V in zip(pairs, outs):
             self.pair_potentials[(i, j)] = V
+            spec.append((p, 1.0 / Phi[j].squared_norm(), Phi[j].tree, V))
+            if i != j:
+                spec.append((p, 1.0 / Phi[i].squared_norm(), Phi[i].tree, V))
+        prods = M.multiply_batch([q[0] for q in spec], [M.FunctionTree(self.mra) for _ in spec],
+                                 [q[1] for q in spec], [q[2] for q in spec], [q[3] for q in spec])
+        n = 0
+        for (i, j, p), V in zip(pairs, outs):
             if i == j:
-                t = M.multiply(p, M.FunctionTree(self.mra), 1.0 / Phi[i].squared_norm(), Phi[i].tree, V)
+                t = prods[n]
+                n += 1
                 self.part_norms[i, i] = math.sqrt(t.getSquareNorm())
                 acc(i, 1.0, t)
             else:
                 fi = spin_factor(Phi[i], Phi[j])
                 fj = spin_factor(Phi[j], Phi[i])
-                tj = M.multiply(p, M.FunctionTree(self.mra), 1.0 / Phi[j].squared_norm(), Phi[j].tree, V)
+                tj, ti = prods[n], prods[n + 1]
+                n += 2
                 self.part_norms[j, i] = math.sqrt(tj.getSquareNorm())
-                ti = M.multiply(p, M.FunctionTree(self.mra), 1.0 / Phi[i].squared_norm(), Phi[i].tree, V)
                 self.part_norms[i, j] = math.sqrt(ti.getSquareNorm())
                 acc(i, fi, tj)
                 acc(j, fj, ti)

@@ -27,7 +27,7 @@ from .gaussians import GaussSum
 
 __all__ = ["BoundingBox", "LegendreBasis", "InterpolatingBasis", "MultiResolutionAnalysis", "FunctionTree",
            "PoissonOperator", "HelmholtzOperator", "apply", "apply_batch", "project", "project_hydrogen", "dot",
-           "multiply", "add", "Predictor", "trace_enable", "trace_get",
+           "multiply", "multiply_batch", "add", "Predictor", "trace_enable", "trace_get",
            "MwgError", "MWG_APPLY_FP64", "MWG_APPLY_MIXED", "MWG_APPLY_PREDICT", "Context", "default_context"]
 
 
@@ -257,6 +257,22 @@ def multiply(prec: float, out: FunctionTree, coef: float, a: FunctionTree, b: Fu
     """out = coef * a * b (mrcpp::multiply): union grid of a and b, refined by the split test."""
     check(lib().mwg_multiply(float(prec), out.h, float(coef), a.h, b.h), a.mra.ctx.h)
     return out
+
+
+def multiply_batch(prec, outs, coefs, as_, bs) -> list:
+    """outs[i] = coefs[i] * as_[i] * bs[i] for all i: the same trees as one multiply per
+    product, with all products refined together (one host round trip per level)."""
+    n = len(outs)
+    if not (len(as_) == n and len(bs) == n):
+        raise MwgError("multiply_batch: length mismatch")
+    if n == 0:
+        return outs
+    precs = np.broadcast_to(np.asarray(prec, np.float64), (n,)).copy()
+    cs = np.broadcast_to(np.asarray(coefs, np.float64), (n,)).copy()
+    arr = C.c_void_p * n
+    check(lib().mwg_multiply_batch(n, precs, arr(*[o.h for o in outs]), cs, arr(*[a.h for a in as_]),
+                                   arr(*[b.h for b in bs])), as_[0].mra.ctx.h)
+    return outs
 
 
 def add(out: FunctionTree, ca: float, a: FunctionTree, cb: float, b: FunctionTree) -> FunctionTree:

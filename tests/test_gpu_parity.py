@@ -269,3 +269,22 @@ def test_handles_destroyed_in_any_order(G):
     G.add(t2, 1.0, t, 1.0, t)
     assert t2.getSquareNorm() == pytest.approx(4.0 * nrm, rel=1e-12)
     del t, t2
+
+
+@pytest.mark.parametrize("k", [5, 9])
+def test_multiply_batch_equals_single_products(G, k):
+    """mwg_multiply_batch: every product (pairs with a repeated factor, different
+    precisions and coefficients) is bit-identical to its own mwg_multiply."""
+    orbs, nuc, mus = configs.orbitals(2, n_orb=3)
+    mra = G.mra_from(configs.BOX20, k)
+    t = [G.project(1e-5, G.FunctionTree(mra), o) for o in orbs]
+    spec = [(1e-5, 1.0, 0, 1), (1e-4, -0.5, 2, 2), (3e-6, 2.0, 1, 2), (1e-5, 1.0, 0, 0)]
+    single = [G.multiply(p, G.FunctionTree(mra), c, t[i], t[j]) for p, c, i, j in spec]
+    batch = G.multiply_batch([q[0] for q in spec], [G.FunctionTree(mra) for _ in spec], [q[1] for q in spec],
+                             [t[q[2]] for q in spec], [t[q[3]] for q in spec])
+    for a, b in zip(batch, single):
+        (d0, l0, c0), (d1, l1, c1) = a.export(), b.export()
+        assert np.array_equal(d0, d1) and np.array_equal(l0, l1) and np.array_equal(c0, c1)
+        assert a.getSquareNorm() == b.getSquareNorm()
+    with pytest.raises(G.MwgError):
+        G.multiply_batch(1e-5, [t[0]], 1.0, [t[0]], [t[1]])
