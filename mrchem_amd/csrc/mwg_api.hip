@@ -24,6 +24,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/mwg.h"
@@ -1579,14 +1580,41 @@ struct MulJob {
     bool first = true;
 };
 
+// f(j) for every job j on up to 16 host threads (host-only work on distinct output
+// trees; the inputs are only read); the first exception is rethrown
+template <class F>
+static void for_each_job(size_t n, F f) {
+    const size_t nt = std::min<size_t>({n, (size_t)16, (size_t)std::max(1u, std::thread::hardware_concurrency())});
+    if (nt <= 1) {
+        for (size_t j = 0; j < n; j++) f(j);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    std::vector<std::exception_ptr> err(nt);
+    std::vector<std::thread> th;
+    for (size_t t = 0; t < nt; t++)
+        th.emplace_back([&, t] {
+            try {
+                for (size_t j = next++; j < n; j = next++) f(j);
+            } catch (...) {
+                err[t] = std::current_exception();
+            }
+        });
+    for (auto &x : th) x.join();
+    for (auto &e : err)
+        if (e) std::rethrow_exception(e);
+}
+
 static void multiply_batch(std::vector<MulJob> &jobs) {
     if (jobs.empty()) return;
     MraH &M = *jobs[0].a->mra;
     Ctx &C = *M.ctx;
     hipStream_t s = C.stream;
-    for (MulJob &J : jobs) {
+    for (MulJob &J : jobs)
+        if (J.a->n() == 0 || J.b->n() == 0) throw Error(MWG_ERR_STATE, "multiply: empty input tree");
+    for_each_job(jobs.size(), [&](size_t j) {
+        MulJob &J = jobs[j];
         TreeH &out = *J.out, &A = *J.a, &B = *J.b;
-        if (A.n() == 0 || B.n() == 0) throw Error(MWG_ERR_STATE, "multiply: empty input tree");
         out.make_roots();
         std::vector<int> order(out.roots.begin(), out.roots.end());
         for (size_t w = 0; w < order.size(); w++) {
@@ -1601,28 +1629,34 @@ static void multiply_batch(std::vector<MulJob> &jobs) {
         }
         J.work.resize(out.n());
         for (int i = 0; i < out.n(); i++) J.work[i] = i;
-    }
+    });
     std::vector<MulItem> items;
     std::vector<int> wl;
     std::vector<size_t> ofs(jobs.size() + 1);
+    std::vector<std::vector<MulItem>> jitems(jobs.size());
     while (true) {
-        items.clear();
-        wl.clear();
-        for (size_t j = 0; j < jobs.size(); j++) {
+        for (MulJob &J : jobs)
+            if (!J.work.empty()) J.out->ensure_dev(J.out->n());
+        for_each_job(jobs.size(), [&](size_t j) {
             MulJob &J = jobs[j];
-            ofs[j] = items.size();
-            if (J.work.empty()) continue;
-            J.out->ensure_dev(J.out->n());
-            for (int i : J.work) {
-                MulItem it;
+            std::vector<MulItem> &v = jitems[j];
+            v.resize(J.work.size());
+            for (size_t w = 0; w < J.work.size(); w++) {
+                const int i = J.work[w];
+                MulItem &it = v[w];
                 it.node = i;
                 it.depth = J.out->depth[i];
                 for (int q = 0; q < 3; q++) it.l[q] = J.out->lr[3 * i + q];
                 locate(*J.a, it.depth, it.l, it.asrc, it.asteps);
                 locate(*J.b, it.depth, it.l, it.bsrc, it.bsteps);
-                items.push_back(it);
-                wl.push_back(i);
             }
+        });
+        items.clear();
+        wl.clear();
+        for (size_t j = 0; j < jobs.size(); j++) {
+            ofs[j] = items.size();
+            items.insert(items.end(), jitems[j].begin(), jitems[j].end());
+            wl.insert(wl.end(), jobs[j].work.begin(), jobs[j].work.end());
         }
         ofs[jobs.size()] = items.size();
         if (items.empty()) break;
@@ -1649,13 +1683,18 @@ static void multiply_batch(std::vector<MulJob> &jobs) {
         }
         stream_wait(C);
         C.stage.off = 0;
+        std::vector<size_t> hofs(jobs.size(), 0);
         off = 0;
         for (size_t j = 0; j < jobs.size(); j++) {
             if (ofs[j + 1] == ofs[j]) continue;
+            hofs[j] = off;
+            off += 8 * (size_t)jobs[j].out->n();
+        }
+        for_each_job(jobs.size(), [&](size_t j) {
+            if (ofs[j + 1] == ofs[j]) return;
             MulJob &J = jobs[j];
             TreeH &out = *J.out;
-            const double *nrm = h + off;
-            off += 8 * (size_t)out.n();
+            const double *nrm = h + hofs[j];
             for (int i : J.work) {
                 std::memcpy(&out.cnorm[8 * (size_t)i], &nrm[8 * (size_t)i], 8 * sizeof(double));
                 if (J.first && out.depth[i] == 0) J.sNorm += nrm[8 * i] * nrm[8 * i];
@@ -1676,7 +1715,7 @@ static void multiply_batch(std::vector<MulJob> &jobs) {
                 }
             }
             J.work.swap(next);
-        }
+        });
     }
     // bottom-up compression, all jobs' parents of a depth in one launch
     std::vector<std::vector<std::vector<int>>> L(jobs.size());
