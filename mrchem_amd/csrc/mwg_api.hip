@@ -949,6 +949,31 @@ struct PhaseTimer {
     }
 };
 
+// f(j) for every job j on up to 16 host threads (host-only work on distinct output
+// trees; the inputs are only read); the first exception is rethrown
+template <class F>
+static void for_each_job(size_t n, F f) {
+    const size_t nt = std::min<size_t>({n, (size_t)16, (size_t)std::max(1u, std::thread::hardware_concurrency())});
+    if (nt <= 1) {
+        for (size_t j = 0; j < n; j++) f(j);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    std::vector<std::exception_ptr> err(nt);
+    std::vector<std::thread> th;
+    for (size_t t = 0; t < nt; t++)
+        th.emplace_back([&, t] {
+            try {
+                for (size_t j = next++; j < n; j = next++) f(j);
+            } catch (...) {
+                err[t] = std::current_exception();
+            }
+        });
+    for (auto &x : th) x.join();
+    for (auto &e : err)
+        if (e) std::rethrow_exception(e);
+}
+
 // device memory for the FP64 partial slots of one refinement iteration
 constexpr double SLOT_BUDGET = 8.0 * (1ull << 30);
 
@@ -1308,7 +1333,10 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         // children are kept only when the norm test splits their parent
         std::vector<int> cand_job, cand_node;
         std::vector<float> cand_prob;
-        for (int j : active) {
+        // without traces or the predictor (shared lists) the jobs' tests are independent:
+        // large batches spread them over host threads
+        const bool par_split = !C.tracing && !use_pred && active.size() >= 16;
+        auto split_job = [&](int j) {
             JobH &J = jobs[j];
             std::vector<int> cur = J.work;
             bool one_depth = true;
@@ -1379,7 +1407,11 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                 q0 = q1;
             }
             J.work.swap(next);
-        }
+        };
+        if (par_split)
+            for_each_job(active.size(), [&](size_t q) { split_job(active[q]); });
+        else
+            for (int j : active) split_job(j);
         // learned lookahead: nodes predicted to split get their children computed now
         if (use_pred && !cand_node.empty()) {
             const int nc = (int)cand_node.size();
@@ -1579,31 +1611,6 @@ struct MulJob {
     double sNorm = 0.0, wNorm = 0.0;
     bool first = true;
 };
-
-// f(j) for every job j on up to 16 host threads (host-only work on distinct output
-// trees; the inputs are only read); the first exception is rethrown
-template <class F>
-static void for_each_job(size_t n, F f) {
-    const size_t nt = std::min<size_t>({n, (size_t)16, (size_t)std::max(1u, std::thread::hardware_concurrency())});
-    if (nt <= 1) {
-        for (size_t j = 0; j < n; j++) f(j);
-        return;
-    }
-    std::atomic<size_t> next{0};
-    std::vector<std::exception_ptr> err(nt);
-    std::vector<std::thread> th;
-    for (size_t t = 0; t < nt; t++)
-        th.emplace_back([&, t] {
-            try {
-                for (size_t j = next++; j < n; j = next++) f(j);
-            } catch (...) {
-                err[t] = std::current_exception();
-            }
-        });
-    for (auto &x : th) x.join();
-    for (auto &e : err)
-        if (e) std::rethrow_exception(e);
-}
 
 static void multiply_batch(std::vector<MulJob> &jobs) {
     if (jobs.empty()) return;

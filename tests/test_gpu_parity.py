@@ -150,6 +150,29 @@ def test_batch_equals_single(G, oracle):
         assert np.abs(c1 - c2).max() <= 1e-13 * np.abs(c2).max()
 
 
+def test_large_batch_equals_single(G):
+    """A batch of 18 applies (large batches take the split tests of their jobs in
+    parallel on the host): the same grids as one apply each, coefficients to FP64 rounding."""
+    mra = G.mra_from(configs.BOX16, 5)
+    gs = [configs.density3(s)[0] for s in (4, 5, 6)]
+    fs = [G.project(1e-4, G.FunctionTree(mra), g) for g in gs]
+    P = G.PoissonOperator(mra, 1e-4)
+    H = G.HelmholtzOperator(mra, 0.8, 1e-4)
+    jobs = [(fs[q % 3], (P, H)[q % 2], (1e-4, 3e-4, 1e-3)[q % 3]) for q in range(18)]
+    singles = []
+    for f, op, p in jobs:
+        o = G.FunctionTree(mra)
+        G.apply(p, o, op, f)
+        singles.append(o)
+    outs = [G.FunctionTree(mra) for _ in jobs]
+    G.apply_batch([p for _, _, p in jobs], outs, [op for _, op, _ in jobs], [f for f, _, _ in jobs])
+    for a, b in zip(outs, singles):
+        d1, l1, c1 = a.export()
+        d2, l2, c2 = b.export()
+        assert np.array_equal(d1, d2) and np.array_equal(l1, l2)
+        assert np.abs(c1 - c2).max() <= 1e-13 * np.abs(c2).max()
+
+
 def test_reference_coulomb_values_on_gpu(G, A):
     """tests/qmoperators/coulomb_operator.cpp:76-82 at its settings, through the C ABI."""
     prec = 1e-3
