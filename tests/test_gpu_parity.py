@@ -222,6 +222,21 @@ def test_mixed_precision_within_prec_of_fp64(G, A, i):
     assert abs(s32["nodes_out"] - s64["nodes_out"]) <= 0.05 * s64["nodes_out"]
 
 
+def test_mixed_precision_interpolating_basis(G):
+    """The FP32 MFMA path on the Interpolating basis (k=7): same grid as the plain FP64
+    apply, difference below prec * ||out|| (L2)."""
+    prec = 1e-6
+    orbs, nuc, mus = configs.orbitals(0, n_orb=2)
+    mra = G.mra_from(configs.BOX20, 7, "I")
+    f = G.project(prec, G.FunctionTree(mra), orbs[1])
+    H = G.HelmholtzOperator(mra, mus[1], prec)
+    o64, o32 = G.FunctionTree(mra), G.FunctionTree(mra)
+    s64 = G.apply(prec, o64, H, f, flags=G.MWG_APPLY_FP64)
+    s32 = G.apply(prec, o32, H, f, flags=G.MWG_APPLY_MIXED)
+    assert s32["slices"] > 0 and s32["nodes_out"] == s64["nodes_out"]
+    assert _l2_diff(o32, o64, 40.0) < prec * math.sqrt(o64.getSquareNorm())
+
+
 def test_mixed_poisson_energy(G, oracle):
     prec = 1e-6
     g, parts = configs.density3(0)
