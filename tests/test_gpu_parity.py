@@ -128,6 +128,24 @@ def test_apply_k9(G, oracle):
     _same_tree(out, oout, 1e-12)
 
 
+@pytest.mark.parametrize("max_iter,max_depth", [(2, 16), (-1, 3)])
+def test_refinement_limits_match_oracle(G, oracle, max_iter, max_depth):
+    """mrcpp::apply's maxIter and the MRA's maximum depth cap the output tree the same way
+    on both sides (same grid, same tuples, coefficients to 1e-12)."""
+    prec = 1e-5
+    g, _ = configs.density3(1)
+    mra = G.mra_from(configs.BOX16, 5, max_depth=max_depth)
+    omra = oracle.MRA(**configs.BOX16, k=5, max_depth=max_depth)
+    of = oracle.Tree(omra).project_gauss(prec, g)
+    f = G.FunctionTree(mra).load(*of.export())
+    out, oout = G.FunctionTree(mra), oracle.Tree(omra)
+    st = G.apply(prec, out, G.PoissonOperator(mra, prec), f, max_iter=max_iter)
+    ost = oracle.apply(prec, oout, oracle.Oper(omra, 0.0, prec), of, max_iter=max_iter)
+    assert st["tuples"] == ost["tuples"] and st["nodes_out"] == ost["nodes_out"]
+    assert out.export()[0].max() <= (max_iter if max_iter >= 0 else max_depth)
+    _same_tree(out, oout, 1e-12)
+
+
 def test_batch_equals_single(G, oracle):
     prec = 1e-4
     mra = G.mra_from(configs.BOX16, 5)
