@@ -146,6 +146,17 @@ def test_refinement_limits_match_oracle(G, oracle, max_iter, max_depth):
     _same_tree(out, oout, 1e-12)
 
 
+def test_non_cubic_root_grid_matches_oracle(G, oracle):
+    """A 3 x 2 x 2 root grid (ragged world box, mrenv.cpp's corner/boxes): same grid, same
+    tuples and coefficients to 1e-12 as the oracle, also across the root boundaries."""
+    prec = 1e-4
+    g, _ = configs.density3(1)
+    box = dict(n0=-4, corner=(-2, -1, -1), boxes=(3, 2, 2), sfac=1.0)
+    out, oout, st, ost = _apply_pair(G, oracle, box, 5, prec, g, 0.0)
+    assert st["tuples"] == ost["tuples"] and st["nodes_out"] == ost["nodes_out"]
+    _same_tree(out, oout, 1e-12)
+
+
 def test_batch_equals_single(G, oracle):
     prec = 1e-4
     mra = G.mra_from(configs.BOX16, 5)
