@@ -128,6 +128,17 @@ def test_apply_k9(G, oracle):
     _same_tree(out, oout, 1e-12)
 
 
+@pytest.mark.parametrize("basis", ["L", "I"])
+def test_helmholtz_apply_k9_matches_oracle(G, oracle, basis):
+    """k=9 (the FP64 contraction with two operator rows live) on a bound-state Helmholtz
+    kernel, both bases: same tuples, grid and coefficients as the oracle."""
+    prec = 1e-4
+    orbs, nuc, mus = configs.orbitals(0, n_orb=2)
+    out, oout, st, ost = _apply_pair(G, oracle, configs.BOX20, 9, prec, orbs[1], mus[1], basis=basis)
+    assert st["tuples"] == ost["tuples"] and st["nodes_out"] == ost["nodes_out"]
+    _same_tree(out, oout, 1e-12)
+
+
 @pytest.mark.parametrize("max_iter,max_depth", [(2, 16), (-1, 3)])
 def test_refinement_limits_match_oracle(G, oracle, max_iter, max_depth):
     """mrcpp::apply's maxIter and the MRA's maximum depth cap the output tree the same way
