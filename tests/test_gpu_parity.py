@@ -128,6 +128,17 @@ def test_apply_k9(G, oracle):
     _same_tree(out, oout, 1e-12)
 
 
+@pytest.mark.parametrize("k", [3, 4, 6, 8])
+def test_apply_other_orders_match_oracle(G, oracle, k):
+    """The remaining orders of the templated kernels (K = 4, 5, 7, 9): same tuples, grid
+    and coefficients as the oracle (Poisson, Legendre)."""
+    prec = 1e-4
+    g, _ = configs.density3(2)
+    out, oout, st, ost = _apply_pair(G, oracle, configs.BOX16, k, prec, g, 0.0)
+    assert st["tuples"] == ost["tuples"] and st["nodes_out"] == ost["nodes_out"]
+    _same_tree(out, oout, 1e-12)
+
+
 @pytest.mark.parametrize("basis", ["L", "I"])
 def test_helmholtz_apply_k9_matches_oracle(G, oracle, basis):
     """k=9 (the FP64 contraction with two operator rows live) on a bound-state Helmholtz
