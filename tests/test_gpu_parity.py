@@ -301,7 +301,7 @@ def test_mixed_poisson_energy(G, oracle):
     assert abs(e32 - e64) < prec * abs(e64)
 
 
-@pytest.mark.parametrize("k", [5, 7, 9])
+@pytest.mark.parametrize("k", [3, 4, 5, 6, 7, 8, 9])
 def test_multiply_matches_oracle(G, oracle, k):
     """mrcpp::multiply on the device vs the oracle on the same input trees."""
     prec = 1e-5
