@@ -43,7 +43,8 @@ def test_library_exports_every_symbol():
         assert hasattr(L, name)
 
 
-@pytest.mark.parametrize("k,basis", [(5, "L"), (7, "L"), (9, "L"), (5, "I"), (9, "I")])
+@pytest.mark.parametrize("k,basis", [(3, "L"), (4, "L"), (5, "L"), (6, "L"), (7, "L"), (8, "L"), (9, "L"),
+                                     (5, "I"), (7, "I"), (9, "I")])
 def test_projection_matches_oracle(G, oracle, k, basis):
     g, _ = configs.density3(1)
     prec = 1e-5
