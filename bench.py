@@ -42,11 +42,14 @@ def build_inputs(G, configs, k, prec, n_orb, seed):
     return mra, orbs, mus, trees, ops, t_proj, t_oper
 
 
-def _l2diff(a, b, n=20000, seed=3):
-    import numpy as np
-    pts = np.random.default_rng(seed).uniform(-20.0, 20.0, (n, 3))
-    d = a.evalf(pts) - b.evalf(pts)
-    return math.sqrt((d ** 2).mean() * 40.0 ** 3)
+def _l2diff(G, a, b):
+    """||a - b|| exactly: the multiwavelet coefficients of a - b on the union grid (mwg_add)."""
+    return math.sqrt(max(G.add(G.FunctionTree(a.mra), 1.0, a, -1.0, b).getSquareNorm(), 0.0))
+
+
+def max_rel_err(G, outs, refs):
+    """Largest ||out - ref|| / ||ref|| over the outputs of a batch (exact coefficient L2)."""
+    return max(_l2diff(G, a, b) / math.sqrt(b.getSquareNorm()) for a, b in zip(outs, refs))
 
 
 def prec_tag(p):
@@ -82,6 +85,7 @@ def run_variant(G, mra, trees, ops, prec, flags, steps, warmup):
     flops = 0.0
     t_con = 0.0
     tuples = 0
+    tuples64 = 0
     slices = 0
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -90,12 +94,14 @@ def run_variant(G, mra, trees, ops, prec, flags, steps, warmup):
         nodes += sum(s["nodes_out"] for s in st)
         flops += sum(s["flops"] for s in st)
         tuples += sum(s["tuples"] for s in st)
+        tuples64 += sum(s["tuples_fp64"] for s in st)
         slices += sum(s["slices"] for s in st)
         t_con += st[0]["t_contract"]
     mra.ctx.sync()
     dt = time.perf_counter() - t0
     return dict(ms_per_step=1e3 * dt / steps, nodes_per_step=nodes / steps, nodes_per_s=nodes / dt,
                 flops_per_step=flops / steps, tuples_per_step=tuples / steps, slices_per_step=slices / steps,
+                fp64_tuples_per_step=tuples64 / steps,
                 t_contract_per_step=t_con / steps, iters=st[0]["iters"],
                 predicted=st[0]["predicted_splits"], wasted=st[0]["predicted_wasted"], outs=outs)
 
@@ -170,49 +176,53 @@ def copy_probe(G, tree):
     log(f"[bench] copy probe: k_scale over {n} nodes, {n * 8 * K ** 3 * 8} bytes read and written")
 
 
-def exchange_main(args, G, configs):
+def exchange_run(args, G, configs, steps, warmup):
     """16 orbitals on 8 nuclei (6 bohr cube), k=7, prec 1e-6: all 136 pair products
-    phi_i phi_j (device multiply) Poisson-applied as one batch; per-pair error vs FP64."""
-    mra = G.mra_from(configs.BOX20, args.k, "L", max_depth=16)
+    phi_i phi_j (device multiply) Poisson-applied as one batch, plain FP64 and mixed;
+    per-pair error of the mixed batch against the FP64 batch (exact coefficient L2)."""
+    k, prec = 7, args.prec
+    mra = G.mra_from(configs.BOX20, k, "L", max_depth=16)
     orbs, nuc, mus = configs.orbitals(args.seed, n_orb=16, n_nuc=8, nuc_box=6.0)
-    trees = [G.project(args.prec, G.FunctionTree(mra), o) for o in orbs]
+    trees = [G.project(prec, G.FunctionTree(mra), o) for o in orbs]
     pairs = [(i, j) for i in range(16) for j in range(i, 16)]
     t0 = time.perf_counter()
-    rho1 = [G.multiply(args.prec, G.FunctionTree(mra), 1.0, trees[i], trees[j]) for i, j in pairs]
-    t_mul1 = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    rho = G.multiply_batch(args.prec, [G.FunctionTree(mra) for _ in pairs], 1.0, [trees[i] for i, _ in pairs],
+    rho = G.multiply_batch(prec, [G.FunctionTree(mra) for _ in pairs], 1.0, [trees[i] for i, _ in pairs],
                            [trees[j] for _, j in pairs])
     t_mul = time.perf_counter() - t0
-    if any(a.getSquareNorm() != b.getSquareNorm() or a.getNNodes() != b.getNNodes() for a, b in zip(rho, rho1)):
-        raise RuntimeError("batched pair products differ from the single products")
-    del rho1
-    P = G.PoissonOperator(mra, args.prec)
+    P = G.PoissonOperator(mra, prec)
     ops = [P] * len(rho)
-    log(f"[bench] exchange: {len(rho)} pair densities ({sum(r.getNNodes() for r in rho)} nodes) in {t_mul:.3f} s "
-        f"batched ({t_mul1:.3f} s one by one)")
-    res64 = run_variant(G, mra, rho, ops, args.prec, G.MWG_APPLY_FP64, args.steps, args.warmup)
-    log(f"[bench] fp64: {res64['ms_per_step']:.1f} ms/step, {res64['nodes_per_s']:.0f} nodes/s")
-    res32 = run_variant(G, mra, rho, ops, args.prec, G.MWG_APPLY_MIXED, args.steps, args.warmup)
-    errs = [_l2diff(a, b) / math.sqrt(b.getSquareNorm()) for a, b in zip(res32["outs"], res64["outs"])]
-    log(f"[bench] mixed: {res32['ms_per_step']:.1f} ms/step, {res32['nodes_per_s']:.0f} nodes/s, "
+    log(f"[bench] exchange: {len(rho)} pair densities ({sum(r.getNNodes() for r in rho)} nodes) in {t_mul:.3f} s")
+    res64 = run_variant(G, mra, rho, ops, prec, G.MWG_APPLY_FP64, steps, warmup)
+    res32 = run_variant(G, mra, rho, ops, prec, G.MWG_APPLY_MIXED, steps, warmup)
+    errs = [_l2diff(G, a, b) / math.sqrt(b.getSquareNorm()) for a, b in zip(res32["outs"], res64["outs"])]
+    log(f"[bench] exchange fp64 {res64['ms_per_step']:.1f} ms/step, mixed {res32['ms_per_step']:.1f} ms/step, "
         f"max per-pair rel L2 err {max(errs):.2e}")
-    best, dtype = (res32, "f32") if res32["nodes_per_s"] > res64["nodes_per_s"] and max(errs) < args.prec else (res64, "f64")
-    tc = best["t_contract_per_step"]
-    achieved = res64["tuples_per_step"] * 6.0 * (args.k + 1) ** 4 / tc if tc > 0 else 0.0
+    tc = res32["t_contract_per_step"]
+    alg = res64["tuples_per_step"] * 6.0 * (k + 1) ** 4
+    return dict(workload=f"exchange pair Poissons, 16 orbitals (136 pairs), k={k}, prec={prec}",
+                fp64_nodes_per_s=res64["nodes_per_s"], mixed_nodes_per_s=res32["nodes_per_s"],
+                speedup_vs_fp64=res32["nodes_per_s"] / res64["nodes_per_s"],
+                fp64_ms_per_step=res64["ms_per_step"], mixed_ms_per_step=res32["ms_per_step"],
+                per_pair_rel_l2_err_max=max(errs), per_pair_rel_l2_err_mean=sum(errs) / len(errs),
+                output_nodes_per_step=res64["nodes_per_step"], plain_tuples_per_step=res64["tuples_per_step"],
+                mixed_contraction_ms_per_step=1e3 * tc,
+                mixed_contraction_tflops=alg / tc / 1e12 if tc > 0 else None, multiply_s=t_mul,
+                steps=steps, warmup=warmup)
+
+
+def exchange_main(args, G, configs):
+    r = exchange_run(args, G, configs, args.steps, args.warmup)
+    best = "mixed" if r["mixed_nodes_per_s"] > r["fp64_nodes_per_s"] and r["per_pair_rel_l2_err_max"] < args.prec \
+        else "fp64"
     line = {
         "metric": "output nodes/s for Poisson/Helmholtz apply at prec 1e-6 vs plain FP64 GPU",
-        "value": best["nodes_per_s"], "unit": "output nodes/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": best["ms_per_step"], "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": dtype, "data": "synthetic (seeded Gaussian orbitals)",
-        "config": {"workload": f"exchange pair Poissons, 16 orbitals (136 pairs), k={args.k}, prec={args.prec}",
-                   "fp64_nodes_per_s": res64["nodes_per_s"], "mixed_nodes_per_s": res32["nodes_per_s"],
-                   "speedup_vs_fp64": best["nodes_per_s"] / res64["nodes_per_s"],
-                   "per_pair_rel_l2_err_max": max(errs), "per_pair_rel_l2_err_mean": sum(errs) / len(errs),
-                   "output_nodes_per_step": best["nodes_per_step"], "multiply_s": t_mul,
-                   "multiply_one_by_one_s": t_mul1},
-        "roofline": dict(bound="mfma", achieved=achieved / 1e12, peak=PEAK[dtype] / 1e12, unit="TFLOP/s",
-                         frac=achieved / PEAK[dtype], traffic=None, contraction_ms_per_step=1e3 * tc),
+        "value": r[f"{best}_nodes_per_s"], "unit": "output nodes/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": r[f"{best}_ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32" if best == "mixed" else "f64",
+        "data": "synthetic (seeded Gaussian orbitals)", "config": r,
+        "roofline": dict(bound="mfma", achieved=r["mixed_contraction_tflops"], peak=PEAK["f32"] / 1e12,
+                         unit="TFLOP/s", frac=(r["mixed_contraction_tflops"] or 0) * 1e12 / PEAK["f32"],
+                         traffic=None),
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
@@ -250,7 +260,7 @@ def predictor_main(args, G, configs):
                 f"discarded {cand[thr]['wasted']}")
         pthr = max(cand, key=lambda t: cand[t]["nodes_per_s"])
         res1 = cand[pthr]
-        errs = [_l2diff(a, b) / math.sqrt(b.getSquareNorm()) for a, b in zip(res1["outs"], res0["outs"])]
+        errs = [_l2diff(G, a, b) / math.sqrt(b.getSquareNorm()) for a, b in zip(res1["outs"], res0["outs"])]
         report[name] = dict(plain_iters=res0["iters"], predict_iters=res1["iters"],
                             plain_nodes_per_s=res0["nodes_per_s"], predict_nodes_per_s=res1["nodes_per_s"],
                             plain_ms=res0["ms_per_step"], predict_ms=res1["ms_per_step"],
@@ -278,86 +288,101 @@ def predictor_main(args, G, configs):
     print(json.dumps(line), flush=True)
 
 
-def scf_main(args, G, configs):
+def scf_run(args, G, configs, steps, warmup, predict=True, cpu=True):
     """SCF operator step: 32 orbitals on 16 nuclei (10 bohr cube), k=9, prec 1e-6; one step =
     the 32 Helmholtz applies (mu_i = sqrt(-2 eps_i), eps_i ~ U[-2, -0.2]) and the 528 pair
-    Poissons phi_i phi_j (i <= j) as one batch.  The FP32 MFMA path covers k=7 only, so the
-    step runs in FP64, plain and with the learned predictor; the CPU oracle is timed on a
-    sample of the same applies."""
-    k = 9
+    Poissons phi_i phi_j (i <= j) as one batch: plain FP64, mixed (FP32 MFMA at k=9) and mixed
+    with the learned predictor; every output's error against the FP64 batch (exact L2); the
+    CPU oracle timed on two of the step's Helmholtz applies."""
+    k, prec = 9, args.prec
     mra = G.mra_from(configs.BOX20, k, "L", max_depth=16)
     orbs, nuc, _ = configs.orbitals(args.seed, n_orb=32, n_nuc=16, nuc_box=10.0)
     rng = configs.SplitMix64(args.seed + 1000)
     eps = [-rng.uniform(0.2, 2.0) for _ in range(32)]
     mus = [math.sqrt(-2.0 * e) for e in eps]
     t0 = time.perf_counter()
-    trees = [G.project(args.prec, G.FunctionTree(mra), o) for o in orbs]
+    trees = [G.project(prec, G.FunctionTree(mra), o) for o in orbs]
     pairs = [(i, j) for i in range(32) for j in range(i, 32)]
-    rho = G.multiply_batch(args.prec, [G.FunctionTree(mra) for _ in pairs], 1.0, [trees[i] for i, _ in pairs],
+    rho = G.multiply_batch(prec, [G.FunctionTree(mra) for _ in pairs], 1.0, [trees[i] for i, _ in pairs],
                            [trees[j] for _, j in pairs])
     t_in = time.perf_counter() - t0
     t0 = time.perf_counter()
-    P = G.PoissonOperator(mra, args.prec)
-    Hs = [G.HelmholtzOperator(mra, mu, args.prec) for mu in mus]
+    P = G.PoissonOperator(mra, prec)
+    Hs = [G.HelmholtzOperator(mra, mu, prec) for mu in mus]
     t_op = time.perf_counter() - t0
     inps = trees + rho
     ops = Hs + [P] * len(rho)
     log(f"[bench] scf step: {len(inps)} applies, input nodes {sum(t.getNNodes() for t in inps)}, "
         f"inputs {t_in:.2f} s, operators {t_op:.2f} s")
-    res64 = run_variant(G, mra, inps, ops, args.prec, G.MWG_APPLY_FP64, args.steps, args.warmup)
+    res64 = run_variant(G, mra, inps, ops, prec, G.MWG_APPLY_FP64, steps, warmup)
     K = k + 1
-    scf_tflops = res64["tuples_per_step"] * 6.0 * K ** 4 / res64["t_contract_per_step"] / 1e12
-    log(f"[bench] fp64: {res64['ms_per_step']:.0f} ms/step, {res64['nodes_per_s']:.0f} nodes/s, "
+    alg = res64["tuples_per_step"] * 6.0 * K ** 4
+    log(f"[bench] scf fp64: {res64['ms_per_step']:.0f} ms/step, {res64['nodes_per_s']:.0f} nodes/s, "
         f"iters {res64['iters']}, {res64['tuples_per_step']:.0f} tuples, contraction "
-        f"{1e3 * res64['t_contract_per_step']:.0f} ms/step = {scf_tflops:.1f} TFLOP/s FP64")
-    best = res64
-    resP = None
-    # keep the Helmholtz outputs and a few pair potentials for the error check, free the rest
-    keep = 32 + 16
-    del res64["outs"][keep:]
-    if not args.no_predict:
+        f"{1e3 * res64['t_contract_per_step']:.0f} ms/step")
+    res32 = run_variant(G, mra, inps, ops, prec, G.MWG_APPLY_MIXED, steps, warmup)
+    err32 = max_rel_err(G, res32["outs"], res64["outs"])
+    res32["outs"] = None
+    log(f"[bench] scf mixed: {res32['ms_per_step']:.0f} ms/step, {res32['nodes_per_s']:.0f} nodes/s, "
+        f"contraction {1e3 * res32['t_contract_per_step']:.0f} ms/step, max rel L2 err vs fp64 {err32:.2e}")
+    resP, errP = None, None
+    if predict:
         pred, ntrain, t_train = train_predictor(G, configs, G.mra_from(configs.BOX20, 7, "L", max_depth=16),
                                                 argparse.Namespace(**dict(vars(args), prec=1e-6, orbitals=5)))
         pred.install(args.pred_threshold if args.pred_threshold > 0 else 0.85)
-        resP = run_variant(G, mra, inps, ops, args.prec, G.MWG_APPLY_FP64 | G.MWG_APPLY_PREDICT, args.steps,
-                           args.warmup)
-        errP = max(_l2diff(a, b) / math.sqrt(b.getSquareNorm()) for a, b in zip(resP["outs"][:keep], res64["outs"]))
-        resP["rel_err_vs_fp64"] = errP
-        log(f"[bench] fp64+predictor: {resP['ms_per_step']:.0f} ms/step, {resP['nodes_per_s']:.0f} nodes/s, "
+        resP = run_variant(G, mra, inps, ops, prec, G.MWG_APPLY_MIXED | G.MWG_APPLY_PREDICT, steps, warmup)
+        errP = max_rel_err(G, resP["outs"], res64["outs"])
+        resP["outs"] = None
+        log(f"[bench] scf mixed+predictor: {resP['ms_per_step']:.0f} ms/step, {resP['nodes_per_s']:.0f} nodes/s, "
             f"iters {resP['iters']}, speculative {resP['predicted']}, discarded {resP['wasted']}, "
             f"max rel L2 err vs fp64 {errP:.2e}")
-        if resP["nodes_per_s"] > best["nodes_per_s"] and errP < args.prec:
-            best = resP
-    cpu = None
-    if not args.no_cpu:
+    cpu_r = None
+    if cpu:
         from oracle import mwref as M
         omra = M.MRA(**configs.BOX20, k=k, basis="L", max_depth=16)
         nodes, t_apply = 0, 0.0
         for i in range(2):
-            f = M.Tree(omra).project_gauss(args.prec, orbs[i])
-            st = M.apply(args.prec, M.Tree(omra), M.helmholtz(omra, mus[i], args.prec), f)
+            f = M.Tree(omra).project_gauss(prec, orbs[i])
+            st = M.apply(prec, M.Tree(omra), M.helmholtz(omra, mus[i], prec), f)
             nodes += st["nodes_out"]
             t_apply += st["t_total"]
-        cpu = dict(value=nodes / t_apply, unit="output nodes/s", cores=M.get_threads(), kind="port",
-                   sample=f"FP64 OpenMP oracle, 2 of the 32 Helmholtz applies of the step, {t_apply:.1f} s")
+        cpu_r = dict(value=nodes / t_apply, unit="output nodes/s", cores=M.get_threads(), kind="port",
+                     sample=f"FP64 OpenMP oracle, 2 of the 32 Helmholtz applies of the step, {t_apply:.1f} s")
+    res64["outs"] = None
+    tc64, tc32 = res64["t_contract_per_step"], res32["t_contract_per_step"]
+    return dict(workload=f"SCF operator step: 32 Helmholtz + 528 pair Poissons, k={k}, prec={prec}",
+                fp64_nodes_per_s=res64["nodes_per_s"], fp64_ms_per_step=res64["ms_per_step"],
+                mixed_nodes_per_s=res32["nodes_per_s"], mixed_ms_per_step=res32["ms_per_step"],
+                mixed_rel_l2_err_vs_fp64=err32, speedup_mixed_vs_fp64=res32["nodes_per_s"] / res64["nodes_per_s"],
+                predict_nodes_per_s=resP["nodes_per_s"] if resP else None,
+                predict_ms_per_step=resP["ms_per_step"] if resP else None,
+                predict_iters=resP["iters"] if resP else None, fp64_iters=res64["iters"],
+                predict_rel_l2_err_vs_fp64=errP,
+                output_nodes_per_step=res64["nodes_per_step"], inputs_s=t_in, operators_s=t_op,
+                plain_tuples_per_step=res64["tuples_per_step"],
+                fp64_contraction_ms_per_step=1e3 * tc64, mixed_contraction_ms_per_step=1e3 * tc32,
+                fp64_contraction_tflops=alg / tc64 / 1e12 if tc64 > 0 else None,
+                mixed_contraction_tflops=alg / tc32 / 1e12 if tc32 > 0 else None,
+                cpu_baseline=cpu_r, steps=steps, warmup=warmup)
+
+
+def scf_main(args, G, configs):
+    r = scf_run(args, G, configs, args.steps, args.warmup, predict=not args.no_predict, cpu=not args.no_cpu)
+    cands = [("fp64", r["fp64_nodes_per_s"], 0.0), ("mixed", r["mixed_nodes_per_s"], r["mixed_rel_l2_err_vs_fp64"])]
+    if r["predict_nodes_per_s"]:
+        cands.append(("predict", r["predict_nodes_per_s"], r["predict_rel_l2_err_vs_fp64"]))
+    best = max((c for c in cands if c[2] < args.prec), key=lambda c: c[1])
+    tfl = r["fp64_contraction_tflops"] if best[0] == "fp64" else r["mixed_contraction_tflops"]
+    peak = PEAK["f64"] if best[0] == "fp64" else PEAK["f32"]
     line = {
         "metric": "output nodes/s for Poisson/Helmholtz apply at prec 1e-6 vs plain FP64 GPU",
-        "value": best["nodes_per_s"], "unit": "output nodes/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": best["ms_per_step"], "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded Gaussian orbitals)",
-        "config": {"workload": f"SCF operator step: 32 Helmholtz + 528 pair Poissons, k={k}, prec={args.prec}",
-                   "fp64_nodes_per_s": res64["nodes_per_s"], "fp64_ms_per_step": res64["ms_per_step"],
-                   "fp64_iters": res64["iters"],
-                   "predict_nodes_per_s": resP["nodes_per_s"] if resP else None,
-                   "predict_iters": resP["iters"] if resP else None,
-                   "predict_rel_l2_err_vs_fp64": resP["rel_err_vs_fp64"] if resP else None,
-                   "output_nodes_per_step": best["nodes_per_step"], "inputs_s": t_in, "operators_s": t_op,
-                   "fp64_tuples_per_step": res64["tuples_per_step"],
-                   "fp64_contraction_ms_per_step": 1e3 * res64["t_contract_per_step"]},
-        # FP64 contraction (plain variant): 6 (k+1)^4 flops per tuple over the contraction time
-        "roofline": {"bound": "mfma", "achieved": scf_tflops, "peak": 78.6, "unit": "TFLOP/s",
-                     "frac": scf_tflops / 78.6, "traffic": None, "kernel": f"k_contract_fp64<{K}> + k_reduce_partials"},
-        "cpu_baseline": cpu,
+        "value": best[1], "unit": "output nodes/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": r[f"{best[0]}_ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if best[0] == "fp64" else "f32",
+        "data": "synthetic (seeded Gaussian orbitals)", "config": {k: v for k, v in r.items() if k != "cpu_baseline"},
+        "roofline": {"bound": "mfma", "achieved": tfl, "peak": peak / 1e12, "unit": "TFLOP/s",
+                     "frac": (tfl or 0) * 1e12 / peak, "traffic": None},
+        "cpu_baseline": r["cpu_baseline"],
     }
     print(json.dumps(line), flush=True)
 
@@ -379,6 +404,10 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default contraction task sizes")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after timing, rescale one output tree in place (HBM counter calibration)")
+    ap.add_argument("--no-exchange", action="store_true",
+                    help="default workload only: skip the exchange (136 pair Poissons) results")
+    ap.add_argument("--no-scf", action="store_true",
+                    help="default workload only: skip the SCF operator step (k=9) results")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -417,10 +446,11 @@ def main():
         f"contraction {1e3 * res64['t_contract_per_step']:.2f} ms/step")
     best, dtype = res64, "f64"
     res32 = None
-    if args.k == 7:
+    mixed_ok = args.k in (7, 9)
+    if mixed_ok:
         tune("mixed", G.MWG_APPLY_MIXED)
         res32 = run_variant(G, mra, trees, ops, args.prec, G.MWG_APPLY_MIXED, args.steps, args.warmup)
-        err = max(_l2diff(a, b) / math.sqrt(b.getSquareNorm()) for a, b in zip(res32["outs"], res64["outs"]))
+        err = max_rel_err(G, res32["outs"], res64["outs"])
         res32["rel_err_vs_fp64"] = err
         log(f"[bench] mixed: {res32['ms_per_step']:.2f} ms/step, {res32['nodes_per_s']:.1f} nodes/s, "
             f"contraction {1e3 * res32['t_contract_per_step']:.2f} ms/step, rel L2 err vs fp64 {err:.2e}")
@@ -429,7 +459,7 @@ def main():
     resP = None
     if not args.no_predict:
         pred, ntrain, t_train = train_predictor(G, configs, mra, args)
-        flags = (G.MWG_APPLY_MIXED if args.k == 7 else G.MWG_APPLY_FP64) | G.MWG_APPLY_PREDICT
+        flags = (G.MWG_APPLY_MIXED if mixed_ok else G.MWG_APPLY_FP64) | G.MWG_APPLY_PREDICT
         # decision threshold of the predictor: the given one, or the fastest of a few
         sweep = {}
         for thr in ([args.pred_threshold] if args.pred_threshold > 0 else [0.5, 0.7, 0.85, 0.95]):
@@ -440,7 +470,7 @@ def main():
                 f"speculative {r['predicted']}, discarded {r['wasted']}")
         pthr = max(sweep, key=lambda t: sweep[t]["nodes_per_s"])
         resP = sweep[pthr]
-        errP = max(_l2diff(a, b) / math.sqrt(b.getSquareNorm()) for a, b in zip(resP["outs"], res64["outs"]))
+        errP = max_rel_err(G, resP["outs"], res64["outs"])
         resP["rel_err_vs_fp64"] = errP
         resP["train_samples"] = ntrain
         resP["train_s"] = t_train
@@ -452,21 +482,32 @@ def main():
             f"iters {resP['iters']} (fp64 {res64['iters']}), threshold {pthr}, rel L2 err vs fp64 {errP:.2e}, "
             f"trained on {ntrain} trace samples in {t_train:.1f} s")
         if resP["nodes_per_s"] > best["nodes_per_s"] and errP < args.prec:
-            best, dtype = resP, ("f32" if args.k == 7 else "f64")
+            best, dtype = resP, ("f32" if mixed_ok else "f64")
 
     # algorithmic flops: the plain FP64 method's surviving tuples x 6 (k+1)^4 (the same work for
     # every variant); issued flops count what the variant's kernels actually execute
-    alg = res64["tuples_per_step"] * 6.0 * (args.k + 1) ** 4
+    K = args.k + 1
+    F = res64["tuples_per_step"]
+    alg = F * 6.0 * K ** 4
     tc = best["t_contract_per_step"]
     achieved = alg / tc if tc > 0 else 0.0
+    kern = ("k_contract_mfma8" if args.k == 7 else f"k_contract_mfma{K}") if dtype == "f32" else "k_contract_fp64"
+    per_launch = measured_traffic(kern, f"helmholtz{args.orbitals}_k{args.k}_prec{prec_tag(args.prec)}")
+    # algorithmic bytes of a step: every input and output node block read or written once (FP64)
+    in_nodes = sum(t.getNNodes() for t in trees)
+    alg_bytes = 8.0 * K ** 3 * 8 * (in_nodes + best["nodes_per_step"])
     roofline = dict(bound="mfma", achieved=achieved / 1e12, peak=PEAK[dtype] / 1e12, unit="TFLOP/s",
                     frac=achieved / PEAK[dtype],
-                    traffic=measured_traffic("k_contract_mfma8" if dtype == "f32" else "k_contract_fp64",
-                                             f"helmholtz{args.orbitals}_k{args.k}_prec{prec_tag(args.prec)}"),
-                    kernel="k_contract_mfma8 + k_contract_fp64 + k_reduce_partials" if dtype == "f32"
+                    traffic=per_launch,
+                    kernel=(kern + " + k_contract_fp64 + k_reduce_partials") if dtype == "f32"
                     else "k_contract_fp64 + k_reduce_partials",
+                    plain_tuples_per_step=F, alg_flops_per_tuple=6.0 * K ** 4,
                     issued_tflops=(best["flops_per_step"] / tc / 1e12) if tc > 0 else None,
-                    contraction_ms_per_step=1e3 * tc)
+                    contraction_ms_per_step=1e3 * tc,
+                    launches_per_step=best["iters"],
+                    traffic_per_step=per_launch * best["iters"] if per_launch else None,
+                    alg_bytes_per_step=alg_bytes,
+                    traffic_over_alg_bytes=(per_launch * best["iters"] / alg_bytes) if per_launch else None)
 
     if args.copy_probe:
         copy_probe(G, res64["outs"][0])
@@ -476,6 +517,11 @@ def main():
         log("[bench] CPU oracle baseline ...")
         cpu = cpu_baseline(orbs, mus, args.k, args.prec)
         log(f"[bench] cpu: {cpu['value']:.1f} nodes/s on {cpu['cores']} threads")
+
+    sub_steps, sub_warm = min(args.steps, 3), 1
+    exch = None if args.no_exchange else exchange_run(args, G, configs, sub_steps, sub_warm)
+    scf = None if args.no_scf else scf_run(args, G, configs, min(args.steps, 2), sub_warm,
+                                           predict=not args.no_predict, cpu=not args.no_cpu)
 
     line = {
         "metric": "output nodes/s for Poisson/Helmholtz apply at prec 1e-6 vs plain FP64 GPU",
@@ -494,11 +540,13 @@ def main():
             "workload": f"Helmholtz apply, {args.orbitals} orbitals, k={args.k}, prec={args.prec}, box [-20,20]^3",
             "orbitals": args.orbitals, "k": args.k, "prec": args.prec, "seed": args.seed,
             "output_nodes_per_step": best["nodes_per_step"], "tuples_per_step": best["tuples_per_step"],
+            "plain_tuples_per_step": F, "input_nodes": in_nodes,
             "fp64_nodes_per_s": res64["nodes_per_s"], "speedup_vs_fp64": best["nodes_per_s"] / res64["nodes_per_s"],
             "fp64_ms_per_step": res64["ms_per_step"],
             "mixed_nodes_per_s": res32["nodes_per_s"] if res32 else None,
             "mixed_rel_l2_err_vs_fp64": res32["rel_err_vs_fp64"] if res32 else None,
-            "mixed_fp64_tuples_per_step": res32["tuples_per_step"] if res32 else None,
+            "mixed_tuples_per_step": res32["tuples_per_step"] if res32 else None,
+            "mixed_fp64_routed_tuples_per_step": res32["fp64_tuples_per_step"] if res32 else None,
             "mixed_slices_per_step": res32["slices_per_step"] if res32 else None,
             "predict_nodes_per_s": resP["nodes_per_s"] if resP else None,
             "predict_rel_l2_err_vs_fp64": resP["rel_err_vs_fp64"] if resP else None,
@@ -511,6 +559,8 @@ def main():
             "operator_build_s": t_oper, "projection_s": t_proj,
             "contraction_ms_per_step": 1e3 * best["t_contract_per_step"],
             "tuning": tuning or None,
+            "exchange": exch,
+            "scf": scf,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -47,8 +47,11 @@ typedef struct mwg_predictor_s *mwg_predictor;
 
 /* apply flags */
 #define MWG_APPLY_FP64 0u     /* plain method: FP64 contractions, norm-test refinement */
-#define MWG_APPLY_MIXED 1u    /* FP32 MFMA slice contractions (FP32 sums over <= 16 slices, FP64 beyond) */
-#define MWG_APPLY_PREDICT 2u  /* learned refinement predictor (norm test stays the judge) */
+#define MWG_APPLY_MIXED 1u    /* FP32 MFMA slice contractions (k = 7: FP32 sums over <= 16 slices,
+                                 FP64 beyond; k = 9: FP64 sums); k = 7 and k = 9 only (MWG_ERR_ARG
+                                 at other orders) */
+#define MWG_APPLY_PREDICT 2u  /* learned refinement predictor (norm test stays the judge; the output
+                                 tree can differ from the plain one, see mwg_predictor_create) */
 
 /* basis types (src/mrenv.cpp:62-69: "I" Interpolating, "L" Legendre) */
 #define MWG_BASIS_LEGENDRE 0
@@ -56,16 +59,18 @@ typedef struct mwg_predictor_s *mwg_predictor;
 
 typedef struct mwg_apply_stats {
     int64_t nodes_out;       /* nodes of the output tree (output nodes of this apply) */
-    int64_t tuples;          /* surviving (g, f, ft, gt, term) tuples contracted */
-    int64_t tuples_fp64;     /* of which contracted in FP64 (mixed mode routes large bounds to FP64) */
-    int64_t gen_nodes;       /* generated (scaling-only) input nodes */
+    int64_t tuples;          /* surviving (g, f, ft, gt, term) tuples contracted (FP64 + FP32 slices) */
+    int64_t tuples_fp64;     /* of which contracted in FP64 (all in plain mode; mixed mode routes
+                                slices with large bounds to FP64 tuples) */
+    int64_t gen_nodes;       /* generated (scaling-only) input nodes of this apply */
     int64_t iters;           /* refinement iterations */
     int64_t slices;          /* FP32 MFMA slices (g, f, ft, term: all 8 gt at once), mixed mode */
     int64_t predicted_splits;/* output nodes computed ahead of their parent's norm test (predictor) */
     int64_t predicted_wasted;/* of which discarded because the norm test did not split the parent */
     double t_apply;          /* seconds: work-list build + contraction + refine + transforms */
     double t_contract;       /* seconds spent in contraction kernels (HIP events) */
-    double flops;            /* flops issued: FP64 tuples * 6 (k+1)^4 + slices * 2*2(k+1)*(k+1)*7(k+1)^2 */
+    double flops;            /* flops issued: FP64 tuples * 6 (k+1)^4 + MFMA flops of the slices
+                                (k = 7: 2*16*8*(64+128+256) per slice; k = 9 padded: 2*32*12*(112+208+400)) */
 } mwg_apply_stats;
 
 /* ---- context (mrenv / device) ------------------------------------------------ */
@@ -76,12 +81,12 @@ int mwg_ctx_destroy(mwg_ctx ctx);
 const char *mwg_last_error(mwg_ctx ctx);
 int mwg_sync(mwg_ctx ctx);
 /* contraction task granularity (no reference counterpart: MRCPP threads over output
- * nodes with OpenMP): tuples per FP64 wave task (1..4096) and FP32 MFMA slices per wave
- * (1..16; a workgroup task takes 4 x slice_chunk slices of one output node, and each
- * wave sums its slice_chunk results in FP32 before the FP64 sum over waves and tasks).
- * Results agree to FP64 rounding for any setting on the FP64 path and to FP32 rounding
- * of the FP32-routed slices on the mixed path; mrchem_amd.autotune picks the fastest on
- * a sample workload. */
+ * nodes with OpenMP): tuples per FP64 wave task (1..4096) and FP32 MFMA slices per
+ * workgroup task (0..4096, 0 = automatic; k = 7 uses at most 64, k = 9 sizes automatic
+ * tasks from each refinement iteration's slice total).  Results agree to FP64 rounding
+ * for any setting on the FP64 path and at k = 9, and to the FP32 rounding of the
+ * FP32-routed slices at k = 7; mrchem_amd.autotune picks the fastest on a sample
+ * workload. */
 int mwg_ctx_set_tuning(mwg_ctx ctx, int fp64_chunk, int slice_chunk);
 int mwg_ctx_get_tuning(mwg_ctx ctx, int *fp64_chunk, int *slice_chunk);
 
@@ -98,7 +103,15 @@ int mwg_tree_destroy(mwg_tree tree);
 int mwg_tree_nnodes(mwg_tree tree, int64_t *n);              /* FunctionTree::getNNodes */
 int mwg_tree_sqnorm(mwg_tree tree, double *out);             /* FunctionTree::getSquareNorm */
 int mwg_tree_maxdepth(mwg_tree tree, int *out);
+/* import refuses (MWG_ERR_ARG) input that breaks the packed layout: the roots first in
+ * x-fastest order, then complete groups of 8 siblings in child order, each after its
+ * parent; depths in [0, max_depth]; translations inside the box; no node twice */
 int mwg_tree_import(mwg_tree tree, int64_t n, const int *depth, const int64_t *lr, const double *coefs);
+/* the same check without a context or device (boxes: the MRA's root boxes); msg (may be
+ * null) receives the first violation.  No reference counterpart: MRCPP reads its own
+ * tree files only. */
+int mwg_tree_layout_check(const int *boxes, int max_depth, int64_t n, const int *depth, const int64_t *lr,
+                          char *msg, int msg_len);
 int mwg_tree_export(mwg_tree tree, int *depth, int64_t *lr, double *coefs);
 int mwg_tree_rescale(mwg_tree tree, double c);               /* FunctionTree::rescale (HelmholtzVector.cpp:81) */
 int mwg_tree_dot(mwg_tree a, mwg_tree b, double *out);       /* mrcpp::dot */

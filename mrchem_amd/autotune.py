@@ -1,17 +1,16 @@
 """Autotuner for the contraction task granularity of the batched apply.
 
 The device apply splits each output node's tuple list into FP64 wave tasks of
-``fp64_chunk`` tuples and its FP32 MFMA slice list into wave tasks of
-``slice_chunk`` slices; partial results of a node's tasks are summed in a fixed
-order afterwards.  Small tasks spread a node over more waves (better occupancy
+``fp64_chunk`` tuples and its FP32 MFMA slice list into tasks of ``slice_chunk``
+slices (0: sized from each refinement iteration's slice total); partial results of
+a node's tasks are summed in FP64 in a fixed order afterwards.  Small tasks spread a node over more waves (better occupancy
 for few, heavy nodes); large tasks cut partial-result traffic and reduction work.
 The best point depends on k, the precision and the tree shapes, so it is measured:
 ``autotune`` times the contraction kernels (HIP events, ``t_contract``) of the
 caller's own batch for each candidate and installs the fastest on the context.
 
 Candidates are applied to fresh output trees each time; the results agree to
-rounding for every setting (only the summation grouping changes: FP64 on the FP64
-path, and the FP32 per-wave runs of FP32-routed slices on the mixed path).
+rounding for every setting (only the FP64 summation grouping changes, on both paths).
 """
 from __future__ import annotations
 
@@ -21,7 +20,7 @@ import time
 from . import mrcpp as G
 
 FP64_CHUNKS = (16, 32, 64, 128, 256)
-SLICE_CHUNKS = (2, 4, 8, 16)
+SLICE_CHUNKS = (0, 16, 32, 64, 128, 256)
 
 
 def _measure(mra, trees, ops, prec, flags, reps):

@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libmwgreen.so")
-SOURCES = ["mwg_core.cpp", "mwg_kernels.hip", "mwg_predict.hip", "mwg_api.hip"]
+SOURCES = ["mwg_core.cpp", "mwg_kernels.hip", "mwg_mfma.hip", "mwg_predict.hip", "mwg_api.hip"]
 ARCH = os.environ.get("MWG_OFFLOAD_ARCH", "gfx950")
 
 

@@ -274,7 +274,7 @@ struct Ctx {
     struct PredH *pred = nullptr;
     float pred_thr = 0.5f;
     int chunk = 64;        // tuples per FP64 contraction task
-    int slice_chunk = 16;  // slices per wave of an MFMA task (at most MAX_SLICE_CHUNK)
+    int slice_chunk = 0;   // slices per MFMA task (at most MAX_SLICE_CHUNK; 0: from each iteration's total)
 };
 
 // Wait for the context's stream by polling: the refinement loop waits once per
@@ -800,7 +800,7 @@ struct JobH {
     double sNorm = 0.0, wNorm = 0.0, gsq = -1.0;
     std::vector<int> work;
     std::vector<int> wpos;  // work-list index of each node of this iteration (-1: none)
-    int64_t tuples = 0, slices = 0;
+    int64_t tuples = 0, tuples_fp64 = 0, slices = 0, gen = 0;
     size_t genOff = 0;
     uint32_t genMask = 0;
     std::vector<char> spec, alive, has_feat;  // per output node
@@ -974,6 +974,14 @@ static void for_each_job(size_t n, F f) {
         if (e) std::rethrow_exception(e);
 }
 
+// FP32 MFMA flops one slice issues: K = 8 three GEMMs of 16 x 8 over 64, 128 and 256
+// columns; K = 10 the padded tiles (2 x 16 rows, chunks of 4) over 112, 208 and 400 columns
+static double slice_flops(int K) {
+    if (K == 8) return 2.0 * 16 * 8 * (64 + 128 + 256);
+    if (K == 10) return 2.0 * 32 * 12 * (112 + 208 + 400);
+    return 0.0;
+}
+
 // device memory for the FP64 partial slots of one refinement iteration
 constexpr double SLOT_BUDGET = 8.0 * (1ull << 30);
 
@@ -986,7 +994,11 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
     Ctx &C = *M.ctx;
     hipStream_t s = C.stream;
     const int K = M.B.K, K3 = M.B.K3;
-    const bool mixed = (flags & MWG_APPLY_MIXED) && K == 8;
+    if (flags & ~(MWG_APPLY_MIXED | MWG_APPLY_PREDICT)) throw Error(MWG_ERR_ARG, "unknown apply flags");
+    const bool mixed = (flags & MWG_APPLY_MIXED) != 0;
+    if (mixed && K != 8 && K != 10)
+        throw Error(MWG_ERR_ARG, "MWG_APPLY_MIXED is built for orders k = 7 and k = 9 only (this MRA has k = " +
+                                     std::to_string(K - 1) + ")");
     auto t_start = std::chrono::steady_clock::now();
     float t_contract_ms = 0.0f;
     // inputs: device hash, norm maxima per depth
@@ -1013,9 +1025,10 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             run = std::max(run, leafmax[d]);
         }
         J.out->make_roots();
-        // output trees usually end near the input's size: reserve once instead of
-        // growing (each growth is an allocation, a copy and a free)
-        const size_t est = std::max<size_t>(64, (size_t)J.out->n() + J.in->n() + J.in->n() / 4);
+        // output trees end between a third of the input's size (pair Poissons) and about its
+        // size (Helmholtz): reserve half of it once (beyond, the arrays grow geometrically:
+        // an allocation and a copy on the stream each time)
+        const size_t est = std::max<size_t>(64, (size_t)J.out->n() + J.in->n() / 2);
         J.out->ensure_dev(est);
         J.out->reserve_host(est);
         J.work = J.out->roots;
@@ -1151,6 +1164,23 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         WorkItem *dwork = (WorkItem *)C.iterbuf.p;
         JobTab *djobs = (JobTab *)(C.iterbuf.p + o_jobs);
         BoxJob *dbjobs = (BoxJob *)(C.iterbuf.p + o_bj);
+        // the iteration's read-back block: schedule summary | per-table counts (FP64 tuples,
+        // slices, slice tuples, generated nodes) | work norms | child probabilities | child
+        // features.  The table counts start at zero (the generated-node counts are added
+        // from the marking onwards).
+        const bool want_feat = use_pred || C.tracing;
+        const size_t o_tab = 256;
+        const size_t o_wn = (o_tab + 4 * jt.size() * sizeof(long long) + 255) & ~(size_t)255;
+        const size_t o_pr = (o_wn + (size_t)nw * 8 * sizeof(double) + 255) & ~(size_t)255;
+        const size_t o_ft = (o_pr + (use_pred ? (size_t)nw * 8 * sizeof(float) : 0) + 255) & ~(size_t)255;
+        const size_t rb_dev = o_ft + (want_feat ? (size_t)nw * 8 * MLP_FEATURES * sizeof(float) : 0);
+        const size_t rb_host = C.tracing ? rb_dev : o_ft;  // the features only when tracing
+        C.rbd.alloc(rb_dev);
+        TaskSummary *dsum = (TaskSummary *)C.rbd.p;
+        long long *dtab = (long long *)(C.rbd.p + o_tab);
+        double *dwn = (double *)(C.rbd.p + o_wn);
+        float *dprob = (float *)(C.rbd.p + o_pr), *dfeat = (float *)(C.rbd.p + o_ft);
+        HC(hipMemsetAsync(dtab, 0, 4 * jt.size() * sizeof(long long), s));
         C.tabs.alloc((size_t)std::max(ntabs, 1));
         C.tbmax.alloc((size_t)std::max<size_t>(bjobs.size() * max_terms, 1));
         HC(launch_job_boxes(s, (int)bjobs.size(), max_terms, dbjobs, C.tabs.p, C.tbmax.p, djobs, C.tpre.p,
@@ -1166,7 +1196,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             }
         }
         GenState gs{C.genKeys.p, C.genJob.p, C.genCount.p, genCap, C.genOverflow.p, C.genPool.p, C.genNorm.p,
-                    genPoolCap};
+                    genPoolCap, dtab};
         HC(launch_gen_mark(s, nw, dwork, djobs, C.tabs.p, gs));
         HC(launch_gen_compute(s, genDone, genPoolCap, djobs, gs, M.W.p, K));
         PT.lap("upload+genmark");
@@ -1185,24 +1215,12 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         C.segs.alloc((size_t)nw * 8);
         C.segcnt.alloc((size_t)nw * nseg);
         C.gslice.alloc((size_t)nw);
-        const bool want_feat = use_pred || C.tracing;
-        const size_t o_tab = 256;
-        const size_t o_wn = (o_tab + 2 * jt.size() * sizeof(long long) + 255) & ~(size_t)255;
-        const size_t o_pr = (o_wn + (size_t)nw * 8 * sizeof(double) + 255) & ~(size_t)255;
-        const size_t o_ft = (o_pr + (use_pred ? (size_t)nw * 8 * sizeof(float) : 0) + 255) & ~(size_t)255;
-        const size_t rb_dev = o_ft + (want_feat ? (size_t)nw * 8 * MLP_FEATURES * sizeof(float) : 0);
-        const size_t rb_host = C.tracing ? rb_dev : o_ft;  // the features only when tracing
-        C.rbd.alloc(rb_dev);
-        TaskSummary *dsum = (TaskSummary *)C.rbd.p;
-        long long *dtab = (long long *)(C.rbd.p + o_tab);
-        double *dwn = (double *)(C.rbd.p + o_wn);
-        float *dprob = (float *)(C.rbd.p + o_pr), *dfeat = (float *)(C.rbd.p + o_ft);
         auto launch_count = [&]() {
             if (!mixed)
                 HC(launch_enum(s, 0, nw, nparts, dwork, djobs, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, nullptr,
                                nullptr, nullptr, nullptr));
             else
-                HC(launch_enum_mixed(s, 0, nw, nparts, dwork, djobs, C.tabs.p, gs, C.thread_off.p, C.gtotal.p,
+                HC(launch_enum_mixed(s, 0, K, nw, nparts, dwork, djobs, C.tabs.p, gs, C.thread_off.p, C.gtotal.p,
                                      nullptr, nullptr, nullptr, nullptr, nullptr, nullptr));
         };
         auto ensure_caps = [&](const TaskSummary *need) {
@@ -1229,21 +1247,26 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             }
         };
         std::vector<double> wn((size_t)nw * 8);
-        std::vector<long long> tabcnt(2 * jt.size());
+        std::vector<long long> tabcnt(4 * jt.size());
         std::vector<float> cprob(use_pred ? (size_t)nw * 8 : 0);
         std::vector<float> cfeat(C.tracing ? (size_t)nw * 8 * MLP_FEATURES : 0);
         TaskSummary hsum;
         int chunk_it = C.chunk;  // FP64 task size of this iteration (raised when partial slots would not fit)
+        // slices per MFMA task (a workgroup task).  k = 7: at most MFMA8_TASK_SLICES (the
+        // waves sum at most a quarter of them in FP32), 64 when automatic; k = 9: about 2048
+        // tasks per iteration when automatic, raised when node slots would not fit
+        int schunk_it = K == 8 ? std::min(C.slice_chunk > 0 ? C.slice_chunk : MFMA8_TASK_SLICES, MFMA8_TASK_SLICES)
+                               : (C.slice_chunk > 0 ? C.slice_chunk : -2048);
         auto launch_schedule = [&]() {
             ensure_caps(nullptr);
-            HC(launch_build_tasks(s, nw, nseg, nparts, chunk_it, 4 * C.slice_chunk, C.gtotal.p, C.caps, C.genCount.p,
+            HC(launch_build_tasks(s, nw, nseg, nparts, chunk_it, schunk_it, C.gtotal.p, C.caps, C.genCount.p,
                                   genPoolCap, C.gofs.p, C.tasks.p, C.stasks.p, C.reds.p, dsum, C.segs.p,
                                   C.segcnt.p, C.gslice.p, C.genOverflow.p, dwork, dtab, (int)jt.size()));
             if (!mixed)
                 HC(launch_enum(s, 1, nw, nparts, dwork, djobs, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, C.gofs.p,
                                C.tuples.p, dsum, C.segs.p));
             else
-                HC(launch_enum_mixed(s, 1, nw, nparts, dwork, djobs, C.tabs.p, gs, C.thread_off.p, C.gtotal.p,
+                HC(launch_enum_mixed(s, 1, K, nw, nparts, dwork, djobs, C.tabs.p, gs, C.thread_off.p, C.gtotal.p,
                                      C.gofs.p, C.tuples.p, C.slices.p, dsum, C.segs.p, C.gslice.p));
             const int grid64 = std::max(1, (C.caps.tasks + 3) / 4);
             const int gridred = std::min(4096, std::max(1, C.caps.reds));
@@ -1252,9 +1275,10 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             HC(launch_contract_fp64(s, K, 0, dsum, grid64, C.tasks.p, dwork, djobs, C.tabs.p, C.tuples.p,
                                     C.genPool.p, C.partials.p, C.ev0));
             if (mixed) {
-                const int gridmf = std::max(1, C.caps.stasks);
+                const int tpb = mfma_tasks_per_block(K);
+                const int gridmf = std::max(1, (C.caps.stasks + tpb - 1) / tpb);
                 HC(launch_contract_mfma(s, K, 0, dsum, gridmf, C.stasks.p, dwork, djobs, C.tabs.p,
-                                        C.slices.p, C.genPool.p, C.nodeslots.p));
+                                        C.slices.p, C.genPool.p, C.nodeslots.p, dtab));
             }
             // (mixed: the reduce items cover every output component and write the norms)
             HC(launch_reduce_partials(s, 0, dsum, gridred, C.reds.p, dwork, djobs, C.partials.p,
@@ -1307,6 +1331,12 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                 chunk_it *= 2;
                 need.nslots = 0;
             }
+            if (K != 8 && (double)need.nnslots * 8 * K3 * sizeof(double) > SLOT_BUDGET &&
+                hsum.schunk < MAX_SLICE_CHUNK) {
+                schunk_it = 2 * hsum.schunk;  // the same for the MFMA tasks' node slots
+                need.nnslots = 0;
+                need.nstasks = 0;
+            }
             ensure_caps(&need);
             launch_schedule();
         }
@@ -1316,8 +1346,11 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             fprintf(stderr, "[mwg timing] iteration %d: %d work nodes, %lld tuples, %lld slices, %d generated (pool %d)\n",
                     iter, nw, hsum.ntuples, hsum.nslices, genLastNew, genPoolCap);
         for (size_t q = 0; q < jt.size(); q++) {
-            jobs[jt_job[q]].tuples += tabcnt[2 * q];
-            jobs[jt_job[q]].slices += tabcnt[2 * q + 1];
+            JobH &J = jobs[jt_job[q]];
+            J.tuples += tabcnt[4 * q] + tabcnt[4 * q + 2];
+            J.tuples_fp64 += tabcnt[4 * q];
+            J.slices += tabcnt[4 * q + 1];
+            J.gen += tabcnt[4 * q + 3];
         }
         PT.lap("worklist+launch");
         float ms = 0.0f;
@@ -1497,15 +1530,15 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         mwg_apply_stats &st = stats[j];
         st.nodes_out = jobs[j].out->n();
         st.tuples = jobs[j].tuples;
-        st.tuples_fp64 = jobs[j].tuples;
+        st.tuples_fp64 = jobs[j].tuples_fp64;
         st.slices = jobs[j].slices;
-        st.gen_nodes = genDone;
+        st.gen_nodes = jobs[j].gen;
         st.iters = iter;
         st.predicted_splits = predicted;
         st.predicted_wasted = wasted;
         st.t_apply = t_total;
         st.t_contract = t_contract_ms * 1e-3;
-        st.flops = fl * (double)jobs[j].tuples + 2.0 * (2 * K) * K * 7.0 * K * K * (double)jobs[j].slices;
+        st.flops = fl * (double)jobs[j].tuples_fp64 + slice_flops(K) * (double)jobs[j].slices;
     }
 }
 
@@ -1974,23 +2007,34 @@ int mwg_tree_import(mwg_tree tree, int64_t n, const int *depth, const int64_t *l
     return guarded(C, [&] {
         TreeH &T = *tree;
         T.clear();
-        const int nr = T.mra->boxes[0] * T.mra->boxes[1] * T.mra->boxes[2];
+        const MraH &M = *T.mra;
+        const std::string bad = check_tree_layout(M.boxes, M.maxDepth, n, depth, lr);
+        if (!bad.empty()) throw Error(MWG_ERR_ARG, "tree import: " + bad);
         for (int64_t i = 0; i < n; i++) {
-            int par = -1;
-            if (depth[i] > 0) {
-                par = T.find(depth[i] - 1, (int)(lr[3 * i] >> 1), (int)(lr[3 * i + 1] >> 1), (int)(lr[3 * i + 2] >> 1));
-                if (par < 0) throw Error(MWG_ERR_ARG, "tree import: node before its parent");
-            }
-            const int id = T.add(depth[i], (int)lr[3 * i], (int)lr[3 * i + 1], (int)lr[3 * i + 2], par);
+            const int d = depth[i], x = (int)lr[3 * i], y = (int)lr[3 * i + 1], z = (int)lr[3 * i + 2];
+            const int par = d > 0 ? T.find(d - 1, x >> 1, y >> 1, z >> 1) : -1;
+            const int id = T.add(d, x, y, z, par);
             if (par < 0) T.roots.push_back(id);
-            else if (((lr[3 * i] & 1) | ((lr[3 * i + 1] & 1) << 1) | ((lr[3 * i + 2] & 1) << 2)) == 0) T.child[par] = id;
+            else if (((x & 1) | (y & 1) | (z & 1)) == 0) T.child[par] = id;
         }
-        if ((int)T.roots.size() != nr) throw Error(MWG_ERR_ARG, "tree import: root count does not match the MRA");
         T.ensure_dev(std::max<int64_t>(n, 1));
         HC(hipMemcpyAsync(T.coef.p, coefs, (size_t)n * 8 * T.K3() * sizeof(double), hipMemcpyHostToDevice,
                           C->stream));
         T.refresh_norms();
     });
+}
+
+int mwg_tree_layout_check(const int *boxes, int max_depth, int64_t n, const int *depth, const int64_t *lr,
+                          char *msg, int msg_len) {
+    if (!boxes || n < 0 || (n > 0 && (!depth || !lr))) return MWG_ERR_ARG;
+    for (int q = 0; q < 3; q++)
+        if (boxes[q] < 1) return MWG_ERR_ARG;
+    const std::string bad = check_tree_layout(boxes, max_depth, n, depth, lr);
+    if (msg && msg_len > 0) {
+        std::strncpy(msg, bad.c_str(), (size_t)msg_len - 1);
+        msg[msg_len - 1] = 0;
+    }
+    return bad.empty() ? MWG_OK : MWG_ERR_ARG;
 }
 
 int mwg_tree_export(mwg_tree tree, int *depth, int64_t *lr, double *coefs) {
@@ -2445,8 +2489,8 @@ int mwg_predictor_train(mwg_predictor pred, int64_t n, const float *X, const flo
 
 int mwg_ctx_set_tuning(mwg_ctx ctx, int fp64_chunk, int slice_chunk) {
     if (!ctx) return MWG_ERR_ARG;
-    if (fp64_chunk < 1 || fp64_chunk > 4096 || slice_chunk < 1 || slice_chunk > MAX_SLICE_CHUNK) {
-        ctx->err = "mwg_ctx_set_tuning: fp64_chunk must be in [1, 4096] and slice_chunk in [1, 16]";
+    if (fp64_chunk < 1 || fp64_chunk > 4096 || slice_chunk < 0 || slice_chunk > MAX_SLICE_CHUNK) {
+        ctx->err = "mwg_ctx_set_tuning: fp64_chunk must be in [1, 4096] and slice_chunk in [0, 4096]";
         return MWG_ERR_ARG;
     }
     ctx->chunk = fp64_chunk;

@@ -274,4 +274,44 @@ bool split_check(int scale, double wnorm, double prec, double tnorm) {
     return wnorm > thr;
 }
 
+std::string check_tree_layout(const int boxes[3], int max_depth, int64_t n, const int *depth, const int64_t *lr) {
+    const int64_t nr = (int64_t)boxes[0] * boxes[1] * boxes[2];
+    if (n < nr || (n - nr) % 8 != 0) return "expected the roots followed by complete sibling groups of 8";
+    if (max_depth < 0 || max_depth > 30) return "max depth outside [0, 30]";
+    std::unordered_map<uint64_t, int64_t> index;
+    index.reserve((size_t)n * 2);
+    std::vector<char> has_children;
+    has_children.reserve((size_t)n);
+    int64_t group_parent = -1;
+    for (int64_t i = 0; i < n; i++) {
+        const std::string at = "node " + std::to_string(i) + ": ";
+        const int d = depth[i];
+        if (d < 0 || d > max_depth) return at + "depth outside [0, max depth]";
+        for (int q = 0; q < 3; q++)
+            if (lr[3 * i + q] < 0 || lr[3 * i + q] >= ((int64_t)boxes[q] << d)) return at + "translation outside the box";
+        const int64_t x = lr[3 * i], y = lr[3 * i + 1], z = lr[3 * i + 2];
+        const uint64_t key = node_key(d, x, y, z);
+        if (index.count(key)) return at + "duplicate node";
+        if (i < nr) {
+            const int64_t ex = i % boxes[0], ey = (i / boxes[0]) % boxes[1], ez = i / ((int64_t)boxes[0] * boxes[1]);
+            if (d != 0 || x != ex || y != ey || z != ez) return at + "the roots must come first, in x-fastest order";
+        } else {
+            const int a = (int)((x & 1) | ((y & 1) << 1) | ((z & 1) << 2));
+            if (d == 0 || a != (int)((i - nr) % 8)) return at + "sibling groups must be complete and in child order";
+            const auto it = index.find(node_key(d - 1, x >> 1, y >> 1, z >> 1));
+            if (it == index.end()) return at + "node before its parent";
+            if (a == 0) {
+                if (has_children[(size_t)it->second]) return at + "parent already has children";
+                has_children[(size_t)it->second] = 1;
+                group_parent = it->second;
+            } else if (it->second != group_parent) {
+                return at + "sibling groups must be complete and in child order";
+            }
+        }
+        index.emplace(key, i);
+        has_children.push_back(0);
+    }
+    return std::string();
+}
+
 }  // namespace mwg

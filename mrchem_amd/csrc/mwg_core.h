@@ -56,4 +56,10 @@ constexpr double GAUSS_CUT = 46.0;  // exp(-46) ~ 1e-20: translation band of a G
 // split rule: ||w|| > prec * ||tree|| * 2^{-(n+1)/2}
 bool split_check(int scale, double wnorm, double prec, double tnorm);
 
+// The packed tree layout of import/export (include/mwg.h): the roots first, x fastest,
+// then complete groups of 8 siblings in child order (x & 1 | (y & 1) << 1 | (z & 1) << 2),
+// each group after its parent; depths in [0, max_depth], translations inside the box at
+// the node's depth, no node twice.  Returns an empty string or the first violation.
+std::string check_tree_layout(const int boxes[3], int max_depth, int64_t n, const int *depth, const int64_t *lr);
+
 }  // namespace mwg

@@ -87,12 +87,16 @@ static_assert(sizeof(Slice) == 32, "slice record is 32 bytes");
 #define MWG_ROUTE_FP64 2097152.0
 #endif
 constexpr double ROUTE_FP64 = MWG_ROUTE_FP64;
-// slices per wave of an MFMA task: a wave sums its slices' results in FP32 before
-// the FP64 sum over waves and tasks, so its run is kept short: the FP32 running sum
-// of n <= 16 slices adds at most (n - 1) 2^-24 of their summed bounds, i.e. below
-// 30 thr for a wave whose 16 slices all sit at the routing bound (the plain method
-// drops every tuple below thr, hundreds per node)
-constexpr int MAX_SLICE_CHUNK = 16;
+// slices per MFMA task.  k = 9: each slice result is added to the task's FP64
+// accumulators, so the length of a task changes only the summation grouping; it sets
+// how many node slots (8 (k+1)^3 FP64 each) an iteration writes and reads back.
+// k = 7: a task's four waves take every fourth slice and each sums its slices in FP32
+// before the FP64 sum over waves, so its run is kept short: the FP32 running sum of
+// n <= 16 slices adds at most (n - 1) 2^-24 of their summed bounds, i.e. below 30 thr
+// for a wave whose 16 slices all sit at the routing bound (the plain method drops
+// every tuple below thr, hundreds per node)
+constexpr int MAX_SLICE_CHUNK = 4096;
+constexpr int MFMA8_TASK_SLICES = 64;
 // on the root level (no screening threshold yet) slices above ROUTE_ROOT times the
 // largest possible bound run in FP64; FP32 rounding of the others stays ~1e-12 of it,
 // small also for outputs far below that bound (pair densities of orthogonal orbitals)

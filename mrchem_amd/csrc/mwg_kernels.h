@@ -68,6 +68,7 @@ struct TaskSummary {
     int ntasks, nstasks, nreds, nslots, nnslots;
     int overflow;  // bit 0: a capacity below, bit 1: generated-node pool
     int gen_count, gen_overflow;  // generated-node counter and table-overflow flag, for the host
+    int schunk;                   // slices per MFMA task used by the schedule (mixed)
 };
 struct TaskCaps {
     long long tuples, slices;
@@ -107,6 +108,7 @@ struct GenState {
     double *pool;  // pcap x K^3
     double *norm;  // pcap
     int pcap;      // entries allocated in pool and norm (<= cap)
+    long long *tabcnt;  // per job table: [4 t + 3] counts the generated nodes it inserted
 };
 
 hipError_t launch_tblock(hipStream_t s, int n, const int *item_term, const long long *item_delta, const double *b,
@@ -177,13 +179,19 @@ struct SliceTask {
     int g;
     int slot;  // node slot (8 K^3 doubles)
 };
-hipError_t launch_enum_mixed(hipStream_t s, int mode, int nw, int nparts, const WorkItem *work, const JobTab *jobs,
+hipError_t launch_enum_mixed(hipStream_t s, int mode, int K, int nw, int nparts, const WorkItem *work,
+                             const JobTab *jobs,
                              const TermTab *tabs, const GenState &gs, int *thread_off, int *seg_total,
                              const long long *seg_ofs, Tuple *out, Slice *sout, const TaskSummary *sum,
                              const SegRec *segs, const long long *gslice);
+// FP32 MFMA slice contraction (K = 8 or 10; mwg_mfma.hip).  A task is a run of slices of
+// one output node; mfma_tasks_per_block(K) tasks share a 256-thread block.  Each task adds
+// the surviving tuples of its slices to tabcnt[4 t + 2] of its job table t.
+int mfma_tasks_per_block(int K);
 hipError_t launch_contract_mfma(hipStream_t s, int K, int ntask, const TaskSummary *sum, int grid,
                                 const SliceTask *tasks, const WorkItem *work, const JobTab *jobs,
-                                const TermTab *tabs, const Slice *slices, const double *genPool, double *nodeslots);
+                                const TermTab *tabs, const Slice *slices, const double *genPool, double *nodeslots,
+                                long long *tabcnt);
 struct MulItem {
     int node;          // output node
     int depth;
@@ -197,7 +205,8 @@ hipError_t launch_add(hipStream_t s, int n, const MulItem *items, const DevBasis
                       double cb, const double *bcoef, double *out);
 hipError_t launch_to_f32(hipStream_t s, size_t n, const double *in, float *out);
 // from the per-segment counts (nseg = 8: plain FP64, 9: mixed): segment offsets, FP64
-// tasks (full chunks first), MFMA slice tasks, reduce items, slot numbering
+// tasks (full chunks first), MFMA slice tasks, reduce items, slot numbering.  wchunk:
+// slices per MFMA task, or (<= 0) about -wchunk tasks over the iteration's slices.
 hipError_t launch_build_tasks(hipStream_t s, int nw, int nseg, int nparts, int chunk, int wchunk, const int *tot,
                               TaskCaps caps, const int *gen_count, int gen_pcap, long long *ofs, ContractTask *tasks,
                               SliceTask *stasks, ReduceItem *reds, TaskSummary *sum, SegRec *segs, int *segcnt,

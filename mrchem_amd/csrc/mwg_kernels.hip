@@ -511,6 +511,7 @@ __device__ inline void gen_insert(const JobTab &J, const GenState &gs, int job, 
         unsigned long long old = atomicCAS(&J.genHash.keys[i], 0ull, kk);
         if (old == 0ull) {
             const int idx = atomicAdd(gs.count, 1);
+            atomicAdd((unsigned long long *)&gs.tabcnt[4 * job + 3], 1ull);
             if (idx < gs.cap) {
                 gs.keys[idx] = key;
                 gs.job[idx] = job;
@@ -1257,7 +1258,8 @@ __global__ void __launch_bounds__(1024)
     // per job table: tuples and slices of its output nodes (statistics for the host),
     // summed in LDS for the first BT_TABS tables, in global memory beyond
     __shared__ unsigned long long lcnt[2 * BT_TABS];
-    for (int q = threadIdx.x; q < 2 * ntab; q += blockDim.x) tabcnt[q] = 0;
+    // (entries 4 t + 3 hold the generated-node counts of the iteration: kept)
+    for (int q = threadIdx.x; q < 3 * ntab; q += blockDim.x) tabcnt[4 * (q / 3) + q % 3] = 0;
     for (int q = threadIdx.x; q < 2 * BT_TABS; q += blockDim.x) lcnt[q] = 0;
     // phase 0: segment counts (sum over the parts) and per-part offsets inside the segment
     // (relative to the segment's start)
@@ -1279,6 +1281,17 @@ __global__ void __launch_bounds__(1024)
     __syncthreads();
     // phase 1: this thread's totals (a contiguous range of output nodes)
     const int g0 = (int)((long long)nw * threadIdx.x / 1024), g1 = (int)((long long)nw * (threadIdx.x + 1) / 1024);
+    if (mixed && wchunk <= 0) {
+        // slices per MFMA task from the iteration's slice total: about -wchunk tasks
+        // (at least 16 and at most 1024 slices each)
+        long long sl1 = 0;
+        for (int g = g0; g < g1; g++) sl1 += sc[(size_t)g * 9 + 8];
+        long long v1[1] = {sl1}, t1[1];
+        block_excl_scan_1024_n<1>(v1, (long long(*)[1])wsum6, t1);
+        const long long want = (t1[0] + (-wchunk) - 1) / (-wchunk);
+        wchunk = (int)min(1024LL, max(16LL, want));
+        __syncthreads();
+    }
     long long tup = 0, sl = 0, nt = 0, nst = 0, slots = 0, nred = 0;
     for (int g = g0; g < g1; g++) {
         if (mixed) {
@@ -1315,6 +1328,7 @@ __global__ void __launch_bounds__(1024)
         sum->overflow = overflow;
         sum->gen_count = *(volatile const int *)gen_count;
         sum->gen_overflow = *(volatile const int *)gen_overflow;
+        sum->schunk = mixed ? wchunk : 0;
     }
     if (overflow) return;
     auto emit_task = [&](const SegRec &r, int sgi, int h) {
@@ -1335,8 +1349,8 @@ __global__ void __launch_bounds__(1024)
             atomicAdd(&lcnt[2 * run_jt], run_t);
             atomicAdd(&lcnt[2 * run_jt + 1], run_s);
         } else {
-            atomicAdd((unsigned long long *)&tabcnt[2 * run_jt], run_t);
-            atomicAdd((unsigned long long *)&tabcnt[2 * run_jt + 1], run_s);
+            atomicAdd((unsigned long long *)&tabcnt[4 * run_jt], run_t);
+            atomicAdd((unsigned long long *)&tabcnt[4 * run_jt + 1], run_s);
         }
     };
     for (int g = g0; g < g1; g++) {
@@ -1392,7 +1406,8 @@ __global__ void __launch_bounds__(1024)
     }
     flush_run();
     __syncthreads();
-    for (int q = threadIdx.x; q < 2 * min(ntab, BT_TABS); q += blockDim.x) tabcnt[q] = (long long)lcnt[q];
+    for (int q = threadIdx.x; q < 2 * min(ntab, BT_TABS); q += blockDim.x)
+        tabcnt[4 * (q >> 1) + (q & 1)] = (long long)lcnt[q];
     // phase 3: the listed segments, one wave per segment with the lanes over its tasks
     // (the fill kernels add the segment starts to the per-part offsets themselves)
     const int nb = min(nbig, BT_LIST);
@@ -1586,11 +1601,11 @@ __global__ void __launch_bounds__(256) k_reduce_partials(int n_arg, const TaskSu
 // thread: 8 tuple segments (per gt) + 1 slice segment; layout of thread_off and
 // seg_total: [g][9][...].
 template <int MODE>
-__global__ void __launch_bounds__(256) k_enum_mixed(int nw, int nparts, const WorkItem *work, const JobTab *jobs,
-                                                    const TermTab *tabs, GenState gs, int *thread_off,
-                                                    int *seg_total, const long long *seg_ofs, Tuple *out,
-                                                    Slice *sout, const TaskSummary *sum, const SegRec *segs,
-                                                    const long long *gslice) {
+__global__ void __launch_bounds__(256) k_enum_mixed(int nw, int nparts, int K2, const WorkItem *work,
+                                                    const JobTab *jobs, const TermTab *tabs, GenState gs,
+                                                    int *thread_off, int *seg_total, const long long *seg_ofs,
+                                                    Tuple *out, Slice *sout, const TaskSummary *sum,
+                                                    const SegRec *segs, const long long *gslice) {
     if (MODE == 1 && sum->overflow) return;
     const int g = blockIdx.x / nparts, part = blockIdx.x % nparts;
     const WorkItem wi = work[g];
@@ -1658,9 +1673,9 @@ __global__ void __launch_bounds__(256) k_enum_mixed(int nw, int nparts, const Wo
                                     // surviving components only (the same tuples as the plain
                                     // work list, FP32 in place of FP64)
                                     const unsigned long long wb = (unsigned long long)__double_as_longlong(tb.w);
-                                    const int xo = (int)(tb.blk + ((dx + tb.band) * 4 + (ft & 1)) * 64);
-                                    const int yo = (int)(tb.blk + ((dy + tb.band) * 4 + ((ft >> 1) & 1)) * 64);
-                                    const int zo = (int)(tb.blk + ((dz + tb.band) * 4 + ((ft >> 2) & 1)) * 64);
+                                    const int xo = (int)(tb.blk + ((dx + tb.band) * 4 + (ft & 1)) * K2);
+                                    const int yo = (int)(tb.blk + ((dy + tb.band) * 4 + ((ft >> 1) & 1)) * K2);
+                                    const int zo = (int)(tb.blk + ((dz + tb.band) * 4 + ((ft >> 2) & 1)) * K2);
                                     uint4 *dst = (uint4 *)(sout + base[8] + cnt[8]);
                                     dst[0] = make_uint4((unsigned)wb, (unsigned)(wb >> 32), (unsigned)f, (unsigned)xo);
                                     dst[1] = make_uint4((unsigned)yo, (unsigned)zo, (unsigned)ft | ((unsigned)surv << 8), 0u);
@@ -1685,239 +1700,6 @@ __global__ void __launch_bounds__(256) k_enum_mixed(int nw, int nparts, const Wo
             for (int q = 0; q < 9; q++) seg_total[((size_t)g * 9 + q) * nparts + part] = scan[q][255];
     }
     (void)nw;
-}
-
-// ----------------------------------------------------------------- apply: FP32 MFMA slice contraction (k = 7)
-
-typedef float floatx4 __attribute__((ext_vector_type(4)));
-
-// One workgroup of 4 waves per task = a chunk of the slices of one output node, the
-// waves taking every fourth slice.  A slice
-// (f, ft, term, delta) produces all 8 output components with three GEMMs on
-// v_mfma_f32_16x16x4_f32, M = 16 = (gt_d, a) stacked [X_{0,ft_d}; X_{1,ft_d}],
-// K = 8, N = 64, 128, 256 (the dimensions already contracted).  Lane layout of
-// the instruction: A[i][k] from lane i%16 + 16 k, B[k][j] from lane j%16 + 16 k,
-// D[4 (lane/16) + r][lane%16] in register r.  Intermediates rotate through LDS
-// (rows padded to 68 and 132 floats: conflict-free for the access patterns).
-// Each slice result is scaled by the term weight (zero for components the slice
-// does not reach) and summed in FP32 over the wave's slices (at most
-// MAX_SLICE_CHUNK), then in FP64 over the waves and over the node's tasks.
-// operands of one slice in registers: A fragments of the three passes and the
-// pass-x B fragments of f, as loaded (FP64: converted to FP32 only when the slice
-// is contracted, so that the loads for a later slice stay in flight meanwhile)
-struct SliceRaw {
-    float ax[2], ay[2], az[2];
-    double fb[4][2];
-    double w;
-    int mask;
-};
-
-struct SliceOps {
-    float ax[2], ay[2], az[2];
-    float fb[4][2];
-    double w;
-    int mask;
-};
-
-// a slice record through a wave-uniform pointer (scalar loads of its two 16-byte halves)
-__device__ inline Slice read_slice(const Slice *p) {
-    const uint4 *q = (const uint4 *)p;
-    const uint4 a = q[0], b = q[1];
-    Slice s;
-    s.w = __hiloint2double((int)a.y, (int)a.x);
-    s.f = (int)a.z;
-    s.xo = (int)a.w;
-    s.yo = (int)b.x;
-    s.zo = (int)b.y;
-    s.ft = (uint8_t)(b.z & 0xffu);
-    s.mask = (uint8_t)((b.z >> 8) & 0xffu);
-    return s;
-}
-
-__device__ inline SliceRaw load_slice_raw(const Slice &sl, const float *B32, const double *inCoef,
-                                          const double *genPool, int m, int q) {
-    constexpr int K = 8, K3 = 512;
-    const int gxm = m >> 3, am = m & 7;
-    SliceRaw o;
-    const gfloat *px = (const gfloat *)B32 + sl.xo + gxm * 128 + am * K;
-    const gfloat *py = (const gfloat *)B32 + sl.yo + gxm * 128 + am * K;
-    const gfloat *pz = (const gfloat *)B32 + sl.zo + gxm * 128 + am * K;
-#pragma unroll
-    for (int s2 = 0; s2 < 2; s2++) {
-        o.ax[s2] = px[4 * s2 + q];
-        o.ay[s2] = py[4 * s2 + q];
-        o.az[s2] = pz[4 * s2 + q];
-    }
-    const gdouble *f = sl.f >= 0 ? (const gdouble *)inCoef + (size_t)sl.f * 8 * K3 + sl.ft * K3
-                                 : (const gdouble *)genPool + (size_t)(-sl.f - 1) * K3;
-#pragma unroll
-    for (int t = 0; t < 4; t++)
-#pragma unroll
-        for (int s2 = 0; s2 < 2; s2++) o.fb[t][s2] = f[(16 * t + m) * K + 4 * s2 + q];
-    o.w = sl.w;
-    o.mask = sl.mask;
-    return o;
-}
-
-__device__ inline SliceOps slice_ops(const SliceRaw &r) {
-    SliceOps o;
-#pragma unroll
-    for (int s2 = 0; s2 < 2; s2++) {
-        o.ax[s2] = r.ax[s2];
-        o.ay[s2] = r.ay[s2];
-        o.az[s2] = r.az[s2];
-    }
-#pragma unroll
-    for (int t = 0; t < 4; t++)
-#pragma unroll
-        for (int s2 = 0; s2 < 2; s2++) o.fb[t][s2] = (float)r.fb[t][s2];
-    o.w = r.w;
-    o.mask = r.mask;
-    return o;
-}
-
-__device__ inline void mfma8_task(const SliceTask task, const WorkItem *work, const JobTab *jobs, const Slice *slices,
-                                  const double *genPool, double *nodeslots, float *lds) {
-    constexpr int K = 8, K3 = 512;
-    constexpr int S1 = 68, S2 = 132;
-    const int wave = wave_id(), lane = threadIdx.x & 63;
-    const WorkItem wi = work[task.g];
-    const JobTab &J = jobs[wi.job];
-    float *L1 = lds + wave * 16 * S1;
-    float *L2 = lds + 4 * 16 * S1 + wave * 16 * S2;
-    const int m = lane & 15, q = lane >> 4;
-    const int gxm = m >> 3, am = m & 7;
-    float acc[16][4];
-#pragma unroll
-    for (int t = 0; t < 16; t++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) acc[t][r] = 0.f;
-    // the four waves take every fourth slice of the task; the next slice's operands
-    // are loaded while the current one is contracted
-    // software pipeline over the wave's slices it = wave, wave + 4, ...: the record of
-    // slice it + 8 and the operands of slice it + 4 are loaded while slice it is contracted
-    // (indices past the end are clamped to the last slice: the loads stay unconditional,
-    // so that the waits for them fall where their data are used)
-    const Slice *srec = slices + task.start;
-    const int last = task.count - 1;  // a task holds at least one slice
-    SliceRaw craw = load_slice_raw(read_slice(srec + min(wave, last)), J.operBlocks32, J.inCoef, genPool, m, q);
-    Slice rnext = read_slice(srec + min(wave + 4, last));
-    for (int it = wave; it < task.count; it += 4) {
-        const SliceRaw nraw = load_slice_raw(rnext, J.operBlocks32, J.inCoef, genPool, m, q);
-        rnext = read_slice(srec + min(it + 8, last));
-        const SliceOps cur = slice_ops(craw);
-        // pass x: C1[(gx,a)][n = l*8 + j] = sum_i X[a][i] f[n*8 + i]
-        // (tiles in groups of 4 independent MFMA chains, operands read before the MFMAs)
-        {
-            floatx4 c[4];
-#pragma unroll
-            for (int t = 0; t < 4; t++)
-                c[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.ax[0], cur.fb[t][0], floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
-#pragma unroll
-            for (int t = 0; t < 4; t++) c[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.ax[1], cur.fb[t][1], c[t], 0, 0, 0);
-#pragma unroll
-            for (int t = 0; t < 4; t++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) L1[(4 * q + r) * S1 + 16 * t + m] = c[t][r];
-        }
-        wave_sync();
-        // pass y: C2[(gy,b)][n2 = l*16 + (gx,a)] = sum_j Y[b][j] C1[(gx,a)][l*8 + j]
-#pragma unroll
-        for (int g4 = 0; g4 < 2; g4++) {
-            float b[4][2];
-#pragma unroll
-            for (int t = 0; t < 4; t++)
-#pragma unroll
-                for (int s2 = 0; s2 < 2; s2++) b[t][s2] = L1[m * S1 + (4 * g4 + t) * 8 + 4 * s2 + q];
-            floatx4 c[4];
-#pragma unroll
-            for (int t = 0; t < 4; t++)
-                c[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.ay[0], b[t][0], floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
-#pragma unroll
-            for (int t = 0; t < 4; t++) c[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.ay[1], b[t][1], c[t], 0, 0, 0);
-#pragma unroll
-            for (int t = 0; t < 4; t++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) L2[(4 * q + r) * S2 + 16 * (4 * g4 + t) + m] = c[t][r];
-        }
-        wave_sync();
-        // pass z: C3[(gz,c)][n3 = (gy,b)*16 + (gx,a)] = sum_l Z[c][l] C2[(gy,b)][l*16 + (gx,a)]
-        // Output component gt = gx | gy << 1 | gz << 2 with gx = m >> 3 (the lane), gy = t3 >> 3
-        // and gz = (4 q + r) >> 3 = q >> 1 (the lane): two weights per lane and slice, masked
-        // by the slice's surviving components.
-        const int gtl = gxm | ((q >> 1) << 2), gth = gtl | 2;
-        const float wlow = ((cur.mask >> gtl) & 1) ? (float)cur.w : 0.f;
-        const float whigh = ((cur.mask >> gth) & 1) ? (float)cur.w : 0.f;
-        // tiles t3 < 8 carry gy = 0 (gt in {0,1,4,5}: mask bits 0x33), t3 >= 8 gy = 1 (0xCC);
-        // a half without a surviving component is skipped (the mask is uniform per wave)
-        const bool need_lo = (cur.mask & 0x33) != 0, need_hi = (cur.mask & 0xCC) != 0;
-#pragma unroll
-        for (int g4 = 0; g4 < 4; g4++) {
-            if (g4 < 2 ? !need_lo : !need_hi) continue;
-            float b[4][2];
-#pragma unroll
-            for (int t = 0; t < 4; t++)
-#pragma unroll
-                for (int s2 = 0; s2 < 2; s2++) b[t][s2] = L2[(4 * g4 + t) * S2 + (4 * s2 + q) * 16 + m];
-            floatx4 c[4];
-#pragma unroll
-            for (int t = 0; t < 4; t++)
-                c[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.az[0], b[t][0], floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
-#pragma unroll
-            for (int t = 0; t < 4; t++) c[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.az[1], b[t][1], c[t], 0, 0, 0);
-            const float wm = g4 < 2 ? wlow : whigh;
-#pragma unroll
-            for (int t = 0; t < 4; t++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) acc[4 * g4 + t][r] = fmaf(wm, c[t][r], acc[4 * g4 + t][r]);
-        }
-        wave_sync();
-        craw = nraw;
-    }
-    // FP64 sum over the waves in LDS, wave 0 first, then 1, 2, 3 (one 32 KB buffer)
-    double *R = (double *)lds;
-    __syncthreads();
-    for (int src = 0; src < 4; src++) {
-        if (wave == src) {
-#pragma unroll
-            for (int t3 = 0; t3 < 16; t3++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    double *e = &R[(t3 * 4 + r) * 64 + lane];
-                    *e = src == 0 ? (double)acc[t3][r] : *e + (double)acc[t3][r];
-                }
-        }
-        __syncthreads();
-    }
-    {
-        double *dst = nodeslots + (size_t)task.slot * 8 * K3;
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int t3 = 4 * wave + u;
-            const int gy = t3 >> 3, b = t3 & 7;
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int m3 = 4 * q + r;
-                const int gz = m3 >> 3, cc = m3 & 7;
-                const int gt = gxm | (gy << 1) | (gz << 2);
-                dst[gt * K3 + (cc * K + b) * K + am] = R[(t3 * 4 + r) * 64 + lane];
-            }
-        }
-    }
-}
-
-__global__ void __launch_bounds__(256)
-k_contract_mfma8(int ntask_arg, const TaskSummary *sum,
-                                                        const SliceTask *tasks, const WorkItem *work,
-                                                        const JobTab *jobs, const TermTab *tabs,
-                                                        const Slice *slices, const double *genPool,
-                                                        double *nodeslots) {
-    constexpr int S1 = 68, S2 = 132;
-    // per-wave operand staging; reused for the cross-wave reduction at the end
-    __shared__ __align__(16) float lds[4 * 16 * S1 + 4 * 16 * S2];
-    const int ntask = sum ? (sum->overflow ? 0 : sum->nstasks) : ntask_arg;
-    if ((int)blockIdx.x >= ntask) return;
-    mfma8_task(tasks[blockIdx.x], work, jobs, slices, genPool, nodeslots, lds);
 }
 
 
@@ -2319,29 +2101,18 @@ hipError_t launch_reduce_partials(hipStream_t s, int n, const TaskSummary *sum, 
     return hipGetLastError();
 }
 
-hipError_t launch_enum_mixed(hipStream_t s, int mode, int nw, int nparts, const WorkItem *work, const JobTab *jobs,
+hipError_t launch_enum_mixed(hipStream_t s, int mode, int K, int nw, int nparts, const WorkItem *work, const JobTab *jobs,
                              const TermTab *tabs, const GenState &gs, int *thread_off, int *seg_total,
                              const long long *seg_ofs, Tuple *out, Slice *sout, const TaskSummary *sum,
                              const SegRec *segs, const long long *gslice) {
     if (nw <= 0) return hipSuccess;
     const unsigned nb = (unsigned)nw * (unsigned)nparts;
     if (mode == 0)
-        hipLaunchKernelGGL(k_enum_mixed<0>, dim3(nb), dim3(256), 0, s, nw, nparts, work, jobs, tabs, gs, thread_off,
-                           seg_total, seg_ofs, out, sout, sum, segs, gslice);
+        hipLaunchKernelGGL(k_enum_mixed<0>, dim3(nb), dim3(256), 0, s, nw, nparts, K * K, work, jobs, tabs, gs,
+                           thread_off, seg_total, seg_ofs, out, sout, sum, segs, gslice);
     else
-        hipLaunchKernelGGL(k_enum_mixed<1>, dim3(nb), dim3(256), 0, s, nw, nparts, work, jobs, tabs, gs, thread_off,
-                           seg_total, seg_ofs, out, sout, sum, segs, gslice);
-    return hipGetLastError();
-}
-
-hipError_t launch_contract_mfma(hipStream_t s, int K, int ntask, const TaskSummary *sum, int grid,
-                                const SliceTask *tasks, const WorkItem *work, const JobTab *jobs,
-                                const TermTab *tabs, const Slice *slices, const double *genPool, double *nodeslots) {
-    if (K != 8) return hipErrorInvalidValue;
-    const int nb = sum ? grid : ntask;
-    if (nb <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_contract_mfma8, dim3(nb), dim3(256), 0, s, ntask, sum, tasks, work, jobs, tabs, slices,
-                       genPool, nodeslots);
+        hipLaunchKernelGGL(k_enum_mixed<1>, dim3(nb), dim3(256), 0, s, nw, nparts, K * K, work, jobs, tabs, gs,
+                           thread_off, seg_total, seg_ofs, out, sout, sum, segs, gslice);
     return hipGetLastError();
 }
 

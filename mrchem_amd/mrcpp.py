@@ -360,6 +360,18 @@ def trace_get(ctx: Context | None = None):
     return X.reshape(n.value, nf.value), y
 
 
+def layout_check(boxes, max_depth: int, depth, lr) -> str:
+    """The packed-layout check of tree import without a device: '' or the first violation."""
+    buf = C.create_string_buffer(256)
+    rc = lib().mwg_tree_layout_check(np.ascontiguousarray(boxes, np.int32), int(max_depth), len(depth),
+                                     np.ascontiguousarray(depth, np.int32),
+                                     np.ascontiguousarray(lr, np.int64).reshape(-1), buf, 256)
+    msg = buf.value.decode()
+    if rc != 0 and not msg:
+        raise MwgError(f"mwg_tree_layout_check: status {rc}")
+    return msg
+
+
 def mra_from(box: dict, order: int, basis: str = "L", max_depth: int = 16, ctx: Context | None = None):
     """MRA from the box dictionaries of ``mrchem_amd.configs`` (n0, corner, boxes, sfac)."""
     b = BoundingBox(box["n0"], box["corner"], box["boxes"], box.get("sfac", 1.0))

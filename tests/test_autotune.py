@@ -1,6 +1,7 @@
 """Contraction task granularity: every setting gives the same tree and coefficients
-(to FP64 rounding on the FP64 path; on the mixed path to the FP32 rounding of the
-slices routed to FP32, far below prec), invalid settings are refused, and the
+(to FP64 rounding on the FP64 path and at k = 9; at k = 7 the mixed path's waves
+sum their slices in FP32, so there a different grouping changes the result at the
+FP32 rounding of the slices routed to FP32, far below prec), invalid settings are refused, and the
 autotuner installs one of its candidates."""
 import numpy as np
 import pytest
@@ -20,7 +21,7 @@ def test_tuning_does_not_change_results():
     default = ctx.get_tuning()
     ref = None
     try:
-        for fc, sc in [(64, 16), (1, 1), (7, 3), (4096, 16)]:
+        for fc, sc in [(64, 0), (1, 1), (7, 3), (4096, 16), (32, 4096)]:
             ctx.set_tuning(fc, sc)
             for flags in (G.MWG_APPLY_FP64, G.MWG_APPLY_MIXED):
                 outs = [G.FunctionTree(mra) for _ in trees]
@@ -44,7 +45,7 @@ def test_invalid_tuning_is_refused():
     from mrchem_amd import mrcpp as G
     ctx = G.default_context()
     before = ctx.get_tuning()
-    for fc, sc in [(0, 16), (64, 0), (5000, 16), (64, 17)]:
+    for fc, sc in [(0, 16), (64, -1), (5000, 16), (64, 4097)]:
         with pytest.raises(G.MwgError):
             ctx.set_tuning(fc, sc)
     assert ctx.get_tuning() == before

@@ -71,3 +71,43 @@ def test_gaussian_product_theorem():
     p = G.product(a, b)
     pts = np.random.default_rng(0).uniform(-2, 2, (200, 3))
     assert np.abs(p(pts) - a(pts) * b(pts)).max() < 1e-13
+
+
+def test_tree_import_refuses_malformed_layouts(oracle):
+    """The packed-layout contract of mwg_tree_import (include/mwg.h), checked on the host:
+    a valid export passes; a node too deep, a translation outside the box, a duplicate,
+    a node before its parent, an incomplete or reordered sibling group and roots out of
+    order are each refused with a message naming the node."""
+    import numpy as np
+    from mrchem_amd import configs
+    from mrchem_amd import mrcpp as G
+    omra = oracle.MRA(**configs.BOX16, k=3, max_depth=6)
+    t = oracle.Tree(omra).project_gauss(1e-3, configs.density3(0)[0])
+    depth, lr, _ = t.export()
+    boxes = configs.BOX16["boxes"]
+    assert G.layout_check(boxes, 6, depth, lr) == ""
+    assert depth.max() >= 2
+
+    def bad(d, l, what, max_depth=6):
+        msg = G.layout_check(boxes, max_depth, d, l)
+        assert what in msg, (what, msg)
+
+    first = int(np.argmax(depth == depth.max()))  # first node of a deepest group
+    bad(depth, lr, "depth outside", max_depth=int(depth.max()) - 1)
+    l2 = lr.copy()
+    l2[first, 0] = (boxes[0] << depth[first]) + 2
+    bad(depth, l2, "translation outside")
+    d2, l2 = depth.copy(), lr.copy()
+    l2[first + 1] = l2[first]
+    bad(d2, l2, "duplicate")
+    perm = np.arange(len(depth))
+    perm[[first, first + 1]] = perm[[first + 1, first]]
+    bad(depth[perm], lr[perm], "child order")
+    bad(depth[:-3], lr[:-3], "complete sibling groups")
+    nr = int(np.prod(boxes))
+    perm = np.arange(len(depth))
+    perm[[0, 1]] = perm[[1, 0]]
+    bad(depth[perm], lr[perm], "roots must come first")
+    g = nr + 8 * ((first - nr) // 8)   # move the deepest group before everything below the roots
+    order = list(range(nr)) + list(range(g, g + 8)) + [i for i in range(nr, len(depth)) if not g <= i < g + 8]
+    bad(depth[order], lr[order], "before its parent")

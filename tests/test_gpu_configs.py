@@ -1,0 +1,292 @@
+"""The benchmark workloads on the GPU, at their configured orders and precisions.
+
+Each workload is checked against the FP64 oracle on a reduced set of applies (same
+tuples, same grid, coefficients to 1e-12), and the full batch through properties:
+the mixed-precision (FP32 MFMA, FP64 accumulation) and the predictor variants against
+the plain FP64 apply on the GPU, by the exact L2 norm of the difference of the two
+trees (computed in the multiwavelet basis on the union grid, mwg_add), per output.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from mrchem_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def G():
+    from mrchem_amd import mrcpp
+    mrcpp.default_context()
+    return mrcpp
+
+
+def _same_tree(t_gpu, t_ref, rtol=1e-12):
+    d1, l1, c1 = t_gpu.export()
+    d2, l2, c2 = t_ref.export()
+    assert len(d1) == len(d2)
+    k1 = {(int(d), *map(int, l)): i for i, (d, l) in enumerate(zip(d1, l1))}
+    idx = np.array([k1[(int(d), *map(int, l))] for d, l in zip(d2, l2)])
+    assert np.abs(c1[idx] - c2).max() <= rtol * np.abs(c2).max()
+
+
+def l2_diff(G, a, b):
+    """||a - b|| exactly (multiwavelet coefficients on the union grid of a and b)."""
+    return math.sqrt(max(G.add(G.FunctionTree(a.mra), 1.0, a, -1.0, b).getSquareNorm(), 0.0))
+
+
+def _vs_oracle(G, oracle, mra, omra, otrees, opers_gpu, opers_ref, precs):
+    """GPU batch (FP64) against one oracle apply each: tuples, grid, coefficients."""
+    trees = [G.FunctionTree(mra).load(*t.export()) for t in otrees]
+    outs = [G.FunctionTree(mra) for _ in trees]
+    st = G.apply_batch(precs, outs, opers_gpu, trees)
+    for o, s, ot, op, p in zip(outs, st, otrees, opers_ref, precs):
+        oo = oracle.Tree(omra)
+        ost = oracle.apply(p, oo, op, ot)
+        assert s["tuples"] == ost["tuples"] and s["tuples_fp64"] == ost["tuples"]
+        assert s["nodes_out"] == ost["nodes_out"]
+        _same_tree(o, oo, 1e-12)
+    return outs, st
+
+
+def test_config2_helmholtz_k7_prec1e6_matches_oracle(G, oracle):
+    """The benchmark's Helmholtz batch (5 orbitals, k=7, prec 1e-6): one batched FP64 apply
+    on the device against the oracle's apply of each orbital."""
+    prec = 1e-6
+    orbs, nuc, mus = configs.orbitals(0, n_orb=5)
+    mra = G.mra_from(configs.BOX20, 7)
+    omra = oracle.MRA(**configs.BOX20, k=7, max_depth=16)
+    otrees = [oracle.Tree(omra).project_gauss(prec, o) for o in orbs]
+    _vs_oracle(G, oracle, mra, omra, otrees, [G.HelmholtzOperator(mra, mu, prec) for mu in mus],
+               [oracle.Oper(omra, mu, prec) for mu in mus], [prec] * 5)
+
+
+@pytest.mark.parametrize("basis", ["L", "I"])
+def test_config2_mixed_and_predictor_within_prec(G, basis):
+    """Mixed (FP32 MFMA slices, FP64 accumulation) and mixed + learned predictor against
+    the plain FP64 apply of the same batch: exact L2 difference of every output far
+    below prec ||out|| (FP32 rounding only), the same tuples as the plain method."""
+    prec = 1e-6
+    orbs, nuc, mus = configs.orbitals(0, n_orb=5)
+    mra = G.mra_from(configs.BOX20, 7, basis)
+    trees = [G.project(prec, G.FunctionTree(mra), o) for o in orbs]
+    ops = [G.HelmholtzOperator(mra, mu, prec) for mu in mus]
+    o64 = [G.FunctionTree(mra) for _ in trees]
+    s64 = G.apply_batch(prec, o64, ops, trees)
+    o32 = [G.FunctionTree(mra) for _ in trees]
+    s32 = G.apply_batch(prec, o32, ops, trees, flags=G.MWG_APPLY_MIXED)
+    for a, b, sa, sb in zip(o32, o64, s32, s64):
+        assert sa["slices"] > 0 and 0 < sa["tuples_fp64"] < sa["tuples"]
+        assert sa["nodes_out"] == sb["nodes_out"] and sa["tuples"] == sb["tuples"]
+        assert l2_diff(G, a, b) < 1e-2 * prec * math.sqrt(b.getSquareNorm())
+    # the predictor: trained on traces of other orbitals (seed 101), judged by the norm test
+    G.trace_enable(True)
+    orbs2, _, mus2 = configs.orbitals(101, n_orb=5)
+    t2 = [G.project(prec, G.FunctionTree(mra), o) for o in orbs2]
+    G.apply_batch(prec, [G.FunctionTree(mra) for _ in t2], [G.HelmholtzOperator(mra, m, prec) for m in mus2], t2)
+    X, y = G.trace_get()
+    G.trace_enable(False)
+    pr = G.Predictor(seed=0)
+    pr.train(X, y, epochs=100, batch=512, lr=3e-3)
+    pr.install(0.9)
+    op_ = [G.FunctionTree(mra) for _ in trees]
+    sp = G.apply_batch(prec, op_, ops, trees, flags=G.MWG_APPLY_MIXED | G.MWG_APPLY_PREDICT)
+    assert sp[0]["iters"] <= s64[0]["iters"]
+    for a, b, sa, sb in zip(op_, o64, sp, s64):
+        assert abs(sa["nodes_out"] - sb["nodes_out"]) <= 0.02 * sb["nodes_out"]
+        assert l2_diff(G, a, b) < prec * math.sqrt(b.getSquareNorm())
+
+
+def _exchange_inputs(G, mra, prec, n_orb=16):
+    orbs, nuc, mus = configs.orbitals(0, n_orb=n_orb, n_nuc=8, nuc_box=6.0)
+    trees = [G.project(prec, G.FunctionTree(mra), o) for o in orbs]
+    return orbs, trees
+
+
+def test_config3_pair_poissons_match_oracle(G, oracle):
+    """Exchange pair Poissons (16 orbitals on 8 nuclei, k=7, prec 1e-6): six pair densities
+    made on the device, Poisson-applied as one batch against the oracle's apply of each."""
+    prec = 1e-6
+    mra = G.mra_from(configs.BOX20, 7)
+    omra = oracle.MRA(**configs.BOX20, k=7, max_depth=16)
+    orbs, trees = _exchange_inputs(G, mra, prec)
+    pairs = [(0, 0), (0, 1), (3, 7), (5, 5), (10, 15), (14, 15)]
+    rho = G.multiply_batch(prec, [G.FunctionTree(mra) for _ in pairs], 1.0, [trees[i] for i, _ in pairs],
+                           [trees[j] for _, j in pairs])
+    otrees = [oracle.Tree(omra).load(*r.export()) for r in rho]
+    _vs_oracle(G, oracle, mra, omra, otrees, [G.PoissonOperator(mra, prec)] * len(pairs),
+               [oracle.poisson(omra, prec)] * len(pairs), [prec] * len(pairs))
+
+
+def test_config3_all_pairs_mixed_within_prec(G):
+    """All 136 pair Poissons in one mixed batch against the FP64 batch: every pair's exact
+    L2 difference below prec ||V_ij|| (and the same tuples as the plain method)."""
+    prec = 1e-6
+    mra = G.mra_from(configs.BOX20, 7)
+    orbs, trees = _exchange_inputs(G, mra, prec)
+    pairs = [(i, j) for i in range(16) for j in range(i, 16)]
+    rho = G.multiply_batch(prec, [G.FunctionTree(mra) for _ in pairs], 1.0, [trees[i] for i, _ in pairs],
+                           [trees[j] for _, j in pairs])
+    P = G.PoissonOperator(mra, prec)
+    o64 = [G.FunctionTree(mra) for _ in rho]
+    s64 = G.apply_batch(prec, o64, [P] * len(rho), rho)
+    o32 = [G.FunctionTree(mra) for _ in rho]
+    s32 = G.apply_batch(prec, o32, [P] * len(rho), rho, flags=G.MWG_APPLY_MIXED)
+    worst = 0.0
+    for a, b, sa, sb in zip(o32, o64, s32, s64):
+        rel = l2_diff(G, a, b) / math.sqrt(b.getSquareNorm())
+        worst = max(worst, rel)
+        assert sa["nodes_out"] == sb["nodes_out"] and sa["tuples"] == sb["tuples"]
+    assert worst < 1e-2 * prec, worst
+
+
+def test_config4_predictor_on_fresh_orbitals_k9(G):
+    """The predictor trained on k=7, prec 1e-6 refinement traces, used on 8 fresh orbitals
+    (seed 200) at k=9, prec 1e-7 (Helmholtz): no more iterations than the plain method,
+    every output within prec of the plain FP64 apply (exact L2), node counts within 2 %."""
+    mra7 = G.mra_from(configs.BOX20, 7)
+    G.trace_enable(True)
+    for seed in (101, 102):
+        orbs, _, mus = configs.orbitals(seed, n_orb=5)
+        t = [G.project(1e-6, G.FunctionTree(mra7), o) for o in orbs]
+        G.apply_batch(1e-6, [G.FunctionTree(mra7) for _ in t], [G.HelmholtzOperator(mra7, m, 1e-6) for m in mus], t)
+    X, y = G.trace_get()
+    G.trace_enable(False)
+    pr = G.Predictor(seed=0)
+    lb, la = pr.train(X, y, epochs=100, batch=512, lr=3e-3)
+    assert la < lb
+    pr.install(0.9)
+    prec = 1e-7
+    mra = G.mra_from(configs.BOX20, 9)
+    orbs, _, mus = configs.orbitals(200, n_orb=8)
+    trees = [G.project(prec, G.FunctionTree(mra), o) for o in orbs]
+    ops = [G.HelmholtzOperator(mra, m, prec) for m in mus]
+    o0 = [G.FunctionTree(mra) for _ in trees]
+    s0 = G.apply_batch(prec, o0, ops, trees)
+    o1 = [G.FunctionTree(mra) for _ in trees]
+    s1 = G.apply_batch(prec, o1, ops, trees, flags=G.MWG_APPLY_PREDICT)
+    assert s1[0]["iters"] <= s0[0]["iters"] and s1[0]["predicted_splits"] > 0
+    for a, b, sa, sb in zip(o1, o0, s1, s0):
+        assert abs(sa["nodes_out"] - sb["nodes_out"]) <= 0.02 * sb["nodes_out"]
+        assert l2_diff(G, a, b) < prec * math.sqrt(b.getSquareNorm())
+
+
+def _scf_inputs(G, mra, prec, n_orb=32):
+    orbs, nuc, _ = configs.orbitals(0, n_orb=n_orb, n_nuc=16, nuc_box=10.0)
+    rng = configs.SplitMix64(1000)
+    eps = [-rng.uniform(0.2, 2.0) for _ in range(n_orb)]
+    mus = [math.sqrt(-2.0 * e) for e in eps]
+    return orbs, mus
+
+
+def test_config5_reduced_step_matches_oracle(G, oracle):
+    """SCF operator step at k=9, prec 1e-6, reduced to two Helmholtz applies (mu from
+    eps ~ U[-2,-0.2]) and two pair Poissons, as one batch against the oracle."""
+    prec = 1e-6
+    mra = G.mra_from(configs.BOX20, 9)
+    omra = oracle.MRA(**configs.BOX20, k=9, max_depth=16)
+    orbs, mus = _scf_inputs(G, mra, prec)
+    ophi = [oracle.Tree(omra).project_gauss(prec, orbs[i]) for i in (0, 1, 2)]
+    phi = [G.FunctionTree(mra).load(*t.export()) for t in ophi]
+    rho = G.multiply_batch(prec, [G.FunctionTree(mra) for _ in range(2)], 1.0, [phi[0], phi[2]], [phi[1], phi[2]])
+    orho = [oracle.Tree(omra).load(*r.export()) for r in rho]
+    P, oP = G.PoissonOperator(mra, prec), oracle.poisson(omra, prec)
+    _vs_oracle(G, oracle, mra, omra, [ophi[0], ophi[1]] + orho,
+               [G.HelmholtzOperator(mra, mus[0], prec), G.HelmholtzOperator(mra, mus[1], prec), P, P],
+               [oracle.Oper(omra, mus[0], prec), oracle.Oper(omra, mus[1], prec), oP, oP], [prec] * 4)
+
+
+def test_config5_full_step_mixed_within_prec(G):
+    """The whole step (32 Helmholtz + 528 pair Poissons, k=9, prec 1e-6) as one mixed batch
+    (FP32 MFMA at k=9) against the FP64 batch: every output within 1e-2 prec of FP64 (exact
+    L2), the same grids and tuples."""
+    prec = 1e-6
+    mra = G.mra_from(configs.BOX20, 9)
+    orbs, mus = _scf_inputs(G, mra, prec)
+    trees = [G.project(prec, G.FunctionTree(mra), o) for o in orbs]
+    pairs = [(i, j) for i in range(32) for j in range(i, 32)]
+    rho = G.multiply_batch(prec, [G.FunctionTree(mra) for _ in pairs], 1.0, [trees[i] for i, _ in pairs],
+                           [trees[j] for _, j in pairs])
+    P = G.PoissonOperator(mra, prec)
+    inps = trees + rho
+    ops = [G.HelmholtzOperator(mra, m, prec) for m in mus] + [P] * len(rho)
+    o64 = [G.FunctionTree(mra) for _ in inps]
+    s64 = G.apply_batch(prec, o64, ops, inps)
+    o32 = [G.FunctionTree(mra) for _ in inps]
+    s32 = G.apply_batch(prec, o32, ops, inps, flags=G.MWG_APPLY_MIXED)
+    assert sum(s["slices"] for s in s32) > 0
+    worst = 0.0
+    for a, b, sa, sb in zip(o32, o64, s32, s64):
+        assert sa["nodes_out"] == sb["nodes_out"] and sa["tuples"] == sb["tuples"]
+        worst = max(worst, l2_diff(G, a, b) / math.sqrt(b.getSquareNorm()))
+    assert worst < 1e-2 * prec, worst
+
+
+@pytest.mark.parametrize("basis,mu", [("L", 0.0), ("I", 1.3)])
+def test_mixed_k9_matches_fp64(G, basis, mu):
+    """The k=9 MFMA slice kernel (padded 20-row operator tiles) against the plain FP64 apply
+    on a small case: same grid and tuples, difference at FP32 rounding."""
+    prec = 1e-5
+    g, _ = configs.density3(3)
+    mra = G.mra_from(configs.BOX16, 9, basis)
+    f = G.project(prec, G.FunctionTree(mra), g)
+    op = G.HelmholtzOperator(mra, mu, prec) if mu > 0 else G.PoissonOperator(mra, prec)
+    a, b = G.FunctionTree(mra), G.FunctionTree(mra)
+    sa = G.apply(prec, a, op, f, flags=G.MWG_APPLY_MIXED)
+    sb = G.apply(prec, b, op, f)
+    assert sa["slices"] > 0 and sa["tuples"] == sb["tuples"] and sa["nodes_out"] == sb["nodes_out"]
+    assert l2_diff(G, a, b) < 1e-3 * prec * math.sqrt(b.getSquareNorm())
+
+
+def test_mixed_is_refused_at_unbuilt_orders(G):
+    mra = G.mra_from(configs.BOX16, 5)
+    f = G.project(1e-4, G.FunctionTree(mra), configs.density3(0)[0])
+    with pytest.raises(G.MwgError, match="k = 7 and k = 9"):
+        G.apply(1e-4, G.FunctionTree(mra), G.PoissonOperator(mra, 1e-4), f, flags=G.MWG_APPLY_MIXED)
+    with pytest.raises(G.MwgError, match="unknown apply flags"):
+        G.apply(1e-4, G.FunctionTree(mra), G.PoissonOperator(mra, 1e-4), f, flags=8)
+
+
+def test_generated_node_counts_are_per_apply(G):
+    """gen_nodes counts the generated input nodes of each apply of a batch: a batch of two
+    reports each apply's own count (the same as applying it alone)."""
+    prec = 1e-5
+    mra = G.mra_from(configs.BOX16, 5)
+    f1 = G.project(prec, G.FunctionTree(mra), configs.density3(1)[0])
+    f2 = G.project(prec, G.FunctionTree(mra), configs.density3(2)[0])
+    P = G.PoissonOperator(mra, prec)
+    s1 = G.apply(prec, G.FunctionTree(mra), P, f1)
+    s2 = G.apply(prec, G.FunctionTree(mra), P, f2)
+    sb = G.apply_batch(prec, [G.FunctionTree(mra), G.FunctionTree(mra)], [P, P], [f1, f2])
+    assert s1["gen_nodes"] > 0 and s2["gen_nodes"] > 0
+    assert [s["gen_nodes"] for s in sb] == [s1["gen_nodes"], s2["gen_nodes"]]
+
+
+def test_exchange_with_screening_matches_oracle(G, oracle):
+    """ExchangePotential.cpp:287-347 with screening: the second setup scales each pair's
+    precision by the norms of the first (getScaledPrecision, :379-387); every pair
+    potential equals the oracle's multiply + apply at that pair's precision."""
+    from mrchem_amd import mrchem as MC
+    prec = 1e-5
+    orbs, nuc, mus = configs.orbitals(4, n_orb=4)
+    mra = G.mra_from(configs.BOX20, 7)
+    omra = oracle.MRA(**configs.BOX20, k=7, max_depth=16)
+    otrees = [oracle.Tree(omra).project_gauss(prec, o) for o in orbs]
+    Phi = [MC.Orbital(G.FunctionTree(mra).load(*t.export())) for t in otrees]
+    K = MC.ExchangePotential(G.PoissonOperator(mra, prec), Phi, mra, screen=True)
+    K.setup(prec)
+    K.clear()
+    K.pair_potentials = {}
+    K.setup(prec)
+    pairs = K.pair_list()
+    precs = {(i, j): p for i, j, p in pairs}
+    assert len(set(precs.values())) > 1   # the norms scale the pair precisions apart
+    OP = oracle.poisson(omra, prec)   # one operator at the SCF precision, applied at each pair's
+    for (i, j), V in K.pair_potentials.items():
+        p = precs[(i, j)]
+        rho = oracle.multiply(p, oracle.Tree(omra), 1.0, otrees[i], otrees[j])
+        ov = oracle.Tree(omra)
+        oracle.apply(p, ov, OP, rho)
+        _same_tree(V, ov, 1e-11)
