@@ -23,7 +23,13 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK = {"f64": 78.6e12, "f32": 157.3e12}
+PEAK = {"f64": 78.6e12, "f32": 157.3e12, "f16": 2.5e15}
+
+
+def mixed_dtype(k):
+    """Arithmetic of the mixed path's slice kernel: split-FP16 operands on f16 MFMA at k = 7
+    (FP32 sums), FP32 MFMA at k = 9."""
+    return "f16" if k == 7 else "f32"
 
 
 def log(*a):
@@ -218,10 +224,11 @@ def exchange_main(args, G, configs):
         "metric": "output nodes/s for Poisson/Helmholtz apply at prec 1e-6 vs plain FP64 GPU",
         "value": r[f"{best}_nodes_per_s"], "unit": "output nodes/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": r[f"{best}_ms_per_step"], "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32" if best == "mixed" else "f64",
+        "scaling": "weak", "vs_baseline": None, "dtype": mixed_dtype(7) if best == "mixed" else "f64",
         "data": "synthetic (seeded Gaussian orbitals)", "config": r,
-        "roofline": dict(bound="mfma", achieved=r["mixed_contraction_tflops"], peak=PEAK["f32"] / 1e12,
-                         unit="TFLOP/s", frac=(r["mixed_contraction_tflops"] or 0) * 1e12 / PEAK["f32"],
+        "roofline": dict(bound="mfma", achieved=r["mixed_contraction_tflops"],
+                         peak=PEAK[mixed_dtype(7)] / 1e12, unit="TFLOP/s",
+                         frac=(r["mixed_contraction_tflops"] or 0) * 1e12 / PEAK[mixed_dtype(7)],
                          traffic=None),
         "cpu_baseline": None,
     }
@@ -373,12 +380,12 @@ def scf_main(args, G, configs):
         cands.append(("predict", r["predict_nodes_per_s"], r["predict_rel_l2_err_vs_fp64"]))
     best = max((c for c in cands if c[2] < args.prec), key=lambda c: c[1])
     tfl = r["fp64_contraction_tflops"] if best[0] == "fp64" else r["mixed_contraction_tflops"]
-    peak = PEAK["f64"] if best[0] == "fp64" else PEAK["f32"]
+    peak = PEAK["f64"] if best[0] == "fp64" else PEAK[mixed_dtype(9)]
     line = {
         "metric": "output nodes/s for Poisson/Helmholtz apply at prec 1e-6 vs plain FP64 GPU",
         "value": best[1], "unit": "output nodes/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": r[f"{best[0]}_ms_per_step"], "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if best[0] == "fp64" else "f32",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if best[0] == "fp64" else mixed_dtype(9),
         "data": "synthetic (seeded Gaussian orbitals)", "config": {k: v for k, v in r.items() if k != "cpu_baseline"},
         "roofline": {"bound": "mfma", "achieved": tfl, "peak": peak / 1e12, "unit": "TFLOP/s",
                      "frac": (tfl or 0) * 1e12 / peak, "traffic": None},
@@ -455,7 +462,7 @@ def main():
         log(f"[bench] mixed: {res32['ms_per_step']:.2f} ms/step, {res32['nodes_per_s']:.1f} nodes/s, "
             f"contraction {1e3 * res32['t_contract_per_step']:.2f} ms/step, rel L2 err vs fp64 {err:.2e}")
         if res32["nodes_per_s"] > res64["nodes_per_s"] and err < args.prec:
-            best, dtype = res32, "f32"
+            best, dtype = res32, mixed_dtype(args.k)
     resP = None
     if not args.no_predict:
         pred, ntrain, t_train = train_predictor(G, configs, mra, args)
@@ -482,7 +489,7 @@ def main():
             f"iters {resP['iters']} (fp64 {res64['iters']}), threshold {pthr}, rel L2 err vs fp64 {errP:.2e}, "
             f"trained on {ntrain} trace samples in {t_train:.1f} s")
         if resP["nodes_per_s"] > best["nodes_per_s"] and errP < args.prec:
-            best, dtype = resP, ("f32" if mixed_ok else "f64")
+            best, dtype = resP, (mixed_dtype(args.k) if mixed_ok else "f64")
 
     # algorithmic flops: the plain FP64 method's surviving tuples x 6 (k+1)^4 (the same work for
     # every variant); issued flops count what the variant's kernels actually execute
@@ -491,7 +498,7 @@ def main():
     alg = F * 6.0 * K ** 4
     tc = best["t_contract_per_step"]
     achieved = alg / tc if tc > 0 else 0.0
-    kern = ("k_contract_mfma8" if args.k == 7 else f"k_contract_mfma{K}") if dtype == "f32" else "k_contract_fp64"
+    kern = f"k_contract_mfma{K}" if dtype != "f64" else "k_contract_fp64"
     per_launch = measured_traffic(kern, f"helmholtz{args.orbitals}_k{args.k}_prec{prec_tag(args.prec)}")
     # algorithmic bytes of a step: every input and output node block read or written once (FP64)
     in_nodes = sum(t.getNNodes() for t in trees)
@@ -499,7 +506,7 @@ def main():
     roofline = dict(bound="mfma", achieved=achieved / 1e12, peak=PEAK[dtype] / 1e12, unit="TFLOP/s",
                     frac=achieved / PEAK[dtype],
                     traffic=per_launch,
-                    kernel=(kern + " + k_contract_fp64 + k_reduce_partials") if dtype == "f32"
+                    kernel=(kern + " + k_contract_fp64 + k_reduce_partials") if dtype != "f64"
                     else "k_contract_fp64 + k_reduce_partials",
                     plain_tuples_per_step=F, alg_flops_per_tuple=6.0 * K ** 4,
                     issued_tflops=(best["flops_per_step"] / tc / 1e12) if tc > 0 else None,

@@ -47,9 +47,10 @@ typedef struct mwg_predictor_s *mwg_predictor;
 
 /* apply flags */
 #define MWG_APPLY_FP64 0u     /* plain method: FP64 contractions, norm-test refinement */
-#define MWG_APPLY_MIXED 1u    /* FP32 MFMA slice contractions (k = 7: FP32 sums over <= 16 slices,
-                                 FP64 beyond; k = 9: FP64 sums); k = 7 and k = 9 only (MWG_ERR_ARG
-                                 at other orders) */
+#define MWG_APPLY_MIXED 1u    /* MFMA slice contractions: k = 7 split-FP16 operands (hi + lo, 22
+                                 significand bits) on f16 MFMA, FP32 sums over <= 16 slices, FP64
+                                 beyond; k = 9 FP32 MFMA, FP64 sums.  Slices with large bounds run as
+                                 FP64 tuples.  k = 7 and k = 9 only (MWG_ERR_ARG at other orders) */
 #define MWG_APPLY_PREDICT 2u  /* learned refinement predictor (norm test stays the judge; the output
                                  tree can differ from the plain one, see mwg_predictor_create) */
 
@@ -64,13 +65,14 @@ typedef struct mwg_apply_stats {
                                 slices with large bounds to FP64 tuples) */
     int64_t gen_nodes;       /* generated (scaling-only) input nodes of this apply */
     int64_t iters;           /* refinement iterations */
-    int64_t slices;          /* FP32 MFMA slices (g, f, ft, term: all 8 gt at once), mixed mode */
+    int64_t slices;          /* MFMA slices (g, f, ft, term: all 8 gt at once), mixed mode */
     int64_t predicted_splits;/* output nodes computed ahead of their parent's norm test (predictor) */
     int64_t predicted_wasted;/* of which discarded because the norm test did not split the parent */
     double t_apply;          /* seconds: work-list build + contraction + refine + transforms */
     double t_contract;       /* seconds spent in contraction kernels (HIP events) */
     double flops;            /* flops issued: FP64 tuples * 6 (k+1)^4 + MFMA flops of the slices
-                                (k = 7: 2*16*8*(64+128+256) per slice; k = 9 padded: 2*32*12*(112+208+400)) */
+                                (k = 7: 28 f16 16x16x32 MFMAs = 2*16*16*32*28 per slice, a skipped z half
+                                counted; k = 9 padded FP32: 2*32*12*(112+208+400)) */
 } mwg_apply_stats;
 
 /* ---- context (mrenv / device) ------------------------------------------------ */
@@ -81,11 +83,11 @@ int mwg_ctx_destroy(mwg_ctx ctx);
 const char *mwg_last_error(mwg_ctx ctx);
 int mwg_sync(mwg_ctx ctx);
 /* contraction task granularity (no reference counterpart: MRCPP threads over output
- * nodes with OpenMP): tuples per FP64 wave task (1..4096) and FP32 MFMA slices per
+ * nodes with OpenMP): tuples per FP64 wave task (1..4096) and MFMA slices per
  * workgroup task (0..4096, 0 = automatic; k = 7 uses at most 64, k = 9 sizes automatic
  * tasks from each refinement iteration's slice total).  Results agree to FP64 rounding
- * for any setting on the FP64 path and at k = 9, and to the FP32 rounding of the
- * FP32-routed slices at k = 7; mrchem_amd.autotune picks the fastest on a sample
+ * for any setting on the FP64 path and at k = 9, and to the rounding of the MFMA
+ * slices' FP32 sums at k = 7; mrchem_amd.autotune picks the fastest on a sample
  * workload. */
 int mwg_ctx_set_tuning(mwg_ctx ctx, int fp64_chunk, int slice_chunk);
 int mwg_ctx_get_tuning(mwg_ctx ctx, int *fp64_chunk, int *slice_chunk);

@@ -25,7 +25,9 @@ def hipcc() -> str:
 def build(verbose: bool = False) -> str:
     objs = []
     flags = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function",
-             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             # MFMA results in ordinary VGPRs (no accumulation-register copies)
+             "-mllvm", "-amdgpu-mfma-vgpr-form"]
     build_dir = os.path.join(HERE, "_build")
     os.makedirs(build_dir, exist_ok=True)
     for src in SOURCES:

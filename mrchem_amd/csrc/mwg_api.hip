@@ -258,6 +258,8 @@ struct Ctx {
     DBuf<uint64_t> genKeys;
     DBuf<int> genJob, genCount, genOverflow;
     DBuf<double> genPool, genNorm;
+    DBuf<uint4> genPool16;  // split-FP16 copy of the generated nodes (mixed precision, k = 7)
+    DBuf<int> genExp16;
     DBuf<unsigned long long> genHashKeys;
     DBuf<int> genHashVals;
     DBuf<TransformItem> titems;
@@ -392,10 +394,16 @@ struct TreeH {
     DBuf<int> hvals;
     uint32_t hmask = 0;
     bool hvalid = false;
+    // split-FP16 copy of the coefficients for the mixed-precision apply at k = 7, made when
+    // the tree is first used as an input and kept until its coefficients change (ver)
+    DBuf<uint4> coef16;
+    DBuf<int> exp16;
+    uint64_t ver = 1, ver16 = 0;
 
     int K3() const { return mra->B.K3; }
     int n() const { return (int)depth.size(); }
     void clear() {
+        ver++;
         depth.clear();
         parent.clear();
         child.clear();
@@ -480,6 +488,15 @@ struct TreeH {
         hmask = index.mask();
         hvalid = true;
     }
+    void ensure16() {
+        if (ver16 == ver) return;
+        const int K = mra->B.K;
+        const size_t nb = (size_t)n() * 8;
+        coef16.alloc(nb * K * K * 2);
+        exp16.alloc(nb);
+        HC(launch_split16(mra->ctx->stream, (int)nb, 0, nullptr, K3(), K, coef.p, coef16.p, exp16.p));
+        ver16 = ver;
+    }
     DHash dhash() const { return DHash{hkeys.p, hvals.p, hmask}; }
     void refresh_norms() {  // device norms -> host, square norm
         hipStream_t s = mra->ctx->stream;
@@ -521,8 +538,11 @@ struct LevelH {
     DBuf<double> blocks, norms;
     DBuf<LevTerm> dterms;  // per-term screening data for the device box builder
     DBuf<double> denv;     // concatenated norm envelopes of the active terms
-    DBuf<float> blocks32;  // FP32 copy for the mixed-precision contraction (made on first use)
+    DBuf<float> blocks32;  // FP32 copy for the mixed-precision contraction at k = 9 (made on first use)
     bool has32 = false;
+    DBuf<uint4> blocks16;  // split-FP16 copy (rows [hi | lo], exponent per block) at k = 7
+    DBuf<int> exp16;
+    bool has16 = false;
 };
 
 struct OperH {
@@ -977,7 +997,7 @@ static void for_each_job(size_t n, F f) {
 // FP32 MFMA flops one slice issues: K = 8 three GEMMs of 16 x 8 over 64, 128 and 256
 // columns; K = 10 the padded tiles (2 x 16 rows, chunks of 4) over 112, 208 and 400 columns
 static double slice_flops(int K) {
-    if (K == 8) return 2.0 * 16 * 8 * (64 + 128 + 256);
+    if (K == 8) return 2.0 * 16 * 16 * 32 * 28;  // split-FP16: 4 + 8 + 16 MFMAs of 16x16x32
     if (K == 10) return 2.0 * 32 * 12 * (112 + 208 + 400);
     return 0.0;
 }
@@ -1111,7 +1131,20 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                 T.inNorm = J.in->norm.p;
                 LevelH &LV = *J.op->lev[d];
                 T.operBlocks = LV.blocks.p;
-                if (mixed) {
+                if (mixed && K == 8) {
+                    if (!LV.has16) {
+                        const size_t nb = LV.blocks.n / ((size_t)K * K);
+                        LV.blocks16.alloc(nb * K * 2);
+                        LV.exp16.alloc(nb);
+                        HC(launch_split16(s, (int)nb, 0, nullptr, K * K, K, LV.blocks.p, LV.blocks16.p, LV.exp16.p));
+                        LV.has16 = true;
+                    }
+                    J.in->ensure16();
+                    T.operBlocks16 = LV.blocks16.p;
+                    T.operExp16 = LV.exp16.p;
+                    T.inCoef16 = J.in->coef16.p;
+                    T.inExp16 = J.in->exp16.p;
+                } else if (mixed) {
                     if (!LV.has32) {
                         LV.blocks32.alloc(LV.blocks.n);
                         HC(launch_to_f32(s, LV.blocks.n, LV.blocks.p, LV.blocks32.p));
@@ -1192,6 +1225,10 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             if (genPoolCap < want) {
                 C.genPool.grow_keep((size_t)want * K3, s);
                 C.genNorm.grow_keep((size_t)want, s);
+                if (mixed && K == 8) {
+                    C.genPool16.grow_keep((size_t)want * K * K * 2, s);
+                    C.genExp16.grow_keep((size_t)want, s);
+                }
                 genPoolCap = want;
             }
         }
@@ -1199,6 +1236,9 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                     genPoolCap, dtab};
         HC(launch_gen_mark(s, nw, dwork, djobs, C.tabs.p, gs));
         HC(launch_gen_compute(s, genDone, genPoolCap, djobs, gs, M.W.p, K));
+        if (mixed && K == 8)
+            HC(launch_split16(s, genPoolCap - genDone, genDone, C.genCount.p, K3, K, C.genPool.p, C.genPool16.p,
+                              C.genExp16.p));
         PT.lap("upload+genmark");
         // Work list and contraction without a host round trip: counts per segment
         // (nseg per output node), a device-built schedule (offsets, tasks, reduce items),
@@ -1278,7 +1318,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                 const int tpb = mfma_tasks_per_block(K);
                 const int gridmf = std::max(1, (C.caps.stasks + tpb - 1) / tpb);
                 HC(launch_contract_mfma(s, K, 0, dsum, gridmf, C.stasks.p, dwork, djobs, C.tabs.p,
-                                        C.slices.p, C.genPool.p, C.nodeslots.p, dtab));
+                                        C.slices.p, C.genPool.p, C.nodeslots.p, dtab, C.genPool16.p, C.genExp16.p));
             }
             // (mixed: the reduce items cover every output component and write the norms)
             HC(launch_reduce_partials(s, 0, dsum, gridred, C.reds.p, dwork, djobs, C.partials.p,
@@ -1322,6 +1362,12 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                 gs.norm = C.genNorm.p;
                 gs.pcap = want;
                 HC(launch_gen_compute(s, genPoolCap, want, djobs, gs, M.W.p, K));
+                if (mixed && K == 8) {
+                    C.genPool16.grow_keep((size_t)want * K * K * 2, s);
+                    C.genExp16.grow_keep((size_t)want, s);
+                    HC(launch_split16(s, want - genPoolCap, genPoolCap, C.genCount.p, K3, K, C.genPool.p,
+                                      C.genPool16.p, C.genExp16.p));
+                }
                 genPoolCap = want;
                 launch_count();
             }
@@ -2056,6 +2102,7 @@ int mwg_tree_rescale(mwg_tree tree, double c) {
     Ctx *C = tree->mra->ctx;
     return guarded(C, [&] {
         TreeH &T = *tree;
+        T.ver++;
         HC(launch_scale(C->stream, (size_t)T.n() * 8 * T.K3(), T.coef.p, c));
         HC(launch_scale(C->stream, (size_t)T.n() * 8, T.norm.p, std::fabs(c)));
         for (double &v : T.cnorm) v *= std::fabs(c);

@@ -38,7 +38,13 @@ struct JobTab {
     const double *inCoef;
     const double *inNorm;
     const double *operBlocks;  // this depth's block array of the job's operator
-    const float *operBlocks32; // FP32 copy (mixed precision)
+    const float *operBlocks32; // FP32 copy (mixed precision, k = 9)
+    // split-FP16 copies (mixed precision, k = 7): rows [hi K | lo K] of each K x K operator
+    // block and of each K^3 input component (K^2 rows), a power-of-two exponent per block
+    const uint4 *operBlocks16;
+    const int *operExp16;
+    const uint4 *inCoef16;
+    const int *inExp16;
     const double *operNorms;   // this depth's norm array
     double *outCoef;
     double *outNorm;
@@ -184,14 +190,21 @@ hipError_t launch_enum_mixed(hipStream_t s, int mode, int K, int nw, int nparts,
                              const TermTab *tabs, const GenState &gs, int *thread_off, int *seg_total,
                              const long long *seg_ofs, Tuple *out, Slice *sout, const TaskSummary *sum,
                              const SegRec *segs, const long long *gslice);
-// FP32 MFMA slice contraction (K = 8 or 10; mwg_mfma.hip).  A task is a run of slices of
-// one output node; mfma_tasks_per_block(K) tasks share a 256-thread block.  Each task adds
-// the surviving tuples of its slices to tabcnt[4 t + 2] of its job table t.
+// Split-FP16 copies of FP64 blocks (mwg_mfma.hip): block b (values src[b bvals ..]) as rows
+// of K values, each row [hi K | lo K] in f16 (2 K halves), scaled by 2^exp[b] so that the
+// block's largest magnitude lies in [0.5, 1) (exp 0 for a zero block); x 2^e = hi + lo up
+// to about 2^-22 of it.  With count, only blocks first .. min(*count, first + n) - 1.
+hipError_t launch_split16(hipStream_t s, int n, int first, const int *count, int bvals, int K, const double *src,
+                          uint4 *dst, int *exp);
+// FP32 / split-FP16 MFMA slice contraction (K = 8 or 10; mwg_mfma.hip).  A task is a run of
+// slices of one output node; mfma_tasks_per_block(K) tasks share a 256-thread block.  Each
+// task adds the surviving tuples of its slices to tabcnt[4 t + 2] of its job table t.
+// genPool16 / genExp16: the generated nodes' split-FP16 copies (K = 8).
 int mfma_tasks_per_block(int K);
 hipError_t launch_contract_mfma(hipStream_t s, int K, int ntask, const TaskSummary *sum, int grid,
                                 const SliceTask *tasks, const WorkItem *work, const JobTab *jobs,
                                 const TermTab *tabs, const Slice *slices, const double *genPool, double *nodeslots,
-                                long long *tabcnt);
+                                long long *tabcnt, const uint4 *genPool16, const int *genExp16);
 struct MulItem {
     int node;          // output node
     int depth;

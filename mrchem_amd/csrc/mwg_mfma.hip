@@ -1,20 +1,20 @@
-// FP32 MFMA slice contractions of the mixed-precision apply (gfx950, wave64).
+// MFMA slice contractions of the mixed-precision apply (gfx950, wave64).
 //
 // A slice (g, f, ft, term) produces all 8 output components gt of node g from input
 // component ft of node f: out_gt[c][b][a] += w * sum Z[c][l] Y[b][j] X[a][i] f[l][j][i],
 // with X = X_{gt_x, ft_x} etc.  Per dimension the two output choices gt_d = 0, 1 are
 // stacked into one 2K x K operator X' (row K gt_d + a), so a slice is three GEMMs.
-// Products and the sums inside a slice are FP32 (v_mfma_f32_16x16x4_f32).  The slice
-// result, scaled by the term weight and masked to the components whose tuples survive
-// the screening, is summed over the task's slices (k = 7: FP32 over at most 16 slices
-// per wave, then FP64 over the waves; k = 9: FP64 per slice), and a task's sum goes
+// k = 7: operands split into two FP16 halves each, products exact and sums FP32
+// (v_mfma_f32_16x16x32_f16); k = 9: FP32 operands and sums (v_mfma_f32_16x16x4_f32).  The
+// slice result, scaled by the term weight and masked to the components whose tuples
+// survive the screening, is summed over the task's slices (k = 7: FP32 over at most 16
+// slices per wave, then FP64 over the waves; k = 9: FP64 per slice), and a task's sum goes
 // to one node slot (8 K^3 FP64) that k_reduce_partials adds, in slot order, to the
 // node's FP64 tuple partials.  A task is a run of slices of one output node, one task
 // per 256-thread workgroup.
 //
-// Instruction layout (v_mfma_f32_16x16x4_f32, D = A B + C): lane L supplies
-// A[L % 16][L / 16] and B[L / 16][L % 16]; D[i][j] is register i % 4 of lane
-// j + 16 (i / 4).
+// Instruction layout of v_mfma_f32_16x16x4_f32 (D = A B + C): lane L supplies
+// A[L % 16][L / 16] and B[L / 16][L % 16]; D[i][j] is register i % 4 of lane j + 16 (i / 4).
 #include <hip/hip_ext.h>
 
 #include "mwg_kernels.h"
@@ -63,225 +63,270 @@ __device__ inline floatx4 mfma0(float a, float b) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
 }
 
+// ------------------------------------------------------------------ split-FP16 copies
+//
+// x 2^e = hi + lo: hi keeps the leading 11 significand bits of x 2^e (exact in f16 in its
+// normal range), lo = x 2^e - hi is exact in FP32 and rounded to nearest in f16, so hi + lo
+// carries about 22 bits.  Below the f16 normal range (2^-14 of the block's largest
+// magnitude, which is scaled into [0.5, 1)) both parts round to the f16 subnormal grid,
+// an absolute error of at most 2^-25 of the block's scale; v_mfma_f32_16x16x32_f16 keeps
+// subnormal operands and forms every product exactly in FP32.
+
+typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
+typedef _Float16 halfx2 __attribute__((ext_vector_type(2)));
+
+__device__ inline unsigned pack_rn(float a, float b) {
+    const halfx2 h = {(_Float16)a, (_Float16)b};
+    return __builtin_bit_cast(unsigned, h);
+}
+// 2 K values (times the power of two sc) -> hi and lo halves (packed pairs)
+template <int N>
+__device__ inline void split_pairs(const float (&v)[N], float sc, unsigned (&hi)[N / 2], unsigned (&lo)[N / 2]) {
+#pragma unroll
+    for (int j = 0; j < N / 2; j++) {
+        const float x0 = v[2 * j] * sc, x1 = v[2 * j + 1] * sc;
+        const float a = __uint_as_float(__float_as_uint(x0) & 0xFFFFE000u);
+        const float b = __uint_as_float(__float_as_uint(x1) & 0xFFFFE000u);
+        hi[j] = pack_rn(a, b);
+        lo[j] = pack_rn(x0 - a, x1 - b);
+    }
+}
+
+// largest value over the wave: across the lane halves and rows by permlane swaps (each
+// returns the pair of values that meet), then within each row of 16 by DPP rotations
+__device__ inline float wave_max(float v) {
+    const auto a = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    v = fmaxf(__uint_as_float(a[0]), __uint_as_float(a[1]));
+    const auto b = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    v = fmaxf(__uint_as_float(b[0]), __uint_as_float(b[1]));
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x128, 0xf, 0xf, false)));  // row_ror:8
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x124, 0xf, 0xf, false)));  // row_ror:4
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x122, 0xf, 0xf, false)));  // row_ror:2
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x121, 0xf, 0xf, false)));  // row_ror:1
+    return v;
+}
+// s with (largest magnitude) 2^s in [0.5, 1); 0 for an all-zero tensor
+__device__ inline int scale_exp(float mx) { return mx > 0.f ? -(ilogbf(mx) + 1) : 0; }
+
+// one wave per block; rows of K = 8 values (16 halves: hi | lo = two uint4)
+__global__ void __launch_bounds__(256) k_split16(int n, int first, const int *count, int bvals, const double *src,
+                                                 uint4 *dst, int *exp) {
+    const int b = first + (int)blockIdx.x * 4 + wave_index(), lane = threadIdx.x & 63;
+    const int end = count ? min(*(volatile const int *)count, first + n) : first + n;
+    if (b >= end) return;
+    const double *v = src + (size_t)b * bvals;
+    double mx = 0.0;
+    for (int e = lane; e < bvals; e += 64) mx = fmax(mx, fabs(v[e]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+    const int ex = mx > 0.0 ? -(ilogb(mx) + 1) : 0;
+    if (lane == 0) exp[b] = ex;
+    const int rows = bvals / 8;
+    for (int r = lane; r < rows; r += 64) {
+        float x[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) x[j] = (float)ldexp(v[r * 8 + j], ex);
+        unsigned hi[4], lo[4];
+        split_pairs<8>(x, 1.f, hi, lo);
+        dst[((size_t)b * rows + r) * 2] = make_uint4(hi[0], hi[1], hi[2], hi[3]);
+        dst[((size_t)b * rows + r) * 2 + 1] = make_uint4(lo[0], lo[1], lo[2], lo[3]);
+    }
+}
+
 // ------------------------------------------------------------------ K = 8 (k = 7)
 //
-// One workgroup of 4 waves per task = a chunk of the slices of one output node, the
-// waves taking every fourth slice.  A slice
-// (f, ft, term, delta) produces all 8 output components with three GEMMs on
-// v_mfma_f32_16x16x4_f32, M = 16 = (gt_d, a) stacked [X_{0,ft_d}; X_{1,ft_d}],
-// K = 8, N = 64, 128, 256 (the dimensions already contracted).  Lane layout of
-// the instruction: A[i][k] from lane i%16 + 16 k, B[k][j] from lane j%16 + 16 k,
-// D[4 (lane/16) + r][lane%16] in register r.  Intermediates rotate through LDS
-// (rows padded to 68 and 132 floats: conflict-free for the access patterns).
-// Each slice result is scaled by the term weight (zero for components the slice
-// does not reach) and summed in FP32 over the wave's slices (at most
-// MAX_SLICE_CHUNK), then in FP64 over the waves and over the node's tasks.
-// operands of one slice in registers: A fragments of the three passes and the
-// pass-x B fragments of f, as loaded (FP64: converted to FP32 only when the slice
-// is contracted, so that the loads for a later slice stay in flight meanwhile)
-struct SliceRaw {
-    float ax[2], ay[2], az[2];
-    double fb[4][2];
+// Split-FP16 slice contraction.  One v_mfma_f32_16x16x32_f16 per 16 x 16 output tile holds
+// a whole K = 8 contraction with both operands split: its K = 32 is four groups of 8 (the
+// lane group kg = lane / 16 of each operand), (A_hi, B_hi), (A_hi, B_lo), (A_lo, B_hi),
+// (A_lo, B_lo).  Lane L supplies A[L % 16][8 kg .. 8 kg + 7] and B[8 kg .. 8 kg + 7][L % 16]
+// as 8 halves: an A-role fragment is the hi row for kg 0, 1 and the lo row for kg 2, 3, a
+// B-role fragment hi for kg 0, 2 and lo for kg 1, 3.  D[i][j] is register i % 4 of lane
+// j + 16 (i / 4).
+//  pass x  T1 = f X'^T: A = f (tile (yh, zq): row i holds y = 4 yh + i % 4, z = 4 zq + i / 4),
+//          B = X' (rows a' = 8 gt_x + a).  Lane a' + 16 q then holds y = 0..7 of the columns
+//          (z = 4 zq + q, a') across the tiles yh = 0, 1.
+//  pass y  T2[z][b'][a'] = Y' T1: A = Y' (rows b' = 8 gt_y + b), B from LDS (per column
+//          (z, a') the row [hi 8 | lo 8] of its y values, written by the lane that holds them).
+//          Lane a' + 16 q then holds z = 0..7 of the columns (b' = 4 q + r, a').
+//  pass z  out[c'][b'][a'] = Z' T2: A = Z', B from LDS rows [hi 8 | lo 8] per column (b', a').
+// Output element (c' = 4 q + r, b', a' = lane % 16) of N-tile b': gt = (a' >> 3) |
+// (b' >> 3) << 1 | (q >> 1) << 2.  The operators and inputs carry power-of-two exponents
+// per block (x 2^e = hi + lo): the weight of output component gt absorbs 2^-(e_x(gt_x) +
+// e_y(gt_y) + e_z(gt_z) + e_f), and that of the intermediates T1 and T2, each scaled per
+// slice by 2^s1, 2^s2 to a largest magnitude in [0.5, 1) before it is split, 2^-(s1 + s2).
+// Each wave sums its slices (every fourth of the task) in FP32, then the four wave sums are
+// added in FP64 in LDS in a fixed order.
+__device__ inline floatx4 mfmah(uint4 a, uint4 b) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(halfx8, a), __builtin_bit_cast(halfx8, b),
+                                                 floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+}
+
+struct OpsH8 {
+    uint4 bx, ay, az;  // X' (B role), Y', Z' (A role) fragments of this lane
+    uint4 f[4];        // f (A role) per tile (yh, zq)
+    int exz;           // exponents of this lane's X' block (gt_x) + Z' block (gt_z) + f
+    int ey0, ey1;      // the two Y' blocks
     double w;
     int mask;
 };
 
-struct SliceOps {
-    float ax[2], ay[2], az[2];
-    float fb[4][2];
-    double w;
-    int mask;
-};
-
-__device__ inline SliceRaw load_slice_raw(const Slice &sl, const float *B32, const double *inCoef,
-                                          const double *genPool, int m, int q) {
-    constexpr int K = 8, K3 = 512;
-    const int gxm = m >> 3, am = m & 7;
-    SliceRaw o;
-    const gflt *px = (const gflt *)B32 + sl.xo + gxm * 128 + am * K;
-    const gflt *py = (const gflt *)B32 + sl.yo + gxm * 128 + am * K;
-    const gflt *pz = (const gflt *)B32 + sl.zo + gxm * 128 + am * K;
-#pragma unroll
-    for (int s2 = 0; s2 < 2; s2++) {
-        o.ax[s2] = px[4 * s2 + q];
-        o.ay[s2] = py[4 * s2 + q];
-        o.az[s2] = pz[4 * s2 + q];
+__device__ inline OpsH8 load_opsh8(const Slice &sl, const JobTab &J, const uint4 *gen16, const int *genExp,
+                                   int m, int q) {
+    OpsH8 o;
+    const int bxb = sl.xo / 64, byb = sl.yo / 64, bzb = sl.zo / 64;  // block indices (gt_d = 0)
+    const int rx = (m >> 3) * 2, row = m & 7;
+    o.bx = J.operBlocks16[((bxb + rx) * 8 + row) * 2 + (q & 1)];
+    o.ay = J.operBlocks16[((byb + rx) * 8 + row) * 2 + (q >> 1)];
+    o.az = J.operBlocks16[((bzb + rx) * 8 + row) * 2 + (q >> 1)];
+    const uint4 *fb;
+    int ef;
+    if (sl.f >= 0) {
+        fb = J.inCoef16 + ((size_t)sl.f * 8 + sl.ft) * 128;
+        ef = J.inExp16[(size_t)sl.f * 8 + sl.ft];
+    } else {
+        fb = gen16 + (size_t)(-sl.f - 1) * 128;
+        ef = genExp[-sl.f - 1];
     }
-    const gdbl *f = sl.f >= 0 ? (const gdbl *)inCoef + (size_t)sl.f * 8 * K3 + sl.ft * K3
-                                 : (const gdbl *)genPool + (size_t)(-sl.f - 1) * K3;
 #pragma unroll
-    for (int t = 0; t < 4; t++)
-#pragma unroll
-        for (int s2 = 0; s2 < 2; s2++) o.fb[t][s2] = f[(16 * t + m) * K + 4 * s2 + q];
+    for (int tl = 0; tl < 4; tl++) {
+        const int y = 4 * (tl & 1) + (m & 3), z = 4 * (tl >> 1) + (m >> 2);
+        o.f[tl] = fb[(z * 8 + y) * 2 + (q >> 1)];
+    }
+    o.exz = J.operExp16[bxb + rx] + J.operExp16[bzb + 2 * (q >> 1)] + ef;
+    o.ey0 = J.operExp16[byb];
+    o.ey1 = J.operExp16[byb + 2];
     o.w = sl.w;
     o.mask = sl.mask;
     return o;
 }
 
-__device__ inline SliceOps slice_ops(const SliceRaw &r) {
-    SliceOps o;
-#pragma unroll
-    for (int s2 = 0; s2 < 2; s2++) {
-        o.ax[s2] = r.ax[s2];
-        o.ay[s2] = r.ay[s2];
-        o.az[s2] = r.az[s2];
-    }
-#pragma unroll
-    for (int t = 0; t < 4; t++)
-#pragma unroll
-        for (int s2 = 0; s2 < 2; s2++) o.fb[t][s2] = (float)r.fb[t][s2];
-    o.w = r.w;
-    o.mask = r.mask;
-    return o;
-}
-
-__device__ inline void mfma8_task(const SliceTask task, const WorkItem *work, const JobTab *jobs, const Slice *slices,
-                                  const double *genPool, double *nodeslots, float *lds, long long *tabcnt) {
-    constexpr int K = 8, K3 = 512;
-    constexpr int S1 = 68, S2 = 132;
-    const int wave = wave_index(), lane = threadIdx.x & 63;
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) k_contract_mfma8(int ntask_arg, const TaskSummary *sum,
+                                                        const SliceTask *tasks, const WorkItem *work,
+                                                        const JobTab *jobs, const Slice *slices,
+                                                        const uint4 *gen16, const int *genExp, double *nodeslots,
+                                                        long long *tabcnt) {
+    constexpr int K3 = 512;
+    // per wave: L1 (8 z x 16 a' rows, 4 KB) | L2 (16 b' x 16 a' rows, 8 KB); reused at the end
+    // for the FP64 sum over the waves (32 KB)
+    __shared__ __align__(16) uint4 lds[4 * 768];
+    const int ntask = sum ? (sum->overflow ? 0 : sum->nstasks) : ntask_arg;
+    if ((int)blockIdx.x >= ntask) return;
+    const SliceTask task = tasks[blockIdx.x];
+    const int wave = wave_index(), lane = threadIdx.x & 63, m = lane & 15, q = lane >> 4;
+    uint4 *L1 = lds + wave * 768, *L2 = L1 + 256;
+    const int gx = m >> 3, gz = q >> 1;
     const WorkItem wi = work[task.g];
     const JobTab &J = jobs[wi.job];
-    float *L1 = lds + wave * 16 * S1;
-    float *L2 = lds + 4 * 16 * S1 + wave * 16 * S2;
-    const int m = lane & 15, q = lane >> 4;
-    const int gxm = m >> 3, am = m & 7;
     float acc[16][4];
 #pragma unroll
     for (int t = 0; t < 16; t++)
 #pragma unroll
         for (int r = 0; r < 4; r++) acc[t][r] = 0.f;
-    // the four waves take every fourth slice of the task; the next slice's operands
-    // are loaded while the current one is contracted
-    // software pipeline over the wave's slices it = wave, wave + 4, ...: the record of
-    // slice it + 8 and the operands of slice it + 4 are loaded while slice it is contracted
-    // (indices past the end are clamped to the last slice: the loads stay unconditional,
-    // so that the waits for them fall where their data are used)
+    // the waves take every fourth slice; the next slice's operands and the record of the
+    // one after are loaded while a slice is contracted (indices clamped to the last slice)
     const Slice *srec = slices + task.start;
     const int last = task.count - 1;  // a task holds at least one slice
-    SliceRaw craw = load_slice_raw(slice_at(srec + min(wave, last)), J.operBlocks32, J.inCoef, genPool, m, q);
+    OpsH8 cur = load_opsh8(slice_at(srec + min(wave, last)), J, gen16, genExp, m, q);
     Slice rnext = slice_at(srec + min(wave + 4, last));
     int ntup = 0;
     for (int it = wave; it < task.count; it += 4) {
-        ntup += __popc((unsigned)craw.mask);
-        const SliceRaw nraw = load_slice_raw(rnext, J.operBlocks32, J.inCoef, genPool, m, q);
+        const OpsH8 nxt = load_opsh8(rnext, J, gen16, genExp, m, q);
         rnext = slice_at(srec + min(it + 8, last));
-        const SliceOps cur = slice_ops(craw);
-        // pass x: C1[(gx,a)][n = l*8 + j] = sum_i X[a][i] f[n*8 + i]
-        // (tiles in groups of 4 independent MFMA chains, operands read before the MFMAs)
-        {
-            floatx4 c[4];
+        ntup += __popc((unsigned)cur.mask);
+        floatx4 t1[4];
 #pragma unroll
-            for (int t = 0; t < 4; t++)
-                c[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.ax[0], cur.fb[t][0], floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        for (int tl = 0; tl < 4; tl++) t1[tl] = mfmah(cur.f[tl], cur.bx);
+        float m1 = 0.f;
 #pragma unroll
-            for (int t = 0; t < 4; t++) c[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.ax[1], cur.fb[t][1], c[t], 0, 0, 0);
+        for (int tl = 0; tl < 4; tl++)
 #pragma unroll
-            for (int t = 0; t < 4; t++)
+            for (int r = 0; r < 4; r++) m1 = fmaxf(m1, fabsf(t1[tl][r]));
+        const int s1 = scale_exp(wave_max(m1));
+        const float sc1 = ldexpf(1.f, s1);
 #pragma unroll
-                for (int r = 0; r < 4; r++) L1[(4 * q + r) * S1 + 16 * t + m] = c[t][r];
+        for (int zq = 0; zq < 2; zq++) {
+            float v[8];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                v[r] = t1[2 * zq][r];
+                v[4 + r] = t1[2 * zq + 1][r];
+            }
+            unsigned hi[4], lo[4];
+            split_pairs<8>(v, sc1, hi, lo);
+            const int z = 4 * zq + q;
+            L1[(z * 16 + m) * 2] = make_uint4(hi[0], hi[1], hi[2], hi[3]);
+            L1[(z * 16 + m) * 2 + 1] = make_uint4(lo[0], lo[1], lo[2], lo[3]);
         }
         wave_sync();
-        // pass y: C2[(gy,b)][n2 = l*16 + (gx,a)] = sum_j Y[b][j] C1[(gx,a)][l*8 + j]
+        floatx4 t2[8];
 #pragma unroll
-        for (int g4 = 0; g4 < 2; g4++) {
-            float b[4][2];
+        for (int z = 0; z < 8; z++) t2[z] = mfmah(cur.ay, L1[(z * 16 + m) * 2 + (q & 1)]);
+        float m2 = 0.f;
 #pragma unroll
-            for (int t = 0; t < 4; t++)
+        for (int z = 0; z < 8; z++)
 #pragma unroll
-                for (int s2 = 0; s2 < 2; s2++) b[t][s2] = L1[m * S1 + (4 * g4 + t) * 8 + 4 * s2 + q];
-            floatx4 c[4];
+            for (int r = 0; r < 4; r++) m2 = fmaxf(m2, fabsf(t2[z][r]));
+        const int s2 = scale_exp(wave_max(m2));
+        const float sc2 = ldexpf(1.f, s2);
 #pragma unroll
-            for (int t = 0; t < 4; t++)
-                c[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.ay[0], b[t][0], floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        for (int r = 0; r < 4; r++) {
+            float v[8];
 #pragma unroll
-            for (int t = 0; t < 4; t++) c[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.ay[1], b[t][1], c[t], 0, 0, 0);
-#pragma unroll
-            for (int t = 0; t < 4; t++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) L2[(4 * q + r) * S2 + 16 * (4 * g4 + t) + m] = c[t][r];
+            for (int z = 0; z < 8; z++) v[z] = t2[z][r];
+            unsigned hi[4], lo[4];
+            split_pairs<8>(v, sc2, hi, lo);
+            const int bp = 4 * q + r;
+            L2[(bp * 16 + m) * 2] = make_uint4(hi[0], hi[1], hi[2], hi[3]);
+            L2[(bp * 16 + m) * 2 + 1] = make_uint4(lo[0], lo[1], lo[2], lo[3]);
         }
         wave_sync();
-        // pass z: C3[(gz,c)][n3 = (gy,b)*16 + (gx,a)] = sum_l Z[c][l] C2[(gy,b)][l*16 + (gx,a)]
-        // Output component gt = gx | gy << 1 | gz << 2 with gx = m >> 3 (the lane), gy = t3 >> 3
-        // and gz = (4 q + r) >> 3 = q >> 1 (the lane): two weights per lane and slice, masked
-        // by the slice's surviving components.
-        const int gtl = gxm | ((q >> 1) << 2), gth = gtl | 2;
-        const float wlow = ((cur.mask >> gtl) & 1) ? (float)cur.w : 0.f;
-        const float whigh = ((cur.mask >> gth) & 1) ? (float)cur.w : 0.f;
-        // tiles t3 < 8 carry gy = 0 (gt in {0,1,4,5}: mask bits 0x33), t3 >= 8 gy = 1 (0xCC);
-        // a half without a surviving component is skipped (the mask is uniform per wave)
-        const bool need_lo = (cur.mask & 0x33) != 0, need_hi = (cur.mask & 0xCC) != 0;
+        // pass z: a half (gt_y) without a surviving component is skipped (uniform mask)
+        const int mk = cur.mask;
+        const float wlo = ((mk >> (gx | (gz << 2))) & 1) ? (float)ldexp(cur.w, -(cur.exz + cur.ey0 + s1 + s2)) : 0.f;
+        const float whi = ((mk >> (gx | 2 | (gz << 2))) & 1) ? (float)ldexp(cur.w, -(cur.exz + cur.ey1 + s1 + s2)) : 0.f;
 #pragma unroll
-        for (int g4 = 0; g4 < 4; g4++) {
-            if (g4 < 2 ? !need_lo : !need_hi) continue;
-            float b[4][2];
+        for (int h = 0; h < 2; h++) {
+            if (h == 0 ? !(mk & 0x33) : !(mk & 0xCC)) continue;
+            floatx4 d[8];
 #pragma unroll
-            for (int t = 0; t < 4; t++)
+            for (int t = 0; t < 8; t++) d[t] = mfmah(cur.az, L2[((8 * h + t) * 16 + m) * 2 + (q & 1)]);
+            const float ws = h == 0 ? wlo : whi;
 #pragma unroll
-                for (int s2 = 0; s2 < 2; s2++) b[t][s2] = L2[(4 * g4 + t) * S2 + (4 * s2 + q) * 16 + m];
-            floatx4 c[4];
+            for (int t = 0; t < 8; t++)
 #pragma unroll
-            for (int t = 0; t < 4; t++)
-                c[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.az[0], b[t][0], floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
-#pragma unroll
-            for (int t = 0; t < 4; t++) c[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.az[1], b[t][1], c[t], 0, 0, 0);
-            const float wm = g4 < 2 ? wlow : whigh;
-#pragma unroll
-            for (int t = 0; t < 4; t++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) acc[4 * g4 + t][r] = fmaf(wm, c[t][r], acc[4 * g4 + t][r]);
+                for (int r = 0; r < 4; r++) acc[8 * h + t][r] = fmaf(ws, d[t][r], acc[8 * h + t][r]);
         }
         wave_sync();
-        craw = nraw;
+        cur = nxt;
     }
     if (lane == 0) atomicAdd((unsigned long long *)&tabcnt[4 * wi.job + 2], (unsigned long long)ntup);
-    // FP64 sum over the waves in LDS, wave 0 first, then 1, 2, 3 (one 32 KB buffer)
+    // FP64 sum over the waves in LDS, wave 0 first, then 1, 2, 3
     double *R = (double *)lds;
     __syncthreads();
     for (int src = 0; src < 4; src++) {
         if (wave == src) {
 #pragma unroll
-            for (int t3 = 0; t3 < 16; t3++)
+            for (int t = 0; t < 16; t++)
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    double *e = &R[(t3 * 4 + r) * 64 + lane];
-                    *e = src == 0 ? (double)acc[t3][r] : *e + (double)acc[t3][r];
+                    double *e = &R[(t * 4 + r) * 64 + lane];
+                    *e = src == 0 ? (double)acc[t][r] : *e + (double)acc[t][r];
                 }
         }
         __syncthreads();
     }
-    {
-        double *dst = nodeslots + (size_t)task.slot * 8 * K3;
+    double *dst = nodeslots + (size_t)task.slot * 8 * K3;
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int t3 = 4 * wave + u;
-            const int gy = t3 >> 3, b = t3 & 7;
+    for (int u = 0; u < 4; u++) {
+        const int bt = 4 * wave + u;
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int m3 = 4 * q + r;
-                const int gz = m3 >> 3, cc = m3 & 7;
-                const int gt = gxm | (gy << 1) | (gz << 2);
-                dst[gt * K3 + (cc * K + b) * K + am] = R[(t3 * 4 + r) * 64 + lane];
-            }
+        for (int r = 0; r < 4; r++) {
+            const int gt = gx | ((bt >> 3) << 1) | (gz << 2);
+            const int c = 4 * (q & 1) + r, b = bt & 7, a = m & 7;
+            dst[gt * K3 + (c * 8 + b) * 8 + a] = R[(bt * 4 + r) * 64 + lane];
         }
     }
 }
-
-__global__ void __launch_bounds__(256)
-k_contract_mfma8(int ntask_arg, const TaskSummary *sum,
-                                                        const SliceTask *tasks, const WorkItem *work,
-                                                        const JobTab *jobs, const Slice *slices,
-                                                        const double *genPool, double *nodeslots,
-                                                        long long *tabcnt) {
-    constexpr int S1 = 68, S2 = 132;
-    // per-wave operand staging; reused for the cross-wave reduction at the end
-    __shared__ __align__(16) float lds[4 * 16 * S1 + 4 * 16 * S2];
-    const int ntask = sum ? (sum->overflow ? 0 : sum->nstasks) : ntask_arg;
-    if ((int)blockIdx.x >= ntask) return;
-    mfma8_task(tasks[blockIdx.x], work, jobs, slices, genPool, nodeslots, lds, tabcnt);
-}
-
 
 // ------------------------------------------------------------------ K = 10 (k = 9)
 //
@@ -522,18 +567,26 @@ int mfma_tasks_per_block(int K) {
 hipError_t launch_contract_mfma(hipStream_t s, int K, int ntask, const TaskSummary *sum, int grid,
                                 const SliceTask *tasks, const WorkItem *work, const JobTab *jobs,
                                 const TermTab *tabs, const Slice *slices, const double *genPool, double *nodeslots,
-                                long long *tabcnt) {
+                                long long *tabcnt, const uint4 *genPool16, const int *genExp16) {
     (void)tabs;
     if (K != 8 && K != 10) return hipErrorInvalidValue;
-    const int tpb = mfma_tasks_per_block(K);
-    const int nb = sum ? grid : (ntask + tpb - 1) / tpb;
+    const int nb = sum ? grid : ntask;
     if (nb <= 0) return hipSuccess;
     if (K == 8)
         hipLaunchKernelGGL(k_contract_mfma8, dim3(nb), dim3(256), 0, s, ntask, sum, tasks, work, jobs, slices,
-                           genPool, nodeslots, tabcnt);
+                           genPool16, genExp16, nodeslots, tabcnt);
     else
         hipLaunchKernelGGL(k_contract_mfma10, dim3(nb), dim3(256), 0, s, ntask, sum, tasks, work, jobs, slices,
                            genPool, nodeslots, tabcnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_split16(hipStream_t s, int n, int first, const int *count, int bvals, int K, const double *src,
+                          uint4 *dst, int *exp) {
+    if (K != 8) return hipErrorInvalidValue;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_split16, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, n, first, count, bvals, src, dst,
+                       exp);
     return hipGetLastError();
 }
 

@@ -65,9 +65,10 @@ def test_config2_helmholtz_k7_prec1e6_matches_oracle(G, oracle):
 
 @pytest.mark.parametrize("basis", ["L", "I"])
 def test_config2_mixed_and_predictor_within_prec(G, basis):
-    """Mixed (FP32 MFMA slices, FP64 accumulation) and mixed + learned predictor against
-    the plain FP64 apply of the same batch: exact L2 difference of every output far
-    below prec ||out|| (FP32 rounding only), the same tuples as the plain method."""
+    """Mixed (split-FP16 MFMA slices, FP32 sums, FP64 accumulation) and mixed + learned
+    predictor against the plain FP64 apply of the same batch: exact L2 difference of every
+    output below 0.1 prec ||out|| (the rounding of the FP32-routed slices only; a wrong
+    contraction is off by O(||out||)), the same tuples as the plain method."""
     prec = 1e-6
     orbs, nuc, mus = configs.orbitals(0, n_orb=5)
     mra = G.mra_from(configs.BOX20, 7, basis)
@@ -80,7 +81,7 @@ def test_config2_mixed_and_predictor_within_prec(G, basis):
     for a, b, sa, sb in zip(o32, o64, s32, s64):
         assert sa["slices"] > 0 and 0 < sa["tuples_fp64"] < sa["tuples"]
         assert sa["nodes_out"] == sb["nodes_out"] and sa["tuples"] == sb["tuples"]
-        assert l2_diff(G, a, b) < 1e-2 * prec * math.sqrt(b.getSquareNorm())
+        assert l2_diff(G, a, b) < 0.1 * prec * math.sqrt(b.getSquareNorm())
     # the predictor: trained on traces of other orbitals (seed 101), judged by the norm test
     G.trace_enable(True)
     orbs2, _, mus2 = configs.orbitals(101, n_orb=5)
@@ -139,7 +140,7 @@ def test_config3_all_pairs_mixed_within_prec(G):
         rel = l2_diff(G, a, b) / math.sqrt(b.getSquareNorm())
         worst = max(worst, rel)
         assert sa["nodes_out"] == sb["nodes_out"] and sa["tuples"] == sb["tuples"]
-    assert worst < 1e-2 * prec, worst
+    assert worst < 0.1 * prec, worst
 
 
 def test_config4_predictor_on_fresh_orbitals_k9(G):
@@ -200,7 +201,7 @@ def test_config5_reduced_step_matches_oracle(G, oracle):
 
 def test_config5_full_step_mixed_within_prec(G):
     """The whole step (32 Helmholtz + 528 pair Poissons, k=9, prec 1e-6) as one mixed batch
-    (FP32 MFMA at k=9) against the FP64 batch: every output within 1e-2 prec of FP64 (exact
+    (FP32 MFMA at k=9) against the FP64 batch: every output within 0.1 prec of FP64 (exact
     L2), the same grids and tuples."""
     prec = 1e-6
     mra = G.mra_from(configs.BOX20, 9)
@@ -221,7 +222,7 @@ def test_config5_full_step_mixed_within_prec(G):
     for a, b, sa, sb in zip(o32, o64, s32, s64):
         assert sa["nodes_out"] == sb["nodes_out"] and sa["tuples"] == sb["tuples"]
         worst = max(worst, l2_diff(G, a, b) / math.sqrt(b.getSquareNorm()))
-    assert worst < 1e-2 * prec, worst
+    assert worst < 0.1 * prec, worst
 
 
 @pytest.mark.parametrize("basis,mu", [("L", 0.0), ("I", 1.3)])
