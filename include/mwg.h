@@ -117,6 +117,14 @@ int mwg_tree_layout_check(const int *boxes, int max_depth, int64_t n, const int 
 int mwg_tree_export(mwg_tree tree, int *depth, int64_t *lr, double *coefs);
 int mwg_tree_rescale(mwg_tree tree, double c);               /* FunctionTree::rescale (HelmholtzVector.cpp:81) */
 int mwg_tree_dot(mwg_tree a, mwg_tree b, double *out);       /* mrcpp::dot */
+/* mwg_tree_dot without device allocation.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Node pairs and partial sums live in a static 8 MiB
+ * device array per GPU (at most 1048576 node pairs, the roots plus every deeper node of a
+ * that b also holds; MWG_ERR_RANGE beyond).  Every call on one GPU shares that array:
+ * calls that overlap on one GPU race.  Returns the same value as mwg_tree_dot up to the
+ * order of the FP64 summation. */
+int mwg_tree_dot_noalloc(mwg_tree a, mwg_tree b, double *out);
 int mwg_tree_eval(mwg_tree tree, int64_t n, const double *xyz, double *out);
 
 /* ---- projection of analytic functions: replaces mrcpp::project (qmfunction_utils.cpp:88-120)

@@ -63,6 +63,7 @@ def _sig():
         "mwg_tree_export": [vp, fi, fl, fd],
         "mwg_tree_rescale": [vp, d],
         "mwg_tree_dot": [vp, vp, C.POINTER(C.c_double)],
+        "mwg_tree_dot_noalloc": [vp, vp, C.POINTER(C.c_double)],
         "mwg_tree_eval": [vp, i64, fd, fd],
         "mwg_project_gauss": [d, vp, i, fd, fd, fd, fd],
         "mwg_project_hydrogen": [d, vp, i, fd, fi, d, fd],
@@ -96,7 +97,8 @@ def _sig():
 EXPORTED = ["mwg_ctx_create", "mwg_ctx_destroy", "mwg_last_error", "mwg_sync", "mwg_ctx_set_tuning",
             "mwg_ctx_get_tuning", "mwg_mra_create", "mwg_mra_destroy",
             "mwg_tree_create", "mwg_tree_destroy", "mwg_tree_nnodes", "mwg_tree_sqnorm", "mwg_tree_maxdepth",
-            "mwg_tree_import", "mwg_tree_layout_check", "mwg_tree_export", "mwg_tree_rescale", "mwg_tree_dot", "mwg_tree_eval",
+            "mwg_tree_import", "mwg_tree_layout_check", "mwg_tree_export", "mwg_tree_rescale", "mwg_tree_dot", "mwg_tree_dot_noalloc",
+            "mwg_tree_eval",
             "mwg_project_gauss", "mwg_project_hydrogen", "mwg_oper_poisson", "mwg_oper_helmholtz",
             "mwg_oper_destroy", "mwg_oper_nterms", "mwg_oper_terms", "mwg_oper_blocks", "mwg_multiply", "mwg_multiply_batch", "mwg_add", "mwg_apply",
             "mwg_apply_batch", "mwg_predictor_create", "mwg_predictor_destroy", "mwg_predictor_sizes",

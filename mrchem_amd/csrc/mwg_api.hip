@@ -238,6 +238,7 @@ struct Ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string err;
     DBuf<double> gx, gw;  // 24-point Gauss-Legendre (operator quadrature)
+    void *dot_scratch = nullptr;  // this device's allocation-free dot scratch (device address)
     // apply scratch
     DBuf<TermTab> tabs;
     DBuf<double> tbmax;
@@ -1931,6 +1932,30 @@ static int guarded(Ctx *ctx, F f) {
 
 extern "C" {
 
+// ------------------------------------------------------------------ allocation-free dot scratch
+//
+// mwg_tree_dot_noalloc stages its node pairs and per-block sums in one static device array
+// per GPU: DOT_NOALLOC_MAX_PAIRS pair words, then DOT_NOALLOC_BLOCKS partial sums, then
+// the total (8 MiB + 4 KiB).  Every call on a GPU shares that GPU's array.  Host code uses
+// the address hipGetSymbolAddress returns for the current device, never the array's name.
+constexpr int DOT_NOALLOC_MAX_PAIRS = 1 << 20;
+constexpr int DOT_NOALLOC_BLOCKS = 512;
+static __device__ __align__(256) unsigned long long scratch[DOT_NOALLOC_MAX_PAIRS + DOT_NOALLOC_BLOCKS + 1];
+
+// the scratch address on `device` (current), looked up once per device
+static void *dot_scratch_address(int device) {
+    static std::mutex mu;
+    static std::map<int, void *> addr;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = addr.find(device);
+    if (it != addr.end()) return it->second;
+    void *p = nullptr;
+    HC(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw Error(MWG_ERR_HIP, "hipGetSymbolAddress returned no address for the dot scratch");
+    addr[device] = p;
+    return p;
+}
+
 int mwg_ctx_create(int device, mwg_ctx *out) {
     if (!out) return MWG_ERR_ARG;
     auto *c = new mwg_ctx_s();
@@ -1940,6 +1965,7 @@ int mwg_ctx_create(int device, mwg_ctx *out) {
         HC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         HC(hipEventCreate(&c->ev0));
         HC(hipEventCreate(&c->ev1));
+        c->dot_scratch = dot_scratch_address(device);
         double x[24], w[24];
         gauss_legendre01(24, x, w);
         c->gx.upload(x, 24, c->stream);
@@ -2116,6 +2142,10 @@ int mwg_tree_dot(mwg_tree a, mwg_tree b, double *out) {
     if (a->mra != b->mra) return MWG_ERR_ARG;
     Ctx *C = a->mra->ctx;
     return guarded(C, [&] {
+        if (a->roots.empty() || b->roots.empty()) {  // an empty tree is the zero function
+            *out = 0.0;
+            return;
+        }
         std::vector<int> ia, ib, c0;
         for (size_t r = 0; r < a->roots.size(); r++) {
             ia.push_back(a->roots[r]);
@@ -2143,6 +2173,66 @@ int mwg_tree_dot(mwg_tree a, mwg_tree b, double *out) {
         double sum = 0.0;
         for (double v : hp) sum += v;
         *out = sum;
+    });
+}
+
+int mwg_tree_dot_noalloc(mwg_tree a, mwg_tree b, double *out) {
+    if (!a || !b || !out) return MWG_ERR_ARG;
+    if (a->mra != b->mra) return MWG_ERR_ARG;
+    Ctx *C = a->mra->ctx;
+    return guarded(C, [&] {
+        if (!C->dot_scratch) throw Error(MWG_ERR_STATE, "dot scratch not set up on this device");
+        unsigned long long *pairs = (unsigned long long *)C->dot_scratch;
+        double *part = (double *)(pairs + DOT_NOALLOC_MAX_PAIRS);
+        double *res = part + DOT_NOALLOC_BLOCKS;
+        // the same pairs as mwg_tree_dot: root with root (all 8 components), then every
+        // deeper node of a that b holds too (wavelet components only); counted first so an
+        // oversized call leaves the scratch untouched
+        const TreeH &A = *a, &B = *b;
+        if (A.roots.empty() || B.roots.empty()) {  // an empty tree is the zero function
+            *out = 0.0;
+            return;
+        }
+        auto match = [&](int i) {
+            return B.find(A.depth[i], A.lr[3 * i], A.lr[3 * i + 1], A.lr[3 * i + 2]);
+        };
+        int64_t np = (int64_t)A.roots.size();
+        for (int i = 0; i < A.n(); i++)
+            if (A.depth[i] > 0 && match(i) >= 0) np++;
+        if (np > DOT_NOALLOC_MAX_PAIRS)
+            throw Error(MWG_ERR_RANGE, "mwg_tree_dot_noalloc: " + std::to_string(np) +
+                                           " node pairs exceed the scratch limit of " +
+                                           std::to_string(DOT_NOALLOC_MAX_PAIRS));
+        hipStream_t s = C->stream;
+        // staged through a host stack buffer, copied (and waited for) whenever it fills
+        constexpr int STAGE = 4096;
+        unsigned long long buf[STAGE];
+        int nb = 0;
+        int64_t off = 0;
+        auto flush = [&]() {
+            if (!nb) return;
+            HC(hipMemcpyAsync(pairs + off, buf, (size_t)nb * sizeof(unsigned long long), hipMemcpyHostToDevice, s));
+            HC(hipStreamSynchronize(s));
+            off += nb;
+            nb = 0;
+        };
+        auto put = [&](int ia, int ib, int skip0) {
+            buf[nb++] = (unsigned long long)(unsigned)ia | ((unsigned long long)skip0 << 31) |
+                        ((unsigned long long)(unsigned)ib << 32);
+            if (nb == STAGE) flush();
+        };
+        for (size_t r = 0; r < A.roots.size(); r++) put(A.roots[r], B.roots[r], 0);
+        for (int i = 0; i < A.n(); i++) {
+            if (A.depth[i] == 0) continue;
+            const int j = match(i);
+            if (j >= 0) put(i, j, 1);
+        }
+        flush();
+        HC(launch_dot_pairs(s, (int)np, pairs, A.coef.p, B.coef.p, A.K3(), DOT_NOALLOC_BLOCKS, part, res));
+        double h = 0.0;
+        HC(hipMemcpyAsync(&h, res, sizeof(double), hipMemcpyDeviceToHost, s));
+        HC(hipStreamSynchronize(s));
+        *out = h;
     });
 }
 

@@ -128,6 +128,9 @@ hipError_t launch_nsblocks(hipStream_t s, int n, const long long *item_delta, co
 hipError_t launch_norms(hipStream_t s, int n, const int *nodes, const double *coef, double *norm, int K3);
 hipError_t launch_dot(hipStream_t s, int n, const int *ia, const int *ib, const int *c0, const double *A,
                       const double *B, int K3, double *part);
+// sum over node pairs of a . b (part: nblocks doubles of block sums, res: the total)
+hipError_t launch_dot_pairs(hipStream_t s, int n, const unsigned long long *pairs, const double *A,
+                            const double *B, int K3, int nblocks, double *part, double *res);
 hipError_t launch_scale(hipStream_t s, size_t n, double *x, double c);
 hipError_t launch_project_gauss(hipStream_t s, int nw, const int *work, const int *depth, const int *lr,
                                 const ProjGeom &geo, const DevBasis &B, int nterm, const double *coef,

@@ -301,6 +301,35 @@ __global__ void k_dot(int n, const int *ia, const int *ib, const int *c0, const 
     if (threadIdx.x == 0) part[i] = s;
 }
 
+// node-pair dot products over a fixed grid, one wave per pair (a pair word holds the a node
+// in bits 0..30, a flag to skip component 0 in bit 31 and the b node in bits 32..63); every
+// block writes its sum to part[blockIdx.x], so part needs no clearing
+__global__ void __launch_bounds__(256) k_dot_pairs(int n, const unsigned long long *pairs, const double *A,
+                                                   const double *B, int K3, double *part) {
+    __shared__ double red[4];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double s = 0.0;
+    for (int i = (int)blockIdx.x * 4 + wave; i < n; i += (int)gridDim.x * 4) {
+        const unsigned long long p = pairs[i];
+        const int ia = (int)(p & 0x7fffffffu), c0 = (int)((p >> 31) & 1u), ib = (int)(p >> 32);
+        const double *a = A + (size_t)ia * 8 * K3;
+        const double *b = B + (size_t)ib * 8 * K3;
+        for (int e = c0 * K3 + lane; e < 8 * K3; e += 64) s += a[e] * b[e];
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) red[wave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// one wave: *res = sum of part[0 .. n)
+__global__ void __launch_bounds__(64) k_sum_parts(int n, const double *part, double *res) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 64) s += part[i];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (threadIdx.x == 0) *res = s;
+}
+
 __global__ void k_scale(size_t n, double *x, double c) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) x[i] *= c;
@@ -1964,6 +1993,14 @@ static size_t tensor_smem(int K) { return (size_t)(8 * K * K * K + 4 * K * K) * 
 hipError_t launch_norms(hipStream_t s, int n, const int *nodes, const double *coef, double *norm, int K3) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_norms, dim3(n), dim3(64), 0, s, n, nodes, coef, norm, K3);
+    return hipGetLastError();
+}
+
+hipError_t launch_dot_pairs(hipStream_t s, int n, const unsigned long long *pairs, const double *A,
+                            const double *B, int K3, int nblocks, double *part, double *res) {
+    if (n < 0 || nblocks < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_dot_pairs, dim3(nblocks), dim3(256), 0, s, n, pairs, A, B, K3, part);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, s, nblocks, part, res);
     return hipGetLastError();
 }
 
