@@ -36,10 +36,14 @@ __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// a slice record through a wave-uniform pointer (scalar loads of its two 16-byte halves)
-__device__ inline Slice slice_at(const Slice *p) {
-    const uint4 *q = (const uint4 *)p;
-    const uint4 a = q[0], b = q[1];
+// a slice record through a wave-uniform pointer: scalar loads of its two 16-byte halves
+// (through the constant address space: the records are written by an earlier kernel and
+// only read here, and the scalar cache starts each kernel clean)
+typedef unsigned u32x8 __attribute__((ext_vector_type(8)));
+typedef const u32x8 __attribute__((address_space(4))) cu32x8;
+__device__ inline Slice slice_at(const Slice *base, int idx) {
+    const u32x8 r = *(cu32x8 *)(base + __builtin_amdgcn_readfirstlane(idx));
+    const uint4 a = make_uint4(r[0], r[1], r[2], r[3]), b = make_uint4(r[4], r[5], r[6], r[7]);
     Slice s;
     s.w = __hiloint2double((int)a.y, (int)a.x);
     s.f = (int)a.z;
@@ -92,18 +96,39 @@ __device__ inline void split_pairs(const float (&v)[N], float sc, unsigned (&hi)
     }
 }
 
-// largest value over the wave: across the lane halves and rows by permlane swaps (each
-// returns the pair of values that meet), then within each row of 16 by DPP rotations
+// The same split in four instructions per pair, the power of two sc folded in:
+// hi = f16(v sc) (v_fma_mix with addend 0), lo = f16(v sc - hi) (v_fma_mix with the f16 hi
+// as negated addend; v sc is exact and the difference is formed exactly before its one
+// rounding).  Both halves of a pair go into one register (mixlo / mixhi).
+__device__ inline void split_pair_sc(float v0, float v1, float sc, unsigned &hi, unsigned &lo) {
+    unsigned h, l;
+    asm volatile("v_fma_mixlo_f16 %0, %1, %3, 0\n\t"
+                 "v_fma_mixhi_f16 %0, %2, %3, 0"
+                 : "=&v"(h)
+                 : "v"(v0), "v"(v1), "v"(sc));
+    asm volatile("v_fma_mixlo_f16 %0, %1, %3, -%4 op_sel_hi:[0,0,1]\n\t"
+                 "v_fma_mixhi_f16 %0, %2, %3, -%4 op_sel:[0,0,1] op_sel_hi:[0,0,1]"
+                 : "=&v"(l)
+                 : "v"(v0), "v"(v1), "v"(sc), "v"(h));
+    hi = h;
+    lo = l;
+}
+
+// largest value over the wave of a non-negative v (compared as unsigned bit patterns, which
+// order non-negative floats like their values): across the lane halves and rows by
+// permlane swaps (each returns the pair of values that meet), then within each row of 16
+// by DPP rotations
 __device__ inline float wave_max(float v) {
-    const auto a = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
-    v = fmaxf(__uint_as_float(a[0]), __uint_as_float(a[1]));
-    const auto b = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
-    v = fmaxf(__uint_as_float(b[0]), __uint_as_float(b[1]));
-    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x128, 0xf, 0xf, false)));  // row_ror:8
-    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x124, 0xf, 0xf, false)));  // row_ror:4
-    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x122, 0xf, 0xf, false)));  // row_ror:2
-    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x121, 0xf, 0xf, false)));  // row_ror:1
-    return v;
+    unsigned u = __float_as_uint(v);
+    const auto a = __builtin_amdgcn_permlane32_swap(u, u, false, false);
+    u = max((unsigned)a[0], (unsigned)a[1]);
+    const auto b = __builtin_amdgcn_permlane16_swap(u, u, false, false);
+    u = max((unsigned)b[0], (unsigned)b[1]);
+    u = max(u, (unsigned)__builtin_amdgcn_mov_dpp((int)u, 0x128, 0xf, 0xf, true));  // row_ror:8
+    u = max(u, (unsigned)__builtin_amdgcn_mov_dpp((int)u, 0x124, 0xf, 0xf, true));  // row_ror:4
+    u = max(u, (unsigned)__builtin_amdgcn_mov_dpp((int)u, 0x122, 0xf, 0xf, true));  // row_ror:2
+    u = max(u, (unsigned)__builtin_amdgcn_mov_dpp((int)u, 0x121, 0xf, 0xf, true));  // row_ror:1
+    return __uint_as_float(u);
 }
 // s with (largest magnitude) 2^s in [0.5, 1); 0 for an all-zero tensor
 __device__ inline int scale_exp(float mx) { return mx > 0.f ? -(ilogbf(mx) + 1) : 0; }
@@ -170,31 +195,55 @@ struct OpsH8 {
     int mask;
 };
 
-__device__ inline OpsH8 load_opsh8(const Slice &sl, const JobTab &J, const uint4 *gen16, const int *genExp,
+// the job's split-FP16 arrays, read from the job table once per task (kept in registers:
+// the LDS fences of the slice loop would otherwise make the compiler reload them per slice)
+struct Arr16 {
+    const uint4 *oper;
+    const int *operExp;
+    const uint4 *in;
+    const int *inExp;
+};
+
+// global-address-space loads (not FLAT): the compiler then counts them exactly and waits
+// only for the slice being contracted, not for the next slice's operands already in flight
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef const u32x4 __attribute__((address_space(1))) gu32x4;
+typedef const int __attribute__((address_space(4))) cint;
+__device__ inline uint4 ld16(const uint4 *p) {
+    const u32x4 v = *(gu32x4 *)p;
+    return make_uint4(v[0], v[1], v[2], v[3]);
+}
+// a wave-uniform exponent (scalar load; written by an earlier kernel, read-only here)
+__device__ inline int ld_exp(const int *p) { return *(cint *)p; }
+
+__device__ inline OpsH8 load_opsh8(const Slice &sl, const Arr16 &J, const uint4 *gen16, const int *genExp,
                                    int m, int q) {
     OpsH8 o;
     const int bxb = sl.xo / 64, byb = sl.yo / 64, bzb = sl.zo / 64;  // block indices (gt_d = 0)
     const int rx = (m >> 3) * 2, row = m & 7;
-    o.bx = J.operBlocks16[((bxb + rx) * 8 + row) * 2 + (q & 1)];
-    o.ay = J.operBlocks16[((byb + rx) * 8 + row) * 2 + (q >> 1)];
-    o.az = J.operBlocks16[((bzb + rx) * 8 + row) * 2 + (q >> 1)];
+    o.bx = ld16(J.oper + ((bxb + rx) * 8 + row) * 2 + (q & 1));
+    o.ay = ld16(J.oper + ((byb + rx) * 8 + row) * 2 + (q >> 1));
+    o.az = ld16(J.oper + ((bzb + rx) * 8 + row) * 2 + (q >> 1));
     const uint4 *fb;
     int ef;
     if (sl.f >= 0) {
-        fb = J.inCoef16 + ((size_t)sl.f * 8 + sl.ft) * 128;
-        ef = J.inExp16[(size_t)sl.f * 8 + sl.ft];
+        fb = J.in + ((size_t)sl.f * 8 + sl.ft) * 128;
+        ef = ld_exp(J.inExp + (size_t)sl.f * 8 + sl.ft);
     } else {
         fb = gen16 + (size_t)(-sl.f - 1) * 128;
-        ef = genExp[-sl.f - 1];
+        ef = ld_exp(genExp + (-sl.f - 1));
     }
 #pragma unroll
     for (int tl = 0; tl < 4; tl++) {
         const int y = 4 * (tl & 1) + (m & 3), z = 4 * (tl >> 1) + (m >> 2);
-        o.f[tl] = fb[(z * 8 + y) * 2 + (q >> 1)];
+        o.f[tl] = ld16(fb + (z * 8 + y) * 2 + (q >> 1));
     }
-    o.exz = J.operExp16[bxb + rx] + J.operExp16[bzb + 2 * (q >> 1)] + ef;
-    o.ey0 = J.operExp16[byb];
-    o.ey1 = J.operExp16[byb + 2];
+    // block exponents: uniform loads, then this lane's gt_x (rx) and gt_z (q >> 1)
+    const int ex0 = ld_exp(J.operExp + bxb), ex1 = ld_exp(J.operExp + bxb + 2);
+    const int ez0 = ld_exp(J.operExp + bzb), ez1 = ld_exp(J.operExp + bzb + 2);
+    o.exz = (rx ? ex1 : ex0) + ((q >> 1) ? ez1 : ez0) + ef;
+    o.ey0 = ld_exp(J.operExp + byb);
+    o.ey1 = ld_exp(J.operExp + byb + 2);
     o.w = sl.w;
     o.mask = sl.mask;
     return o;
@@ -213,10 +262,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
     if ((int)blockIdx.x >= ntask) return;
     const SliceTask task = tasks[blockIdx.x];
     const int wave = wave_index(), lane = threadIdx.x & 63, m = lane & 15, q = lane >> 4;
-    uint4 *L1 = lds + wave * 768, *L2 = L1 + 256;
+    // hi and lo in separate planes: a lane's 16-byte writes are contiguous across the wave
+    uint4 *L1H = lds + wave * 768, *L1L = L1H + 128, *L2H = L1H + 256, *L2L = L2H + 256;
+    const uint4 *L1r = (q & 1) ? L1L : L1H, *L2r = (q & 1) ? L2L : L2H;  // B role: hi for kg 0, 2
     const int gx = m >> 3, gz = q >> 1;
     const WorkItem wi = work[task.g];
-    const JobTab &J = jobs[wi.job];
+    const Arr16 J{jobs[wi.job].operBlocks16, jobs[wi.job].operExp16, jobs[wi.job].inCoef16, jobs[wi.job].inExp16};
     float acc[16][4];
 #pragma unroll
     for (int t = 0; t < 16; t++)
@@ -226,78 +277,90 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
     // one after are loaded while a slice is contracted (indices clamped to the last slice)
     const Slice *srec = slices + task.start;
     const int last = task.count - 1;  // a task holds at least one slice
-    OpsH8 cur = load_opsh8(slice_at(srec + min(wave, last)), J, gen16, genExp, m, q);
-    Slice rnext = slice_at(srec + min(wave + 4, last));
     int ntup = 0;
-    for (int it = wave; it < task.count; it += 4) {
-        const OpsH8 nxt = load_opsh8(rnext, J, gen16, genExp, m, q);
-        rnext = slice_at(srec + min(it + 8, last));
-        ntup += __popc((unsigned)cur.mask);
-        floatx4 t1[4];
+    // one slice: passes x, y, z and the weighted FP32 accumulation
+    auto contract = [&](const OpsH8 &cur) {
+            ntup += __popc((unsigned)cur.mask);
+            floatx4 t1[4];
 #pragma unroll
-        for (int tl = 0; tl < 4; tl++) t1[tl] = mfmah(cur.f[tl], cur.bx);
-        float m1 = 0.f;
+            for (int tl = 0; tl < 4; tl++) t1[tl] = mfmah(cur.f[tl], cur.bx);
+            float m1 = 0.f;
 #pragma unroll
-        for (int tl = 0; tl < 4; tl++)
+            for (int tl = 0; tl < 4; tl++)
 #pragma unroll
-            for (int r = 0; r < 4; r++) m1 = fmaxf(m1, fabsf(t1[tl][r]));
-        const int s1 = scale_exp(wave_max(m1));
-        const float sc1 = ldexpf(1.f, s1);
+                for (int r = 0; r < 4; r++) m1 = fmaxf(m1, fabsf(t1[tl][r]));
+            const int s1 = scale_exp(wave_max(m1));
+            const float sc1 = ldexpf(1.f, s1);
 #pragma unroll
-        for (int zq = 0; zq < 2; zq++) {
-            float v[8];
+            for (int zq = 0; zq < 2; zq++) {
+                float v[8];
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    v[r] = t1[2 * zq][r];
+                    v[4 + r] = t1[2 * zq + 1][r];
+                }
+                unsigned hi[4], lo[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) split_pair_sc(v[2 * j], v[2 * j + 1], sc1, hi[j], lo[j]);
+                const int z = 4 * zq + q;
+                L1H[z * 16 + m] = make_uint4(hi[0], hi[1], hi[2], hi[3]);
+                L1L[z * 16 + m] = make_uint4(lo[0], lo[1], lo[2], lo[3]);
+            }
+            wave_sync();
+            floatx4 t2[8];
+#pragma unroll
+            for (int z = 0; z < 8; z++) t2[z] = mfmah(cur.ay, L1r[z * 16 + m]);
+            float m2 = 0.f;
+#pragma unroll
+            for (int z = 0; z < 8; z++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) m2 = fmaxf(m2, fabsf(t2[z][r]));
+            const int s2 = scale_exp(wave_max(m2));
+            const float sc2 = ldexpf(1.f, s2);
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                v[r] = t1[2 * zq][r];
-                v[4 + r] = t1[2 * zq + 1][r];
+                float v[8];
+#pragma unroll
+                for (int z = 0; z < 8; z++) v[z] = t2[z][r];
+                unsigned hi[4], lo[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) split_pair_sc(v[2 * j], v[2 * j + 1], sc2, hi[j], lo[j]);
+                const int bp = 4 * q + r;
+                L2H[bp * 16 + m] = make_uint4(hi[0], hi[1], hi[2], hi[3]);
+                L2L[bp * 16 + m] = make_uint4(lo[0], lo[1], lo[2], lo[3]);
             }
-            unsigned hi[4], lo[4];
-            split_pairs<8>(v, sc1, hi, lo);
-            const int z = 4 * zq + q;
-            L1[(z * 16 + m) * 2] = make_uint4(hi[0], hi[1], hi[2], hi[3]);
-            L1[(z * 16 + m) * 2 + 1] = make_uint4(lo[0], lo[1], lo[2], lo[3]);
-        }
-        wave_sync();
-        floatx4 t2[8];
+            wave_sync();
+            // pass z: a half (gt_y) without a surviving component is skipped (uniform mask)
+            const int mk = cur.mask;
+            const float wlo = ((mk >> (gx | (gz << 2))) & 1) ? (float)ldexp(cur.w, -(cur.exz + cur.ey0 + s1 + s2)) : 0.f;
+            const float whi = ((mk >> (gx | 2 | (gz << 2))) & 1) ? (float)ldexp(cur.w, -(cur.exz + cur.ey1 + s1 + s2)) : 0.f;
 #pragma unroll
-        for (int z = 0; z < 8; z++) t2[z] = mfmah(cur.ay, L1[(z * 16 + m) * 2 + (q & 1)]);
-        float m2 = 0.f;
+            for (int h = 0; h < 2; h++) {
+                if (h == 0 ? !(mk & 0x33) : !(mk & 0xCC)) continue;
+                floatx4 d[8];
 #pragma unroll
-        for (int z = 0; z < 8; z++)
+                for (int t = 0; t < 8; t++) d[t] = mfmah(cur.az, L2r[(8 * h + t) * 16 + m]);
+                const float ws = h == 0 ? wlo : whi;
 #pragma unroll
-            for (int r = 0; r < 4; r++) m2 = fmaxf(m2, fabsf(t2[z][r]));
-        const int s2 = scale_exp(wave_max(m2));
-        const float sc2 = ldexpf(1.f, s2);
+                for (int t = 0; t < 8; t++)
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            float v[8];
-#pragma unroll
-            for (int z = 0; z < 8; z++) v[z] = t2[z][r];
-            unsigned hi[4], lo[4];
-            split_pairs<8>(v, sc2, hi, lo);
-            const int bp = 4 * q + r;
-            L2[(bp * 16 + m) * 2] = make_uint4(hi[0], hi[1], hi[2], hi[3]);
-            L2[(bp * 16 + m) * 2 + 1] = make_uint4(lo[0], lo[1], lo[2], lo[3]);
-        }
-        wave_sync();
-        // pass z: a half (gt_y) without a surviving component is skipped (uniform mask)
-        const int mk = cur.mask;
-        const float wlo = ((mk >> (gx | (gz << 2))) & 1) ? (float)ldexp(cur.w, -(cur.exz + cur.ey0 + s1 + s2)) : 0.f;
-        const float whi = ((mk >> (gx | 2 | (gz << 2))) & 1) ? (float)ldexp(cur.w, -(cur.exz + cur.ey1 + s1 + s2)) : 0.f;
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-            if (h == 0 ? !(mk & 0x33) : !(mk & 0xCC)) continue;
-            floatx4 d[8];
-#pragma unroll
-            for (int t = 0; t < 8; t++) d[t] = mfmah(cur.az, L2[((8 * h + t) * 16 + m) * 2 + (q & 1)]);
-            const float ws = h == 0 ? wlo : whi;
-#pragma unroll
-            for (int t = 0; t < 8; t++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) acc[8 * h + t][r] = fmaf(ws, d[t][r], acc[8 * h + t][r]);
-        }
-        wave_sync();
-        cur = nxt;
+                    for (int r = 0; r < 4; r++) acc[8 * h + t][r] = fmaf(ws, d[t][r], acc[8 * h + t][r]);
+            }
+            wave_sync();
+    };
+    // two operand sets in turn (no register copies between slices)
+    OpsH8 opA = load_opsh8(slice_at(srec, min(wave, last)), J, gen16, genExp, m, q), opB;
+    Slice rnext = slice_at(srec, min(wave + 4, last));
+    for (int it = wave; it < task.count;) {
+        opB = load_opsh8(rnext, J, gen16, genExp, m, q);
+        rnext = slice_at(srec, min(it + 8, last));
+        contract(opA);
+        it += 4;
+        if (it >= task.count) break;
+        opA = load_opsh8(rnext, J, gen16, genExp, m, q);
+        rnext = slice_at(srec, min(it + 8, last));
+        contract(opB);
+        it += 4;
     }
     if (lane == 0) atomicAdd((unsigned long long *)&tabcnt[4 * wi.job + 2], (unsigned long long)ntup);
     // FP64 sum over the waves in LDS, wave 0 first, then 1, 2, 3
@@ -456,7 +519,7 @@ k_contract_mfma10(int ntask_arg, const TaskSummary *sum, const SliceTask *tasks,
     }
     const Slice *srec = slices + task.start;
     const int last = task.count - 1;
-    Slice cs = slice_at(srec);
+    Slice cs = slice_at(srec, 0);
     double fv[4];
     load_f10(fv, cs, J.inCoef, genPool);
     __syncthreads();
