@@ -105,6 +105,8 @@ int mwg_tree_destroy(mwg_tree tree);
 int mwg_tree_nnodes(mwg_tree tree, int64_t *n);              /* FunctionTree::getNNodes */
 int mwg_tree_sqnorm(mwg_tree tree, double *out);             /* FunctionTree::getSquareNorm */
 int mwg_tree_maxdepth(mwg_tree tree, int *out);
+int mwg_tree_mra(mwg_tree tree, mwg_mra *out);              /* FunctionTree::getMRA (borrowed handle) */
+int mwg_mra_order(mwg_mra mra, int *order);                  /* MultiResolutionAnalysis::getOrder */
 /* import refuses (MWG_ERR_ARG) input that breaks the packed layout: the roots first in
  * x-fastest order, then complete groups of 8 siblings in child order, each after its
  * parent; depths in [0, max_depth]; translations inside the box; no node twice */
@@ -126,6 +128,42 @@ int mwg_tree_dot(mwg_tree a, mwg_tree b, double *out);       /* mrcpp::dot */
  * order of the FP64 summation. */
 int mwg_tree_dot_noalloc(mwg_tree a, mwg_tree b, double *out);
 int mwg_tree_eval(mwg_tree tree, int64_t n, const double *xyz, double *out);
+
+/* ---- MRChem orbital files: replaces Orbital::saveOrbital / loadOrbital
+ * (src/qmfunctions/Orbital.cpp:118-190) and Density::saveDensity / loadDensity
+ * (src/qmfunctions/Density.cpp:55-125).  "<prefix>.meta" holds the raw FunctionData
+ * (ComplexFunction.h:33-37) and, for an orbital, OrbitalData (Orbital.h:51-56): 12 or 36
+ * bytes in the x86-64 layout (padding written as zeros); the parts go to
+ * "<prefix>_re.tree" and "<prefix>_im.tree". */
+typedef struct mwg_orbital_meta {
+    int32_t real_size, imag_size; /* tree chunks of each part (0: the part is absent) */
+    int32_t is_shared;
+    int32_t rank_id, spin, occ;   /* spin: 0 paired, 1 alpha, 2 beta (src/mrchem.h:41) */
+    double error;
+} mwg_orbital_meta;
+int mwg_meta_write(const char *path, const mwg_orbital_meta *meta, int orbital); /* orbital: 36 bytes, else 12 */
+int mwg_meta_read(const char *path, mwg_orbital_meta *meta, int *orbital);
+/* FunctionTree::saveTree(name) / loadTree(name) (Orbital.cpp:138, 180): the file
+ * "<name>.tree" in this library's format (MRCPP's is not part of the reference): a 96-byte
+ * header (magic "MWGTREE1", order, basis, root scale, max depth, corner[3], boxes[3],
+ * scaling factor, node count, chunk nodes (1024), chunk count, record bytes), then per
+ * node in the import layout int32 depth, int32 0, int64 l[3], 8 (k+1)^3 doubles.  Load
+ * refuses a file written for another MRA.  nchunks (may be null) receives the chunk
+ * count, the .meta's real_size or imag_size. */
+int mwg_tree_save(mwg_tree tree, const char *name, int *nchunks);
+int mwg_tree_load(mwg_tree tree, const char *name);
+/* the same file without a device, from and into host arrays in the import layout; read
+ * with null arrays returns the header (node count) only */
+typedef struct mwg_tree_file_info {
+    int32_t order, basis, root_scale, max_depth;
+    int32_t corner[3], boxes[3];
+    double scaling_factor;
+    int64_t nnodes;
+    int32_t chunk_nodes, nchunks; /* set by the writer */
+} mwg_tree_file_info;
+int mwg_tree_file_write(const char *name, const mwg_tree_file_info *info, const int *depth, const int64_t *lr,
+                        const double *coefs);
+int mwg_tree_file_read(const char *name, mwg_tree_file_info *info, int *depth, int64_t *lr, double *coefs);
 
 /* ---- projection of analytic functions: replaces mrcpp::project (qmfunction_utils.cpp:88-120)
  * Gaussian sum: term i = coef[i] * prod_d poly[15i+5d+e] (x_d-c_d)^e * exp(-alpha[i] |x-c|^2). */

@@ -28,6 +28,18 @@ class ApplyStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class OrbitalMeta(C.Structure):
+    """FunctionData + OrbitalData of an MRChem .meta file (include/mwg.h)."""
+    _fields_ = [("real_size", C.c_int32), ("imag_size", C.c_int32), ("is_shared", C.c_int32),
+                ("rank_id", C.c_int32), ("spin", C.c_int32), ("occ", C.c_int32), ("error", C.c_double)]
+
+
+class TreeFileInfo(C.Structure):
+    _fields_ = [("order", C.c_int32), ("basis", C.c_int32), ("root_scale", C.c_int32), ("max_depth", C.c_int32),
+                ("corner", C.c_int32 * 3), ("boxes", C.c_int32 * 3), ("scaling_factor", C.c_double),
+                ("nnodes", C.c_int64), ("chunk_nodes", C.c_int32), ("nchunks", C.c_int32)]
+
+
 class MwgError(RuntimeError):
     pass
 
@@ -58,12 +70,20 @@ def _sig():
         "mwg_tree_nnodes": [vp, C.POINTER(C.c_int64)],
         "mwg_tree_sqnorm": [vp, C.POINTER(C.c_double)],
         "mwg_tree_maxdepth": [vp, C.POINTER(C.c_int)],
+        "mwg_tree_mra": [vp, pvp],
+        "mwg_mra_order": [vp, C.POINTER(C.c_int)],
         "mwg_tree_import": [vp, i64, fi, fl, fd],
         "mwg_tree_layout_check": [fi, i, i64, fi, fl, C.c_char_p, i],
         "mwg_tree_export": [vp, fi, fl, fd],
         "mwg_tree_rescale": [vp, d],
         "mwg_tree_dot": [vp, vp, C.POINTER(C.c_double)],
         "mwg_tree_dot_noalloc": [vp, vp, C.POINTER(C.c_double)],
+        "mwg_meta_write": [C.c_char_p, C.POINTER(OrbitalMeta), i],
+        "mwg_meta_read": [C.c_char_p, C.POINTER(OrbitalMeta), C.POINTER(C.c_int)],
+        "mwg_tree_save": [vp, C.c_char_p, C.POINTER(C.c_int)],
+        "mwg_tree_load": [vp, C.c_char_p],
+        "mwg_tree_file_write": [C.c_char_p, C.POINTER(TreeFileInfo), vp, vp, vp],
+        "mwg_tree_file_read": [C.c_char_p, C.POINTER(TreeFileInfo), vp, vp, vp],
         "mwg_tree_eval": [vp, i64, fd, fd],
         "mwg_project_gauss": [d, vp, i, fd, fd, fd, fd],
         "mwg_project_hydrogen": [d, vp, i, fd, fi, d, fd],
@@ -97,8 +117,10 @@ def _sig():
 EXPORTED = ["mwg_ctx_create", "mwg_ctx_destroy", "mwg_last_error", "mwg_sync", "mwg_ctx_set_tuning",
             "mwg_ctx_get_tuning", "mwg_mra_create", "mwg_mra_destroy",
             "mwg_tree_create", "mwg_tree_destroy", "mwg_tree_nnodes", "mwg_tree_sqnorm", "mwg_tree_maxdepth",
+            "mwg_tree_mra", "mwg_mra_order",
             "mwg_tree_import", "mwg_tree_layout_check", "mwg_tree_export", "mwg_tree_rescale", "mwg_tree_dot", "mwg_tree_dot_noalloc",
-            "mwg_tree_eval",
+            "mwg_tree_eval", "mwg_meta_write", "mwg_meta_read", "mwg_tree_save", "mwg_tree_load",
+            "mwg_tree_file_write", "mwg_tree_file_read",
             "mwg_project_gauss", "mwg_project_hydrogen", "mwg_oper_poisson", "mwg_oper_helmholtz",
             "mwg_oper_destroy", "mwg_oper_nterms", "mwg_oper_terms", "mwg_oper_blocks", "mwg_multiply", "mwg_multiply_batch", "mwg_add", "mwg_apply",
             "mwg_apply_batch", "mwg_predictor_create", "mwg_predictor_destroy", "mwg_predictor_sizes",

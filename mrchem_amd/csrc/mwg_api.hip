@@ -2073,6 +2073,18 @@ int mwg_tree_maxdepth(mwg_tree tree, int *out) {
     return MWG_OK;
 }
 
+int mwg_tree_mra(mwg_tree tree, mwg_mra *out) {
+    if (!tree || !out) return MWG_ERR_ARG;
+    *out = static_cast<mwg_mra>(tree->mra);  // every MRA is created as an mwg_mra_s
+    return MWG_OK;
+}
+
+int mwg_mra_order(mwg_mra mra, int *order) {
+    if (!mra || !order) return MWG_ERR_ARG;
+    *order = mra->B.k;
+    return MWG_OK;
+}
+
 int mwg_tree_import(mwg_tree tree, int64_t n, const int *depth, const int64_t *lr, const double *coefs) {
     if (!tree || n < 0 || (n > 0 && (!depth || !lr || !coefs))) return MWG_ERR_ARG;
     Ctx *C = tree->mra->ctx;
@@ -2121,6 +2133,149 @@ int mwg_tree_export(mwg_tree tree, int *depth, int64_t *lr, double *coefs) {
         }
         if (coefs) T.coef.download(coefs, (size_t)T.n() * 8 * T.K3(), C->stream);
     });
+}
+
+// ------------------------------------------------------------------ orbital files
+
+static OrbitalMeta meta_in(const mwg_orbital_meta &m) {
+    OrbitalMeta o;
+    o.real_size = m.real_size;
+    o.imag_size = m.imag_size;
+    o.is_shared = m.is_shared;
+    o.rank_id = m.rank_id;
+    o.spin = m.spin;
+    o.occ = m.occ;
+    o.error = m.error;
+    return o;
+}
+
+int mwg_meta_write(const char *path, const mwg_orbital_meta *meta, int orbital) {
+    if (!path || !meta) return MWG_ERR_ARG;
+    return guarded(nullptr, [&] {
+        const std::string bad = write_meta(path, meta_in(*meta), orbital != 0);
+        if (!bad.empty()) throw Error(MWG_ERR_ARG, bad);
+    });
+}
+
+int mwg_meta_read(const char *path, mwg_orbital_meta *meta, int *orbital) {
+    if (!path || !meta) return MWG_ERR_ARG;
+    return guarded(nullptr, [&] {
+        OrbitalMeta o;
+        bool orb = false;
+        const std::string bad = read_meta(path, o, orb);
+        if (!bad.empty()) throw Error(MWG_ERR_ARG, bad);
+        *meta = mwg_orbital_meta{o.real_size, o.imag_size, o.is_shared, o.rank_id, o.spin, o.occ, o.error};
+        if (orbital) *orbital = orb ? 1 : 0;
+    });
+}
+
+static TreeFileInfo info_in(const mwg_tree_file_info &i) {
+    TreeFileInfo t;
+    t.order = i.order;
+    t.basis = i.basis;
+    t.root_scale = i.root_scale;
+    t.max_depth = i.max_depth;
+    for (int d = 0; d < 3; d++) {
+        t.corner[d] = i.corner[d];
+        t.boxes[d] = i.boxes[d];
+    }
+    t.scaling_factor = i.scaling_factor;
+    t.nnodes = i.nnodes;
+    return t;
+}
+
+static mwg_tree_file_info info_out(const TreeFileInfo &t) {
+    mwg_tree_file_info i{};
+    i.order = t.order;
+    i.basis = t.basis;
+    i.root_scale = t.root_scale;
+    i.max_depth = t.max_depth;
+    for (int d = 0; d < 3; d++) {
+        i.corner[d] = t.corner[d];
+        i.boxes[d] = t.boxes[d];
+    }
+    i.scaling_factor = t.scaling_factor;
+    i.nnodes = t.nnodes;
+    i.chunk_nodes = t.chunk_nodes;
+    i.nchunks = t.nchunks;
+    return i;
+}
+
+int mwg_tree_file_write(const char *name, const mwg_tree_file_info *info, const int *depth, const int64_t *lr,
+                        const double *coefs) {
+    if (!name || !info || info->nnodes < 0 || (info->nnodes > 0 && (!depth || !lr || !coefs))) return MWG_ERR_ARG;
+    return guarded(nullptr, [&] {
+        const TreeFileInfo t = info_in(*info);
+        for (int q = 0; q < 3; q++)
+            if (t.boxes[q] < 1) throw Error(MWG_ERR_ARG, "tree file: boxes must be positive");
+        const std::string lay = check_tree_layout(t.boxes, t.max_depth, t.nnodes, depth, lr);
+        if (!lay.empty()) throw Error(MWG_ERR_ARG, "tree file: " + lay);
+        const std::string bad = write_tree_file(std::string(name) + ".tree", t, depth, lr, coefs);
+        if (!bad.empty()) throw Error(MWG_ERR_ARG, bad);
+    });
+}
+
+int mwg_tree_file_read(const char *name, mwg_tree_file_info *info, int *depth, int64_t *lr, double *coefs) {
+    if (!name || !info) return MWG_ERR_ARG;
+    return guarded(nullptr, [&] {
+        TreeFileInfo t;
+        const std::string bad = read_tree_file(std::string(name) + ".tree", t, depth, lr, coefs);
+        if (!bad.empty()) throw Error(MWG_ERR_ARG, bad);
+        *info = info_out(t);
+    });
+}
+
+int mwg_tree_save(mwg_tree tree, const char *name, int *nchunks) {
+    if (!tree || !name) return MWG_ERR_ARG;
+    Ctx *C = tree->mra->ctx;
+    return guarded(C, [&] {
+        const TreeH &T = *tree;
+        const MraH &M = *T.mra;
+        TreeFileInfo t;
+        t.order = M.B.k;
+        t.basis = M.B.type;
+        t.root_scale = M.n0;
+        t.max_depth = M.maxDepth;
+        for (int d = 0; d < 3; d++) {
+            t.corner[d] = M.corner[d];
+            t.boxes[d] = M.boxes[d];
+        }
+        t.scaling_factor = M.sfac;
+        t.nnodes = T.n();
+        std::vector<int64_t> lr(T.lr.begin(), T.lr.end());
+        std::vector<double> coefs((size_t)T.n() * 8 * T.K3());
+        if (T.n()) T.coef.download(coefs.data(), coefs.size(), C->stream);
+        const std::string bad = write_tree_file(std::string(name) + ".tree", t, T.depth.data(), lr.data(),
+                                                coefs.data());
+        if (!bad.empty()) throw Error(MWG_ERR_ARG, bad);
+        if (nchunks) *nchunks = (int)((t.nnodes + TREE_FILE_CHUNK - 1) / TREE_FILE_CHUNK);
+    });
+}
+
+int mwg_tree_load(mwg_tree tree, const char *name) {
+    if (!tree || !name) return MWG_ERR_ARG;
+    Ctx *C = tree->mra->ctx;
+    std::vector<int> depth;
+    std::vector<int64_t> lr;
+    std::vector<double> coefs;
+    const int rc = guarded(C, [&] {
+        const MraH &M = *tree->mra;
+        TreeFileInfo t;
+        const std::string path = std::string(name) + ".tree";
+        std::string bad = read_tree_file(path, t, nullptr, nullptr, nullptr);
+        if (!bad.empty()) throw Error(MWG_ERR_ARG, bad);
+        bool same = t.order == M.B.k && t.basis == M.B.type && t.root_scale == M.n0 && t.max_depth == M.maxDepth &&
+                    t.scaling_factor == M.sfac;
+        for (int d = 0; d < 3; d++) same = same && t.corner[d] == M.corner[d] && t.boxes[d] == M.boxes[d];
+        if (!same) throw Error(MWG_ERR_ARG, path + ": written for another MRA (order, basis, box or depth)");
+        depth.resize((size_t)t.nnodes);
+        lr.resize(3 * (size_t)t.nnodes);
+        coefs.resize((size_t)t.nnodes * 8 * M.B.K3);
+        bad = read_tree_file(path, t, depth.data(), lr.data(), coefs.data());
+        if (!bad.empty()) throw Error(MWG_ERR_ARG, bad);
+    });
+    if (rc) return rc;
+    return mwg_tree_import(tree, (int64_t)depth.size(), depth.data(), lr.data(), coefs.data());
 }
 
 int mwg_tree_rescale(mwg_tree tree, double c) {

@@ -11,6 +11,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
 
 namespace mwg {
 
@@ -311,6 +314,144 @@ std::string check_tree_layout(const int boxes[3], int max_depth, int64_t n, cons
         index.emplace(key, i);
         has_children.push_back(0);
     }
+    return std::string();
+}
+
+// ------------------------------------------------------------------ orbital files
+
+namespace {
+struct FileCloser {
+    void operator()(FILE *f) const {
+        if (f) std::fclose(f);
+    }
+};
+using File = std::unique_ptr<FILE, FileCloser>;
+constexpr char TREE_MAGIC[8] = {'M', 'W', 'G', 'T', 'R', 'E', 'E', '1'};
+constexpr size_t TREE_HEADER = 96;
+
+template <class T>
+void put(unsigned char *b, size_t off, T v) {
+    std::memcpy(b + off, &v, sizeof(T));
+}
+template <class T>
+T get(const unsigned char *b, size_t off) {
+    T v;
+    std::memcpy(&v, b + off, sizeof(T));
+    return v;
+}
+}  // namespace
+
+// byte layout of FunctionData + OrbitalData on x86-64 (the padding is written as zeros)
+std::string write_meta(const std::string &path, const OrbitalMeta &m, bool orbital) {
+    unsigned char b[36] = {};
+    put<int32_t>(b, 0, m.real_size);
+    put<int32_t>(b, 4, m.imag_size);
+    b[8] = m.is_shared ? 1 : 0;
+    put<int32_t>(b, 12, m.rank_id);
+    put<int32_t>(b, 16, m.spin);
+    put<int32_t>(b, 20, m.occ);
+    put<double>(b, 28, m.error);
+    File f(std::fopen(path.c_str(), "wb"));
+    if (!f) return "cannot open " + path + " for writing";
+    const size_t n = orbital ? 36 : 12;
+    if (std::fwrite(b, 1, n, f.get()) != n) return "short write to " + path;
+    return std::string();
+}
+
+std::string read_meta(const std::string &path, OrbitalMeta &m, bool &orbital) {
+    unsigned char b[37];
+    File f(std::fopen(path.c_str(), "rb"));
+    if (!f) return "cannot open " + path;
+    const size_t n = std::fread(b, 1, sizeof(b), f.get());
+    if (n != 12 && n != 36) return path + ": a .meta file holds 12 or 36 bytes";
+    m = OrbitalMeta{};
+    m.real_size = get<int32_t>(b, 0);
+    m.imag_size = get<int32_t>(b, 4);
+    if (b[8] > 1) return path + ": is_shared is not a bool";
+    m.is_shared = b[8];
+    orbital = n == 36;
+    if (orbital) {
+        m.rank_id = get<int32_t>(b, 12);
+        m.spin = get<int32_t>(b, 16);
+        m.occ = get<int32_t>(b, 20);
+        m.error = get<double>(b, 28);
+    }
+    if (m.real_size < 0 || m.imag_size < 0) return path + ": negative tree size";
+    return std::string();
+}
+
+std::string write_tree_file(const std::string &path, TreeFileInfo info, const int *depth, const int64_t *lr,
+                            const double *coefs) {
+    if (info.order < 1 || info.order > MAXK - 1 || info.nnodes < 0) return "tree file: bad order or node count";
+    const int K = info.order + 1;
+    const size_t rec = 32 + sizeof(double) * 8 * (size_t)K * K * K;
+    info.chunk_nodes = TREE_FILE_CHUNK;
+    info.nchunks = (int32_t)((info.nnodes + TREE_FILE_CHUNK - 1) / TREE_FILE_CHUNK);
+    unsigned char h[TREE_HEADER] = {};
+    std::memcpy(h, TREE_MAGIC, 8);
+    put<int32_t>(h, 8, info.order);
+    put<int32_t>(h, 12, info.basis);
+    put<int32_t>(h, 16, info.root_scale);
+    put<int32_t>(h, 20, info.max_depth);
+    for (int d = 0; d < 3; d++) {
+        put<int32_t>(h, 24 + 4 * d, info.corner[d]);
+        put<int32_t>(h, 36 + 4 * d, info.boxes[d]);
+    }
+    put<double>(h, 48, info.scaling_factor);
+    put<int64_t>(h, 56, info.nnodes);
+    put<int32_t>(h, 64, info.chunk_nodes);
+    put<int32_t>(h, 68, info.nchunks);
+    put<int64_t>(h, 72, (int64_t)rec);
+    File f(std::fopen(path.c_str(), "wb"));
+    if (!f) return "cannot open " + path + " for writing";
+    if (std::fwrite(h, 1, TREE_HEADER, f.get()) != TREE_HEADER) return "short write to " + path;
+    std::vector<unsigned char> r(rec);
+    for (int64_t i = 0; i < info.nnodes; i++) {
+        std::memset(r.data(), 0, 32);
+        put<int32_t>(r.data(), 0, depth[i]);
+        for (int q = 0; q < 3; q++) put<int64_t>(r.data(), 8 + 8 * q, lr[3 * i + q]);
+        std::memcpy(r.data() + 32, coefs + (size_t)i * 8 * K * K * K, rec - 32);
+        if (std::fwrite(r.data(), 1, rec, f.get()) != rec) return "short write to " + path;
+    }
+    return std::string();
+}
+
+std::string read_tree_file(const std::string &path, TreeFileInfo &info, int *depth, int64_t *lr, double *coefs) {
+    File f(std::fopen(path.c_str(), "rb"));
+    if (!f) return "cannot open " + path;
+    unsigned char h[TREE_HEADER];
+    if (std::fread(h, 1, TREE_HEADER, f.get()) != TREE_HEADER) return path + ": truncated header";
+    if (std::memcmp(h, TREE_MAGIC, 8) != 0) return path + ": not a tree file";
+    TreeFileInfo t;
+    t.order = get<int32_t>(h, 8);
+    t.basis = get<int32_t>(h, 12);
+    t.root_scale = get<int32_t>(h, 16);
+    t.max_depth = get<int32_t>(h, 20);
+    for (int d = 0; d < 3; d++) {
+        t.corner[d] = get<int32_t>(h, 24 + 4 * d);
+        t.boxes[d] = get<int32_t>(h, 36 + 4 * d);
+    }
+    t.scaling_factor = get<double>(h, 48);
+    t.nnodes = get<int64_t>(h, 56);
+    t.chunk_nodes = get<int32_t>(h, 64);
+    t.nchunks = get<int32_t>(h, 68);
+    const int64_t rec = get<int64_t>(h, 72);
+    if (t.order < 1 || t.order > MAXK - 1 || t.nnodes < 0 || t.chunk_nodes <= 0) return path + ": bad header";
+    const int K = t.order + 1;
+    if (rec != (int64_t)(32 + sizeof(double) * 8 * (size_t)K * K * K)) return path + ": bad record size";
+    if (t.nchunks != (int32_t)((t.nnodes + t.chunk_nodes - 1) / t.chunk_nodes)) return path + ": bad chunk count";
+    info = t;
+    if (!depth && !lr && !coefs) return std::string();
+    std::vector<unsigned char> r((size_t)rec);
+    for (int64_t i = 0; i < t.nnodes; i++) {
+        if (std::fread(r.data(), 1, (size_t)rec, f.get()) != (size_t)rec) return path + ": truncated node records";
+        if (depth) depth[i] = get<int32_t>(r.data(), 0);
+        if (lr)
+            for (int q = 0; q < 3; q++) lr[3 * i + q] = get<int64_t>(r.data(), 8 + 8 * q);
+        if (coefs) std::memcpy(coefs + (size_t)i * 8 * K * K * K, r.data() + 32, (size_t)rec - 32);
+    }
+    unsigned char extra;
+    if (std::fread(&extra, 1, 1, f.get()) != 0) return path + ": trailing bytes after the node records";
     return std::string();
 }
 

@@ -62,4 +62,36 @@ bool split_check(int scale, double wnorm, double prec, double tnorm);
 // the node's depth, no node twice.  Returns an empty string or the first violation.
 std::string check_tree_layout(const int boxes[3], int max_depth, int64_t n, const int *depth, const int64_t *lr);
 
+// ---- MRChem orbital files (src/qmfunctions/Orbital.cpp:118-190, ComplexFunction.h:33-37,
+// Orbital.h:51-56): the .meta file is the raw FunctionData {int real_size, int imag_size,
+// bool is_shared} (12 bytes with padding) followed, for an orbital, by OrbitalData {int
+// rank_id, int spin, int occ, double error} (24 bytes with padding), little-endian x86-64
+// layout.  A density's .meta holds the FunctionData only (Density.cpp:55-67).
+struct OrbitalMeta {
+    int32_t real_size = 0, imag_size = 0;  // tree chunks of each part (0: part absent)
+    int32_t is_shared = 0;
+    int32_t rank_id = 0, spin = 0, occ = 0;  // spin: 0 paired, 1 alpha, 2 beta
+    double error = 0.0;
+};
+std::string write_meta(const std::string &path, const OrbitalMeta &m, bool orbital);
+// reads 12 (density) or 36 (orbital) bytes; `orbital` tells which
+std::string read_meta(const std::string &path, OrbitalMeta &m, bool &orbital);
+
+// Tree files ("<name>.tree", this library's own format; MRCPP's FunctionTree::saveTree
+// format is not part of the reference): a 96-byte header, then the nodes in the packed
+// import layout, each as int32 depth, int32 0, int64 l[3], double coefficients[8 K^3]
+// (the 8 components, each K^3 values with x fastest), in chunks of `chunk_nodes` nodes.
+struct TreeFileInfo {
+    int32_t order = 0, basis = 0, root_scale = 0, max_depth = 0;
+    int32_t corner[3] = {0, 0, 0}, boxes[3] = {0, 0, 0};
+    double scaling_factor = 1.0;
+    int64_t nnodes = 0;
+    int32_t chunk_nodes = 0, nchunks = 0;
+};
+constexpr int TREE_FILE_CHUNK = 1024;  // nodes per chunk written
+std::string write_tree_file(const std::string &path, TreeFileInfo info, const int *depth, const int64_t *lr,
+                            const double *coefs);
+// header only (depth, lr, coefs null) or the whole file into arrays sized from the header
+std::string read_tree_file(const std::string &path, TreeFileInfo &info, int *depth, int64_t *lr, double *coefs);
+
 }  // namespace mwg

@@ -157,6 +157,17 @@ class FunctionTree:
         self._check(lib().mwg_tree_export(self.h, depth, lr, coefs))
         return depth, lr.reshape(n, 3), coefs.reshape(n, 8, K3)
 
+    def saveTree(self, name: str) -> int:
+        """FunctionTree::saveTree: writes "<name>.tree"; returns the file's chunk count."""
+        n = C.c_int()
+        self._check(lib().mwg_tree_save(self.h, name.encode(), C.byref(n)))
+        return n.value
+
+    def loadTree(self, name: str) -> "FunctionTree":
+        """FunctionTree::loadTree: reads "<name>.tree" (written for this MRA)."""
+        self._check(lib().mwg_tree_load(self.h, name.encode()))
+        return self
+
     def load(self, depth, lr, coefs):
         self._check(lib().mwg_tree_import(self.h, len(depth), np.ascontiguousarray(depth, np.int32),
                                           np.ascontiguousarray(lr, np.int64).reshape(-1),

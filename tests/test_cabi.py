@@ -42,6 +42,35 @@ def test_mrcpp_facade_compiles(tmp_path):
     assert exe.exists()
 
 
+def test_cxx_poisson_caller_compiles(tmp_path):
+    """The C++ caller of the widened facade (project, multiply, add, grids, files) builds."""
+    from mrchem_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libmwgreen.so not built")
+    exe = tmp_path / "poisson_caller"
+    cmd = ["g++", "-std=c++17", "-O1", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "tests", "cxx", "poisson_caller.cpp"), "-L", os.path.dirname(_lib.LIB_PATH),
+           "-lmwgreen", "-o", str(exe)]
+    subprocess.run(cmd, check=True)
+    assert exe.exists()
+
+
+@pytest.mark.gpu
+def test_cxx_poisson_caller_runs(tmp_path):
+    """A C++ program over the mrcpp facade: a real Poisson apply against the erf closed form,
+    and the tree utilities (tests/cxx/poisson_caller.cpp prints one line per check)."""
+    from mrchem_amd import _lib
+    exe = tmp_path / "poisson_caller"
+    libdir = os.path.dirname(_lib.LIB_PATH)
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cxx", "poisson_caller.cpp"), "-L", libdir, "-lmwgreen",
+                    "-Wl,-rpath," + libdir, "-o", str(exe)], check=True)
+    r = subprocess.run([str(exe), str(tmp_path)], capture_output=True, text=True, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "poisson caller: ok" in r.stdout
+
+
 def test_bench_refuses_multi_gpu():
     r = subprocess.run(["python", os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True)
     assert r.returncode == 2
