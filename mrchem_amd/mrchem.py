@@ -15,6 +15,17 @@ Mirrors (same names, argument meaning and error behaviour):
       skipped above 1 and capped at 0.1 (:306-308, :379-387), spin factors (:415-424),
       K[i] += f_ij phi_j V_ij / ||phi_j||^2 and K[j] += f_ji phi_i V_ij / ||phi_i||^2
       (:287-347); all pair Poissons of one setup in one batched apply.
+  * orbital vector algebra — src/qmfunctions/orbital_utils.cpp: ``add`` (:128-144),
+      ``rotate`` (:151-183; out_i = sum_j U_ij phi_j), ``calc_overlap_matrix``,
+      ``calc_lowdin_matrix`` (:436-449, S^-1/2 by math_utils::hermitian_matrix_pow,
+      eigenvalues below MachineZero dropped), ``orthonormalize`` (:578-582), ``get_norms``.
+      The linear combinations are exact sums on the union grid of the terms (the
+      reference's numerically exact mode, no crop).
+  * ``helmholtz_step`` — one SCF orbital update of OrbitalOptimizer::optimize
+      (src/scf_solver/OrbitalOptimizer.cpp:148-170): Lambda = diag(F) (HelmholtzVector
+      clamps positive entries to -0.5), Psi = rotate(Lambda - F, Phi), Phi' = H(V, Phi, Psi),
+      U = orthonormalize(Phi'), F <- U F U^T, dPhi = Phi' - Phi, errors = ||dPhi_i||,
+      Phi <- Phi + dPhi (no KAIN acceleration).
 """
 from __future__ import annotations
 
@@ -231,3 +242,94 @@ class ExchangePotential:
 
     def clear(self):
         self.exchange = None
+
+
+# ------------------------------------------------------------------ orbital vector algebra
+
+MACHINE_ZERO = 1e-14  # mrcpp::MachineZero
+
+
+def _lincomb(mra, terms, grid=None):
+    """sum_k c_k f_k for (c_k, tree_k) terms, exact on the union grid (pairwise adds in
+    term order); coefficients below MachineZero are skipped (qmfunction_utils.cpp:152-165)
+    and no remaining term gives the zero function on the grid of `grid`."""
+    kept = [(c, t) for c, t in terms if abs(c) > MACHINE_ZERO]
+    if not kept:
+        g = grid if grid is not None else terms[0][1]
+        return M.add(M.FunctionTree(mra), 0.0, g, 0.0, g)
+    terms = kept
+    c0, t0 = terms[0]
+    acc = M.add(M.FunctionTree(mra), c0, t0, 0.0, t0)
+    for c, t in terms[1:]:
+        acc = M.add(M.FunctionTree(mra), 1.0, acc, c, t)
+    return acc
+
+
+def add(a: float, Phi_a, b: float, Phi_b):
+    """orbital::add: out_i = a (Phi_a)_i + b (Phi_b)_i."""
+    if len(Phi_a) != len(Phi_b):
+        raise M.MwgError("Size mismatch")
+    out = []
+    for pa, pb in zip(Phi_a, Phi_b):
+        mra = pa.tree.mra
+        out.append(Orbital(_lincomb(mra, [(a, pa.tree), (b, pb.tree)]), pa.occ, pa.spin))
+    return out
+
+
+def rotate(U, Phi):
+    """orbital::rotate: out_i = sum_j U_ij phi_j (occupation and spin of phi_i)."""
+    U = np.asarray(U, float)
+    if U.shape != (len(Phi), len(Phi)):
+        raise M.MwgError("rotation matrix size mismatch")
+    out = []
+    for i, p in enumerate(Phi):
+        out.append(Orbital(_lincomb(p.tree.mra, [(U[i, j], Phi[j].tree) for j in range(len(Phi))], p.tree),
+                           p.occ, p.spin))
+    return out
+
+
+def get_norms(Phi) -> np.ndarray:
+    return np.array([math.sqrt(p.squared_norm()) for p in Phi])
+
+
+def calc_overlap_matrix(Phi) -> np.ndarray:
+    n = len(Phi)
+    S = np.zeros((n, n))
+    for i in range(n):
+        for j in range(i, n):
+            S[i, j] = S[j, i] = M.dot(Phi[i].tree, Phi[j].tree)
+    return S
+
+
+def hermitian_matrix_pow(A, b: float) -> np.ndarray:
+    """math_utils::hermitian_matrix_pow: U diag(lambda^b) U^T, eigenvalues below
+    MachineZero set to 0."""
+    w, V = np.linalg.eigh(np.asarray(A, float))
+    d = np.array([0.0 if abs(x) < MACHINE_ZERO else x ** b for x in w])
+    return (V * d) @ V.T
+
+
+def calc_lowdin_matrix(Phi) -> np.ndarray:
+    return hermitian_matrix_pow(calc_overlap_matrix(Phi), -0.5)
+
+
+def orthonormalize(prec: float, Phi):
+    """orbital::orthonormalize: Phi <- S^-1/2 Phi; returns (Phi, U)."""
+    U = calc_lowdin_matrix(Phi)
+    return rotate(U, Phi), U
+
+
+def helmholtz_step(prec: float, F, Phi, V: M.FunctionTree, mra, flags: int = M.MWG_APPLY_FP64):
+    """One orbital update of OrbitalOptimizer::optimize (OrbitalOptimizer.cpp:148-170) with a
+    local potential V: returns (Phi_new, F_new, errors)."""
+    F = np.asarray(F, float)
+    H = HelmholtzVector(prec, np.diag(F), mra, flags)
+    L = np.diag(H.lambdas)
+    Psi = rotate(L - F, Phi)
+    Phi_np1 = H(Phi, V=V, Psi=Psi)
+    Phi_np1, U = orthonormalize(prec, Phi_np1)
+    F_new = U @ F @ U.T
+    dPhi = add(1.0, Phi_np1, -1.0, Phi)
+    errors = get_norms(dPhi)
+    Phi_new = add(1.0, Phi, 1.0, dPhi)
+    return Phi_new, F_new, errors
