@@ -1389,9 +1389,20 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         }
         genLastNew = hsum.gen_count - genDone;
         genDone = hsum.gen_count;
-        if (PT.on)
-            fprintf(stderr, "[mwg timing] iteration %d: %d work nodes, %lld tuples, %lld slices, %d generated (pool %d)\n",
-                    iter, nw, hsum.ntuples, hsum.nslices, genLastNew, genPoolCap);
+        if (PT.on) {
+            // candidates walked by the work-list kernels: per work node, its table's prefix total
+            std::vector<int> pre((size_t)ntabs + jt.size() + 1);
+            C.tpre.download(pre.data(), pre.size(), s);
+            long long cand = 0;
+            for (int i = 0; i < nw; i++) {
+                const JobTab &T = jt[work[i].job];
+                cand += pre[(size_t)T.tab0 + work[i].job + T.ntab];
+            }
+            fprintf(stderr,
+                    "[mwg timing] iteration %d: %d work nodes, %lld candidates, %lld tuples, %lld slices, %d generated "
+                    "(pool %d)\n",
+                    iter, nw, cand, hsum.ntuples, hsum.nslices, genLastNew, genPoolCap);
+        }
         for (size_t q = 0; q < jt.size(); q++) {
             JobH &J = jobs[jt_job[q]];
             J.tuples += tabcnt[4 * q] + tabcnt[4 * q + 2];

@@ -813,6 +813,42 @@ __device__ inline int lookup_f(const int *cache, const FCache &fc, const WorkIte
 }
 
 
+// The screening of one candidate (term, delta, input node): bit gt of surv[ft] is set when
+// the combination is inside its screening box (bit 8 gt + ft of cm) and its bound
+// (((|w| ||X_tx||) ||Y_ty||) ||Z_tz||) ||f_ft|| exceeds thr; maxb[ft] is the largest such
+// bound.  The products are formed in that order, as in the oracle's screening, so the
+// decisions are the same to the bit.  Unrolled and free of branches: the lanes of a wave
+// hold different candidates.
+template <bool MAXB>
+__device__ inline void screen_candidate(double aw, const double *nx, const double *ny, const double *nz,
+                                        const double *fn, unsigned long long cm, double thr, unsigned (&surv)[8],
+                                        double (&maxb)[8]) {
+    double pxy[4][4];
+#pragma unroll
+    for (int tx = 0; tx < 4; tx++) {
+        const double ax = aw * nx[tx];
+#pragma unroll
+        for (int ty = 0; ty < 4; ty++) pxy[tx][ty] = ax * ny[ty];
+    }
+#pragma unroll
+    for (int ft = 0; ft < 8; ft++) {
+        unsigned sv = 0u;
+        double mb = 0.0;
+#pragma unroll
+        for (int gt = 0; gt < 8; gt++) {
+            const int tx = 2 * (gt & 1) + (ft & 1);
+            const int ty = 2 * ((gt >> 1) & 1) + ((ft >> 1) & 1);
+            const int tz = 2 * ((gt >> 2) & 1) + ((ft >> 2) & 1);
+            const double b = (pxy[tx][ty] * nz[tz]) * fn[ft];
+            const bool ok = ((cm >> (8 * gt + ft)) & 1ull) && b > thr;
+            sv |= (unsigned)ok << gt;
+            if (MAXB) mb = ok ? fmax(mb, b) : mb;
+        }
+        surv[ft] = sv;
+        maxb[ft] = mb;
+    }
+}
+
 // Flattened candidate walk: every (term, delta) cell of the job's screened boxes is
 // one candidate, spread over all threads of the block (term prefix sums in LDS).
 // For each candidate with an input node the visitor gets the 12 operator norms
@@ -857,6 +893,7 @@ __device__ inline void walk_candidates(const CandTerms &ct, const int *fcache, c
         } else {
             // an entry past the allocated pool (its count is redone after the pool grows)
             fn[0] = -f - 1 < gs.pcap ? gs.norm[-f - 1] : 0.0;
+            for (int q = 1; q < 8; q++) fn[q] = 0.0;  // (no wavelet part: no bound above thr >= 0)
             nft = 1;
         }
         const double *nrm = J.operNorms + tb.nrm;
@@ -1068,22 +1105,24 @@ __global__ void __launch_bounds__(256) k_enum(int nw, int nparts, const WorkItem
                         const double *nx, const double *ny, const double *nz) {
                         const double aw = fabs(tb.w);
                         const unsigned long long cm = combo_mask(tb, dx, dy, dz);
-                        for (int ft = 0; ft < nft; ft++) {
-                            const double fv = fn[ft];
-                            const unsigned long long cf = (cm >> ft) & FT0_COMBOS;  // bit 8 gt
-                            if (!cf) continue;
+                        unsigned surv[8];
+                        double maxb[8];
+                        screen_candidate<false>(aw, nx, ny, nz, fn, cm, J.thr, surv, maxb);
+                        (void)nft;
+                        if (MODE == 0) {
+#pragma unroll
+                            for (int ft = 0; ft < 8; ft++)
+#pragma unroll
+                                for (int gt = 0; gt < 8; gt++) cnt[gt] += (surv[ft] >> gt) & 1u;
+                        } else {
+                            // per gt (static index of base and cnt) its surviving ft, in order
+#pragma unroll
                             for (int gt = 0; gt < 8; gt++) {
-                                if (!((cf >> (8 * gt)) & 1ull)) continue;
-                                const int tx = 2 * (gt & 1) + (ft & 1);
-                                const int ty = 2 * ((gt >> 1) & 1) + ((ft >> 1) & 1);
-                                const int tz = 2 * ((gt >> 2) & 1) + ((ft >> 2) & 1);
-                                double on = aw;
-                                on *= nx[tx];
-                                on *= ny[ty];
-                                on *= nz[tz];
-                                const double bound = on * fv;
-                                if (!(bound > J.thr)) continue;
-                                if (MODE == 1) {
+                                unsigned sg = 0u;
+#pragma unroll
+                                for (int ft = 0; ft < 8; ft++) sg |= ((surv[ft] >> gt) & 1u) << ft;
+                                for (unsigned m = sg; m; m &= m - 1) {
+                                    const int ft = __builtin_ctz(m);
                                     Tuple tp;
                                     tp.g = g;
                                     tp.f = f;
@@ -1093,8 +1132,8 @@ __global__ void __launch_bounds__(256) k_enum(int nw, int nparts, const WorkItem
                                     tp.dy = (int16_t)dy;
                                     tp.dz = (int16_t)dz;
                                     out[base[gt] + cnt[gt]] = tp;
+                                    cnt[gt]++;
                                 }
-                                cnt[gt]++;
                             }
                         }
                     });
@@ -1657,26 +1696,14 @@ __global__ void __launch_bounds__(256) k_enum_mixed(int nw, int nparts, int K2, 
                         const double *nx, const double *ny, const double *nz) {
                         const double aw = fabs(tb.w);
                         const unsigned long long cm = combo_mask(tb, dx, dy, dz);
-                        for (int ft = 0; ft < nft; ft++) {
-                            const double fv = fn[ft];
-                            int surv = 0;
-                            double maxb = 0.0;
-                            const unsigned long long cf = (cm >> ft) & FT0_COMBOS;  // bit 8 gt
-                            if (!cf) continue;
-                            for (int gt = 0; gt < 8; gt++) {
-                                if (!((cf >> (8 * gt)) & 1ull)) continue;
-                                const int tx = 2 * (gt & 1) + (ft & 1);
-                                const int ty = 2 * ((gt >> 1) & 1) + ((ft >> 1) & 1);
-                                const int tz = 2 * ((gt >> 2) & 1) + ((ft >> 2) & 1);
-                                double on = aw;
-                                on *= nx[tx];
-                                on *= ny[ty];
-                                on *= nz[tz];
-                                const double bound = on * fv;
-                                if (!(bound > J.thr)) continue;
-                                surv |= 1 << gt;
-                                maxb = fmax(maxb, bound);
-                            }
+                        unsigned survs[8];
+                        double maxbs[8];
+                        screen_candidate<true>(aw, nx, ny, nz, fn, cm, J.thr, survs, maxbs);
+                        (void)nft;  // (survs[ft] = 0 beyond the node's components)
+#pragma unroll
+                        for (int ft = 0; ft < 8; ft++) {
+                            const int surv = (int)survs[ft];
+                            const double maxb = maxbs[ft];
                             if (!surv) continue;
                             if (maxb > J.route) {
 #pragma unroll
