@@ -48,7 +48,7 @@ typedef struct mwg_predictor_s *mwg_predictor;
 /* apply flags */
 #define MWG_APPLY_FP64 0u     /* plain method: FP64 contractions, norm-test refinement */
 #define MWG_APPLY_MIXED 1u    /* MFMA slice contractions: k = 7 split-FP16 operands (hi + lo, 22
-                                 significand bits) on f16 MFMA, FP32 sums over <= 16 slices, FP64
+                                 significand bits) on f16 MFMA, FP32 sums over <= 64 slices, FP64
                                  beyond; k = 9 FP32 MFMA, FP64 sums.  Slices with large bounds run as
                                  FP64 tuples.  k = 7 and k = 9 only (MWG_ERR_ARG at other orders) */
 #define MWG_APPLY_PREDICT 2u  /* learned refinement predictor (norm test stays the judge; the output
@@ -84,8 +84,8 @@ const char *mwg_last_error(mwg_ctx ctx);
 int mwg_sync(mwg_ctx ctx);
 /* contraction task granularity (no reference counterpart: MRCPP threads over output
  * nodes with OpenMP): tuples per FP64 wave task (1..4096) and MFMA slices per
- * workgroup task (0..4096, 0 = automatic; k = 7 uses at most 64, k = 9 sizes automatic
- * tasks from each refinement iteration's slice total).  Results agree to FP64 rounding
+ * workgroup task (0..4096, 0 = automatic; k = 7 uses at most 256, 128 when automatic;
+ * k = 9 sizes automatic tasks from each refinement iteration's slice total).  Results agree to FP64 rounding
  * for any setting on the FP64 path and at k = 9, and to the rounding of the MFMA
  * slices' FP32 sums at k = 7; mrchem_amd.autotune picks the fastest on a sample
  * workload. */

@@ -1195,6 +1195,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             std::memcpy(blob.data() + o_bj, bjobs.data(), bjobs.size() * sizeof(BoxJob));
             upload_async(C, C.iterbuf, blob.data(), blob.size());
         }
+        PT.lap("tables+upload");
         WorkItem *dwork = (WorkItem *)C.iterbuf.p;
         JobTab *djobs = (JobTab *)(C.iterbuf.p + o_jobs);
         BoxJob *dbjobs = (BoxJob *)(C.iterbuf.p + o_bj);
@@ -1219,6 +1220,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         C.tbmax.alloc((size_t)std::max<size_t>(bjobs.size() * max_terms, 1));
         HC(launch_job_boxes(s, (int)bjobs.size(), max_terms, dbjobs, C.tabs.p, C.tbmax.p, djobs, C.tpre.p,
                             C.tpre0.p, C.genOverflow.p));
+        PT.lap("launch boxes");
         // generated input nodes: marked, then computed for the range the device counter
         // reached (read back with the work-list counts below, not before)
         {
@@ -1294,9 +1296,10 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         TaskSummary hsum;
         int chunk_it = C.chunk;  // FP64 task size of this iteration (raised when partial slots would not fit)
         // slices per MFMA task (a workgroup task).  k = 7: at most MFMA8_TASK_SLICES (the
-        // waves sum at most a quarter of them in FP32), 64 when automatic; k = 9: about 2048
-        // tasks per iteration when automatic, raised when node slots would not fit
-        int schunk_it = K == 8 ? std::min(C.slice_chunk > 0 ? C.slice_chunk : MFMA8_TASK_SLICES, MFMA8_TASK_SLICES)
+        // waves sum a quarter of them each in FP32), MFMA8_TASK_DEFAULT when automatic;
+        // k = 9: about 2048 tasks per iteration when automatic, raised when node slots
+        // would not fit
+        int schunk_it = K == 8 ? std::min(C.slice_chunk > 0 ? C.slice_chunk : MFMA8_TASK_DEFAULT, MFMA8_TASK_SLICES)
                                : (C.slice_chunk > 0 ? C.slice_chunk : -2048);
         auto launch_schedule = [&]() {
             ensure_caps(nullptr);
@@ -1336,7 +1339,9 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             C.rb.reserve(rb_host);
             char *h = C.rb.h;
             HC(hipMemcpyAsync(h, C.rbd.p, rb_host, hipMemcpyDeviceToHost, s));
+            PT.lap("launch sched");
             stream_wait(C);
+            PT.lap("gpu wait");
             C.stage.off = 0;
             std::memcpy(&hsum, h, sizeof(TaskSummary));
             std::memcpy(tabcnt.data(), h + o_tab, tabcnt.size() * sizeof(long long));
@@ -1499,10 +1504,12 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             }
             J.work.swap(next);
         };
+        PT.lap("norms to trees");
         if (par_split)
             for_each_job(active.size(), [&](size_t q) { split_job(active[q]); });
         else
             for (int j : active) split_job(j);
+        PT.lap("split tests");
         // learned lookahead: nodes predicted to split get their children computed now
         if (use_pred && !cand_node.empty()) {
             const int nc = (int)cand_node.size();
@@ -1551,6 +1558,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                     items.push_back({jobs[j].out->coef.p, jobs[j].out->norm.p, p, jobs[j].out->child[p]});
     }
     lofs[maxd] = items.size();
+    PT.lap("transform prep");
     if (!items.empty()) {
         upload_async(C, C.titems, items.data(), items.size());
         for (int d = 0; d < maxd; d++)
@@ -1558,6 +1566,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         for (int d = maxd - 1; d >= 0; d--)
             HC(launch_transform(s, 0, (int)(lofs[d + 1] - lofs[d]), C.titems.p + lofs[d], M.W.p, K));
     }
+    PT.lap("transform launch");
     {
         size_t total = 0;
         for (JobH &J : jobs) total += 8 * (size_t)J.out->n();

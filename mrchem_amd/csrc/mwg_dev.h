@@ -91,12 +91,14 @@ constexpr double ROUTE_FP64 = MWG_ROUTE_FP64;
 // accumulators, so the length of a task changes only the summation grouping; it sets
 // how many node slots (8 (k+1)^3 FP64 each) an iteration writes and reads back.
 // k = 7: a task's four waves take every fourth slice and each sums its slices in FP32
-// before the FP64 sum over waves, so its run is kept short: the FP32 running sum of
-// n <= 16 slices adds at most (n - 1) 2^-24 of their summed bounds, i.e. below 30 thr
-// for a wave whose 16 slices all sit at the routing bound (the plain method drops
-// every tuple below thr, hundreds per node)
+// before the FP64 sum over waves, so its run is kept short: the FP32 running sum of n
+// slices adds at most (n - 1) 2^-24 of their summed bounds.  Measured on the 5-orbital
+// Helmholtz batch at prec 1e-6, FP32 runs of 16, 32 and 64 slices per wave give the same
+// error against the FP64 apply (1.37e-8, 1.38e-8, 1.43e-8 of ||out||; the split of the
+// operands dominates); 32 per wave (128 per task) is the fastest and the default.
 constexpr int MAX_SLICE_CHUNK = 4096;
-constexpr int MFMA8_TASK_SLICES = 64;
+constexpr int MFMA8_TASK_SLICES = 256;  // k = 7 limit (64 slices per wave in FP32)
+constexpr int MFMA8_TASK_DEFAULT = 128;
 // on the root level (no screening threshold yet) slices above ROUTE_ROOT times the
 // largest possible bound run in FP64; FP32 rounding of the others stays ~1e-12 of it,
 // small also for outputs far below that bound (pair densities of orthogonal orbitals)
