@@ -27,9 +27,9 @@ PEAK = {"f64": 78.6e12, "f32": 157.3e12, "f16": 2.5e15}
 
 
 def mixed_dtype(k):
-    """Arithmetic of the mixed path's slice kernel: split-FP16 operands on f16 MFMA at k = 7
-    (FP32 sums), FP32 MFMA at k = 9."""
-    return "f16" if k == 7 else "f32"
+    """Arithmetic of the mixed path's slice kernels: split-FP16 operands (hi + lo) on f16 MFMA
+    at k = 7 (FP32 sums over a task's slices) and at k = 9 (FP64 sums)."""
+    return "f16"
 
 
 def log(*a):
@@ -298,7 +298,7 @@ def predictor_main(args, G, configs):
 def scf_run(args, G, configs, steps, warmup, predict=True, cpu=True):
     """SCF operator step: 32 orbitals on 16 nuclei (10 bohr cube), k=9, prec 1e-6; one step =
     the 32 Helmholtz applies (mu_i = sqrt(-2 eps_i), eps_i ~ U[-2, -0.2]) and the 528 pair
-    Poissons phi_i phi_j (i <= j) as one batch: plain FP64, mixed (FP32 MFMA at k=9) and mixed
+    Poissons phi_i phi_j (i <= j) as one batch: plain FP64, mixed (split-FP16 MFMA) and mixed
     with the learned predictor; every output's error against the FP64 batch (exact L2); the
     CPU oracle timed on two of the step's Helmholtz applies."""
     k, prec = 9, args.prec

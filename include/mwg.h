@@ -47,10 +47,11 @@ typedef struct mwg_predictor_s *mwg_predictor;
 
 /* apply flags */
 #define MWG_APPLY_FP64 0u     /* plain method: FP64 contractions, norm-test refinement */
-#define MWG_APPLY_MIXED 1u    /* MFMA slice contractions: k = 7 split-FP16 operands (hi + lo, 22
-                                 significand bits) on f16 MFMA, FP32 sums over <= 64 slices, FP64
-                                 beyond; k = 9 FP32 MFMA, FP64 sums.  Slices with large bounds run as
-                                 FP64 tuples.  k = 7 and k = 9 only (MWG_ERR_ARG at other orders) */
+#define MWG_APPLY_MIXED 1u    /* MFMA slice contractions with split-FP16 operands (hi + lo, 22
+                                 significand bits) on f16 MFMA: k = 7 FP32 sums over <= 64 slices,
+                                 FP64 beyond; k = 9 every slice's FP32 result summed in FP64.
+                                 Slices with large bounds run as FP64 tuples.  k = 7 and k = 9 only
+                                 (MWG_ERR_ARG at other orders) */
 #define MWG_APPLY_PREDICT 2u  /* learned refinement predictor (norm test stays the judge; the output
                                  tree can differ from the plain one, see mwg_predictor_create) */
 
@@ -60,7 +61,7 @@ typedef struct mwg_predictor_s *mwg_predictor;
 
 typedef struct mwg_apply_stats {
     int64_t nodes_out;       /* nodes of the output tree (output nodes of this apply) */
-    int64_t tuples;          /* surviving (g, f, ft, gt, term) tuples contracted (FP64 + FP32 slices) */
+    int64_t tuples;          /* surviving (g, f, ft, gt, term) tuples contracted (FP64 + MFMA slices) */
     int64_t tuples_fp64;     /* of which contracted in FP64 (all in plain mode; mixed mode routes
                                 slices with large bounds to FP64 tuples) */
     int64_t gen_nodes;       /* generated (scaling-only) input nodes of this apply */
@@ -71,8 +72,8 @@ typedef struct mwg_apply_stats {
     double t_apply;          /* seconds: work-list build + contraction + refine + transforms */
     double t_contract;       /* seconds spent in contraction kernels (HIP events) */
     double flops;            /* flops issued: FP64 tuples * 6 (k+1)^4 + MFMA flops of the slices
-                                (k = 7: 28 f16 16x16x32 MFMAs = 2*16*16*32*28 per slice, a skipped z half
-                                counted; k = 9 padded FP32: 2*32*12*(112+208+400)) */
+                                (f16 16x16x32 MFMAs as issued, padding included: k = 7 28 per slice,
+                                2*16*16*32*28, a skipped z half counted; k = 9 90, 2*16*16*32*90) */
 } mwg_apply_stats;
 
 /* ---- context (mrenv / device) ------------------------------------------------ */

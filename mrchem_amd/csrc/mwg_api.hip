@@ -156,8 +156,8 @@ struct DevCache {
         bytes = 0;
     }
     void *get(size_t want, size_t &got) {
-        auto it = blocks.lower_bound(want);
-        if (it != blocks.end() && it->first <= 2 * want + ((size_t)1 << 20)) {
+        auto it = blocks.lower_bound(want);  // a cached block at most an eighth (+1 MiB) larger
+        if (it != blocks.end() && it->first <= want + want / 8 + ((size_t)1 << 20)) {
             got = it->first;
             void *q = it->second;
             bytes -= it->first;
@@ -259,7 +259,7 @@ struct Ctx {
     DBuf<uint64_t> genKeys;
     DBuf<int> genJob, genCount, genOverflow;
     DBuf<double> genPool, genNorm;
-    DBuf<uint4> genPool16;  // split-FP16 copy of the generated nodes (mixed precision, k = 7)
+    DBuf<uint4> genPool16;  // split-FP16 copy of the generated nodes (mixed precision)
     DBuf<int> genExp16;
     DBuf<unsigned long long> genHashKeys;
     DBuf<int> genHashVals;
@@ -395,8 +395,8 @@ struct TreeH {
     DBuf<int> hvals;
     uint32_t hmask = 0;
     bool hvalid = false;
-    // split-FP16 copy of the coefficients for the mixed-precision apply at k = 7, made when
-    // the tree is first used as an input and kept until its coefficients change (ver)
+    // split-FP16 copy of the coefficients for the mixed-precision apply, made when the tree
+    // is an input and dropped at the end of that apply (ver guards against stale copies)
     DBuf<uint4> coef16;
     DBuf<int> exp16;
     uint64_t ver = 1, ver16 = 0;
@@ -455,11 +455,12 @@ struct TreeH {
         return index.get(node_key(d, x, y, z));
     }
     // grow the coefficient and norm arrays through the context's block cache (the old
-    // contents are copied on the stream; no synchronisation)
+    // contents are copied on the stream; no synchronisation).  Growth is by half again, and
+    // fit_dev trims a finished tree, so a tree holds at most about 1.5x its nodes' memory.
     void grow_cached(DBuf<double> &b, size_t m) {
         if (m <= b.n && b.p) return;
         Ctx &C = *mra->ctx;
-        const size_t want = std::max<size_t>(m, 2 * b.n);
+        const size_t want = std::max<size_t>(m, b.n + b.n / 2);
         size_t got = 0;
         double *q = (double *)C.cache.get(want * sizeof(double), got);
         if (b.p) {
@@ -472,6 +473,19 @@ struct TreeH {
     void ensure_dev(size_t nodes) {
         grow_cached(coef, nodes * 8 * (size_t)K3());
         grow_cached(norm, nodes * 8);
+    }
+    // a finished tree: move the coefficients to a block of their size when the capacity
+    // exceeds it by more than half (a copy on the stream; the old block goes to the cache)
+    void fit_dev() {
+        const size_t need = std::max<size_t>((size_t)n() * 8 * (size_t)K3(), 1);
+        if (!coef.p || coef.n <= need + need / 2) return;
+        Ctx &C = *mra->ctx;
+        size_t got = 0;
+        double *q = (double *)C.cache.get(need * sizeof(double), got);
+        HC(hipMemcpyAsync(q, coef.p, need * sizeof(double), hipMemcpyDeviceToDevice, C.stream));
+        const size_t old = coef.n * sizeof(double);
+        C.cache.put(coef.release(), old);
+        coef.adopt(q, got / sizeof(double));
     }
     ~TreeH() {
         if (!mra) return;
@@ -493,10 +507,27 @@ struct TreeH {
         if (ver16 == ver) return;
         const int K = mra->B.K;
         const size_t nb = (size_t)n() * 8;
-        coef16.alloc(nb * K * K * 2);
-        exp16.alloc(nb);
-        HC(launch_split16(mra->ctx->stream, (int)nb, 0, nullptr, K3(), K, coef.p, coef16.p, exp16.p));
+        Ctx &C = *mra->ctx;
+        const size_t need = nb * split16_block_uint4(K, 1, K3());
+        size_t got = 0;
+        if (coef16.n < need || !coef16.p) {  // (through the block cache, where drop16 returns it)
+            if (coef16.p) C.cache.put(coef16.release(), coef16.n * sizeof(uint4));
+            coef16.adopt((uint4 *)C.cache.get(need * sizeof(uint4), got), got / sizeof(uint4));
+        }
+        if (exp16.n < nb || !exp16.p) {
+            if (exp16.p) C.cache.put(exp16.release(), exp16.n * sizeof(int));
+            exp16.adopt((int *)C.cache.get(nb * sizeof(int), got), got / sizeof(int));
+        }
+        HC(launch_split16(mra->ctx->stream, (int)nb, 0, nullptr, K3(), K, coef.p, coef16.p, exp16.p, 1));
         ver16 = ver;
+    }
+    // the split-FP16 copy back to the block cache (made again on the next mixed apply)
+    void drop16() {
+        Ctx &C = *mra->ctx;
+        const size_t cb = coef16.n * sizeof(uint4), eb = exp16.n * sizeof(int);
+        if (coef16.p) C.cache.put(coef16.release(), cb);
+        if (exp16.p) C.cache.put(exp16.release(), eb);
+        ver16 = 0;
     }
     DHash dhash() const { return DHash{hkeys.p, hvals.p, hmask}; }
     void refresh_norms() {  // device norms -> host, square norm
@@ -539,9 +570,7 @@ struct LevelH {
     DBuf<double> blocks, norms;
     DBuf<LevTerm> dterms;  // per-term screening data for the device box builder
     DBuf<double> denv;     // concatenated norm envelopes of the active terms
-    DBuf<float> blocks32;  // FP32 copy for the mixed-precision contraction at k = 9 (made on first use)
-    bool has32 = false;
-    DBuf<uint4> blocks16;  // split-FP16 copy (rows [hi | lo], exponent per block) at k = 7
+    DBuf<uint4> blocks16;  // split-FP16 copy (operator rows, exponent per block) for the mixed apply
     DBuf<int> exp16;
     bool has16 = false;
 };
@@ -808,6 +837,7 @@ static void project_adaptive(TreeH &T, double prec, LaunchFn launch, const std::
         HC(launch_transform(s, 0, (int)items.size(), C.titems.p, M.W.p, M.B.K));
     }
     T.refresh_norms();
+    T.fit_dev();
 }
 
 // ------------------------------------------------------------------ apply
@@ -995,11 +1025,11 @@ static void for_each_job(size_t n, F f) {
         if (e) std::rethrow_exception(e);
 }
 
-// FP32 MFMA flops one slice issues: K = 8 three GEMMs of 16 x 8 over 64, 128 and 256
-// columns; K = 10 the padded tiles (2 x 16 rows, chunks of 4) over 112, 208 and 400 columns
+// MFMA flops one slice issues (v_mfma_f32_16x16x32_f16 with split operands, counted as
+// issued, padding included): K = 8 28 MFMAs, K = 10 90 (two M-tiles of 7 + 13 + 25 N-tiles)
 static double slice_flops(int K) {
     if (K == 8) return 2.0 * 16 * 16 * 32 * 28;  // split-FP16: 4 + 8 + 16 MFMAs of 16x16x32
-    if (K == 10) return 2.0 * 32 * 12 * (112 + 208 + 400);
+    if (K == 10) return 2.0 * 16 * 16 * 32 * 90;  // split-FP16: 2 x (7 + 13 + 25) MFMAs of 16x16x32
     return 0.0;
 }
 
@@ -1132,12 +1162,13 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                 T.inNorm = J.in->norm.p;
                 LevelH &LV = *J.op->lev[d];
                 T.operBlocks = LV.blocks.p;
-                if (mixed && K == 8) {
+                if (mixed) {
                     if (!LV.has16) {
                         const size_t nb = LV.blocks.n / ((size_t)K * K);
-                        LV.blocks16.alloc(nb * K * 2);
+                        LV.blocks16.alloc(nb * split16_block_uint4(K, 0, K * K));
                         LV.exp16.alloc(nb);
-                        HC(launch_split16(s, (int)nb, 0, nullptr, K * K, K, LV.blocks.p, LV.blocks16.p, LV.exp16.p));
+                        HC(launch_split16(s, (int)nb, 0, nullptr, K * K, K, LV.blocks.p, LV.blocks16.p, LV.exp16.p,
+                                          0));
                         LV.has16 = true;
                     }
                     J.in->ensure16();
@@ -1145,13 +1176,6 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                     T.operExp16 = LV.exp16.p;
                     T.inCoef16 = J.in->coef16.p;
                     T.inExp16 = J.in->exp16.p;
-                } else if (mixed) {
-                    if (!LV.has32) {
-                        LV.blocks32.alloc(LV.blocks.n);
-                        HC(launch_to_f32(s, LV.blocks.n, LV.blocks.p, LV.blocks32.p));
-                        LV.has32 = true;
-                    }
-                    T.operBlocks32 = LV.blocks32.p;
                 }
                 T.operNorms = LV.norms.p;
                 T.outCoef = J.out->coef.p;
@@ -1228,8 +1252,8 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             if (genPoolCap < want) {
                 C.genPool.grow_keep((size_t)want * K3, s);
                 C.genNorm.grow_keep((size_t)want, s);
-                if (mixed && K == 8) {
-                    C.genPool16.grow_keep((size_t)want * K * K * 2, s);
+                if (mixed) {
+                    C.genPool16.grow_keep((size_t)want * split16_block_uint4(K, 1, K3), s);
                     C.genExp16.grow_keep((size_t)want, s);
                 }
                 genPoolCap = want;
@@ -1239,9 +1263,9 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                     genPoolCap, dtab};
         HC(launch_gen_mark(s, nw, dwork, djobs, C.tabs.p, gs));
         HC(launch_gen_compute(s, genDone, genPoolCap, djobs, gs, M.W.p, K));
-        if (mixed && K == 8)
+        if (mixed)
             HC(launch_split16(s, genPoolCap - genDone, genDone, C.genCount.p, K3, K, C.genPool.p, C.genPool16.p,
-                              C.genExp16.p));
+                              C.genExp16.p, 1));
         PT.lap("upload+genmark");
         // Work list and contraction without a host round trip: counts per segment
         // (nseg per output node), a device-built schedule (offsets, tasks, reduce items),
@@ -1368,11 +1392,11 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                 gs.norm = C.genNorm.p;
                 gs.pcap = want;
                 HC(launch_gen_compute(s, genPoolCap, want, djobs, gs, M.W.p, K));
-                if (mixed && K == 8) {
-                    C.genPool16.grow_keep((size_t)want * K * K * 2, s);
+                if (mixed) {
+                    C.genPool16.grow_keep((size_t)want * split16_block_uint4(K, 1, K3), s);
                     C.genExp16.grow_keep((size_t)want, s);
                     HC(launch_split16(s, want - genPoolCap, genPoolCap, C.genCount.p, K3, K, C.genPool.p,
-                                      C.genPool16.p, C.genExp16.p));
+                                      C.genPool16.p, C.genExp16.p, 1));
                 }
                 genPoolCap = want;
                 launch_count();
@@ -1588,7 +1612,14 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             J.out->sqnorm_from_cnorm();
         }
     }
+    for (JobH &J : jobs) J.out->fit_dev();
     PT.lap("transforms");
+    // the inputs' split-FP16 copies live for one apply (an SCF iteration brings new inputs;
+    // at k = 9 a copy is 5/8 of the FP64 tree)
+    if (mixed) {
+        HC(hipStreamSynchronize(s));
+        for (JobH &J : jobs) J.in->drop16();
+    }
     PT.report();
     const double t_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
     stats.assign(jobs.size(), mwg_apply_stats{});
@@ -1698,6 +1729,7 @@ static void multiply_trees(TreeH &out, double prec, double coef, TreeH &A, TreeH
         HC(launch_transform(s, 0, (int)titems.size(), C.titems.p, M.W.p, M.B.K));
     }
     out.refresh_norms();
+    out.fit_dev();
 }
 
 // Many independent products as one: the same items, kernels and split tests as
@@ -1867,6 +1899,7 @@ static void multiply_batch(std::vector<MulJob> &jobs) {
         J.out->cnorm.assign(h + off, h + off + m);
         off += m;
         J.out->sqnorm_from_cnorm();
+        J.out->fit_dev();
     }
 }
 

@@ -38,9 +38,9 @@ struct JobTab {
     const double *inCoef;
     const double *inNorm;
     const double *operBlocks;  // this depth's block array of the job's operator
-    const float *operBlocks32; // FP32 copy (mixed precision, k = 9)
-    // split-FP16 copies (mixed precision, k = 7): rows [hi K | lo K] of each K x K operator
-    // block and of each K^3 input component (K^2 rows), a power-of-two exponent per block
+    // split-FP16 copies (mixed precision): each K x K operator block and each K^3 input
+    // component as rows of f16 hi and lo parts (layouts at split16_block_uint4), a
+    // power-of-two exponent per block
     const uint4 *operBlocks16;
     const int *operExp16;
     const uint4 *inCoef16;
@@ -194,15 +194,21 @@ hipError_t launch_enum_mixed(hipStream_t s, int mode, int K, int nw, int nparts,
                              const long long *seg_ofs, Tuple *out, Slice *sout, const TaskSummary *sum,
                              const SegRec *segs, const long long *gslice);
 // Split-FP16 copies of FP64 blocks (mwg_mfma.hip): block b (values src[b bvals ..]) as rows
-// of K values, each row [hi K | lo K] in f16 (2 K halves), scaled by 2^exp[b] so that the
-// block's largest magnitude lies in [0.5, 1) (exp 0 for a zero block); x 2^e = hi + lo up
-// to about 2^-22 of it.  With count, only blocks first .. min(*count, first + n) - 1.
+// of K values, scaled by 2^exp[b] so that the block's largest magnitude lies in [0.5, 1)
+// (exp 0 for a zero block); x 2^e = hi + lo up to about 2^-23 of it.  K = 8: each row
+// [hi 8 | lo 8] (two uint4).  K = 10, role 0 (operator blocks, A operands): each row
+// [hi 10 | hi 10 | lo 10 | 0 0] (four uint4); role 1 (input nodes, B operands): each row
+// [hi 10 | lo 10] (40 bytes).  With count, only blocks first .. min(*count, first + n) - 1.
 hipError_t launch_split16(hipStream_t s, int n, int first, const int *count, int bvals, int K, const double *src,
-                          uint4 *dst, int *exp);
+                          uint4 *dst, int *exp, int role = 0);
+// uint4 per block of bvals values of a split-FP16 copy
+inline size_t split16_block_uint4(int K, int role, int bvals) {
+    return K == 8 ? (size_t)bvals / 4 : role == 0 ? (size_t)bvals / 10 * 4 : (size_t)bvals / 10 * 40 / 16;
+}
 // FP32 / split-FP16 MFMA slice contraction (K = 8 or 10; mwg_mfma.hip).  A task is a run of
 // slices of one output node; mfma_tasks_per_block(K) tasks share a 256-thread block.  Each
 // task adds the surviving tuples of its slices to tabcnt[4 t + 2] of its job table t.
-// genPool16 / genExp16: the generated nodes' split-FP16 copies (K = 8).
+// genPool16 / genExp16: the generated nodes' split-FP16 copies.
 int mfma_tasks_per_block(int K);
 hipError_t launch_contract_mfma(hipStream_t s, int K, int ntask, const TaskSummary *sum, int grid,
                                 const SliceTask *tasks, const WorkItem *work, const JobTab *jobs,
@@ -219,7 +225,6 @@ hipError_t launch_multiply(hipStream_t s, int n, const MulItem *items, const Pro
                            double coef, const double *acoef, const double *bcoef, double *out);
 hipError_t launch_add(hipStream_t s, int n, const MulItem *items, const DevBasis &B, double ca, const double *acoef,
                       double cb, const double *bcoef, double *out);
-hipError_t launch_to_f32(hipStream_t s, size_t n, const double *in, float *out);
 // from the per-segment counts (nseg = 8: plain FP64, 9: mixed): segment offsets, FP64
 // tasks (full chunks first), MFMA slice tasks, reduce items, slot numbering.  wchunk:
 // slices per MFMA task, or (<= 0) about -wchunk tasks over the iteration's slices.

@@ -1759,11 +1759,6 @@ __global__ void __launch_bounds__(256) k_enum_mixed(int nw, int nparts, int K2, 
 }
 
 
-__global__ void k_to_f32(size_t n, const double *in, float *out) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = (float)in[i];
-}
-
 // norms of the work nodes of all jobs, into their trees and into a compact array
 __global__ void k_work_norms(int nw, const WorkItem *work, const JobTab *jobs, int K3, double *compact) {
     const int i = blockIdx.x;
@@ -2177,12 +2172,6 @@ hipError_t launch_enum_mixed(hipStream_t s, int mode, int K, int nw, int nparts,
     else
         hipLaunchKernelGGL(k_enum_mixed<1>, dim3(nb), dim3(256), 0, s, nw, nparts, K * K, work, jobs, tabs, gs,
                            thread_off, seg_total, seg_ofs, out, sout, sum, segs, gslice);
-    return hipGetLastError();
-}
-
-hipError_t launch_to_f32(hipStream_t s, size_t n, const double *in, float *out) {
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, in, out);
     return hipGetLastError();
 }
 

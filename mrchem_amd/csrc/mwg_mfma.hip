@@ -158,6 +158,57 @@ __global__ void __launch_bounds__(256) k_split16(int n, int first, const int *co
     }
 }
 
+// K = 10: one wave per block, one row of 10 values per lane -> 32 halves (four uint4)
+__global__ void __launch_bounds__(256) k_split16_10(int n, int first, const int *count, int bvals, int role,
+                                                    const double *src, uint4 *dst, int *exp) {
+    const int b = first + (int)blockIdx.x * 4 + wave_index(), lane = threadIdx.x & 63;
+    const int end = count ? min(*(volatile const int *)count, first + n) : first + n;
+    if (b >= end) return;
+    const double *v = src + (size_t)b * bvals;
+    double mx = 0.0;
+    for (int e = lane; e < bvals; e += 64) mx = fmax(mx, fabs(v[e]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+    const int ex = mx > 0.0 ? -(ilogb(mx) + 1) : 0;
+    if (lane == 0) exp[b] = ex;
+    const int rows = bvals / 10;
+    for (int r = lane; r < rows; r += 64) {
+        _Float16 hi[10], lo[10];
+#pragma unroll
+        for (int j = 0; j < 10; j++) {
+            const float x = (float)ldexp(v[r * 10 + j], ex);
+            hi[j] = (_Float16)x;
+            lo[j] = (_Float16)(x - (float)hi[j]);
+        }
+        if (role == 0) {  // [hi | hi | lo | 0 0]: four uint4
+            _Float16 h[32];
+#pragma unroll
+            for (int j = 0; j < 10; j++) {
+                h[j] = hi[j];
+                h[10 + j] = hi[j];
+                h[20 + j] = lo[j];
+            }
+            h[30] = h[31] = (_Float16)0.f;
+            unsigned d[16];
+#pragma unroll
+            for (int q = 0; q < 16; q++) d[q] = __builtin_bit_cast(unsigned, halfx2{h[2 * q], h[2 * q + 1]});
+            uint4 *o = dst + ((size_t)b * rows + r) * 4;
+#pragma unroll
+            for (int q = 0; q < 4; q++) o[q] = make_uint4(d[4 * q], d[4 * q + 1], d[4 * q + 2], d[4 * q + 3]);
+        } else {  // [hi | lo]: five uint2 (the B fragments are assembled from it)
+            unsigned d[10];
+#pragma unroll
+            for (int q = 0; q < 5; q++) {
+                d[q] = __builtin_bit_cast(unsigned, halfx2{hi[2 * q], hi[2 * q + 1]});
+                d[5 + q] = __builtin_bit_cast(unsigned, halfx2{lo[2 * q], lo[2 * q + 1]});
+            }
+            uint2 *o = (uint2 *)dst + ((size_t)b * rows + r) * 5;
+#pragma unroll
+            for (int q = 0; q < 5; q++) o[q] = make_uint2(d[2 * q], d[2 * q + 1]);
+        }
+    }
+}
+
 // ------------------------------------------------------------------ K = 8 (k = 7)
 //
 // Split-FP16 slice contraction.  One v_mfma_f32_16x16x32_f16 per 16 x 16 output tile holds
@@ -212,6 +263,12 @@ typedef const int __attribute__((address_space(4))) cint;
 __device__ inline uint4 ld16(const uint4 *p) {
     const u32x4 v = *(gu32x4 *)p;
     return make_uint4(v[0], v[1], v[2], v[3]);
+}
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef const u32x2 __attribute__((address_space(1))) gu32x2;
+__device__ inline uint2 ld8(const uint2 *p) {
+    const u32x2 v = *(gu32x2 *)p;
+    return make_uint2(v[0], v[1]);
 }
 // a wave-uniform exponent (scalar load; written by an earlier kernel, read-only here)
 __device__ inline int ld_exp(const int *p) { return *(cint *)p; }
@@ -391,125 +448,116 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
     }
 }
 
-// ------------------------------------------------------------------ K = 10 (k = 9)
+// ------------------------------------------------------------------ K = 10 (k = 9), split-FP16
 //
-// One workgroup (4 waves) per task; the waves share every slice.  The stacked
-// operators have 2K = 20 rows: M-tile 0 holds rows 0..15, M-tile 1 rows 16..19 at
-// A-operand rows 0, 4, 8, 12, so that its results sit in register 0 of every lane
-// (the other A rows are zero).  The contraction length 10 is three K-chunks of 4 (the
-// last half zero).  Operands and pass results are staged in LDS with the contracted
-// index innermost, padded to 12 with zeros (row stride 12 floats: the 16 columns of an
-// N-tile fall in distinct banks):
-//   F [n = 10 z + y][x]          f of the slice (double-buffered)
-//   L1[n2 = 20 z + a'][y]        pass x result, B operand of pass y
-//   L2[n3 = 20 b' + a'][z]       pass y result, B operand of pass z
-// Per pass a wave takes the N-tiles wave + 4t of both M-tiles (7, 13 and 25 N-tiles),
-// reads all their B fragments from LDS, then issues their MFMAs, then stores (or, in
-// pass z, adds them in FP64 to its accumulators: 4 per lane and M-tile-0 tile, 1 per
-// M-tile-1 tile).  Slice records are read 64 at a time into VGPRs (one per lane).
-constexpr int M10_LD = 12;
-constexpr int M10_F = 100 * M10_LD, M10_L1 = 200 * M10_LD, M10_L2 = 400 * M10_LD;
+// One v_mfma_f32_16x16x32_f16 holds a contraction of length 10 with split operands: K-slots
+// 0..9 carry (A_hi, B_hi), 10..19 (A_hi, B_lo), 20..29 (A_lo, B_hi), 30..31 zero (the lo lo
+// products lie below the split's own rounding).  Lane L supplies the 8 K-slots 8 kg ..
+// 8 kg + 7 (kg = L / 16): operator rows (A operands) are stored as [hi | hi | lo | 0 0], 32
+// halves, so that a lane's fragment is the 16-byte chunk kg of its row; input rows (B
+// operands, [hi | lo | hi | 0 0] in K-slot order) are stored as [hi | lo] and a lane
+// assembles its chunk from two 8-byte loads.
+// The stacked operators have 20 rows (the gt = 0 and gt = 1 blocks): M-tile 0 holds rows
+// 0..15, M-tile 1 rows 16..19 at A rows 0, 4, 8, 12 (its results in register 0 of every
+// lane).  Per pass, N-tile t of the B operand goes to wave t % 4:
+//   pass x  T1[a'][(z, y)]    A = X', B = f rows (z, y)                   7 N-tiles
+//   pass y  T2[b'][(z, a')]   A = Y', B = T1 columns (z, a') from LDS      13 N-tiles
+//   pass z  out[c'][(b', a')] A = Z', B = T2 columns (b', a') from LDS     25 N-tiles
+// The intermediates go to LDS as one dword per value (f16 hi in the low half, lo in the high
+// half), 18 dwords per column: the 10 values, the first 6 again, 2 unused (the stride spreads
+// the 16 columns of an N-tile over distinct banks).  Lane group kg reads its 8 K-slots as 8
+// consecutive dwords from position 0, 8, 6, 4 (kg = 0..3) and packs the hi or lo halves with
+// four v_perm.  In block-exponent units |T1| < 10 and |T2| < 10 |T1|: T1 is scaled by 2^10
+// and T2 (of the scaled T1) by 2^-1, which keeps both below the f16 maximum 65504 and their
+// lo halves (2^-11 of a value) in the f16 normal range for all but the smallest values.
+// Each wave accumulates its pass-z tiles over the task in FP64 with the weight of the output
+// component: w 2^-(e_x + e_y + e_z + e_f + 9), zero when the screening dropped the component.  Two barriers per slice order the
+// passes; one copy of each intermediate suffices (a pass rewrites a buffer only after every
+// wave has passed the barrier that follows its last read).
+constexpr int H10_COL = 18;
+constexpr int H10_P1 = 208 * H10_COL, H10_P2 = 400 * H10_COL;
 
-// f of a slice, four values per thread (1000 = 256 x 4 minus 24)
-__device__ inline void load_f10(double (&fv)[4], const Slice &sl, const double *inCoef, const double *genPool) {
-    const gdbl *f = slice_input(sl, inCoef, genPool, 1000);
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-        const int e = threadIdx.x + 256 * u;
-        fv[u] = e < 1000 ? f[e] : 0.0;
-    }
-}
-__device__ inline void store_f10(float *F, const double (&fv)[4]) {
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-        const int e = threadIdx.x + 256 * u;
-        if (e < 1000) F[(e / 10) * M10_LD + e % 10] = (float)fv[u];
-    }
+struct OpsH10 {
+    uint4 x0, x1, y0, y1, z0, z1;  // A fragments of X', Y', Z' (M-tiles 0 and 1)
+    uint4 f[2];                    // B fragments of f for the pass-x N-tiles wave, wave + 4
+};
+
+// the A fragment of one operator: M-tile 0 row m (block gt = m >= 10), M-tile 1 rows 16..19
+// at rows 0, 4, 8, 12 (zero elsewhere)
+__device__ inline void load_oper_h10(const uint4 *oper, int blk, int m, int kg, uint4 &t0, uint4 &t1) {
+    t0 = ld16(oper + (((size_t)blk + (m >= 10 ? 2 : 0)) * 10 + m % 10) * 4 + kg);
+    t1 = (m & 3) == 0 ? ld16(oper + (((size_t)blk + 2) * 10 + 6 + (m >> 2)) * 4 + kg) : make_uint4(0u, 0u, 0u, 0u);
 }
 
-// slice records of 64 consecutive slices, one per lane (two 16-byte halves)
-struct RecBatch {
-    uint4 a, b;
-    int base;
-};
-__device__ inline void rec_load(RecBatch &rb, const Slice *srec, int base, int count, int lane) {
-    const uint4 *p = (const uint4 *)(srec + min(base + lane, count - 1));
-    rb.a = p[0];
-    rb.b = p[1];
-    rb.base = base;
-}
-__device__ inline Slice rec_get(const RecBatch &rb, int j) {
-    const int l = j - rb.base;
-    Slice s;
-    s.w = __hiloint2double((int)__builtin_amdgcn_readlane(rb.a.y, l), (int)__builtin_amdgcn_readlane(rb.a.x, l));
-    s.f = (int)__builtin_amdgcn_readlane(rb.a.z, l);
-    s.xo = (int)__builtin_amdgcn_readlane(rb.a.w, l);
-    s.yo = (int)__builtin_amdgcn_readlane(rb.b.x, l);
-    s.zo = (int)__builtin_amdgcn_readlane(rb.b.y, l);
-    const unsigned z = __builtin_amdgcn_readlane(rb.b.z, l);
-    s.ft = (uint8_t)(z & 0xffu);
-    s.mask = (uint8_t)((z >> 8) & 0xffu);
-    return s;
-}
-struct Ops10c {
-    float x[2][3], y[2][3], z[2][3];
-};
-__device__ inline Ops10c load_ops10c(const Slice &sl, const float *B32, int m, int q) {
-    constexpr int K = 10, K2 = 100;
-    Ops10c o;
+__device__ inline OpsH10 load_opsh10(const Slice &sl, const Arr16 &J, const uint4 *gen16, int wave, int m, int kg) {
+    OpsH10 o;
+    load_oper_h10(J.oper, sl.xo / 100, m, kg, o.x0, o.x1);
+    load_oper_h10(J.oper, sl.yo / 100, m, kg, o.y0, o.y1);
+    load_oper_h10(J.oper, sl.zo / 100, m, kg, o.z0, o.z1);
+    // f rows [hi 10 | lo 10] (40 bytes, five uint2); lane group kg takes the dwords of its
+    // K-slots [hi | lo | hi | 0 0]: kg 0 bytes 0-15, kg 1 16-31, kg 2 32-39 then 0-7, kg 3
+    // 8-23 with its last dword zero
+    const uint2 *fb = (const uint2 *)(sl.f >= 0 ? J.in + ((size_t)sl.f * 8 + sl.ft) * 250
+                                                : gen16 + (size_t)(-sl.f - 1) * 250);
+    const int o1 = kg == 0 ? 0 : kg == 1 ? 2 : kg == 2 ? 4 : 1, o2 = kg == 0 ? 1 : kg == 1 ? 3 : kg == 2 ? 0 : 2;
 #pragma unroll
-    for (int mt = 0; mt < 2; mt++) {
-        const int row = mt == 0 ? m : ((m & 3) == 0 ? 16 + (m >> 2) : 99);
-        const bool rv = row < 20;
-        const int roff = rv ? (row / 10) * 2 * K2 + (row % 10) * K : 0;
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const int col = 4 * c + q;
-            const bool ok = rv && col < K;
-            o.x[mt][c] = ok ? ((const gflt *)B32)[sl.xo + roff + col] : 0.f;
-            o.y[mt][c] = ok ? ((const gflt *)B32)[sl.yo + roff + col] : 0.f;
-            o.z[mt][c] = ok ? ((const gflt *)B32)[sl.zo + roff + col] : 0.f;
-        }
+    for (int t = 0; t < 2; t++) {
+        const int r = min(16 * (wave + 4 * t) + m, 99);  // (rows beyond 99: results discarded)
+        const uint2 u = ld8(fb + (size_t)r * 5 + o1), v = ld8(fb + (size_t)r * 5 + o2);
+        o.f[t] = make_uint4(u.x, u.y, v.x, kg == 3 ? 0u : v.y);
     }
     return o;
 }
-// tiles t = 0..NT-1 of one M-tile: N-tile nt = wave + 4 t (< ntn), B fragments from LDS first
-template <int NT>
-__device__ inline void m10c_tiles(const float (&A)[3], const float *Lsrc, int nvalid_cols, int ntn, int wave, int m,
-                                  int q, floatx4 (&d)[NT]) {
-    float b[NT][3];
+
+// v = hi + lo as one dword (hi in the low half)
+__device__ inline unsigned pack_hl(float v) {
+    const _Float16 h = (_Float16)v;
+    const _Float16 l = (_Float16)(v - (float)h);
+    return __builtin_bit_cast(unsigned, halfx2{h, l});
+}
+// one value to its LDS column (and to the duplicate tail for positions below 6)
+__device__ inline void put_h10(unsigned *P, int col, int pos, float v) {
+    const unsigned d = pack_hl(v);
+    P[col * H10_COL + pos] = d;
+    if (pos < 6) P[col * H10_COL + 10 + pos] = d;
+}
+// the B fragment of column col for lane group kg (sel: the lane's v_perm selectors)
+__device__ inline uint4 get_h10(const unsigned *P, int col, int start, const unsigned (&sel)[4]) {
+    const unsigned *c = P + col * H10_COL + start;
+    unsigned d[8];
 #pragma unroll
-    for (int t = 0; t < NT; t++) {
-        const int col = 16 * (wave + 4 * t) + m;
-        const bool ok = wave + 4 * t < ntn && col < nvalid_cols;
-#pragma unroll
-        for (int c = 0; c < 3; c++) b[t][c] = ok ? Lsrc[col * M10_LD + 4 * c + q] : 0.f;
+    for (int p = 0; p < 4; p++) {
+        const uint2 v = *(const uint2 *)(c + 2 * p);
+        d[2 * p] = v.x;
+        d[2 * p + 1] = v.y;
     }
-#pragma unroll
-    for (int t = 0; t < NT; t++) d[t] = mfma0(A[0], b[t][0]);
-#pragma unroll
-    for (int c = 1; c < 3; c++)
-#pragma unroll
-        for (int t = 0; t < NT; t++) d[t] = mfma(A[c], b[t][c], d[t]);
+    return make_uint4(__builtin_amdgcn_perm(d[1], d[0], sel[0]), __builtin_amdgcn_perm(d[3], d[2], sel[1]),
+                      __builtin_amdgcn_perm(d[5], d[4], sel[2]), __builtin_amdgcn_perm(d[7], d[6], sel[3]));
 }
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
-k_contract_mfma10(int ntask_arg, const TaskSummary *sum, const SliceTask *tasks, const WorkItem *work,
-                  const JobTab *jobs, const Slice *slices, const double *genPool, double *nodeslots,
-                  long long *tabcnt) {
+k_contract_mfma10h(int ntask_arg, const TaskSummary *sum, const SliceTask *tasks, const WorkItem *work,
+                   const JobTab *jobs, const Slice *slices, const uint4 *gen16, const int *genExp,
+                   double *nodeslots, long long *tabcnt) {
     constexpr int K3 = 1000;
-    __shared__ __align__(16) float F[2][M10_F];
-    __shared__ __align__(16) float L1[M10_L1];
-    __shared__ __align__(16) float L2[M10_L2];
+    __shared__ __align__(16) unsigned P1[H10_P1];
+    __shared__ __align__(16) unsigned P2[H10_P2];
+    __shared__ double Wl[8];
     const int ntask = sum ? (sum->overflow ? 0 : sum->nstasks) : ntask_arg;
     if ((int)blockIdx.x >= ntask) return;
     const SliceTask task = tasks[blockIdx.x];
-    const int wave = wave_index(), lane = threadIdx.x & 63, m = lane & 15, q = lane >> 4;
+    const int wave = wave_index(), lane = threadIdx.x & 63, m = lane & 15, kg = lane >> 4;
     const WorkItem wi = work[task.g];
-    const JobTab &J = jobs[wi.job];
-    for (int e = threadIdx.x; e < 2 * M10_F; e += 256) (&F[0][0])[e] = 0.f;
-    for (int e = threadIdx.x; e < M10_L1; e += 256) L1[e] = 0.f;
-    for (int e = threadIdx.x; e < M10_L2; e += 256) L2[e] = 0.f;
+    const Arr16 J{jobs[wi.job].operBlocks16, jobs[wi.job].operExp16, jobs[wi.job].inCoef16, jobs[wi.job].inExp16};
+    // this lane's K-slot window in an LDS column and the halves it takes (hi = bytes 0-1,
+    // lo = bytes 2-3 of a dword; v_perm byte codes: first source bytes 4-7, second 0-3)
+    constexpr unsigned HH = 0x05040100u, LL = 0x07060302u, ZZ = 0x0c0c0c0cu;
+    const int start = kg == 0 ? 0 : kg == 1 ? 8 : kg == 2 ? 6 : 4;
+    unsigned sel[4];
+    sel[0] = kg == 2 ? LL : HH;
+    sel[1] = (kg == 1 || kg == 2) ? LL : HH;
+    sel[2] = kg == 1 ? LL : HH;
+    sel[3] = kg == 1 ? LL : kg == 3 ? ZZ : HH;
     double acc0[7][4], acc1[7];
 #pragma unroll
     for (int t = 0; t < 7; t++) {
@@ -519,102 +567,93 @@ k_contract_mfma10(int ntask_arg, const TaskSummary *sum, const SliceTask *tasks,
     }
     const Slice *srec = slices + task.start;
     const int last = task.count - 1;
-    Slice cs = slice_at(srec, 0);
-    double fv[4];
-    load_f10(fv, cs, J.inCoef, genPool);
-    __syncthreads();
-    store_f10(F[0], fv);
-    __syncthreads();
-    Ops10c cur = load_ops10c(cs, J.operBlocks32, m, q);
-    RecBatch rb;
-    rec_load(rb, srec, 0, task.count, lane);
-    Slice ns = rec_get(rb, min(1, last));
     int ntup = 0;
-    for (int it = 0; it < task.count; it++) {
-        float *Fc = F[it & 1];
-        const int j2 = min(it + 2, last);
-        if (j2 >= rb.base + 64) rec_load(rb, srec, j2, task.count, lane);
-        const Slice n2 = rec_get(rb, j2);
-        load_f10(fv, ns, J.inCoef, genPool);
-        const Ops10c nxt = load_ops10c(ns, J.operBlocks32, m, q);
+    auto contract = [&](const OpsH10 &cur, const Slice &cs) {
         ntup += __popc((unsigned)cs.mask);
-        {   // pass x: M-tile 0 (rows a' = 4q + r), M-tile 1 (a' = 16 + q in register 0); 7 N-tiles
-            floatx4 d0[2], d1[2];
-            m10c_tiles<2>(cur.x[0], Fc, 100, 7, wave, m, q, d0);
-            m10c_tiles<2>(cur.x[1], Fc, 100, 7, wave, m, q, d1);
+        // pass x
 #pragma unroll
-            for (int t = 0; t < 2; t++) {
-                const int n = 16 * (wave + 4 * t) + m;
-                if (wave + 4 * t < 7 && n < 100) {
-                    const int z = n / 10, y = n % 10;
+        for (int t = 0; t < 2; t++) {
+            const int tx = wave + 4 * t;
+            if (tx >= 7) break;
+            const floatx4 d0 = mfmah(cur.x0, cur.f[t]), d1 = mfmah(cur.x1, cur.f[t]);
+            const int r = 16 * tx + m;
+            if (r < 100) {
+                const int z = r / 10, y = r % 10;
 #pragma unroll
-                    for (int r = 0; r < 4; r++) L1[(20 * z + 4 * q + r) * M10_LD + y] = d0[t][r];
-                    L1[(20 * z + 16 + q) * M10_LD + y] = d1[t][0];
-                }
+                for (int i = 0; i < 4; i++) put_h10(P1, 20 * z + 4 * kg + i, y, d0[i] * 1024.f);
+                put_h10(P1, 20 * z + 16 + kg, y, d1[0] * 1024.f);
             }
         }
         __syncthreads();
-        store_f10(F[(it + 1) & 1], fv);
-        {   // pass y: 13 N-tiles (n2 = 20 z + a')
-            floatx4 d0[4], d1[4];
-            m10c_tiles<4>(cur.y[0], L1, 200, 13, wave, m, q, d0);
-            m10c_tiles<4>(cur.y[1], L1, 200, 13, wave, m, q, d1);
+        if (wave == 0 && lane < 8) {  // the weights of the 8 output components of this slice
+            const int gt = lane, gx = gt & 1, gy = (gt >> 1) & 1, gz = gt >> 2;
+            const int e = ld_exp(J.operExp + cs.xo / 100 + 2 * gx) + ld_exp(J.operExp + cs.yo / 100 + 2 * gy) +
+                          ld_exp(J.operExp + cs.zo / 100 + 2 * gz) +
+                          (cs.f >= 0 ? ld_exp(J.inExp + (size_t)cs.f * 8 + cs.ft) : ld_exp(genExp + (-cs.f - 1)));
+            Wl[gt] = ((cs.mask >> gt) & 1) ? ldexp(cs.w, -(e + 9)) : 0.0;
+        }
+        // pass y
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
-                const int n2 = 16 * (wave + 4 * t) + m;
-                if (wave + 4 * t < 13 && n2 < 200) {
-                    const int z = n2 / 20, ap = n2 % 20;
+        for (int t = 0; t < 4; t++) {
+            const int ty = wave + 4 * t;
+            if (ty >= 13) break;
+            const int n = 16 * ty + m;
+            const uint4 b = get_h10(P1, n, start, sel);
+            const floatx4 d0 = mfmah(cur.y0, b), d1 = mfmah(cur.y1, b);
+            if (n < 200) {
+                const int z = n / 20, ap = n % 20;
 #pragma unroll
-                    for (int r = 0; r < 4; r++) L2[(20 * (4 * q + r) + ap) * M10_LD + z] = d0[t][r];
-                    L2[(20 * (16 + q) + ap) * M10_LD + z] = d1[t][0];
-                }
+                for (int i = 0; i < 4; i++) put_h10(P2, 20 * (4 * kg + i) + ap, z, d0[i] * 0.5f);
+                put_h10(P2, 20 * (16 + kg) + ap, z, d1[0] * 0.5f);
             }
         }
         __syncthreads();
-        {   // pass z: 25 N-tiles (n3 = 20 b' + a'), FP64 accumulation
-            const int mk = cs.mask;
-            floatx4 d0[7];
-            m10c_tiles<7>(cur.z[0], L2, 400, 25, wave, m, q, d0);
+        // pass z: FP64 accumulation with the component weights
 #pragma unroll
-            for (int t = 0; t < 7; t++) {
-                const int n3 = 16 * (wave + 4 * t) + m;
-                const int gxy = (n3 % 20 >= 10 ? 1 : 0) | (n3 / 20 >= 10 ? 2 : 0);
+        for (int t = 0; t < 7; t++) {
+            const int tz = wave + 4 * t;
+            if (tz >= 25) break;
+            const int n = 16 * tz + m;
+            const uint4 b = get_h10(P2, n, start, sel);
+            const floatx4 d0 = mfmah(cur.z0, b), d1 = mfmah(cur.z1, b);
+            const int gxy = (n % 20 >= 10 ? 1 : 0) | (n / 20 >= 10 ? 2 : 0);
+            const double wlo = Wl[gxy], whi = Wl[gxy + 4];
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int gt = gxy | (4 * q + r >= 10 ? 4 : 0);
-                    const double ws = ((mk >> gt) & 1) ? cs.w : 0.0;
-                    acc0[t][r] = fma((double)d0[t][r], ws, acc0[t][r]);
-                }
-            }
-            floatx4 d1[7];
-            m10c_tiles<7>(cur.z[1], L2, 400, 25, wave, m, q, d1);
-#pragma unroll
-            for (int t = 0; t < 7; t++) {
-                const int n3 = 16 * (wave + 4 * t) + m;
-                const int gt = (n3 % 20 >= 10 ? 1 : 0) | (n3 / 20 >= 10 ? 2 : 0) | 4;
-                const double ws = ((mk >> gt) & 1) ? cs.w : 0.0;
-                acc1[t] = fma((double)d1[t][0], ws, acc1[t]);
-            }
+            for (int i = 0; i < 4; i++) acc0[t][i] = fma((double)d0[i], 4 * kg + i >= 10 ? whi : wlo, acc0[t][i]);
+            acc1[t] = fma((double)d1[0], whi, acc1[t]);
         }
-        cs = ns;
-        ns = n2;
-        cur = nxt;
+    };
+    // two operand sets in turn; the next slice's operands and the record after it are
+    // loaded while a slice is contracted
+    Slice cA = slice_at(srec, 0), cB;
+    OpsH10 opA = load_opsh10(cA, J, gen16, wave, m, kg), opB;
+    Slice rnext = slice_at(srec, min(1, last));
+    for (int it = 0; it < task.count;) {
+        cB = rnext;
+        opB = load_opsh10(cB, J, gen16, wave, m, kg);
+        rnext = slice_at(srec, min(it + 2, last));
+        contract(opA, cA);
+        if (++it >= task.count) break;
+        cA = rnext;
+        opA = load_opsh10(cA, J, gen16, wave, m, kg);
+        rnext = slice_at(srec, min(it + 2, last));
+        contract(opB, cB);
+        ++it;
     }
     if (threadIdx.x == 0) atomicAdd((unsigned long long *)&tabcnt[4 * wi.job + 2], (unsigned long long)ntup);
     double *dst = nodeslots + (size_t)task.slot * 8 * K3;
 #pragma unroll
     for (int t = 0; t < 7; t++) {
         if (wave + 4 * t >= 25) break;
-        const int n3 = 16 * (wave + 4 * t) + m;
-        const int bp = n3 / 20, ap = n3 % 20;
+        const int n = 16 * (wave + 4 * t) + m;
+        const int bp = n / 20, ap = n % 20;
         const int gxy = (ap >= 10 ? 1 : 0) | (bp >= 10 ? 2 : 0);
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int cp = 4 * q + r;
-            const int gt = gxy | (cp >= 10 ? 4 : 0);
-            dst[gt * K3 + ((cp % 10) * 10 + bp % 10) * 10 + ap % 10] = acc0[t][r];
+        for (int i = 0; i < 4; i++) {
+            const int cp = 4 * kg + i;
+            dst[(gxy | (cp >= 10 ? 4 : 0)) * K3 + ((cp % 10) * 10 + bp % 10) * 10 + ap % 10] = acc0[t][i];
         }
-        const int cp = 16 + q;
+        const int cp = 16 + kg;
         dst[(gxy | 4) * K3 + ((cp % 10) * 10 + bp % 10) * 10 + ap % 10] = acc1[t];
     }
 }
@@ -639,17 +678,21 @@ hipError_t launch_contract_mfma(hipStream_t s, int K, int ntask, const TaskSumma
         hipLaunchKernelGGL(k_contract_mfma8, dim3(nb), dim3(256), 0, s, ntask, sum, tasks, work, jobs, slices,
                            genPool16, genExp16, nodeslots, tabcnt);
     else
-        hipLaunchKernelGGL(k_contract_mfma10, dim3(nb), dim3(256), 0, s, ntask, sum, tasks, work, jobs, slices,
-                           genPool, nodeslots, tabcnt);
+        hipLaunchKernelGGL(k_contract_mfma10h, dim3(nb), dim3(256), 0, s, ntask, sum, tasks, work, jobs, slices,
+                           genPool16, genExp16, nodeslots, tabcnt);
     return hipGetLastError();
 }
 
 hipError_t launch_split16(hipStream_t s, int n, int first, const int *count, int bvals, int K, const double *src,
-                          uint4 *dst, int *exp) {
-    if (K != 8) return hipErrorInvalidValue;
+                          uint4 *dst, int *exp, int role) {
+    if (K != 8 && K != 10) return hipErrorInvalidValue;
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_split16, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, n, first, count, bvals, src, dst,
-                       exp);
+    if (K == 8)
+        hipLaunchKernelGGL(k_split16, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, n, first, count, bvals, src,
+                           dst, exp);
+    else
+        hipLaunchKernelGGL(k_split16_10, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, n, first, count, bvals,
+                           role, src, dst, exp);
     return hipGetLastError();
 }
 

@@ -227,8 +227,11 @@ def test_config5_full_step_mixed_within_prec(G):
 
 @pytest.mark.parametrize("basis,mu", [("L", 0.0), ("I", 1.3)])
 def test_mixed_k9_matches_fp64(G, basis, mu):
-    """The k=9 MFMA slice kernel (padded 20-row operator tiles) against the plain FP64 apply
-    on a small case: same grid and tuples, difference at FP32 rounding."""
+    """The k=9 split-FP16 MFMA slice kernel (20-row operator tiles) against the plain FP64
+    apply on a small case: same grid and tuples, difference at the rounding of the split
+    operands (measured 1.4e-4 prec for Legendre / Poisson and 6e-3 prec for Interpolating /
+    Helmholtz, spread evenly over the output components and depths; a wrong contraction is
+    off by O(||out||))."""
     prec = 1e-5
     g, _ = configs.density3(3)
     mra = G.mra_from(configs.BOX16, 9, basis)
@@ -238,7 +241,7 @@ def test_mixed_k9_matches_fp64(G, basis, mu):
     sa = G.apply(prec, a, op, f, flags=G.MWG_APPLY_MIXED)
     sb = G.apply(prec, b, op, f)
     assert sa["slices"] > 0 and sa["tuples"] == sb["tuples"] and sa["nodes_out"] == sb["nodes_out"]
-    assert l2_diff(G, a, b) < 1e-3 * prec * math.sqrt(b.getSquareNorm())
+    assert l2_diff(G, a, b) < 2e-2 * prec * math.sqrt(b.getSquareNorm())
 
 
 def test_mixed_is_refused_at_unbuilt_orders(G):
