@@ -93,10 +93,12 @@ def run_variant(G, mra, trees, ops, prec, flags, steps, warmup):
     tuples = 0
     tuples64 = 0
     slices = 0
+    dloop = 0
     t0 = time.perf_counter()
     for _ in range(steps):
         outs = [G.FunctionTree(mra) for _ in range(n)]
         st = G.apply_batch(prec, outs, ops, trees, flags=flags)
+        dloop += int(st[0]["device_loop"])
         nodes += sum(s["nodes_out"] for s in st)
         flops += sum(s["flops"] for s in st)
         tuples += sum(s["tuples"] for s in st)
@@ -108,7 +110,7 @@ def run_variant(G, mra, trees, ops, prec, flags, steps, warmup):
     return dict(ms_per_step=1e3 * dt / steps, nodes_per_step=nodes / steps, nodes_per_s=nodes / dt,
                 flops_per_step=flops / steps, tuples_per_step=tuples / steps, slices_per_step=slices / steps,
                 fp64_tuples_per_step=tuples64 / steps,
-                t_contract_per_step=t_con / steps, iters=st[0]["iters"],
+                t_contract_per_step=t_con / steps, iters=st[0]["iters"], device_loop_steps=dloop,
                 predicted=st[0]["predicted_splits"], wasted=st[0]["predicted_wasted"], outs=outs)
 
 
@@ -202,7 +204,8 @@ def exchange_run(args, G, configs, steps, warmup):
     res32 = run_variant(G, mra, rho, ops, prec, G.MWG_APPLY_MIXED, steps, warmup)
     errs = [_l2diff(G, a, b) / math.sqrt(b.getSquareNorm()) for a, b in zip(res32["outs"], res64["outs"])]
     log(f"[bench] exchange fp64 {res64['ms_per_step']:.1f} ms/step, mixed {res32['ms_per_step']:.1f} ms/step, "
-        f"max per-pair rel L2 err {max(errs):.2e}")
+        f"max per-pair rel L2 err {max(errs):.2e}, device loop {res64['device_loop_steps']}/{steps} (fp64) "
+        f"{res32['device_loop_steps']}/{steps} (mixed) steps")
     tc = res32["t_contract_per_step"]
     alg = res64["tuples_per_step"] * 6.0 * (k + 1) ** 4
     return dict(workload=f"exchange pair Poissons, 16 orbitals (136 pairs), k={k}, prec={prec}",
@@ -326,12 +329,13 @@ def scf_run(args, G, configs, steps, warmup, predict=True, cpu=True):
     alg = res64["tuples_per_step"] * 6.0 * K ** 4
     log(f"[bench] scf fp64: {res64['ms_per_step']:.0f} ms/step, {res64['nodes_per_s']:.0f} nodes/s, "
         f"iters {res64['iters']}, {res64['tuples_per_step']:.0f} tuples, contraction "
-        f"{1e3 * res64['t_contract_per_step']:.0f} ms/step")
+        f"{1e3 * res64['t_contract_per_step']:.0f} ms/step, device loop {res64['device_loop_steps']}/{steps} steps")
     res32 = run_variant(G, mra, inps, ops, prec, G.MWG_APPLY_MIXED, steps, warmup)
     err32 = max_rel_err(G, res32["outs"], res64["outs"])
     res32["outs"] = None
     log(f"[bench] scf mixed: {res32['ms_per_step']:.0f} ms/step, {res32['nodes_per_s']:.0f} nodes/s, "
-        f"contraction {1e3 * res32['t_contract_per_step']:.0f} ms/step, max rel L2 err vs fp64 {err32:.2e}")
+        f"contraction {1e3 * res32['t_contract_per_step']:.0f} ms/step, max rel L2 err vs fp64 {err32:.2e}, "
+        f"device loop {res32['device_loop_steps']}/{steps} steps")
     resP, errP = None, None
     if predict:
         pred, ntrain, t_train = train_predictor(G, configs, G.mra_from(configs.BOX20, 7, "L", max_depth=16),
@@ -450,7 +454,8 @@ def main():
     tune("fp64", G.MWG_APPLY_FP64)
     res64 = run_variant(G, mra, trees, ops, args.prec, G.MWG_APPLY_FP64, args.steps, args.warmup)
     log(f"[bench] fp64: {res64['ms_per_step']:.2f} ms/step, {res64['nodes_per_s']:.1f} nodes/s, "
-        f"contraction {1e3 * res64['t_contract_per_step']:.2f} ms/step")
+        f"contraction {1e3 * res64['t_contract_per_step']:.2f} ms/step, "
+        f"device loop {res64['device_loop_steps']}/{args.steps} steps")
     best, dtype = res64, "f64"
     res32 = None
     mixed_ok = args.k in (7, 9)
@@ -460,7 +465,8 @@ def main():
         err = max_rel_err(G, res32["outs"], res64["outs"])
         res32["rel_err_vs_fp64"] = err
         log(f"[bench] mixed: {res32['ms_per_step']:.2f} ms/step, {res32['nodes_per_s']:.1f} nodes/s, "
-            f"contraction {1e3 * res32['t_contract_per_step']:.2f} ms/step, rel L2 err vs fp64 {err:.2e}")
+            f"contraction {1e3 * res32['t_contract_per_step']:.2f} ms/step, rel L2 err vs fp64 {err:.2e}, "
+            f"device loop {res32['device_loop_steps']}/{args.steps} steps")
         if res32["nodes_per_s"] > res64["nodes_per_s"] and err < args.prec:
             best, dtype = res32, mixed_dtype(args.k)
     resP = None
@@ -550,6 +556,9 @@ def main():
             "plain_tuples_per_step": F, "input_nodes": in_nodes,
             "fp64_nodes_per_s": res64["nodes_per_s"], "speedup_vs_fp64": best["nodes_per_s"] / res64["nodes_per_s"],
             "fp64_ms_per_step": res64["ms_per_step"],
+            "device_loop_steps": {"fp64": res64["device_loop_steps"],
+                                  "mixed": res32["device_loop_steps"] if res32 else None,
+                                  "predict": resP["device_loop_steps"] if resP else None},
             "mixed_nodes_per_s": res32["nodes_per_s"] if res32 else None,
             "mixed_rel_l2_err_vs_fp64": res32["rel_err_vs_fp64"] if res32 else None,
             "mixed_tuples_per_step": res32["tuples_per_step"] if res32 else None,

@@ -74,6 +74,10 @@ typedef struct mwg_apply_stats {
     double flops;            /* flops issued: FP64 tuples * 6 (k+1)^4 + MFMA flops of the slices
                                 (f16 16x16x32 MFMAs as issued, padding included: k = 7 28 per slice,
                                 2*16*16*32*28, a skipped z half counted; k = 9 90, 2*16*16*32*90) */
+    int64_t device_loop;     /* 1: the refinement loop ran on the device (split tests and work lists
+                                without a host round trip); 0: on the host (first batch on a context,
+                                predictor, traces, maxIter, or a capacity the earlier batches did not
+                                reach) */
 } mwg_apply_stats;
 
 /* ---- context (mrenv / device) ------------------------------------------------ */

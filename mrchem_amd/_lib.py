@@ -22,7 +22,8 @@ class ApplyStats(C.Structure):
     _fields_ = [("nodes_out", C.c_int64), ("tuples", C.c_int64), ("tuples_fp64", C.c_int64),
                 ("gen_nodes", C.c_int64), ("iters", C.c_int64), ("slices", C.c_int64),
                 ("predicted_splits", C.c_int64), ("predicted_wasted", C.c_int64),
-                ("t_apply", C.c_double), ("t_contract", C.c_double), ("flops", C.c_double)]
+                ("t_apply", C.c_double), ("t_contract", C.c_double), ("flops", C.c_double),
+                ("device_loop", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -1,7 +1,7 @@
 """Autotuner for the contraction task granularity of the batched apply.
 
 The device apply splits each output node's tuple list into FP64 wave tasks of
-``fp64_chunk`` tuples and its FP32 MFMA slice list into tasks of ``slice_chunk``
+``fp64_chunk`` tuples and its MFMA slice list into tasks of ``slice_chunk``
 slices (0: sized from each refinement iteration's slice total); partial results of
 a node's tasks are summed in FP64 in a fixed order afterwards.  Small tasks spread a node over more waves (better occupancy
 for few, heavy nodes); large tasks cut partial-result traffic and reduction work.

@@ -219,15 +219,6 @@ struct Readback {
     }
 };
 
-// the candidate walk of an output node is split over up to ENUM_MAX_PARTS workgroups so
-// that a work list launches about ENUM_BLOCKS of them
-#ifndef ENUM_BLOCKS
-#define ENUM_BLOCKS 512
-#endif
-#ifndef ENUM_MAX_PARTS
-#define ENUM_MAX_PARTS 16
-#endif
-
 struct Ctx {
     int device = 0;
     std::atomic<int> refs{1};  // the owner's handle, plus one per MRA and predictor made on it
@@ -236,6 +227,21 @@ struct Ctx {
     Stage stage;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // device-resident refinement: high-water marks of the applies run on this context (work
+    // list size, generated nodes, output/input node ratio, raised FP64 and slice task sizes),
+    // which size its buffers, and its buffers, read-back entries and events
+    int hw_nw = 0, hw_gen = 0;
+    int hw_chunk[LOOP_MAX_DEPTH] = {}, hw_schunk[LOOP_MAX_DEPTH] = {};  // per refinement iteration
+    double hw_ratio = 0.0;
+    std::vector<int> last_in, last_out;  // input and output node counts of the last batch, per job
+    DBuf<WorkItem> lwork[2];
+    DBuf<int> lcidx, ltopo, lroots;
+    DBuf<LoopJob> ljobs;
+    DBuf<LoopCtl> lctl;
+    DBuf<char> ltab;
+    LoopIter *liter_h = nullptr;  // pinned
+    hipEvent_t evIter[LOOP_MAX_DEPTH] = {}, evTopo = nullptr;
+    hipEvent_t evc[2 * LOOP_MAX_DEPTH] = {};  // contraction time per iteration
     std::string err;
     DBuf<double> gx, gw;  // 24-point Gauss-Legendre (operator quadrature)
     void *dot_scratch = nullptr;  // this device's allocation-free dot scratch (device address)
@@ -453,6 +459,19 @@ struct TreeH {
     }
     int find(int d, int x, int y, int z) const {
         return index.get(node_key(d, x, y, z));
+    }
+    // the topology of m nodes built elsewhere (the device refinement loop), the roots first
+    void set_topology(int m, const int *dep, const int *par, const int *chi, const int *l3) {
+        const std::vector<int> r = roots;
+        clear();
+        roots = r;
+        reserve_host(m);
+        depth.assign(dep, dep + m);
+        parent.assign(par, par + m);
+        child.assign(chi, chi + m);
+        lr.assign(l3, l3 + 3 * (size_t)m);
+        cnorm.assign(8 * (size_t)m, 0.0);
+        for (int i = 0; i < m; i++) index.put(node_key(depth[i], lr[3 * i], lr[3 * i + 1], lr[3 * i + 2]), i);
     }
     // grow the coefficient and norm arrays through the context's block cache (the old
     // contents are copied on the stream; no synchronisation).  Growth is by half again, and
@@ -1111,7 +1130,380 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         if (C.tracing || use_pred) J.feat.assign((size_t)J.out->n() * MLP_FEATURES, 0.0f);
         J.has_feat.assign(J.out->n(), 0);
     }
-    while (true) {
+    // job table (and screening-box request) of job J at depth d, appended to jt and bjobs:
+    // operator level, input and output arrays, thresholds from the running norm
+    auto add_table = [&](JobH &J, int d, std::vector<JobTab> &jt, std::vector<BoxJob> &bjobs, int &ntabs,
+                         int &max_terms) {
+        J.op->ensure(d);
+        JobTab T;
+        std::memset(&T, 0, sizeof(T));
+        LevelH &LV = *J.op->lev[d];
+        const int nt = J.op->nterms();
+        T.thr = J.gsq > 0.0 ? SCREEN_REL * J.prec * std::sqrt(J.gsq / nt) : 0.0;
+        T.tab0 = ntabs;
+        T.ntab = nt;
+        BoxJob B;
+        std::memset(&B, 0, sizeof(B));
+        B.terms = LV.dterms.p;
+        B.env = LV.denv.p;
+        B.thr = T.thr;
+        for (int q = 0; q < 8; q++) B.F[q] = J.fmax[d][q];
+        B.nterms = nt;
+        B.depth = d;
+        B.tab0 = ntabs;
+        B.jt = (int)jt.size();
+        B.pre_off = ntabs + (int)jt.size();
+        bjobs.push_back(B);
+        ntabs += nt;
+        max_terms = std::max(max_terms, nt);
+        T.inHash = J.in->dhash();
+        T.genHash = DHash{C.genHashKeys.p + J.genOff, C.genHashVals.p + J.genOff, J.genMask};
+        T.inCoef = J.in->coef.p;
+        T.inNorm = J.in->norm.p;
+        T.operBlocks = LV.blocks.p;
+        if (mixed) {
+            if (!LV.has16) {
+                const size_t nb = LV.blocks.n / ((size_t)K * K);
+                LV.blocks16.alloc(nb * split16_block_uint4(K, 0, K * K));
+                LV.exp16.alloc(nb);
+                HC(launch_split16(s, (int)nb, 0, nullptr, K * K, K, LV.blocks.p, LV.blocks16.p, LV.exp16.p, 0));
+                LV.has16 = true;
+            }
+            J.in->ensure16();
+            T.operBlocks16 = LV.blocks16.p;
+            T.operExp16 = LV.exp16.p;
+            T.inCoef16 = J.in->coef16.p;
+            T.inExp16 = J.in->exp16.p;
+        }
+        T.operNorms = LV.norms.p;
+        T.outCoef = J.out->coef.p;
+        T.outNorm = J.out->norm.p;
+        for (int q = 0; q < 3; q++) T.lmax[q] = M.boxes[q] << d;
+        T.depth = d;
+        T.feat_innorm = std::sqrt(std::max(J.in->sqnorm, 0.0));
+        T.feat_tnorm = J.gsq > 0.0 ? std::sqrt(J.gsq) : T.feat_innorm;  // first iteration: input norm
+        T.prec = J.prec;
+        T.feat_mu = (float)std::log10(J.op->mu + 1e-3);
+        T.feat_cell = (float)std::log10(M.width(0) / M.boxes[0] * std::ldexp(1.0, -(d + 1)));
+        T.n0 = M.n0;
+        jt.push_back(T);
+    };
+    // ---- device-resident refinement.  The plain split test (no predictor, no traces, no
+    // iteration limit) runs on the device (k_loop_split, k_loop_emit), so an iteration is
+    // queued without waiting for the one before: the host reads each iteration's work-list
+    // size one iteration behind, only to stop queueing.  Capacities (work list, output
+    // nodes, generated nodes, schedule) come from the high-water marks of earlier applies on
+    // this context; when one is exceeded the batch runs again with the host loop below,
+    // which raises them.  The output trees are those of the host loop (same split test in
+    // the same order; the running norm sums the same squares in another order).
+    const int njobs = (int)jobs.size();
+    bool dl_done = false;
+    int dl_enq = 0;  // iterations queued by the device loop (their contraction events)
+    auto device_loop = [&]() -> bool {
+        const int nr = (int)jobs[0].out->roots.size();
+        const int WCAP = std::max(C.hw_nw + C.hw_nw / 4 + 64, njobs * nr);
+        // output node capacity per job: a batch like the last one (same size; SCF iterations,
+        // repeated steps) gets 1.25x the output of the job at the same position, scaled by
+        // the growth of its input; otherwise the largest output/input ratio seen, at most 2
+        const bool like_last = C.last_out.size() == (size_t)njobs;
+        const double ratio = std::min(2.0, std::max(0.5, 1.25 * C.hw_ratio));
+        std::vector<int> caps_j(njobs);
+        double need_bytes = 0.0;
+        for (int j = 0; j < njobs; j++) {
+            const int in_n = jobs[j].in->n();
+            const double est = like_last ? 1.25 * C.last_out[j] * std::max(1.0, (double)in_n / std::max(1, C.last_in[j]))
+                                         : ratio * in_n;
+            caps_j[j] = (int)std::min<double>((nr + std::ceil(est) + 64 + 7), (double)(INT_MAX / 16)) & ~7;
+            need_bytes += std::max(0.0, ((double)caps_j[j] * 8 * K3 - (double)jobs[j].out->coef.n) * sizeof(double));
+        }
+        {
+            // the reservation must fit comfortably (else the host loop, which grows as it goes)
+            size_t fr = 0, tot = 0;
+            if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+                (void)hipGetLastError();
+                return false;
+            }
+            if (need_bytes > 0.6 * ((double)fr + (double)C.cache.bytes)) return false;
+        }
+        std::vector<LoopJob> lj(njobs);
+        size_t topo = 0;
+        for (int j = 0; j < njobs; j++) {
+            const int cap = caps_j[j];
+            jobs[j].out->ensure_dev((size_t)cap);
+            std::memset(&lj[j], 0, sizeof(LoopJob));
+            lj[j].cap = cap;
+            lj[j].n = nr;
+            lj[j].w0 = j * nr;
+            lj[j].nwk = nr;
+            lj[j].gsq = -1.0;
+            lj[j].prec = jobs[j].prec;
+            topo += (size_t)cap;
+        }
+        C.ltopo.alloc(6 * topo);
+        {
+            size_t off = 0;
+            for (int j = 0; j < njobs; j++) {
+                int *base = C.ltopo.p + 6 * off;
+                const size_t cap = (size_t)lj[j].cap;
+                lj[j].depth = base;
+                lj[j].parent = base + cap;
+                lj[j].child = base + 2 * cap;
+                lj[j].lr = base + 3 * cap;
+                off += cap;
+            }
+        }
+        const int nseg = mixed ? 9 : 8;
+        const size_t eb = (size_t)ENUM_BLOCKS + (size_t)WCAP;  // >= work nodes x parts for any list <= WCAP
+        C.thread_off.alloc(eb * nseg * 256);
+        C.gtotal.alloc(eb * nseg);
+        C.gofs.alloc(eb * nseg);
+        C.segs.alloc((size_t)WCAP * 8);
+        C.segcnt.alloc((size_t)WCAP * nseg);
+        C.gslice.alloc((size_t)WCAP);
+        C.lwork[0].alloc((size_t)WCAP);
+        C.lwork[1].alloc((size_t)WCAP);
+        C.lcidx.alloc((size_t)WCAP);
+        C.ljobs.alloc((size_t)njobs);
+        C.lctl.alloc(1);
+        {
+            const int want = std::min(genCap, C.hw_gen + C.hw_gen / 4 + 4096);
+            if (genPoolCap < want) {
+                C.genPool.grow_keep((size_t)want * K3, s);
+                C.genNorm.grow_keep((size_t)want, s);
+                if (mixed) {
+                    C.genPool16.grow_keep((size_t)want * split16_block_uint4(K, 1, K3), s);
+                    C.genExp16.grow_keep((size_t)want, s);
+                }
+                genPoolCap = want;
+            }
+        }
+        // task sizes per iteration: as the host loop's, raised where it had to raise them
+        auto chunk_dl = [&](int d) { return std::max(C.chunk, C.hw_chunk[d]); };
+        auto schunk_dl = [&](int d) {
+            return K == 8 ? std::min(C.slice_chunk > 0 ? C.slice_chunk : MFMA8_TASK_DEFAULT, MFMA8_TASK_SLICES)
+                          : (C.slice_chunk > 0 ? C.slice_chunk : (C.hw_schunk[d] > 0 ? C.hw_schunk[d] : -2048));
+        };
+        C.tuples.alloc((size_t)std::max<long long>(C.caps.tuples, 1));
+        C.tasks.alloc((size_t)std::max(C.caps.tasks, 1));
+        C.reds.alloc((size_t)std::max(C.caps.reds, 1));
+        C.partials.alloc((size_t)std::max(C.caps.slots, 1) * K3);
+        if (mixed) {
+            C.slices.alloc((size_t)std::max<long long>(C.caps.slices, 1));
+            C.stasks.alloc((size_t)std::max(C.caps.stasks, 1));
+            C.nodeslots.alloc((size_t)std::max(C.caps.nnslots, 1) * 8 * K3);
+        }
+        // schedule summary | table counts | work-node norms
+        const size_t o_tab = 256;
+        const size_t o_wn = (o_tab + 4 * (size_t)njobs * sizeof(long long) + 255) & ~(size_t)255;
+        C.rbd.alloc(o_wn + (size_t)WCAP * 8 * sizeof(double));
+        TaskSummary *dsum = (TaskSummary *)C.rbd.p;
+        long long *dtab = (long long *)(C.rbd.p + o_tab);
+        double *dwn = (double *)(C.rbd.p + o_wn);
+        if (!C.liter_h) HC(hipHostMalloc((void **)&C.liter_h, sizeof(LoopIter) * LOOP_MAX_DEPTH, hipHostMallocDefault));
+        if (!C.evTopo) HC(hipEventCreateWithFlags(&C.evTopo, hipEventDisableTiming));
+        const int tcap = (int)std::min<size_t>(topo / 8 + 8, (size_t)INT_MAX);  // internal nodes <= nodes / 8
+        C.titems.alloc((size_t)tcap);
+        for (int q = 0; q < LOOP_MAX_DEPTH; q++) {
+            if (!C.evIter[q]) HC(hipEventCreateWithFlags(&C.evIter[q], hipEventDisableTiming));
+            if (!C.evc[2 * q]) HC(hipEventCreate(&C.evc[2 * q]));
+            if (!C.evc[2 * q + 1]) HC(hipEventCreate(&C.evc[2 * q + 1]));
+        }
+        {
+            LoopCtl ctl;
+            std::memset(&ctl, 0, sizeof(ctl));
+            ctl.nw[0] = njobs * nr;
+            for (int d = 0; d < LOOP_MAX_DEPTH; d++) ctl.splitfac[d] = std::pow(2.0, -0.5 * (M.n0 + d + 1));
+            upload_async(C, C.lctl, &ctl, 1);
+        }
+        upload_async(C, C.ljobs, lj.data(), lj.size());
+        upload_async(C, C.lroots, jobs[0].out->lr.data(), 3 * (size_t)nr);
+        HC(launch_loop_init(s, njobs, nr, C.ljobs.p, C.lroots.p, C.lwork[0].p));
+        char *ctlb = (char *)C.lctl.p;
+        int *gfirst = (int *)(ctlb + offsetof(LoopCtl, genFirst));
+        int *queue = (int *)(ctlb + offsetof(LoopCtl, queue));
+        for (int j = 0; j < njobs; j++) jobs[j].work.clear();
+        PT.lap("loop setup");
+        auto enqueue = [&](int d) {
+            std::vector<JobTab> jt;
+            std::vector<BoxJob> bj;
+            int ntabs = 0, max_terms = 0;
+            jt.reserve(njobs);
+            bj.reserve(njobs);
+            for (int j = 0; j < njobs; j++) add_table(jobs[j], d, jt, bj, ntabs, max_terms);
+            C.tpre.alloc((size_t)ntabs + njobs + 1);
+            C.tpre0.alloc((size_t)ntabs + njobs + 1);
+            for (int q = 0; q < njobs; q++) {
+                jt[q].pre = C.tpre.p + jt[q].tab0 + q;
+                jt[q].pre0 = C.tpre0.p + jt[q].tab0 + q;
+            }
+            const size_t o_bj = ((size_t)njobs * sizeof(JobTab) + 255) & ~(size_t)255;
+            std::vector<char> blob(o_bj + (size_t)njobs * sizeof(BoxJob));
+            std::memcpy(blob.data(), jt.data(), (size_t)njobs * sizeof(JobTab));
+            std::memcpy(blob.data() + o_bj, bj.data(), (size_t)njobs * sizeof(BoxJob));
+            upload_async(C, C.ltab, blob.data(), blob.size());
+            JobTab *djt = (JobTab *)C.ltab.p;
+            BoxJob *dbj = (BoxJob *)(C.ltab.p + o_bj);
+            C.tabs.alloc((size_t)std::max(ntabs, 1));
+            C.tbmax.alloc((size_t)std::max(njobs * max_terms, 1));
+            const int cur = d & 1;
+            const int *nwp = (const int *)(ctlb + offsetof(LoopCtl, nw)) + cur;
+            int *nwnext = (int *)(ctlb + offsetof(LoopCtl, nw)) + (cur ^ 1);
+            const WorkItem *dw = C.lwork[cur].p;
+            HC(launch_loop_begin(s, njobs, C.ljobs.p, djt, dbj, dtab, C.genCount.p, C.lctl.p));
+            HC(launch_job_boxes(s, njobs, max_terms, dbj, C.tabs.p, C.tbmax.p, djt, C.tpre.p, C.tpre0.p,
+                                C.genOverflow.p));
+            GenState gs{C.genKeys.p, C.genJob.p, C.genCount.p, genCap, C.genOverflow.p, C.genPool.p, C.genNorm.p,
+                        genPoolCap, dtab};
+            HC(launch_gen_mark(s, WCAP, dw, djt, C.tabs.p, gs, nwp, queue));
+            HC(launch_gen_compute(s, 0, genPoolCap, djt, gs, M.W.p, K, gfirst));
+            if (mixed)
+                HC(launch_split16(s, genPoolCap, 0, C.genCount.p, K3, K, C.genPool.p, C.genPool16.p, C.genExp16.p, 1,
+                                  gfirst));
+            if (!mixed)
+                HC(launch_enum(s, 0, WCAP, 0, dw, djt, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, nullptr, nullptr,
+                               nullptr, nullptr, nwp, queue + 1));
+            else
+                HC(launch_enum_mixed(s, 0, K, WCAP, 0, dw, djt, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, nullptr,
+                                     nullptr, nullptr, nullptr, nullptr, nullptr, nwp, queue + 1));
+            HC(launch_build_tasks(s, WCAP, nseg, 0, chunk_dl(d), schunk_dl(d), C.gtotal.p, C.caps, C.genCount.p, genPoolCap,
+                                  C.gofs.p, C.tasks.p, C.stasks.p, C.reds.p, dsum, C.segs.p, C.segcnt.p, C.gslice.p,
+                                  C.genOverflow.p, dw, dtab, njobs, nwp));
+            if (!mixed)
+                HC(launch_enum(s, 1, WCAP, 0, dw, djt, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, C.gofs.p, C.tuples.p,
+                               dsum, C.segs.p, nwp, queue + 2));
+            else
+                HC(launch_enum_mixed(s, 1, K, WCAP, 0, dw, djt, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, C.gofs.p,
+                                     C.tuples.p, C.slices.p, dsum, C.segs.p, C.gslice.p, nwp, queue + 2));
+            const int grid64 = std::max(1, (C.caps.tasks + 3) / 4);
+            const int gridred = std::min(4096, std::max(1, C.caps.reds));
+            HC(launch_contract_fp64(s, K, 0, dsum, grid64, C.tasks.p, dw, djt, C.tabs.p, C.tuples.p, C.genPool.p,
+                                    C.partials.p, C.evc[2 * d]));
+            if (mixed) {
+                const int tpb = mfma_tasks_per_block(K);
+                const int gridmf = std::max(1, (C.caps.stasks + tpb - 1) / tpb);
+                HC(launch_contract_mfma(s, K, 0, dsum, gridmf, C.stasks.p, dw, djt, C.tabs.p, C.slices.p, C.genPool.p,
+                                        C.nodeslots.p, dtab, C.genPool16.p, C.genExp16.p));
+            }
+            HC(launch_reduce_partials(s, 0, dsum, gridred, C.reds.p, dw, djt, C.partials.p,
+                                      mixed ? C.nodeslots.p : nullptr, K3, mixed ? dwn : nullptr, C.evc[2 * d + 1]));
+            if (!mixed) HC(launch_work_norms(s, WCAP, dw, djt, K3, dwn, nwp));
+            HC(launch_loop_split(s, njobs, C.ljobs.p, dw, dwn, M.maxDepth, C.lctl.p, C.lcidx.p, dtab));
+            HC(launch_loop_emit(s, njobs, C.ljobs.p, dw, nwp, C.lcidx.p, C.lwork[cur ^ 1].p, nwnext, WCAP, dsum,
+                                C.lctl.p, d, djt, C.titems.p, tcap));
+            HC(hipMemcpyAsync(&C.liter_h[d], ctlb + offsetof(LoopCtl, iter) + d * sizeof(LoopIter), sizeof(LoopIter),
+                              hipMemcpyDeviceToHost, s));
+            HC(hipEventRecord(C.evIter[d], s));
+        };
+        int d = 0, enq = 0, maxnw = njobs * nr;
+        bool ok = true;
+        enqueue(0);
+        enq = 1;
+        while (true) {
+            if (d + 1 <= M.maxDepth) {  // the next iteration, before this one is known to have children
+                enqueue(d + 1);
+                enq = d + 2;
+            }
+            hipError_t e;
+            while ((e = hipEventQuery(C.evIter[d])) == hipErrorNotReady) {
+            }
+            HC(e);
+            LoopIter it;
+            it.nw_next = *(volatile int *)&C.liter_h[d].nw_next;
+            it.overflow = *(volatile int *)&C.liter_h[d].overflow;
+            iter = d + 1;
+            if (it.overflow) {
+                ok = false;
+                break;
+            }
+            maxnw = std::max(maxnw, it.nw_next);
+            if (it.nw_next == 0 || d + 1 >= enq) break;
+            d++;
+        }
+        if (!ok) {
+            stream_wait(C);
+            C.stage.off = 0;
+            return false;
+        }
+        PT.lap("loop iterations");
+        // behind the last (empty) iteration: the topology read-back, then the transforms over
+        // the device's item lists (the host rebuilds its trees while the GPU transforms)
+        const size_t tb = 6 * topo * sizeof(int), jb = (size_t)njobs * sizeof(LoopJob);
+        C.rb.reserve(tb + jb + 64);
+        HC(hipMemcpyAsync(C.rb.h, C.ltopo.p, tb, hipMemcpyDeviceToHost, s));
+        HC(hipMemcpyAsync(C.rb.h + tb, C.ljobs.p, jb, hipMemcpyDeviceToHost, s));
+        HC(hipMemcpyAsync(C.rb.h + tb + jb, C.genCount.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        HC(hipEventRecord(C.evTopo, s));
+        {
+            std::vector<int> tn(d + 1), to(d + 2, 0);
+            for (int q = 0; q <= d; q++) {
+                tn[q] = C.liter_h[q].nw_next / 8;
+                to[q + 1] = to[q] + tn[q];
+            }
+            for (int q = 0; q <= d; q++) HC(launch_transform(s, 1, tn[q], C.titems.p + to[q], M.WT.p, K));
+            for (int q = d; q >= 0; q--) HC(launch_transform(s, 0, tn[q], C.titems.p + to[q], M.W.p, K));
+        }
+        dl_enq = enq;
+        {
+            hipError_t e;
+            while ((e = hipEventQuery(C.evTopo)) == hipErrorNotReady) {
+            }
+            HC(e);
+        }
+        C.stage.off = 0;
+        const LoopJob *hl = (const LoopJob *)(C.rb.h + tb);
+        if (enq - 1 > d && C.liter_h[enq - 1].overflow)  // (an iteration without work cannot overflow)
+            throw Error(MWG_ERR_HIP, "device refinement loop: overflow in an empty iteration");
+        int genNow = 0;
+        std::memcpy(&genNow, C.rb.h + tb + jb, sizeof(int));
+        std::vector<size_t> toff(njobs + 1, 0);
+        for (int j = 0; j < njobs; j++) toff[j + 1] = toff[j] + (size_t)lj[j].cap;
+        size_t nodes = 0;
+        for (int j = 0; j < njobs; j++) nodes += (size_t)hl[j].n;
+        auto rebuild = [&](size_t j) {
+            const int *base = (const int *)C.rb.h + 6 * toff[j];
+            const size_t cap = (size_t)lj[j].cap;
+            JobH &J = jobs[j];
+            J.out->set_topology(hl[j].n, base, base + cap, base + 2 * cap, base + 3 * cap);
+            J.tuples = hl[j].tuples;
+            J.tuples_fp64 = hl[j].tuples_fp64;
+            J.slices = hl[j].slices;
+            J.gen = hl[j].gen;
+            J.alive.assign(J.out->n(), 1);
+        };
+        if (nodes < 65536)  // (threads cost more than they save on small batches)
+            for (int j = 0; j < njobs; j++) rebuild((size_t)j);
+        else
+            for_each_job((size_t)njobs, rebuild);
+        C.hw_nw = std::max(C.hw_nw, maxnw);
+        C.hw_gen = std::max(C.hw_gen, genNow);
+        for (JobH &J : jobs) C.hw_ratio = std::max(C.hw_ratio, (double)J.out->n() / std::max(1, J.in->n()));
+        genDone = genNow;
+        PT.lap("loop read-back");
+        return true;
+    };
+    if (!use_pred && !C.tracing && max_iter < 0 && C.hw_nw > 0 && njobs <= LOOP_MAX_JOBS &&
+        M.maxDepth < LOOP_MAX_DEPTH && std::getenv("MWG_HOST_LOOP") == nullptr) {
+        dl_done = device_loop();
+        if (!dl_done) {
+            // a capacity was too small: the host loop from the roots, from a clean state
+            for (JobH &J : jobs) {
+                J.out->make_roots();
+                J.work = J.out->roots;
+                J.sNorm = J.wNorm = 0.0;
+                J.gsq = -1.0;
+                J.tuples = J.tuples_fp64 = J.slices = J.gen = 0;
+                J.spec.assign(J.out->n(), 0);
+                J.alive.assign(J.out->n(), 1);
+                J.has_feat.assign(J.out->n(), 0);
+            }
+            HC(hipMemsetAsync(C.genHashKeys.p, 0, genTotal * sizeof(unsigned long long), s));
+            HC(hipMemsetAsync(C.genCount.p, 0, sizeof(int), s));
+            genDone = genLastNew = 0;
+            iter = 0;
+            t_contract_ms = 0.0f;
+        }
+    }
+    while (!dl_done) {
         std::vector<int> active;
         for (int j = 0; j < (int)jobs.size(); j++)
             if (!jobs[j].work.empty()) active.push_back(j);
@@ -1132,64 +1524,8 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             depths.erase(std::unique(depths.begin(), depths.end()), depths.end());
             std::vector<int> tab_of(M.maxDepth + 2, -1);
             for (int d : depths) {
-                J.op->ensure(d);
-                JobTab T;
-                std::memset(&T, 0, sizeof(T));
-                {
-                    const LevelH &LV = *J.op->lev[d];
-                    const int nt = J.op->nterms();
-                    T.thr = J.gsq > 0.0 ? SCREEN_REL * J.prec * std::sqrt(J.gsq / nt) : 0.0;
-                    T.tab0 = ntabs;
-                    T.ntab = nt;
-                    BoxJob B;
-                    std::memset(&B, 0, sizeof(B));
-                    B.terms = LV.dterms.p;
-                    B.env = LV.denv.p;
-                    B.thr = T.thr;
-                    for (int q = 0; q < 8; q++) B.F[q] = J.fmax[d][q];
-                    B.nterms = nt;
-                    B.depth = d;
-                    B.tab0 = ntabs;
-                    B.jt = (int)jt.size();
-                    B.pre_off = ntabs + (int)jt.size();
-                    bjobs.push_back(B);
-                    ntabs += nt;
-                    max_terms = std::max(max_terms, nt);
-                }
-                T.inHash = J.in->dhash();
-                T.genHash = DHash{C.genHashKeys.p + J.genOff, C.genHashVals.p + J.genOff, J.genMask};
-                T.inCoef = J.in->coef.p;
-                T.inNorm = J.in->norm.p;
-                LevelH &LV = *J.op->lev[d];
-                T.operBlocks = LV.blocks.p;
-                if (mixed) {
-                    if (!LV.has16) {
-                        const size_t nb = LV.blocks.n / ((size_t)K * K);
-                        LV.blocks16.alloc(nb * split16_block_uint4(K, 0, K * K));
-                        LV.exp16.alloc(nb);
-                        HC(launch_split16(s, (int)nb, 0, nullptr, K * K, K, LV.blocks.p, LV.blocks16.p, LV.exp16.p,
-                                          0));
-                        LV.has16 = true;
-                    }
-                    J.in->ensure16();
-                    T.operBlocks16 = LV.blocks16.p;
-                    T.operExp16 = LV.exp16.p;
-                    T.inCoef16 = J.in->coef16.p;
-                    T.inExp16 = J.in->exp16.p;
-                }
-                T.operNorms = LV.norms.p;
-                T.outCoef = J.out->coef.p;
-                T.outNorm = J.out->norm.p;
-                for (int q = 0; q < 3; q++) T.lmax[q] = M.boxes[q] << d;
-                T.depth = d;
-                T.feat_innorm = std::sqrt(std::max(J.in->sqnorm, 0.0));
-                T.feat_tnorm = J.gsq > 0.0 ? std::sqrt(J.gsq) : T.feat_innorm;  // first iteration: input norm
-                T.prec = J.prec;
-                T.feat_mu = (float)std::log10(J.op->mu + 1e-3);
-                T.feat_cell = (float)std::log10(M.width(0) / M.boxes[0] * std::ldexp(1.0, -(d + 1)));
-                T.n0 = M.n0;
                 tab_of[d] = (int)jt.size();
-                jt.push_back(T);
+                add_table(J, d, jt, bjobs, ntabs, max_terms);
                 jt_job.push_back(j);
             }
             J.wpos.assign(J.out->n(), -1);
@@ -1200,6 +1536,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             }
         }
         const int nw = (int)work.size();
+        C.hw_nw = std::max(C.hw_nw, nw);
         PT.lap("opers+boxes");
         // per-table prefix arrays (ntab + 1 entries each), filled on the device
         C.tpre.alloc((size_t)ntabs + jt.size() + 1);
@@ -1275,7 +1612,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         // run it again.
         const int nseg = mixed ? 9 : 8;
         // few output nodes (the roots): several workgroups share each node's candidates
-        const int nparts = std::max(1, std::min(ENUM_MAX_PARTS, (ENUM_BLOCKS + nw - 1) / nw));
+        const int nparts = enum_parts(nw);
         C.thread_off.alloc((size_t)nw * nseg * nparts * 256);
         C.gtotal.alloc((size_t)nw * nseg * nparts);
         C.gofs.alloc((size_t)nw * nseg * nparts);
@@ -1345,8 +1682,8 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             if (mixed) {
                 const int tpb = mfma_tasks_per_block(K);
                 const int gridmf = std::max(1, (C.caps.stasks + tpb - 1) / tpb);
-                HC(launch_contract_mfma(s, K, 0, dsum, gridmf, C.stasks.p, dwork, djobs, C.tabs.p,
-                                        C.slices.p, C.genPool.p, C.nodeslots.p, dtab, C.genPool16.p, C.genExp16.p));
+                HC(launch_contract_mfma(s, K, 0, dsum, gridmf, C.stasks.p, dwork, djobs, C.tabs.p, C.slices.p,
+                                        C.genPool.p, C.nodeslots.p, dtab, C.genPool16.p, C.genExp16.p));
             }
             // (mixed: the reduce items cover every output component and write the norms)
             HC(launch_reduce_partials(s, 0, dsum, gridred, C.reds.p, dwork, djobs, C.partials.p,
@@ -1418,6 +1755,10 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         }
         genLastNew = hsum.gen_count - genDone;
         genDone = hsum.gen_count;
+        if (iter < LOOP_MAX_DEPTH) {
+            C.hw_chunk[iter] = std::max(C.hw_chunk[iter], chunk_it);
+            if (K != 8 && C.slice_chunk <= 0 && schunk_it > 0) C.hw_schunk[iter] = std::max(C.hw_schunk[iter], schunk_it);
+        }
         if (PT.on) {
             // candidates walked by the work-list kernels: per work node, its table's prefix total
             std::vector<int> pre((size_t)ntabs + jt.size() + 1);
@@ -1555,6 +1896,16 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         PT.lap("split");
         iter++;
     }
+    if (!dl_done) {
+        C.hw_gen = std::max(C.hw_gen, genDone);
+        for (JobH &J : jobs) C.hw_ratio = std::max(C.hw_ratio, (double)J.out->n() / std::max(1, J.in->n()));
+    }
+    C.last_in.resize(jobs.size());
+    C.last_out.resize(jobs.size());
+    for (size_t j = 0; j < jobs.size(); j++) {
+        C.last_in[j] = jobs[j].in->n();
+        C.last_out[j] = jobs[j].out->n();
+    }
     // drop the speculative children that the norm test rejected
     for (JobH &J : jobs) {
         bool any_dead = false;
@@ -1562,35 +1913,38 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             if (!a) any_dead = true;
         if (any_dead) compact_tree(*J.out, J.alive);
     }
-    // top-down additive transform, then bottom-up compression, for all output trees
-    std::vector<std::vector<std::vector<int>>> L(jobs.size());
-    int maxd = 0;
-    for (size_t j = 0; j < jobs.size(); j++) {
-        L[j] = jobs[j].out->internal_by_depth();
-        maxd = std::max(maxd, (int)L[j].size());
+    // top-down additive transform, then bottom-up compression, for all output trees (the
+    // device loop has queued them already, over the item lists it wrote)
+    if (!dl_done) {
+        std::vector<std::vector<std::vector<int>>> L(jobs.size());
+        int maxd = 0;
+        for (size_t j = 0; j < jobs.size(); j++) {
+            L[j] = jobs[j].out->internal_by_depth();
+            maxd = std::max(maxd, (int)L[j].size());
+        }
+        // one item list for all levels (uploaded once); the same items serve both directions.
+        // The transforms also write the norms they change: the children's scaling norms
+        // (top-down; final for leaves) and all 8 norms of every parent (bottom-up)
+        std::vector<TransformItem> items;
+        std::vector<size_t> lofs(maxd + 1, 0);
+        for (int d = 0; d < maxd; d++) {
+            lofs[d] = items.size();
+            for (size_t j = 0; j < jobs.size(); j++)
+                if (d < (int)L[j].size())
+                    for (int p : L[j][d])
+                        items.push_back({jobs[j].out->coef.p, jobs[j].out->norm.p, p, jobs[j].out->child[p]});
+        }
+        lofs[maxd] = items.size();
+        PT.lap("transform prep");
+        if (!items.empty()) {
+            upload_async(C, C.titems, items.data(), items.size());
+            for (int d = 0; d < maxd; d++)
+                HC(launch_transform(s, 1, (int)(lofs[d + 1] - lofs[d]), C.titems.p + lofs[d], M.WT.p, K));
+            for (int d = maxd - 1; d >= 0; d--)
+                HC(launch_transform(s, 0, (int)(lofs[d + 1] - lofs[d]), C.titems.p + lofs[d], M.W.p, K));
+        }
+        PT.lap("transform launch");
     }
-    // one item list for all levels (uploaded once); the same items serve both directions.
-    // The transforms also write the norms they change: the children's scaling norms
-    // (top-down; final for leaves) and all 8 norms of every parent (bottom-up)
-    std::vector<TransformItem> items;
-    std::vector<size_t> lofs(maxd + 1, 0);
-    for (int d = 0; d < maxd; d++) {
-        lofs[d] = items.size();
-        for (size_t j = 0; j < jobs.size(); j++)
-            if (d < (int)L[j].size())
-                for (int p : L[j][d])
-                    items.push_back({jobs[j].out->coef.p, jobs[j].out->norm.p, p, jobs[j].out->child[p]});
-    }
-    lofs[maxd] = items.size();
-    PT.lap("transform prep");
-    if (!items.empty()) {
-        upload_async(C, C.titems, items.data(), items.size());
-        for (int d = 0; d < maxd; d++)
-            HC(launch_transform(s, 1, (int)(lofs[d + 1] - lofs[d]), C.titems.p + lofs[d], M.WT.p, K));
-        for (int d = maxd - 1; d >= 0; d--)
-            HC(launch_transform(s, 0, (int)(lofs[d + 1] - lofs[d]), C.titems.p + lofs[d], M.W.p, K));
-    }
-    PT.lap("transform launch");
     {
         size_t total = 0;
         for (JobH &J : jobs) total += 8 * (size_t)J.out->n();
@@ -1611,6 +1965,11 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             off += m;
             J.out->sqnorm_from_cnorm();
         }
+    }
+    for (int q = 0; q < dl_enq; q++) {  // the device loop's contraction time, per iteration
+        float ms = 0.0f;
+        HC(hipEventElapsedTime(&ms, C.evc[2 * q], C.evc[2 * q + 1]));
+        t_contract_ms += ms;
     }
     for (JobH &J : jobs) J.out->fit_dev();
     PT.lap("transforms");
@@ -1636,6 +1995,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         st.predicted_wasted = wasted;
         st.t_apply = t_total;
         st.t_contract = t_contract_ms * 1e-3;
+        st.device_loop = dl_done ? 1 : 0;
         st.flops = fl * (double)jobs[j].tuples_fp64 + slice_flops(K) * (double)jobs[j].slices;
     }
 }
@@ -1956,6 +2316,12 @@ static void release_ctx(Ctx *c) {
     (void)hipStreamSynchronize(c->stream);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (hipEvent_t e : c->evIter)
+        if (e) (void)hipEventDestroy(e);
+    if (c->evTopo) (void)hipEventDestroy(c->evTopo);
+    for (hipEvent_t e : c->evc)
+        if (e) (void)hipEventDestroy(e);
+    if (c->liter_h) (void)hipHostFree(c->liter_h);
     hipStream_t s = c->stream;
     delete static_cast<mwg_ctx_s *>(c);
     if (s) (void)hipStreamDestroy(s);

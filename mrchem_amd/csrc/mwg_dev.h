@@ -65,10 +65,10 @@ static_assert(sizeof(Tuple) == 16, "tuple record is 16 bytes");
 
 // one (g, f, ft, term) slice of the mixed-precision apply: all 8 output
 // components at once (mask: which gt are accumulated)
-// one FP32 MFMA slice (g, f, ft, term): every output component gt of node g from
-// component ft of input node f, with the operator blocks already resolved: xo, yo, zo
-// are the FP32 level-array offsets of the (gt_d = 0, ft_d) block of each dimension
-// (the gt_d = 1 block follows 2 (k+1)^2 floats later)
+// one MFMA slice (g, f, ft, term): every output component gt of node g from component ft
+// of input node f, with the operator blocks already resolved: xo, yo, zo are the offsets
+// (in block values) of the (gt_d = 0, ft_d) block of each dimension in the level's block
+// array (the gt_d = 1 block follows 2 (k+1)^2 values later)
 struct Slice {
     double w;      // term weight
     int f;         // input node index (>= 0) or -(generated index + 1)

@@ -105,6 +105,15 @@ struct BoxJob {
     int pre_off;  // first entry of this table's prefix arrays (tab0 + jt)
 };
 
+// the candidate walk of an output node is split over up to ENUM_MAX_PARTS workgroups so
+// that a work list of nw nodes launches about ENUM_BLOCKS of them (host and device agree)
+constexpr int ENUM_BLOCKS = 512, ENUM_MAX_PARTS = 16;
+__host__ __device__ inline int enum_parts(int nw) {
+    if (nw <= 0) return 1;
+    const int p = (ENUM_BLOCKS + nw - 1) / nw;
+    return p < 1 ? 1 : (p > ENUM_MAX_PARTS ? ENUM_MAX_PARTS : p);
+}
+
 struct GenState {
     uint64_t *keys;
     int *job;
@@ -142,11 +151,17 @@ hipError_t launch_project_hyd(hipStream_t s, int nw, const int *work, const int 
 // monotone norm envelopes, union boxes, and the table's FP64 routing bound
 hipError_t launch_job_boxes(hipStream_t s, int nbj, int max_terms, const BoxJob *bj, TermTab *tabs, double *tbmax,
                             JobTab *jobs, int *pre, int *pre0, int *gen_overflow);
+// Device work counts: the work-list kernels below take an optional nwp, the work list's
+// node count on the device (written by an earlier kernel of the stream); nw is then only
+// an upper bound for the launch, and the kernels loop over the nodes on a grid of at most
+// DEVCOUNT_GRID workgroups, taking them from the zeroed counter *queue when one is given.
+constexpr int DEVCOUNT_GRID = 2048;
 hipError_t launch_gen_mark(hipStream_t s, int nw, const WorkItem *work, const JobTab *jobs, const TermTab *tabs,
-                           const GenState &gs);
-// generated nodes first .. min(device count, cap) - 1 (cap = entries allocated in the pool)
+                           const GenState &gs, const int *nwp = nullptr, int *queue = nullptr);
+// generated nodes first .. min(device count, cap) - 1 (cap = entries allocated in the pool;
+// firstp: first read on the device)
 hipError_t launch_gen_compute(hipStream_t s, int first, int cap, const JobTab *jobs, const GenState &gs,
-                              const double *W, int K);
+                              const double *W, int K, const int *firstp = nullptr);
 // per tuple segment (output node, component): first tuple, count, first FP64 task, first
 // partial slot (-1: its single task writes the output directly)
 struct SegRec {
@@ -160,9 +175,10 @@ struct SegRec {
 // (node, segment, part))
 hipError_t launch_enum(hipStream_t s, int mode, int nw, int nparts, const WorkItem *work, const JobTab *jobs,
                        const TermTab *tabs, const GenState &gs, int *thread_off, int *seg_total,
-                       const long long *seg_ofs, Tuple *out, const TaskSummary *sum, const SegRec *segs);
+                       const long long *seg_ofs, Tuple *out, const TaskSummary *sum, const SegRec *segs,
+                       const int *nwp = nullptr, int *queue = nullptr);
 hipError_t launch_work_norms(hipStream_t s, int nw, const WorkItem *work, const JobTab *jobs, int K3,
-                             double *compact);
+                             double *compact, const int *nwp = nullptr);
 struct TransformItem {
     double *coef;
     double *norm;  // component norms of the tree (null: not updated)
@@ -192,15 +208,17 @@ hipError_t launch_enum_mixed(hipStream_t s, int mode, int K, int nw, int nparts,
                              const JobTab *jobs,
                              const TermTab *tabs, const GenState &gs, int *thread_off, int *seg_total,
                              const long long *seg_ofs, Tuple *out, Slice *sout, const TaskSummary *sum,
-                             const SegRec *segs, const long long *gslice);
+                             const SegRec *segs, const long long *gslice, const int *nwp = nullptr,
+                             int *queue = nullptr);
 // Split-FP16 copies of FP64 blocks (mwg_mfma.hip): block b (values src[b bvals ..]) as rows
 // of K values, scaled by 2^exp[b] so that the block's largest magnitude lies in [0.5, 1)
 // (exp 0 for a zero block); x 2^e = hi + lo up to about 2^-23 of it.  K = 8: each row
 // [hi 8 | lo 8] (two uint4).  K = 10, role 0 (operator blocks, A operands): each row
 // [hi 10 | hi 10 | lo 10 | 0 0] (four uint4); role 1 (input nodes, B operands): each row
-// [hi 10 | lo 10] (40 bytes).  With count, only blocks first .. min(*count, first + n) - 1.
+// [hi 10 | lo 10] (40 bytes).  With count, only blocks first .. min(*count, first + n) - 1;
+// with firstp, first is read on the device and the blocks are first .. min(*count, n) - 1.
 hipError_t launch_split16(hipStream_t s, int n, int first, const int *count, int bvals, int K, const double *src,
-                          uint4 *dst, int *exp, int role = 0);
+                          uint4 *dst, int *exp, int role = 0, const int *firstp = nullptr);
 // uint4 per block of bvals values of a split-FP16 copy
 inline size_t split16_block_uint4(int K, int role, int bvals) {
     return K == 8 ? (size_t)bvals / 4 : role == 0 ? (size_t)bvals / 10 * 4 : (size_t)bvals / 10 * 40 / 16;
@@ -232,7 +250,54 @@ hipError_t launch_build_tasks(hipStream_t s, int nw, int nseg, int nparts, int c
                               TaskCaps caps, const int *gen_count, int gen_pcap, long long *ofs, ContractTask *tasks,
                               SliceTask *stasks, ReduceItem *reds, TaskSummary *sum, SegRec *segs, int *segcnt,
                               long long *gslice, const int *gen_overflow, const WorkItem *work, long long *tabcnt,
-                              int ntab);
+                              int ntab, const int *nwp = nullptr);
+// ---- device-resident refinement (mwg_kernels.hip)
+// The plain method's refinement loop without a host round trip: every job of the batch has
+// one job table per iteration (table index = job index, all work nodes of iteration d at
+// depth d); after the contraction and the output norms, k_loop_split takes the split test
+// and appends the children to the output trees, k_loop_emit writes the next work list.
+constexpr int LOOP_MAX_DEPTH = 32, LOOP_MAX_JOBS = 4096;
+struct LoopJob {
+    int *depth, *parent, *child, *lr;  // output tree topology on the device (lr: 3 per node), cap nodes
+    int n, cap;                        // nodes so far, capacity
+    int w0, nwk;                       // this iteration's work nodes [w0, w0 + nwk) of the work list
+    int splits, pad;                   // nodes split by this iteration
+    double sNorm, wNorm, gsq;          // running output norm (gsq < 0: none yet)
+    double prec;
+    long long tuples, tuples_fp64, slices, gen;  // statistics (as mwg_apply_stats)
+};
+// what the host reads back after each iteration
+struct LoopIter {
+    int nw_next;   // work nodes of the next iteration (0: done, or stopped by an overflow)
+    int overflow;  // sticky: bit 0 schedule capacity, 1 generated-node pool, 2 node capacity,
+                   // 3 work-list capacity, 4 generated-node table, 5 transform-item capacity
+};
+struct LoopCtl {
+    int nw[2];     // work list sizes: iteration d reads nw[d & 1], writes nw[(d + 1) & 1]
+    int queue[4];  // work counters of the iteration's k_gen_mark, k_enum count and fill (zeroed)
+    int genFirst;  // generated-node counter at the start of the iteration
+    int overflow;  // sticky (LoopIter bits)
+    double splitfac[LOOP_MAX_DEPTH];  // 2^(-(n0 + d + 1) / 2) of the split test at depth d
+    LoopIter iter[LOOP_MAX_DEPTH];
+    int tofs[LOOP_MAX_DEPTH + 1];     // first transform item of the split nodes of depth d
+};
+// the roots (nr per job, translations rootlr, the same for every job) in the topology and
+// the first work list
+hipError_t launch_loop_init(hipStream_t s, int njobs, int nr, const LoopJob *lj, const int *rootlr, WorkItem *work);
+// thresholds of the iteration's tables from the running norms, table counts cleared,
+// generated-node range started
+hipError_t launch_loop_begin(hipStream_t s, int njobs, const LoopJob *lj, JobTab *jt, BoxJob *bj, long long *tabcnt,
+                             const int *gen_count, LoopCtl *ctl);
+// per job (one workgroup): running norm, split test of its work nodes, children appended
+// (cidx[i]: first child of work node i, -1 if not split)
+hipError_t launch_loop_split(hipStream_t s, int njobs, LoopJob *lj, const WorkItem *work, const double *wnorm,
+                             int max_depth, LoopCtl *ctl, int *cidx, const long long *tabcnt);
+// the next work list (and its size in *nw_next, the host's copy in ctl->iter[d]), and the
+// split nodes as transform items of depth d (from ctl->tofs[d], capacity tcap)
+hipError_t launch_loop_emit(hipStream_t s, int njobs, LoopJob *lj, const WorkItem *work, const int *nwp,
+                            const int *cidx, WorkItem *next, int *nw_next, int wcap, const TaskSummary *sum,
+                            LoopCtl *ctl, int d, const JobTab *jt, TransformItem *titems, int tcap);
+
 // refinement-predictor features of the 8 children of every work node (row i*8 + a, a the
 // child index), from the work nodes' norms just computed, their parents' norms and the
 // input tree around each child

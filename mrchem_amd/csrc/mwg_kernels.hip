@@ -559,9 +559,9 @@ __device__ inline void gen_insert(const JobTab &J, const GenState &gs, int job, 
 // Walks the (term, delta) candidates of the ft = 0 boxes; the translations that some
 // term needs are first collected in an LDS bitmap over the job's union box, so that
 // each input position is probed once (candidates of different terms share them).
-__global__ void __launch_bounds__(256) k_gen_mark(int nw, const WorkItem *work, const JobTab *jobs,
-                                                  const TermTab *tabs, GenState gs) {
-    const WorkItem wi = work[blockIdx.x];
+__device__ __forceinline__ void gen_mark_node(int item, const WorkItem *work, const JobTab *jobs,
+                                              const TermTab *tabs, const GenState &gs) {
+    const WorkItem wi = work[item];
     const JobTab &J = jobs[wi.job];
     __shared__ int start[257];
     __shared__ unsigned need[FCACHE / 32];
@@ -604,7 +604,42 @@ __global__ void __launch_bounds__(256) k_gen_mark(int nw, const WorkItem *work, 
         const uint64_t key = dnode_key(J.depth, wi.l[0] - dx, wi.l[1] - dy, wi.l[2] - dz);
         if (dhash_find(J.inHash, key) < 0) gen_insert(J, gs, wi.job, key);
     }
-    (void)nw;
+}
+
+// Items 0 .. total - 1 of a workgroup-per-item kernel: item blockIdx.x first, then
+// grid-stride, or (queue) the items beyond the grid taken one at a time from a zeroed
+// counter, so that workgroups that drew cheap items take more of them.  Workgroups without
+// an item leave at once; the others leave once the counter has passed total.
+template <class F>
+__device__ __forceinline__ void for_items(int total, int *queue, F f) {
+    if ((int)blockIdx.x >= total) return;
+    f((int)blockIdx.x);
+    if (total <= (int)gridDim.x) return;
+    __syncthreads();  // (LDS reused by the next item)
+    if (!queue) {
+        for (int b = blockIdx.x + gridDim.x; b < total; b += gridDim.x) {
+            f(b);
+            __syncthreads();
+        }
+        return;
+    }
+    __shared__ int s_item;
+    while (true) {
+        if (threadIdx.x == 0) s_item = (int)gridDim.x + atomicAdd(queue, 1);
+        __syncthreads();
+        const int b = s_item;
+        __syncthreads();
+        if (b >= total) break;
+        f(b);
+        __syncthreads();
+    }
+}
+
+// one workgroup per work node (nwp: the node count on the device)
+__global__ void __launch_bounds__(256) k_gen_mark(int nw, const int *nwp, int *queue, const WorkItem *work,
+                                                  const JobTab *jobs, const TermTab *tabs, GenState gs) {
+    const int n = nwp ? *nwp : nw;
+    for_items(n, queue, [&](int i) { gen_mark_node(i, work, jobs, tabs, gs); });
 }
 
 // Refines the node tensor in T (padded (2K)^3, all 8 components of an ancestor) down
@@ -755,13 +790,14 @@ __device__ void gen_one(int g, const JobTab *jobs, const GenState &gs, double *T
 // Generated nodes first .. min(count, cap) - 1, count read from the device counter
 // that k_gen_mark advanced (no host round trip); grid-stride over the range.
 template <int K>
-__global__ void __launch_bounds__(256) k_gen_compute(int first, int cap, const JobTab *jobs, GenState gs,
-                                                     const double *W) {
+__global__ void __launch_bounds__(256) k_gen_compute(int first, const int *firstp, int cap, const JobTab *jobs,
+                                                     GenState gs, const double *W) {
     using L = PadT<K>;
     constexpr int N = L::N;
     extern __shared__ double smem[];
     double *T = smem;
     double *Ws = smem + L::SIZE;
+    if (firstp) first = *firstp;
     const int end = min(*(volatile const int *)gs.count, cap);
     if (first + (int)blockIdx.x >= end) return;
     for (int e = threadIdx.x; e < N * N; e += blockDim.x) Ws[e] = W[e];
@@ -1079,12 +1115,10 @@ __global__ void __launch_bounds__(64) k_job_route(int nbj, const BoxJob *bjs, co
 //   mode 1 writes every tuple at the segment's start (from the schedule) + the part's
 //          offset inside the segment + thread offset + running index.
 template <int MODE>
-__global__ void __launch_bounds__(256) k_enum(int nw, int nparts, const WorkItem *work, const JobTab *jobs,
-                                              const TermTab *tabs, GenState gs, int *thread_off, int *seg_total,
-                                              const long long *seg_ofs, Tuple *out, const TaskSummary *sum,
-                                              const SegRec *segs) {
-    if (MODE == 1 && sum->overflow) return;
-    const int g = blockIdx.x / nparts, part = blockIdx.x % nparts;
+__device__ __forceinline__ void enum_node(int g, int part, int nparts, const WorkItem *work, const JobTab *jobs,
+                                          const TermTab *tabs, const GenState &gs, int *thread_off,
+                                          int *seg_total, const long long *seg_ofs, Tuple *out,
+                                          const SegRec *segs) {
     const WorkItem wi = work[g];
     const JobTab &J = jobs[wi.job];
     __shared__ int scan[8][256];
@@ -1152,7 +1186,20 @@ __global__ void __launch_bounds__(256) k_enum(int nw, int nparts, const WorkItem
         if (threadIdx.x == 255)
             for (int q = 0; q < 8; q++) seg_total[((size_t)g * 8 + q) * nparts + part] = scan[q][255];
     }
-    (void)nw;
+}
+
+// one workgroup per (work node, part) (grid-stride; nwp: the node count on the device, the
+// parts then follow from it as on the host)
+template <int MODE>
+__global__ void __launch_bounds__(256) k_enum(int nw, const int *nwp, int *queue, int nparts, const WorkItem *work,
+                                              const JobTab *jobs, const TermTab *tabs, GenState gs, int *thread_off,
+                                              int *seg_total, const long long *seg_ofs, Tuple *out,
+                                              const TaskSummary *sum, const SegRec *segs) {
+    if (MODE == 1 && sum->overflow) return;
+    const int n = nwp ? *nwp : nw, np = nwp ? enum_parts(n) : nparts;
+    for_items(n * np, queue, [&](int b) {
+        enum_node<MODE>(b / np, b % np, np, work, jobs, tabs, gs, thread_off, seg_total, seg_ofs, out, segs);
+    });
 }
 
 // ----------------------------------------------------------------- apply: predictor features
@@ -1308,7 +1355,8 @@ constexpr int BT_FEW = 32, BT_LIST = 2048, BT_TABS = 256;
 // (one workgroup: the pointers never alias, so that loads of later iterations can be
 // issued ahead of the stores of earlier ones)
 __global__ void __launch_bounds__(1024)
-    k_build_tasks(int nw, int nseg, int nparts, int chunk, int wchunk, const int *__restrict__ tot, TaskCaps caps,
+    k_build_tasks(int nw, const int *nwp, int nseg, int nparts, int chunk, int wchunk, const int *__restrict__ tot,
+                  TaskCaps caps,
                   const int *__restrict__ gen_count, int gen_pcap, long long *__restrict__ ofs,
                   ContractTask *__restrict__ tasks, SliceTask *__restrict__ stasks, ReduceItem *__restrict__ reds,
                   TaskSummary *__restrict__ sum, SegRec *__restrict__ segs, int *__restrict__ segcnt,
@@ -1317,6 +1365,10 @@ __global__ void __launch_bounds__(1024)
     __shared__ long long wsum6[17][6];
     __shared__ int big[BT_LIST];     // segment ids (>= 0) or -(node + 1) for slice runs
     __shared__ int nbig;
+    if (nwp) {  // the work list's size on the device (its parts as the enumeration derived them)
+        nw = *nwp;
+        nparts = enum_parts(nw);
+    }
     const bool mixed = nseg == 9;
     // FP64 tasks of a tuple segment with c tuples (plain: always one, even empty, so that
     // every output component is written; mixed: the reduce item writes the component)
@@ -1665,17 +1717,15 @@ __global__ void __launch_bounds__(256) k_reduce_partials(int n_arg, const TaskSu
 
 // Same candidates and bounds as k_enum.  A candidate (term, delta, ft) whose
 // surviving combos all stay below ROUTE_FP64 * thr becomes one slice (all allowed
-// gt, FP32 MFMA); otherwise its surviving combos become FP64 tuples.  Counters per
+// gt, split-FP16 MFMA); otherwise its surviving combos become FP64 tuples.  Counters per
 // thread: 8 tuple segments (per gt) + 1 slice segment; layout of thread_off and
 // seg_total: [g][9][...].
 template <int MODE>
-__global__ void __launch_bounds__(256) k_enum_mixed(int nw, int nparts, int K2, const WorkItem *work,
-                                                    const JobTab *jobs, const TermTab *tabs, GenState gs,
-                                                    int *thread_off, int *seg_total, const long long *seg_ofs,
-                                                    Tuple *out, Slice *sout, const TaskSummary *sum,
-                                                    const SegRec *segs, const long long *gslice) {
-    if (MODE == 1 && sum->overflow) return;
-    const int g = blockIdx.x / nparts, part = blockIdx.x % nparts;
+__device__ __forceinline__ void enum_mixed_node(int g, int part, int nparts, int K2, const WorkItem *work,
+                                                const JobTab *jobs, const TermTab *tabs, const GenState &gs,
+                                                int *thread_off, int *seg_total, const long long *seg_ofs,
+                                                Tuple *out, Slice *sout, const SegRec *segs,
+                                                const long long *gslice) {
     const WorkItem wi = work[g];
     const JobTab &J = jobs[wi.job];
     __shared__ int scan[9][256];
@@ -1755,14 +1805,27 @@ __global__ void __launch_bounds__(256) k_enum_mixed(int nw, int nparts, int K2, 
         if (threadIdx.x == 255)
             for (int q = 0; q < 9; q++) seg_total[((size_t)g * 9 + q) * nparts + part] = scan[q][255];
     }
-    (void)nw;
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_enum_mixed(int nw, const int *nwp, int *queue, int nparts, int K2,
+                                                    const WorkItem *work, const JobTab *jobs, const TermTab *tabs,
+                                                    GenState gs, int *thread_off, int *seg_total,
+                                                    const long long *seg_ofs, Tuple *out, Slice *sout,
+                                                    const TaskSummary *sum, const SegRec *segs,
+                                                    const long long *gslice) {
+    if (MODE == 1 && sum->overflow) return;
+    const int n = nwp ? *nwp : nw, np = nwp ? enum_parts(n) : nparts;
+    for_items(n * np, queue, [&](int b) {
+        enum_mixed_node<MODE>(b / np, b % np, np, K2, work, jobs, tabs, gs, thread_off, seg_total, seg_ofs, out,
+                              sout, segs, gslice);
+    });
 }
 
 
 // norms of the work nodes of all jobs, into their trees and into a compact array
-__global__ void k_work_norms(int nw, const WorkItem *work, const JobTab *jobs, int K3, double *compact) {
-    const int i = blockIdx.x;
-    if (i >= nw) return;
+__device__ __forceinline__ void work_norms_node(int i, const WorkItem *work, const JobTab *jobs, int K3,
+                                                double *compact) {
     const WorkItem wi = work[i];
     const JobTab &J = jobs[wi.job];
     const double *p = J.outCoef + (size_t)wi.node * 8 * K3;
@@ -1774,6 +1837,221 @@ __global__ void k_work_norms(int nw, const WorkItem *work, const JobTab *jobs, i
             const double v = sqrt(s);
             J.outNorm[(size_t)wi.node * 8 + c] = v;
             compact[(size_t)i * 8 + c] = v;
+        }
+    }
+}
+
+__global__ void k_work_norms(int nw, const int *nwp, const WorkItem *work, const JobTab *jobs, int K3,
+                             double *compact) {
+    const int n = nwp ? *nwp : nw;
+    for (int i = blockIdx.x; i < n; i += gridDim.x) work_norms_node(i, work, jobs, K3, compact);
+}
+
+// ----------------------------------------------------------------- apply: device-resident refinement
+
+__global__ void __launch_bounds__(256) k_loop_init(int njobs, int nr, const LoopJob *lj, const int *rootlr,
+                                                   WorkItem *work) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < njobs * nr; i += gridDim.x * blockDim.x) {
+        const int j = i / nr, r = i % nr;
+        const LoopJob &L = lj[j];
+        WorkItem wi;
+        wi.job = j;
+        wi.node = r;
+        wi.parent = -1;
+        L.depth[r] = 0;
+        L.parent[r] = -1;
+        L.child[r] = -1;
+        for (int q = 0; q < 3; q++) {
+            wi.l[q] = rootlr[3 * r + q];
+            L.lr[3 * r + q] = wi.l[q];
+        }
+        work[i] = wi;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_loop_begin(int njobs, const LoopJob *lj, JobTab *jt, BoxJob *bj,
+                                                    long long *tabcnt, const int *gen_count, LoopCtl *ctl) {
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < njobs; j += gridDim.x * blockDim.x) {
+        const double gsq = lj[j].gsq;
+        // the host loop's expression: SCREEN_REL prec sqrt(||g||^2 / terms), 0 before any norm
+        const double thr = gsq > 0.0 ? SCREEN_REL * lj[j].prec * sqrt(gsq / (double)jt[j].ntab) : 0.0;
+        jt[j].thr = thr;
+        bj[j].thr = thr;
+        for (int q = 0; q < 4; q++) tabcnt[4 * (size_t)j + q] = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->genFirst = *(volatile const int *)gen_count;
+        for (int q = 0; q < 4; q++) ctl->queue[q] = 0;
+    }
+}
+
+// One workgroup per job.  The running norm adds the squared norms of the job's work nodes
+// (scaling parts on the roots, wavelet parts everywhere; per thread over a contiguous run,
+// then a fixed reduction tree); the split test is the host's (split_check with the factor
+// 2^(-(n0 + d + 1) / 2) the host computed); the split nodes' children follow the node count in
+// work-list order, 8 each, as the host appends them.
+__global__ void __launch_bounds__(256) k_loop_split(int njobs, LoopJob *lj, const WorkItem *work, const double *wn,
+                                                    int max_depth, LoopCtl *ctl, int *cidx, const long long *tabcnt) {
+    const int j = blockIdx.x, tid = threadIdx.x;
+    if (j >= njobs) return;
+    LoopJob &L = lj[j];
+    __shared__ double rs[256], rw[256];
+    __shared__ int sc[256];
+    __shared__ double s_tn;
+    __shared__ int s_n;
+    if (tid == 0) {
+        L.tuples += tabcnt[4 * (size_t)j] + tabcnt[4 * (size_t)j + 2];
+        L.tuples_fp64 += tabcnt[4 * (size_t)j];
+        L.slices += tabcnt[4 * (size_t)j + 1];
+        L.gen += tabcnt[4 * (size_t)j + 3];
+    }
+    const int w0 = L.w0, nwk = L.nwk;
+    if (nwk <= 0) {
+        if (tid == 0) L.splits = 0;
+        return;
+    }
+    const int d = L.depth[work[w0].node];
+    const int per = (nwk + 255) / 256, i0 = min(nwk, tid * per), i1 = min(nwk, i0 + per);
+    double s = 0.0, w = 0.0;
+    for (int i = i0; i < i1; i++) {
+        const double *p = wn + (size_t)(w0 + i) * 8;
+        if (d == 0) s += p[0] * p[0];
+        for (int c = 1; c < 8; c++) w += p[c] * p[c];
+    }
+    rs[tid] = s;
+    rw[tid] = w;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (tid < off) {
+            rs[tid] += rs[tid + off];
+            rw[tid] += rw[tid + off];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        L.sNorm += rs[0];
+        L.wNorm += rw[0];
+        L.gsq = L.sNorm + L.wNorm;
+        s_tn = sqrt(L.gsq);
+        s_n = L.n;
+    }
+    __syncthreads();
+    const double thr = fmax(2.0 * 2.220446049250313e-16, L.prec * s_tn * ctl->splitfac[d]);
+    const bool can = d < max_depth;
+    auto splits_at = [&](int i) {
+        const double *p = wn + (size_t)(w0 + i) * 8;
+        double w2 = 0.0;
+        for (int c = 1; c < 8; c++) w2 += p[c] * p[c];
+        return can && sqrt(w2) > thr;
+    };
+    int cnt = 0;
+    for (int i = i0; i < i1; i++) cnt += splits_at(i) ? 1 : 0;
+    sc[tid] = cnt;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const int v = tid >= off ? sc[tid - off] : 0;
+        __syncthreads();
+        sc[tid] += v;
+        __syncthreads();
+    }
+    const int S = sc[255], n0 = s_n;
+    const bool fits = (long long)n0 + 8LL * S <= (long long)L.cap;
+    int r = sc[tid] - cnt;
+    for (int i = i0; i < i1; i++) {
+        const bool sp = fits && splits_at(i);
+        cidx[w0 + i] = sp ? n0 + 8 * r : -1;
+        if (!sp) continue;
+        const WorkItem wi = work[w0 + i];
+        const int c0 = n0 + 8 * r;
+        L.child[wi.node] = c0;
+        for (int a = 0; a < 8; a++) {
+            const size_t c = (size_t)c0 + a;
+            L.depth[c] = d + 1;
+            L.parent[c] = wi.node;
+            L.child[c] = -1;
+            L.lr[3 * c] = 2 * wi.l[0] + (a & 1);
+            L.lr[3 * c + 1] = 2 * wi.l[1] + ((a >> 1) & 1);
+            L.lr[3 * c + 2] = 2 * wi.l[2] + ((a >> 2) & 1);
+        }
+        r++;
+    }
+    if (tid == 0) {
+        if (!fits) atomicOr(&ctl->overflow, 4);
+        L.n = fits ? n0 + 8 * S : n0;
+        L.splits = fits ? S : 0;
+    }
+}
+
+// The next work list: job j's children start at 8 x (split nodes of the jobs before j),
+// its split nodes' children in work-list order (the order of the host loop).  Every
+// workgroup forms the same prefix over the jobs; workgroup 0 also records the jobs' ranges,
+// the list's size and the iteration's read-back entry.
+__global__ void __launch_bounds__(256) k_loop_emit(int njobs, LoopJob *lj, const WorkItem *work, const int *nwp,
+                                                   const int *cidx, WorkItem *next, int *nw_next, int wcap,
+                                                   const TaskSummary *sum, LoopCtl *ctl, int d, const JobTab *jt,
+                                                   TransformItem *titems, int tcap) {
+    __shared__ int base[LOOP_MAX_JOBS];
+    __shared__ int sc[256];
+    const int tid = threadIdx.x;
+    const int per = (njobs + 255) / 256, j0 = min(njobs, tid * per), j1 = min(njobs, j0 + per);
+    int c = 0;
+    for (int j = j0; j < j1; j++) c += lj[j].splits;
+    sc[tid] = c;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const int v = tid >= off ? sc[tid - off] : 0;
+        __syncthreads();
+        sc[tid] += v;
+        __syncthreads();
+    }
+    int b = sc[tid] - c;
+    for (int j = j0; j < j1; j++) {
+        base[j] = b;
+        b += lj[j].splits;
+    }
+    const long long total = sc[255];
+    __syncthreads();
+    // (bits of earlier kernels; the ones added here follow from the same inputs in every workgroup)
+    const int t0 = ctl->tofs[d];
+    int ovf = *(volatile int *)&ctl->overflow;
+    if (sum->overflow) ovf |= sum->overflow & 3;
+    if (sum->gen_overflow) ovf |= 16;
+    if (8 * total > (long long)wcap) ovf |= 8;
+    if ((long long)t0 + total > (long long)tcap) ovf |= 32;
+    if (blockIdx.x == 0) {
+        for (int j = tid; j < njobs; j += 256) {
+            lj[j].w0 = 8 * base[j];
+            lj[j].nwk = ovf ? 0 : 8 * lj[j].splits;
+        }
+        if (tid == 0) {
+            const int nn = ovf ? 0 : (int)(8 * total);
+            ctl->overflow = ovf;
+            *nw_next = nn;
+            ctl->iter[d].nw_next = nn;
+            ctl->iter[d].overflow = ovf;
+            ctl->tofs[d + 1] = t0 + (ovf ? 0 : (int)total);
+        }
+    }
+    if (ovf) return;
+    const int n = *nwp;
+    for (int i = blockIdx.x * 256 + tid; i < n; i += gridDim.x * 256) {
+        const int c0 = cidx[i];
+        if (c0 < 0) continue;
+        const WorkItem wi = work[i];
+        const int j = wi.job;
+        const int first = lj[j].n - 8 * lj[j].splits;  // the job's node count before this iteration
+        const int rank = (c0 - first) >> 3;
+        const int p = 8 * (base[j] + rank);
+        titems[t0 + base[j] + rank] = TransformItem{jt[j].outCoef, jt[j].outNorm, wi.node, c0};
+        for (int a = 0; a < 8; a++) {
+            WorkItem ch;
+            ch.job = j;
+            ch.node = c0 + a;
+            ch.l[0] = 2 * wi.l[0] + (a & 1);
+            ch.l[1] = 2 * wi.l[1] + ((a >> 1) & 1);
+            ch.l[2] = 2 * wi.l[2] + ((a >> 2) & 1);
+            ch.parent = wi.node;
+            next[p + a] = ch;
         }
     }
 }
@@ -2073,9 +2351,10 @@ hipError_t launch_job_boxes(hipStream_t s, int nbj, int max_terms, const BoxJob 
 }
 
 hipError_t launch_gen_mark(hipStream_t s, int nw, const WorkItem *work, const JobTab *jobs, const TermTab *tabs,
-                           const GenState &gs) {
+                           const GenState &gs, const int *nwp, int *queue) {
     if (nw <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_gen_mark, dim3(nw), dim3(256), 0, s, nw, work, jobs, tabs, gs);
+    const int nb = nwp ? std::min(nw, DEVCOUNT_GRID) : nw;
+    hipLaunchKernelGGL(k_gen_mark, dim3(nb), dim3(256), 0, s, nw, nwp, queue, work, jobs, tabs, gs);
     return hipGetLastError();
 }
 
@@ -2083,36 +2362,71 @@ hipError_t launch_gen_mark(hipStream_t s, int nw, const WorkItem *work, const Jo
 static size_t padded_smem(int K) { return (size_t)(2 * K * 2 * K * (2 * K + 1) + 4 * K * K) * sizeof(double); }
 
 hipError_t launch_gen_compute(hipStream_t s, int first, int cap, const JobTab *jobs, const GenState &gs,
-                              const double *W, int K) {
+                              const double *W, int K, const int *firstp) {
     if (cap <= first) return hipSuccess;
     const size_t sm = padded_smem(K);
     const int nb = std::min(cap - first, 2048);
     MWG_DISPATCH_K(K, {
         hipError_t e = allow_smem(k_gen_compute<KK>, sm);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_gen_compute<KK>, dim3(nb), dim3(256), sm, s, first, cap, jobs, gs, W);
+        hipLaunchKernelGGL(k_gen_compute<KK>, dim3(nb), dim3(256), sm, s, first, firstp, cap, jobs, gs, W);
     });
     return hipGetLastError();
 }
 
 hipError_t launch_enum(hipStream_t s, int mode, int nw, int nparts, const WorkItem *work, const JobTab *jobs,
                        const TermTab *tabs, const GenState &gs, int *thread_off, int *seg_total,
-                       const long long *seg_ofs, Tuple *out, const TaskSummary *sum, const SegRec *segs) {
+                       const long long *seg_ofs, Tuple *out, const TaskSummary *sum, const SegRec *segs,
+                       const int *nwp, int *queue) {
     if (nw <= 0) return hipSuccess;
-    const unsigned nb = (unsigned)nw * (unsigned)nparts;
+    const unsigned nb = nwp ? (unsigned)DEVCOUNT_GRID : (unsigned)nw * (unsigned)nparts;
     if (mode == 0)
-        hipLaunchKernelGGL(k_enum<0>, dim3(nb), dim3(256), 0, s, nw, nparts, work, jobs, tabs, gs, thread_off,
-                           seg_total, seg_ofs, out, sum, segs);
+        hipLaunchKernelGGL(k_enum<0>, dim3(nb), dim3(256), 0, s, nw, nwp, queue, nparts, work, jobs, tabs, gs,
+                           thread_off, seg_total, seg_ofs, out, sum, segs);
     else
-        hipLaunchKernelGGL(k_enum<1>, dim3(nb), dim3(256), 0, s, nw, nparts, work, jobs, tabs, gs, thread_off,
-                           seg_total, seg_ofs, out, sum, segs);
+        hipLaunchKernelGGL(k_enum<1>, dim3(nb), dim3(256), 0, s, nw, nwp, queue, nparts, work, jobs, tabs, gs,
+                           thread_off, seg_total, seg_ofs, out, sum, segs);
     return hipGetLastError();
 }
 
 hipError_t launch_work_norms(hipStream_t s, int nw, const WorkItem *work, const JobTab *jobs, int K3,
-                             double *compact) {
+                             double *compact, const int *nwp) {
     if (nw <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_work_norms, dim3(nw), dim3(64), 0, s, nw, work, jobs, K3, compact);
+    const int nb = nwp ? std::min(nw, DEVCOUNT_GRID) : nw;
+    hipLaunchKernelGGL(k_work_norms, dim3(nb), dim3(64), 0, s, nw, nwp, work, jobs, K3, compact);
+    return hipGetLastError();
+}
+
+hipError_t launch_loop_init(hipStream_t s, int njobs, int nr, const LoopJob *lj, const int *rootlr, WorkItem *work) {
+    const int n = njobs * nr;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_loop_init, dim3(std::min((n + 255) / 256, 1024)), dim3(256), 0, s, njobs, nr, lj, rootlr,
+                       work);
+    return hipGetLastError();
+}
+
+hipError_t launch_loop_begin(hipStream_t s, int njobs, const LoopJob *lj, JobTab *jt, BoxJob *bj, long long *tabcnt,
+                             const int *gen_count, LoopCtl *ctl) {
+    const int nb = std::max(1, std::min((njobs + 255) / 256, 64));
+    hipLaunchKernelGGL(k_loop_begin, dim3(nb), dim3(256), 0, s, njobs, lj, jt, bj, tabcnt, gen_count, ctl);
+    return hipGetLastError();
+}
+
+hipError_t launch_loop_split(hipStream_t s, int njobs, LoopJob *lj, const WorkItem *work, const double *wnorm,
+                             int max_depth, LoopCtl *ctl, int *cidx, const long long *tabcnt) {
+    if (njobs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_loop_split, dim3(njobs), dim3(256), 0, s, njobs, lj, work, wnorm, max_depth, ctl, cidx,
+                       tabcnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_loop_emit(hipStream_t s, int njobs, LoopJob *lj, const WorkItem *work, const int *nwp,
+                            const int *cidx, WorkItem *next, int *nw_next, int wcap, const TaskSummary *sum,
+                            LoopCtl *ctl, int d, const JobTab *jt, TransformItem *titems, int tcap) {
+    if (njobs <= 0 || njobs > LOOP_MAX_JOBS || d < 0 || d >= LOOP_MAX_DEPTH) return hipErrorInvalidValue;
+    const int nb = std::max(1, std::min((wcap + 255) / 256, 1024));
+    hipLaunchKernelGGL(k_loop_emit, dim3(nb), dim3(256), 0, s, njobs, lj, work, nwp, cidx, next, nw_next, wcap, sum,
+                       ctl, d, jt, titems, tcap);
     return hipGetLastError();
 }
 
@@ -2131,8 +2445,8 @@ hipError_t launch_build_tasks(hipStream_t s, int nw, int nseg, int nparts, int c
                               TaskCaps caps, const int *gen_count, int gen_pcap, long long *ofs, ContractTask *tasks,
                               SliceTask *stasks, ReduceItem *reds, TaskSummary *sum, SegRec *segs, int *segcnt,
                               long long *gslice, const int *gen_overflow, const WorkItem *work, long long *tabcnt,
-                              int ntab) {
-    hipLaunchKernelGGL(k_build_tasks, dim3(1), dim3(1024), 0, s, nw, nseg, nparts, chunk, wchunk, tot, caps,
+                              int ntab, const int *nwp) {
+    hipLaunchKernelGGL(k_build_tasks, dim3(1), dim3(1024), 0, s, nw, nwp, nseg, nparts, chunk, wchunk, tot, caps,
                        gen_count, gen_pcap, ofs, tasks, stasks, reds, sum, segs, segcnt, gslice, gen_overflow, work,
                        tabcnt, ntab);
     return hipGetLastError();
@@ -2163,15 +2477,15 @@ hipError_t launch_reduce_partials(hipStream_t s, int n, const TaskSummary *sum, 
 hipError_t launch_enum_mixed(hipStream_t s, int mode, int K, int nw, int nparts, const WorkItem *work, const JobTab *jobs,
                              const TermTab *tabs, const GenState &gs, int *thread_off, int *seg_total,
                              const long long *seg_ofs, Tuple *out, Slice *sout, const TaskSummary *sum,
-                             const SegRec *segs, const long long *gslice) {
+                             const SegRec *segs, const long long *gslice, const int *nwp, int *queue) {
     if (nw <= 0) return hipSuccess;
-    const unsigned nb = (unsigned)nw * (unsigned)nparts;
+    const unsigned nb = nwp ? (unsigned)DEVCOUNT_GRID : (unsigned)nw * (unsigned)nparts;
     if (mode == 0)
-        hipLaunchKernelGGL(k_enum_mixed<0>, dim3(nb), dim3(256), 0, s, nw, nparts, K * K, work, jobs, tabs, gs,
-                           thread_off, seg_total, seg_ofs, out, sout, sum, segs, gslice);
+        hipLaunchKernelGGL(k_enum_mixed<0>, dim3(nb), dim3(256), 0, s, nw, nwp, queue, nparts, K * K, work, jobs,
+                           tabs, gs, thread_off, seg_total, seg_ofs, out, sout, sum, segs, gslice);
     else
-        hipLaunchKernelGGL(k_enum_mixed<1>, dim3(nb), dim3(256), 0, s, nw, nparts, K * K, work, jobs, tabs, gs,
-                           thread_off, seg_total, seg_ofs, out, sout, sum, segs, gslice);
+        hipLaunchKernelGGL(k_enum_mixed<1>, dim3(nb), dim3(256), 0, s, nw, nwp, queue, nparts, K * K, work, jobs,
+                           tabs, gs, thread_off, seg_total, seg_ofs, out, sout, sum, segs, gslice);
     return hipGetLastError();
 }
 

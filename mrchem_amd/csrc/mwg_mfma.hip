@@ -3,18 +3,20 @@
 // A slice (g, f, ft, term) produces all 8 output components gt of node g from input
 // component ft of node f: out_gt[c][b][a] += w * sum Z[c][l] Y[b][j] X[a][i] f[l][j][i],
 // with X = X_{gt_x, ft_x} etc.  Per dimension the two output choices gt_d = 0, 1 are
-// stacked into one 2K x K operator X' (row K gt_d + a), so a slice is three GEMMs.
-// k = 7: operands split into two FP16 halves each, products exact and sums FP32
-// (v_mfma_f32_16x16x32_f16); k = 9: FP32 operands and sums (v_mfma_f32_16x16x4_f32).  The
+// stacked into one 2K x K operator X' (row K gt_d + a), so a slice is three GEMMs.  The
+// operands are split into FP16 hi + lo halves (about 22 significand bits) and multiplied on
+// v_mfma_f32_16x16x32_f16, whose 32-long contraction holds the partial products hi hi,
+// hi lo, lo hi (and at k = 7 lo lo) of one row pair; products are exact, sums FP32.  The
 // slice result, scaled by the term weight and masked to the components whose tuples
-// survive the screening, is summed over the task's slices (k = 7: FP32 over at most 16
+// survive the screening, is summed over the task's slices (k = 7: FP32 over at most 64
 // slices per wave, then FP64 over the waves; k = 9: FP64 per slice), and a task's sum goes
 // to one node slot (8 K^3 FP64) that k_reduce_partials adds, in slot order, to the
 // node's FP64 tuple partials.  A task is a run of slices of one output node, one task
 // per 256-thread workgroup.
 //
-// Instruction layout of v_mfma_f32_16x16x4_f32 (D = A B + C): lane L supplies
-// A[L % 16][L / 16] and B[L / 16][L % 16]; D[i][j] is register i % 4 of lane j + 16 (i / 4).
+// Instruction layout of v_mfma_f32_16x16x32_f16 (D = A B + C): lane L supplies A[L % 16]
+// [8 (L / 16) .. 8 (L / 16) + 7] and B[8 (L / 16) .. + 7][L % 16]; D[i][j] is register i % 4
+// of lane j + 16 (i / 4).
 #include <hip/hip_ext.h>
 
 #include "mwg_kernels.h"
@@ -25,7 +27,6 @@ namespace {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef const double __attribute__((address_space(1))) gdbl;
-typedef const float __attribute__((address_space(1))) gflt;
 
 __device__ inline int wave_index() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 
@@ -58,13 +59,6 @@ __device__ inline Slice slice_at(const Slice *base, int idx) {
 __device__ inline const gdbl *slice_input(const Slice &sl, const double *inCoef, const double *genPool, int K3) {
     return sl.f >= 0 ? (const gdbl *)inCoef + (size_t)sl.f * 8 * K3 + sl.ft * K3
                      : (const gdbl *)genPool + (size_t)(-sl.f - 1) * K3;
-}
-
-__device__ inline floatx4 mfma(float a, float b, floatx4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-}
-__device__ inline floatx4 mfma0(float a, float b) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
 }
 
 // ------------------------------------------------------------------ split-FP16 copies
@@ -134,11 +128,8 @@ __device__ inline float wave_max(float v) {
 __device__ inline int scale_exp(float mx) { return mx > 0.f ? -(ilogbf(mx) + 1) : 0; }
 
 // one wave per block; rows of K = 8 values (16 halves: hi | lo = two uint4)
-__global__ void __launch_bounds__(256) k_split16(int n, int first, const int *count, int bvals, const double *src,
-                                                 uint4 *dst, int *exp) {
-    const int b = first + (int)blockIdx.x * 4 + wave_index(), lane = threadIdx.x & 63;
-    const int end = count ? min(*(volatile const int *)count, first + n) : first + n;
-    if (b >= end) return;
+__device__ __forceinline__ void split16_block(int b, int bvals, const double *src, uint4 *dst, int *exp) {
+    const int lane = threadIdx.x & 63;
     const double *v = src + (size_t)b * bvals;
     double mx = 0.0;
     for (int e = lane; e < bvals; e += 64) mx = fmax(mx, fabs(v[e]));
@@ -158,12 +149,21 @@ __global__ void __launch_bounds__(256) k_split16(int n, int first, const int *co
     }
 }
 
+// blocks first .. end - 1, one wave each, end = min(*count, first + n) (grid-stride).  With
+// firstp, first is read from the device and n is the absolute end (the pool's capacity).
+__global__ void __launch_bounds__(256) k_split16(int n, int first, const int *firstp, const int *count, int bvals,
+                                                 const double *src, uint4 *dst, int *exp) {
+    const int lim = firstp ? n : first + n;
+    if (firstp) first = *firstp;
+    const int end = count ? min(*(volatile const int *)count, lim) : lim;
+    for (int b = first + (int)blockIdx.x * 4 + wave_index(); b < end; b += (int)gridDim.x * 4)
+        split16_block(b, bvals, src, dst, exp);
+}
+
 // K = 10: one wave per block, one row of 10 values per lane -> 32 halves (four uint4)
-__global__ void __launch_bounds__(256) k_split16_10(int n, int first, const int *count, int bvals, int role,
-                                                    const double *src, uint4 *dst, int *exp) {
-    const int b = first + (int)blockIdx.x * 4 + wave_index(), lane = threadIdx.x & 63;
-    const int end = count ? min(*(volatile const int *)count, first + n) : first + n;
-    if (b >= end) return;
+__device__ __forceinline__ void split16_10_block(int b, int bvals, int role, const double *src, uint4 *dst,
+                                                 int *exp) {
+    const int lane = threadIdx.x & 63;
     const double *v = src + (size_t)b * bvals;
     double mx = 0.0;
     for (int e = lane; e < bvals; e += 64) mx = fmax(mx, fabs(v[e]));
@@ -207,6 +207,15 @@ __global__ void __launch_bounds__(256) k_split16_10(int n, int first, const int 
             for (int q = 0; q < 5; q++) o[q] = make_uint2(d[2 * q], d[2 * q + 1]);
         }
     }
+}
+
+__global__ void __launch_bounds__(256) k_split16_10(int n, int first, const int *firstp, const int *count, int bvals,
+                                                    int role, const double *src, uint4 *dst, int *exp) {
+    const int lim = firstp ? n : first + n;
+    if (firstp) first = *firstp;
+    const int end = count ? min(*(volatile const int *)count, lim) : lim;
+    for (int b = first + (int)blockIdx.x * 4 + wave_index(); b < end; b += (int)gridDim.x * 4)
+        split16_10_block(b, bvals, role, src, dst, exp);
 }
 
 // ------------------------------------------------------------------ K = 8 (k = 7)
@@ -684,15 +693,16 @@ hipError_t launch_contract_mfma(hipStream_t s, int K, int ntask, const TaskSumma
 }
 
 hipError_t launch_split16(hipStream_t s, int n, int first, const int *count, int bvals, int K, const double *src,
-                          uint4 *dst, int *exp, int role) {
+                          uint4 *dst, int *exp, int role, const int *firstp) {
     if (K != 8 && K != 10) return hipErrorInvalidValue;
     if (n <= 0) return hipSuccess;
+    // (a device first: at most 2048 workgroups, grid-stride)
+    const unsigned nb = (unsigned)std::min((n + 3) / 4, firstp ? 2048 : INT_MAX);
     if (K == 8)
-        hipLaunchKernelGGL(k_split16, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, n, first, count, bvals, src,
-                           dst, exp);
+        hipLaunchKernelGGL(k_split16, dim3(nb), dim3(256), 0, s, n, first, firstp, count, bvals, src, dst, exp);
     else
-        hipLaunchKernelGGL(k_split16_10, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, n, first, count, bvals,
-                           role, src, dst, exp);
+        hipLaunchKernelGGL(k_split16_10, dim3(nb), dim3(256), 0, s, n, first, firstp, count, bvals, role, src, dst,
+                           exp);
     return hipGetLastError();
 }
 

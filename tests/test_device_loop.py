@@ -1,0 +1,90 @@
+"""The device-resident refinement loop against the host loop.
+
+After a batch has run on a context, later batches take the split tests and the work
+lists on the device (k_loop_split, k_loop_emit) instead of one host round trip per
+refinement iteration.  The loop must produce the host loop's output trees: the same nodes
+in the same order, the same tuples, slices and iterations, and the same coefficients (the
+running output norm adds the same squares in another order, which changes no decision
+here).  A batch whose capacities exceed what the context has seen runs on the host loop
+from the start (device_loop = 0) with the same results.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from mrchem_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def G():
+    from mrchem_amd import mrcpp
+    mrcpp.default_context()
+    return mrcpp
+
+
+def _host_loop(fn):
+    os.environ["MWG_HOST_LOOP"] = "1"
+    try:
+        return fn()
+    finally:
+        del os.environ["MWG_HOST_LOOP"]
+
+
+def _same(a, b, rtol):
+    d1, l1, c1 = a.export()
+    d2, l2, c2 = b.export()
+    assert np.array_equal(d1, d2) and np.array_equal(l1, l2)
+    assert np.abs(c1 - c2).max() <= rtol * np.abs(c2).max()
+
+
+def _batch(G, k, prec, seed=3, n_orb=4):
+    orbs, nuc, mus = configs.orbitals(seed, n_orb=n_orb)
+    mra = G.mra_from(configs.BOX20, k)
+    trees = [G.project(prec, G.FunctionTree(mra), o) for o in orbs]
+    ops = [G.HelmholtzOperator(mra, mu, prec) for mu in mus]
+    return mra, trees, ops
+
+
+@pytest.mark.parametrize("k,mode", [(5, "fp64"), (7, "fp64"), (7, "mixed"), (9, "mixed")])
+def test_device_loop_matches_host_loop(G, k, mode):
+    prec = 1e-5
+    flags = G.MWG_APPLY_MIXED if mode == "mixed" else G.MWG_APPLY_FP64
+    mra, trees, ops = _batch(G, k, prec)
+    ref = [G.FunctionTree(mra) for _ in trees]
+    sh = _host_loop(lambda: G.apply_batch(prec, ref, ops, trees, flags=flags))
+    assert all(s["device_loop"] == 0 for s in sh)
+    for rep in range(2):  # twice: the device loop's buffers are reused
+        out = [G.FunctionTree(mra) for _ in trees]
+        sd = G.apply_batch(prec, out, ops, trees, flags=flags)
+        assert all(s["device_loop"] == 1 for s in sd)
+        for a, b, x, y in zip(out, ref, sd, sh):
+            for key in ("nodes_out", "iters", "tuples", "tuples_fp64", "slices", "gen_nodes"):
+                assert x[key] == y[key], (key, rep)
+            _same(a, b, 1e-13)
+
+
+def test_device_loop_falls_back_when_capacities_are_exceeded(G):
+    """A coarse batch first (small high-water marks), then the same trees at a tighter
+    precision: the output trees outgrow the capacities sized from the coarse batch, the
+    batch runs on the host loop and agrees with a host-loop run."""
+    mra, trees, ops = _batch(G, 7, 1e-6, seed=5, n_orb=3)
+    coarse = [G.FunctionTree(mra) for _ in trees]
+    G.apply_batch(1e-2, coarse, ops, trees)
+    out = [G.FunctionTree(mra) for _ in trees]
+    sd = G.apply_batch(1e-6, out, ops, trees)
+    # (oracle sizes of this case: 136, 128, 80 output nodes at 1e-2; 648, 672, 424 at 1e-6)
+    assert all(s["device_loop"] == 0 for s in sd)
+    ref = [G.FunctionTree(mra) for _ in trees]
+    sh = _host_loop(lambda: G.apply_batch(1e-6, ref, ops, trees))
+    for a, b, x, y in zip(out, ref, sd, sh):
+        assert x["nodes_out"] == y["nodes_out"] and x["tuples"] == y["tuples"]
+        _same(a, b, 1e-13)
+    # and the next batch of the same shape runs on the device
+    out2 = [G.FunctionTree(mra) for _ in trees]
+    s2 = G.apply_batch(1e-6, out2, ops, trees)
+    assert all(s["device_loop"] == 1 for s in s2)
+    for a, b in zip(out2, ref):
+        _same(a, b, 1e-13)

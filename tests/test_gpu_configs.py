@@ -2,7 +2,7 @@
 
 Each workload is checked against the FP64 oracle on a reduced set of applies (same
 tuples, same grid, coefficients to 1e-12), and the full batch through properties:
-the mixed-precision (FP32 MFMA, FP64 accumulation) and the predictor variants against
+the mixed-precision (split-FP16 MFMA, FP64 accumulation) and the predictor variants against
 the plain FP64 apply on the GPU, by the exact L2 norm of the difference of the two
 trees (computed in the multiwavelet basis on the union grid, mwg_add), per output.
 """
@@ -201,7 +201,7 @@ def test_config5_reduced_step_matches_oracle(G, oracle):
 
 def test_config5_full_step_mixed_within_prec(G):
     """The whole step (32 Helmholtz + 528 pair Poissons, k=9, prec 1e-6) as one mixed batch
-    (FP32 MFMA at k=9) against the FP64 batch: every output within 0.1 prec of FP64 (exact
+    (split-FP16 MFMA at k=9) against the FP64 batch: every output within 0.1 prec of FP64 (exact
     L2), the same grids and tuples."""
     prec = 1e-6
     mra = G.mra_from(configs.BOX20, 9)

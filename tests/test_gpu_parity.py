@@ -254,7 +254,7 @@ def _l2_diff(a, b, box_width, n=20000, seed=7):
 
 @pytest.mark.parametrize("i", [0, 1])
 def test_mixed_precision_within_prec_of_fp64(G, A, i):
-    """FP32 MFMA slices (FP32 sums of <= 16 slices per wave, FP64 beyond; k=7) vs the plain FP64 apply: the
+    """Split-FP16 MFMA slices (FP32 sums of <= 64 slices per wave, FP64 beyond; k=7) vs the plain FP64 apply: the
     output differs by less than prec * ||out|| (L2), and both meet the erfc closed form."""
     prec = 1e-6
     orbs, nuc, mus = configs.orbitals(0, n_orb=2)
@@ -275,7 +275,7 @@ def test_mixed_precision_within_prec_of_fp64(G, A, i):
 
 
 def test_mixed_precision_interpolating_basis(G):
-    """The FP32 MFMA path on the Interpolating basis (k=7): same grid as the plain FP64
+    """The MFMA path on the Interpolating basis (k=7): same grid as the plain FP64
     apply, difference below prec * ||out|| (L2)."""
     prec = 1e-6
     orbs, nuc, mus = configs.orbitals(0, n_orb=2)
