@@ -70,7 +70,8 @@ typedef struct mwg_apply_stats {
     int64_t predicted_splits;/* output nodes computed ahead of their parent's norm test (predictor) */
     int64_t predicted_wasted;/* of which discarded because the norm test did not split the parent */
     double t_apply;          /* seconds: work-list build + contraction + refine + transforms */
-    double t_contract;       /* seconds spent in contraction kernels (HIP events) */
+    double t_contract;       /* seconds spent in contraction kernels (device clock: first contraction
+                                workgroup to last reduction workgroup, summed over the iterations) */
     double flops;            /* flops issued: FP64 tuples * 6 (k+1)^4 + MFMA flops of the slices
                                 (f16 16x16x32 MFMAs as issued, padding included: k = 7 28 per slice,
                                 2*16*16*32*28, a skipped z half counted; k = 9 90, 2*16*16*32*90) */
@@ -213,7 +214,11 @@ int mwg_oper_blocks(mwg_oper oper, int term, int depth, int64_t dlo, int64_t dhi
 int mwg_apply(double prec, mwg_tree out, mwg_oper oper, mwg_tree inp, int max_iter, unsigned flags,
               mwg_apply_stats *stats);
 /* n independent applies (exchange pair Poissons, per-orbital Helmholtz) in one batch:
- * one work list and one contraction launch per refinement level for all of them. */
+ * one work list and one contraction launch per refinement level for all of them.  After a
+ * first batch on the context, the refinement levels (split tests, children, work lists) run
+ * on the device without host round trips, with capacities taken from the earlier batches;
+ * a batch that exceeds them runs with the host loop instead (same results;
+ * stats.device_loop tells which; MWG_HOST_LOOP=1 in the environment forces the host loop). */
 int mwg_apply_batch(int n, const double *prec, mwg_tree *out, mwg_oper *oper, mwg_tree *inp, int max_iter,
                     unsigned flags, mwg_apply_stats *stats);
 

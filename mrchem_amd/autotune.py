@@ -6,7 +6,7 @@ slices (0: sized from each refinement iteration's slice total); partial results 
 a node's tasks are summed in FP64 in a fixed order afterwards.  Small tasks spread a node over more waves (better occupancy
 for few, heavy nodes); large tasks cut partial-result traffic and reduction work.
 The best point depends on k, the precision and the tree shapes, so it is measured:
-``autotune`` times the contraction kernels (HIP events, ``t_contract``) of the
+``autotune`` times the contraction kernels (device clock, ``t_contract``) of the
 caller's own batch for each candidate and installs the fastest on the context.
 
 Candidates are applied to fresh output trees each time; the results agree to

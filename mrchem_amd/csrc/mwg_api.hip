@@ -226,7 +226,7 @@ struct Ctx {
     Readback rb;
     Stage stage;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double wall_khz = 0.0;  // device wall clock (contraction timestamps), ticks per ms
     // device-resident refinement: high-water marks of the applies run on this context (work
     // list size, generated nodes, output/input node ratio, raised FP64 and slice task sizes),
     // which size its buffers, and its buffers, read-back entries and events
@@ -239,9 +239,8 @@ struct Ctx {
     DBuf<LoopJob> ljobs;
     DBuf<LoopCtl> lctl;
     DBuf<char> ltab;
-    LoopIter *liter_h = nullptr;  // pinned
+    LoopIter *liter_h = nullptr, *liter_d = nullptr;  // pinned (coherent), and its device address
     hipEvent_t evIter[LOOP_MAX_DEPTH] = {}, evTopo = nullptr;
-    hipEvent_t evc[2 * LOOP_MAX_DEPTH] = {};  // contraction time per iteration
     std::string err;
     DBuf<double> gx, gw;  // 24-point Gauss-Legendre (operator quadrature)
     void *dot_scratch = nullptr;  // this device's allocation-free dot scratch (device address)
@@ -476,22 +475,23 @@ struct TreeH {
     // grow the coefficient and norm arrays through the context's block cache (the old
     // contents are copied on the stream; no synchronisation).  Growth is by half again, and
     // fit_dev trims a finished tree, so a tree holds at most about 1.5x its nodes' memory.
-    void grow_cached(DBuf<double> &b, size_t m) {
+    void grow_cached(DBuf<double> &b, size_t m, size_t valid) {
         if (m <= b.n && b.p) return;
         Ctx &C = *mra->ctx;
         const size_t want = std::max<size_t>(m, b.n + b.n / 2);
         size_t got = 0;
         double *q = (double *)C.cache.get(want * sizeof(double), got);
         if (b.p) {
-            HC(hipMemcpyAsync(q, b.p, b.n * sizeof(double), hipMemcpyDeviceToDevice, C.stream));
+            const size_t keep = std::min(valid, b.n);  // (only the tree's nodes hold data)
+            if (keep) HC(hipMemcpyAsync(q, b.p, keep * sizeof(double), hipMemcpyDeviceToDevice, C.stream));
             const size_t old = b.n * sizeof(double);
             C.cache.put(b.release(), old);
         }
         b.adopt(q, got / sizeof(double));
     }
     void ensure_dev(size_t nodes) {
-        grow_cached(coef, nodes * 8 * (size_t)K3());
-        grow_cached(norm, nodes * 8);
+        grow_cached(coef, nodes * 8 * (size_t)K3(), (size_t)n() * 8 * (size_t)K3());
+        grow_cached(norm, nodes * 8, (size_t)n() * 8);
     }
     // a finished tree: move the coefficients to a block of their size when the capacity
     // exceeds it by more than half (a copy on the stream; the old block goes to the cache)
@@ -1044,6 +1044,12 @@ static void for_each_job(size_t n, F f) {
         if (e) std::rethrow_exception(e);
 }
 
+// milliseconds between two device-clock stamps of a contraction (0 when they are missing)
+static float clock_ms(const Ctx &C, unsigned long long start, unsigned long long end) {
+    if (start == 0ull || end == 0ull || end < start) return 0.0f;
+    return (float)((double)(end - start) / C.wall_khz);
+}
+
 // MFMA flops one slice issues (v_mfma_f32_16x16x32_f16 with split operands, counted as
 // issued, padding included): K = 8 28 MFMAs, K = 10 90 (two M-tiles of 7 + 13 + 25 N-tiles)
 static double slice_flops(int K) {
@@ -1071,6 +1077,11 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                                      std::to_string(K - 1) + ")");
     auto t_start = std::chrono::steady_clock::now();
     float t_contract_ms = 0.0f;
+    const bool use_pred = (flags & MWG_APPLY_PREDICT) && C.pred != nullptr;
+    // the device-resident refinement loop (see device_loop below) is tried when a batch ran
+    // on this context before; it reserves the output trees itself
+    const bool dl_plan = !use_pred && !C.tracing && max_iter < 0 && C.hw_nw > 0 && jobs.size() <= (size_t)LOOP_MAX_JOBS &&
+                         M.maxDepth < LOOP_MAX_DEPTH && std::getenv("MWG_HOST_LOOP") == nullptr;
     // inputs: device hash, norm maxima per depth
     size_t genTotal = 0;
     for (JobH &J : jobs) {
@@ -1099,7 +1110,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         // size (Helmholtz): reserve half of it once (beyond, the arrays grow geometrically:
         // an allocation and a copy on the stream each time)
         const size_t est = std::max<size_t>(64, (size_t)J.out->n() + J.in->n() / 2);
-        J.out->ensure_dev(est);
+        if (!dl_plan) J.out->ensure_dev(est);
         J.out->reserve_host(est);
         J.work = J.out->roots;
         uint32_t cap = 1u << 14;
@@ -1123,7 +1134,6 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
 
     int iter = 0;
     int64_t predicted = 0, wasted = 0;
-    const bool use_pred = (flags & MWG_APPLY_PREDICT) && C.pred != nullptr;
     for (JobH &J : jobs) {
         J.spec.assign(J.out->n(), 0);
         J.alive.assign(J.out->n(), 1);
@@ -1198,7 +1208,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
     // the same order; the running norm sums the same squares in another order).
     const int njobs = (int)jobs.size();
     bool dl_done = false;
-    int dl_enq = 0;  // iterations queued by the device loop (their contraction events)
+
     auto device_loop = [&]() -> bool {
         const int nr = (int)jobs[0].out->roots.size();
         const int WCAP = std::max(C.hw_nw + C.hw_nw / 4 + 64, njobs * nr);
@@ -1299,15 +1309,16 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         TaskSummary *dsum = (TaskSummary *)C.rbd.p;
         long long *dtab = (long long *)(C.rbd.p + o_tab);
         double *dwn = (double *)(C.rbd.p + o_wn);
-        if (!C.liter_h) HC(hipHostMalloc((void **)&C.liter_h, sizeof(LoopIter) * LOOP_MAX_DEPTH, hipHostMallocDefault));
+        if (!C.liter_h) {
+            HC(hipHostMalloc((void **)&C.liter_h, sizeof(LoopIter) * LOOP_MAX_DEPTH,
+                             hipHostMallocMapped | hipHostMallocCoherent));
+            HC(hipHostGetDevicePointer((void **)&C.liter_d, C.liter_h, 0));
+        }
         if (!C.evTopo) HC(hipEventCreateWithFlags(&C.evTopo, hipEventDisableTiming));
         const int tcap = (int)std::min<size_t>(topo / 8 + 8, (size_t)INT_MAX);  // internal nodes <= nodes / 8
         C.titems.alloc((size_t)tcap);
-        for (int q = 0; q < LOOP_MAX_DEPTH; q++) {
+        for (int q = 0; q < LOOP_MAX_DEPTH; q++)
             if (!C.evIter[q]) HC(hipEventCreateWithFlags(&C.evIter[q], hipEventDisableTiming));
-            if (!C.evc[2 * q]) HC(hipEventCreate(&C.evc[2 * q]));
-            if (!C.evc[2 * q + 1]) HC(hipEventCreate(&C.evc[2 * q + 1]));
-        }
         {
             LoopCtl ctl;
             std::memset(&ctl, 0, sizeof(ctl));
@@ -1349,9 +1360,10 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             const int *nwp = (const int *)(ctlb + offsetof(LoopCtl, nw)) + cur;
             int *nwnext = (int *)(ctlb + offsetof(LoopCtl, nw)) + (cur ^ 1);
             const WorkItem *dw = C.lwork[cur].p;
-            HC(launch_loop_begin(s, njobs, C.ljobs.p, djt, dbj, dtab, C.genCount.p, C.lctl.p));
+            unsigned long long *dts = (unsigned long long *)(ctlb + offsetof(LoopCtl, ts)) + 2 * d;
+            const LoopInit li{C.ljobs.p, dtab, C.lctl.p, dts, C.genCount.p};
             HC(launch_job_boxes(s, njobs, max_terms, dbj, C.tabs.p, C.tbmax.p, djt, C.tpre.p, C.tpre0.p,
-                                C.genOverflow.p));
+                                C.genOverflow.p, li));
             GenState gs{C.genKeys.p, C.genJob.p, C.genCount.p, genCap, C.genOverflow.p, C.genPool.p, C.genNorm.p,
                         genPoolCap, dtab};
             HC(launch_gen_mark(s, WCAP, dw, djt, C.tabs.p, gs, nwp, queue));
@@ -1377,7 +1389,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             const int grid64 = std::max(1, (C.caps.tasks + 3) / 4);
             const int gridred = std::min(4096, std::max(1, C.caps.reds));
             HC(launch_contract_fp64(s, K, 0, dsum, grid64, C.tasks.p, dw, djt, C.tabs.p, C.tuples.p, C.genPool.p,
-                                    C.partials.p, C.evc[2 * d]));
+                                    C.partials.p, dts));
             if (mixed) {
                 const int tpb = mfma_tasks_per_block(K);
                 const int gridmf = std::max(1, (C.caps.stasks + tpb - 1) / tpb);
@@ -1385,14 +1397,14 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                                         C.nodeslots.p, dtab, C.genPool16.p, C.genExp16.p));
             }
             HC(launch_reduce_partials(s, 0, dsum, gridred, C.reds.p, dw, djt, C.partials.p,
-                                      mixed ? C.nodeslots.p : nullptr, K3, mixed ? dwn : nullptr, C.evc[2 * d + 1]));
-            if (!mixed) HC(launch_work_norms(s, WCAP, dw, djt, K3, dwn, nwp));
-            HC(launch_loop_split(s, njobs, C.ljobs.p, dw, dwn, M.maxDepth, C.lctl.p, C.lcidx.p, dtab));
+                                      mixed ? C.nodeslots.p : nullptr, K3, mixed ? dwn : nullptr));
+            // (the kernel after the reduction stamps the contraction's end)
+            if (!mixed) HC(launch_work_norms(s, WCAP, dw, djt, K3, dwn, nwp, dts + 1));
+            HC(launch_loop_split(s, njobs, C.ljobs.p, dw, dwn, M.maxDepth, C.lctl.p, C.lcidx.p, dtab,
+                                 mixed ? dts + 1 : nullptr));
             HC(launch_loop_emit(s, njobs, C.ljobs.p, dw, nwp, C.lcidx.p, C.lwork[cur ^ 1].p, nwnext, WCAP, dsum,
-                                C.lctl.p, d, djt, C.titems.p, tcap));
-            HC(hipMemcpyAsync(&C.liter_h[d], ctlb + offsetof(LoopCtl, iter) + d * sizeof(LoopIter), sizeof(LoopIter),
-                              hipMemcpyDeviceToHost, s));
-            HC(hipEventRecord(C.evIter[d], s));
+                                C.lctl.p, d, djt, C.titems.p, tcap, C.liter_d));
+            HC(hipEventRecord(C.evIter[d], s));  // (k_loop_emit wrote the host's entry d)
         };
         int d = 0, enq = 0, maxnw = njobs * nr;
         bool ok = true;
@@ -1428,10 +1440,12 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         // behind the last (empty) iteration: the topology read-back, then the transforms over
         // the device's item lists (the host rebuilds its trees while the GPU transforms)
         const size_t tb = 6 * topo * sizeof(int), jb = (size_t)njobs * sizeof(LoopJob);
-        C.rb.reserve(tb + jb + 64);
+        const size_t sb = sizeof(unsigned long long) * 2 * LOOP_MAX_DEPTH;
+        C.rb.reserve(tb + jb + sb + 64);
         HC(hipMemcpyAsync(C.rb.h, C.ltopo.p, tb, hipMemcpyDeviceToHost, s));
         HC(hipMemcpyAsync(C.rb.h + tb, C.ljobs.p, jb, hipMemcpyDeviceToHost, s));
-        HC(hipMemcpyAsync(C.rb.h + tb + jb, C.genCount.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        HC(hipMemcpyAsync(C.rb.h + tb + jb, ctlb + offsetof(LoopCtl, ts), sb, hipMemcpyDeviceToHost, s));
+        HC(hipMemcpyAsync(C.rb.h + tb + jb + sb, C.genCount.p, sizeof(int), hipMemcpyDeviceToHost, s));
         HC(hipEventRecord(C.evTopo, s));
         {
             std::vector<int> tn(d + 1), to(d + 2, 0);
@@ -1442,7 +1456,6 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             for (int q = 0; q <= d; q++) HC(launch_transform(s, 1, tn[q], C.titems.p + to[q], M.WT.p, K));
             for (int q = d; q >= 0; q--) HC(launch_transform(s, 0, tn[q], C.titems.p + to[q], M.W.p, K));
         }
-        dl_enq = enq;
         {
             hipError_t e;
             while ((e = hipEventQuery(C.evTopo)) == hipErrorNotReady) {
@@ -1454,7 +1467,11 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         if (enq - 1 > d && C.liter_h[enq - 1].overflow)  // (an iteration without work cannot overflow)
             throw Error(MWG_ERR_HIP, "device refinement loop: overflow in an empty iteration");
         int genNow = 0;
-        std::memcpy(&genNow, C.rb.h + tb + jb, sizeof(int));
+        std::memcpy(&genNow, C.rb.h + tb + jb + sb, sizeof(int));
+        {
+            const unsigned long long *hts = (const unsigned long long *)(C.rb.h + tb + jb);
+            for (int q = 0; q < enq; q++) t_contract_ms += clock_ms(C, hts[2 * q], hts[2 * q + 1]);
+        }
         std::vector<size_t> toff(njobs + 1, 0);
         for (int j = 0; j < njobs; j++) toff[j + 1] = toff[j] + (size_t)lj[j].cap;
         size_t nodes = 0;
@@ -1481,8 +1498,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         PT.lap("loop read-back");
         return true;
     };
-    if (!use_pred && !C.tracing && max_iter < 0 && C.hw_nw > 0 && njobs <= LOOP_MAX_JOBS &&
-        M.maxDepth < LOOP_MAX_DEPTH && std::getenv("MWG_HOST_LOOP") == nullptr) {
+    if (dl_plan) {
         dl_done = device_loop();
         if (!dl_done) {
             // a capacity was too small: the host loop from the roots, from a clean state
@@ -1576,7 +1592,10 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         long long *dtab = (long long *)(C.rbd.p + o_tab);
         double *dwn = (double *)(C.rbd.p + o_wn);
         float *dprob = (float *)(C.rbd.p + o_pr), *dfeat = (float *)(C.rbd.p + o_ft);
-        HC(hipMemsetAsync(dtab, 0, 4 * jt.size() * sizeof(long long), s));
+        // (contraction timestamps at byte 192 of the header, cleared with the table counts)
+        unsigned long long *dts = (unsigned long long *)(C.rbd.p + 192);
+        static_assert(sizeof(TaskSummary) <= 192, "read-back header layout");
+        HC(hipMemsetAsync(C.rbd.p + 192, 0, (o_tab - 192) + 4 * jt.size() * sizeof(long long), s));
         C.tabs.alloc((size_t)std::max(ntabs, 1));
         C.tbmax.alloc((size_t)std::max<size_t>(bjobs.size() * max_terms, 1));
         HC(launch_job_boxes(s, (int)bjobs.size(), max_terms, dbjobs, C.tabs.p, C.tbmax.p, djobs, C.tpre.p,
@@ -1655,6 +1674,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         std::vector<float> cprob(use_pred ? (size_t)nw * 8 : 0);
         std::vector<float> cfeat(C.tracing ? (size_t)nw * 8 * MLP_FEATURES : 0);
         TaskSummary hsum;
+        unsigned long long hts[2] = {0ull, 0ull};
         int chunk_it = C.chunk;  // FP64 task size of this iteration (raised when partial slots would not fit)
         // slices per MFMA task (a workgroup task).  k = 7: at most MFMA8_TASK_SLICES (the
         // waves sum a quarter of them each in FP32), MFMA8_TASK_DEFAULT when automatic;
@@ -1675,10 +1695,10 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                                      C.gofs.p, C.tuples.p, C.slices.p, dsum, C.segs.p, C.gslice.p));
             const int grid64 = std::max(1, (C.caps.tasks + 3) / 4);
             const int gridred = std::min(4096, std::max(1, C.caps.reds));
-            // contraction time: ev0 and ev1 are stamped by the dispatches of the first and
-            // the last contraction kernel (no separate event packets on the stream)
+            // contraction time: the device clock at the first workgroup of the first and the
+            // last workgroup of the last contraction kernel
             HC(launch_contract_fp64(s, K, 0, dsum, grid64, C.tasks.p, dwork, djobs, C.tabs.p, C.tuples.p,
-                                    C.genPool.p, C.partials.p, C.ev0));
+                                    C.genPool.p, C.partials.p, dts));
             if (mixed) {
                 const int tpb = mfma_tasks_per_block(K);
                 const int gridmf = std::max(1, (C.caps.stasks + tpb - 1) / tpb);
@@ -1687,8 +1707,10 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             }
             // (mixed: the reduce items cover every output component and write the norms)
             HC(launch_reduce_partials(s, 0, dsum, gridred, C.reds.p, dwork, djobs, C.partials.p,
-                                      mixed ? C.nodeslots.p : nullptr, K3, mixed ? dwn : nullptr, C.ev1));
-            if (!mixed) HC(launch_work_norms(s, nw, dwork, djobs, K3, dwn));
+                                      mixed ? C.nodeslots.p : nullptr, K3, mixed ? dwn : nullptr));
+            // (the contraction's end: stamped by the next kernel on the stream)
+            if (!mixed) HC(launch_work_norms(s, nw, dwork, djobs, K3, dwn, nullptr, dts + 1));
+            else HC(launch_stamp(s, dts + 1));
             // refinement predictor: features of every work node's children and their split
             // probabilities, read back with the norms (no extra round trip)
             if (want_feat) {
@@ -1705,6 +1727,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             PT.lap("gpu wait");
             C.stage.off = 0;
             std::memcpy(&hsum, h, sizeof(TaskSummary));
+            std::memcpy(hts, h + 192, sizeof(hts));
             std::memcpy(tabcnt.data(), h + o_tab, tabcnt.size() * sizeof(long long));
             std::memcpy(wn.data(), h + o_wn, wn.size() * sizeof(double));
             if (use_pred) std::memcpy(cprob.data(), h + o_pr, cprob.size() * sizeof(float));
@@ -1751,6 +1774,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                 need.nstasks = 0;
             }
             ensure_caps(&need);
+            HC(hipMemsetAsync(dts, 0, 2 * sizeof(unsigned long long), s));  // (time the last attempt only)
             launch_schedule();
         }
         genLastNew = hsum.gen_count - genDone;
@@ -1781,9 +1805,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             J.gen += tabcnt[4 * q + 3];
         }
         PT.lap("worklist+launch");
-        float ms = 0.0f;
-        HC(hipEventElapsedTime(&ms, C.ev0, C.ev1));
-        t_contract_ms += ms;
+        t_contract_ms += clock_ms(C, hts[0], hts[1]);
         PT.lap("contract+norms");
         for (int i = 0; i < nw; i++) {
             JobH &J = jobs[jt_job[work[i].job]];
@@ -1966,11 +1988,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             J.out->sqnorm_from_cnorm();
         }
     }
-    for (int q = 0; q < dl_enq; q++) {  // the device loop's contraction time, per iteration
-        float ms = 0.0f;
-        HC(hipEventElapsedTime(&ms, C.evc[2 * q], C.evc[2 * q + 1]));
-        t_contract_ms += ms;
-    }
+
     for (JobH &J : jobs) J.out->fit_dev();
     PT.lap("transforms");
     // the inputs' split-FP16 copies live for one apply (an SCF iteration brings new inputs;
@@ -2314,13 +2332,9 @@ struct mwg_predictor_s : PredH {};
 static void release_ctx(Ctx *c) {
     if (!c || c->refs.fetch_sub(1) != 1) return;
     (void)hipStreamSynchronize(c->stream);
-    if (c->ev0) (void)hipEventDestroy(c->ev0);
-    if (c->ev1) (void)hipEventDestroy(c->ev1);
     for (hipEvent_t e : c->evIter)
         if (e) (void)hipEventDestroy(e);
     if (c->evTopo) (void)hipEventDestroy(c->evTopo);
-    for (hipEvent_t e : c->evc)
-        if (e) (void)hipEventDestroy(e);
     if (c->liter_h) (void)hipHostFree(c->liter_h);
     hipStream_t s = c->stream;
     delete static_cast<mwg_ctx_s *>(c);
@@ -2382,8 +2396,12 @@ int mwg_ctx_create(int device, mwg_ctx *out) {
         c->device = device;
         HC(hipSetDevice(device));
         HC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-        HC(hipEventCreate(&c->ev0));
-        HC(hipEventCreate(&c->ev1));
+        {
+            int khz = 0;
+            HC(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
+            if (khz <= 0) throw Error(MWG_ERR_HIP, "device wall clock rate unavailable");
+            c->wall_khz = (double)khz;
+        }
         c->dot_scratch = dot_scratch_address(device);
         double x[24], w[24];
         gauss_legendre01(24, x, w);

@@ -147,10 +147,23 @@ hipError_t launch_project_gauss(hipStream_t s, int nw, const int *work, const in
 hipError_t launch_project_hyd(hipStream_t s, int nw, const int *work, const int *depth, const int *lr,
                               const ProjGeom &geo, const DevBasis &B, int nterm, const double *coef, const int *q,
                               double Z, const double *origin, double *out);
+// the device loop's start of an iteration, done by the screening-box kernels (LoopJob etc.
+// are declared below): thresholds from the jobs' running norms instead of the tables'
+// (table jb = job jb), table counts cleared, work counters and contraction stamps reset,
+// generated-node range started
+struct LoopJob;
+struct LoopCtl;
+struct LoopInit {
+    const LoopJob *lj;  // null: the host loop (thresholds from the tables)
+    long long *tabcnt;
+    LoopCtl *ctl;
+    unsigned long long *ts;
+    const int *gen_count;
+};
 // TermTab[tab0 + t] for every (table, term): per-combination screening boxes from the
 // monotone norm envelopes, union boxes, and the table's FP64 routing bound
 hipError_t launch_job_boxes(hipStream_t s, int nbj, int max_terms, const BoxJob *bj, TermTab *tabs, double *tbmax,
-                            JobTab *jobs, int *pre, int *pre0, int *gen_overflow);
+                            JobTab *jobs, int *pre, int *pre0, int *gen_overflow, const LoopInit &li = LoopInit{});
 // Device work counts: the work-list kernels below take an optional nwp, the work list's
 // node count on the device (written by an earlier kernel of the stream); nw is then only
 // an upper bound for the launch, and the kernels loop over the nodes on a grid of at most
@@ -178,7 +191,9 @@ hipError_t launch_enum(hipStream_t s, int mode, int nw, int nparts, const WorkIt
                        const long long *seg_ofs, Tuple *out, const TaskSummary *sum, const SegRec *segs,
                        const int *nwp = nullptr, int *queue = nullptr);
 hipError_t launch_work_norms(hipStream_t s, int nw, const WorkItem *work, const JobTab *jobs, int K3,
-                             double *compact, const int *nwp = nullptr);
+                             double *compact, const int *nwp = nullptr, unsigned long long *ts_end = nullptr);
+// the device clock into *t (one thread; marks the end of the kernels before it on the stream)
+hipError_t launch_stamp(hipStream_t s, unsigned long long *t);
 struct TransformItem {
     double *coef;
     double *norm;  // component norms of the tree (null: not updated)
@@ -280,23 +295,23 @@ struct LoopCtl {
     double splitfac[LOOP_MAX_DEPTH];  // 2^(-(n0 + d + 1) / 2) of the split test at depth d
     LoopIter iter[LOOP_MAX_DEPTH];
     int tofs[LOOP_MAX_DEPTH + 1];     // first transform item of the split nodes of depth d
+    unsigned long long ts[LOOP_MAX_DEPTH][2];  // contraction start and end, device clock
 };
 // the roots (nr per job, translations rootlr, the same for every job) in the topology and
 // the first work list
 hipError_t launch_loop_init(hipStream_t s, int njobs, int nr, const LoopJob *lj, const int *rootlr, WorkItem *work);
-// thresholds of the iteration's tables from the running norms, table counts cleared,
-// generated-node range started
-hipError_t launch_loop_begin(hipStream_t s, int njobs, const LoopJob *lj, JobTab *jt, BoxJob *bj, long long *tabcnt,
-                             const int *gen_count, LoopCtl *ctl);
 // per job (one workgroup): running norm, split test of its work nodes, children appended
 // (cidx[i]: first child of work node i, -1 if not split)
 hipError_t launch_loop_split(hipStream_t s, int njobs, LoopJob *lj, const WorkItem *work, const double *wnorm,
-                             int max_depth, LoopCtl *ctl, int *cidx, const long long *tabcnt);
-// the next work list (and its size in *nw_next, the host's copy in ctl->iter[d]), and the
-// split nodes as transform items of depth d (from ctl->tofs[d], capacity tcap)
+                             int max_depth, LoopCtl *ctl, int *cidx, const long long *tabcnt,
+                             unsigned long long *ts_end = nullptr);
+// the next work list (and its size in *nw_next and in ctl->iter[d] and host_iter[d], pinned
+// host memory the host reads once the iteration is done), and the split nodes as transform
+// items of depth d (from ctl->tofs[d], capacity tcap)
 hipError_t launch_loop_emit(hipStream_t s, int njobs, LoopJob *lj, const WorkItem *work, const int *nwp,
                             const int *cidx, WorkItem *next, int *nw_next, int wcap, const TaskSummary *sum,
-                            LoopCtl *ctl, int d, const JobTab *jt, TransformItem *titems, int tcap);
+                            LoopCtl *ctl, int d, const JobTab *jt, TransformItem *titems, int tcap,
+                            LoopIter *host_iter);
 
 // refinement-predictor features of the 8 children of every work node (row i*8 + a, a the
 // child index), from the work nodes' norms just computed, their parents' norms and the
@@ -307,11 +322,10 @@ hipError_t launch_child_features(hipStream_t s, int nw, const WorkItem *work, co
 hipError_t launch_contract_fp64(hipStream_t s, int K, int ntask, const TaskSummary *sum, int grid,
                                 const ContractTask *tasks, const WorkItem *work, const JobTab *jobs,
                                 const TermTab *tabs, const Tuple *tuples, const double *genPool, double *partials,
-                                hipEvent_t start = nullptr);
+                                unsigned long long *ts = nullptr);
 hipError_t launch_move_groups(hipStream_t s, int npairs, const int *from_to, double *coef, double *norm, int nb);
 hipError_t launch_reduce_partials(hipStream_t s, int n, const TaskSummary *sum, int grid, const ReduceItem *items,
                                   const WorkItem *work, const JobTab *jobs, const double *partials,
-                                  const double *nodeslots, int K3, double *wnorm = nullptr,
-                                  hipEvent_t stop = nullptr);
+                                  const double *nodeslots, int K3, double *wnorm = nullptr);
 
 }  // namespace mwg

@@ -962,12 +962,20 @@ __device__ inline double wave_max_double(double v) {
 }
 
 constexpr int ENV_LDS = 512;  // envelope entries per type staged in LDS
+// the tuple threshold of a device-loop job: the host loop's expression,
+// SCREEN_REL prec sqrt(||g||^2 / terms), 0 before any norm
+__device__ inline double loop_thr(const LoopJob &L, int nterms) {
+    const double gsq = L.gsq;
+    return gsq > 0.0 ? SCREEN_REL * L.prec * sqrt(gsq / (double)nterms) : 0.0;
+}
+
 __global__ void __launch_bounds__(64) k_job_boxes(int nbj, const BoxJob *bjs, TermTab *tabs, double *tbmax,
-                                                  int max_terms) {
+                                                  int max_terms, LoopInit li) {
     const int jb = blockIdx.x, t = blockIdx.y;
     if (jb >= nbj) return;
     const BoxJob &B = bjs[jb];
     if (t >= B.nterms) return;
+    const double thr = li.lj ? loop_thr(li.lj[jb], B.nterms) : B.thr;
     const LevTerm &lt = B.terms[t];  // (read in place: maxn is indexed by the lane's types)
     TermTab *tb = tabs + B.tab0 + t;
     const int c = threadIdx.x, gt = c >> 3, ft = c & 7;
@@ -990,7 +998,7 @@ __global__ void __launch_bounds__(64) k_job_boxes(int nbj, const BoxJob *bjs, Te
             for (int q = 0; q < 3; q++) ty[q] = 2 * ((gt >> q) & 1) + ((ft >> q) & 1);
             const double base = fabs(lt.w) * F;
             const double f3 = base * lt.maxn[ty[0]] * lt.maxn[ty[1]] * lt.maxn[ty[2]];
-            if (f3 > B.thr) {
+            if (f3 > thr) {
                 full = f3;
                 ok = true;
 #pragma unroll
@@ -999,7 +1007,7 @@ __global__ void __launch_bounds__(64) k_job_boxes(int nbj, const BoxJob *bjs, Te
 #pragma unroll
                     for (int q2 = 0; q2 < 3; q2++)
                         if (q2 != q) other *= lt.maxn[ty[q2]];
-                    const double tau = B.thr / other;
+                    const double tau = thr / other;
                     const double *env = envb + (long long)ty[q] * (lt.band + 1);
                     int m = -1;
                     if (env[0] > tau) {
@@ -1060,12 +1068,22 @@ __device__ inline int wave_incl_scan(int v) {
 
 __global__ void __launch_bounds__(64) k_job_route(int nbj, const BoxJob *bjs, const TermTab *tabs,
                                                   const double *tbmax, int max_terms, JobTab *jobs, int *pre,
-                                                  int *pre0, int *gen_overflow) {
+                                                  int *pre0, int *gen_overflow, LoopInit li) {
     const int jb = blockIdx.x;
     if (jb >= nbj) return;
     if (jb == 0 && threadIdx.x == 0) *gen_overflow = 0;  // the iteration's generated-table flag
     const BoxJob &B = bjs[jb];
     const int lane = threadIdx.x;
+    const double thr = li.lj ? loop_thr(li.lj[jb], B.nterms) : B.thr;
+    if (li.lj) {  // the device loop's iteration start (before any kernel that uses these)
+        if (lane == 0) jobs[B.jt].thr = thr;
+        if (lane < 4) li.tabcnt[4 * (size_t)jb + lane] = 0;
+        if (jb == 0 && lane == 0) {
+            li.ctl->genFirst = *(volatile const int *)li.gen_count;
+            for (int q = 0; q < 4; q++) li.ctl->queue[q] = 0;
+            li.ts[0] = li.ts[1] = 0ull;
+        }
+    }
     double m = 0.0;
     int ux = -1, uy = -1, uz = -1, carry = 0, carry0 = 0;
     for (int base = 0; base < B.nterms; base += 64) {
@@ -1098,7 +1116,7 @@ __global__ void __launch_bounds__(64) k_job_route(int nbj, const BoxJob *bjs, co
         pre[B.pre_off + B.nterms] = carry;
         pre0[B.pre_off + B.nterms] = carry0;
         JobTab &J = jobs[B.jt];
-        J.route = B.thr > 0.0 ? ROUTE_FP64 * B.thr : ROUTE_ROOT * m;
+        J.route = thr > 0.0 ? ROUTE_FP64 * thr : ROUTE_ROOT * m;
         J.ubox[0] = ux;
         J.ubox[1] = uy;
         J.ubox[2] = uz;
@@ -1667,6 +1685,8 @@ __global__ void __launch_bounds__(256) k_contract_fp64(int ntask_arg, const Task
                                                        const JobTab *jobs, const TermTab *tabs,
                                                        const Tuple *tuples, const double *genPool,
                                                        double *partials) {
+    // (no stores before the task loop: the operator rows are read through scalar loads, which
+    // the compiler only uses for memory no store of the kernel may have changed)
     constexpr int K3 = K * K * K;
     __shared__ double buf[4][K3];
     const int wave = wave_id(), lane = threadIdx.x & 63;
@@ -1842,7 +1862,9 @@ __device__ __forceinline__ void work_norms_node(int i, const WorkItem *work, con
 }
 
 __global__ void k_work_norms(int nw, const int *nwp, const WorkItem *work, const JobTab *jobs, int K3,
-                             double *compact) {
+                             double *compact, unsigned long long *ts_end) {
+    // (the contraction's end on the device clock: this kernel starts once the reduction is done)
+    if (ts_end && blockIdx.x == 0 && threadIdx.x == 0) *ts_end = (unsigned long long)wall_clock64();
     const int n = nwp ? *nwp : nw;
     for (int i = blockIdx.x; i < n; i += gridDim.x) work_norms_node(i, work, jobs, K3, compact);
 }
@@ -1869,30 +1891,16 @@ __global__ void __launch_bounds__(256) k_loop_init(int njobs, int nr, const Loop
     }
 }
 
-__global__ void __launch_bounds__(256) k_loop_begin(int njobs, const LoopJob *lj, JobTab *jt, BoxJob *bj,
-                                                    long long *tabcnt, const int *gen_count, LoopCtl *ctl) {
-    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < njobs; j += gridDim.x * blockDim.x) {
-        const double gsq = lj[j].gsq;
-        // the host loop's expression: SCREEN_REL prec sqrt(||g||^2 / terms), 0 before any norm
-        const double thr = gsq > 0.0 ? SCREEN_REL * lj[j].prec * sqrt(gsq / (double)jt[j].ntab) : 0.0;
-        jt[j].thr = thr;
-        bj[j].thr = thr;
-        for (int q = 0; q < 4; q++) tabcnt[4 * (size_t)j + q] = 0;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        ctl->genFirst = *(volatile const int *)gen_count;
-        for (int q = 0; q < 4; q++) ctl->queue[q] = 0;
-    }
-}
-
 // One workgroup per job.  The running norm adds the squared norms of the job's work nodes
 // (scaling parts on the roots, wavelet parts everywhere; per thread over a contiguous run,
 // then a fixed reduction tree); the split test is the host's (split_check with the factor
 // 2^(-(n0 + d + 1) / 2) the host computed); the split nodes' children follow the node count in
 // work-list order, 8 each, as the host appends them.
 __global__ void __launch_bounds__(256) k_loop_split(int njobs, LoopJob *lj, const WorkItem *work, const double *wn,
-                                                    int max_depth, LoopCtl *ctl, int *cidx, const long long *tabcnt) {
+                                                    int max_depth, LoopCtl *ctl, int *cidx, const long long *tabcnt,
+                                                    unsigned long long *ts_end) {
     const int j = blockIdx.x, tid = threadIdx.x;
+    if (ts_end && j == 0 && tid == 0) *ts_end = (unsigned long long)wall_clock64();
     if (j >= njobs) return;
     LoopJob &L = lj[j];
     __shared__ double rs[256], rw[256];
@@ -1989,7 +1997,7 @@ __global__ void __launch_bounds__(256) k_loop_split(int njobs, LoopJob *lj, cons
 __global__ void __launch_bounds__(256) k_loop_emit(int njobs, LoopJob *lj, const WorkItem *work, const int *nwp,
                                                    const int *cidx, WorkItem *next, int *nw_next, int wcap,
                                                    const TaskSummary *sum, LoopCtl *ctl, int d, const JobTab *jt,
-                                                   TransformItem *titems, int tcap) {
+                                                   TransformItem *titems, int tcap, LoopIter *host_iter) {
     __shared__ int base[LOOP_MAX_JOBS];
     __shared__ int sc[256];
     const int tid = threadIdx.x;
@@ -2029,6 +2037,8 @@ __global__ void __launch_bounds__(256) k_loop_emit(int njobs, LoopJob *lj, const
             *nw_next = nn;
             ctl->iter[d].nw_next = nn;
             ctl->iter[d].overflow = ovf;
+            host_iter[d].nw_next = nn;
+            host_iter[d].overflow = ovf;
             ctl->tofs[d + 1] = t0 + (ovf ? 0 : (int)total);
         }
     }
@@ -2342,11 +2352,11 @@ hipError_t launch_project_hyd(hipStream_t s, int nw, const int *work, const int 
 }
 
 hipError_t launch_job_boxes(hipStream_t s, int nbj, int max_terms, const BoxJob *bj, TermTab *tabs, double *tbmax,
-                            JobTab *jobs, int *pre, int *pre0, int *gen_overflow) {
+                            JobTab *jobs, int *pre, int *pre0, int *gen_overflow, const LoopInit &li) {
     if (nbj <= 0 || max_terms <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_job_boxes, dim3(nbj, max_terms), dim3(64), 0, s, nbj, bj, tabs, tbmax, max_terms);
+    hipLaunchKernelGGL(k_job_boxes, dim3(nbj, max_terms), dim3(64), 0, s, nbj, bj, tabs, tbmax, max_terms, li);
     hipLaunchKernelGGL(k_job_route, dim3(nbj), dim3(64), 0, s, nbj, bj, tabs, tbmax, max_terms, jobs, pre, pre0,
-                       gen_overflow);
+                       gen_overflow, li);
     return hipGetLastError();
 }
 
@@ -2390,10 +2400,17 @@ hipError_t launch_enum(hipStream_t s, int mode, int nw, int nparts, const WorkIt
 }
 
 hipError_t launch_work_norms(hipStream_t s, int nw, const WorkItem *work, const JobTab *jobs, int K3,
-                             double *compact, const int *nwp) {
+                             double *compact, const int *nwp, unsigned long long *ts_end) {
     if (nw <= 0) return hipSuccess;
     const int nb = nwp ? std::min(nw, DEVCOUNT_GRID) : nw;
-    hipLaunchKernelGGL(k_work_norms, dim3(nb), dim3(64), 0, s, nw, nwp, work, jobs, K3, compact);
+    hipLaunchKernelGGL(k_work_norms, dim3(nb), dim3(64), 0, s, nw, nwp, work, jobs, K3, compact, ts_end);
+    return hipGetLastError();
+}
+
+__global__ void k_stamp(unsigned long long *t) { *t = (unsigned long long)wall_clock64(); }
+
+hipError_t launch_stamp(hipStream_t s, unsigned long long *t) {
+    hipLaunchKernelGGL(k_stamp, dim3(1), dim3(1), 0, s, t);
     return hipGetLastError();
 }
 
@@ -2405,28 +2422,23 @@ hipError_t launch_loop_init(hipStream_t s, int njobs, int nr, const LoopJob *lj,
     return hipGetLastError();
 }
 
-hipError_t launch_loop_begin(hipStream_t s, int njobs, const LoopJob *lj, JobTab *jt, BoxJob *bj, long long *tabcnt,
-                             const int *gen_count, LoopCtl *ctl) {
-    const int nb = std::max(1, std::min((njobs + 255) / 256, 64));
-    hipLaunchKernelGGL(k_loop_begin, dim3(nb), dim3(256), 0, s, njobs, lj, jt, bj, tabcnt, gen_count, ctl);
-    return hipGetLastError();
-}
-
 hipError_t launch_loop_split(hipStream_t s, int njobs, LoopJob *lj, const WorkItem *work, const double *wnorm,
-                             int max_depth, LoopCtl *ctl, int *cidx, const long long *tabcnt) {
+                             int max_depth, LoopCtl *ctl, int *cidx, const long long *tabcnt,
+                             unsigned long long *ts_end) {
     if (njobs <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_loop_split, dim3(njobs), dim3(256), 0, s, njobs, lj, work, wnorm, max_depth, ctl, cidx,
-                       tabcnt);
+                       tabcnt, ts_end);
     return hipGetLastError();
 }
 
 hipError_t launch_loop_emit(hipStream_t s, int njobs, LoopJob *lj, const WorkItem *work, const int *nwp,
                             const int *cidx, WorkItem *next, int *nw_next, int wcap, const TaskSummary *sum,
-                            LoopCtl *ctl, int d, const JobTab *jt, TransformItem *titems, int tcap) {
+                            LoopCtl *ctl, int d, const JobTab *jt, TransformItem *titems, int tcap,
+                            LoopIter *host_iter) {
     if (njobs <= 0 || njobs > LOOP_MAX_JOBS || d < 0 || d >= LOOP_MAX_DEPTH) return hipErrorInvalidValue;
     const int nb = std::max(1, std::min((wcap + 255) / 256, 1024));
     hipLaunchKernelGGL(k_loop_emit, dim3(nb), dim3(256), 0, s, njobs, lj, work, nwp, cidx, next, nw_next, wcap, sum,
-                       ctl, d, jt, titems, tcap);
+                       ctl, d, jt, titems, tcap, host_iter);
     return hipGetLastError();
 }
 
@@ -2455,22 +2467,25 @@ hipError_t launch_build_tasks(hipStream_t s, int nw, int nseg, int nparts, int c
 hipError_t launch_contract_fp64(hipStream_t s, int K, int ntask, const TaskSummary *sum, int grid,
                                 const ContractTask *tasks, const WorkItem *work, const JobTab *jobs,
                                 const TermTab *tabs, const Tuple *tuples, const double *genPool, double *partials,
-                                hipEvent_t start) {
+                                unsigned long long *ts) {
     const int nb = sum ? grid : (ntask + 3) / 4;
     if (nb <= 0) return hipSuccess;
-    // the start event (if any) is stamped by the dispatch packet itself
-    MWG_DISPATCH_K(K, hipExtLaunchKernelGGL(k_contract_fp64<KK>, dim3(nb), dim3(256), 0, s, start, nullptr, 0,
-                                            ntask, sum, tasks, work, jobs, tabs, tuples, genPool, partials));
+    if (ts) {  // the contraction's start on the device clock
+        hipError_t e = launch_stamp(s, ts);
+        if (e != hipSuccess) return e;
+    }
+    MWG_DISPATCH_K(K, hipLaunchKernelGGL(k_contract_fp64<KK>, dim3(nb), dim3(256), 0, s, ntask, sum, tasks, work,
+                                         jobs, tabs, tuples, genPool, partials));
     return hipGetLastError();
 }
 
 hipError_t launch_reduce_partials(hipStream_t s, int n, const TaskSummary *sum, int grid, const ReduceItem *items,
                                   const WorkItem *work, const JobTab *jobs, const double *partials,
-                                  const double *nodeslots, int K3, double *wnorm, hipEvent_t stop) {
+                                  const double *nodeslots, int K3, double *wnorm) {
     const int nb = sum ? grid : n;
     if (nb <= 0) return hipSuccess;
-    hipExtLaunchKernelGGL(k_reduce_partials, dim3(nb), dim3(256), 0, s, nullptr, stop, 0, n, sum, items, work, jobs,
-                          partials, nodeslots, K3, wnorm);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(nb), dim3(256), 0, s, n, sum, items, work, jobs, partials, nodeslots,
+                       K3, wnorm);
     return hipGetLastError();
 }
 
