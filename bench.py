@@ -173,6 +173,29 @@ def cpu_baseline(orbs, mus, k, prec, budget_s=15.0, budget_1t_s=6.0):
                 sample_1_thread=f"same oracle on 1 thread, {d1} orbital(s), {t1:.2f} s of apply time")
 
 
+def best_flags(res32, res64, G):
+    return G.MWG_APPLY_MIXED if res32 is not None and res32["nodes_per_s"] > res64["nodes_per_s"] else G.MWG_APPLY_FP64
+
+
+def new_operators_cost(G, mra, trees, mus, prec, flags):
+    """One step with freshly built Helmholtz operators (their NS levels are computed during
+    the first apply, as after an SCF update of the orbital energies), then the same step
+    again with the operators built."""
+    t0 = time.perf_counter()
+    ops = [G.HelmholtzOperator(mra, mu, prec) for mu in mus]
+    fit = time.perf_counter() - t0
+    times = []
+    for _ in range(2):
+        outs = [G.FunctionTree(mra) for _ in trees]
+        mra.ctx.sync()
+        t0 = time.perf_counter()
+        G.apply_batch(prec, outs, ops, trees, flags=flags)
+        mra.ctx.sync()
+        times.append(time.perf_counter() - t0)
+    return dict(fit_s=fit, first_apply_ms=1e3 * times[0], warm_apply_ms=1e3 * times[1],
+                levels_ms=1e3 * (times[0] - times[1]))
+
+
 def copy_probe(G, tree):
     """One in-place rescale by 1.0 of a tree's coefficients (k_scale): a streaming kernel
     of known size (reads and writes nnodes * 8 (k+1)^3 doubles) that calibrates HBM
@@ -469,6 +492,11 @@ def main():
             f"device loop {res32['device_loop_steps']}/{args.steps} steps")
         if res32["nodes_per_s"] > res64["nodes_per_s"] and err < args.prec:
             best, dtype = res32, mixed_dtype(args.k)
+    # an SCF iteration brings new mu values: fresh operators build their NS levels during
+    # their first apply (the timed steps above reuse built operators)
+    new_ops = new_operators_cost(G, mra, trees, mus, args.prec, best_flags(res32, res64, G))
+    log(f"[bench] new operators: first apply {new_ops['first_apply_ms']:.2f} ms, then "
+        f"{new_ops['warm_apply_ms']:.2f} ms (kernel fit {1e3 * new_ops['fit_s']:.1f} ms)")
     resP = None
     if not args.no_predict:
         pred, ntrain, t_train = train_predictor(G, configs, mra, args)
@@ -573,6 +601,7 @@ def main():
             "predict_threshold": resP["threshold"] if resP else None,
             "predict_threshold_sweep": resP["threshold_sweep"] if resP else None,
             "operator_build_s": t_oper, "projection_s": t_proj,
+            "new_operators": new_ops,
             "contraction_ms_per_step": 1e3 * best["t_contract_per_step"],
             "tuning": tuning or None,
             "exchange": exch,

@@ -519,14 +519,21 @@ __device__ inline OpsH10 load_opsh10(const Slice &sl, const Arr16 &J, const uint
 }
 
 // v = hi + lo as one dword (hi in the low half)
-__device__ inline unsigned pack_hl(float v) {
-    const _Float16 h = (_Float16)v;
-    const _Float16 l = (_Float16)(v - (float)h);
-    return __builtin_bit_cast(unsigned, halfx2{h, l});
+// v sc (sc a power of two) as one dword: hi = f16(v sc) in the low half, lo = f16(v sc - hi) in
+// the high half, in two v_fma_mix instructions (the product is exact, the difference is
+// formed exactly before its one rounding: the same bits as converting, subtracting and
+// converting again)
+__device__ inline unsigned pack_hl(float v, float sc) {
+    unsigned d;
+    asm volatile("v_fma_mixlo_f16 %0, %1, %2, 0\n\t"
+                 "v_fma_mixhi_f16 %0, %1, %2, -%0 op_sel_hi:[0,0,1]"
+                 : "=&v"(d)
+                 : "v"(v), "v"(sc));
+    return d;
 }
-// one value to its LDS column (and to the duplicate tail for positions below 6)
-__device__ inline void put_h10(unsigned *P, int col, int pos, float v) {
-    const unsigned d = pack_hl(v);
+// one value (times sc) to its LDS column (and to the duplicate tail for positions below 6)
+__device__ inline void put_h10(unsigned *P, int col, int pos, float v, float sc) {
+    const unsigned d = pack_hl(v, sc);
     P[col * H10_COL + pos] = d;
     if (pos < 6) P[col * H10_COL + 10 + pos] = d;
 }
@@ -589,8 +596,8 @@ k_contract_mfma10h(int ntask_arg, const TaskSummary *sum, const SliceTask *tasks
             if (r < 100) {
                 const int z = r / 10, y = r % 10;
 #pragma unroll
-                for (int i = 0; i < 4; i++) put_h10(P1, 20 * z + 4 * kg + i, y, d0[i] * 1024.f);
-                put_h10(P1, 20 * z + 16 + kg, y, d1[0] * 1024.f);
+                for (int i = 0; i < 4; i++) put_h10(P1, 20 * z + 4 * kg + i, y, d0[i], 1024.f);
+                put_h10(P1, 20 * z + 16 + kg, y, d1[0], 1024.f);
             }
         }
         __syncthreads();
@@ -612,8 +619,8 @@ k_contract_mfma10h(int ntask_arg, const TaskSummary *sum, const SliceTask *tasks
             if (n < 200) {
                 const int z = n / 20, ap = n % 20;
 #pragma unroll
-                for (int i = 0; i < 4; i++) put_h10(P2, 20 * (4 * kg + i) + ap, z, d0[i] * 0.5f);
-                put_h10(P2, 20 * (16 + kg) + ap, z, d1[0] * 0.5f);
+                for (int i = 0; i < 4; i++) put_h10(P2, 20 * (4 * kg + i) + ap, z, d0[i], 0.5f);
+                put_h10(P2, 20 * (16 + kg) + ap, z, d1[0], 0.5f);
             }
         }
         __syncthreads();
