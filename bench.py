@@ -177,12 +177,13 @@ def best_flags(res32, res64, G):
     return G.MWG_APPLY_MIXED if res32 is not None and res32["nodes_per_s"] > res64["nodes_per_s"] else G.MWG_APPLY_FP64
 
 
-def new_operators_cost(G, mra, trees, mus, prec, flags):
-    """One step with freshly built Helmholtz operators (their NS levels are computed during
-    the first apply, as after an SCF update of the orbital energies), then the same step
-    again with the operators built."""
+def new_operators_cost(G, mra, trees, mus, prec, flags, fixed=()):
+    """One step with freshly built Helmholtz operators for the first len(mus) trees (their NS
+    levels are computed during the first apply, as after an SCF update of the orbital
+    energies) and the operators `fixed` for the rest, then the same step again with the
+    operators built."""
     t0 = time.perf_counter()
-    ops = [G.HelmholtzOperator(mra, mu, prec) for mu in mus]
+    ops = [G.HelmholtzOperator(mra, mu, prec) for mu in mus] + list(fixed)
     fit = time.perf_counter() - t0
     times = []
     for _ in range(2):
@@ -359,6 +360,9 @@ def scf_run(args, G, configs, steps, warmup, predict=True, cpu=True):
     log(f"[bench] scf mixed: {res32['ms_per_step']:.0f} ms/step, {res32['nodes_per_s']:.0f} nodes/s, "
         f"contraction {1e3 * res32['t_contract_per_step']:.0f} ms/step, max rel L2 err vs fp64 {err32:.2e}, "
         f"device loop {res32['device_loop_steps']}/{steps} steps")
+    new_ops = new_operators_cost(G, mra, inps, mus, prec, G.MWG_APPLY_MIXED, fixed=[P] * len(rho))
+    log(f"[bench] scf new Helmholtz operators (mixed): first step {new_ops['first_apply_ms']:.0f} ms, then "
+        f"{new_ops['warm_apply_ms']:.0f} ms (kernel fits {1e3 * new_ops['fit_s']:.0f} ms)")
     resP, errP = None, None
     if predict:
         pred, ntrain, t_train = train_predictor(G, configs, G.mra_from(configs.BOX20, 7, "L", max_depth=16),
@@ -393,6 +397,7 @@ def scf_run(args, G, configs, steps, warmup, predict=True, cpu=True):
                 predict_iters=resP["iters"] if resP else None, fp64_iters=res64["iters"],
                 predict_rel_l2_err_vs_fp64=errP,
                 output_nodes_per_step=res64["nodes_per_step"], inputs_s=t_in, operators_s=t_op,
+                new_operators=new_ops,
                 plain_tuples_per_step=res64["tuples_per_step"],
                 fp64_contraction_ms_per_step=1e3 * tc64, mixed_contraction_ms_per_step=1e3 * tc32,
                 fp64_contraction_tflops=alg / tc64 / 1e12 if tc64 > 0 else None,

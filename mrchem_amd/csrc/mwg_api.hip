@@ -219,12 +219,20 @@ struct Readback {
     }
 };
 
+// scratch of the operator level builder: item lists, child-scale blocks, norms
+struct NSScratch {
+    DBuf<double> b1, t1, blk, nrm;
+    DBuf<int> tterm, row0, tlo, thi;
+    DBuf<long long> tdelta, ndelta;
+};
+
 struct Ctx {
     int device = 0;
     std::atomic<int> refs{1};  // the owner's handle, plus one per MRA and predictor made on it
     DevCache cache;
     Readback rb;
     Stage stage;
+    NSScratch ns;
     hipStream_t stream = nullptr;
     double wall_khz = 0.0;  // device wall clock (contraction timestamps), ticks per ms
     // device-resident refinement: high-water marks of the applies run on this context (work
@@ -600,170 +608,248 @@ struct OperH {
     KernelFit fit;
     std::vector<double> beta, weff;
     std::vector<std::unique_ptr<LevelH>> lev;
-    DBuf<double> t1buf, tmpblk, tmpnrm, bvals;
-    DBuf<int> iterm, irow0, itlo, ithi;
-    DBuf<long long> idelta;
 
     int nterms() const { return (int)beta.size(); }
-    void build_level(int d);
-    void ensure(int d) {
-        if ((int)lev.size() <= d) lev.resize(d + 1);
-        for (int q = 0; q <= d; q++)
-            if (!lev[q]) build_level(q);
-    }
+    bool has(int d) const { return (int)lev.size() > d && lev[d]; }
 };
 
-// NS blocks for a set of (term, delta range) requests into dense device arrays
+// NS blocks of a list of (term, translation range) requests at depth d, for one or more
+// operators of one MRA.  b1[term] is the term's exponent at the child scale (operators
+// concatenated).  Rows (one per translation, 4 K^2 doubles: types by (gt, ft) bit) are
+// numbered in request order (first_row, plus the total at the end); the requests from
+// groups[i].r0 up to the next group's r0 write their rows to groups[i].blocks (row 0 = the
+// group's first row), and the 4 norms of every row go to C.ns.nrm.  The child-scale blocks
+// are computed in chunks of whole groups that fit a bounded scratch.  Asynchronous on the
+// context's stream: the caller synchronises (by downloading the norms).
 struct NSRequest {
     int term;
     long long dlo, dhi;
 };
+struct NSGroup {
+    size_t r0;
+    double *blocks;
+};
 
-static void compute_ns(OperH &O, int d, const std::vector<NSRequest> &req, DBuf<double> &blocks, DBuf<double> &norms,
-                       std::vector<long long> &first_row) {
-    MraH &M = *O.mra;
+static void compute_ns(MraH &M, int d, const std::vector<double> &b1, const std::vector<NSRequest> &req,
+                       const std::vector<NSGroup> &groups, std::vector<long long> &first_row) {
     Ctx &C = *M.ctx;
+    NSScratch &S = C.ns;
     const Basis &B = M.B;
     const int K2 = B.K2;
+    const size_t nr = req.size();
     const int m1 = M.n0 + d + 1;
     const double pre1 = std::ldexp(1.0, -m1);
-    long long boxmax = std::max({M.boxes[0], M.boxes[1], M.boxes[2]});
+    const long long boxmax = std::max({M.boxes[0], M.boxes[1], M.boxes[2]});
     const long long lmax1 = (boxmax << (d + 1)) - 1;
-    // child-scale scaling blocks
-    std::vector<int> tterm, row0(req.size()), tlo(req.size()), thi(req.size());
-    std::vector<long long> tdelta;
-    std::vector<double> b1(O.nterms());
-    for (int t = 0; t < O.nterms(); t++) b1[t] = O.beta[t] * std::ldexp(1.0, -2 * m1);
-    for (size_t r = 0; r < req.size(); r++) {
-        const int t = req[r].term;
-        const long long band1 = std::min<long long>(lmax1, (long long)std::ceil(1.0 + std::sqrt(GAUSS_CUT / b1[t])));
-        const long long lo = std::max<long long>(-band1, 2 * req[r].dlo - 1);
-        const long long hi = std::min<long long>(band1, 2 * req[r].dhi + 1);
-        row0[r] = (int)tdelta.size();
-        tlo[r] = (int)lo;
-        thi[r] = (int)hi;
-        for (long long q = lo; q <= hi; q++) {
-            tterm.push_back(t);
-            tdelta.push_back(q);
+    first_row.assign(nr + 1, 0);
+    std::vector<long long> lo1(nr), hi1(nr), trow(nr + 1, 0);
+    for (size_t r = 0; r < nr; r++) {
+        const double b = b1[req[r].term];
+        const long long band1 = std::min<long long>(lmax1, (long long)std::ceil(1.0 + std::sqrt(GAUSS_CUT / b)));
+        lo1[r] = std::max<long long>(-band1, 2 * req[r].dlo - 1);
+        hi1[r] = std::min<long long>(band1, 2 * req[r].dhi + 1);
+        first_row[r + 1] = first_row[r] + (req[r].dhi - req[r].dlo + 1);
+        trow[r + 1] = trow[r] + (hi1[r] - lo1[r] + 1);
+    }
+    const long long nrows = first_row[nr];
+    if (nrows >= ((long long)1 << 31) || trow[nr] >= ((long long)1 << 31))
+        throw Error(MWG_ERR_RANGE, "operator level too large");
+    // chunks of groups whose child-scale rows fit the scratch (at least one group each)
+    const long long cap = std::max<long long>(1, ((long long)256 << 20) / (8 * K2));
+    const size_t ng = groups.size();
+    auto gend = [&](size_t g) { return g + 1 < ng ? groups[g + 1].r0 : nr; };
+    std::vector<size_t> chunk0{0};
+    long long tmax = 0;
+    for (size_t g = 0; g < ng;) {
+        const size_t c0 = g;
+        while (g < ng && (g == c0 || trow[gend(g)] - trow[groups[c0].r0] <= cap)) g++;
+        chunk0.push_back(g);
+        tmax = std::max(tmax, trow[gend(g - 1)] - trow[groups[c0].r0]);
+    }
+    // item lists of all requests; child-scale rows count from their chunk's first row
+    std::vector<int> tterm((size_t)trow[nr]), row0((size_t)nrows), tlo((size_t)nrows), thi((size_t)nrows);
+    std::vector<long long> tdelta((size_t)trow[nr]), ndelta((size_t)nrows);
+    for (size_t c = 0; c + 1 < chunk0.size(); c++) {
+        const long long base = trow[groups[chunk0[c]].r0];
+        for (size_t r = groups[chunk0[c]].r0; r < gend(chunk0[c + 1] - 1); r++) {
+            for (long long q = lo1[r]; q <= hi1[r]; q++) {
+                tterm[trow[r] + q - lo1[r]] = req[r].term;
+                tdelta[trow[r] + q - lo1[r]] = q;
+            }
+            for (long long q = req[r].dlo; q <= req[r].dhi; q++) {
+                const size_t i = (size_t)(first_row[r] + q - req[r].dlo);
+                ndelta[i] = q;
+                row0[i] = (int)(trow[r] - base);
+                tlo[i] = (int)lo1[r];
+                thi[i] = (int)hi1[r];
+            }
         }
     }
+    upload_async(C, S.b1, b1.data(), b1.size());
+    upload_async(C, S.tterm, tterm.data(), tterm.size());
+    upload_async(C, S.tdelta, tdelta.data(), tdelta.size());
+    upload_async(C, S.ndelta, ndelta.data(), ndelta.size());
+    upload_async(C, S.row0, row0.data(), row0.size());
+    upload_async(C, S.tlo, tlo.data(), tlo.size());
+    upload_async(C, S.thi, thi.data(), thi.size());
+    S.t1.alloc((size_t)std::max<long long>(tmax, 1) * K2);
+    S.nrm.alloc((size_t)std::max<long long>(nrows, 1) * 4);
     hipStream_t s = C.stream;
-    O.bvals.upload(b1.data(), b1.size(), s);
-    const int nt1 = (int)tdelta.size();
-    O.iterm.upload(tterm.data(), tterm.size(), s);
-    O.idelta.upload(tdelta.data(), tdelta.size(), s);
-    O.t1buf.alloc((size_t)std::max(nt1, 1) * K2);
-    HC(launch_tblock_mom(s, nt1, O.iterm.p, O.idelta.p, O.bvals.p, pre1, B.K, M.ccoef.p, C.gx.p, C.gw.p, O.t1buf.p));
-    // NS transform per delta
-    std::vector<long long> ndelta;
-    std::vector<int> nrow0, ntlo, nthi;
-    first_row.assign(req.size(), 0);
-    for (size_t r = 0; r < req.size(); r++) {
-        first_row[r] = (long long)ndelta.size();
-        for (long long q = req[r].dlo; q <= req[r].dhi; q++) {
-            ndelta.push_back(q);
-            nrow0.push_back(row0[r]);
-            ntlo.push_back(tlo[r]);
-            nthi.push_back(thi[r]);
+    for (size_t c = 0; c + 1 < chunk0.size(); c++) {
+        const long long t0 = trow[groups[chunk0[c]].r0], nt1 = trow[gend(chunk0[c + 1] - 1)] - t0;
+        HC(launch_tblock_mom(s, (int)nt1, S.tterm.p + t0, S.tdelta.p + t0, S.b1.p, pre1, B.K, M.ccoef.p, C.gx.p,
+                             C.gw.p, S.t1.p));
+        for (size_t g = chunk0[c]; g < chunk0[c + 1]; g++) {
+            const long long f0 = first_row[groups[g].r0], n = first_row[gend(g)] - f0;
+            if (n > 0)
+                HC(launch_nsblocks(s, (int)n, S.ndelta.p + f0, S.row0.p + f0, S.tlo.p + f0, S.thi.p + f0, S.t1.p, M.W.p,
+                                   B.K, groups[g].blocks, S.nrm.p + 4 * f0));
         }
     }
-    const int nn = (int)ndelta.size();
-    DBuf<long long> dd;
-    dd.upload(ndelta.data(), ndelta.size(), s);
-    O.irow0.upload(nrow0.data(), nrow0.size(), s);
-    O.itlo.upload(ntlo.data(), ntlo.size(), s);
-    O.ithi.upload(nthi.data(), nthi.size(), s);
-    blocks.alloc((size_t)std::max(nn, 1) * 4 * K2);
-    norms.alloc((size_t)std::max(nn, 1) * 4);
-    HC(launch_nsblocks(s, nn, dd.p, O.irow0.p, O.itlo.p, O.ithi.p, O.t1buf.p, M.W.p, B.K, blocks.p, norms.p));
-    HC(hipStreamSynchronize(s));
 }
 
-void OperH::build_level(int d) {
-    MraH &M = *mra;
+// Depth-d levels of operators of one MRA, built together: one round trip for the near-
+// diagonal blocks that decide which terms are active, one for the full bands of the active
+// terms, whatever the number of operators.
+static void build_levels(const std::vector<OperH *> &ops, int d) {
+    MraH &M = *ops[0]->mra;
     Ctx &C = *M.ctx;
-    const int nt = nterms();
-    auto L = std::make_unique<LevelH>();
-    L->t.resize(nt);
+    const int K2 = M.B.K2;
+    const size_t no = ops.size();
     const long long boxmax = std::max({M.boxes[0], M.boxes[1], M.boxes[2]});
     const int m1 = M.n0 + d + 1;
-    std::vector<long long> band(nt);
-    for (int t = 0; t < nt; t++) {
-        const double b1 = beta[t] * std::ldexp(1.0, -2 * m1);
-        const long long lmax1 = (boxmax << (d + 1)) - 1;
-        const long long band1 = std::min<long long>(lmax1, (long long)std::ceil(1.0 + std::sqrt(GAUSS_CUT / b1)));
-        band[t] = std::min<long long>((boxmax << d) - 1, (band1 + 2) / 2);
-    }
+    const long long lmax1 = (boxmax << (d + 1)) - 1;
+    std::vector<int> toff(no + 1, 0);
+    for (size_t o = 0; o < no; o++) toff[o + 1] = toff[o] + ops[o]->nterms();
+    std::vector<double> b1(toff[no]);
+    std::vector<long long> band(toff[no]);
+    for (size_t o = 0; o < no; o++)
+        for (int t = 0; t < ops[o]->nterms(); t++) {
+            const int g = toff[o] + t;
+            b1[g] = ops[o]->beta[t] * std::ldexp(1.0, -2 * m1);
+            const long long band1 = std::min<long long>(lmax1, (long long)std::ceil(1.0 + std::sqrt(GAUSS_CUT / b1[g])));
+            band[g] = std::min<long long>((boxmax << d) - 1, (band1 + 2) / 2);
+        }
     // phase 1: near-diagonal blocks decide which terms are active at this depth
-    std::vector<NSRequest> req;
-    for (int t = 0; t < nt; t++) {
-        const long long nb = std::min<long long>(band[t], 1);
-        req.push_back({t, -nb, nb});
+    std::vector<NSRequest> req(toff[no]);
+    for (int g = 0; g < toff[no]; g++) {
+        const long long nb = std::min<long long>(band[g], 1);
+        req[g] = {g, -nb, nb};
     }
     std::vector<long long> rows;
-    compute_ns(*this, d, req, tmpblk, tmpnrm, rows);
-    std::vector<double> nn(tmpnrm.n);
-    tmpnrm.download(nn.data(), tmpnrm.n, C.stream);
+    long long n1 = 0;
+    for (const NSRequest &r : req) n1 += r.dhi - r.dlo + 1;
+    C.ns.blk.alloc((size_t)std::max<long long>(n1, 1) * 4 * K2);
+    compute_ns(M, d, b1, req, {NSGroup{0, C.ns.blk.p}}, rows);
+    std::vector<double> nn((size_t)rows[req.size()] * 4);
+    C.ns.nrm.download(nn.data(), nn.size(), C.stream);
+    std::vector<std::unique_ptr<LevelH>> L(no);
     std::vector<NSRequest> full;
-    for (int t = 0; t < nt; t++) {
-        double tn = 0.0, wn = 0.0;
-        for (long long q = 0; q <= req[t].dhi - req[t].dlo; q++) {
-            const double *p = &nn[(rows[t] + q) * 4];
-            tn = std::max(tn, p[0]);
-            wn = std::max({wn, p[1], p[2], p[3]});
-        }
-        const bool active = (d == 0) || (wn > ACT_REL * prec * tn);
-        L->t[t].active = active;
-        if (active) {
-            L->t[t].band = (int)band[t];
-            full.push_back({t, -band[t], band[t]});
+    std::vector<size_t> req0(no + 1, 0);
+    for (size_t o = 0; o < no; o++) {
+        L[o] = std::make_unique<LevelH>();
+        L[o]->t.resize(ops[o]->nterms());
+        req0[o] = full.size();
+        for (int t = 0; t < ops[o]->nterms(); t++) {
+            const int g = toff[o] + t;
+            double tn = 0.0, wn = 0.0;
+            for (long long q = 0; q <= req[g].dhi - req[g].dlo; q++) {
+                const double *p = &nn[(rows[g] + q) * 4];
+                tn = std::max(tn, p[0]);
+                wn = std::max({wn, p[1], p[2], p[3]});
+            }
+            const bool active = (d == 0) || (wn > ACT_REL * ops[o]->prec * tn);
+            L[o]->t[t].active = active;
+            if (active) {
+                L[o]->t[t].band = (int)band[g];
+                full.push_back({g, -band[g], band[g]});
+            }
         }
     }
-    // phase 2: full bands of the active terms
+    req0[no] = full.size();
+    // phase 2: full bands of the active terms, straight into each operator's level arrays
     if (!full.empty()) {
-        compute_ns(*this, d, full, L->blocks, L->norms, rows);
-        // slice records address the level's FP32 block array with 32-bit offsets
-        if (L->blocks.n >= ((size_t)1 << 31)) throw Error(MWG_ERR_RANGE, "operator level too large");
-        std::vector<double> hn(L->norms.n);
-        L->norms.download(hn.data(), hn.size(), C.stream);
-        for (size_t r = 0; r < full.size(); r++) {
-            TermLevel &tl = L->t[full[r].term];
-            const int b = tl.band;
-            tl.blk = rows[r] * 4 * (long long)M.B.K2;
-            tl.nrm = rows[r] * 4;
-            tl.norms.assign(hn.begin() + rows[r] * 4, hn.begin() + (rows[r] + 2 * b + 1) * 4);
-            for (int q = 0; q < 2 * b + 1; q++)
-                for (int type = 0; type < 4; type++) tl.maxn[type] = std::max(tl.maxn[type], tl.norms[q * 4 + type]);
-            tl.env.assign(4 * (b + 1), 0.0);
-            for (int type = 0; type < 4; type++) {
-                double run = 0.0;
-                for (int m = b; m >= 0; m--) {
-                    run = std::max(run, std::max(tl.norms[(b + m) * 4 + type], tl.norms[(b - m) * 4 + type]));
-                    tl.env[type * (b + 1) + m] = run;
+        std::vector<NSGroup> groups;
+        for (size_t o = 0; o < no; o++) {
+            long long nrow = 0;
+            for (size_t r = req0[o]; r < req0[o + 1]; r++) nrow += full[r].dhi - full[r].dlo + 1;
+            if (!nrow) continue;
+            // slice records address the level's FP32 block array with 32-bit offsets
+            if (nrow * 4 * K2 >= ((long long)1 << 31)) throw Error(MWG_ERR_RANGE, "operator level too large");
+            L[o]->blocks.alloc((size_t)nrow * 4 * K2);
+            L[o]->norms.alloc((size_t)nrow * 4);
+            groups.push_back({req0[o], L[o]->blocks.p});
+        }
+        compute_ns(M, d, b1, full, groups, rows);
+        std::vector<double> hn((size_t)rows[full.size()] * 4);
+        C.ns.nrm.download(hn.data(), hn.size(), C.stream);
+        for (size_t o = 0; o < no; o++) {
+            if (req0[o + 1] == req0[o]) continue;
+            const long long f0 = rows[req0[o]];
+            HC(hipMemcpyAsync(L[o]->norms.p, C.ns.nrm.p + 4 * f0, (size_t)(rows[req0[o + 1]] - f0) * 4 * sizeof(double),
+                              hipMemcpyDeviceToDevice, C.stream));
+            for (size_t r = req0[o]; r < req0[o + 1]; r++) {
+                TermLevel &tl = L[o]->t[full[r].term - toff[o]];
+                const int b = tl.band;
+                tl.blk = (rows[r] - f0) * 4 * (long long)K2;
+                tl.nrm = (rows[r] - f0) * 4;
+                tl.norms.assign(hn.begin() + rows[r] * 4, hn.begin() + (rows[r] + 2 * b + 1) * 4);
+                for (int q = 0; q < 2 * b + 1; q++)
+                    for (int type = 0; type < 4; type++) tl.maxn[type] = std::max(tl.maxn[type], tl.norms[q * 4 + type]);
+                tl.env.assign(4 * (b + 1), 0.0);
+                for (int type = 0; type < 4; type++) {
+                    double run = 0.0;
+                    for (int m = b; m >= 0; m--) {
+                        run = std::max(run, std::max(tl.norms[(b + m) * 4 + type], tl.norms[(b - m) * 4 + type]));
+                        tl.env[type * (b + 1) + m] = run;
+                    }
                 }
             }
         }
     }
-    std::vector<LevTerm> lts(nt);
-    std::vector<double> envs;
-    for (int t = 0; t < nt; t++) {
-        const TermLevel &tl = L->t[t];
-        LevTerm &x = lts[t];
-        std::memset(&x, 0, sizeof(x));
-        x.w = weff[t];
-        x.active = tl.active ? 1 : 0;
-        x.band = tl.band;
-        if (!tl.active) continue;
-        x.blk = tl.blk;
-        x.nrm = tl.nrm;
-        x.env = (long long)envs.size();
-        for (int q = 0; q < 4; q++) x.maxn[q] = tl.maxn[q];
-        envs.insert(envs.end(), tl.env.begin(), tl.env.end());
+    for (size_t o = 0; o < no; o++) {
+        const int nt = ops[o]->nterms();
+        std::vector<LevTerm> lts(nt);
+        std::vector<double> envs;
+        for (int t = 0; t < nt; t++) {
+            const TermLevel &tl = L[o]->t[t];
+            LevTerm &x = lts[t];
+            std::memset(&x, 0, sizeof(x));
+            x.w = ops[o]->weff[t];
+            x.active = tl.active ? 1 : 0;
+            x.band = tl.band;
+            if (!tl.active) continue;
+            x.blk = tl.blk;
+            x.nrm = tl.nrm;
+            x.env = (long long)envs.size();
+            for (int q = 0; q < 4; q++) x.maxn[q] = tl.maxn[q];
+            envs.insert(envs.end(), tl.env.begin(), tl.env.end());
+        }
+        if (envs.empty()) envs.push_back(0.0);
+        upload_async(C, L[o]->dterms, lts.data(), lts.size());
+        upload_async(C, L[o]->denv, envs.data(), envs.size());
+        if ((int)ops[o]->lev.size() <= d) ops[o]->lev.resize(d + 1);
+        ops[o]->lev[d] = std::move(L[o]);
     }
-    if (envs.empty()) envs.push_back(0.0);
-    L->dterms.upload(lts.data(), lts.size(), C.stream);
-    L->denv.upload(envs.data(), envs.size(), C.stream);
-    lev[d] = std::move(L);
+}
+
+// Levels 0..depth of each (operator, depth) pair, built depth by depth for all operators
+// of an MRA together
+static void ensure_levels(const std::vector<std::pair<OperH *, int>> &need) {
+    int dmax = -1;
+    for (const auto &x : need) dmax = std::max(dmax, x.second);
+    for (int q = 0; q <= dmax; q++) {
+        std::vector<OperH *> todo;
+        for (const auto &x : need)
+            if (q <= x.second && !x.first->has(q) && std::find(todo.begin(), todo.end(), x.first) == todo.end())
+                todo.push_back(x.first);
+        while (!todo.empty()) {
+            std::vector<OperH *> same, rest;
+            for (OperH *o : todo) (o->mra == todo[0]->mra ? same : rest).push_back(o);
+            build_levels(same, q);
+            todo.swap(rest);
+        }
+    }
 }
 
 static void build_oper(OperH &O, MraH *mra, double mu, double prec) {
@@ -1084,6 +1170,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                          M.maxDepth < LOOP_MAX_DEPTH && std::getenv("MWG_HOST_LOOP") == nullptr;
     // inputs: device hash, norm maxima per depth
     size_t genTotal = 0;
+    std::vector<std::pair<OperH *, int>> need;
     for (JobH &J : jobs) {
         if (J.in->mra != &M || J.out->mra != &M || J.op->mra != &M)
             throw Error(MWG_ERR_ARG, "all trees and operators of an apply must share one MRA");
@@ -1091,8 +1178,10 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         J.in->build_hash();
         J.fmax.assign(M.maxDepth + 2, std::array<double, 8>{0, 0, 0, 0, 0, 0, 0, 0});
         std::vector<double> leafmax(M.maxDepth + 2, 0.0);
+        int dmax_in = 0;
         for (int i = 0; i < J.in->n(); i++) {
             const int d = J.in->depth[i];
+            dmax_in = std::max(dmax_in, d);
             for (int c = 0; c < 8; c++) J.fmax[d][c] = std::max(J.fmax[d][c], J.in->cnorm[8 * (size_t)i + c]);
             if (J.in->child[i] < 0) {
                 double fn = 0.0;
@@ -1105,6 +1194,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             J.fmax[d][0] = std::max(J.fmax[d][0], run);
             run = std::max(run, leafmax[d]);
         }
+        need.push_back({J.op, dmax_in});
         J.out->make_roots();
         // output trees end between a third of the input's size (pair Poissons) and about its
         // size (Helmholtz): reserve half of it once (beyond, the arrays grow geometrically:
@@ -1119,6 +1209,8 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         J.genMask = cap - 1;
         genTotal += cap;
     }
+    // operator levels down to the inputs' depths, all operators at once (deeper ones on demand)
+    ensure_levels(need);
     // generated-node tables
     C.genHashKeys.alloc(genTotal);
     C.genHashVals.alloc(genTotal);
@@ -1144,7 +1236,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
     // operator level, input and output arrays, thresholds from the running norm
     auto add_table = [&](JobH &J, int d, std::vector<JobTab> &jt, std::vector<BoxJob> &bjobs, int &ntabs,
                          int &max_terms) {
-        J.op->ensure(d);
+        if (!J.op->has(d)) ensure_levels({{J.op, d}});
         JobTab T;
         std::memset(&T, 0, sizeof(T));
         LevelH &LV = *J.op->lev[d];
@@ -1340,6 +1432,12 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             int ntabs = 0, max_terms = 0;
             jt.reserve(njobs);
             bj.reserve(njobs);
+            {
+                std::vector<std::pair<OperH *, int>> deeper;
+                for (int j = 0; j < njobs; j++)
+                    if (!jobs[j].op->has(d)) deeper.push_back({jobs[j].op, d});
+                if (!deeper.empty()) ensure_levels(deeper);
+            }
             for (int j = 0; j < njobs; j++) add_table(jobs[j], d, jt, bj, ntabs, max_terms);
             C.tpre.alloc((size_t)ntabs + njobs + 1);
             C.tpre0.alloc((size_t)ntabs + njobs + 1);
@@ -1532,6 +1630,13 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         std::vector<WorkItem> work;
         for (int j : active) jobs[j].out->ensure_dev(jobs[j].out->n());
         PT.lap("ensure_dev");
+        {
+            std::vector<std::pair<OperH *, int>> deeper;
+            for (int j : active)
+                for (int g : jobs[j].work)
+                    if (!jobs[j].op->has(jobs[j].out->depth[g])) deeper.push_back({jobs[j].op, jobs[j].out->depth[g]});
+            if (!deeper.empty()) ensure_levels(deeper);
+        }
         for (int j : active) {
             JobH &J = jobs[j];
             std::vector<int> depths;
@@ -2960,7 +3065,6 @@ static int make_oper(mwg_mra mra, double mu, double prec, mwg_oper *out) {
         if (!(prec > 0.0)) throw Error(MWG_ERR_ARG, "operator precision must be positive");
         if (mu < 0.0) throw Error(MWG_ERR_ARG, "Helmholtz mu must be non-negative");
         build_oper(*o, mra, mu, prec);
-        o->ensure(0);
     });
     if (rc) {
         delete o;
@@ -3002,10 +3106,13 @@ int mwg_oper_blocks(mwg_oper oper, int term, int depth, int64_t dlo, int64_t dhi
     if (!oper || !out || term < 0 || term >= oper->nterms() || depth < 0 || dhi < dlo) return MWG_ERR_ARG;
     Ctx *C = oper->mra->ctx;
     return guarded(C, [&] {
-        DBuf<double> b, nrm;
+        MraH &M = *oper->mra;
+        DBuf<double> b;
+        b.alloc((size_t)(dhi - dlo + 1) * 4 * M.B.K2);
         std::vector<long long> rows;
-        compute_ns(*oper, depth, {NSRequest{term, dlo, dhi}}, b, nrm, rows);
-        b.download(out, (size_t)(dhi - dlo + 1) * 4 * oper->mra->B.K2, C->stream);
+        const std::vector<double> b1{oper->beta[term] * std::ldexp(1.0, -2 * (M.n0 + depth + 1))};
+        compute_ns(M, depth, b1, {NSRequest{0, dlo, dhi}}, {NSGroup{0, b.p}}, rows);
+        b.download(out, (size_t)(dhi - dlo + 1) * 4 * M.B.K2, C->stream);
     });
 }
 

@@ -207,14 +207,15 @@ void build_basis(Basis &B, int k, int type) {
         }
 }
 
-static double fit_error(const KernelFit &F, double mu) {
-    const int J = 400;
+// max over a log grid of r in [rmin, rmax] of r |sum_i w_i exp(-a_i r^2) - exp(-mu r) / r|;
+// returns early (with a value above stop) once a point exceeds stop
+static double fit_error(const KernelFit &F, const std::vector<double> &r, const std::vector<double> &g, double stop) {
     double err = 0.0;
-    for (int j = 0; j < J; j++) {
-        double r = F.rmin * std::pow(F.rmax / F.rmin, j / (double)(J - 1));
+    for (size_t j = 0; j < r.size(); j++) {
         double f = 0.0;
-        for (size_t i = 0; i < F.a.size(); i++) f += F.w[i] * std::exp(-F.a[i] * r * r);
-        err = std::max(err, std::fabs(f - std::exp(-mu * r) / r) * r);
+        for (size_t i = 0; i < F.a.size(); i++) f += F.w[i] * std::exp(-F.a[i] * r[j] * r[j]);
+        err = std::max(err, std::fabs(f - g[j]) * r[j]);
+        if (err > stop) return err;
     }
     return err;
 }
@@ -229,6 +230,12 @@ KernelFit fit_kernel(double mu, double eps, double rmin, double rmax) {
     F.rmin = rmin;
     F.rmax = rmax;
     F.eps = eps;
+    const int J = 400;
+    std::vector<double> rg(J), gg(J);
+    for (int j = 0; j < J; j++) {
+        rg[j] = rmin * std::pow(rmax / rmin, j / (double)(J - 1));
+        gg[j] = std::exp(-mu * rg[j]) / rg[j];
+    }
     for (int m = 0; m < 400; m++) {
         const double h0 = std::pow(0.95, m);
         const int N = (int)std::ceil((s_hi - s_lo) / h0) + 1;
@@ -266,9 +273,10 @@ KernelFit fit_kernel(double mu, double eps, double rmin, double rmax) {
             }
         }
         F.h = h;
-        F.err = fit_error(F, mu);
-        if (F.err <= eps) break;
+        F.err = fit_error(F, rg, gg, eps);
+        if (F.err <= eps) return F;
     }
+    F.err = fit_error(F, rg, gg, INFINITY);
     return F;
 }
 
