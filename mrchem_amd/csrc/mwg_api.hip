@@ -221,9 +221,8 @@ struct Readback {
 
 // scratch of the operator level builder: item lists, child-scale blocks, norms
 struct NSScratch {
-    DBuf<double> b1, t1, blk, nrm;
-    DBuf<int> tterm, row0, tlo, thi;
-    DBuf<long long> tdelta, ndelta;
+    DBuf<double> t1, blk, nrm;
+    DBuf<char> lists;
 };
 
 struct Ctx {
@@ -305,11 +304,8 @@ static void stream_wait(Ctx &C) {
 }
 
 // asynchronous upload through the context's staging ring
-template <class T>
-static void upload_async(Ctx &C, DBuf<T> &d, const T *src, size_t n) {
-    d.alloc(n);
-    if (!n) return;
-    const size_t bytes = n * sizeof(T);
+static void stage_copy(Ctx &C, void *dst, const void *src, size_t bytes) {
+    if (!bytes) return;
     const size_t need = (bytes + 255) & ~(size_t)255;
     if (C.stage.off + need > C.stage.cap) {
         HC(hipStreamSynchronize(C.stream));
@@ -325,7 +321,13 @@ static void upload_async(Ctx &C, DBuf<T> &d, const T *src, size_t n) {
     char *p = C.stage.h + C.stage.off;
     std::memcpy(p, src, bytes);
     C.stage.off += need;
-    HC(hipMemcpyAsync(d.p, p, bytes, hipMemcpyHostToDevice, C.stream));
+    HC(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, C.stream));
+}
+
+template <class T>
+static void upload_async(Ctx &C, DBuf<T> &d, const T *src, size_t n) {
+    d.alloc(n);
+    if (n) stage_copy(C, d.p, src, n * sizeof(T));
 }
 
 // read back (synchronises the stream; the staging ring is free again afterwards)
@@ -592,13 +594,39 @@ struct TermLevel {
     double maxn[4] = {0, 0, 0, 0};
 };
 
+// one device allocation holding the arrays of several levels of an operator (those built
+// together), from the context's block cache and returned there with the last of them (an
+// operator keeps its MRA, and so the context, alive)
+struct DevArena {
+    Ctx *ctx;
+    char *p = nullptr;
+    size_t got = 0, used = 0;
+    DevArena(Ctx &C, size_t bytes) : ctx(&C) { p = (char *)C.cache.get(std::max<size_t>(bytes, 256), got); }
+    DevArena(const DevArena &) = delete;
+    DevArena &operator=(const DevArena &) = delete;
+    ~DevArena() { ctx->cache.put(p, got); }
+    template <class T>
+    T *take(size_t n, size_t align = 256) {
+        T *q = (T *)(p + used);
+        used += (n * sizeof(T) + align - 1) & ~(align - 1);
+        return q;
+    }
+};
+template <class T>
+static size_t arena_bytes(size_t n) {
+    return (n * sizeof(T) + 255) & ~(size_t)255;
+}
+
 struct LevelH {
     std::vector<TermLevel> t;
-    DBuf<double> blocks, norms;
-    DBuf<LevTerm> dterms;  // per-term screening data for the device box builder
-    DBuf<double> denv;     // concatenated norm envelopes of the active terms
-    DBuf<uint4> blocks16;  // split-FP16 copy (operator rows, exponent per block) for the mixed apply
-    DBuf<int> exp16;
+    std::shared_ptr<DevArena> mem, mem16, small;  // device arrays below, shared with other levels
+    double *blocks = nullptr, *norms = nullptr;
+    size_t nblocks = 0;       // doubles in blocks (4 K^2 per translation row)
+    size_t nnorms = 0;        // doubles in norms (4 per row)
+    LevTerm *dterms = nullptr;  // per-term screening data for the device box builder
+    double *denv = nullptr;     // concatenated norm envelopes of the active terms
+    uint4 *blocks16 = nullptr;  // split-FP16 copy (operator rows, exponent per block) for the mixed apply
+    int *exp16 = nullptr;
     bool has16 = false;
 };
 
@@ -613,14 +641,15 @@ struct OperH {
     bool has(int d) const { return (int)lev.size() > d && lev[d]; }
 };
 
-// NS blocks of a list of (term, translation range) requests at depth d, for one or more
-// operators of one MRA.  b1[term] is the term's exponent at the child scale (operators
-// concatenated).  Rows (one per translation, 4 K^2 doubles: types by (gt, ft) bit) are
-// numbered in request order (first_row, plus the total at the end); the requests from
+// NS blocks of lists of (term, translation range) requests, one list per depth (NSBatch),
+// for one or more operators of one MRA.  b1[term] is the term's exponent at the child scale
+// of the batch's depth.  Rows (one per translation, 4 K^2 doubles: types by (gt, ft) bit)
+// are numbered in request order (rows, plus the total at the end); the requests from
 // groups[i].r0 up to the next group's r0 write their rows to groups[i].blocks (row 0 = the
-// group's first row), and the 4 norms of every row go to C.ns.nrm.  The child-scale blocks
-// are computed in chunks of whole groups that fit a bounded scratch.  Asynchronous on the
-// context's stream: the caller synchronises (by downloading the norms).
+// group's first row), and their norms to groups[i].norms when given; the 4 norms of every
+// row also go to nrm.  The item lists of all batches travel in one upload; the child-scale
+// blocks are computed in chunks of whole groups that fit a bounded scratch.  Asynchronous on
+// the context's stream: the caller synchronises (by downloading the norms).
 struct NSRequest {
     int term;
     long long dlo, dhi;
@@ -628,226 +657,456 @@ struct NSRequest {
 struct NSGroup {
     size_t r0;
     double *blocks;
+    double *norms;
+};
+struct NSBatch {
+    int d = 0;
+    std::vector<double> b1;
+    std::vector<NSRequest> req;
+    std::vector<NSGroup> groups;
+    double *nrm = nullptr;
+    std::vector<long long> rows;
 };
 
-static void compute_ns(MraH &M, int d, const std::vector<double> &b1, const std::vector<NSRequest> &req,
-                       const std::vector<NSGroup> &groups, std::vector<long long> &first_row) {
+static void compute_ns(MraH &M, const std::vector<NSBatch *> &bs) {
     Ctx &C = *M.ctx;
     NSScratch &S = C.ns;
     const Basis &B = M.B;
     const int K2 = B.K2;
-    const size_t nr = req.size();
-    const int m1 = M.n0 + d + 1;
-    const double pre1 = std::ldexp(1.0, -m1);
     const long long boxmax = std::max({M.boxes[0], M.boxes[1], M.boxes[2]});
-    const long long lmax1 = (boxmax << (d + 1)) - 1;
-    first_row.assign(nr + 1, 0);
-    std::vector<long long> lo1(nr), hi1(nr), trow(nr + 1, 0);
-    for (size_t r = 0; r < nr; r++) {
-        const double b = b1[req[r].term];
-        const long long band1 = std::min<long long>(lmax1, (long long)std::ceil(1.0 + std::sqrt(GAUSS_CUT / b)));
-        lo1[r] = std::max<long long>(-band1, 2 * req[r].dlo - 1);
-        hi1[r] = std::min<long long>(band1, 2 * req[r].dhi + 1);
-        first_row[r + 1] = first_row[r] + (req[r].dhi - req[r].dlo + 1);
-        trow[r + 1] = trow[r] + (hi1[r] - lo1[r] + 1);
-    }
-    const long long nrows = first_row[nr];
-    if (nrows >= ((long long)1 << 31) || trow[nr] >= ((long long)1 << 31))
-        throw Error(MWG_ERR_RANGE, "operator level too large");
-    // chunks of groups whose child-scale rows fit the scratch (at least one group each)
     const long long cap = std::max<long long>(1, ((long long)256 << 20) / (8 * K2));
-    const size_t ng = groups.size();
-    auto gend = [&](size_t g) { return g + 1 < ng ? groups[g + 1].r0 : nr; };
-    std::vector<size_t> chunk0{0};
+    struct Plan {
+        std::vector<long long> lo1, hi1, trow;
+        std::vector<size_t> chunk0;
+        size_t b1off = 0, toff = 0, noff = 0;
+    };
+    std::vector<Plan> pl(bs.size());
+    size_t nb1 = 0, nt = 0, nn = 0;
     long long tmax = 0;
-    for (size_t g = 0; g < ng;) {
-        const size_t c0 = g;
-        while (g < ng && (g == c0 || trow[gend(g)] - trow[groups[c0].r0] <= cap)) g++;
-        chunk0.push_back(g);
-        tmax = std::max(tmax, trow[gend(g - 1)] - trow[groups[c0].r0]);
-    }
-    // item lists of all requests; child-scale rows count from their chunk's first row
-    std::vector<int> tterm((size_t)trow[nr]), row0((size_t)nrows), tlo((size_t)nrows), thi((size_t)nrows);
-    std::vector<long long> tdelta((size_t)trow[nr]), ndelta((size_t)nrows);
-    for (size_t c = 0; c + 1 < chunk0.size(); c++) {
-        const long long base = trow[groups[chunk0[c]].r0];
-        for (size_t r = groups[chunk0[c]].r0; r < gend(chunk0[c + 1] - 1); r++) {
-            for (long long q = lo1[r]; q <= hi1[r]; q++) {
-                tterm[trow[r] + q - lo1[r]] = req[r].term;
-                tdelta[trow[r] + q - lo1[r]] = q;
-            }
-            for (long long q = req[r].dlo; q <= req[r].dhi; q++) {
-                const size_t i = (size_t)(first_row[r] + q - req[r].dlo);
-                ndelta[i] = q;
-                row0[i] = (int)(trow[r] - base);
-                tlo[i] = (int)lo1[r];
-                thi[i] = (int)hi1[r];
-            }
+    for (size_t b = 0; b < bs.size(); b++) {
+        NSBatch &X = *bs[b];
+        Plan &P = pl[b];
+        const size_t nr = X.req.size(), ng = X.groups.size();
+        const long long lmax1 = (boxmax << (X.d + 1)) - 1;
+        X.rows.assign(nr + 1, 0);
+        P.lo1.resize(nr);
+        P.hi1.resize(nr);
+        P.trow.assign(nr + 1, 0);
+        for (size_t r = 0; r < nr; r++) {
+            const double bb = X.b1[X.req[r].term];
+            const long long band1 = std::min<long long>(lmax1, (long long)std::ceil(1.0 + std::sqrt(GAUSS_CUT / bb)));
+            P.lo1[r] = std::max<long long>(-band1, 2 * X.req[r].dlo - 1);
+            P.hi1[r] = std::min<long long>(band1, 2 * X.req[r].dhi + 1);
+            X.rows[r + 1] = X.rows[r] + (X.req[r].dhi - X.req[r].dlo + 1);
+            P.trow[r + 1] = P.trow[r] + (P.hi1[r] - P.lo1[r] + 1);
         }
-    }
-    upload_async(C, S.b1, b1.data(), b1.size());
-    upload_async(C, S.tterm, tterm.data(), tterm.size());
-    upload_async(C, S.tdelta, tdelta.data(), tdelta.size());
-    upload_async(C, S.ndelta, ndelta.data(), ndelta.size());
-    upload_async(C, S.row0, row0.data(), row0.size());
-    upload_async(C, S.tlo, tlo.data(), tlo.size());
-    upload_async(C, S.thi, thi.data(), thi.size());
-    S.t1.alloc((size_t)std::max<long long>(tmax, 1) * K2);
-    S.nrm.alloc((size_t)std::max<long long>(nrows, 1) * 4);
-    hipStream_t s = C.stream;
-    for (size_t c = 0; c + 1 < chunk0.size(); c++) {
-        const long long t0 = trow[groups[chunk0[c]].r0], nt1 = trow[gend(chunk0[c + 1] - 1)] - t0;
-        HC(launch_tblock_mom(s, (int)nt1, S.tterm.p + t0, S.tdelta.p + t0, S.b1.p, pre1, B.K, M.ccoef.p, C.gx.p,
-                             C.gw.p, S.t1.p));
-        for (size_t g = chunk0[c]; g < chunk0[c + 1]; g++) {
-            const long long f0 = first_row[groups[g].r0], n = first_row[gend(g)] - f0;
-            if (n > 0)
-                HC(launch_nsblocks(s, (int)n, S.ndelta.p + f0, S.row0.p + f0, S.tlo.p + f0, S.thi.p + f0, S.t1.p, M.W.p,
-                                   B.K, groups[g].blocks, S.nrm.p + 4 * f0));
+        if (X.rows[nr] >= ((long long)1 << 31) || P.trow[nr] >= ((long long)1 << 31))
+            throw Error(MWG_ERR_RANGE, "operator level too large");
+        // chunks of groups whose child-scale rows fit the scratch (at least one group each)
+        auto gend = [&](size_t g) { return g + 1 < ng ? X.groups[g + 1].r0 : nr; };
+        P.chunk0 = {0};
+        for (size_t g = 0; g < ng;) {
+            const size_t c0 = g;
+            while (g < ng && (g == c0 || P.trow[gend(g)] - P.trow[X.groups[c0].r0] <= cap)) g++;
+            P.chunk0.push_back(g);
+            tmax = std::max(tmax, P.trow[gend(g - 1)] - P.trow[X.groups[c0].r0]);
         }
+        P.b1off = nb1;
+        P.toff = nt;
+        P.noff = nn;
+        nb1 += X.b1.size();
+        nt += (size_t)P.trow[nr];
+        nn += (size_t)X.rows[nr];
     }
-}
-
-// Depth-d levels of operators of one MRA, built together: one round trip for the near-
-// diagonal blocks that decide which terms are active, one for the full bands of the active
-// terms, whatever the number of operators.
-static void build_levels(const std::vector<OperH *> &ops, int d) {
-    MraH &M = *ops[0]->mra;
-    Ctx &C = *M.ctx;
-    const int K2 = M.B.K2;
-    const size_t no = ops.size();
-    const long long boxmax = std::max({M.boxes[0], M.boxes[1], M.boxes[2]});
-    const int m1 = M.n0 + d + 1;
-    const long long lmax1 = (boxmax << (d + 1)) - 1;
-    std::vector<int> toff(no + 1, 0);
-    for (size_t o = 0; o < no; o++) toff[o + 1] = toff[o] + ops[o]->nterms();
-    std::vector<double> b1(toff[no]);
-    std::vector<long long> band(toff[no]);
-    for (size_t o = 0; o < no; o++)
-        for (int t = 0; t < ops[o]->nterms(); t++) {
-            const int g = toff[o] + t;
-            b1[g] = ops[o]->beta[t] * std::ldexp(1.0, -2 * m1);
-            const long long band1 = std::min<long long>(lmax1, (long long)std::ceil(1.0 + std::sqrt(GAUSS_CUT / b1[g])));
-            band[g] = std::min<long long>((boxmax << d) - 1, (band1 + 2) / 2);
-        }
-    // phase 1: near-diagonal blocks decide which terms are active at this depth
-    std::vector<NSRequest> req(toff[no]);
-    for (int g = 0; g < toff[no]; g++) {
-        const long long nb = std::min<long long>(band[g], 1);
-        req[g] = {g, -nb, nb};
-    }
-    std::vector<long long> rows;
-    long long n1 = 0;
-    for (const NSRequest &r : req) n1 += r.dhi - r.dlo + 1;
-    C.ns.blk.alloc((size_t)std::max<long long>(n1, 1) * 4 * K2);
-    compute_ns(M, d, b1, req, {NSGroup{0, C.ns.blk.p}}, rows);
-    std::vector<double> nn((size_t)rows[req.size()] * 4);
-    C.ns.nrm.download(nn.data(), nn.size(), C.stream);
-    std::vector<std::unique_ptr<LevelH>> L(no);
-    std::vector<NSRequest> full;
-    std::vector<size_t> req0(no + 1, 0);
-    for (size_t o = 0; o < no; o++) {
-        L[o] = std::make_unique<LevelH>();
-        L[o]->t.resize(ops[o]->nterms());
-        req0[o] = full.size();
-        for (int t = 0; t < ops[o]->nterms(); t++) {
-            const int g = toff[o] + t;
-            double tn = 0.0, wn = 0.0;
-            for (long long q = 0; q <= req[g].dhi - req[g].dlo; q++) {
-                const double *p = &nn[(rows[g] + q) * 4];
-                tn = std::max(tn, p[0]);
-                wn = std::max({wn, p[1], p[2], p[3]});
-            }
-            const bool active = (d == 0) || (wn > ACT_REL * ops[o]->prec * tn);
-            L[o]->t[t].active = active;
-            if (active) {
-                L[o]->t[t].band = (int)band[g];
-                full.push_back({g, -band[g], band[g]});
-            }
-        }
-    }
-    req0[no] = full.size();
-    // phase 2: full bands of the active terms, straight into each operator's level arrays
-    if (!full.empty()) {
-        std::vector<NSGroup> groups;
-        for (size_t o = 0; o < no; o++) {
-            long long nrow = 0;
-            for (size_t r = req0[o]; r < req0[o + 1]; r++) nrow += full[r].dhi - full[r].dlo + 1;
-            if (!nrow) continue;
-            // slice records address the level's FP32 block array with 32-bit offsets
-            if (nrow * 4 * K2 >= ((long long)1 << 31)) throw Error(MWG_ERR_RANGE, "operator level too large");
-            L[o]->blocks.alloc((size_t)nrow * 4 * K2);
-            L[o]->norms.alloc((size_t)nrow * 4);
-            groups.push_back({req0[o], L[o]->blocks.p});
-        }
-        compute_ns(M, d, b1, full, groups, rows);
-        std::vector<double> hn((size_t)rows[full.size()] * 4);
-        C.ns.nrm.download(hn.data(), hn.size(), C.stream);
-        for (size_t o = 0; o < no; o++) {
-            if (req0[o + 1] == req0[o]) continue;
-            const long long f0 = rows[req0[o]];
-            HC(hipMemcpyAsync(L[o]->norms.p, C.ns.nrm.p + 4 * f0, (size_t)(rows[req0[o + 1]] - f0) * 4 * sizeof(double),
-                              hipMemcpyDeviceToDevice, C.stream));
-            for (size_t r = req0[o]; r < req0[o + 1]; r++) {
-                TermLevel &tl = L[o]->t[full[r].term - toff[o]];
-                const int b = tl.band;
-                tl.blk = (rows[r] - f0) * 4 * (long long)K2;
-                tl.nrm = (rows[r] - f0) * 4;
-                tl.norms.assign(hn.begin() + rows[r] * 4, hn.begin() + (rows[r] + 2 * b + 1) * 4);
-                for (int q = 0; q < 2 * b + 1; q++)
-                    for (int type = 0; type < 4; type++) tl.maxn[type] = std::max(tl.maxn[type], tl.norms[q * 4 + type]);
-                tl.env.assign(4 * (b + 1), 0.0);
-                for (int type = 0; type < 4; type++) {
-                    double run = 0.0;
-                    for (int m = b; m >= 0; m--) {
-                        run = std::max(run, std::max(tl.norms[(b + m) * 4 + type], tl.norms[(b - m) * 4 + type]));
-                        tl.env[type * (b + 1) + m] = run;
-                    }
+    // item lists of all batches; child-scale rows count from their chunk's first row
+    std::vector<double> b1(nb1);
+    std::vector<int> tterm(nt), row0(nn), tlo(nn), thi(nn);
+    std::vector<long long> tdelta(nt), ndelta(nn);
+    for (size_t b = 0; b < bs.size(); b++) {
+        const NSBatch &X = *bs[b];
+        const Plan &P = pl[b];
+        const size_t nr = X.req.size(), ng = X.groups.size();
+        auto gend = [&](size_t g) { return g + 1 < ng ? X.groups[g + 1].r0 : nr; };
+        std::copy(X.b1.begin(), X.b1.end(), b1.begin() + P.b1off);
+        for (size_t c = 0; c + 1 < P.chunk0.size(); c++) {
+            const long long base = P.trow[X.groups[P.chunk0[c]].r0];
+            for (size_t r = X.groups[P.chunk0[c]].r0; r < gend(P.chunk0[c + 1] - 1); r++) {
+                for (long long q = P.lo1[r]; q <= P.hi1[r]; q++) {
+                    const size_t i = P.toff + (size_t)(P.trow[r] + q - P.lo1[r]);
+                    tterm[i] = (int)P.b1off + X.req[r].term;
+                    tdelta[i] = q;
+                }
+                for (long long q = X.req[r].dlo; q <= X.req[r].dhi; q++) {
+                    const size_t i = P.noff + (size_t)(X.rows[r] + q - X.req[r].dlo);
+                    ndelta[i] = q;
+                    row0[i] = (int)(P.trow[r] - base);
+                    tlo[i] = (int)P.lo1[r];
+                    thi[i] = (int)P.hi1[r];
                 }
             }
         }
     }
-    for (size_t o = 0; o < no; o++) {
-        const int nt = ops[o]->nterms();
-        std::vector<LevTerm> lts(nt);
-        std::vector<double> envs;
-        for (int t = 0; t < nt; t++) {
-            const TermLevel &tl = L[o]->t[t];
-            LevTerm &x = lts[t];
-            std::memset(&x, 0, sizeof(x));
-            x.w = ops[o]->weff[t];
-            x.active = tl.active ? 1 : 0;
-            x.band = tl.band;
-            if (!tl.active) continue;
-            x.blk = tl.blk;
-            x.nrm = tl.nrm;
-            x.env = (long long)envs.size();
-            for (int q = 0; q < 4; q++) x.maxn[q] = tl.maxn[q];
-            envs.insert(envs.end(), tl.env.begin(), tl.env.end());
+    size_t off = 0;
+    auto sec = [&](size_t bytes) {
+        const size_t o = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return o;
+    };
+    const size_t o_b1 = sec(nb1 * sizeof(double)), o_tt = sec(nt * sizeof(int)), o_td = sec(nt * sizeof(long long)),
+                 o_nd = sec(nn * sizeof(long long)), o_r0 = sec(nn * sizeof(int)), o_lo = sec(nn * sizeof(int)),
+                 o_hi = sec(nn * sizeof(int));
+    std::vector<char> pk(std::max<size_t>(off, 1));
+    std::memcpy(pk.data() + o_b1, b1.data(), nb1 * sizeof(double));
+    std::memcpy(pk.data() + o_tt, tterm.data(), nt * sizeof(int));
+    std::memcpy(pk.data() + o_td, tdelta.data(), nt * sizeof(long long));
+    std::memcpy(pk.data() + o_nd, ndelta.data(), nn * sizeof(long long));
+    std::memcpy(pk.data() + o_r0, row0.data(), nn * sizeof(int));
+    std::memcpy(pk.data() + o_lo, tlo.data(), nn * sizeof(int));
+    std::memcpy(pk.data() + o_hi, thi.data(), nn * sizeof(int));
+    S.lists.alloc(pk.size());
+    stage_copy(C, S.lists.p, pk.data(), off);
+    const char *L = S.lists.p;
+    const double *db1 = (const double *)(L + o_b1);
+    const int *dtt = (const int *)(L + o_tt), *dr0 = (const int *)(L + o_r0), *dlo = (const int *)(L + o_lo),
+              *dhi = (const int *)(L + o_hi);
+    const long long *dtd = (const long long *)(L + o_td), *dnd = (const long long *)(L + o_nd);
+    S.t1.alloc((size_t)std::max<long long>(tmax, 1) * K2);
+    hipStream_t s = C.stream;
+    for (size_t b = 0; b < bs.size(); b++) {
+        const NSBatch &X = *bs[b];
+        const Plan &P = pl[b];
+        const size_t nr = X.req.size(), ng = X.groups.size();
+        auto gend = [&](size_t g) { return g + 1 < ng ? X.groups[g + 1].r0 : nr; };
+        const double pre1 = std::ldexp(1.0, -(M.n0 + X.d + 1));
+        for (size_t c = 0; c + 1 < P.chunk0.size(); c++) {
+            const long long t0 = P.trow[X.groups[P.chunk0[c]].r0], nt1 = P.trow[gend(P.chunk0[c + 1] - 1)] - t0;
+            HC(launch_tblock_mom(s, (int)nt1, dtt + P.toff + t0, dtd + P.toff + t0, db1, pre1, B.K, M.ccoef.p, C.gx.p,
+                                 C.gw.p, S.t1.p));
+            for (size_t g = P.chunk0[c]; g < P.chunk0[c + 1]; g++) {
+                const long long f0 = X.rows[X.groups[g].r0], n = X.rows[gend(g)] - f0;
+                const size_t i0 = P.noff + (size_t)f0;
+                if (n > 0)
+                    HC(launch_nsblocks(s, (int)n, dnd + i0, dr0 + i0, dlo + i0, dhi + i0, S.t1.p, M.W.p, B.K,
+                                       X.groups[g].blocks, X.nrm + 4 * f0, X.groups[g].norms));
+            }
         }
-        if (envs.empty()) envs.push_back(0.0);
-        upload_async(C, L[o]->dterms, lts.data(), lts.size());
-        upload_async(C, L[o]->denv, envs.data(), envs.size());
-        if ((int)ops[o]->lev.size() <= d) ops[o]->lev.resize(d + 1);
-        ops[o]->lev[d] = std::move(L[o]);
     }
 }
 
-// Levels 0..depth of each (operator, depth) pair, built depth by depth for all operators
-// of an MRA together
-static void ensure_levels(const std::vector<std::pair<OperH *, int>> &need) {
-    int dmax = -1;
-    for (const auto &x : need) dmax = std::max(dmax, x.second);
-    for (int q = 0; q <= dmax; q++) {
-        std::vector<OperH *> todo;
+static long long ns_rows(const std::vector<NSRequest> &req) {
+    long long n = 0;
+    for (const NSRequest &r : req) n += r.dhi - r.dlo + 1;
+    return n;
+}
+
+// the levels of one depth being built for some operators of an MRA
+struct LevelJob {
+    int d = 0;
+    std::vector<OperH *> ops;
+    std::vector<int> toff;         // first term of each operator in the batches' b1
+    std::vector<long long> band;   // translation bands
+    NSBatch near, full;            // near-diagonal blocks of every term; full bands of the active ones
+    std::vector<size_t> req0;      // first full-band request of each operator
+    std::vector<long long> nenv;   // envelope doubles of each operator's level
+    std::vector<std::unique_ptr<LevelH>> L;
+};
+
+// Levels of operators of one MRA at one or more depths, built together: one round trip for
+// the near-diagonal blocks that decide which terms are active at each depth, one for the full
+// bands of the active terms, whatever the number of operators and depths.  Each operator's
+// new blocks and norms (and with16: their split-FP16 copies, one conversion launch per
+// operator) share one allocation, the blocks of its levels adjacent; the per-term data of
+// all new levels share another, sent in one upload.
+static void build_levels(MraH &M, std::vector<LevelJob> &lj, bool with16) {
+    Ctx &C = *M.ctx;
+    const int K2 = M.B.K2;
+    const long long boxmax = std::max({M.boxes[0], M.boxes[1], M.boxes[2]});
+    // phase 1: near-diagonal blocks decide which terms are active
+    long long n1tot = 0, n1max = 0;
+    std::vector<long long> nrm0(lj.size());
+    for (size_t j = 0; j < lj.size(); j++) {
+        LevelJob &J = lj[j];
+        const int d = J.d, m1 = M.n0 + d + 1;
+        const long long lmax1 = (boxmax << (d + 1)) - 1;
+        const size_t no = J.ops.size();
+        J.toff.assign(no + 1, 0);
+        for (size_t o = 0; o < no; o++) J.toff[o + 1] = J.toff[o] + J.ops[o]->nterms();
+        J.near.d = J.full.d = d;
+        J.near.b1.resize(J.toff[no]);
+        J.band.resize(J.toff[no]);
+        J.near.req.resize(J.toff[no]);
+        for (size_t o = 0; o < no; o++)
+            for (int t = 0; t < J.ops[o]->nterms(); t++) {
+                const int g = J.toff[o] + t;
+                const double b1 = J.ops[o]->beta[t] * std::ldexp(1.0, -2 * m1);
+                J.near.b1[g] = b1;
+                const long long band1 = std::min<long long>(lmax1, (long long)std::ceil(1.0 + std::sqrt(GAUSS_CUT / b1)));
+                J.band[g] = std::min<long long>((boxmax << d) - 1, (band1 + 2) / 2);
+                const long long nb = std::min<long long>(J.band[g], 1);
+                J.near.req[g] = {g, -nb, nb};
+            }
+        J.full.b1 = J.near.b1;
+        const long long n1 = ns_rows(J.near.req);
+        nrm0[j] = n1tot;
+        n1tot += n1;
+        n1max = std::max(n1max, n1);
+    }
+    C.ns.blk.alloc((size_t)std::max<long long>(n1max, 1) * 4 * K2);
+    C.ns.nrm.alloc((size_t)std::max<long long>(n1tot, 1) * 4);
+    std::vector<NSBatch *> batches;
+    for (size_t j = 0; j < lj.size(); j++) {
+        lj[j].near.groups = {NSGroup{0, C.ns.blk.p, nullptr}};
+        lj[j].near.nrm = C.ns.nrm.p + 4 * nrm0[j];
+        batches.push_back(&lj[j].near);
+    }
+    compute_ns(M, batches);
+    std::vector<double> nn((size_t)n1tot * 4);
+    C.ns.nrm.download(nn.data(), nn.size(), C.stream);
+    // activity; sizes of the new levels
+    long long n2tot = 0;
+    struct OpMem {
+        OperH *op;
+        size_t bytes = 0, nb16 = 0;  // blocks and norms; K x K blocks
+        std::shared_ptr<DevArena> mem;
+        double *blocks = nullptr;  // first block of its new levels
+        uint4 *b16 = nullptr;
+        int *e16 = nullptr;
+    };
+    std::vector<OpMem> om;
+    auto op_mem = [&](OperH *op) -> OpMem & {
+        for (auto &x : om)
+            if (x.op == op) return x;
+        om.push_back(OpMem{op});
+        return om.back();
+    };
+    const int K = M.B.K;
+    const size_t per16 = with16 ? split16_block_uint4(K, 0, K * K) : 0;
+    size_t small = 0;  // per-term data of all new levels
+    for (size_t j = 0; j < lj.size(); j++) {
+        LevelJob &J = lj[j];
+        const size_t no = J.ops.size();
+        J.L.resize(no);
+        J.req0.assign(no + 1, 0);
+        J.nenv.assign(no, 0);
+        for (size_t o = 0; o < no; o++) {
+            J.L[o] = std::make_unique<LevelH>();
+            J.L[o]->t.resize(J.ops[o]->nterms());
+            J.req0[o] = J.full.req.size();
+            long long nrow = 0;
+            for (int t = 0; t < J.ops[o]->nterms(); t++) {
+                const int g = J.toff[o] + t;
+                double tn = 0.0, wn = 0.0;
+                for (long long q = 0; q <= J.near.req[g].dhi - J.near.req[g].dlo; q++) {
+                    const double *p = &nn[(nrm0[j] + J.near.rows[g] + q) * 4];
+                    tn = std::max(tn, p[0]);
+                    wn = std::max({wn, p[1], p[2], p[3]});
+                }
+                const bool active = (J.d == 0) || (wn > ACT_REL * J.ops[o]->prec * tn);
+                J.L[o]->t[t].active = active;
+                if (active) {
+                    J.L[o]->t[t].band = (int)J.band[g];
+                    J.full.req.push_back({g, -J.band[g], J.band[g]});
+                    nrow += 2 * J.band[g] + 1;
+                    J.nenv[o] += 4 * (J.band[g] + 1);
+                }
+            }
+            // slice records address the level's FP32 block array with 32-bit offsets
+            if (nrow * 4 * K2 >= ((long long)1 << 31)) throw Error(MWG_ERR_RANGE, "operator level too large");
+            J.L[o]->nblocks = (size_t)nrow * 4 * K2;
+            J.L[o]->nnorms = (size_t)nrow * 4;
+            OpMem &X = op_mem(J.ops[o]);
+            X.bytes += (J.L[o]->nblocks + J.L[o]->nnorms) * sizeof(double);
+            X.nb16 += J.L[o]->nblocks / K2;
+            small += arena_bytes<LevTerm>(J.ops[o]->nterms()) + arena_bytes<double>((size_t)std::max<long long>(J.nenv[o], 1));
+        }
+        J.req0[no] = J.full.req.size();
+        nrm0[j] = n2tot;
+        n2tot += ns_rows(J.full.req);
+    }
+    // each operator: the blocks of all its new levels, then their norms (rows of 32 bytes:
+    // adjacent arrays stay aligned), then the split-FP16 copies
+    for (OpMem &X : om) {
+        const size_t b16 = with16 ? arena_bytes<uint4>(X.nb16 * per16) + arena_bytes<int>(X.nb16) : 0;
+        X.mem = std::make_shared<DevArena>(C, X.bytes + 256 + b16);
+    }
+    auto small_mem = std::make_shared<DevArena>(C, small);
+    for (int pass = 0; pass < 2; pass++)
+        for (LevelJob &J : lj)
+            for (size_t o = 0; o < J.ops.size(); o++) {
+                LevelH &LV = *J.L[o];
+                OpMem &X = op_mem(J.ops[o]);
+                LV.mem = X.mem;
+                if (pass == 0) {
+                    LV.blocks = LV.mem->take<double>(LV.nblocks, 8);
+                    if (!X.blocks) X.blocks = LV.blocks;
+                } else {
+                    LV.norms = LV.mem->take<double>(LV.nnorms, 8);
+                }
+            }
+    if (with16)
+        for (OpMem &X : om) {
+            X.mem->used = (X.mem->used + 255) & ~(size_t)255;
+            X.b16 = X.mem->take<uint4>(X.nb16 * per16);
+            X.e16 = X.mem->take<int>(X.nb16);
+            size_t at = 0;
+            for (LevelJob &J : lj)
+                for (size_t o = 0; o < J.ops.size(); o++)
+                    if (J.ops[o] == X.op) {
+                        LevelH &LV = *J.L[o];
+                        LV.blocks16 = X.b16 + at * per16;
+                        LV.exp16 = X.e16 + at;
+                        LV.has16 = true;
+                        at += LV.nblocks / K2;
+                    }
+        }
+    // phase 2: full bands of the active terms, straight into the level arrays
+    batches.clear();
+    if (n2tot > 0) C.ns.nrm.alloc((size_t)n2tot * 4);
+    for (size_t j = 0; j < lj.size(); j++) {
+        LevelJob &J = lj[j];
+        if (J.full.req.empty()) continue;
+        for (size_t o = 0; o < J.ops.size(); o++)
+            if (J.L[o]->nblocks) J.full.groups.push_back({J.req0[o], J.L[o]->blocks, J.L[o]->norms});
+        J.full.nrm = C.ns.nrm.p + 4 * nrm0[j];
+        batches.push_back(&J.full);
+    }
+    if (!batches.empty()) compute_ns(M, batches);
+    if (with16)
+        for (const OpMem &X : om)
+            if (X.nb16) HC(launch_split16(C.stream, (int)X.nb16, 0, nullptr, K2, K, X.blocks, X.b16, X.e16, 0));
+    std::vector<double> hn((size_t)n2tot * 4);
+    if (n2tot > 0) C.ns.nrm.download(hn.data(), hn.size(), C.stream);
+    // host-side term data, and the device copy of the per-term data in one upload
+    std::vector<char> img(std::max<size_t>(small, 1));
+    for (size_t j = 0; j < lj.size(); j++) {
+        LevelJob &J = lj[j];
+        for (size_t o = 0; o < J.ops.size(); o++) {
+            LevelH &LV = *J.L[o];
+            if (J.req0[o + 1] > J.req0[o]) {
+                const long long f0 = J.full.rows[J.req0[o]];
+                for (size_t r = J.req0[o]; r < J.req0[o + 1]; r++) {
+                    TermLevel &tl = LV.t[J.full.req[r].term - J.toff[o]];
+                    const int b = tl.band;
+                    const long long h0 = (nrm0[j] + J.full.rows[r]) * 4;
+                    tl.blk = (J.full.rows[r] - f0) * 4 * (long long)K2;
+                    tl.nrm = (J.full.rows[r] - f0) * 4;
+                    tl.norms.assign(hn.begin() + h0, hn.begin() + h0 + (2 * b + 1) * 4);
+                    for (int q = 0; q < 2 * b + 1; q++)
+                        for (int type = 0; type < 4; type++)
+                            tl.maxn[type] = std::max(tl.maxn[type], tl.norms[q * 4 + type]);
+                    tl.env.assign(4 * (b + 1), 0.0);
+                    for (int type = 0; type < 4; type++) {
+                        double run = 0.0;
+                        for (int m = b; m >= 0; m--) {
+                            run = std::max(run, std::max(tl.norms[(b + m) * 4 + type], tl.norms[(b - m) * 4 + type]));
+                            tl.env[type * (b + 1) + m] = run;
+                        }
+                    }
+                }
+            }
+            const int nt = J.ops[o]->nterms();
+            std::vector<LevTerm> lts(nt);
+            std::vector<double> envs;
+            for (int t = 0; t < nt; t++) {
+                const TermLevel &tl = LV.t[t];
+                LevTerm &x = lts[t];
+                std::memset(&x, 0, sizeof(x));
+                x.w = J.ops[o]->weff[t];
+                x.active = tl.active ? 1 : 0;
+                x.band = tl.band;
+                if (!tl.active) continue;
+                x.blk = tl.blk;
+                x.nrm = tl.nrm;
+                x.env = (long long)envs.size();
+                for (int q = 0; q < 4; q++) x.maxn[q] = tl.maxn[q];
+                envs.insert(envs.end(), tl.env.begin(), tl.env.end());
+            }
+            if (envs.empty()) envs.push_back(0.0);
+            LV.dterms = small_mem->take<LevTerm>(nt);
+            LV.denv = small_mem->take<double>(envs.size());
+            std::memcpy(img.data() + ((char *)LV.dterms - small_mem->p), lts.data(), nt * sizeof(LevTerm));
+            std::memcpy(img.data() + ((char *)LV.denv - small_mem->p), envs.data(), envs.size() * sizeof(double));
+            LV.small = small_mem;
+        }
+    }
+    stage_copy(C, small_mem->p, img.data(), small);
+    for (LevelJob &J : lj)
+        for (size_t o = 0; o < J.ops.size(); o++) {
+            if ((int)J.ops[o]->lev.size() <= J.d) J.ops[o]->lev.resize(J.d + 1);
+            J.ops[o]->lev[J.d] = std::move(J.L[o]);
+        }
+}
+
+// Levels 0..depth of each (operator, depth) pair: every missing level of the operators of
+// one MRA is built in one build_levels call (with16: with their split-FP16 copies)
+static void ensure_levels(const std::vector<std::pair<OperH *, int>> &need, bool with16 = false) {
+    std::vector<MraH *> mras;
+    for (const auto &x : need)
+        if (std::find(mras.begin(), mras.end(), x.first->mra) == mras.end()) mras.push_back(x.first->mra);
+    for (MraH *m : mras) {
+        int dmax = -1;
         for (const auto &x : need)
-            if (q <= x.second && !x.first->has(q) && std::find(todo.begin(), todo.end(), x.first) == todo.end())
-                todo.push_back(x.first);
-        while (!todo.empty()) {
-            std::vector<OperH *> same, rest;
-            for (OperH *o : todo) (o->mra == todo[0]->mra ? same : rest).push_back(o);
-            build_levels(same, q);
-            todo.swap(rest);
+            if (x.first->mra == m) dmax = std::max(dmax, x.second);
+        std::vector<LevelJob> lj;
+        for (int q = dmax; q >= 0; q--) {
+            LevelJob J;
+            J.d = q;
+            for (const auto &x : need)
+                if (x.first->mra == m && q <= x.second && !x.first->has(q) &&
+                    std::find(J.ops.begin(), J.ops.end(), x.first) == J.ops.end())
+                    J.ops.push_back(x.first);
+            if (!J.ops.empty()) lj.push_back(std::move(J));
+        }
+        if (!lj.empty()) build_levels(*m, lj, with16);
+    }
+}
+
+// Split-FP16 copies (mixed apply) of levels 0..depth of each (operator, depth) pair, whose
+// levels exist: one allocation per operator for all its levels that lack one, and one
+// conversion launch per run of levels whose blocks are adjacent in memory
+static void ensure_levels16(Ctx &C, const std::vector<std::pair<OperH *, int>> &need, int K) {
+    std::vector<std::pair<OperH *, int>> ops;
+    for (const auto &x : need) {
+        auto it = std::find_if(ops.begin(), ops.end(), [&](const std::pair<OperH *, int> &y) { return y.first == x.first; });
+        if (it == ops.end())
+            ops.push_back(x);
+        else
+            it->second = std::max(it->second, x.second);
+    }
+    const size_t per = split16_block_uint4(K, 0, K * K), KK = (size_t)K * K;
+    for (const auto &x : ops) {
+        OperH &O = *x.first;
+        std::vector<LevelH *> todo;
+        for (int q = 0; q <= x.second; q++) {
+            LevelH &LV = *O.lev[q];
+            if (LV.has16) continue;
+            if (LV.nblocks)
+                todo.push_back(&LV);
+            else
+                LV.has16 = true;
+        }
+        if (todo.empty()) continue;
+        std::sort(todo.begin(), todo.end(), [](const LevelH *a, const LevelH *b) { return a->blocks < b->blocks; });
+        size_t nb = 0;
+        for (const LevelH *LV : todo) nb += LV->nblocks / KK;
+        auto A = std::make_shared<DevArena>(C, arena_bytes<uint4>(nb * per) + arena_bytes<int>(nb));
+        uint4 *b16 = A->take<uint4>(nb * per);
+        int *e16 = A->take<int>(nb);
+        size_t at = 0;
+        for (size_t i = 0; i < todo.size();) {
+            size_t j = i, run = 0;
+            while (j < todo.size() && (j == i || todo[j]->blocks == todo[j - 1]->blocks + todo[j - 1]->nblocks)) {
+                LevelH &LV = *todo[j];
+                LV.mem16 = A;
+                LV.blocks16 = b16 + (at + run) * per;
+                LV.exp16 = e16 + at + run;
+                LV.has16 = true;
+                run += LV.nblocks / KK;
+                j++;
+            }
+            HC(launch_split16(C.stream, (int)run, 0, nullptr, (int)KK, K, todo[i]->blocks, b16 + at * per, e16 + at, 0));
+            at += run;
+            i = j;
         }
     }
 }
@@ -1194,7 +1453,8 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             J.fmax[d][0] = std::max(J.fmax[d][0], run);
             run = std::max(run, leafmax[d]);
         }
-        need.push_back({J.op, dmax_in});
+        // output trees reach at most a level or so below their inputs: build one more
+        need.push_back({J.op, std::min(dmax_in + 1, M.maxDepth)});
         J.out->make_roots();
         // output trees end between a third of the input's size (pair Poissons) and about its
         // size (Helmholtz): reserve half of it once (beyond, the arrays grow geometrically:
@@ -1210,7 +1470,10 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         genTotal += cap;
     }
     // operator levels down to the inputs' depths, all operators at once (deeper ones on demand)
-    ensure_levels(need);
+    PT.lap("inputs");
+    ensure_levels(need, mixed);
+    if (mixed) ensure_levels16(C, need, K);
+    PT.lap("operator levels");
     // generated-node tables
     C.genHashKeys.alloc(genTotal);
     C.genHashVals.alloc(genTotal);
@@ -1236,7 +1499,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
     // operator level, input and output arrays, thresholds from the running norm
     auto add_table = [&](JobH &J, int d, std::vector<JobTab> &jt, std::vector<BoxJob> &bjobs, int &ntabs,
                          int &max_terms) {
-        if (!J.op->has(d)) ensure_levels({{J.op, d}});
+        if (!J.op->has(d)) ensure_levels({{J.op, d}}, mixed);
         JobTab T;
         std::memset(&T, 0, sizeof(T));
         LevelH &LV = *J.op->lev[d];
@@ -1246,8 +1509,8 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         T.ntab = nt;
         BoxJob B;
         std::memset(&B, 0, sizeof(B));
-        B.terms = LV.dterms.p;
-        B.env = LV.denv.p;
+        B.terms = LV.dterms;
+        B.env = LV.denv;
         B.thr = T.thr;
         for (int q = 0; q < 8; q++) B.F[q] = J.fmax[d][q];
         B.nterms = nt;
@@ -1262,22 +1525,16 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         T.genHash = DHash{C.genHashKeys.p + J.genOff, C.genHashVals.p + J.genOff, J.genMask};
         T.inCoef = J.in->coef.p;
         T.inNorm = J.in->norm.p;
-        T.operBlocks = LV.blocks.p;
+        T.operBlocks = LV.blocks;
         if (mixed) {
-            if (!LV.has16) {
-                const size_t nb = LV.blocks.n / ((size_t)K * K);
-                LV.blocks16.alloc(nb * split16_block_uint4(K, 0, K * K));
-                LV.exp16.alloc(nb);
-                HC(launch_split16(s, (int)nb, 0, nullptr, K * K, K, LV.blocks.p, LV.blocks16.p, LV.exp16.p, 0));
-                LV.has16 = true;
-            }
+            if (!LV.has16) ensure_levels16(C, {{J.op, d}}, K);
             J.in->ensure16();
-            T.operBlocks16 = LV.blocks16.p;
-            T.operExp16 = LV.exp16.p;
+            T.operBlocks16 = LV.blocks16;
+            T.operExp16 = LV.exp16;
             T.inCoef16 = J.in->coef16.p;
             T.inExp16 = J.in->exp16.p;
         }
-        T.operNorms = LV.norms.p;
+        T.operNorms = LV.norms;
         T.outCoef = J.out->coef.p;
         T.outNorm = J.out->norm.p;
         for (int q = 0; q < 3; q++) T.lmax[q] = M.boxes[q] << d;
@@ -1436,7 +1693,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
                 std::vector<std::pair<OperH *, int>> deeper;
                 for (int j = 0; j < njobs; j++)
                     if (!jobs[j].op->has(d)) deeper.push_back({jobs[j].op, d});
-                if (!deeper.empty()) ensure_levels(deeper);
+                if (!deeper.empty()) ensure_levels(deeper, mixed);
             }
             for (int j = 0; j < njobs; j++) add_table(jobs[j], d, jt, bj, ntabs, max_terms);
             C.tpre.alloc((size_t)ntabs + njobs + 1);
@@ -1635,7 +1892,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             for (int j : active)
                 for (int g : jobs[j].work)
                     if (!jobs[j].op->has(jobs[j].out->depth[g])) deeper.push_back({jobs[j].op, jobs[j].out->depth[g]});
-            if (!deeper.empty()) ensure_levels(deeper);
+            if (!deeper.empty()) ensure_levels(deeper, mixed);
         }
         for (int j : active) {
             JobH &J = jobs[j];
@@ -3107,11 +3364,16 @@ int mwg_oper_blocks(mwg_oper oper, int term, int depth, int64_t dlo, int64_t dhi
     Ctx *C = oper->mra->ctx;
     return guarded(C, [&] {
         MraH &M = *oper->mra;
-        DBuf<double> b;
+        DBuf<double> b, nrm;
         b.alloc((size_t)(dhi - dlo + 1) * 4 * M.B.K2);
-        std::vector<long long> rows;
-        const std::vector<double> b1{oper->beta[term] * std::ldexp(1.0, -2 * (M.n0 + depth + 1))};
-        compute_ns(M, depth, b1, {NSRequest{0, dlo, dhi}}, {NSGroup{0, b.p}}, rows);
+        nrm.alloc((size_t)(dhi - dlo + 1) * 4);
+        NSBatch X;
+        X.d = depth;
+        X.b1 = {oper->beta[term] * std::ldexp(1.0, -2 * (M.n0 + depth + 1))};
+        X.req = {NSRequest{0, dlo, dhi}};
+        X.groups = {NSGroup{0, b.p, nullptr}};
+        X.nrm = nrm.p;
+        compute_ns(M, {&X});
         b.download(out, (size_t)(dhi - dlo + 1) * 4 * M.B.K2, C->stream);
     });
 }

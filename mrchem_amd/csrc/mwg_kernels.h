@@ -133,7 +133,7 @@ hipError_t launch_tblock_mom(hipStream_t s, int n, const int *item_term, const l
                              const double *gw, double *out);
 hipError_t launch_nsblocks(hipStream_t s, int n, const long long *item_delta, const int *item_row0,
                            const int *item_tlo, const int *item_thi, const double *t1, const double *W, int K,
-                           double *blocks, double *norms);
+                           double *blocks, double *norms, double *norms2 = nullptr);
 hipError_t launch_norms(hipStream_t s, int n, const int *nodes, const double *coef, double *norm, int K3);
 hipError_t launch_dot(hipStream_t s, int n, const int *ia, const int *ib, const int *c0, const double *A,
                       const double *B, int K3, double *part);

@@ -117,23 +117,27 @@ __global__ void k_tblock(int n, const int *item_term, const long long *item_delt
 
 // Same block by moments: t[i][j] = pre * sum_h sum_p ccoef[h][i][j][p] M_h[p] with
 // M_h[p] = int_half exp(-b (delta+z)^2) L_p(t(z)) dz on the same Gauss-Legendre panels.
-// Threads take quadrature points independently; the 4K moments are reduced at the end.
-__global__ void __launch_bounds__(256) k_tblock_mom(int n, const int *item_term, const long long *item_delta,
-                                                    const double *bvals, double pre, int K, const double *ccoef,
-                                                    const double *gx, const double *gw, double *out) {
+// One wave per block: lanes take quadrature points independently and keep the 2 x 2K
+// moments in registers; one wave reduction, then the K^2 entries.
+template <int K>
+__global__ void __launch_bounds__(64) k_tblock_mom(int n, const int *item_term, const long long *item_delta,
+                                                   const double *bvals, double pre, const double *ccoef,
+                                                   const double *gx, const double *gw, double *out) {
     const int it = blockIdx.x;
     if (it >= n) return;
-    const int P = 2 * K, K2 = K * K;
-    __shared__ double red[4][4 * MAXK];
-    __shared__ double Ms[4 * MAXK];
+    constexpr int P = 2 * K, K2 = K * K;
+    __shared__ double Ms[2 * P];
     const double b = bvals[item_term[it]];
     const double delta = (double)item_delta[it];
     const double zc = -delta;
     const double R = sqrt(GAUSS_CUT_D / b);
     const double pw = (b > 4.0) ? 0.5 / sqrt(b) : 0.25;
-    double M[2][2 * MAXK];
+    double M[2][P];
+#pragma unroll
     for (int h = 0; h < 2; h++)
+#pragma unroll
         for (int p = 0; p < P; p++) M[h][p] = 0.0;
+#pragma unroll
     for (int half = 0; half < 2; half++) {
         const double lo = half == 0 ? -1.0 : 0.0, hi = half == 0 ? 0.0 : 1.0;
         const double a0 = fmax(lo, zc - R), a1 = fmin(hi, zc + R);
@@ -141,7 +145,7 @@ __global__ void __launch_bounds__(256) k_tblock_mom(int n, const int *item_term,
         int np = (int)ceil((a1 - a0) / pw);
         if (np < 1) np = 1;
         const double ph = (a1 - a0) / np;
-        for (int e = threadIdx.x; e < np * 24; e += blockDim.x) {
+        for (int e = threadIdx.x; e < np * 24; e += 64) {
             const int pnl = e / 24, q = e % 24;
             const double z = a0 + pnl * ph + ph * gx[q];
             const double g = exp(-b * (delta + z) * (delta + z)) * ph * gw[q];
@@ -149,6 +153,7 @@ __global__ void __launch_bounds__(256) k_tblock_mom(int n, const int *item_term,
             double p0 = 1.0, p1 = x;
             M[half][0] += g;
             if (P > 1) M[half][1] += g * sqrt(3.0) * x;
+#pragma unroll
             for (int k = 2; k < P; k++) {
                 const double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
                 p0 = p1;
@@ -157,31 +162,31 @@ __global__ void __launch_bounds__(256) k_tblock_mom(int n, const int *item_term,
             }
         }
     }
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
     for (int h = 0; h < 2; h++)
+#pragma unroll
         for (int p = 0; p < P; p++) {
             double v = M[h][p];
             for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-            if (lane == 0) red[wave][h * P + p] = v;
+            if (threadIdx.x == 0) Ms[h * P + p] = v;
         }
     __syncthreads();
-    if (threadIdx.x < 2 * P) Ms[threadIdx.x] = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
-    __syncthreads();
-    if (threadIdx.x < K2) {
+    for (int e = threadIdx.x; e < K2; e += 64) {
         double t = 0.0;
         for (int h = 0; h < 2; h++) {
-            const double *c = ccoef + ((size_t)h * K2 + threadIdx.x) * P;
+            const double *c = ccoef + ((size_t)h * K2 + e) * P;
             for (int p = 0; p < P; p++) t += c[p] * Ms[h * P + p];
         }
-        out[(size_t)it * K2 + threadIdx.x] = t * pre;
+        out[(size_t)it * K2 + e] = t * pre;
     }
 }
 
 // NS blocks from the child-scale scaling blocks: R = W M W^T with
-// M = [[t(2d), t(2d-1)], [t(2d+1), t(2d)]]; type = 2*gt_bit + ft_bit.
+// M = [[t(2d), t(2d-1)], [t(2d+1), t(2d)]]; type = 2*gt_bit + ft_bit.  The 4 block norms of
+// each translation go to norms and, when given, to norms2.
 __global__ void k_nsblocks(int n, const long long *item_delta, const int *item_row0, const int *item_tlo,
                            const int *item_thi, const double *t1, const double *W, int K, double *blocks,
-                           double *norms) {
+                           double *norms, double *norms2) {
     const int it = blockIdx.x;
     if (it >= n) return;
     const int N = 2 * K, K2 = K * K, tid = threadIdx.x, nt = blockDim.x;
@@ -223,7 +228,11 @@ __global__ void k_nsblocks(int n, const long long *item_delta, const int *item_r
             for (int q = 0; q < 4; q++) red[q][tid] += red[q][tid + s];
         __syncthreads();
     }
-    if (tid < 4) norms[(size_t)it * 4 + tid] = sqrt(red[tid][0]);
+    if (tid < 4) {
+        const double v = sqrt(red[tid][0]);
+        norms[(size_t)it * 4 + tid] = v;
+        if (norms2) norms2[(size_t)it * 4 + tid] = v;
+    }
 }
 
 // ----------------------------------------------------------------- tensor line passes
@@ -2285,16 +2294,17 @@ hipError_t launch_tblock_mom(hipStream_t s, int n, const int *item_term, const l
                              const double *b, double pre, int K, const double *ccoef, const double *gx,
                              const double *gw, double *out) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_tblock_mom, dim3(n), dim3(256), 0, s, n, item_term, item_delta, b, pre, K, ccoef, gx, gw, out);
+    MWG_DISPATCH_K(K, hipLaunchKernelGGL(k_tblock_mom<KK>, dim3(n), dim3(64), 0, s, n, item_term, item_delta, b, pre,
+                                         ccoef, gx, gw, out));
     return hipGetLastError();
 }
 
 hipError_t launch_nsblocks(hipStream_t s, int n, const long long *item_delta, const int *item_row0,
                            const int *item_tlo, const int *item_thi, const double *t1, const double *W, int K,
-                           double *blocks, double *norms) {
+                           double *blocks, double *norms, double *norms2) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_nsblocks, dim3(n), dim3(256), 0, s, n, item_delta, item_row0, item_tlo, item_thi, t1, W, K,
-                       blocks, norms);
+                       blocks, norms, norms2);
     return hipGetLastError();
 }
 
