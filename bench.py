@@ -477,7 +477,8 @@ def main():
         r = autotune(mra, trees, ops, args.prec, flags)
         tuning[name] = dict(fp64_chunk=r["fp64_chunk"], slice_chunk=r["slice_chunk"])
         log(f"[bench] autotune {name}: fp64_chunk {r['fp64_chunk']}, slice_chunk {r['slice_chunk']}, "
-            f"contraction {1e3 * r['t_contract']:.2f} ms ({len(r['table'])} candidates, {r['seconds']:.1f} s)")
+            f"apply {1e3 * r['t_best']:.2f} ms, contraction {1e3 * r['t_contract']:.2f} ms "
+            f"({len(r['table'])} candidates, {r['seconds']:.1f} s)")
 
     tune("fp64", G.MWG_APPLY_FP64)
     res64 = run_variant(G, mra, trees, ops, args.prec, G.MWG_APPLY_FP64, args.steps, args.warmup)

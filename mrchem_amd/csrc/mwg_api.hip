@@ -1681,36 +1681,61 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         char *ctlb = (char *)C.lctl.p;
         int *gfirst = (int *)(ctlb + offsetof(LoopCtl, genFirst));
         int *queue = (int *)(ctlb + offsetof(LoopCtl, queue));
+        unsigned *sctr = (unsigned *)(ctlb + offsetof(LoopCtl, stampctr));
         for (int j = 0; j < njobs; j++) jobs[j].work.clear();
         PT.lap("loop setup");
-        auto enqueue = [&](int d) {
+        // job tables of every depth in a region of their own: the depths whose operator levels
+        // exist go up in one upload before the first iteration, deeper ones as they are queued
+        int ntabs = 0, max_terms = 0;
+        for (const JobH &J : jobs) {
+            ntabs += J.op->nterms();
+            max_terms = std::max(max_terms, J.op->nterms());
+        }
+        C.tpre.alloc((size_t)ntabs + njobs + 1);
+        C.tpre0.alloc((size_t)ntabs + njobs + 1);
+        C.tabs.alloc((size_t)std::max(ntabs, 1));
+        C.tbmax.alloc((size_t)std::max(njobs * max_terms, 1));
+        const size_t o_bj = ((size_t)njobs * sizeof(JobTab) + 255) & ~(size_t)255;
+        const size_t tstride = (o_bj + (size_t)njobs * sizeof(BoxJob) + 255) & ~(size_t)255;
+        C.ltab.alloc((size_t)(M.maxDepth + 1) * tstride);
+        auto fill_tables = [&](int d, char *dst) {
             std::vector<JobTab> jt;
             std::vector<BoxJob> bj;
-            int ntabs = 0, max_terms = 0;
+            int nt = 0, mt = 0;
             jt.reserve(njobs);
             bj.reserve(njobs);
-            {
-                std::vector<std::pair<OperH *, int>> deeper;
-                for (int j = 0; j < njobs; j++)
-                    if (!jobs[j].op->has(d)) deeper.push_back({jobs[j].op, d});
-                if (!deeper.empty()) ensure_levels(deeper, mixed);
-            }
-            for (int j = 0; j < njobs; j++) add_table(jobs[j], d, jt, bj, ntabs, max_terms);
-            C.tpre.alloc((size_t)ntabs + njobs + 1);
-            C.tpre0.alloc((size_t)ntabs + njobs + 1);
+            for (int j = 0; j < njobs; j++) add_table(jobs[j], d, jt, bj, nt, mt);
             for (int q = 0; q < njobs; q++) {
                 jt[q].pre = C.tpre.p + jt[q].tab0 + q;
                 jt[q].pre0 = C.tpre0.p + jt[q].tab0 + q;
             }
-            const size_t o_bj = ((size_t)njobs * sizeof(JobTab) + 255) & ~(size_t)255;
-            std::vector<char> blob(o_bj + (size_t)njobs * sizeof(BoxJob));
-            std::memcpy(blob.data(), jt.data(), (size_t)njobs * sizeof(JobTab));
-            std::memcpy(blob.data() + o_bj, bj.data(), (size_t)njobs * sizeof(BoxJob));
-            upload_async(C, C.ltab, blob.data(), blob.size());
-            JobTab *djt = (JobTab *)C.ltab.p;
-            BoxJob *dbj = (BoxJob *)(C.ltab.p + o_bj);
-            C.tabs.alloc((size_t)std::max(ntabs, 1));
-            C.tbmax.alloc((size_t)std::max(njobs * max_terms, 1));
+            std::memcpy(dst, jt.data(), (size_t)njobs * sizeof(JobTab));
+            std::memcpy(dst + o_bj, bj.data(), (size_t)njobs * sizeof(BoxJob));
+        };
+        int dpre = -1;  // levels 0..dpre exist for every job
+        while (dpre < M.maxDepth) {
+            bool all = true;
+            for (const JobH &J : jobs) all = all && J.op->has(dpre + 1);
+            if (!all) break;
+            dpre++;
+        }
+        if (dpre >= 0) {
+            std::vector<char> blob((size_t)(dpre + 1) * tstride, 0);
+            for (int d = 0; d <= dpre; d++) fill_tables(d, blob.data() + (size_t)d * tstride);
+            stage_copy(C, C.ltab.p, blob.data(), blob.size());
+        }
+        auto enqueue = [&](int d) {
+            if (d > dpre) {
+                std::vector<std::pair<OperH *, int>> deeper;
+                for (int j = 0; j < njobs; j++)
+                    if (!jobs[j].op->has(d)) deeper.push_back({jobs[j].op, d});
+                if (!deeper.empty()) ensure_levels(deeper, mixed);
+                std::vector<char> blob(tstride, 0);
+                fill_tables(d, blob.data());
+                stage_copy(C, C.ltab.p + (size_t)d * tstride, blob.data(), tstride);
+            }
+            JobTab *djt = (JobTab *)(C.ltab.p + (size_t)d * tstride);
+            BoxJob *dbj = (BoxJob *)(C.ltab.p + (size_t)d * tstride + o_bj);
             const int cur = d & 1;
             const int *nwp = (const int *)(ctlb + offsetof(LoopCtl, nw)) + cur;
             int *nwnext = (int *)(ctlb + offsetof(LoopCtl, nw)) + (cur ^ 1);
@@ -1735,16 +1760,17 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             HC(launch_build_tasks(s, WCAP, nseg, 0, chunk_dl(d), schunk_dl(d), C.gtotal.p, C.caps, C.genCount.p, genPoolCap,
                                   C.gofs.p, C.tasks.p, C.stasks.p, C.reds.p, dsum, C.segs.p, C.segcnt.p, C.gslice.p,
                                   C.genOverflow.p, dw, dtab, njobs, nwp));
+            // (the fill's last workgroup stamps the contraction's start)
             if (!mixed)
                 HC(launch_enum(s, 1, WCAP, 0, dw, djt, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, C.gofs.p, C.tuples.p,
-                               dsum, C.segs.p, nwp, queue + 2));
+                               dsum, C.segs.p, nwp, queue + 2, sctr, dts));
             else
                 HC(launch_enum_mixed(s, 1, K, WCAP, 0, dw, djt, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, C.gofs.p,
-                                     C.tuples.p, C.slices.p, dsum, C.segs.p, C.gslice.p, nwp, queue + 2));
+                                     C.tuples.p, C.slices.p, dsum, C.segs.p, C.gslice.p, nwp, queue + 2, sctr, dts));
             const int grid64 = std::max(1, (C.caps.tasks + 3) / 4);
             const int gridred = std::min(4096, std::max(1, C.caps.reds));
             HC(launch_contract_fp64(s, K, 0, dsum, grid64, C.tasks.p, dw, djt, C.tabs.p, C.tuples.p, C.genPool.p,
-                                    C.partials.p, dts));
+                                    C.partials.p));
             if (mixed) {
                 const int tpb = mfma_tasks_per_block(K);
                 const int gridmf = std::max(1, (C.caps.stasks + tpb - 1) / tpb);

@@ -1215,18 +1215,31 @@ __device__ __forceinline__ void enum_node(int g, int part, int nparts, const Wor
     }
 }
 
+// the last workgroup of the grid to get here writes the device clock into *ts and re-arms
+// the counter for the next launch
+__device__ inline void stamp_last(unsigned *ctr, unsigned long long *ts) {
+    __syncthreads();
+    if (threadIdx.x == 0 && atomicAdd(ctr, 1u) == gridDim.x - 1) {
+        *ts = (unsigned long long)wall_clock64();
+        *ctr = 0u;
+    }
+}
+
 // one workgroup per (work node, part) (grid-stride; nwp: the node count on the device, the
 // parts then follow from it as on the host)
 template <int MODE>
 __global__ void __launch_bounds__(256) k_enum(int nw, const int *nwp, int *queue, int nparts, const WorkItem *work,
                                               const JobTab *jobs, const TermTab *tabs, GenState gs, int *thread_off,
                                               int *seg_total, const long long *seg_ofs, Tuple *out,
-                                              const TaskSummary *sum, const SegRec *segs) {
-    if (MODE == 1 && sum->overflow) return;
-    const int n = nwp ? *nwp : nw, np = nwp ? enum_parts(n) : nparts;
-    for_items(n * np, queue, [&](int b) {
-        enum_node<MODE>(b / np, b % np, np, work, jobs, tabs, gs, thread_off, seg_total, seg_ofs, out, segs);
-    });
+                                              const TaskSummary *sum, const SegRec *segs, unsigned *sctr,
+                                              unsigned long long *sts) {
+    if (!(MODE == 1 && sum->overflow)) {
+        const int n = nwp ? *nwp : nw, np = nwp ? enum_parts(n) : nparts;
+        for_items(n * np, queue, [&](int b) {
+            enum_node<MODE>(b / np, b % np, np, work, jobs, tabs, gs, thread_off, seg_total, seg_ofs, out, segs);
+        });
+    }
+    if (MODE == 1 && sts) stamp_last(sctr, sts);
 }
 
 // ----------------------------------------------------------------- apply: predictor features
@@ -1842,13 +1855,16 @@ __global__ void __launch_bounds__(256) k_enum_mixed(int nw, const int *nwp, int 
                                                     GenState gs, int *thread_off, int *seg_total,
                                                     const long long *seg_ofs, Tuple *out, Slice *sout,
                                                     const TaskSummary *sum, const SegRec *segs,
-                                                    const long long *gslice) {
-    if (MODE == 1 && sum->overflow) return;
-    const int n = nwp ? *nwp : nw, np = nwp ? enum_parts(n) : nparts;
-    for_items(n * np, queue, [&](int b) {
-        enum_mixed_node<MODE>(b / np, b % np, np, K2, work, jobs, tabs, gs, thread_off, seg_total, seg_ofs, out,
-                              sout, segs, gslice);
-    });
+                                                    const long long *gslice, unsigned *sctr,
+                                                    unsigned long long *sts) {
+    if (!(MODE == 1 && sum->overflow)) {
+        const int n = nwp ? *nwp : nw, np = nwp ? enum_parts(n) : nparts;
+        for_items(n * np, queue, [&](int b) {
+            enum_mixed_node<MODE>(b / np, b % np, np, K2, work, jobs, tabs, gs, thread_off, seg_total, seg_ofs, out,
+                                  sout, segs, gslice);
+        });
+    }
+    if (MODE == 1 && sts) stamp_last(sctr, sts);
 }
 
 
@@ -2397,15 +2413,15 @@ hipError_t launch_gen_compute(hipStream_t s, int first, int cap, const JobTab *j
 hipError_t launch_enum(hipStream_t s, int mode, int nw, int nparts, const WorkItem *work, const JobTab *jobs,
                        const TermTab *tabs, const GenState &gs, int *thread_off, int *seg_total,
                        const long long *seg_ofs, Tuple *out, const TaskSummary *sum, const SegRec *segs,
-                       const int *nwp, int *queue) {
+                       const int *nwp, int *queue, unsigned *stamp_ctr, unsigned long long *stamp_ts) {
     if (nw <= 0) return hipSuccess;
     const unsigned nb = nwp ? (unsigned)DEVCOUNT_GRID : (unsigned)nw * (unsigned)nparts;
     if (mode == 0)
         hipLaunchKernelGGL(k_enum<0>, dim3(nb), dim3(256), 0, s, nw, nwp, queue, nparts, work, jobs, tabs, gs,
-                           thread_off, seg_total, seg_ofs, out, sum, segs);
+                           thread_off, seg_total, seg_ofs, out, sum, segs, nullptr, nullptr);
     else
         hipLaunchKernelGGL(k_enum<1>, dim3(nb), dim3(256), 0, s, nw, nwp, queue, nparts, work, jobs, tabs, gs,
-                           thread_off, seg_total, seg_ofs, out, sum, segs);
+                           thread_off, seg_total, seg_ofs, out, sum, segs, stamp_ctr, stamp_ts);
     return hipGetLastError();
 }
 
@@ -2502,15 +2518,17 @@ hipError_t launch_reduce_partials(hipStream_t s, int n, const TaskSummary *sum, 
 hipError_t launch_enum_mixed(hipStream_t s, int mode, int K, int nw, int nparts, const WorkItem *work, const JobTab *jobs,
                              const TermTab *tabs, const GenState &gs, int *thread_off, int *seg_total,
                              const long long *seg_ofs, Tuple *out, Slice *sout, const TaskSummary *sum,
-                             const SegRec *segs, const long long *gslice, const int *nwp, int *queue) {
+                             const SegRec *segs, const long long *gslice, const int *nwp, int *queue,
+                             unsigned *stamp_ctr, unsigned long long *stamp_ts) {
     if (nw <= 0) return hipSuccess;
     const unsigned nb = nwp ? (unsigned)DEVCOUNT_GRID : (unsigned)nw * (unsigned)nparts;
     if (mode == 0)
         hipLaunchKernelGGL(k_enum_mixed<0>, dim3(nb), dim3(256), 0, s, nw, nwp, queue, nparts, K * K, work, jobs,
-                           tabs, gs, thread_off, seg_total, seg_ofs, out, sout, sum, segs, gslice);
+                           tabs, gs, thread_off, seg_total, seg_ofs, out, sout, sum, segs, gslice, nullptr, nullptr);
     else
         hipLaunchKernelGGL(k_enum_mixed<1>, dim3(nb), dim3(256), 0, s, nw, nwp, queue, nparts, K * K, work, jobs,
-                           tabs, gs, thread_off, seg_total, seg_ofs, out, sout, sum, segs, gslice);
+                           tabs, gs, thread_off, seg_total, seg_ofs, out, sout, sum, segs, gslice, stamp_ctr,
+                           stamp_ts);
     return hipGetLastError();
 }
 
