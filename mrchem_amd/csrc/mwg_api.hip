@@ -1640,7 +1640,8 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         auto chunk_dl = [&](int d) { return std::max(C.chunk, C.hw_chunk[d]); };
         auto schunk_dl = [&](int d) {
             return K == 8 ? std::min(C.slice_chunk > 0 ? C.slice_chunk : MFMA8_TASK_DEFAULT, MFMA8_TASK_SLICES)
-                          : (C.slice_chunk > 0 ? C.slice_chunk : (C.hw_schunk[d] > 0 ? C.hw_schunk[d] : -2048));
+                          : (C.slice_chunk > 0 ? std::max(C.slice_chunk, C.hw_schunk[d])
+                                               : (C.hw_schunk[d] > 0 ? C.hw_schunk[d] : -2048));
         };
         C.tuples.alloc((size_t)std::max<long long>(C.caps.tuples, 1));
         C.tasks.alloc((size_t)std::max(C.caps.tasks, 1));
@@ -2066,8 +2067,8 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         int chunk_it = C.chunk;  // FP64 task size of this iteration (raised when partial slots would not fit)
         // slices per MFMA task (a workgroup task).  k = 7: at most MFMA8_TASK_SLICES (the
         // waves sum a quarter of them each in FP32), MFMA8_TASK_DEFAULT when automatic;
-        // k = 9: about 2048 tasks per iteration when automatic, raised when node slots
-        // would not fit
+        // k = 9: about 2048 tasks per iteration when automatic (16 to 128 slices each),
+        // raised when node slots would not fit
         int schunk_it = K == 8 ? std::min(C.slice_chunk > 0 ? C.slice_chunk : MFMA8_TASK_DEFAULT, MFMA8_TASK_SLICES)
                                : (C.slice_chunk > 0 ? C.slice_chunk : -2048);
         auto launch_schedule = [&]() {
@@ -2169,7 +2170,8 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         genDone = hsum.gen_count;
         if (iter < LOOP_MAX_DEPTH) {
             C.hw_chunk[iter] = std::max(C.hw_chunk[iter], chunk_it);
-            if (K != 8 && C.slice_chunk <= 0 && schunk_it > 0) C.hw_schunk[iter] = std::max(C.hw_schunk[iter], schunk_it);
+            // (a set slice_chunk is raised the same way when the node slots would not fit)
+            if (K != 8 && schunk_it > 0) C.hw_schunk[iter] = std::max(C.hw_schunk[iter], schunk_it);
         }
         if (PT.on) {
             // candidates walked by the work-list kernels: per work node, its table's prefix total

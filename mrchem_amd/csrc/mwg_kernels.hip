@@ -1443,13 +1443,14 @@ __global__ void __launch_bounds__(1024)
     const int g0 = (int)((long long)nw * threadIdx.x / 1024), g1 = (int)((long long)nw * (threadIdx.x + 1) / 1024);
     if (mixed && wchunk <= 0) {
         // slices per MFMA task from the iteration's slice total: about -wchunk tasks
-        // (at least 16 and at most 1024 slices each)
+        // (at least 16 and at most 128 slices each: larger tasks leave too few to balance
+        // the k = 9 slices over the CUs)
         long long sl1 = 0;
         for (int g = g0; g < g1; g++) sl1 += sc[(size_t)g * 9 + 8];
         long long v1[1] = {sl1}, t1[1];
         block_excl_scan_1024_n<1>(v1, (long long(*)[1])wsum6, t1);
         const long long want = (t1[0] + (-wchunk) - 1) / (-wchunk);
-        wchunk = (int)min(1024LL, max(16LL, want));
+        wchunk = (int)min(128LL, max(16LL, want));
         __syncthreads();
     }
     long long tup = 0, sl = 0, nt = 0, nst = 0, slots = 0, nred = 0;

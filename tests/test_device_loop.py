@@ -88,3 +88,25 @@ def test_device_loop_falls_back_when_capacities_are_exceeded(G):
     assert all(s["device_loop"] == 1 for s in s2)
     for a, b in zip(out2, ref):
         _same(a, b, 1e-13)
+
+
+def test_device_loop_with_a_set_slice_chunk_k9(G):
+    """k = 9 mixed with the slice task size set on the context (as an autotuner leaves it):
+    the device loop runs from the second batch on and agrees with the host loop (the task
+    size changes only the FP64 summation grouping)."""
+    prec = 1e-5
+    mra, trees, ops = _batch(G, 9, prec, seed=4, n_orb=3)
+    saved = mra.ctx.get_tuning()
+    try:
+        mra.ctx.set_tuning(saved[0], 128)
+        ref = [G.FunctionTree(mra) for _ in trees]
+        sh = _host_loop(lambda: G.apply_batch(prec, ref, ops, trees, flags=G.MWG_APPLY_MIXED))
+        for _ in range(2):
+            out = [G.FunctionTree(mra) for _ in trees]
+            sd = G.apply_batch(prec, out, ops, trees, flags=G.MWG_APPLY_MIXED)
+            assert all(s["device_loop"] == 1 for s in sd)
+            for a, b, x, y in zip(out, ref, sd, sh):
+                assert x["nodes_out"] == y["nodes_out"] and x["slices"] == y["slices"]
+                _same(a, b, 1e-12)
+    finally:
+        mra.ctx.set_tuning(*saved)
