@@ -1682,7 +1682,6 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         char *ctlb = (char *)C.lctl.p;
         int *gfirst = (int *)(ctlb + offsetof(LoopCtl, genFirst));
         int *queue = (int *)(ctlb + offsetof(LoopCtl, queue));
-        unsigned *sctr = (unsigned *)(ctlb + offsetof(LoopCtl, stampctr));
         for (int j = 0; j < njobs; j++) jobs[j].work.clear();
         PT.lap("loop setup");
         // job tables of every depth in a region of their own: the depths whose operator levels
@@ -1761,17 +1760,16 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             HC(launch_build_tasks(s, WCAP, nseg, 0, chunk_dl(d), schunk_dl(d), C.gtotal.p, C.caps, C.genCount.p, genPoolCap,
                                   C.gofs.p, C.tasks.p, C.stasks.p, C.reds.p, dsum, C.segs.p, C.segcnt.p, C.gslice.p,
                                   C.genOverflow.p, dw, dtab, njobs, nwp));
-            // (the fill's last workgroup stamps the contraction's start)
             if (!mixed)
                 HC(launch_enum(s, 1, WCAP, 0, dw, djt, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, C.gofs.p, C.tuples.p,
-                               dsum, C.segs.p, nwp, queue + 2, sctr, dts));
+                               dsum, C.segs.p, nwp, queue + 2));
             else
                 HC(launch_enum_mixed(s, 1, K, WCAP, 0, dw, djt, C.tabs.p, gs, C.thread_off.p, C.gtotal.p, C.gofs.p,
-                                     C.tuples.p, C.slices.p, dsum, C.segs.p, C.gslice.p, nwp, queue + 2, sctr, dts));
+                                     C.tuples.p, C.slices.p, dsum, C.segs.p, C.gslice.p, nwp, queue + 2));
             const int grid64 = std::max(1, (C.caps.tasks + 3) / 4);
             const int gridred = std::min(4096, std::max(1, C.caps.reds));
             HC(launch_contract_fp64(s, K, 0, dsum, grid64, C.tasks.p, dw, djt, C.tabs.p, C.tuples.p, C.genPool.p,
-                                    C.partials.p));
+                                    C.partials.p, dts));
             if (mixed) {
                 const int tpb = mfma_tasks_per_block(K);
                 const int gridmf = std::max(1, (C.caps.stasks + tpb - 1) / tpb);

@@ -185,14 +185,11 @@ struct SegRec {
     int pad_stask0;  // on component 0 of a node: its first slice task (mixed)
 };
 // nparts workgroups share each output node's candidates (counts and offsets per
-// (node, segment, part)).  Mode 1 with stamp_ts: the last workgroup to finish writes the
-// device clock into *stamp_ts (the end of the fill, where the contraction starts), counting
-// the workgroups in *stamp_ctr (zero before the launch, zero again after it).
+// (node, segment, part))
 hipError_t launch_enum(hipStream_t s, int mode, int nw, int nparts, const WorkItem *work, const JobTab *jobs,
                        const TermTab *tabs, const GenState &gs, int *thread_off, int *seg_total,
                        const long long *seg_ofs, Tuple *out, const TaskSummary *sum, const SegRec *segs,
-                       const int *nwp = nullptr, int *queue = nullptr, unsigned *stamp_ctr = nullptr,
-                       unsigned long long *stamp_ts = nullptr);
+                       const int *nwp = nullptr, int *queue = nullptr);
 hipError_t launch_work_norms(hipStream_t s, int nw, const WorkItem *work, const JobTab *jobs, int K3,
                              double *compact, const int *nwp = nullptr, unsigned long long *ts_end = nullptr);
 // the device clock into *t (one thread; marks the end of the kernels before it on the stream)
@@ -227,8 +224,7 @@ hipError_t launch_enum_mixed(hipStream_t s, int mode, int K, int nw, int nparts,
                              const TermTab *tabs, const GenState &gs, int *thread_off, int *seg_total,
                              const long long *seg_ofs, Tuple *out, Slice *sout, const TaskSummary *sum,
                              const SegRec *segs, const long long *gslice, const int *nwp = nullptr,
-                             int *queue = nullptr, unsigned *stamp_ctr = nullptr,
-                             unsigned long long *stamp_ts = nullptr);
+                             int *queue = nullptr);
 // Split-FP16 copies of FP64 blocks (mwg_mfma.hip): block b (values src[b bvals ..]) as rows
 // of K values, scaled by 2^exp[b] so that the block's largest magnitude lies in [0.5, 1)
 // (exp 0 for a zero block); x 2^e = hi + lo up to about 2^-23 of it.  K = 8: each row
@@ -296,7 +292,6 @@ struct LoopCtl {
     int queue[4];  // work counters of the iteration's k_gen_mark, k_enum count and fill (zeroed)
     int genFirst;  // generated-node counter at the start of the iteration
     int overflow;  // sticky (LoopIter bits)
-    unsigned stampctr;  // workgroups of the fill kernel that have finished (its end stamp)
     double splitfac[LOOP_MAX_DEPTH];  // 2^(-(n0 + d + 1) / 2) of the split test at depth d
     LoopIter iter[LOOP_MAX_DEPTH];
     int tofs[LOOP_MAX_DEPTH + 1];     // first transform item of the split nodes of depth d

@@ -1215,31 +1215,18 @@ __device__ __forceinline__ void enum_node(int g, int part, int nparts, const Wor
     }
 }
 
-// the last workgroup of the grid to get here writes the device clock into *ts and re-arms
-// the counter for the next launch
-__device__ inline void stamp_last(unsigned *ctr, unsigned long long *ts) {
-    __syncthreads();
-    if (threadIdx.x == 0 && atomicAdd(ctr, 1u) == gridDim.x - 1) {
-        *ts = (unsigned long long)wall_clock64();
-        *ctr = 0u;
-    }
-}
-
 // one workgroup per (work node, part) (grid-stride; nwp: the node count on the device, the
 // parts then follow from it as on the host)
 template <int MODE>
 __global__ void __launch_bounds__(256) k_enum(int nw, const int *nwp, int *queue, int nparts, const WorkItem *work,
                                               const JobTab *jobs, const TermTab *tabs, GenState gs, int *thread_off,
                                               int *seg_total, const long long *seg_ofs, Tuple *out,
-                                              const TaskSummary *sum, const SegRec *segs, unsigned *sctr,
-                                              unsigned long long *sts) {
-    if (!(MODE == 1 && sum->overflow)) {
-        const int n = nwp ? *nwp : nw, np = nwp ? enum_parts(n) : nparts;
-        for_items(n * np, queue, [&](int b) {
-            enum_node<MODE>(b / np, b % np, np, work, jobs, tabs, gs, thread_off, seg_total, seg_ofs, out, segs);
-        });
-    }
-    if (MODE == 1 && sts) stamp_last(sctr, sts);
+                                              const TaskSummary *sum, const SegRec *segs) {
+    if (MODE == 1 && sum->overflow) return;
+    const int n = nwp ? *nwp : nw, np = nwp ? enum_parts(n) : nparts;
+    for_items(n * np, queue, [&](int b) {
+        enum_node<MODE>(b / np, b % np, np, work, jobs, tabs, gs, thread_off, seg_total, seg_ofs, out, segs);
+    });
 }
 
 // ----------------------------------------------------------------- apply: predictor features
@@ -1707,14 +1694,20 @@ __global__ void __launch_bounds__(256) k_contract_fp64(int ntask_arg, const Task
                                                        const ContractTask *tasks, const WorkItem *work,
                                                        const JobTab *jobs, const TermTab *tabs,
                                                        const Tuple *tuples, const double *genPool,
-                                                       double *partials) {
-    // (no stores before the task loop: the operator rows are read through scalar loads, which
-    // the compiler only uses for memory no store of the kernel may have changed)
+                                                       double *partials, unsigned long long *ts) {
+    // (no stores or clock reads on the way to the task loop: the operator rows are read
+    // through scalar loads, which the compiler only uses for memory nothing before them may
+    // have changed.  With ts, block 0, dispatched first, only stamps the contraction's start
+    // and the tasks start at block 1.)
     constexpr int K3 = K * K * K;
     __shared__ double buf[4][K3];
+    if (ts && blockIdx.x == 0) {
+        if (threadIdx.x == 0) *ts = (unsigned long long)wall_clock64();
+        return;
+    }
     const int wave = wave_id(), lane = threadIdx.x & 63;
     const int ntask = sum ? (sum->overflow ? 0 : sum->ntasks) : ntask_arg;
-    const int ti = blockIdx.x * 4 + wave;
+    const int ti = ((int)blockIdx.x - (ts ? 1 : 0)) * 4 + wave;
     if (ti >= ntask) return;
     contract_fp64_task<K>(tasks[ti], work, jobs, tabs, tuples, genPool, partials, buf[wave], lane);
 }
@@ -1856,16 +1849,13 @@ __global__ void __launch_bounds__(256) k_enum_mixed(int nw, const int *nwp, int 
                                                     GenState gs, int *thread_off, int *seg_total,
                                                     const long long *seg_ofs, Tuple *out, Slice *sout,
                                                     const TaskSummary *sum, const SegRec *segs,
-                                                    const long long *gslice, unsigned *sctr,
-                                                    unsigned long long *sts) {
-    if (!(MODE == 1 && sum->overflow)) {
-        const int n = nwp ? *nwp : nw, np = nwp ? enum_parts(n) : nparts;
-        for_items(n * np, queue, [&](int b) {
-            enum_mixed_node<MODE>(b / np, b % np, np, K2, work, jobs, tabs, gs, thread_off, seg_total, seg_ofs, out,
-                                  sout, segs, gslice);
-        });
-    }
-    if (MODE == 1 && sts) stamp_last(sctr, sts);
+                                                    const long long *gslice) {
+    if (MODE == 1 && sum->overflow) return;
+    const int n = nwp ? *nwp : nw, np = nwp ? enum_parts(n) : nparts;
+    for_items(n * np, queue, [&](int b) {
+        enum_mixed_node<MODE>(b / np, b % np, np, K2, work, jobs, tabs, gs, thread_off, seg_total, seg_ofs, out,
+                              sout, segs, gslice);
+    });
 }
 
 
@@ -2414,15 +2404,15 @@ hipError_t launch_gen_compute(hipStream_t s, int first, int cap, const JobTab *j
 hipError_t launch_enum(hipStream_t s, int mode, int nw, int nparts, const WorkItem *work, const JobTab *jobs,
                        const TermTab *tabs, const GenState &gs, int *thread_off, int *seg_total,
                        const long long *seg_ofs, Tuple *out, const TaskSummary *sum, const SegRec *segs,
-                       const int *nwp, int *queue, unsigned *stamp_ctr, unsigned long long *stamp_ts) {
+                       const int *nwp, int *queue) {
     if (nw <= 0) return hipSuccess;
     const unsigned nb = nwp ? (unsigned)DEVCOUNT_GRID : (unsigned)nw * (unsigned)nparts;
     if (mode == 0)
         hipLaunchKernelGGL(k_enum<0>, dim3(nb), dim3(256), 0, s, nw, nwp, queue, nparts, work, jobs, tabs, gs,
-                           thread_off, seg_total, seg_ofs, out, sum, segs, nullptr, nullptr);
+                           thread_off, seg_total, seg_ofs, out, sum, segs);
     else
         hipLaunchKernelGGL(k_enum<1>, dim3(nb), dim3(256), 0, s, nw, nwp, queue, nparts, work, jobs, tabs, gs,
-                           thread_off, seg_total, seg_ofs, out, sum, segs, stamp_ctr, stamp_ts);
+                           thread_off, seg_total, seg_ofs, out, sum, segs);
     return hipGetLastError();
 }
 
@@ -2497,12 +2487,8 @@ hipError_t launch_contract_fp64(hipStream_t s, int K, int ntask, const TaskSumma
                                 unsigned long long *ts) {
     const int nb = sum ? grid : (ntask + 3) / 4;
     if (nb <= 0) return hipSuccess;
-    if (ts) {  // the contraction's start on the device clock
-        hipError_t e = launch_stamp(s, ts);
-        if (e != hipSuccess) return e;
-    }
-    MWG_DISPATCH_K(K, hipLaunchKernelGGL(k_contract_fp64<KK>, dim3(nb), dim3(256), 0, s, ntask, sum, tasks, work,
-                                         jobs, tabs, tuples, genPool, partials));
+    MWG_DISPATCH_K(K, hipLaunchKernelGGL(k_contract_fp64<KK>, dim3(nb + (ts ? 1 : 0)), dim3(256), 0, s, ntask, sum,
+                                         tasks, work, jobs, tabs, tuples, genPool, partials, ts));
     return hipGetLastError();
 }
 
@@ -2519,17 +2505,15 @@ hipError_t launch_reduce_partials(hipStream_t s, int n, const TaskSummary *sum, 
 hipError_t launch_enum_mixed(hipStream_t s, int mode, int K, int nw, int nparts, const WorkItem *work, const JobTab *jobs,
                              const TermTab *tabs, const GenState &gs, int *thread_off, int *seg_total,
                              const long long *seg_ofs, Tuple *out, Slice *sout, const TaskSummary *sum,
-                             const SegRec *segs, const long long *gslice, const int *nwp, int *queue,
-                             unsigned *stamp_ctr, unsigned long long *stamp_ts) {
+                             const SegRec *segs, const long long *gslice, const int *nwp, int *queue) {
     if (nw <= 0) return hipSuccess;
     const unsigned nb = nwp ? (unsigned)DEVCOUNT_GRID : (unsigned)nw * (unsigned)nparts;
     if (mode == 0)
         hipLaunchKernelGGL(k_enum_mixed<0>, dim3(nb), dim3(256), 0, s, nw, nwp, queue, nparts, K * K, work, jobs,
-                           tabs, gs, thread_off, seg_total, seg_ofs, out, sout, sum, segs, gslice, nullptr, nullptr);
+                           tabs, gs, thread_off, seg_total, seg_ofs, out, sout, sum, segs, gslice);
     else
         hipLaunchKernelGGL(k_enum_mixed<1>, dim3(nb), dim3(256), 0, s, nw, nwp, queue, nparts, K * K, work, jobs,
-                           tabs, gs, thread_off, seg_total, seg_ofs, out, sout, sum, segs, gslice, stamp_ctr,
-                           stamp_ts);
+                           tabs, gs, thread_off, seg_total, seg_ofs, out, sout, sum, segs, gslice);
     return hipGetLastError();
 }
 
