@@ -282,14 +282,32 @@ __device__ inline uint2 ld8(const uint2 *p) {
 // a wave-uniform exponent (scalar load; written by an earlier kernel, read-only here)
 __device__ inline int ld_exp(const int *p) { return *(cint *)p; }
 
+// a lane's fragment offsets (uint4 units) inside an operator block pair and an f block: the
+// same for every slice, so that a load is a uniform base (scalar registers) plus a lane offset
+struct LaneOffs8 {
+    unsigned bx, ayz, f[4];
+};
+__device__ inline LaneOffs8 lane_offs8(int m, int q) {
+    const int rx = (m >> 3) * 2, row = m & 7;
+    LaneOffs8 L;
+    L.bx = (unsigned)((rx * 8 + row) * 2 + (q & 1));
+    L.ayz = (unsigned)((rx * 8 + row) * 2 + (q >> 1));
+#pragma unroll
+    for (int tl = 0; tl < 4; tl++) {
+        const int y = 4 * (tl & 1) + (m & 3), z = 4 * (tl >> 1) + (m >> 2);
+        L.f[tl] = (unsigned)((z * 8 + y) * 2 + (q >> 1));
+    }
+    return L;
+}
+
 __device__ inline OpsH8 load_opsh8(const Slice &sl, const Arr16 &J, const uint4 *gen16, const int *genExp,
-                                   int m, int q) {
+                                   int m, int q, const LaneOffs8 &L) {
     OpsH8 o;
     const int bxb = sl.xo / 64, byb = sl.yo / 64, bzb = sl.zo / 64;  // block indices (gt_d = 0)
-    const int rx = (m >> 3) * 2, row = m & 7;
-    o.bx = ld16(J.oper + ((bxb + rx) * 8 + row) * 2 + (q & 1));
-    o.ay = ld16(J.oper + ((byb + rx) * 8 + row) * 2 + (q >> 1));
-    o.az = ld16(J.oper + ((bzb + rx) * 8 + row) * 2 + (q >> 1));
+    const int rx = (m >> 3) * 2;
+    o.bx = ld16(J.oper + (size_t)bxb * 16 + L.bx);
+    o.ay = ld16(J.oper + (size_t)byb * 16 + L.ayz);
+    o.az = ld16(J.oper + (size_t)bzb * 16 + L.ayz);
     const uint4 *fb;
     int ef;
     if (sl.f >= 0) {
@@ -300,10 +318,7 @@ __device__ inline OpsH8 load_opsh8(const Slice &sl, const Arr16 &J, const uint4 
         ef = ld_exp(genExp + (-sl.f - 1));
     }
 #pragma unroll
-    for (int tl = 0; tl < 4; tl++) {
-        const int y = 4 * (tl & 1) + (m & 3), z = 4 * (tl >> 1) + (m >> 2);
-        o.f[tl] = ld16(fb + (z * 8 + y) * 2 + (q >> 1));
-    }
+    for (int tl = 0; tl < 4; tl++) o.f[tl] = ld16(fb + L.f[tl]);
     // block exponents: uniform loads, then this lane's gt_x (rx) and gt_z (q >> 1)
     const int ex0 = ld_exp(J.operExp + bxb), ex1 = ld_exp(J.operExp + bxb + 2);
     const int ez0 = ld_exp(J.operExp + bzb), ez1 = ld_exp(J.operExp + bzb + 2);
@@ -415,15 +430,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
             wave_sync();
     };
     // two operand sets in turn (no register copies between slices)
-    OpsH8 opA = load_opsh8(slice_at(srec, min(wave, last)), J, gen16, genExp, m, q), opB;
+    const LaneOffs8 LO = lane_offs8(m, q);
+    OpsH8 opA = load_opsh8(slice_at(srec, min(wave, last)), J, gen16, genExp, m, q, LO), opB;
     Slice rnext = slice_at(srec, min(wave + 4, last));
     for (int it = wave; it < task.count;) {
-        opB = load_opsh8(rnext, J, gen16, genExp, m, q);
+        opB = load_opsh8(rnext, J, gen16, genExp, m, q, LO);
         rnext = slice_at(srec, min(it + 8, last));
         contract(opA);
         it += 4;
         if (it >= task.count) break;
-        opA = load_opsh8(rnext, J, gen16, genExp, m, q);
+        opA = load_opsh8(rnext, J, gen16, genExp, m, q, LO);
         rnext = slice_at(srec, min(it + 8, last));
         contract(opB);
         it += 4;
@@ -491,28 +507,48 @@ struct OpsH10 {
     uint4 f[2];                    // B fragments of f for the pass-x N-tiles wave, wave + 4
 };
 
-// the A fragment of one operator: M-tile 0 row m (block gt = m >= 10), M-tile 1 rows 16..19
-// at rows 0, 4, 8, 12 (zero elsewhere)
-__device__ inline void load_oper_h10(const uint4 *oper, int blk, int m, int kg, uint4 &t0, uint4 &t1) {
-    t0 = ld16(oper + (((size_t)blk + (m >= 10 ? 2 : 0)) * 10 + m % 10) * 4 + kg);
-    t1 = (m & 3) == 0 ? ld16(oper + (((size_t)blk + 2) * 10 + 6 + (m >> 2)) * 4 + kg) : make_uint4(0u, 0u, 0u, 0u);
-}
-
-__device__ inline OpsH10 load_opsh10(const Slice &sl, const Arr16 &J, const uint4 *gen16, int wave, int m, int kg) {
-    OpsH10 o;
-    load_oper_h10(J.oper, sl.xo / 100, m, kg, o.x0, o.x1);
-    load_oper_h10(J.oper, sl.yo / 100, m, kg, o.y0, o.y1);
-    load_oper_h10(J.oper, sl.zo / 100, m, kg, o.z0, o.z1);
+// a lane's fragment offsets inside an operator block pair (uint4 units: M-tile 0 row m, block
+// gt = m >= 10; M-tile 1 rows 16..19 at rows 0, 4, 8, 12) and inside an f block (uint2 units,
+// the pass-x rows of N-tiles wave and wave + 4): the same for every slice, so that a load is a
+// uniform base (scalar registers) plus a lane offset
+struct LaneOffs10 {
+    unsigned op0, op1, f1[2], f2[2];
+};
+__device__ inline LaneOffs10 lane_offs10(int wave, int m, int kg) {
+    LaneOffs10 L;
+    L.op0 = (unsigned)((((m >= 10 ? 2 : 0)) * 10 + m % 10) * 4 + kg);
+    L.op1 = (unsigned)((2 * 10 + 6 + (m >> 2)) * 4 + kg);
     // f rows [hi 10 | lo 10] (40 bytes, five uint2); lane group kg takes the dwords of its
     // K-slots [hi | lo | hi | 0 0]: kg 0 bytes 0-15, kg 1 16-31, kg 2 32-39 then 0-7, kg 3
     // 8-23 with its last dword zero
-    const uint2 *fb = (const uint2 *)(sl.f >= 0 ? J.in + ((size_t)sl.f * 8 + sl.ft) * 250
-                                                : gen16 + (size_t)(-sl.f - 1) * 250);
     const int o1 = kg == 0 ? 0 : kg == 1 ? 2 : kg == 2 ? 4 : 1, o2 = kg == 0 ? 1 : kg == 1 ? 3 : kg == 2 ? 0 : 2;
 #pragma unroll
     for (int t = 0; t < 2; t++) {
         const int r = min(16 * (wave + 4 * t) + m, 99);  // (rows beyond 99: results discarded)
-        const uint2 u = ld8(fb + (size_t)r * 5 + o1), v = ld8(fb + (size_t)r * 5 + o2);
+        L.f1[t] = (unsigned)(r * 5 + o1);
+        L.f2[t] = (unsigned)(r * 5 + o2);
+    }
+    return L;
+}
+
+// the A fragments of one operator (M-tile 1: rows 16..19 at rows 0, 4, 8, 12, zero elsewhere)
+__device__ inline void load_oper_h10(const uint4 *oper, int blk, int m, const LaneOffs10 &L, uint4 &t0, uint4 &t1) {
+    const uint4 *b = oper + (size_t)blk * 40;
+    t0 = ld16(b + L.op0);
+    t1 = (m & 3) == 0 ? ld16(b + L.op1) : make_uint4(0u, 0u, 0u, 0u);
+}
+
+__device__ inline OpsH10 load_opsh10(const Slice &sl, const Arr16 &J, const uint4 *gen16, int m, int kg,
+                                     const LaneOffs10 &L) {
+    OpsH10 o;
+    load_oper_h10(J.oper, sl.xo / 100, m, L, o.x0, o.x1);
+    load_oper_h10(J.oper, sl.yo / 100, m, L, o.y0, o.y1);
+    load_oper_h10(J.oper, sl.zo / 100, m, L, o.z0, o.z1);
+    const uint2 *fb = (const uint2 *)(sl.f >= 0 ? J.in + ((size_t)sl.f * 8 + sl.ft) * 250
+                                                : gen16 + (size_t)(-sl.f - 1) * 250);
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        const uint2 u = ld8(fb + L.f1[t]), v = ld8(fb + L.f2[t]);
         o.f[t] = make_uint4(u.x, u.y, v.x, kg == 3 ? 0u : v.y);
     }
     return o;
@@ -641,17 +677,18 @@ k_contract_mfma10h(int ntask_arg, const TaskSummary *sum, const SliceTask *tasks
     };
     // two operand sets in turn; the next slice's operands and the record after it are
     // loaded while a slice is contracted
+    const LaneOffs10 LO = lane_offs10(wave, m, kg);
     Slice cA = slice_at(srec, 0), cB;
-    OpsH10 opA = load_opsh10(cA, J, gen16, wave, m, kg), opB;
+    OpsH10 opA = load_opsh10(cA, J, gen16, m, kg, LO), opB;
     Slice rnext = slice_at(srec, min(1, last));
     for (int it = 0; it < task.count;) {
         cB = rnext;
-        opB = load_opsh10(cB, J, gen16, wave, m, kg);
+        opB = load_opsh10(cB, J, gen16, m, kg, LO);
         rnext = slice_at(srec, min(it + 2, last));
         contract(opA, cA);
         if (++it >= task.count) break;
         cA = rnext;
-        opA = load_opsh10(cA, J, gen16, wave, m, kg);
+        opA = load_opsh10(cA, J, gen16, m, kg, LO);
         rnext = slice_at(srec, min(it + 2, last));
         contract(opB, cB);
         ++it;
