@@ -91,7 +91,9 @@ int mwg_sync(mwg_ctx ctx);
 /* contraction task granularity (no reference counterpart: MRCPP threads over output
  * nodes with OpenMP): tuples per FP64 wave task (1..4096) and MFMA slices per
  * workgroup task (0..4096, 0 = automatic; k = 7 uses at most 256, 128 when automatic;
- * k = 9 sizes automatic tasks from each refinement iteration's slice total).  Results agree to FP64 rounding
+ * k = 9 sizes automatic tasks from each refinement iteration's slice total, 16 to 128
+ * slices, and raises a task size, automatic or set, where the FP64 node slots of an
+ * iteration would exceed 8 GiB).  Results agree to FP64 rounding
  * for any setting on the FP64 path and at k = 9, and to the rounding of the MFMA
  * slices' FP32 sums at k = 7; mrchem_amd.autotune picks the fastest on a sample
  * workload. */
