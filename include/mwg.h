@@ -89,7 +89,8 @@ int mwg_ctx_destroy(mwg_ctx ctx);
 const char *mwg_last_error(mwg_ctx ctx);
 int mwg_sync(mwg_ctx ctx);
 /* contraction task granularity (no reference counterpart: MRCPP threads over output
- * nodes with OpenMP): tuples per FP64 wave task (1..4096) and MFMA slices per
+ * nodes with OpenMP): at most this many tuples per FP64 wave task (1..4096; an iteration
+ * with fewer than 4096 tasks' worth of tuples uses smaller tasks, down to 16) and MFMA slices per
  * workgroup task (0..4096, 0 = automatic; k = 7 uses at most 256, 128 when automatic;
  * k = 9 sizes automatic tasks from each refinement iteration's slice total, 16 to 128
  * slices, and raises a task size, automatic or set, where the FP64 node slots of an

@@ -1378,6 +1378,7 @@ __device__ inline void block_excl_scan_1024_n(long long (&v)[NV], long long (*ws
 // in LDS.  Phase 3: one wave per listed segment writes its tasks.  Every task has a fixed
 // position, so the schedule does not depend on the order of the list.
 constexpr int BT_FEW = 32, BT_LIST = 2048, BT_TABS = 256;
+constexpr long long FP64_TASKS = 4096;  // FP64 tasks an iteration aims at (task size permitting)
 
 // (one workgroup: the pointers never alias, so that loads of later iterations can be
 // issued ahead of the stores of earlier ones)
@@ -1428,16 +1429,24 @@ __global__ void __launch_bounds__(1024)
     __syncthreads();
     // phase 1: this thread's totals (a contiguous range of output nodes)
     const int g0 = (int)((long long)nw * threadIdx.x / 1024), g1 = (int)((long long)nw * (threadIdx.x + 1) / 1024);
-    if (mixed && wchunk <= 0) {
-        // slices per MFMA task from the iteration's slice total: about -wchunk tasks
-        // (at least 16 and at most 128 slices each: larger tasks leave too few to balance
-        // the k = 9 slices over the CUs)
-        long long sl1 = 0;
-        for (int g = g0; g < g1; g++) sl1 += sc[(size_t)g * 9 + 8];
-        long long v1[1] = {sl1}, t1[1];
-        block_excl_scan_1024_n<1>(v1, (long long(*)[1])wsum6, t1);
-        const long long want = (t1[0] + (-wchunk) - 1) / (-wchunk);
-        wchunk = (int)min(128LL, max(16LL, want));
+    {
+        // task sizes from the iteration's totals.  FP64 tasks: at most chunk tuples, fewer
+        // (down to 16) when the iteration has fewer than FP64_TASKS x chunk tuples, so that a
+        // coarse iteration still spreads over the GPU.  Slices per MFMA task when automatic
+        // (wchunk <= 0): about -wchunk tasks, at least 16 and at most 128 slices each (larger
+        // tasks leave too few to balance the k = 9 slices over the CUs)
+        long long tu1 = 0, sl1 = 0;
+        for (int g = g0; g < g1; g++) {
+            for (int q = 0; q < 8; q++) tu1 += sc[(size_t)g * nseg + q];
+            if (mixed) sl1 += sc[(size_t)g * 9 + 8];
+        }
+        long long v2[2] = {tu1, sl1}, t2[2];
+        block_excl_scan_1024_n<2>(v2, (long long(*)[2])wsum6, t2);
+        chunk = (int)min((long long)chunk, max(16LL, (t2[0] + FP64_TASKS - 1) / FP64_TASKS));
+        if (mixed && wchunk <= 0) {
+            const long long want = (t2[1] + (-wchunk) - 1) / (-wchunk);
+            wchunk = (int)min(128LL, max(16LL, want));
+        }
         __syncthreads();
     }
     long long tup = 0, sl = 0, nt = 0, nst = 0, slots = 0, nred = 0;
