@@ -225,6 +225,39 @@ def test_large_batch_equals_single(G):
         assert np.abs(c1 - c2).max() <= 1e-13 * np.abs(c2).max()
 
 
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_operators_built_together_equal_operators_built_alone(G, mode):
+    """Operator levels built for a whole batch at once (all operators and depths in two
+    round trips, one allocation per operator with its split-FP16 copies, down to one depth
+    below the inputs) give the same applies as levels built for one operator at a time:
+    the same grids, coefficients to FP64 rounding (plain) or to the rounding of the FP32
+    slice sums (mixed: the order of a node's slices follows how the batch's work list is
+    split over workgroups)."""
+    prec = 1e-5
+    flags = G.MWG_APPLY_MIXED if mode == "mixed" else G.MWG_APPLY_FP64
+    mra = G.mra_from(configs.BOX20, 7)
+    orbs, nuc, mus = configs.orbitals(7, n_orb=3)
+    fs = [G.project(prec, G.FunctionTree(mra), o) for o in orbs]
+    jobs = [(fs[0], mus[0]), (fs[1], mus[1]), (fs[2], mus[2]), (fs[0], 0.0)]
+
+    def make(mu):
+        return G.HelmholtzOperator(mra, mu, prec) if mu > 0 else G.PoissonOperator(mra, prec)
+
+    alone = []
+    for f, mu in jobs:
+        o = G.FunctionTree(mra)
+        G.apply_batch(prec, [o], [make(mu)], [f], flags=flags)
+        alone.append(o)
+    outs = [G.FunctionTree(mra) for _ in jobs]
+    G.apply_batch(prec, outs, [make(mu) for _, mu in jobs], [f for f, _ in jobs], flags=flags)
+    tol = 1e-13 if mode == "fp64" else 1e-2 * prec
+    for a, b in zip(outs, alone):
+        d1, l1, c1 = a.export()
+        d2, l2, c2 = b.export()
+        assert np.array_equal(d1, d2) and np.array_equal(l1, l2)
+        assert np.abs(c1 - c2).max() <= tol * np.abs(c2).max()
+
+
 def test_reference_coulomb_values_on_gpu(G, A):
     """tests/qmoperators/coulomb_operator.cpp:76-82 at its settings, through the C ABI."""
     prec = 1e-3
