@@ -1427,20 +1427,26 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
     // on this context before; it reserves the output trees itself
     const bool dl_plan = !use_pred && !C.tracing && max_iter < 0 && C.hw_nw > 0 && jobs.size() <= (size_t)LOOP_MAX_JOBS &&
                          M.maxDepth < LOOP_MAX_DEPTH && std::getenv("MWG_HOST_LOOP") == nullptr;
-    // inputs: device hash, norm maxima per depth
-    size_t genTotal = 0;
+    // inputs: device hash, norm maxima per depth (host work per job, on threads for large
+    // batches)
+    size_t genTotal = 0, in_nodes = 0;
     std::vector<std::pair<OperH *, int>> need;
     for (JobH &J : jobs) {
         if (J.in->mra != &M || J.out->mra != &M || J.op->mra != &M)
             throw Error(MWG_ERR_ARG, "all trees and operators of an apply must share one MRA");
         if (J.in->n() == 0) throw Error(MWG_ERR_STATE, "input tree is empty");
         J.in->build_hash();
+        in_nodes += J.in->n();
+    }
+    std::vector<int> dmax_in(jobs.size(), 0);
+    auto scan_input = [&](size_t j) {
+        JobH &J = jobs[j];
         J.fmax.assign(M.maxDepth + 2, std::array<double, 8>{0, 0, 0, 0, 0, 0, 0, 0});
         std::vector<double> leafmax(M.maxDepth + 2, 0.0);
-        int dmax_in = 0;
+        int dm = 0;
         for (int i = 0; i < J.in->n(); i++) {
             const int d = J.in->depth[i];
-            dmax_in = std::max(dmax_in, d);
+            dm = std::max(dm, d);
             for (int c = 0; c < 8; c++) J.fmax[d][c] = std::max(J.fmax[d][c], J.in->cnorm[8 * (size_t)i + c]);
             if (J.in->child[i] < 0) {
                 double fn = 0.0;
@@ -1453,8 +1459,16 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             J.fmax[d][0] = std::max(J.fmax[d][0], run);
             run = std::max(run, leafmax[d]);
         }
+        dmax_in[j] = dm;
+    };
+    if (in_nodes < 65536)  // (threads cost more than they save on small batches)
+        for (size_t j = 0; j < jobs.size(); j++) scan_input(j);
+    else
+        for_each_job(jobs.size(), scan_input);
+    for (size_t j = 0; j < jobs.size(); j++) {
+        JobH &J = jobs[j];
         // output trees reach at most a level or so below their inputs: build one more
-        need.push_back({J.op, std::min(dmax_in + 1, M.maxDepth)});
+        need.push_back({J.op, std::min(dmax_in[j] + 1, M.maxDepth)});
         J.out->make_roots();
         // output trees end between a third of the input's size (pair Poissons) and about its
         // size (Helmholtz): reserve half of it once (beyond, the arrays grow geometrically:
