@@ -201,7 +201,10 @@ int mwg_add(mwg_tree out, double ca, mwg_tree a, double cb, mwg_tree b);
 
 /* ---- operators: replace new mrcpp::PoissonOperator(MRA, prec) (SCFDriver.cpp:279) and
  * mrcpp::HelmholtzOperator(MRA, mu, prec) (HelmholtzVector.cpp:171).  Kernels 1/r and
- * exp(-mu r)/r (the -1/(2 pi) rescale stays with the caller, HelmholtzVector.cpp:81,135). */
+ * exp(-mu r)/r (the -1/(2 pi) rescale stays with the caller, HelmholtzVector.cpp:81,135).
+ * Creation fits the Gaussian expansion (host); the non-standard-form blocks of each scale
+ * are computed on the device by the first apply that needs them, for all operators of the
+ * batch together, and kept until the operator is destroyed. */
 int mwg_oper_poisson(mwg_mra mra, double prec, mwg_oper *out);
 int mwg_oper_helmholtz(mwg_mra mra, double mu, double prec, mwg_oper *out);
 int mwg_oper_destroy(mwg_oper oper);
