@@ -141,11 +141,13 @@ static std::vector<DevCache *> g_caches;
 struct DevCache {
     std::multimap<size_t, void *> blocks;  // bytes -> block
     size_t bytes = 0;
-    // bytes kept for reuse: a quarter of the device memory (at least 8 GiB), set at the
-    // first return (with the context's device current).  Freed trees come back here
+    // bytes kept for reuse: half of the device memory (at least 8 GiB), set at the first
+    // return (with the context's device current).  (A quarter made a 560-apply k = 9 batch
+    // free and allocate again ~80 blocks per step, each free waiting for the device.)  Freed trees come back here
     // instead of hipFree (which waits for the device); an allocation that fails empties
     // every cache first (dev_malloc).
     size_t LIMIT = 0;
+    long long evicted = 0;  // blocks freed because the cache was full (MWG_TIMING reports it)
     DevCache() {
         std::lock_guard<std::mutex> lk(g_cache_mu);
         g_caches.push_back(this);
@@ -174,7 +176,7 @@ struct DevCache {
         if (!LIMIT) {
             size_t fr = 0, tot = 0;
             LIMIT = (size_t)8 << 30;
-            if (hipMemGetInfo(&fr, &tot) == hipSuccess) LIMIT = std::max(LIMIT, tot / 4);
+            if (hipMemGetInfo(&fr, &tot) == hipSuccess) LIMIT = std::max(LIMIT, tot / 2);
             else (void)hipGetLastError();
         }
         blocks.insert({nbytes, q});
@@ -184,6 +186,7 @@ struct DevCache {
             bytes -= it->first;
             (void)hipFree(it->second);
             blocks.erase(it);
+            evicted++;
         }
     }
     ~DevCache() {
@@ -2400,6 +2403,9 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         for (JobH &J : jobs) J.in->drop16();
     }
     PT.report();
+    if (PT.on)
+        fprintf(stderr, "[mwg timing] block cache: %.2f GiB kept, %lld blocks freed so far\n",
+                (double)C.cache.bytes / (1 << 30), C.cache.evicted);
     const double t_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
     stats.assign(jobs.size(), mwg_apply_stats{});
     const double fl = 6.0 * K * K * K * K;
