@@ -876,6 +876,11 @@ static void build_levels(MraH &M, std::vector<LevelJob> &lj, bool with16) {
         int *e16 = nullptr;
     };
     std::vector<OpMem> om;
+    {
+        size_t nops = 0;  // (an upper bound: references into om stay valid)
+        for (const LevelJob &J : lj) nops += J.ops.size();
+        om.reserve(nops);
+    }
     auto op_mem = [&](OperH *op) -> OpMem & {
         for (auto &x : om)
             if (x.op == op) return x;
