@@ -235,6 +235,7 @@ struct Ctx {
     Readback rb;
     Stage stage;
     NSScratch ns;
+    DBuf<char> s16items;  // item list of the batched split-FP16 conversion of an apply's inputs
     hipStream_t stream = nullptr;
     double wall_khz = 0.0;  // device wall clock (contraction timestamps), ticks per ms
     // device-resident refinement: high-water marks of the applies run on this context (work
@@ -535,14 +536,15 @@ struct TreeH {
         hmask = index.mask();
         hvalid = true;
     }
-    void ensure16() {
-        if (ver16 == ver) return;
+    // the split-FP16 arrays, sized for the tree (through the block cache, where drop16
+    // returns them)
+    void alloc16() {
         const int K = mra->B.K;
         const size_t nb = (size_t)n() * 8;
         Ctx &C = *mra->ctx;
         const size_t need = nb * split16_block_uint4(K, 1, K3());
         size_t got = 0;
-        if (coef16.n < need || !coef16.p) {  // (through the block cache, where drop16 returns it)
+        if (coef16.n < need || !coef16.p) {
             if (coef16.p) C.cache.put(coef16.release(), coef16.n * sizeof(uint4));
             coef16.adopt((uint4 *)C.cache.get(need * sizeof(uint4), got), got / sizeof(uint4));
         }
@@ -550,7 +552,12 @@ struct TreeH {
             if (exp16.p) C.cache.put(exp16.release(), exp16.n * sizeof(int));
             exp16.adopt((int *)C.cache.get(nb * sizeof(int), got), got / sizeof(int));
         }
-        HC(launch_split16(mra->ctx->stream, (int)nb, 0, nullptr, K3(), K, coef.p, coef16.p, exp16.p, 1));
+    }
+    void ensure16() {
+        if (ver16 == ver) return;
+        alloc16();
+        const int K = mra->B.K;
+        HC(launch_split16(mra->ctx->stream, n() * 8, 0, nullptr, K3(), K, coef.p, coef16.p, exp16.p, 1));
         ver16 = ver;
     }
     // the split-FP16 copy back to the block cache (made again on the next mixed apply)
@@ -1494,7 +1501,40 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
     // operator levels down to the inputs' depths, all operators at once (deeper ones on demand)
     PT.lap("inputs");
     ensure_levels(need, mixed);
-    if (mixed) ensure_levels16(C, need, K);
+    if (mixed) {
+        ensure_levels16(C, need, K);
+        // split-FP16 copies of all inputs that lack one: a launch per group of 64 trees (the
+        // GPU converts a group while the host takes the next group's arrays from the cache)
+        std::vector<TreeH *> todo;
+        for (JobH &J : jobs)
+            if (J.in->ver16 != J.in->ver && std::find(todo.begin(), todo.end(), J.in) == todo.end())
+                todo.push_back(J.in);
+        if (!todo.empty()) {
+            constexpr size_t GROUP = 64;
+            const size_t ng = (todo.size() + GROUP - 1) / GROUP;
+            const size_t o_first = (GROUP * sizeof(Split16Item) + 255) & ~(size_t)255;
+            const size_t gbytes = (o_first + (GROUP + 1) * sizeof(long long) + 255) & ~(size_t)255;
+            C.s16items.alloc(ng * gbytes);
+            for (size_t g = 0; g < ng; g++) {
+                const size_t i0 = g * GROUP, i1 = std::min(todo.size(), i0 + GROUP);
+                std::vector<char> blob(gbytes, 0);
+                Split16Item *items = (Split16Item *)blob.data();
+                long long *first = (long long *)(blob.data() + o_first);
+                first[0] = 0;
+                for (size_t i = i0; i < i1; i++) {
+                    TreeH &T = *todo[i];
+                    T.alloc16();
+                    items[i - i0] = Split16Item{T.coef.p, T.coef16.p, T.exp16.p};
+                    first[i - i0 + 1] = first[i - i0] + (long long)T.n() * 8;
+                    T.ver16 = T.ver;
+                }
+                char *dst = C.s16items.p + g * gbytes;
+                stage_copy(C, dst, blob.data(), gbytes);
+                HC(launch_split16_multi(s, (int)(i1 - i0), (const Split16Item *)dst, (const long long *)(dst + o_first),
+                                        first[i1 - i0], K3, K, 1));
+            }
+        }
+    }
     PT.lap("operator levels");
     // generated-node tables
     C.genHashKeys.alloc(genTotal);

@@ -232,6 +232,15 @@ hipError_t launch_enum_mixed(hipStream_t s, int mode, int K, int nw, int nparts,
 // [hi 10 | hi 10 | lo 10 | 0 0] (four uint4); role 1 (input nodes, B operands): each row
 // [hi 10 | lo 10] (40 bytes).  With count, only blocks first .. min(*count, first + n) - 1;
 // with firstp, first is read on the device and the blocks are first .. min(*count, n) - 1.
+// the same for the blocks of several arrays (e.g. the input trees of a batch) in one launch:
+// item i's blocks are first[i] .. first[i + 1] - 1 of the concatenation (first[nitems] = total)
+struct Split16Item {
+    const double *src;
+    uint4 *dst;
+    int *exp;
+};
+hipError_t launch_split16_multi(hipStream_t s, int nitems, const Split16Item *items, const long long *first,
+                                long long total, int bvals, int K, int role);
 hipError_t launch_split16(hipStream_t s, int n, int first, const int *count, int bvals, int K, const double *src,
                           uint4 *dst, int *exp, int role = 0, const int *firstp = nullptr);
 // uint4 per block of bvals values of a split-FP16 copy

@@ -736,6 +736,38 @@ hipError_t launch_contract_mfma(hipStream_t s, int K, int ntask, const TaskSumma
     return hipGetLastError();
 }
 
+// the blocks of several arrays, one wave per block over their concatenation (item i's blocks
+// are first[i] .. first[i + 1] - 1, its own blocks 0 ..; first[nitems] = the total)
+template <int K>
+__global__ void __launch_bounds__(256) k_split16_multi(int nitems, const Split16Item *items, const long long *first,
+                                                       int bvals, int role) {
+    const long long total = first[nitems];
+    for (long long b = (long long)blockIdx.x * 4 + wave_index(); b < total; b += (long long)gridDim.x * 4) {
+        int lo = 0, hi = nitems - 1;  // the item holding block b
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) / 2;
+            if (first[mid] <= b) lo = mid;
+            else hi = mid - 1;
+        }
+        const Split16Item it = items[lo];
+        const int lb = (int)(b - first[lo]);
+        if (K == 8) split16_block(lb, bvals, it.src, it.dst, it.exp);
+        else split16_10_block(lb, bvals, role, it.src, it.dst, it.exp);
+    }
+}
+
+hipError_t launch_split16_multi(hipStream_t s, int nitems, const Split16Item *items, const long long *first,
+                                long long total, int bvals, int K, int role) {
+    if (K != 8 && K != 10) return hipErrorInvalidValue;
+    if (nitems <= 0 || total <= 0) return hipSuccess;
+    const unsigned nb = (unsigned)std::min<long long>((total + 3) / 4, 8192);
+    if (K == 8)
+        hipLaunchKernelGGL(k_split16_multi<8>, dim3(nb), dim3(256), 0, s, nitems, items, first, bvals, role);
+    else
+        hipLaunchKernelGGL(k_split16_multi<10>, dim3(nb), dim3(256), 0, s, nitems, items, first, bvals, role);
+    return hipGetLastError();
+}
+
 hipError_t launch_split16(hipStream_t s, int n, int first, const int *count, int bvals, int K, const double *src,
                           uint4 *dst, int *exp, int role, const int *firstp) {
     if (K != 8 && K != 10) return hipErrorInvalidValue;
