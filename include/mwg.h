@@ -210,7 +210,7 @@ int mwg_oper_helmholtz(mwg_mra mra, double mu, double prec, mwg_oper *out);
 int mwg_oper_destroy(mwg_oper oper);
 int mwg_oper_nterms(mwg_oper oper, int *n);
 int mwg_oper_terms(mwg_oper oper, double *exponents, double *weights);
-/* NS blocks of one term at one depth for translations [dlo, dhi]:
+/* NS blocks of one term at one depth (0 .. the MRA's max depth) for translations [dlo, dhi]:
  * out[((delta-dlo)*4 + type)*(k+1)^2 + a*(k+1) + i], type = 2*gt_bit + ft_bit (T, C, B, A) */
 int mwg_oper_blocks(mwg_oper oper, int term, int depth, int64_t dlo, int64_t dhi, double *out);
 

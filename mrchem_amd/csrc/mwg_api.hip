@@ -3451,7 +3451,8 @@ int mwg_oper_terms(mwg_oper oper, double *exponents, double *weights) {
     return MWG_OK;
 }
 int mwg_oper_blocks(mwg_oper oper, int term, int depth, int64_t dlo, int64_t dhi, double *out) {
-    if (!oper || !out || term < 0 || term >= oper->nterms() || depth < 0 || dhi < dlo) return MWG_ERR_ARG;
+    if (!oper || !out || term < 0 || term >= oper->nterms() || depth < 0 || depth > oper->mra->maxDepth || dhi < dlo)
+        return MWG_ERR_ARG;
     Ctx *C = oper->mra->ctx;
     return guarded(C, [&] {
         MraH &M = *oper->mra;

@@ -277,6 +277,8 @@ def test_errors_are_reported(G):
         G.HelmholtzOperator(mra, -1.0, 1e-4)
     with pytest.raises(G.MwgError):
         G.apply(1e-4, G.FunctionTree(mra), G.PoissonOperator(mra, 1e-4), G.FunctionTree(mra))
+    with pytest.raises(G.MwgError):  # blocks below the MRA's deepest scale
+        G.PoissonOperator(mra, 1e-4).blocks(0, mra.max_depth + 1, 0, 0)
 
 
 def _l2_diff(a, b, box_width, n=20000, seed=7):
