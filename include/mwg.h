@@ -100,6 +100,11 @@ int mwg_sync(mwg_ctx ctx);
  * workload. */
 int mwg_ctx_set_tuning(mwg_ctx ctx, int fp64_chunk, int slice_chunk);
 int mwg_ctx_get_tuning(mwg_ctx ctx, int *fp64_chunk, int *slice_chunk);
+/* return the device memory the context keeps for reuse (blocks of destroyed trees and
+ * operators, up to half the device memory) to the device, after the context's work has
+ * finished; *released (may be null) gets the bytes.  No reference counterpart: for callers
+ * that share the GPU with other allocators (as torch.cuda.empty_cache). */
+int mwg_ctx_trim(mwg_ctx ctx, int64_t *released);
 
 /* ---- MRA: BoundingBox + scaling basis + max depth
  * replaces mrcpp::MultiResolutionAnalysis<3>(BoundingBox<3>(min_scale, corner, boxes), basis, max_depth)

@@ -64,6 +64,7 @@ def _sig():
         "mwg_sync": [vp],
         "mwg_ctx_set_tuning": [vp, i, i],
         "mwg_ctx_get_tuning": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+        "mwg_ctx_trim": [vp, C.POINTER(C.c_int64)],
         "mwg_mra_create": [vp, i, fi, fi, d, i, i, i, pvp],
         "mwg_mra_destroy": [vp],
         "mwg_tree_create": [vp, pvp],
@@ -116,7 +117,7 @@ def _sig():
 
 
 EXPORTED = ["mwg_ctx_create", "mwg_ctx_destroy", "mwg_last_error", "mwg_sync", "mwg_ctx_set_tuning",
-            "mwg_ctx_get_tuning", "mwg_mra_create", "mwg_mra_destroy",
+            "mwg_ctx_get_tuning", "mwg_ctx_trim", "mwg_mra_create", "mwg_mra_destroy",
             "mwg_tree_create", "mwg_tree_destroy", "mwg_tree_nnodes", "mwg_tree_sqnorm", "mwg_tree_maxdepth",
             "mwg_tree_mra", "mwg_mra_order",
             "mwg_tree_import", "mwg_tree_layout_check", "mwg_tree_export", "mwg_tree_rescale", "mwg_tree_dot", "mwg_tree_dot_noalloc",

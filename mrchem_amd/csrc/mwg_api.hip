@@ -3695,6 +3695,16 @@ int mwg_ctx_get_tuning(mwg_ctx ctx, int *fp64_chunk, int *slice_chunk) {
     return MWG_OK;
 }
 
+int mwg_ctx_trim(mwg_ctx ctx, int64_t *released) {
+    if (!ctx) return MWG_ERR_ARG;
+    return guarded(ctx, [&] {
+        HC(hipStreamSynchronize(ctx->stream));
+        const int64_t b = (int64_t)ctx->cache.bytes;
+        ctx->cache.flush();
+        if (released) *released = b;
+    });
+}
+
 int mwg_ctx_set_predictor(mwg_ctx ctx, mwg_predictor pred, float threshold) {
     if (!ctx) return MWG_ERR_ARG;
     if (pred && pred->ctx != ctx) return MWG_ERR_ARG;

@@ -52,6 +52,12 @@ class Context:
         check(lib().mwg_ctx_get_tuning(self.h, C.byref(a), C.byref(b)), self.h)
         return int(a.value), int(b.value)
 
+    def trim(self) -> int:
+        """Return the device memory kept for reuse to the device; the bytes released."""
+        r = C.c_int64()
+        check(lib().mwg_ctx_trim(self.h, C.byref(r)), self.h)
+        return int(r.value)
+
     def __del__(self):
         if getattr(self, "h", None) and _lib._LIB is not None:
             _lib._LIB.mwg_ctx_destroy(self.h)

@@ -281,6 +281,25 @@ def test_errors_are_reported(G):
         G.PoissonOperator(mra, 1e-4).blocks(0, mra.max_depth + 1, 0, 0)
 
 
+def test_trim_returns_cached_memory(G):
+    """mwg_ctx_trim hands the device blocks the context keeps for reuse back to the device;
+    applies afterwards allocate again and give the same trees."""
+    prec = 1e-4
+    mra = G.mra_from(configs.BOX16, 5)
+    f = G.project(prec, G.FunctionTree(mra), configs.density3(4)[0])
+    P = G.PoissonOperator(mra, prec)
+    o = G.FunctionTree(mra)
+    G.apply(prec, o, P, f)
+    ref = o.export()
+    del o
+    assert mra.ctx.trim() > 0
+    assert mra.ctx.trim() == 0
+    o2 = G.FunctionTree(mra)
+    G.apply(prec, o2, P, f)
+    d, l, c = o2.export()
+    assert np.array_equal(d, ref[0]) and np.array_equal(l, ref[1]) and np.array_equal(c, ref[2])
+
+
 def _l2_diff(a, b, box_width, n=20000, seed=7):
     pts = np.random.default_rng(seed).uniform(-box_width / 2, box_width / 2, (n, 3))
     d = a.evalf(pts) - b.evalf(pts)
