@@ -38,6 +38,10 @@ struct Error : std::runtime_error {
     int code;
     Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
 };
+// the generated-node tables of an apply were too small (the batch runs again with larger ones)
+struct GenOverflow : Error {
+    GenOverflow() : Error(MWG_ERR_RANGE, "generated-node table overflow") {}
+};
 
 #define HC(x)                                                                                         \
     do {                                                                                              \
@@ -1421,7 +1425,10 @@ static double slice_flops(int K) {
 // device memory for the FP64 partial slots of one refinement iteration
 constexpr double SLOT_BUDGET = 8.0 * (1ull << 30);
 
-static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, std::vector<mwg_apply_stats> &stats) {
+// gen_shift: the generated-node tables hold 4 << gen_shift entries per input node, and at
+// least 2^(14 + gen_shift)
+static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, std::vector<mwg_apply_stats> &stats,
+                       int gen_shift) {
     if (jobs.empty()) return;
     PhaseTimer PT;
     PT.on = std::getenv("MWG_TIMING") != nullptr;
@@ -1492,12 +1499,14 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
         if (!dl_plan) J.out->ensure_dev(est);
         J.out->reserve_host(est);
         J.work = J.out->roots;
-        uint32_t cap = 1u << 14;
-        while (cap < 4u * (uint32_t)J.in->n()) cap <<= 1;
+        size_t cap = (size_t)1 << (14 + gen_shift);
+        while (cap < ((size_t)4 << gen_shift) * (size_t)J.in->n()) cap <<= 1;
+        if (cap > ((size_t)1 << 31)) throw Error(MWG_ERR_RANGE, "generated-node table beyond 2^31 entries");
         J.genOff = genTotal;
-        J.genMask = cap - 1;
+        J.genMask = (uint32_t)(cap - 1);
         genTotal += cap;
     }
+    if (genTotal > ((size_t)1 << 31)) throw Error(MWG_ERR_RANGE, "generated-node tables beyond 2^31 entries");
     // operator levels down to the inputs' depths, all operators at once (deeper ones on demand)
     PT.lap("inputs");
     ensure_levels(need, mixed);
@@ -2181,7 +2190,7 @@ static void apply_jobs(std::vector<JobH> &jobs, int max_iter, unsigned flags, st
             std::memcpy(wn.data(), h + o_wn, wn.size() * sizeof(double));
             if (use_pred) std::memcpy(cprob.data(), h + o_pr, cprob.size() * sizeof(float));
             if (C.tracing) std::memcpy(cfeat.data(), h + o_ft, cfeat.size() * sizeof(float));
-            if (hsum.gen_overflow) throw Error(MWG_ERR_RANGE, "generated-node table overflow");
+            if (hsum.gen_overflow) throw GenOverflow();
         };
         launch_count();
         PT.lap("enum_count");
@@ -3526,19 +3535,32 @@ int mwg_apply_batch(int n, const double *prec, mwg_tree *out, mwg_oper *oper, mw
         if (flags & ~(MWG_APPLY_MIXED | MWG_APPLY_PREDICT)) throw Error(MWG_ERR_ARG, "unknown apply flags");
         if ((flags & MWG_APPLY_PREDICT) && !C->pred)
             throw Error(MWG_ERR_ARG, "MWG_APPLY_PREDICT needs a predictor installed with mwg_ctx_set_predictor");
-        std::vector<JobH> jobs(n);
         for (int i = 0; i < n; i++) {
             if (!(prec[i] > 0.0)) throw Error(MWG_ERR_ARG, "apply precision must be positive");
             for (int j = 0; j < n; j++)
                 if (out[i] == inp[j] || (j != i && out[i] == out[j]))
                     throw Error(MWG_ERR_ARG, "output trees must be distinct from inputs and each other");
-            jobs[i].prec = prec[i];
-            jobs[i].out = out[i];
-            jobs[i].op = oper[i];
-            jobs[i].in = inp[i];
         }
         std::vector<mwg_apply_stats> st;
-        apply_jobs(jobs, max_iter, flags, st);
+        // The generated-node tables are sized from the inputs (4 entries per input node); an
+        // output that refines far below a coarse input can need more: run the batch again
+        // from fresh output roots with 8x, 64x and 512x larger tables.
+        for (int gen_shift = 0;; gen_shift += 3) {
+            std::vector<JobH> jobs(n);
+            for (int i = 0; i < n; i++) {
+                jobs[i].prec = prec[i];
+                jobs[i].out = out[i];
+                jobs[i].op = oper[i];
+                jobs[i].in = inp[i];
+            }
+            st.clear();
+            try {
+                apply_jobs(jobs, max_iter, flags, st, gen_shift);
+                break;
+            } catch (const GenOverflow &) {
+                if (gen_shift >= 9) throw;
+            }
+        }
         if (stats)
             for (int i = 0; i < n; i++) stats[i] = st[i];
     });

@@ -110,3 +110,25 @@ def test_device_loop_with_a_set_slice_chunk_k9(G):
                 _same(a, b, 1e-12)
     finally:
         mra.ctx.set_tuning(*saved)
+
+
+def test_device_loop_builds_deeper_operator_levels_on_demand(G):
+    """Inputs projected coarsely (depth 4) and applied at a tight precision: the outputs
+    reach depth 7, below the operator levels built up front (the inputs' depth + 1), so the
+    device loop builds the deeper levels of its fresh operators as it reaches them.  The
+    trees equal the host loop's (with operators built there), and the oracle's depths say
+    the case does reach two levels below (oracle: input depth 4, output depth 7)."""
+    mra = G.mra_from(configs.BOX20, 5)
+    orbs, nuc, mus = configs.orbitals(3, n_orb=2)
+    fs = [G.project(1e-2, G.FunctionTree(mra), o) for o in orbs]
+    ref = [G.FunctionTree(mra) for _ in fs]
+    sh = _host_loop(lambda: G.apply_batch(1e-7, ref, [G.HelmholtzOperator(mra, mu, 1e-7) for mu in mus], fs))
+    assert all(s["device_loop"] == 0 for s in sh)
+    assert max(t.getDepth() for t in ref) >= max(f.getDepth() for f in fs) + 2
+    out = [G.FunctionTree(mra) for _ in fs]
+    sd = G.apply_batch(1e-7, out, [G.HelmholtzOperator(mra, mu, 1e-7) for mu in mus], fs)
+    assert all(s["device_loop"] == 1 for s in sd)
+    for a, b, x, y in zip(out, ref, sd, sh):
+        for key in ("nodes_out", "iters", "tuples", "gen_nodes"):
+            assert x[key] == y[key], key
+        _same(a, b, 1e-13)
