@@ -112,21 +112,26 @@ def test_device_loop_with_a_set_slice_chunk_k9(G):
         mra.ctx.set_tuning(*saved)
 
 
-def test_device_loop_builds_deeper_operator_levels_on_demand(G):
-    """Inputs projected coarsely (depth 4) and applied at a tight precision: the outputs
-    reach depth 7, below the operator levels built up front (the inputs' depth + 1), so the
-    device loop builds the deeper levels of its fresh operators as it reaches them.  The
-    trees equal the host loop's (with operators built there), and the oracle's depths say
-    the case does reach two levels below (oracle: input depth 4, output depth 7)."""
-    mra = G.mra_from(configs.BOX20, 5)
+@pytest.mark.parametrize("k,mode", [(5, "fp64"), (7, "mixed")])
+def test_device_loop_builds_deeper_operator_levels_on_demand(G, k, mode):
+    """Inputs projected coarsely and applied at a tight precision: the outputs reach two or
+    more levels below the operator levels built up front (the inputs' depth + 1), so the
+    device loop builds the deeper levels of its fresh operators (and their split-FP16
+    copies) as it reaches them.  The generated-node tables are sized from the small inputs;
+    at k = 5 they overflow on the first attempt and the batch runs again with larger ones.  The trees
+    equal the host loop's (with operators built there); the oracle's depths say the k = 5
+    case reaches three levels below (oracle: input depth 4, output depth 7)."""
+    flags = G.MWG_APPLY_MIXED if mode == "mixed" else G.MWG_APPLY_FP64
+    mra = G.mra_from(configs.BOX20, k)
     orbs, nuc, mus = configs.orbitals(3, n_orb=2)
     fs = [G.project(1e-2, G.FunctionTree(mra), o) for o in orbs]
     ref = [G.FunctionTree(mra) for _ in fs]
-    sh = _host_loop(lambda: G.apply_batch(1e-7, ref, [G.HelmholtzOperator(mra, mu, 1e-7) for mu in mus], fs))
+    sh = _host_loop(lambda: G.apply_batch(1e-7, ref, [G.HelmholtzOperator(mra, mu, 1e-7) for mu in mus], fs,
+                                          flags=flags))
     assert all(s["device_loop"] == 0 for s in sh)
     assert max(t.getDepth() for t in ref) >= max(f.getDepth() for f in fs) + 2
     out = [G.FunctionTree(mra) for _ in fs]
-    sd = G.apply_batch(1e-7, out, [G.HelmholtzOperator(mra, mu, 1e-7) for mu in mus], fs)
+    sd = G.apply_batch(1e-7, out, [G.HelmholtzOperator(mra, mu, 1e-7) for mu in mus], fs, flags=flags)
     assert all(s["device_loop"] == 1 for s in sd)
     for a, b, x, y in zip(out, ref, sd, sh):
         for key in ("nodes_out", "iters", "tuples", "gen_nodes"):
