@@ -246,6 +246,9 @@ struct Ctx {
     // list size, generated nodes, output/input node ratio, raised FP64 and slice task sizes),
     // which size its buffers, and its buffers, read-back entries and events
     int hw_nw = 0, hw_gen = 0;
+    // generated-node table size of the applies (4 << gen_shift entries per input node, at
+    // least 2^(14 + gen_shift) per job): raised when a batch overflowed the tables
+    int gen_shift = 0;
     int hw_chunk[LOOP_MAX_DEPTH] = {}, hw_schunk[LOOP_MAX_DEPTH] = {};  // per refinement iteration
     double hw_ratio = 0.0;
     std::vector<int> last_in, last_out;  // input and output node counts of the last batch, per job
@@ -3544,8 +3547,9 @@ int mwg_apply_batch(int n, const double *prec, mwg_tree *out, mwg_oper *oper, mw
         std::vector<mwg_apply_stats> st;
         // The generated-node tables are sized from the inputs (4 entries per input node); an
         // output that refines far below a coarse input can need more: run the batch again
-        // from fresh output roots with 8x, 64x and 512x larger tables.
-        for (int gen_shift = 0;; gen_shift += 3) {
+        // from fresh output roots with 8x, 64x and 512x larger tables; later batches on the
+        // context start at the size that sufficed.
+        for (int gen_shift = C->gen_shift;; gen_shift += 3) {
             std::vector<JobH> jobs(n);
             for (int i = 0; i < n; i++) {
                 jobs[i].prec = prec[i];
@@ -3556,6 +3560,7 @@ int mwg_apply_batch(int n, const double *prec, mwg_tree *out, mwg_oper *oper, mw
             st.clear();
             try {
                 apply_jobs(jobs, max_iter, flags, st, gen_shift);
+                C->gen_shift = gen_shift;
                 break;
             } catch (const GenOverflow &) {
                 if (gen_shift >= 9) throw;
